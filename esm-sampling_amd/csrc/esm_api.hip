@@ -1,0 +1,486 @@
+// esm_api.hip — the C ABI of libesmhip.so (include/esm.h): handle, state transfer, energy,
+// scores and the Langevin step. Host code; the kernels live in the k_*.hip files.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "esm_internal.h"
+
+namespace esm {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+template <typename S>
+static void *upload_conv(const std::vector<double> &src, hipStream_t st, int *rc) {
+    void *p = nullptr;
+    if (src.empty()) return nullptr;
+    if (hipMalloc(&p, src.size() * sizeof(S)) != hipSuccess) {
+        *rc = ESM_ENOMEM;
+        return nullptr;
+    }
+    std::vector<S> tmp(src.begin(), src.end());
+    if (hipMemcpyAsync(p, tmp.data(), tmp.size() * sizeof(S), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        *rc = ESM_EHIP;
+    return p;
+}
+
+template <typename S>
+DevModel<S> dev_model(const esm_handle *h) {
+    DevModel<S> m;
+    m.N = h->N;
+    m.L = h->L;
+    m.T = h->T;
+    m.R = h->R;
+    m.dt = (S)h->dt;
+    m.cv = (S)h->m.c_viol;
+    m.c_shed = (S)h->m.c_shed;
+    m.c_dump = (S)h->m.c_dump;
+    m.kappa0 = (S)h->m.kappa0;
+    m.bus0 = h->d_bus0;
+    m.bus1 = h->d_bus1;
+    m.lb = (const S *)h->mb.lb;
+    m.ls = (const S *)h->mb.ls;
+    m.adj_ptr = h->d_adj_ptr;
+    m.adj_line = h->d_adj_line;
+    m.adj_other = h->d_adj_other;
+    m.adj_b = (const S *)h->mb.adj_b;
+    m.load = (const S *)h->mb.load;
+    m.mc = (const S *)h->mb.mc;
+    m.ef = (const S *)h->mb.ef;
+    m.kc = (const S *)h->mb.kc;
+    m.hours = (const S *)h->mb.hours;
+    m.eta_in = (const S *)h->mb.eta_in;
+    m.eta_out = (const S *)h->mb.eta_out;
+    m.qt = (const S *)h->mb.qt;
+    return m;
+}
+template DevModel<float> dev_model<float>(const esm_handle *);
+template DevModel<double> dev_model<double>(const esm_handle *);
+
+// The clamped capacities in the state precision (re-uploaded whenever k changes).
+static int upload_k(esm_handle *h) {
+    const int n = ESM_NK * h->N;
+    std::vector<double> kc(n);
+    for (int u = 0; u < n; u++) kc[u] = std::min(std::max(h->k[u], h->klo[u]), h->khi[u]);
+    ESM_HIP_CHECK(hipMemcpyAsync(h->d_k, h->k.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->prec == ESM_FP64) {
+        ESM_HIP_CHECK(hipMemcpyAsync(h->mb.kc, kc.data(), n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    } else {
+        std::vector<float> kf(kc.begin(), kc.end());
+        ESM_HIP_CHECK(hipMemcpyAsync(h->mb.kc, kf.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    }
+    h->scores_valid = false;
+    return 0;
+}
+
+// Convert a host FP64 array to the state precision and copy it to the device.
+static int put_state(esm_handle *h, void *dst, const double *src, size_t n) {
+    if (h->prec == ESM_FP64) {
+        ESM_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    const size_t chunk = (size_t)1 << 24;
+    std::vector<float> tmp(std::min(n, chunk));
+    for (size_t o = 0; o < n; o += chunk) {
+        const size_t c = std::min(chunk, n - o);
+        for (size_t i = 0; i < c; i++) tmp[i] = (float)src[o + i];
+        ESM_HIP_CHECK(hipMemcpyAsync((float *)dst + o, tmp.data(), c * sizeof(float), hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    }
+    return 0;
+}
+
+static int get_state(esm_handle *h, double *dst, const void *src, size_t n) {
+    if (h->prec == ESM_FP64) {
+        ESM_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+        return 0;
+    }
+    const size_t chunk = (size_t)1 << 24;
+    std::vector<float> tmp(std::min(n, chunk));
+    for (size_t o = 0; o < n; o += chunk) {
+        const size_t c = std::min(chunk, n - o);
+        ESM_HIP_CHECK(hipMemcpyAsync(tmp.data(), (const float *)src + o, c * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < c; i++) dst[o + i] = tmp[i];
+    }
+    return 0;
+}
+
+static int rebuild(esm_handle *h) {
+    return h->prec == ESM_FP64 ? launch_energy<double>(h, false, false) : launch_energy<float>(h, false, false);
+}
+
+static void clear_prof(esm_handle *h) {
+    for (int i = 0; i < ESM_NKERN; i++) {
+        for (auto &e : h->prof_ev[i]) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        h->prof_ev[i].clear();
+    }
+}
+
+static void free_handle(esm_handle *h) {
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    clear_prof(h);
+    void *ps[] = {h->d_bus0, h->d_bus1, h->d_adj_ptr, h->d_adj_line, h->d_adj_other, h->mb.lb, h->mb.ls, h->mb.adj_b,
+                  h->mb.load, h->mb.mc, h->mb.ef, h->mb.kc, h->mb.hours, h->mb.eta_in, h->mb.eta_out, h->mb.qt,
+                  h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,
+                  h->d_out, h->d_sk, h->d_fk};
+    for (void *p : ps)
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+}  // namespace esm
+
+using namespace esm;
+
+extern "C" {
+
+int32_t esm_version(void) { return ESM_ABI_VERSION; }
+const char *esm_last_error(void) { return g_err.c_str(); }
+
+int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_handle **out) {
+    if (!mdl || !out) {
+        set_error("esm_create: null model or output pointer");
+        return ESM_EINVAL;
+    }
+    *out = nullptr;
+    if (precision != ESM_FP64 && precision != ESM_FP32) {
+        set_error("esm_create: precision must be ESM_FP64 (64) or ESM_FP32 (32)");
+        return ESM_EINVAL;
+    }
+    if (mdl->N < 2 || mdl->L < 1 || mdl->T < 2 || nrep < 1 || !(mdl->dt > 0)) {
+        set_error("esm_create: need N >= 2, L >= 1, T >= 2, nrep >= 1 and dt > 0");
+        return ESM_EINVAL;
+    }
+    const double *req[] = {mdl->line_b, mdl->line_s, mdl->load, mdl->mc, mdl->ef, mdl->capex, mdl->klo, mdl->khi,
+                           mdl->stor_hours, mdl->stor_eta_in, mdl->stor_eta_out, mdl->stor_qt};
+    for (const double *p : req)
+        if (!p || !mdl->line_bus0 || !mdl->line_bus1) {
+            set_error("esm_create: a model table is NULL");
+            return ESM_EINVAL;
+        }
+    const int N = mdl->N, L = mdl->L, T = mdl->T;
+    for (int e = 0; e < L; e++) {
+        const int a = mdl->line_bus0[e], b = mdl->line_bus1[e];
+        if (a < 0 || a >= N || b < 0 || b >= N || a == b) {
+            set_error("esm_create: line " + std::to_string(e) + " has an invalid end");
+            return ESM_EINVAL;
+        }
+    }
+    esm_handle *h = new esm_handle();
+    h->N = N;
+    h->L = L;
+    h->T = T;
+    h->R = nrep;
+    h->prec = precision;
+    h->es = precision == ESM_FP64 ? 8 : 4;
+    h->dt = mdl->dt;
+    h->m = *mdl;
+    auto cp = [](const double *p, size_t n) { return std::vector<double>(p, p + n); };
+    h->bus0.assign(mdl->line_bus0, mdl->line_bus0 + L);
+    h->bus1.assign(mdl->line_bus1, mdl->line_bus1 + L);
+    h->line_b = cp(mdl->line_b, L);
+    h->line_s = cp(mdl->line_s, L);
+    h->load = cp(mdl->load, (size_t)N * T);
+    h->mc = cp(mdl->mc, ESM_NK * N);
+    h->ef = cp(mdl->ef, N);
+    h->capex = cp(mdl->capex, ESM_NK * N);
+    h->klo = cp(mdl->klo, ESM_NK * N);
+    h->khi = cp(mdl->khi, ESM_NK * N);
+    h->hours = cp(mdl->stor_hours, ESM_NSTOR * N);
+    h->eta_in = cp(mdl->stor_eta_in, ESM_NSTOR * N);
+    h->eta_out = cp(mdl->stor_eta_out, ESM_NSTOR * N);
+    h->qt = cp(mdl->stor_qt, ESM_NSTOR * N);
+    h->k.assign(ESM_NK * N, 0.0);
+    // CSR adjacency: lines incident to every node, in line order
+    h->adj_ptr.assign(N + 1, 0);
+    for (int e = 0; e < L; e++) {
+        h->adj_ptr[h->bus0[e] + 1]++;
+        h->adj_ptr[h->bus1[e] + 1]++;
+    }
+    for (int n = 0; n < N; n++) h->adj_ptr[n + 1] += h->adj_ptr[n];
+    h->adj_line.assign(2 * L, 0);
+    h->adj_other.assign(2 * L, 0);
+    h->adj_bv.assign(2 * L, 0.0);
+    {
+        std::vector<int> fill(h->adj_ptr.begin(), h->adj_ptr.end() - 1);
+        for (int e = 0; e < L; e++) {
+            int a = h->bus0[e], b = h->bus1[e];
+            int qa = fill[a]++, qb = fill[b]++;
+            h->adj_line[qa] = e;
+            h->adj_other[qa] = b;
+            h->adj_bv[qa] = h->line_b[e];
+            h->adj_line[qb] = e;
+            h->adj_other[qb] = a;
+            h->adj_bv[qb] = h->line_b[e];
+        }
+    }
+    int rc = 0;
+    if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
+        set_error("esm_create: no HIP device or stream");
+        free_handle(h);
+        return ESM_EHIP;
+    }
+    auto imalloc = [&](int **p, const std::vector<int> &src) {
+        if (rc) return;
+        if (hipMalloc((void **)p, src.size() * sizeof(int)) != hipSuccess) { rc = ESM_ENOMEM; return; }
+        if (hipMemcpy(*p, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) rc = ESM_EHIP;
+    };
+    auto dmalloc = [&](double **p, const std::vector<double> &src) {
+        if (rc) return;
+        if (hipMalloc((void **)p, src.size() * sizeof(double)) != hipSuccess) { rc = ESM_ENOMEM; return; }
+        if (hipMemcpy(*p, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = ESM_EHIP;
+    };
+    imalloc(&h->d_bus0, std::vector<int>(h->bus0.begin(), h->bus0.end()));
+    imalloc(&h->d_bus1, std::vector<int>(h->bus1.begin(), h->bus1.end()));
+    imalloc(&h->d_adj_ptr, h->adj_ptr);
+    imalloc(&h->d_adj_line, h->adj_line);
+    imalloc(&h->d_adj_other, h->adj_other);
+    dmalloc(&h->d_k, h->k);
+    dmalloc(&h->d_klo, h->klo);
+    dmalloc(&h->d_khi, h->khi);
+    dmalloc(&h->d_capex, h->capex);
+    std::vector<double> kc0(ESM_NK * N, 0.0);
+    for (int u = 0; u < ESM_NK * N; u++) kc0[u] = std::min(std::max(0.0, h->klo[u]), h->khi[u]);
+    if (!rc) {
+        if (precision == ESM_FP64) {
+            h->mb.lb = upload_conv<double>(h->line_b, h->stream, &rc);
+            h->mb.ls = upload_conv<double>(h->line_s, h->stream, &rc);
+            h->mb.adj_b = upload_conv<double>(h->adj_bv, h->stream, &rc);
+            h->mb.load = upload_conv<double>(h->load, h->stream, &rc);
+            h->mb.mc = upload_conv<double>(h->mc, h->stream, &rc);
+            h->mb.ef = upload_conv<double>(h->ef, h->stream, &rc);
+            h->mb.kc = upload_conv<double>(kc0, h->stream, &rc);
+            h->mb.hours = upload_conv<double>(h->hours, h->stream, &rc);
+            h->mb.eta_in = upload_conv<double>(h->eta_in, h->stream, &rc);
+            h->mb.eta_out = upload_conv<double>(h->eta_out, h->stream, &rc);
+            h->mb.qt = upload_conv<double>(h->qt, h->stream, &rc);
+        } else {
+            h->mb.lb = upload_conv<float>(h->line_b, h->stream, &rc);
+            h->mb.ls = upload_conv<float>(h->line_s, h->stream, &rc);
+            h->mb.adj_b = upload_conv<float>(h->adj_bv, h->stream, &rc);
+            h->mb.load = upload_conv<float>(h->load, h->stream, &rc);
+            h->mb.mc = upload_conv<float>(h->mc, h->stream, &rc);
+            h->mb.ef = upload_conv<float>(h->ef, h->stream, &rc);
+            h->mb.kc = upload_conv<float>(kc0, h->stream, &rc);
+            h->mb.hours = upload_conv<float>(h->hours, h->stream, &rc);
+            h->mb.eta_in = upload_conv<float>(h->eta_in, h->stream, &rc);
+            h->mb.eta_out = upload_conv<float>(h->eta_out, h->stream, &rc);
+            h->mb.qt = upload_conv<float>(h->qt, h->stream, &rc);
+        }
+    }
+    const size_t TR = (size_t)T * nrep;
+    auto zmalloc = [&](void **p, size_t bytes) {
+        if (rc) return;
+        if (hipMalloc(p, bytes) != hipSuccess) { rc = ESM_ENOMEM; return; }
+        if (hipMemset(*p, 0, bytes) != hipSuccess) rc = ESM_EHIP;
+    };
+    zmalloc(&h->d_v, (size_t)ESM_NV * N * TR * h->es);
+    zmalloc(&h->d_x, (size_t)ESM_NREN * N * TR * h->es);
+    zmalloc(&h->d_r, (size_t)N * TR * h->es);
+    zmalloc(&h->d_F, (size_t)L * TR * h->es);
+    zmalloc((void **)&h->d_hparts, 6 * TR * sizeof(double));
+    const int nchunk = (T + 63) / 64;
+    h->part2_len = (size_t)nchunk * std::max(6, ESM_NK * N) * nrep;
+    zmalloc((void **)&h->d_part2, h->part2_len * sizeof(double));
+    zmalloc((void **)&h->d_out, (size_t)std::max(nrep * 8, ESM_NK * N * 2 + 64) * sizeof(double));
+    zmalloc((void **)&h->d_sk, (size_t)nrep * ESM_NK * N * sizeof(double));
+    zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
+    if (rc) {
+        set_error(rc == ESM_ENOMEM ? "esm_create: device allocation failed" : "esm_create: HIP copy failed");
+        free_handle(h);
+        return rc;
+    }
+    *out = h;
+    return ESM_OK;
+}
+
+int32_t esm_destroy(esm_handle *h) {
+    if (!h) return ESM_OK;
+    (void)hipStreamSynchronize(h->stream);
+    free_handle(h);
+    return ESM_OK;
+}
+
+int32_t esm_upload_state(esm_handle *h, const double *v, const double *k, const double *x) {
+    if (!h) { set_error("esm_upload_state: null handle"); return ESM_EINVAL; }
+    const size_t TR = (size_t)h->T * h->R;
+    int rc;
+    if (k) {
+        for (int u = 0; u < ESM_NK * h->N; u++)
+            if (!std::isfinite(k[u])) { set_error("esm_upload_state: non-finite capacity"); return ESM_EINVAL; }
+        h->k.assign(k, k + ESM_NK * h->N);
+        if ((rc = upload_k(h))) return rc;
+    }
+    if (v && (rc = put_state(h, h->d_v, v, (size_t)ESM_NV * h->N * TR))) return rc;
+    if (x && (rc = put_state(h, h->d_x, x, (size_t)ESM_NREN * h->N * TR))) return rc;
+    h->scores_valid = false;
+    if ((v || k || x) && (rc = rebuild(h))) return rc;
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return ESM_OK;
+}
+
+int32_t esm_download_state(esm_handle *h, double *v, double *k, double *x) {
+    if (!h) { set_error("esm_download_state: null handle"); return ESM_EINVAL; }
+    const size_t TR = (size_t)h->T * h->R;
+    int rc;
+    if (v && (rc = get_state(h, v, h->d_v, (size_t)ESM_NV * h->N * TR))) return rc;
+    if (x && (rc = get_state(h, x, h->d_x, (size_t)ESM_NREN * h->N * TR))) return rc;
+    if (k) std::copy(h->k.begin(), h->k.end(), k);
+    return ESM_OK;
+}
+
+int32_t esm_energy(esm_handle *h, double *parts, double *per_hour) {
+    if (!h || !parts) { set_error("esm_energy: null handle or output"); return ESM_EINVAL; }
+    int rc = h->prec == ESM_FP64 ? launch_energy<double>(h, true, false) : launch_energy<float>(h, true, false);
+    if (rc) return rc;
+    if ((rc = launch_reduce_parts(h, h->d_out))) return rc;
+    const int R = h->R;
+    std::vector<double> p6((size_t)R * 6);
+    ESM_HIP_CHECK(hipMemcpyAsync(p6.data(), h->d_out, p6.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> ph;
+    if (per_hour) {
+        // per-step op = gen + storage + c_shed shed + c_dump dump + penalty
+        std::vector<double> hp(6 * (size_t)h->T * R);
+        ESM_HIP_CHECK(hipMemcpyAsync(hp.data(), h->d_hparts, hp.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+        const size_t TR = (size_t)h->T * R;
+        for (size_t g = 0; g < TR; g++)
+            per_hour[g] = hp[g] + hp[TR + g] + h->m.c_shed * hp[2 * TR + g] + h->m.c_dump * hp[3 * TR + g] + hp[4 * TR + g];
+    }
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    // capex from the shared capacities (src/energy.jl:557-568), FP64 on the host
+    double capex = 0;
+    for (int f = 0; f < ESM_NK; f++) {
+        double dot = 0, pen = 0;
+        for (int n = 0; n < h->N; n++) {
+            const int u = f * h->N + n;
+            const double kc = std::min(std::max(h->k[u], h->klo[u]), h->khi[u]);
+            dot += h->capex[u] * h->k[u];
+            pen += h->capex[u] * fabs(h->k[u] - kc);
+        }
+        capex += dot + h->m.gamma_k * pen;
+    }
+    for (int j = 0; j < R; j++) {
+        const double *q = &p6[(size_t)j * 6];
+        double *o = parts + (size_t)j * ESM_NPARTS;
+        const double cbal = h->m.c_shed * q[2] + h->m.c_dump * q[3];
+        o[0] = capex;
+        o[1] = q[0] + q[1] + cbal + q[4];
+        o[2] = q[0];
+        o[3] = q[1];
+        o[4] = q[2];
+        o[5] = q[3];
+        o[6] = q[4];
+        o[7] = q[5];
+    }
+    return ESM_OK;
+}
+
+int32_t esm_rebuild_residual(esm_handle *h, double *drift) {
+    if (!h) { set_error("esm_rebuild_residual: null handle"); return ESM_EINVAL; }
+    int rc = h->prec == ESM_FP64 ? launch_energy<double>(h, false, true) : launch_energy<float>(h, false, true);
+    if (rc) return rc;
+    if ((rc = launch_reduce_max(h, h->d_out))) return rc;
+    if (drift) ESM_HIP_CHECK(hipMemcpyAsync(drift, h->d_out, h->R * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return ESM_OK;
+}
+
+int32_t esm_download_caches(esm_handle *h, double *r, double *F) {
+    if (!h) { set_error("esm_download_caches: null handle"); return ESM_EINVAL; }
+    const size_t TR = (size_t)h->T * h->R;
+    int rc;
+    if (r && (rc = get_state(h, r, h->d_r, (size_t)h->N * TR))) return rc;
+    if (F && (rc = get_state(h, F, h->d_F, (size_t)h->L * TR))) return rc;
+    return ESM_OK;
+}
+
+int32_t esm_scores(esm_handle *h, double *sk, double *fk) {
+    if (!h) { set_error("esm_scores: null handle"); return ESM_EINVAL; }
+    int rc = h->prec == ESM_FP64 ? launch_scores<double>(h) : launch_scores<float>(h);
+    if (rc) return rc;
+    if (sk) ESM_HIP_CHECK(hipMemcpyAsync(sk, h->d_sk, (size_t)h->R * ESM_NK * h->N * sizeof(double),
+                                         hipMemcpyDeviceToHost, h->stream));
+    if (fk) ESM_HIP_CHECK(hipMemcpyAsync(fk, h->d_fk, (size_t)ESM_NK * h->N * sizeof(double), hipMemcpyDeviceToHost,
+                                         h->stream));
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->scores_valid = true;
+    return ESM_OK;
+}
+
+int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, const double *eta,
+                     const double *kappa_s, double kmin, double clip, int32_t box, double *knew, int32_t *nclip) {
+    if (!h || !weights || !xi || !eta || !kappa_s || !knew) {
+        set_error("esm_langevin: null argument");
+        return ESM_EINVAL;
+    }
+    if (!h->scores_valid) {
+        set_error("esm_langevin: no scores of the current state (call esm_scores first)");
+        return ESM_ESTATE;
+    }
+    const int n = ESM_NK * h->N, R = h->R;
+    // staging in d_out: weights [R], xi [n], eta [6], kappa_s [6], knew [n], nclip (int)
+    std::vector<double> stage((size_t)R + n + 2 * ESM_NK);
+    std::copy(weights, weights + R, stage.begin());
+    std::copy(xi, xi + n, stage.begin() + R);
+    std::copy(eta, eta + ESM_NK, stage.begin() + R + n);
+    std::copy(kappa_s, kappa_s + ESM_NK, stage.begin() + R + n + ESM_NK);
+    for (int f = 0; f < ESM_NK; f++)
+        if (!(eta[f] >= 0) || !(kappa_s[f] > -kmin)) {
+            set_error("esm_langevin: eta must be >= 0 and kappa_s + kmin > 0");
+            return ESM_EINVAL;
+        }
+    double *d = nullptr;
+    const size_t bytes = (stage.size() + n + 1) * sizeof(double);
+    ESM_HIP_CHECK(hipMallocAsync((void **)&d, bytes, h->stream));
+    ESM_HIP_CHECK(hipMemcpyAsync(d, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    int *d_nc = (int *)(d + stage.size() + n);
+    ESM_HIP_CHECK(hipMemsetAsync(d_nc, 0, sizeof(int), h->stream));
+    int rc = launch_langevin(h, d, d + R, d + R + n, d + R + n + ESM_NK, kmin, clip, box, d + stage.size(), d_nc);
+    if (rc) return rc;
+    int nc = 0;
+    ESM_HIP_CHECK(hipMemcpyAsync(knew, d + stage.size(), n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ESM_HIP_CHECK(hipMemcpyAsync(&nc, d_nc, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    ESM_HIP_CHECK(hipFreeAsync(d, h->stream));
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (nclip) *nclip = nc;
+    return ESM_OK;
+}
+
+int32_t esm_profile(esm_handle *h, int32_t enable) {
+    if (!h) { set_error("esm_profile: null handle"); return ESM_EINVAL; }
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (enable) clear_prof(h);
+    h->prof = enable != 0;
+    return ESM_OK;
+}
+
+int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches) {
+    if (!h || kernel < 0 || kernel >= ESM_NKERN) { set_error("esm_profile_read: bad handle or kernel id"); return ESM_EINVAL; }
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    double tot = 0;
+    for (auto &e : h->prof_ev[kernel]) {
+        float ms = 0;
+        ESM_HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (int64_t)h->prof_ev[kernel].size();
+    return ESM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// esm_internal.h — the handle and the kernel launchers of libesmhip.so (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/esm.h"
+#include "esm_device.h"
+
+namespace esm {
+
+void set_error(const std::string &msg);
+
+#define ESM_HIP_CHECK(expr)                                                                   \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess) {                                                               \
+            ::esm::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));              \
+            return ESM_EHIP;                                                                  \
+        }                                                                                     \
+    } while (0)
+
+// Device copies of the model tables in one precision.
+struct ModelBuf {
+    void *lb = nullptr, *ls = nullptr, *adj_b = nullptr, *load = nullptr, *mc = nullptr, *ef = nullptr,
+         *kc = nullptr, *hours = nullptr, *eta_in = nullptr, *eta_out = nullptr, *qt = nullptr;
+};
+
+}  // namespace esm
+
+struct esm_handle {
+    int N = 0, L = 0, T = 0, R = 0, prec = 64;
+    double dt = 1.0;
+    size_t es = 8;  // element size of the state precision
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // host copy of the model scalars and tables (FP64)
+    esm_model m{};
+    std::vector<int32_t> bus0, bus1;
+    std::vector<double> line_b, line_s, load, mc, ef, capex, klo, khi, hours, eta_in, eta_out, qt;
+    std::vector<int> adj_ptr, adj_line, adj_other;
+    std::vector<double> adj_bv;
+    std::vector<double> k;   // [ESM_NK][N] raw capacities (shared)
+    // device model
+    int *d_bus0 = nullptr, *d_bus1 = nullptr, *d_adj_ptr = nullptr, *d_adj_line = nullptr, *d_adj_other = nullptr;
+    esm::ModelBuf mb;
+    double *d_k = nullptr;       // [ESM_NK][N] FP64 raw capacities
+    double *d_klo = nullptr, *d_khi = nullptr, *d_capex = nullptr;
+    // device state (precision S)
+    void *d_v = nullptr;   // [ESM_NV][N][T][R]
+    void *d_x = nullptr;   // [ESM_NREN][N][T][R]
+    void *d_r = nullptr;   // [N][T][R]
+    void *d_F = nullptr;   // [L][T][R]
+    // FP64 scratch
+    double *d_hparts = nullptr;   // [6][T][R] per-step parts (gen, stor, shed, dump, pen, co2) or drift
+    double *d_part2 = nullptr;    // partial sums (reductions)
+    size_t part2_len = 0;
+    double *d_out = nullptr;      // small outputs
+    double *d_sk = nullptr;       // [R][ESM_NK][N]
+    double *d_fk = nullptr;       // [ESM_NK][N]
+    bool scores_valid = false;
+    // kernel timing (esm_profile)
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[ESM_NKERN];
+};
+
+namespace esm {
+
+// RAII bracket of one launch with HIP events on the handle's stream (when profiling).
+struct ProfScope {
+    esm_handle *h;
+    int id;
+    hipEvent_t b = nullptr;
+    ProfScope(esm_handle *h_, int id_) : h(h_), id(id_) {
+        if (h->prof) {
+            hipEvent_t a;
+            if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+                (void)hipEventRecord(a, h->stream);
+                h->prof_ev[id].push_back({a, b});
+            } else {
+                b = nullptr;
+            }
+        }
+    }
+    ~ProfScope() {
+        if (b) (void)hipEventRecord(b, h->stream);
+    }
+};
+
+template <typename S>
+DevModel<S> dev_model(const esm_handle *h);
+
+// k_energy.hip
+template <typename S>
+int launch_energy(esm_handle *h, bool cost, bool drift);
+int launch_reduce_parts(esm_handle *h, double *d_parts /*[R][6]*/);
+int launch_reduce_max(esm_handle *h, double *d_drift /*[R]*/);
+template <typename S>
+int launch_scores(esm_handle *h);
+int launch_langevin(esm_handle *h, const double *d_w, const double *d_xi, const double *d_eta, const double *d_ks,
+                    double kmin, double clip, int box, double *d_knew, int *d_nclip);
+
+}  // namespace esm

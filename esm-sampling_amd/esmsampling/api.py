@@ -1,0 +1,150 @@
+"""Host-side mirror of the reference's per-sweep API over the C ABI of libesmhip.so.
+
+The reference's boundary for this path is the Julia function API that ``run_replicas``
+(src/replicas.jl:841) calls. This module keeps its names and argument meanings, for a batch of
+replicas resident on one GPU (one ``Replicas`` object = one ``esm_handle``):
+
+=================================  ============================================  ===================
+reference (Julia)                  here                                          C entry point
+=================================  ============================================  ===================
+``energy_cost`` src/energy.jl:551  ``energy_cost(R)`` -> CostParts per replica   ``esm_energy``
+``LocalCost`` src/local.jl:313     ``LocalCost(R)`` -> (r, F) caches             ``esm_rebuild_residual``
+``capacity_scores`` replicas:329   ``capacity_scores(R)`` -> (sk, fk)            ``esm_scores``
+``langevin_capacities`` :391       ``langevin_capacities(R, ...)``               ``esm_langevin``
+=================================  ============================================  ===================
+
+Errors raise ``EsmError`` with the library's message (the reference raises Julia exceptions).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import FP32, FP64, NK, NPARTS, NREN, NV, check, dptr, f64, iptr
+
+PART_NAMES = ("capex", "op", "gen", "storage", "shed", "dump", "penalty", "co2")
+
+
+class Replicas:
+    """The device state of ``nrep`` replicas of one system (an ``esm_handle``).
+
+    Arrays use the layout [field][node][hour][replica] (replica fastest).
+    """
+
+    def __init__(self, sys, nrep: int, precision: int = FP64):
+        self.lib = _lib.load()
+        self.sys = sys
+        self.N, self.L, self.T, self.R = sys.N, sys.L, sys.T, int(nrep)
+        self.precision = precision
+        cm, keep = _lib.make_cmodel(sys)
+        h = C.c_void_p()
+        check(self.lib.esm_create(C.byref(cm), self.R, precision, C.byref(h)), "esm_create")
+        self._h = h
+        del keep
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            self.lib.esm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------------------------
+    def upload(self, v=None, k=None, x=None):
+        N, T, R = self.N, self.T, self.R
+        args = []
+        keep = []
+        for a, shape in ((v, (NV, N, T, R)), (k, (NK, N)), (x, (NREN, N, T, R))):
+            if a is None:
+                args.append(None)
+                continue
+            a = f64(a)
+            if a.shape != shape:
+                raise ValueError(f"expected shape {shape}, got {a.shape}")
+            keep.append(a)
+            args.append(dptr(a))
+        check(self.lib.esm_upload_state(self._h, *args), "esm_upload_state")
+
+    def download(self):
+        N, T, R = self.N, self.T, self.R
+        v = np.empty((NV, N, T, R))
+        k = np.empty((NK, N))
+        x = np.empty((NREN, N, T, R))
+        check(self.lib.esm_download_state(self._h, dptr(v), dptr(k), dptr(x)), "esm_download_state")
+        return v, k, x
+
+    def caches(self):
+        r = np.empty((self.N, self.T, self.R))
+        F = np.empty((self.L, self.T, self.R))
+        check(self.lib.esm_download_caches(self._h, dptr(r), dptr(F)), "esm_download_caches")
+        return r, F
+
+
+def energy_cost(R: Replicas, per_hour: bool = False):
+    """CostParts of every replica ``[R, 8]`` (capex, op, gen, storage, shed, dump, penalty, co2);
+    with ``per_hour`` also the operational cost of every step ``[T, R]``."""
+    parts = np.empty((R.R, NPARTS))
+    ph = np.empty((R.T, R.R)) if per_hour else None
+    check(R.lib.esm_energy(R.handle, dptr(parts), dptr(ph) if per_hour else None), "esm_energy")
+    return (parts, ph) if per_hour else parts
+
+
+def LocalCost(R: Replicas):
+    """Rebuild the caches from the node arrays and return ``(r [N,T,R], F [L,T,R], drift [R])``."""
+    drift = np.empty(R.R)
+    check(R.lib.esm_rebuild_residual(R.handle, dptr(drift)), "esm_rebuild_residual")
+    r, F = R.caches()
+    return r, F, drift
+
+
+def capacity_scores(R: Replicas):
+    """``(sk [R, 6, N], fk [6, N])``: the sampled capacity score −β_v ∂C/∂k per replica and the capex force."""
+    sk = np.empty((R.R, NK, R.N))
+    fk = np.empty((NK, R.N))
+    check(R.lib.esm_scores(R.handle, dptr(sk), dptr(fk)), "esm_scores")
+    return sk, fk
+
+
+def langevin_capacities(R: Replicas, weights, xi, eta, kappa_s, kmin: float = 1.0, clip: float = 0.02,
+                        box: bool = True):
+    """One Langevin step of the shared capacities from the last scores: ``(knew [6, N], nclip)``."""
+    w = f64(weights)
+    xi = f64(xi)
+    eta = f64(np.broadcast_to(eta, (NK,)))
+    ks = f64(np.broadcast_to(kappa_s, (NK,)))
+    if w.shape != (R.R,) or xi.shape != (NK, R.N):
+        raise ValueError("weights must be [R] and xi [6, N]")
+    knew = np.empty((NK, R.N))
+    nclip = C.c_int32(0)
+    check(R.lib.esm_langevin(R.handle, dptr(w), dptr(xi), dptr(eta), dptr(ks), float(kmin), float(clip), int(box),
+                             dptr(knew), C.byref(nclip)), "esm_langevin")
+    return knew, int(nclip.value)
+
+
+def uniform_weights(active):
+    """Weights 1/M_active over the active replicas (all if none is active), src/replicas.jl:310-314."""
+    a = np.asarray(active, dtype=bool)
+    if not a.any():
+        a = np.ones_like(a)
+    w = a.astype(np.float64)
+    return w / w.sum()
+
+
+def tilt_weights(lam: float, phis, active):
+    """Self-normalised weights exp(λΦ_j) over the active replicas, src/replicas.jl:303-308."""
+    a = np.asarray(active, dtype=bool)
+    if not a.any():
+        a = np.ones_like(a)
+    x = np.where(a, lam * np.asarray(phis, dtype=np.float64), -np.inf)
+    w = np.exp(x - x.max())
+    return w / w.sum()

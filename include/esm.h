@@ -1,0 +1,189 @@
+/*
+ * esm.h — C ABI of libesmhip.so, the MI355X per-sweep path of the replica sampler.
+ *
+ * The reference (FHell/ESM-Sampling, Julia) has no FFI for this path: its boundary is the
+ * Julia function API that `run_replicas` calls. Every entry point below replaces one of those
+ * functions; the citation next to each declaration names it (file:line in the reference).
+ *
+ * Conventions
+ *   - every function returns 0 on success and a negative ESM_E* code on failure; the message
+ *     of the last failure of the calling thread is returned by esm_last_error();
+ *   - host arrays are caller-owned, FP64, and laid out as documented per argument. Arrays of
+ *     the device state use the layout [field][node][hour][replica] (replica fastest), i.e. a
+ *     Julia Array{Float64,4} of size (R, T, N, F), the reference device port's [chain, step,
+ *     item] order (device/src/core.jl);
+ *   - device buffers are owned by the handle; a handle is not thread-safe (one handle per host
+ *     thread); all work is ordered on the handle's HIP stream and every call returns after it
+ *     has finished (host results are valid on return);
+ *   - node values are stored, not Haar coefficients: at the spatially complete level every
+ *     basis is orthonormal and square, so the two carry the same information
+ *     (docs/system/device.md:29-38). The angle field keeps all N node angles; the gauge mode
+ *     it carries does not enter any flow.
+ *
+ * Tech set of a node (the synthetic system): three renewable carriers as load (solar, onwind,
+ * offwind: output A k, curtailment is nodal dump, src/energy.jl:331-337), one dispatchable
+ * carrier (gas, utilization field), two storage carriers in the state-of-charge form
+ * (battery, H2; src/energy.jl:382-398).
+ */
+#ifndef ESM_H
+#define ESM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ESM_ABI_VERSION 1
+
+/* carriers */
+#define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
+#define ESM_NSTOR 2  /* storage (SoC form): 0 battery, 1 H2                            */
+#define ESM_NK 6     /* capacity fields: solar, onwind, offwind, gas, battery, H2      */
+#define ESM_NV 4     /* operation fields: 0 theta, 1 gas utilization, 2 battery SoC, 3 H2 SoC */
+#define ESM_KGAS 3   /* index of the gas capacity field */
+#define ESM_KSTOR 4  /* index of the first storage capacity field */
+#define ESM_NPARTS 8 /* CostParts: capex, op, gen, storage, shed, dump, penalty, co2 (src/energy.jl:527-536) */
+
+/* error codes */
+#define ESM_OK 0
+#define ESM_EINVAL -1    /* invalid argument (shape, pointer, range) */
+#define ESM_EHIP -2      /* HIP runtime error */
+#define ESM_ENOMEM -3    /* device allocation failed */
+#define ESM_ESTATE -4    /* call out of order (e.g. weather step before the weather is drawn) */
+
+/* precision of the device state */
+#define ESM_FP64 64
+#define ESM_FP32 32
+
+/*
+ * The static model of one complete level. All pointers are host FP64 arrays (or int32 for the
+ * line ends), read once by esm_create. Layouts: [carrier][node] for per-unit tables; [node][hour]
+ * for the load.
+ */
+typedef struct esm_model {
+    int32_t N;                 /* nodes                                                      */
+    int32_t L;                 /* AC lines                                                   */
+    int32_t T;                 /* steps of the horizon (cyclic: step T is followed by step 1) */
+    double dt;                 /* hours per step (Δt)                                        */
+    const int32_t *line_bus0;  /* [L] 0-based                                                */
+    const int32_t *line_bus1;  /* [L] 0-based                                                */
+    const double *line_b;      /* [L] susceptance: F = b (θ_a − θ_b) in MW                   */
+    const double *line_s;      /* [L] thermal limit (MW), hard: cost c_viol Δt (|F| − s)₊      */
+    const double *load;        /* [N][T] raw load (MW); the data map is max(x, 0)            */
+    const double *mc;          /* [ESM_NK][N] marginal cost (€/MWh) of every carrier          */
+    const double *ef;          /* [N] gas emission factor (t CO2 per MWh)                    */
+    const double *capex;       /* [ESM_NK][N] annualised capex (€/MW)                        */
+    const double *klo;         /* [ESM_NK][N] capacity box, lower (MW)                       */
+    const double *khi;         /* [ESM_NK][N] capacity box, upper (MW)                       */
+    const double *stor_hours;  /* [ESM_NSTOR][N] energy-to-power ratio (h)                   */
+    const double *stor_eta_in; /* [ESM_NSTOR][N] charging efficiency                         */
+    const double *stor_eta_out;/* [ESM_NSTOR][N] discharging efficiency                      */
+    const double *stor_qt;     /* [ESM_NSTOR][N] inflow per unit power                       */
+    double c_shed;             /* €/MWh of positive residual                                 */
+    double c_dump;             /* €/MWh of negative residual                                 */
+    double c_viol;             /* price of constraint violations                             */
+    double gamma_k;            /* capacity box penalty factor                                */
+    double kappa0;             /* capacity floor of the penalty weights (MW)                 */
+    double beta_v;             /* operational temperature (1/€)                              */
+    double beta_k;             /* investment temperature (1/€)                               */
+    /* weather: ESM_NREN*N capacity-factor series i = c*N + n, logit CF = xbar + z with z a
+     * K-regime VAR(1), z_τ = φ z_{τ-1} + μ_{m_τ}(τ) + σ e_τ, z_1 = σ0 e_1 (src/weather.jl:461-481
+     * with Φ = φ I, Σ_k = σ² I, Σ0 = σ0² I and harmonics (1, cos 2πτ/T, sin 2πτ/T)).      */
+    int32_t K;                 /* regimes                                                    */
+    const double *Ptr;         /* [K][K] regime transition matrix (row: from)               */
+    const double *pi0;         /* [K] initial regime law                                     */
+    double phi;                /* AR coefficient                                             */
+    double sigma;              /* innovation standard deviation                             */
+    double sigma0;             /* standard deviation of z at step 1                         */
+    const double *mu;          /* [K][ESM_NREN*N][3] regime intercept, cos and sin coefficients */
+    const double *xbar;        /* [ESM_NREN*N][24] climatological logit by hour of day      */
+} esm_model;
+
+typedef struct esm_handle esm_handle;
+
+/* Library version (ESM_ABI_VERSION) and the message of the last failure on this thread. */
+int32_t esm_version(void);
+const char *esm_last_error(void);
+
+/*
+ * Create a handle for `nrep` replicas (chains × replicas) of one model, with the device state in
+ * `precision` (ESM_FP64 or ESM_FP32). Replaces the set-up of `run_replicas`
+ * (src/replicas.jl:841-900: `setup`, `LocalModel`, `init_replicas`) as far as the device state
+ * goes. The state starts at zero; upload one with esm_upload_state.
+ */
+int32_t esm_create(const esm_model *model, int32_t nrep, int32_t precision, esm_handle **out);
+int32_t esm_destroy(esm_handle *h);
+
+/*
+ * Device state transfer. v: [ESM_NV][N][T][R] node values of the operation fields; k: [ESM_NK][N]
+ * raw node capacities shared by every replica (src/replicas.jl:702 keeps them bitwise equal);
+ * x: [ESM_NREN][N][T][R] raw node weather (capacity factors before the data map clamp).
+ * Any pointer may be NULL to leave that part unchanged / not downloaded. Uploading v, k or x
+ * rebuilds the residual and flow caches.
+ */
+int32_t esm_upload_state(esm_handle *h, const double *v, const double *k, const double *x);
+int32_t esm_download_state(esm_handle *h, double *v, double *k, double *x);
+
+/*
+ * `energy_cost` of every replica (src/energy.jl:551-677) and `LocalCost` rebuild
+ * (src/local.jl:313-374): rebuilds the residual r [N][T][R] and flow F [L][T][R] caches from
+ * the node arrays, and writes per replica the CostParts [R][ESM_NPARTS] (capex, op, gen,
+ * storage, shed MWh, dump MWh, penalty, co2). `per_hour` ([T][R], may be NULL) receives the
+ * operational cost of every step (`cache_cost` terms, src/local.jl:2065-2098). Terms are
+ * evaluated in the state precision; totals are accumulated in FP64.
+ */
+int32_t esm_energy(esm_handle *h, double *parts, double *per_hour);
+
+/*
+ * Rebuild the residual and flow caches from the node arrays (`rebuild_flat`,
+ * device/src/residual.jl:16) and report per replica the largest drift max|r_cache − r| [R]
+ * (may be NULL).
+ */
+int32_t esm_rebuild_residual(esm_handle *h, double *drift);
+
+/* Download the caches: r [N][T][R] and F [L][T][R] (either may be NULL). */
+int32_t esm_download_caches(esm_handle *h, double *r, double *F);
+
+/*
+ * `capacity_scores` (src/replicas.jl:329-338 via `energy_cost(grad = true)`,
+ * src/energy.jl:680-757): per replica the sampled score ŝ_k = −β_v ∂C/∂k (node capacity
+ * coordinates; 0 where the capacity is clamped), sk [R][ESM_NK][N], and the capex force
+ * f_k = β_k ∂K/∂k, fk [ESM_NK][N] (replica independent). Uses the residual cache, so call
+ * esm_energy or esm_rebuild_residual first when the caches may have drifted.
+ */
+int32_t esm_scores(esm_handle *h, double *sk, double *fk);
+
+/*
+ * `langevin_capacities` / `langevin_update` (src/replicas.jl:368-403) of the shared
+ * capacities, in FP64 on the device: per k field f with eta[f] > 0,
+ * w = log(k + κ_s), w' = w + clamp(η((k + κ_s) g + 1), ±clip) + √(2η) ξ, with
+ * g = −fk + (β_k/β_v) Σ_j ω_j sk_j, reflected at log(kmin + κ_s) and (box != 0) at
+ * log(max(khi, kmin + 1) + κ_s) when crossed from below. weights [R] (ω_j; all replicas),
+ * xi [ESM_NK][N] standard normals (fields with eta = 0 draw none and are ignored), eta and
+ * kappa_s [ESM_NK]. Reads the scores of the last esm_scores call. Writes knew [ESM_NK][N] and
+ * *nclip (may be NULL); does not change the state's k (see esm_follow_capacities).
+ */
+int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, const double *eta,
+                     const double *kappa_s, double kmin, double clip, int32_t box,
+                     double *knew, int32_t *nclip);
+
+/*
+ * Kernel timing with HIP events recorded on the handle's stream around every launch of the
+ * named kernel (the stage timing of the reference, src/stages.jl:26-53, at kernel grain).
+ * esm_profile(h, 1) clears the records and starts recording, esm_profile(h, 0) stops;
+ * esm_profile_read returns the summed event time (ms) and the launch count of one kernel.
+ */
+#define ESM_KERN_ENERGY 0  /* energy, flows and residual caches          */
+#define ESM_KERN_SCORE 1   /* capacity score partial sums                */
+#define ESM_KERN_SWEEP 2   /* exact line moves (all rounds of a sweep)   */
+#define ESM_KERN_WEATHER 3 /* weather process                            */
+#define ESM_NKERN 4
+int32_t esm_profile(esm_handle *h, int32_t enable);
+int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESM_H */

@@ -1,0 +1,42 @@
+/*
+ * esm_oracle.h — FP64 CPU restatement of the reference's per-sweep path (TEST INFRASTRUCTURE).
+ *
+ * This library is the checker of the HIP path: only the tests, the smoke check and the CPU
+ * baseline leg of the benchmark may load it. It is never part of the product path.
+ *
+ * Inputs use the data format of include/esm.h (esm_model) and the state layout
+ * [field][node][hour][replica] with replica fastest.
+ */
+#ifndef ESM_ORACLE_H
+#define ESM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/esm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* energy_cost (value) of every replica plus the LocalCost caches r [N][T][R] and F [L][T][R]
+ * (either may be NULL) and the per-step operational cost per_hour [T][R] (may be NULL). */
+int orc_energy(const esm_model *m, int R, const double *v, const double *k, const double *x,
+               double *parts, double *per_hour, double *r, double *F);
+
+/* capacity_scores: sk [R][ESM_NK][N], fk [ESM_NK][N]; sk_abs (may be NULL) receives the sums of
+ * the absolute values of the summed terms (the scale of the round-off of each entry). */
+int orc_scores(const esm_model *m, int R, const double *v, const double *k, const double *x,
+               double *sk, double *fk, double *sk_abs);
+
+/* langevin_update of every k field with eta[f] > 0 (see esm_langevin). */
+int orc_langevin(const esm_model *m, int R, const double *k, const double *sk, const double *fk,
+                 const double *weights, const double *xi, const double *eta, const double *kappa_s,
+                 double kmin, double clip, int box, double *knew, int *nclip);
+
+/* Philox4x32-10 (Salmon et al., SC'11): four 32-bit outputs of counter ctr under key key. */
+void orc_philox(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,94 @@
+"""ctypes binding of the FP64 CPU oracle (TEST INFRASTRUCTURE).
+
+Only tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` may import this
+module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import sys
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+sys.path.insert(0, str(HERE.parent / "esm-sampling_amd"))
+from esmsampling import _lib as L  # noqa: E402  (the esm_model struct layout only)
+
+NK, NV, NREN, NPARTS = L.NK, L.NV, L.NREN, L.NPARTS
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} missing: run `python oracle/build.py`")
+        lib = C.CDLL(str(LIB_PATH))
+        M = C.POINTER(L.EsmModel)
+        lib.orc_energy.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.orc_scores.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.orc_langevin.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
+                                     _dp, _ip]
+        lib.orc_philox.argtypes = [_u32p, _u32p, _u32p]
+        lib.orc_philox.restype = None
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    def __init__(self, sys):
+        self.sys = sys
+        self.cm, self.keep = L.make_cmodel(sys)
+        self.lib = load()
+
+    def energy(self, v, k, x, want_caches=True):
+        s = self.sys
+        R = v.shape[-1]
+        v, k, x = (np.ascontiguousarray(a, dtype=np.float64) for a in (v, k, x))
+        parts = np.empty((R, NPARTS))
+        ph = np.empty((s.T, R))
+        r = np.empty((s.N, s.T, R)) if want_caches else None
+        F = np.empty((s.L, s.T, R)) if want_caches else None
+        rc = self.lib.orc_energy(C.byref(self.cm), R, _p(v), _p(k), _p(x), _p(parts), _p(ph), _p(r), _p(F))
+        assert rc == 0
+        return parts, ph, r, F
+
+    def scores(self, v, k, x):
+        s = self.sys
+        R = v.shape[-1]
+        v, k, x = (np.ascontiguousarray(a, dtype=np.float64) for a in (v, k, x))
+        sk = np.empty((R, NK, s.N))
+        fk = np.empty((NK, s.N))
+        sa = np.empty((R, NK, s.N))
+        assert self.lib.orc_scores(C.byref(self.cm), R, _p(v), _p(k), _p(x), _p(sk), _p(fk), _p(sa)) == 0
+        return sk, fk, sa
+
+    def langevin(self, k, sk, fk, weights, xi, eta, kappa_s, kmin=1.0, clip=0.02, box=True):
+        R = sk.shape[0]
+        arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (k, sk, fk, weights, xi,
+                                                                   np.broadcast_to(eta, (NK,)),
+                                                                   np.broadcast_to(kappa_s, (NK,)))]
+        knew = np.empty((NK, self.sys.N))
+        nc = C.c_int32(0)
+        assert self.lib.orc_langevin(C.byref(self.cm), R, *[_p(a) for a in arr], float(kmin), float(clip), int(box),
+                                     _p(knew), C.byref(nc)) == 0
+        return knew, int(nc.value)
+
+
+def philox(key, ctr):
+    lib = load()
+    k = (C.c_uint32 * 2)(*key)
+    c = (C.c_uint32 * 4)(*ctr)
+    o = (C.c_uint32 * 4)()
+    lib.orc_philox(k, c, o)
+    return list(o)
