@@ -135,7 +135,8 @@ static void free_handle(esm_handle *h) {
     void *ps[] = {h->d_bus0, h->d_bus1, h->d_adj_ptr, h->d_adj_line, h->d_adj_other, h->mb.lb, h->mb.ls, h->mb.adj_b,
                   h->mb.load, h->mb.mc, h->mb.ef, h->mb.kc, h->mb.hours, h->mb.eta_in, h->mb.eta_out, h->mb.qt,
                   h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,
-                  h->d_out, h->d_sk, h->d_fk};
+                  h->d_out, h->d_sk, h->d_fk, h->d_z, h->d_ytmp, h->d_carry, h->d_zin, h->d_mu, h->d_xbar, h->d_m,
+                  h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -299,6 +300,43 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     zmalloc((void **)&h->d_out, (size_t)std::max(nrep * 8, ESM_NK * N * 2 + 64) * sizeof(double));
     zmalloc((void **)&h->d_sk, (size_t)nrep * ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
+    // weather process (optional: a model without weather tables has none)
+    h->has_weather = mdl->Ptr && mdl->pi0 && mdl->mu && mdl->xbar && mdl->K >= 1 && mdl->K <= ESM_MAXK &&
+                     mdl->sigma > 0 && mdl->sigma0 > 0;
+    if (h->has_weather && !rc) {
+        const int K = mdl->K, nser = ESM_NREN * N;
+        const int nch = (T + h->wx_chunk - 1) / h->wx_chunk;
+        zmalloc(&h->d_z, (size_t)nser * TR * h->es);
+        zmalloc(&h->d_ytmp, (size_t)nser * TR * h->es);
+        zmalloc(&h->d_carry, (size_t)nch * nser * nrep * h->es);
+        zmalloc(&h->d_zin, (size_t)nch * nser * nrep * h->es);
+        zmalloc((void **)&h->d_m, TR * sizeof(int));
+        zmalloc((void **)&h->d_mnew, TR * sizeof(int));
+        zmalloc((void **)&h->d_moved, (size_t)nrep * sizeof(int));
+        zmalloc((void **)&h->d_wpar, 3 * (size_t)nrep * sizeof(double));
+        std::vector<double> mu(mdl->mu, mdl->mu + (size_t)K * nser * 3);
+        h->xbar_h.assign(mdl->xbar, mdl->xbar + (size_t)nser * 24);
+        h->hod_h.resize(T);
+        std::vector<double> harm(2 * (size_t)T);
+        for (int t = 0; t < T; t++) {
+            h->hod_h[t] = (int)fmod(t * mdl->dt, 24.0);
+            harm[2 * t] = cos(2 * M_PI * (t + 1) / T);
+            harm[2 * t + 1] = sin(2 * M_PI * (t + 1) / T);
+        }
+        dmalloc(&h->d_harm, harm);
+        dmalloc(&h->d_Ptr, std::vector<double>(mdl->Ptr, mdl->Ptr + K * K));
+        dmalloc(&h->d_pi0, std::vector<double>(mdl->pi0, mdl->pi0 + K));
+        imalloc(&h->d_hod, h->hod_h);
+        if (!rc) {
+            if (precision == ESM_FP64) {
+                h->d_mu = upload_conv<double>(mu, h->stream, &rc);
+                h->d_xbar = upload_conv<double>(h->xbar_h, h->stream, &rc);
+            } else {
+                h->d_mu = upload_conv<float>(mu, h->stream, &rc);
+                h->d_xbar = upload_conv<float>(h->xbar_h, h->stream, &rc);
+            }
+        }
+    }
     if (rc) {
         set_error(rc == ESM_ENOMEM ? "esm_create: device allocation failed" : "esm_create: HIP copy failed");
         free_handle(h);
@@ -458,6 +496,90 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
     ESM_HIP_CHECK(hipFreeAsync(d, h->stream));
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     if (nclip) *nclip = nc;
+    return ESM_OK;
+}
+
+static int weather_call(esm_handle *h, bool init, int B, int compensate, const uint32_t key[2], uint32_t ctr) {
+    return h->prec == ESM_FP64 ? launch_weather<double>(h, init, B, compensate, key[0], key[1], ctr)
+                               : launch_weather<float>(h, init, B, compensate, key[0], key[1], ctr);
+}
+
+int32_t esm_weather_init(esm_handle *h, const uint32_t key[2], uint32_t counter) {
+    if (!h || !key) { set_error("esm_weather_init: null argument"); return ESM_EINVAL; }
+    if (!h->has_weather) { set_error("esm_weather_init: the model has no weather tables"); return ESM_ESTATE; }
+    const int R = h->R;
+    std::vector<double> wpar(3 * (size_t)R, 0.0);
+    for (int j = 0; j < R; j++) wpar[R + j] = 1.0;    // c = 0, s = 1: z from fresh innovations
+    ESM_HIP_CHECK(hipMemcpyAsync(h->d_wpar, wpar.data(), wpar.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    int rc = weather_call(h, true, 0, 0, key, counter);
+    if (rc) return rc;
+    if ((rc = rebuild(h))) return rc;
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->weather_ready = true;
+    h->scores_valid = false;
+    return ESM_OK;
+}
+
+int32_t esm_weather_step(esm_handle *h, const double *theta, const double *win_prob, int32_t B, int32_t compensate,
+                         const uint32_t key[2], uint32_t counter, int32_t *moved) {
+    if (!h || !theta || !win_prob || !key) { set_error("esm_weather_step: null argument"); return ESM_EINVAL; }
+    if (!h->weather_ready) {
+        set_error("esm_weather_step: no weather drawn or uploaded (esm_weather_init / esm_upload_weather)");
+        return ESM_ESTATE;
+    }
+    if (B < 0 || B > ESM_MAXWIN) { set_error("esm_weather_step: window length out of range"); return ESM_EINVAL; }
+    const int R = h->R;
+    std::vector<double> wpar(3 * (size_t)R);
+    for (int j = 0; j < R; j++) {
+        if (!(theta[j] >= 0) || !(win_prob[j] >= 0)) { set_error("esm_weather_step: negative angle or probability"); return ESM_EINVAL; }
+        wpar[j] = theta[j] > 0 ? cos(theta[j]) : 1.0;
+        wpar[R + j] = theta[j] > 0 ? sin(theta[j]) : 0.0;
+        wpar[2 * R + j] = win_prob[j];
+    }
+    ESM_HIP_CHECK(hipMemcpyAsync(h->d_wpar, wpar.data(), wpar.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    int rc = weather_call(h, false, B, compensate, key, counter);
+    if (rc) return rc;
+    if (moved) ESM_HIP_CHECK(hipMemcpyAsync(moved, h->d_moved, R * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->scores_valid = false;
+    return ESM_OK;
+}
+
+int32_t esm_download_weather(esm_handle *h, int32_t *m, double *z) {
+    if (!h) { set_error("esm_download_weather: null handle"); return ESM_EINVAL; }
+    if (!h->has_weather) { set_error("esm_download_weather: the model has no weather tables"); return ESM_ESTATE; }
+    const size_t TR = (size_t)h->T * h->R;
+    if (m) ESM_HIP_CHECK(hipMemcpyAsync(m, h->d_m, TR * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    int rc;
+    if (z && (rc = get_state(h, z, h->d_z, (size_t)ESM_NREN * h->N * TR))) return rc;
+    return ESM_OK;
+}
+
+int32_t esm_upload_weather(esm_handle *h, const int32_t *m, const double *z) {
+    if (!h || !m || !z) { set_error("esm_upload_weather: null argument"); return ESM_EINVAL; }
+    if (!h->has_weather) { set_error("esm_upload_weather: the model has no weather tables"); return ESM_ESTATE; }
+    const int T = h->T, R = h->R, nser = ESM_NREN * h->N;
+    const size_t TR = (size_t)T * R;
+    for (size_t g = 0; g < TR; g++)
+        if (m[g] < 0 || m[g] >= h->m.K) { set_error("esm_upload_weather: regime out of range"); return ESM_EINVAL; }
+    ESM_HIP_CHECK(hipMemcpyAsync(h->d_m, m, TR * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    int rc;
+    if ((rc = put_state(h, h->d_z, z, (size_t)nser * TR))) return rc;
+    std::vector<double> x((size_t)nser * TR);
+    for (int i = 0; i < nser; i++)
+        for (int t = 0; t < T; t++) {
+            const double xb = h->xbar_h[(size_t)i * 24 + h->hod_h[t]];
+            for (int j = 0; j < R; j++) {
+                const size_t o = ((size_t)i * T + t) * R + j;
+                x[o] = 1.0 / (1.0 + exp(-(xb + z[o])));
+            }
+        }
+    if ((rc = put_state(h, h->d_x, x.data(), x.size()))) return rc;
+    if ((rc = rebuild(h))) return rc;
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->weather_ready = true;
+    h->scores_valid = false;
     return ESM_OK;
 }
 

@@ -12,6 +12,9 @@
 
 namespace esm {
 
+constexpr int ESM_MAXK = 4;       // largest regime count of the weather model
+constexpr int ESM_MAXWIN = 256;   // longest regime window
+
 void set_error(const std::string &msg);
 
 #define ESM_HIP_CHECK(expr)                                                                   \
@@ -62,6 +65,23 @@ struct esm_handle {
     double *d_sk = nullptr;       // [R][ESM_NK][N]
     double *d_fk = nullptr;       // [ESM_NK][N]
     bool scores_valid = false;
+    // weather process
+    bool has_weather = false, weather_ready = false;
+    int wx_chunk = 128;
+    void *d_z = nullptr;        // [3N][T][R] anomalies (S)
+    void *d_ytmp = nullptr;     // [3N][T][R] chunk-local recurrence (S)
+    void *d_carry = nullptr;    // [nchunk][3N][R] (S)
+    void *d_zin = nullptr;      // [nchunk][3N][R] (S)
+    void *d_mu = nullptr;       // [K][3N][3] (S)
+    void *d_xbar = nullptr;     // [3N][24] (S)
+    int *d_m = nullptr, *d_mnew = nullptr;   // [T][R]
+    int *d_hod = nullptr;       // [T]
+    int *d_moved = nullptr;     // [R]
+    double *d_harm = nullptr;   // [T][2]
+    double *d_wpar = nullptr;   // [3][R] cos θ, sin θ, window probability
+    double *d_Ptr = nullptr, *d_pi0 = nullptr;
+    std::vector<double> xbar_h;  // [3N][24]
+    std::vector<int> hod_h;      // [T]
     // kernel timing (esm_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[ESM_NKERN];
@@ -102,5 +122,9 @@ template <typename S>
 int launch_scores(esm_handle *h);
 int launch_langevin(esm_handle *h, const double *d_w, const double *d_xi, const double *d_eta, const double *d_ks,
                     double kmin, double clip, int box, double *d_knew, int *d_nclip);
+
+// k_weather.hip
+template <typename S>
+int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0, uint32_t k1, uint32_t ctr);
 
 }  // namespace esm

@@ -131,6 +131,60 @@ def langevin_capacities(R: Replicas, weights, xi, eta, kappa_s, kmin: float = 1.
     return knew, int(nclip.value)
 
 
+def _key(key):
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    return k
+
+
+def sample_weather(R: Replicas, key=(1, 0), counter: int = 0):
+    """Exact prior draw of every replica's weather (``sample_weather``, src/weather.jl:461-481):
+    regimes, VAR anomalies and the raw weather x; the caches are rebuilt."""
+    check(R.lib.esm_weather_init(R.handle, _key(key), int(counter)), "esm_weather_init")
+
+
+def weather_process_step(R: Replicas, theta, win_prob, window: int = 60, compensate: bool = True, key=(1, 0),
+                         counter: int = 0):
+    """One weather step of every replica (``replica_weather!`` / ``weather_process_step`` and
+    ``compensate_weather!``, src/replicas.jl:179-236): pCN of angle theta[j], a regime window of
+    length ``window`` with probability win_prob[j]. Returns the move flags [R] (0 none, 1 pCN, 2 window)."""
+    th = f64(np.broadcast_to(theta, (R.R,)))
+    wp = f64(np.broadcast_to(win_prob, (R.R,)))
+    moved = np.zeros(R.R, dtype=np.int32)
+    check(R.lib.esm_weather_step(R.handle, dptr(th), dptr(wp), int(window), int(compensate), _key(key), int(counter),
+                                 iptr(moved)), "esm_weather_step")
+    return moved
+
+
+def weather_state(R: Replicas):
+    """(m [T, R] regimes, z [3N, T, R] anomalies) of every replica."""
+    m = np.empty((R.T, R.R), dtype=np.int32)
+    z = np.empty((NREN * R.N, R.T, R.R))
+    check(R.lib.esm_download_weather(R.handle, iptr(m), dptr(z)), "esm_download_weather")
+    return m, z
+
+
+def set_weather(R: Replicas, m, z):
+    m = np.ascontiguousarray(m, dtype=np.int32)
+    z = f64(z)
+    if m.shape != (R.T, R.R) or z.shape != (NREN * R.N, R.T, R.R):
+        raise ValueError("m must be [T, R] and z [3N, T, R]")
+    check(R.lib.esm_upload_weather(R.handle, iptr(m), dptr(z)), "esm_upload_weather")
+
+
+def weather_schedule(it: int, j: int, M: int, theta: float, burst_every: int, phi: float):
+    """The weather angle of replica j (0-based) at step ``it`` and whether it is a burst
+    (src/replicas.jl:571-576): bursts every ``burst_every`` steps, staggered by
+    (j)·⌊burst_every/M⌋; a burst composes the continuous angle θ with φ."""
+    import math
+    nb = burst_every
+    burst = nb > 0 and (it + j * (nb // M)) % nb == 0
+    if burst:
+        th = math.acos(max(-1.0, min(1.0, math.cos(theta) * math.cos(phi)))) if theta > 0 else phi
+    else:
+        th = theta
+    return th, burst
+
+
 def uniform_weights(active):
     """Weights 1/M_active over the active replicas (all if none is active), src/replicas.jl:310-314."""
     a = np.asarray(active, dtype=bool)

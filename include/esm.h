@@ -169,6 +169,37 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
                      double *knew, int32_t *nclip);
 
 /*
+ * Weather process of every replica (src/replicas.jl:29-236, src/weather.jl:461-481), with
+ * counter-based Philox4x32-10 random numbers under `key`; `counter` must differ between calls
+ * that should draw independent numbers (the streams of the reference are per replica,
+ * src/replicas.jl:872-874; here a replica's numbers are addressed by its index).
+ *
+ * esm_weather_init: an exact prior draw per replica (`sample_weather`): the regime chain from
+ * π0 and Ptr, then the VAR(1) recursion; sets the raw weather x = sigmoid(xbar + z) and
+ * rebuilds the caches. The operation state v is left as it is.
+ */
+int32_t esm_weather_init(esm_handle *h, const uint32_t key[2], uint32_t counter);
+
+/*
+ * esm_weather_step: one step of every replica's weather process (`replica_weather!` ->
+ * `weather_process_step`, src/replicas.jl:179-197 and :586-607): a pCN step of angle theta[j]
+ * in the innovations at fixed regimes (theta[j] = 0: none), then, with probability
+ * win_prob[j] (one uniform of the replica), a regime window move of length B (exact Gibbs
+ * redraw of the regimes in a uniformly placed window by backward messages, z re-built from its
+ * innovations at the new regimes). With `compensate`, `compensate_weather!` follows: at every
+ * node and step the gas unit absorbs the change of the residual (up to its headroom, or its
+ * output). moved (may be NULL) receives [R] flags: 1 pCN only, 2 window (and pCN if theta > 0),
+ * 0 untouched.
+ */
+int32_t esm_weather_step(esm_handle *h, const double *theta, const double *win_prob, int32_t B,
+                         int32_t compensate, const uint32_t key[2], uint32_t counter, int32_t *moved);
+
+/* Regimes m [T][R] (0-based) and the VAR anomalies z [ESM_NREN*N][T][R] (either may be NULL). */
+int32_t esm_download_weather(esm_handle *h, int32_t *m, double *z);
+/* Set the regimes and anomalies; x = sigmoid(xbar + z) follows and the caches are rebuilt. */
+int32_t esm_upload_weather(esm_handle *h, const int32_t *m, const double *z);
+
+/*
  * Kernel timing with HIP events recorded on the handle's stream around every launch of the
  * named kernel (the stage timing of the reference, src/stages.jl:26-53, at kernel grain).
  * esm_profile(h, 1) clears the records and starts recording, esm_profile(h, 0) stops;

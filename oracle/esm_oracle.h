@@ -32,6 +32,18 @@ int orc_langevin(const esm_model *m, int R, const double *k, const double *sk, c
                  const double *weights, const double *xi, const double *eta, const double *kappa_s,
                  double kmin, double clip, int box, double *knew, int *nclip);
 
+/* Weather process (oracle_weather.c): prior draw of regimes mreg [T][R], anomalies z [3N][T][R]
+ * and raw weather x [3][N][T][R]; one weather step of every replica with the compensated
+ * re-dispatch of the gas utilization in v (see esm_weather_step). */
+int orc_weather_init(const esm_model *m, int R, const uint32_t key[2], uint32_t ctr, int *mreg, double *z, double *x);
+int orc_weather_step(const esm_model *m, int R, const double *theta, const double *win_prob, int B, int compensate,
+                     const uint32_t key[2], uint32_t ctr, int *mreg, double *z, double *x, double *v, const double *k,
+                     int *moved);
+/* Explicit-uniform building blocks (the reference's own explicit forms): categorical inversion
+ * (0-based result) and the regime window redraw of one regime path m [T] over [a, b] (1-based). */
+int orc_categorical(const double *p, int K, double u);
+void orc_regime_window_path(const esm_model *m, int *path, int a, int b, const double *us);
+
 /* Philox4x32-10 (Salmon et al., SC'11): four 32-bit outputs of counter ctr under key key. */
 void orc_philox(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
 

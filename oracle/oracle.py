@@ -37,6 +37,13 @@ def load():
                                      _dp, _ip]
         lib.orc_philox.argtypes = [_u32p, _u32p, _u32p]
         lib.orc_philox.restype = None
+        lib.orc_weather_init.argtypes = [M, C.c_int, _u32p, C.c_uint32, _ip, _dp, _dp]
+        lib.orc_weather_step.argtypes = [M, C.c_int, _dp, _dp, C.c_int, C.c_int, _u32p, C.c_uint32, _ip, _dp, _dp,
+                                         _dp, _dp, _ip]
+        lib.orc_categorical.argtypes = [_dp, C.c_int, C.c_double]
+        lib.orc_categorical.restype = C.c_int
+        lib.orc_regime_window_path.argtypes = [M, _ip, C.c_int, C.c_int, _dp]
+        lib.orc_regime_window_path.restype = None
         _lib = lib
     return _lib
 
@@ -83,6 +90,43 @@ class Oracle:
         assert self.lib.orc_langevin(C.byref(self.cm), R, *[_p(a) for a in arr], float(kmin), float(clip), int(box),
                                      _p(knew), C.byref(nc)) == 0
         return knew, int(nc.value)
+
+
+    def weather_init(self, R, key=(1, 0), counter=0):
+        s = self.sys
+        m = np.empty((s.T, R), dtype=np.int32)
+        z = np.empty((NREN * s.N, s.T, R))
+        x = np.empty((NREN, s.N, s.T, R))
+        k = (C.c_uint32 * 2)(*key)
+        assert self.lib.orc_weather_init(C.byref(self.cm), R, k, int(counter), m.ctypes.data_as(_ip), _p(z), _p(x)) == 0
+        return m, z, x
+
+    def weather_step(self, m, z, x, v, kcap, theta, win_prob, window=60, compensate=True, key=(1, 0), counter=0):
+        """In place on copies: returns (m, z, x, v, moved)."""
+        R = m.shape[1]
+        m = np.ascontiguousarray(m, dtype=np.int32).copy()
+        z, x, v = (np.ascontiguousarray(a, dtype=np.float64).copy() for a in (z, x, v))
+        kcap = np.ascontiguousarray(kcap, dtype=np.float64)
+        th = np.ascontiguousarray(np.broadcast_to(theta, (R,)), dtype=np.float64)
+        wp = np.ascontiguousarray(np.broadcast_to(win_prob, (R,)), dtype=np.float64)
+        moved = np.zeros(R, dtype=np.int32)
+        k = (C.c_uint32 * 2)(*key)
+        assert self.lib.orc_weather_step(C.byref(self.cm), R, _p(th), _p(wp), int(window), int(compensate), k,
+                                         int(counter), m.ctypes.data_as(_ip), _p(z), _p(x), _p(v), _p(kcap),
+                                         moved.ctypes.data_as(_ip)) == 0
+        return m, z, x, v, moved
+
+    def regime_window_path(self, path, a, b, us):
+        path = np.ascontiguousarray(path, dtype=np.int32).copy()
+        us = np.ascontiguousarray(us, dtype=np.float64)
+        self.lib.orc_regime_window_path(C.byref(self.cm), path.ctypes.data_as(_ip), int(a), int(b), _p(us))
+        return path
+
+
+def categorical(p, u):
+    lib = load()
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    return lib.orc_categorical(_p(p), len(p), float(u))
 
 
 def philox(key, ctr):

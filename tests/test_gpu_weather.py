@@ -1,0 +1,70 @@
+"""GPU parity of the weather process with the oracle: the same Philox numbers give bit-identical
+regimes and move decisions; anomalies, weather and the compensated gas utilization within the
+precision tolerance (FP64 1e-12, FP32 1e-5 of the quantity's scale)."""
+import numpy as np
+import pytest
+
+import oracle
+from esmsampling import FP32, FP64, api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+def test_weather_init_matches_oracle(prec):
+    print(f"weather init prec={prec}", flush=True)
+    s = synth.make_system(N=8, L=12, T=500, seed=2)
+    R = 6
+    dev = api.Replicas(s, R, prec)
+    api.sample_weather(dev, key=(11, 5), counter=7)
+    m, z = api.weather_state(dev)
+    _, _, x = dev.download()
+    m_, z_, x_ = oracle.Oracle(s).weather_init(R, key=(11, 5), counter=7)
+    assert np.array_equal(m, m_)
+    tol = 1e-12 if prec == FP64 else 2e-5
+    np.testing.assert_allclose(z, z_, atol=tol * 10)
+    np.testing.assert_allclose(x, x_, atol=tol)
+    dev.close()
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32])
+@pytest.mark.parametrize("compensate", [True, False])
+def test_weather_step_matches_oracle(prec, compensate):
+    print(f"weather step prec={prec} compensate={compensate}", flush=True)
+    s = synth.make_system(N=8, L=12, T=700, seed=3)
+    R = 8
+    o = oracle.Oracle(s)
+    m, z, x = o.weather_init(R, key=(2, 2), counter=0)
+    if prec == FP32:
+        z = z.astype(np.float32).astype(np.float64)
+    x = synth.weather_from_z(s, z)
+    if prec == FP32:
+        x = x.astype(np.float32).astype(np.float64)
+    v = synth.heuristic_state(s, x)
+    if prec == FP32:
+        v = v.astype(np.float32).astype(np.float64)
+    k = s.k0
+    dev = api.Replicas(s, R, prec)
+    dev.upload(v=v, k=k, x=x)
+    api.set_weather(dev, m, z)
+    dev.upload(x=x)           # the uploaded x (rounded like the oracle's input)
+    theta = np.array([0.3, 0.0, 0.3, 0.0, 0.1, 0.5, 0.0, 0.2])
+    wp = np.array([0.0, 1.0, 1.0, 0.0, 0.5, 0.5, 0.5, 1.0])
+    for it in range(3):
+        moved = api.weather_process_step(dev, theta, wp, window=60, compensate=compensate, key=(5, 6), counter=it)
+        m, z, x, v, moved_ = o.weather_step(m, z, x, v, k, theta, wp, window=60, compensate=compensate,
+                                            key=(5, 6), counter=it)
+        assert np.array_equal(moved, moved_)
+    md, zd = api.weather_state(dev)
+    vd, _, xd = dev.download()
+    assert np.array_equal(md, m)
+    assert np.all(moved_[3] == 0)
+    tol = 1e-11 if prec == FP64 else 5e-5
+    np.testing.assert_allclose(zd, z, atol=tol * 10)
+    np.testing.assert_allclose(xd, x, atol=tol)
+    np.testing.assert_allclose(vd, v, atol=tol, rtol=tol)
+    # the residual cache follows the weather and the re-dispatch
+    r, F = dev.caches()
+    r_ = o.energy(v, k, x)[2]
+    np.testing.assert_allclose(r, r_, atol=(1e-9 if prec == FP64 else 1e-5) * np.abs(s.load).max())
+    dev.close()
