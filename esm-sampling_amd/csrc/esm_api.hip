@@ -136,7 +136,7 @@ static void free_handle(esm_handle *h) {
                   h->mb.load, h->mb.mc, h->mb.ef, h->mb.kc, h->mb.hours, h->mb.eta_in, h->mb.eta_out, h->mb.qt,
                   h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,
                   h->d_out, h->d_sk, h->d_fk, h->d_z, h->d_ytmp, h->d_carry, h->d_zin, h->d_mu, h->d_xbar, h->d_m,
-                  h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0};
+                  h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0, h->d_theta_nodes, h->d_stats};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -229,6 +229,62 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
             h->adj_bv[qb] = h->line_b[e];
         }
     }
+    // sweep schedule: distance-2 colouring of the nodes (greedy in node order: two nodes of one
+    // colour share no line and no neighbour, so their angle moves touch disjoint terms), degree
+    // buckets of the angle kernels, and the time-parity classes of the storage moves
+    // (src/local.jl:1948-1950: odd and even steps, three classes when T is odd)
+    {
+        std::vector<std::vector<int>> nbr(N);
+        for (int e = 0; e < L; e++) {
+            nbr[h->bus0[e]].push_back(h->bus1[e]);
+            nbr[h->bus1[e]].push_back(h->bus0[e]);
+        }
+        h->colour.assign(N, -1);
+        std::vector<char> used;
+        for (int n = 0; n < N; n++) {
+            used.assign(N + 1, 0);
+            for (int a : nbr[n]) {
+                if (h->colour[a] >= 0) used[h->colour[a]] = 1;
+                for (int b : nbr[a])
+                    if (b != n && h->colour[b] >= 0) used[h->colour[b]] = 1;
+            }
+            int c = 0;
+            while (used[c]) c++;
+            h->colour[n] = c;
+            h->ncolour = std::max(h->ncolour, c + 1);
+        }
+        std::vector<int> flat;
+        const int buckets[3] = {4, 8, 16};
+        for (int n = 0; n < N; n++)
+            if ((int)nbr[n].size() > 16) {
+                set_error("esm_create: node " + std::to_string(n) + " has more than 16 lines");
+                delete h;
+                return ESM_EINVAL;
+            }
+        for (int c = 0; c < h->ncolour; c++)
+            for (int bi = 0; bi < 3; bi++) {
+                const int lo = bi == 0 ? 0 : buckets[bi - 1] + 1, hi = buckets[bi];
+                esm_handle::ThetaRound tr{c, hi, (int)flat.size(), 0};
+                for (int n = 0; n < N; n++) {
+                    const int d = (int)nbr[n].size();
+                    if (h->colour[n] == c && d >= lo && d <= hi && d > 0) {
+                        flat.push_back(n);
+                        tr.count++;
+                    }
+                }
+                if (tr.count) h->theta_rounds.push_back(tr);
+            }
+        h->tclass.clear();
+        if (T % 2 == 0) {
+            h->tclass.push_back({0, T / 2});
+            h->tclass.push_back({1, T / 2});
+        } else {
+            h->tclass.push_back({0, (T - 1) / 2});
+            h->tclass.push_back({1, (T - 1) / 2});
+            h->tclass.push_back({T - 1, 1});
+        }
+        h->theta_nodes_h = flat;
+    }
     int rc = 0;
     if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
         set_error("esm_create: no HIP device or stream");
@@ -250,6 +306,7 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     imalloc(&h->d_adj_ptr, h->adj_ptr);
     imalloc(&h->d_adj_line, h->adj_line);
     imalloc(&h->d_adj_other, h->adj_other);
+    if (!h->theta_nodes_h.empty()) imalloc(&h->d_theta_nodes, h->theta_nodes_h);
     dmalloc(&h->d_k, h->k);
     dmalloc(&h->d_klo, h->klo);
     dmalloc(&h->d_khi, h->khi);
@@ -300,6 +357,7 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     zmalloc((void **)&h->d_out, (size_t)std::max(nrep * 8, ESM_NK * N * 2 + 64) * sizeof(double));
     zmalloc((void **)&h->d_sk, (size_t)nrep * ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
+    zmalloc((void **)&h->d_stats, 16 * sizeof(unsigned long long));
     // weather process (optional: a model without weather tables has none)
     h->has_weather = mdl->Ptr && mdl->pi0 && mdl->mu && mdl->xbar && mdl->K >= 1 && mdl->K <= ESM_MAXK &&
                      mdl->sigma > 0 && mdl->sigma0 > 0;
@@ -496,6 +554,32 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
     ESM_HIP_CHECK(hipFreeAsync(d, h->stream));
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     if (nclip) *nclip = nc;
+    return ESM_OK;
+}
+
+int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
+                   int64_t *stats) {
+    if (!h || !key) { set_error("esm_sweeps: null argument"); return ESM_EINVAL; }
+    if (nsweeps < 0 || !(maxnats > 0)) { set_error("esm_sweeps: nsweeps must be >= 0 and maxnats > 0"); return ESM_EINVAL; }
+    ESM_HIP_CHECK(hipMemsetAsync(h->d_stats, 0, 16 * sizeof(unsigned long long), h->stream));
+    int rc = h->prec == ESM_FP64 ? launch_sweeps<double>(h, nsweeps, key[0], key[1], counter, maxnats)
+                                 : launch_sweeps<float>(h, nsweeps, key[0], key[1], counter, maxnats);
+    if (rc) return rc;
+    if (stats) {
+        unsigned long long st[16];
+        ESM_HIP_CHECK(hipMemcpyAsync(st, h->d_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+        for (int i = 0; i < 16; i++) stats[i] = (int64_t)st[i];
+    }
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->scores_valid = false;
+    return ESM_OK;
+}
+
+int32_t esm_schedule(esm_handle *h, int32_t *colour, int32_t *ncolours) {
+    if (!h) { set_error("esm_schedule: null handle"); return ESM_EINVAL; }
+    if (colour) std::copy(h->colour.begin(), h->colour.end(), colour);
+    if (ncolours) *ncolours = h->ncolour;
     return ESM_OK;
 }
 

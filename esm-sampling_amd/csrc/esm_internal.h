@@ -82,6 +82,17 @@ struct esm_handle {
     double *d_Ptr = nullptr, *d_pi0 = nullptr;
     std::vector<double> xbar_h;  // [3N][24]
     std::vector<int> hod_h;      // [T]
+    // sweep schedule
+    struct ThetaRound {
+        int colour, dmax, off, count;
+    };
+    std::vector<int> colour;                      // [N] distance-2 colouring
+    int ncolour = 0;
+    std::vector<ThetaRound> theta_rounds;         // per (colour, degree bucket)
+    std::vector<int> theta_nodes_h;
+    int *d_theta_nodes = nullptr;
+    std::vector<std::pair<int, int>> tclass;      // time-parity classes: (first step, count), stride 2
+    unsigned long long *d_stats = nullptr;        // [4 families][4]
     // kernel timing (esm_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[ESM_NKERN];
@@ -122,6 +133,10 @@ template <typename S>
 int launch_scores(esm_handle *h);
 int launch_langevin(esm_handle *h, const double *d_w, const double *d_xi, const double *d_eta, const double *d_ks,
                     double kmin, double clip, int box, double *d_knew, int *d_nclip);
+
+// k_sweep.hip
+template <typename S>
+int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t ctr0, double maxnats);
 
 // k_weather.hip
 template <typename S>

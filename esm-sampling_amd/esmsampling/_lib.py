@@ -83,6 +83,9 @@ SIGNATURES = [
     ("esm_download_caches", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("esm_scores", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("esm_langevin", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32, _dp, _ip]),
+    ("esm_sweeps", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32), C.c_uint32, C.c_double,
+                               C.POINTER(C.c_int64)]),
+    ("esm_schedule", C.c_int32, [C.c_void_p, _ip, _ip]),
     ("esm_weather_init", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32]),
     ("esm_weather_step", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int32, C.c_int32, C.POINTER(C.c_uint32), C.c_uint32,
                                      _ip]),

@@ -136,6 +136,27 @@ def _key(key):
     return k
 
 
+FAMILY_NAMES = ("gen", "theta", "soc", "spair")
+
+
+def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0, maxnats: float = 40.0):
+    """``operation_sweep!`` / ``local_sweep!`` (src/local.jl:1927-2062, :2401-2425) of every replica,
+    ``nsweeps`` times at fixed capacities and weather. Returns the move statistics [4 families, 4]
+    (attempted, accepted, improper, point-cap)."""
+    st = np.zeros(16, dtype=np.int64)
+    check(R.lib.esm_sweeps(R.handle, int(nsweeps), _key(key), int(counter) & 0xFFFFFFFF, float(maxnats),
+                           st.ctypes.data_as(C.POINTER(C.c_int64))), "esm_sweeps")
+    return st.reshape(4, 4)
+
+
+def schedule(R: Replicas):
+    """(colour [N], number of colours) of the angle rounds."""
+    col = np.zeros(R.N, dtype=np.int32)
+    nc = C.c_int32()
+    check(R.lib.esm_schedule(R.handle, iptr(col), C.byref(nc)), "esm_schedule")
+    return col, int(nc.value)
+
+
 def sample_weather(R: Replicas, key=(1, 0), counter: int = 0):
     """Exact prior draw of every replica's weather (``sample_weather``, src/weather.jl:461-481):
     regimes, VAR anomalies and the raw weather x; the caches are rebuilt."""

@@ -169,6 +169,26 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
                      double *knew, int32_t *nclip);
 
 /*
+ * `operation_sweep!` -> `local_sweep!` (src/local.jl:1927-2062, :2401-2425) of every replica,
+ * `nsweeps` times, at fixed capacities and weather: exact line-conditional moves drawn from
+ * exp(−β_v ΔC(t)) (`sample_line!`, :1501-1560; `maxnats` is the walk's stop, 40 in the
+ * reference). One sweep runs, in this order, conflict-free rounds of every family:
+ *   0 gas utilization atoms at every (node, step);
+ *   1 voltage-angle (potential) atoms, one round per colour of a distance-2 colouring;
+ *   2 state-of-charge atoms, per storage carrier and time-parity class;
+ *   3 storage–gas pairs (`prep_spair!`, :1708-1726), per storage carrier and parity class.
+ * Each (site, replica) draws its two uniforms from Philox4x32-10 under `key` with counter
+ * (replica, node·T + step, round id, counter + sweep index). The residual and flow caches are
+ * updated incrementally. stats ([4][4], may be NULL): per family the moves attempted,
+ * accepted (t ≠ 0), improper lines (t = 0) and walks cut at the point cap (t = 0).
+ */
+int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
+                   int64_t *stats);
+
+/* The sweep schedule: the colour of every node [N] and the number of colours. */
+int32_t esm_schedule(esm_handle *h, int32_t *colour, int32_t *ncolours);
+
+/*
  * Weather process of every replica (src/replicas.jl:29-236, src/weather.jl:461-481), with
  * counter-based Philox4x32-10 random numbers under `key`; `counter` must differ between calls
  * that should draw independent numbers (the streams of the reference are per replica,

@@ -44,6 +44,22 @@ int orc_weather_step(const esm_model *m, int R, const double *theta, const doubl
 int orc_categorical(const double *p, int K, double u);
 void orc_regime_window_path(const esm_model *m, int *path, int a, int b, const double *us);
 
+/* Operation sweeps (oracle_sweep.c), in place on v and the caches r, F (built by orc_energy);
+ * stats [4][4] as esm_sweeps; nthreads OpenMP threads over the replicas. */
+int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
+               const uint32_t key[2], uint32_t ctr0, double maxnats, int64_t *stats, int nthreads);
+int orc_colouring(const esm_model *m, int *colour);
+/* One move of family fam (0 gas, 1 potential, 2 SoC, 3 storage–gas pair) at (n, τ) of replica j
+ * with given uniforms (no commit); its ΔC(t) at given t; its commit at t. */
+int orc_move_draw(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int fam, int s,
+                  int n, int tau, double maxnats, double useg, double uoff, double *t_out, int *status);
+int orc_move_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int fam, int s,
+                  int n, int tau, const double *t, int nt, double *dc);
+int orc_move_apply(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int fam, int s,
+                   int n, int tau, double t);
+double orc_logseg(double L, double x);
+double orc_trexp_offset(double g, double L, double u);
+
 /* Philox4x32-10 (Salmon et al., SC'11): four 32-bit outputs of counter ctr under key key. */
 void orc_philox(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
 

@@ -44,6 +44,20 @@ def load():
         lib.orc_categorical.restype = C.c_int
         lib.orc_regime_window_path.argtypes = [M, _ip, C.c_int, C.c_int, _dp]
         lib.orc_regime_window_path.restype = None
+        i64p = C.POINTER(C.c_int64)
+        lib.orc_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double, i64p,
+                                   C.c_int]
+        lib.orc_colouring.argtypes = [M, _ip]
+        lib.orc_move_draw.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_double, C.c_double, C.c_double, _dp, _ip]
+        lib.orc_move_cost.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      _dp, C.c_int, _dp]
+        lib.orc_move_apply.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_double]
+        lib.orc_logseg.argtypes = [C.c_double, C.c_double]
+        lib.orc_logseg.restype = C.c_double
+        lib.orc_trexp_offset.argtypes = [C.c_double, C.c_double, C.c_double]
+        lib.orc_trexp_offset.restype = C.c_double
         _lib = lib
     return _lib
 
@@ -115,6 +129,53 @@ class Oracle:
                                          int(counter), m.ctypes.data_as(_ip), _p(z), _p(x), _p(v), _p(kcap),
                                          moved.ctypes.data_as(_ip)) == 0
         return m, z, x, v, moved
+
+    def sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1):
+        """Operation sweeps in place on copies; returns (v, r, F, stats [4, 4])."""
+        R = v.shape[-1]
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        if r is None or F is None:
+            _, _, r, F = self.energy(v, k, x)
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        F = np.ascontiguousarray(F, dtype=np.float64).copy()
+        st = np.zeros(16, dtype=np.int64)
+        kk = (C.c_uint32 * 2)(*key)
+        rc = self.lib.orc_sweeps(C.byref(self.cm), R, _p(v), _p(k), _p(r), _p(F), int(nsweeps), kk, int(counter),
+                                 float(maxnats), st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
+        assert rc == 0
+        return v, r, F, st.reshape(4, 4)
+
+    def colouring(self):
+        col = np.zeros(self.sys.N, dtype=np.int32)
+        nc = self.lib.orc_colouring(C.byref(self.cm), col.ctypes.data_as(_ip))
+        return col, nc
+
+    def _mv_args(self, v, k, r, F):
+        return [_p(v), _p(k), _p(r), _p(F)]
+
+    def move_draw(self, v, k, r, F, j, fam, s, n, tau, useg, uoff, maxnats=40.0):
+        R = v.shape[-1]
+        t = C.c_double()
+        st = C.c_int32()
+        rc = self.lib.orc_move_draw(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, fam, s, n, tau,
+                                    float(maxnats), float(useg), float(uoff), C.byref(t), C.byref(st))
+        assert rc == 0, rc
+        return t.value, st.value
+
+    def move_cost(self, v, k, r, F, j, fam, s, n, tau, ts):
+        R = v.shape[-1]
+        ts = np.ascontiguousarray(ts, dtype=np.float64)
+        out = np.empty_like(ts)
+        rc = self.lib.orc_move_cost(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, fam, s, n, tau, _p(ts),
+                                    len(ts), _p(out))
+        assert rc == 0, rc
+        return out
+
+    def move_apply(self, v, k, r, F, j, fam, s, n, tau, t):
+        R = v.shape[-1]
+        rc = self.lib.orc_move_apply(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, fam, s, n, tau, float(t))
+        assert rc == 0, rc
 
     def regime_window_path(self, path, a, b, us):
         path = np.ascontiguousarray(path, dtype=np.int32).copy()
