@@ -576,6 +576,56 @@ int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32
     return ESM_OK;
 }
 
+int32_t esm_follow_capacities(esm_handle *h, const double *knew, double kmin, double *kout) {
+    if (!h || !knew) { set_error("esm_follow_capacities: null argument"); return ESM_EINVAL; }
+    const int N = h->N, n = ESM_NK * N;
+    std::vector<double> k0 = h->k, k1(n), knext(n), kc0(n), kc1(n);
+    std::vector<int> moves(n, 0);
+    for (int u = 0; u < n; u++) {
+        if (!(knew[u] > 0) || !std::isfinite(knew[u])) {
+            set_error("esm_follow_capacities: capacities must be positive and finite");
+            return ESM_EINVAL;
+        }
+        const int f = u / N;
+        if (f != ESM_KGAS && k0[u] >= kmin) {
+            k1[u] = k0[u] * exp(log(knew[u] / k0[u]));   // the kop curve at u = log(k'/k)
+            moves[u] = 1;
+        } else {
+            k1[u] = knew[u];
+        }
+        knext[u] = k0[u] + (k1[u] - k0[u]);
+        kc0[u] = std::min(std::max(k0[u], h->klo[u]), h->khi[u]);
+        kc1[u] = std::min(std::max(k1[u], h->klo[u]), h->khi[u]);
+    }
+    // device staging: kc0, kc1 in the state precision, then the move flags
+    void *d = nullptr;
+    const size_t bytes = 2 * n * h->es + n * sizeof(int);
+    ESM_HIP_CHECK(hipMallocAsync(&d, bytes, h->stream));
+    if (h->prec == ESM_FP64) {
+        ESM_HIP_CHECK(hipMemcpyAsync(d, kc0.data(), n * 8, hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipMemcpyAsync((char *)d + n * 8, kc1.data(), n * 8, hipMemcpyHostToDevice, h->stream));
+    } else {
+        std::vector<float> a(kc0.begin(), kc0.end()), b(kc1.begin(), kc1.end());
+        ESM_HIP_CHECK(hipMemcpyAsync(d, a.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipMemcpyAsync((char *)d + n * 4, b.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    }
+    int *d_moves = (int *)((char *)d + 2 * n * h->es);
+    ESM_HIP_CHECK(hipMemcpyAsync(d_moves, moves.data(), n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    int rc = h->prec == ESM_FP64 ? launch_follow<double>(h, d, (char *)d + n * 8, d_moves, kmin)
+                                 : launch_follow<float>(h, d, (char *)d + n * 4, d_moves, kmin);
+    if (rc) return rc;
+    ESM_HIP_CHECK(hipFreeAsync(d, h->stream));
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    h->k = knext;
+    if ((rc = upload_k(h))) return rc;
+    if ((rc = rebuild(h))) return rc;
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (kout) std::copy(knext.begin(), knext.end(), kout);
+    h->scores_valid = false;
+    return ESM_OK;
+}
+
 int32_t esm_schedule(esm_handle *h, int32_t *colour, int32_t *ncolours) {
     if (!h) { set_error("esm_schedule: null handle"); return ESM_EINVAL; }
     if (colour) std::copy(h->colour.begin(), h->colour.end(), colour);

@@ -138,6 +138,10 @@ int launch_langevin(esm_handle *h, const double *d_w, const double *d_xi, const 
 template <typename S>
 int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t ctr0, double maxnats);
 
+// k_follow.hip
+template <typename S>
+int launch_follow(esm_handle *h, const void *d_kc0, const void *d_kc1, const int *d_moves, double kmin);
+
 // k_weather.hip
 template <typename S>
 int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0, uint32_t k1, uint32_t ctr);

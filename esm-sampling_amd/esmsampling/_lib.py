@@ -86,6 +86,7 @@ SIGNATURES = [
     ("esm_sweeps", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32), C.c_uint32, C.c_double,
                                C.POINTER(C.c_int64)]),
     ("esm_schedule", C.c_int32, [C.c_void_p, _ip, _ip]),
+    ("esm_follow_capacities", C.c_int32, [C.c_void_p, _dp, C.c_double, _dp]),
     ("esm_weather_init", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32]),
     ("esm_weather_step", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int32, C.c_int32, C.POINTER(C.c_uint32), C.c_uint32,
                                      _ip]),

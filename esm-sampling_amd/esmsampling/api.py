@@ -149,6 +149,17 @@ def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0,
     return st.reshape(4, 4)
 
 
+def follow_capacities(R: Replicas, knew, kmin: float = 1.0):
+    """``follow_capacities!`` (src/replicas.jl:448-472): move the shared capacities to ``knew`` [6, N]
+    with the dispatch following on every replica; returns the committed capacities [6, N]."""
+    kn = f64(knew)
+    if kn.shape != (NK, R.N):
+        raise ValueError("knew must be [6, N]")
+    kout = np.empty((NK, R.N))
+    check(R.lib.esm_follow_capacities(R.handle, dptr(kn), float(kmin), dptr(kout)), "esm_follow_capacities")
+    return kout
+
+
 def schedule(R: Replicas):
     """(colour [N], number of colours) of the angle rounds."""
     col = np.zeros(R.N, dtype=np.int32)

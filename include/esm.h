@@ -185,6 +185,18 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
 int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
                    int64_t *stats);
 
+/*
+ * `follow_capacities!` (src/replicas.jl:448-472) of every replica: move the shared capacities
+ * from the current ones to knew [ESM_NK][N] with the dispatch following: gas at fixed output
+ * (utilizations × c(k0)/c(k1), :412-435), then every renewable and storage unit with
+ * k0 >= kmin along its kop curve at u = log(knew/k0) (src/kop.jl:200-271: fixed utilizations,
+ * the node's gas unit following share-preservingly); units below kmin move their capacity
+ * alone. The caches are rebuilt. kout [ESM_NK][N] (may be NULL) receives the capacities as
+ * committed: k0 + (k0 e^u − k0), the same bits in every replica and every call with the same
+ * inputs (the reference asserts this, :702).
+ */
+int32_t esm_follow_capacities(esm_handle *h, const double *knew, double kmin, double *kout);
+
 /* The sweep schedule: the colour of every node [N] and the number of colours. */
 int32_t esm_schedule(esm_handle *h, int32_t *colour, int32_t *ncolours);
 

@@ -60,6 +60,9 @@ int orc_move_apply(const esm_model *m, int R, double *v, const double *k, double
 double orc_logseg(double L, double x);
 double orc_trexp_offset(double g, double L, double u);
 
+/* follow_capacities! (oracle_follow.c): in place on v (gas utilization) and k. */
+int orc_follow(const esm_model *m, int R, double *v, double *k, const double *x, const double *knew, double kmin);
+
 /* Philox4x32-10 (Salmon et al., SC'11): four 32-bit outputs of counter ctr under key key. */
 void orc_philox(const uint32_t key[2], const uint32_t ctr[4], uint32_t out[4]);
 

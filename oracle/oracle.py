@@ -54,6 +54,7 @@ def load():
                                       _dp, C.c_int, _dp]
         lib.orc_move_apply.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        C.c_double]
+        lib.orc_follow.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_double]
         lib.orc_logseg.argtypes = [C.c_double, C.c_double]
         lib.orc_logseg.restype = C.c_double
         lib.orc_trexp_offset.argtypes = [C.c_double, C.c_double, C.c_double]
@@ -145,6 +146,16 @@ class Oracle:
                                  float(maxnats), st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
         assert rc == 0
         return v, r, F, st.reshape(4, 4)
+
+    def follow(self, v, k, x, knew, kmin=1.0):
+        """follow_capacities! on copies: returns (v, k)."""
+        R = v.shape[-1]
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        k = np.ascontiguousarray(k, dtype=np.float64).copy()
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        kn = np.ascontiguousarray(knew, dtype=np.float64)
+        assert self.lib.orc_follow(C.byref(self.cm), R, _p(v), _p(k), _p(x), _p(kn), float(kmin)) == 0
+        return v, k
 
     def colouring(self):
         col = np.zeros(self.sys.N, dtype=np.int32)
