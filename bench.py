@@ -1,17 +1,23 @@
 #!/usr/bin/env python3
-"""Throughput of the replica sampler's per-sweep path on one MI355X.
+"""Throughput of the plain GPU replica sampler on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas R] [--hours T] [--fp64]
 
-A step is one replica step of the plain GPU sampler over all replicas (``replica_step!``,
-src/replicas.jl:676-705, with ``replica_inner = 1``): the pieces that the HIP library provides,
-each over every (replica, hour) of the synthetic 40-node / 70-line system. Inputs are synthetic
-(seeded) and resident in HBM before the timed region. Prints one JSON line.
+A step is one replica step (``replica_step!``, src/replicas.jl:676-705, ``replica_inner = 1``)
+over all replicas of one chain: every replica's weather step (pCN bursts staggered over the
+replicas, regime windows, compensated re-dispatch), one operation sweep of exact line moves,
+the energy totals and the capacity score, one Langevin step of the shared capacities and the
+dispatch following on every replica. The workload is the 40-node / 70-line synthetic system at
+an hourly horizon (8760 steps) with 64 weather replicas; the state is FP32 with FP64 totals,
+MH-free moves and FP64 Langevin. Inputs are seeded and resident in HBM before the timed region.
+Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
+import math
 import os
 import pathlib
 import sys
@@ -20,81 +26,128 @@ import time
 ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "esm-sampling_amd"))
 
-# MI355X published peaks
-HBM_PEAK_GBS = 8000.0
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--hours", type=int, default=8760)
-    ap.add_argument("--fp64", action="store_true", help="FP64 device state (default FP32 terms, FP64 totals)")
+    ap.add_argument("--nodes", type=int, default=40)
+    ap.add_argument("--lines", type=int, default=70)
+    ap.add_argument("--fp64", action="store_true", help="FP64 device state")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
     return ap.parse_args()
 
 
-def energy_bytes_per_unit(N: int, L: int, es: int) -> int:
-    """Algorithmic HBM bytes of the energy kernel per (hour, replica): reads θ, gas, 2 SoC, 3
-    capacity factors per node (7N), writes r per node and F per line, plus 6 FP64 step parts."""
-    return es * (7 * N + N + L) + 6 * 8
+class Workload:
+    """The device-resident replica set and the controls of the replica step."""
+
+    def __init__(self, N, L, T, R, prec, seed=1):
+        import numpy as np
+        from esmsampling import api, synth
+        self.np, self.api = np, api
+        self.s = synth.make_system(N=N, L=L, T=T, dt=8760.0 / T, seed=seed)
+        self.R, self.seed = R, seed
+        self.dev = api.Replicas(self.s, R, prec)
+        api.sample_weather(self.dev, key=(seed, 7), counter=0)
+        _, _, x = self.dev.download()
+        v = synth.heuristic_state(self.s, x)
+        self.dev.upload(v=v, k=self.s.k0)
+        self.w = api.uniform_weights(np.ones(R, dtype=bool))
+        self.eta = np.full(6, 5e-4)
+        self.ks = np.array([1000.0, 500.0, 200.0, 300.0, 100.0, 100.0])   # κ_s per carrier (src/replicas.jl:522)
+        self.rng = np.random.default_rng(seed + 100)
+        self.burst_every, self.phi, self.window, self.window_prob = 40, 0.3, 60, 0.5
+
+    def controls(self, it):
+        np, api = self.np, self.api
+        th = np.zeros(self.R)
+        wp = np.zeros(self.R)
+        for j in range(self.R):
+            th[j], burst = api.weather_schedule(it, j, self.R, 0.0, self.burst_every, self.phi)
+            wp[j] = self.window_prob if burst else 0.0
+        return th, wp
+
+    def step(self, it):
+        api, dev = self.api, self.dev
+        th, wp = self.controls(it)
+        api.weather_process_step(dev, th, wp, window=self.window, compensate=True, key=(self.seed, 11), counter=it)
+        api.operation_sweep(dev, 1, key=(self.seed, 13), counter=it)
+        parts = api.energy_cost(dev)
+        sk, fk = api.capacity_scores(dev)
+        xi = self.rng.standard_normal((6, self.s.N))
+        knew, _ = api.langevin_capacities(dev, self.w, xi, self.eta, self.ks, kmin=1.0, clip=0.02, box=True)
+        api.follow_capacities(dev, knew, kmin=1.0)
+        return parts
+
+
+def kernel_bytes(s, R, es, ntheta_launch):
+    """Algorithmic HBM bytes per launch of each kernel (reads + writes of state and caches that
+    the kernel's algorithm needs, each value once)."""
+    N, L, T = s.N, s.L, s.T
+    half = T // 2
+    return {
+        "k_energy": (es * (7 * N + N + L) + 6 * 8) * T * R,        # θ, gas, 2 SoC, 3 CF in; r, F, 6 step parts out
+        "k_gen": 4 * es * N * T * R,                                 # y, r in and out
+        "k_theta": (8 * L + 4 * N) * es * T * R / max(ntheta_launch, 1),   # per site 4·deg + 4 values
+        "k_soc": 8 * es * N * half * R,                              # e(τm, τ, τ+1), r(τm, τ) in; e(τ), r(τm, τ) out
+        "k_spair": 12 * es * N * half * R,                           # + gas at τm and τ in and out
+        "k_follow": 9 * es * N * T * R,                              # y, r, 3 CF, 2 SoC in; y, r out
+    }
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: every box of this build has a single MI355X "
-              f"(asked for --gpus {args.gpus}); the multi-GPU design is described, not run", file=sys.stderr)
+        print(f"bench.py measures one GPU: every box of this build has a single MI355X (asked for --gpus "
+              f"{args.gpus}); the multi-GPU design (one RCCL allreduce of the weighted score per step) is "
+              f"described in DESIGN.md, not run", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
-    from esmsampling import FP32, FP64, _lib, api, synth
+    from esmsampling import FP32, FP64, _lib
 
-    T, R = args.hours, args.replicas
     prec = FP64 if args.fp64 else FP32
-    s = synth.make_system(N=40, L=70, T=T, dt=8760.0 / T, seed=1)
-    m, z = synth.sample_weather_host(s, R, seed=2)
-    x = synth.weather_from_z(s, z)
-    v = synth.heuristic_state(s, x)
-    k = s.k0.copy()
-    dev = api.Replicas(s, R, prec)
-    dev.upload(v=v, k=k, x=x)
-    w = api.uniform_weights(np.ones(R, dtype=bool))
-    rng = np.random.default_rng(5)
-    eta = np.full(6, 5e-4)
-    ks = np.array([1000.0, 500.0, 200.0, 300.0, 100.0, 100.0])
-
-    def step():
-        api.energy_cost(dev)
-        api.capacity_scores(dev)
-        xi = rng.standard_normal((6, s.N))
-        api.langevin_capacities(dev, w, xi, eta, ks, kmin=1.0, clip=0.02, box=True)
-
-    for _ in range(args.warmup):
-        step()
+    es = 8 if prec == FP64 else 4
+    T, R = args.hours, args.replicas
+    wl = Workload(args.nodes, args.lines, T, R, prec)
+    lib, h = wl.dev.lib, wl.dev.handle
+    for it in range(args.warmup):
+        wl.step(it)
     torch.cuda.synchronize()
-    dev.lib.esm_profile(dev.handle, 1)
+    lib.esm_profile(h, 1)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    for it in range(args.warmup, args.warmup + args.steps):
+        parts = wl.step(it)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    import ctypes as C
-    tot = C.c_double()
-    cnt = C.c_int64()
-    dev.lib.esm_profile_read(dev.handle, _lib.KERN_ENERGY, C.byref(tot), C.byref(cnt))
-    dev.lib.esm_profile(dev.handle, 0)
+    names = {_lib.KERN_ENERGY: "k_energy", _lib.KERN_SCORE: "k_score", _lib.KERN_WEATHER: "weather",
+             _lib.KERN_GEN: "k_gen", _lib.KERN_THETA: "k_theta", _lib.KERN_SOC: "k_soc", _lib.KERN_SPAIR: "k_spair",
+             _lib.KERN_FOLLOW: "k_follow", _lib.KERN_SWEEP: "sweep"}
+    prof = {}
+    for kid, name in names.items():
+        tot, cnt = C.c_double(), C.c_int64()
+        lib.esm_profile_read(h, kid, C.byref(tot), C.byref(cnt))
+        prof[name] = (tot.value, cnt.value)
+    lib.esm_profile(h, 0)
     ms_step = 1e3 * (t1 - t0) / args.steps
     value = args.steps / (t1 - t0)
-    es = 8 if prec == FP64 else 4
-    unit_bytes = energy_bytes_per_unit(s.N, s.L, es)
-    kern_ms = tot.value / max(cnt.value, 1)
-    achieved = unit_bytes * T * R / (kern_ms * 1e-3) / 1e9
+    ntheta = prof["k_theta"][1] / max(args.steps, 1)
+    kb = kernel_bytes(wl.s, R, es, ntheta)
+    cand = {k: prof[k][0] for k in kb}
+    dom = max(cand, key=cand.get)
+    tot_ms, cnt = prof[dom]
+    kern_ms = tot_ms / max(cnt, 1)
+    achieved = kb[dom] / (kern_ms * 1e-3) / 1e9
+    breakdown = {k: round(v[0] / args.steps, 4) for k, v in prof.items()}
+    line_moves = (wl.s.N * 6) * T * R          # gas, potential, 2 SoC, 2 pairs per node and step
     out = {
-        "metric": "replica steps/s (sweeps/s of the replica sampler)",
+        "metric": "sweeps/s (replica steps/s of the plain GPU replica sampler)",
         "value": value,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -104,40 +157,51 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == FP64 else "f32 terms, f64 totals",
-        "data": "synthetic (seeded 40-node/70-line system, VAR weather prior draws)",
-        "config": {"workload": f"replica sampler, 40 nodes / 70 lines, {T} steps, {R} replicas; "
-                               "step = energy + flows + residual caches, capacity score, Langevin",
-                   "replicas": R, "hours": T, "nodes": s.N, "lines": s.L},
-        "roofline": {"kernel": "k_energy", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel_ms": kern_ms, "bytes_per_unit": unit_bytes, "units_per_launch": T * R},
+        "dtype": "f64" if prec == FP64 else "f32",
+        "data": "synthetic (seeded 40-node/70-line system, weather drawn from the VAR prior on the device)",
+        "config": {"workload": f"replica sampler step, {wl.s.N} nodes / {wl.s.L} lines, {T} hourly steps, "
+                               f"{R} weather replicas (weather + operation sweep + energy + score + Langevin + "
+                               f"following)", "replicas": R, "hours": T, "nodes": wl.s.N, "lines": wl.s.L,
+                   "line_moves_per_step": line_moves, "line_moves_per_s": line_moves * value},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kern_ms,
+                     "bytes_per_launch": kb[dom], "launches": cnt},
+        "kernel_ms_per_step": breakdown,
+        "total_cost_mean": float(np.mean(parts[:, 0] + parts[:, 1])),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(s, v, k, x, R, T, args.cpu_seconds)
-    dev.close()
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_threads)
+    wl.dev.close()
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(s, v, k, x, R, T, budget_s):
-    """The FP64 C oracle on the host cores: energy + scores on a bounded sample of replicas,
-    scaled to whole steps over all replicas."""
+def cpu_baseline(wl, threads):
+    """The FP64 C oracle on the host cores: one full replica step (weather step, operation sweep,
+    energy, score, Langevin, following) on a bounded sample of replicas, scaled to all replicas."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
     import oracle
+    nth = threads or os.cpu_count() or 1
+    s, R = wl.s, wl.R
+    rs = max(1, min(R, nth))
     o = oracle.Oracle(s)
-    nr = 1
-    vs, xs = np.ascontiguousarray(v[..., :nr]), np.ascontiguousarray(x[..., :nr])
+    m, z, x = o.weather_init(rs, key=(1, 7), counter=0)
+    from esmsampling import synth
+    v = synth.heuristic_state(s, x)
+    k = s.k0.copy()
+    th, wp = wl.controls(0)
     t0 = time.perf_counter()
-    n = 0
-    while time.perf_counter() - t0 < budget_s:
-        o.energy(vs, k, xs, want_caches=False)
-        o.scores(vs, k, xs)
-        n += 1
-    dt = (time.perf_counter() - t0) / n
-    per_step = dt * R / nr
-    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} x (energy + scores) of {nr} replica over {T} steps, scaled x{R}",
+    m, z, x, v, _ = o.weather_step(m, z, x, v, k, th[:rs], wp[:rs], window=60, compensate=True, key=(1, 11), counter=0)
+    v, r, F, _ = o.sweeps(v, k, 1, key=(1, 13), counter=0, x=x, threads=nth)
+    o.energy(v, k, x, want_caches=False)
+    sk, fk, _ = o.scores(v, k, x)
+    knew, _ = o.langevin(k, sk, fk, np.full(rs, 1.0 / rs), np.zeros((6, s.N)), wl.eta, wl.ks)
+    o.follow(v, k, x, knew)
+    dt = time.perf_counter() - t0
+    per_step = dt * R / rs
+    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": nth, "kind": "port",
+            "sample": f"one full replica step of {rs} replicas x {s.T} steps (oracle, OpenMP over replicas in the "
+                      f"sweep), {dt:.1f} s, scaled x{R / rs:g} to {R} replicas",
             "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
 
 

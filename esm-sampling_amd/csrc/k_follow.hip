@@ -96,6 +96,7 @@ int launch_follow(esm_handle *h, const void *d_kc0, const void *d_kc1, const int
     const DevModel<S> m = dev_model<S>(h);
     FollowArgs<S> fa{(const S *)d_kc0, (const S *)d_kc1, d_moves, (S)kmin};
     const size_t n = (size_t)h->N * h->T * h->R;
+    ProfScope ps(h, ESM_KERN_FOLLOW);
     hipLaunchKernelGGL((k_follow<S>), dim3(nblk(n, 256)), dim3(256), 0, h->stream, m, fa, (S *)h->d_v,
                        (const S *)h->d_x, (S *)h->d_r);
     ESM_HIP_CHECK(hipGetLastError());

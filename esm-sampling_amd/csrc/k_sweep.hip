@@ -291,6 +291,7 @@ __device__ int walk(const F &f, S beta, S maxnats, const S *bp, int nb, S sgn, c
             else delta = delta == 0 ? fmax(fabs(tp), S(1e-6)) : 2 * delta;
             t = tp + sgn * delta;
             if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;   // the ray left the floating-point range: improper
         }
         const S c = beta * f.eval(t, sec);
         S roots[NS], groots[NS];
@@ -405,7 +406,12 @@ __device__ S sample_line(const F &f, const S *bps, int nbp, S beta, S maxnats, S
     }
     const S L = pt(j + 1) - pt(j);
     const S g = (cv(j + 1) - cv(j)) / L;
-    return (g >= 0 || fabs(g * L) < S(1e-10)) ? pt(j) + trexp_offset(g, L, uoff) : pt(j + 1) - trexp_offset(-g, L, uoff);
+    const S t = (g >= 0 || fabs(g * L) < S(1e-10)) ? pt(j) + trexp_offset(g, L, uoff) : pt(j + 1) - trexp_offset(-g, L, uoff);
+    if (!isfinite(t)) {   // a non-finite draw (round-off at the range limits): the move is not made
+        status = 2;
+        return 0;
+    }
+    return t;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -433,6 +439,14 @@ __device__ __forceinline__ void uniforms2(uint32_t k0, uint32_t k1, int j, uint3
     philox4x32(k0, k1, (uint32_t)j, site, round, ctr, w);
     u1 = u01(w[0], w[1]);
     u2 = u01(w[2], w[3]);
+}
+
+// A uniform of [0, 1) in the state precision: the 53-bit value rounded to S stays below 1 (in
+// FP32 a value within 2^-25 of 1 would round to 1, and log1p(−u(1 − e^{−x})) would be −∞).
+template <typename S>
+__device__ __forceinline__ S unit_open(double u) {
+    const S v = (S)u;
+    return v < S(1) ? v : (S)0x1.fffffep-1;   // the largest float below 1 (a double u is always < 1)
 }
 
 // wave-aggregated statistics
@@ -474,7 +488,7 @@ __global__ void __launch_bounds__(256) k_gen(DevModel<S> m, SweepState<S> st, Sw
         double u1, u2;
         uniforms2(a.k0, a.k1, j, (uint32_t)(n * m.T + tau), a.round, a.ctr, u1, u2);
         att = true;
-        const S t = sample_line(f, bps, 2, beta, maxnats, (S)u1, (S)u2, status);
+        const S t = sample_line(f, bps, 2, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
         if (t != 0) {
             acc = true;
             S sec[1], dr;
@@ -530,7 +544,7 @@ __global__ void __launch_bounds__(256) k_theta(DevModel<S> m, SweepState<S> st, 
         double u1, u2;
         uniforms2(a.k0, a.k1, j, (uint32_t)(n * m.T + tau), a.round, a.ctr, u1, u2);
         att = true;
-        const S t = sample_line(f, bps, 2 * f.nl, beta, maxnats, (S)u1, (S)u2, status);
+        const S t = sample_line(f, bps, 2 * f.nl, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
         if (t != 0) {
             acc = true;
             S sec[D + 1], drn, dro[D];
@@ -617,7 +631,7 @@ __global__ void __launch_bounds__(256) k_soc(DevModel<S> m, SweepState<S> st, Sw
             double u1, u2;
             uniforms2(a.k0, a.k1, j, (uint32_t)(n * m.T + tau), a.round, a.ctr, u1, u2);
             att = true;
-            const S t = sample_line(f, bps, nb, beta, maxnats, (S)u1, (S)u2, status);
+            const S t = sample_line(f, bps, nb, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
             if (t != 0) {
                 acc = true;
                 S sec[8], drt, drm;
@@ -656,7 +670,10 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
         a.tstride = 1;
         a.ntau = T;
         a.stats = h->d_stats + 0;
-        hipLaunchKernelGGL((k_gen<S>), dim3(nblk((size_t)N * T * R, B)), dim3(B), 0, h->stream, m, st, a, beta, mxn);
+        {
+            ProfScope pg(h, ESM_KERN_GEN);
+            hipLaunchKernelGGL((k_gen<S>), dim3(nblk((size_t)N * T * R, B)), dim3(B), 0, h->stream, m, st, a, beta, mxn);
+        }
         // THETA: colour by colour, degree buckets
         for (size_t rnd = 0; rnd < h->theta_rounds.size(); rnd++) {
             const auto &tr = h->theta_rounds[rnd];
@@ -665,6 +682,7 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
             a.nnodes = tr.count;
             a.stats = h->d_stats + 4;
             const unsigned nb = nblk((size_t)tr.count * T * R, B);
+            ProfScope pt(h, ESM_KERN_THETA);
             switch (tr.dmax) {
                 case 4: hipLaunchKernelGGL((k_theta<S, 4>), dim3(nb), dim3(B), 0, h->stream, m, st, a, beta, mxn); break;
                 case 8: hipLaunchKernelGGL((k_theta<S, 8>), dim3(nb), dim3(B), 0, h->stream, m, st, a, beta, mxn); break;
@@ -684,6 +702,7 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
                     a.round = (fam == 0 ? 2000u : 3000u) + (uint32_t)(s * 4 + (int)c);
                     a.stats = h->d_stats + (fam == 0 ? 8 : 12);
                     const unsigned nb = nblk((size_t)N * a.ntau * R, B);
+                    ProfScope pf(h, fam == 0 ? ESM_KERN_SOC : ESM_KERN_SPAIR);
                     if (fam == 0)
                         hipLaunchKernelGGL((k_soc<S, false>), dim3(nb), dim3(B), 0, h->stream, m, st, a, beta, mxn, kmin);
                     else

@@ -95,7 +95,7 @@ SIGNATURES = [
     ("esm_profile", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_profile_read", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
 ]
-KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER = 0, 1, 2, 3
+KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW = range(9)
 
 _lib = None
 

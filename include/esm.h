@@ -241,7 +241,12 @@ int32_t esm_upload_weather(esm_handle *h, const int32_t *m, const double *z);
 #define ESM_KERN_SCORE 1   /* capacity score partial sums                */
 #define ESM_KERN_SWEEP 2   /* exact line moves (all rounds of a sweep)   */
 #define ESM_KERN_WEATHER 3 /* weather process                            */
-#define ESM_NKERN 4
+#define ESM_KERN_GEN 4     /* one round of gas atoms                     */
+#define ESM_KERN_THETA 5   /* one round of potential atoms (one bucket)  */
+#define ESM_KERN_SOC 6     /* one round of SoC atoms                     */
+#define ESM_KERN_SPAIR 7   /* one round of storage-gas pairs             */
+#define ESM_KERN_FOLLOW 8  /* dispatch following                         */
+#define ESM_NKERN 9
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
 

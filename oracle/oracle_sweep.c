@@ -397,6 +397,7 @@ static int walk(Block *b, double beta, const double *bp, int nb, double sgn, dou
             else delta = delta == 0.0 ? fmax(fabs(tp), 1e-6) : 2 * delta;
             t = tp + sgn * delta;
             if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;   /* the ray left the floating-point range: improper */
         }
         double c = beta * mv_eval(b, t);
         memcpy(sec, b->sec, sizeof(double) * ns);
@@ -523,7 +524,12 @@ static double sample_line(Block *b, double beta, double maxnats, double useg, do
     }
     double L = pts[jj + 1] - pts[jj];
     double g = (cs[jj + 1] - cs[jj]) / L;
-    return (g >= 0 || fabs(g * L) < 1e-10) ? pts[jj] + trexp_offset(g, L, uoff) : pts[jj + 1] - trexp_offset(-g, L, uoff);
+    double t = (g >= 0 || fabs(g * L) < 1e-10) ? pts[jj] + trexp_offset(g, L, uoff) : pts[jj + 1] - trexp_offset(-g, L, uoff);
+    if (!isfinite(t)) {   /* a non-finite draw: the move is not made */
+        *status = 2;
+        return 0.0;
+    }
+    return t;
 }
 
 /* commit_block! / apply_step! */
