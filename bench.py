@@ -77,7 +77,7 @@ class Workload:
         api, dev = self.api, self.dev
         th, wp = self.controls(it)
         api.weather_process_step(dev, th, wp, window=self.window, compensate=True, key=(self.seed, 11), counter=it)
-        api.operation_sweep(dev, 1, key=(self.seed, 13), counter=it)
+        self.last_stats = api.operation_sweep(dev, 1, key=(self.seed, 13), counter=it)
         parts = api.energy_cost(dev)
         sk, fk = api.capacity_scores(dev)
         xi = self.rng.standard_normal((6, self.s.N))
@@ -167,6 +167,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kern_ms,
                      "bytes_per_launch": kb[dom], "launches": cnt},
         "kernel_ms_per_step": breakdown,
+        "sweep_stats": {name: {"moves": int(st[0]), "accepted": int(st[1]), "improper": int(st[2]),
+                               "cut": int(st[3]), "evals_per_move": round(float(st[4]) / max(int(st[0]), 1), 3)}
+                        for name, st in zip(("gas", "potential", "soc", "spair"), wl.last_stats)},
         "total_cost_mean": float(np.mean(parts[:, 0] + parts[:, 1])),
     }
     if not args.no_cpu:

@@ -357,7 +357,7 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     zmalloc((void **)&h->d_out, (size_t)std::max(nrep * 8, ESM_NK * N * 2 + 64) * sizeof(double));
     zmalloc((void **)&h->d_sk, (size_t)nrep * ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
-    zmalloc((void **)&h->d_stats, 16 * sizeof(unsigned long long));
+    zmalloc((void **)&h->d_stats, 20 * STAT_SLOTS * sizeof(unsigned long long));
     // weather process (optional: a model without weather tables has none)
     h->has_weather = mdl->Ptr && mdl->pi0 && mdl->mu && mdl->xbar && mdl->K >= 1 && mdl->K <= ESM_MAXK &&
                      mdl->sigma > 0 && mdl->sigma0 > 0;
@@ -561,15 +561,20 @@ int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32
                    int64_t *stats) {
     if (!h || !key) { set_error("esm_sweeps: null argument"); return ESM_EINVAL; }
     if (nsweeps < 0 || !(maxnats > 0)) { set_error("esm_sweeps: nsweeps must be >= 0 and maxnats > 0"); return ESM_EINVAL; }
-    ESM_HIP_CHECK(hipMemsetAsync(h->d_stats, 0, 16 * sizeof(unsigned long long), h->stream));
+    ESM_HIP_CHECK(hipMemsetAsync(h->d_stats, 0, 20 * STAT_SLOTS * sizeof(unsigned long long), h->stream));
     int rc = h->prec == ESM_FP64 ? launch_sweeps<double>(h, nsweeps, key[0], key[1], counter, maxnats)
                                  : launch_sweeps<float>(h, nsweeps, key[0], key[1], counter, maxnats);
     if (rc) return rc;
     if (stats) {
-        unsigned long long st[16];
-        ESM_HIP_CHECK(hipMemcpyAsync(st, h->d_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+        std::vector<unsigned long long> st(20 * STAT_SLOTS);
+        ESM_HIP_CHECK(hipMemcpyAsync(st.data(), h->d_stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                     h->stream));
         ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
-        for (int i = 0; i < 16; i++) stats[i] = (int64_t)st[i];
+        for (int i = 0; i < 20; i++) {
+            unsigned long long acc = 0;
+            for (int q = 0; q < STAT_SLOTS; q++) acc += st[(size_t)i * STAT_SLOTS + q];
+            stats[i] = (int64_t)acc;
+        }
     }
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     h->scores_valid = false;

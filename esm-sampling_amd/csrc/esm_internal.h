@@ -14,6 +14,7 @@ namespace esm {
 
 constexpr int ESM_MAXK = 4;       // largest regime count of the weather model
 constexpr int ESM_MAXWIN = 256;   // longest regime window
+constexpr int STAT_SLOTS = 256;   // sweep counters are spread over this many addresses per statistic
 
 void set_error(const std::string &msg);
 
@@ -92,7 +93,7 @@ struct esm_handle {
     std::vector<int> theta_nodes_h;
     int *d_theta_nodes = nullptr;
     std::vector<std::pair<int, int>> tclass;      // time-parity classes: (first step, count), stride 2
-    unsigned long long *d_stats = nullptr;        // [4 families][4]
+    unsigned long long *d_stats = nullptr;        // [4 families][5 statistics][STAT_SLOTS]
     // kernel timing (esm_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[ESM_NKERN];

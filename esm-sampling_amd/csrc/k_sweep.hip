@@ -28,7 +28,7 @@
 
 namespace esm {
 
-constexpr int PMAX = 48;      // walk points kept per side (guard: t = 0 beyond)
+constexpr int PMAX = 48;      // walk points per side (guard: t = 0 beyond)
 
 template <typename S>
 __device__ __forceinline__ S phibal(S x, S cs, S cd) { return x > 0 ? cs * x : -cd * x; }
@@ -87,7 +87,9 @@ struct FGen {
     __device__ static constexpr bool gmask(int) { return false; }
     Ctx<S> k;
     S y0, kap, w, mc, a, c0, p0, S0, phi0;
+    mutable int nev = 0;   // ΔC evaluations (statistics)
     __device__ S eval(S t, S *sec, S &dr) const {
+        nev++;
         S c1, p1;
         gen_cost(mc, kap, w, k.dt, k.cv, y0 + t * a, c1, p1);
         S d = c1 - c0;
@@ -114,7 +116,9 @@ struct FTheta {
     int nl, npos;          // lines; position of node n among the balances (0 or 1)
     S f0[D], df[D], se[D], c0[D], sgn_n[D];   // sgn_n: +1 if n is the line's bus0
     S S0n, phin, S0o[D], phio[D];
+    mutable int nev = 0;   // ΔC evaluations (statistics)
     __device__ S eval(S t, S *sec, S &drn, S *dro) const {
+        nev++;
         S d = 0;
         drn = 0;
 #pragma unroll
@@ -135,15 +139,11 @@ struct FTheta {
 #pragma unroll
         for (int p = 0; p <= D; p++) {
             if (p <= nl) {
-                S S1, ph0;
-                if (p == npos) {
-                    S1 = S1n;
-                    ph0 = phin;
-                } else {
-                    const int l = p < npos ? p : p - 1;
-                    S1 = S1o[l];
-                    ph0 = phio[l];
-                }
+                // neighbour of line p (before n) or p − 1 (after n): static register indices
+                const S s_lo = p < D ? S1o[p < D ? p : 0] : S(0), f_lo = p < D ? phio[p < D ? p : 0] : S(0);
+                const S s_hi = p > 0 ? S1o[p > 0 ? p - 1 : 0] : S(0), f_hi = p > 0 ? phio[p > 0 ? p - 1 : 0] : S(0);
+                const S S1 = p == npos ? S1n : (p < npos ? s_lo : s_hi);
+                const S ph0 = p == npos ? phin : (p < npos ? f_lo : f_hi);
                 d += k.dt * (phibal(S1, k.cs, k.cd) - ph0);
                 sec[p] = S1;
             } else {
@@ -158,19 +158,47 @@ struct FTheta {
     }
 };
 
+// per-unit storage parameters (the SoC form of src/energy.jl:437-463)
+template <typename S>
+struct StoPar {
+    S h, ein, eout, qt, mcs;
+};
+
+// soc_cost with the unit's parameters in registers: returns the cost, writes d and the injection
+template <typename S>
+__device__ __forceinline__ S soc_cost_p(const StoPar<S> &p, S dt, S cv, S k, S w, S ye, S ye1, S &d, S &inj) {
+    const S ec = clampv(ye, S(0), S(1)), ec1 = clampv(ye1, S(0), S(1));
+    d = p.h * (ec1 - ec) / dt - p.qt;
+    S pc, mcp;
+    if (d >= 0) {
+        pc = p.ein > 0 ? d / p.ein : S(0);
+        mcp = 0;
+    } else {
+        pc = 0;
+        mcp = fmin(-d * p.eout, S(1));
+    }
+    S c = dt * p.mcs * k * (pc + mcp) + cv * w * p.h * fabs(ye - ec);
+    if (pc > 1) c += cv * dt * w * (pc - 1);
+    if (d < 0 && p.qt <= 0) {
+        const S ex = -d * p.eout - S(1);
+        if (ex > 0) c += cv * dt * w * ex;
+    }
+    if (d >= 0 && p.ein <= 0) c += cv * dt * w * d;
+    inj = k * (pc - mcp);
+    return c;
+}
+
 // storage term of a unit at one step, with both ends moving linearly
 template <typename S>
 struct StoTerm {
     S ye, ye1, dye, dye1, kk, w, c0, j0;
-    __device__ S eval(const DevModel<S> &m, int s, int n, S t, S *sec3, S &dj) const {
-        S d1, j1, run;
-        const S c1 = soc_cost(m, s, n, kk, w, ye + t * dye, ye1 + t * dye1, d1, j1, run);
+    __device__ S eval(const StoPar<S> &p, S dt, S cv, S t, S *sec3, S &dj) const {
+        S d1, j1;
+        const S c1 = soc_cost_p(p, dt, cv, kk, w, ye + t * dye, ye1 + t * dye1, d1, j1);
         dj = j1 - j0;
-        const int u = s * m.N + n;
-        const S ein = m.eta_in[u];
         sec3[0] = d1;
-        sec3[1] = ein > 0 ? d1 - ein : S(1);
-        sec3[2] = -d1 * m.eta_out[u] - S(1);
+        sec3[1] = p.ein > 0 ? d1 - p.ein : S(1);
+        sec3[2] = -d1 * p.eout - S(1);
         return c1 - c0;
     }
 };
@@ -184,13 +212,14 @@ struct FSoc {
     static constexpr bool STOR = true;
     __device__ static constexpr bool gmask(int j) { return j != 3 && j != 7; }
     Ctx<S> k;
-    const DevModel<S> *m;
-    int s, n;
+    StoPar<S> p;
     StoTerm<S> st_t, st_m;     // the storage term at τ and at τm
     S S0t, phit, S0m, phim;
     // gas terms (GEN): at τ (b) and τm (a)
     S ya, yb, kap, w, mc, aa, ab, c0a, p0a, c0b, p0b;
+    mutable int nev = 0;   // ΔC evaluations (statistics)
     __device__ S eval(S t, S *sec, S &drt, S &drm) const {
+        nev++;
         // step τ first (the reference evaluates the later step first)
         S d = 0;
         drt = 0;
@@ -201,7 +230,7 @@ struct FSoc {
             drt -= p1 - p0b;
         }
         S dj;
-        d += st_t.eval(*m, s, n, t, sec, dj);
+        d += st_t.eval(p, k.dt, k.cv, t, sec, dj);
         drt += dj;
         {
             const S S1 = S0t + drt;
@@ -216,7 +245,7 @@ struct FSoc {
             e += c1 - c0a;
             drm -= p1 - p0a;
         }
-        e += st_m.eval(*m, s, n, t, sec + 4, dj);
+        e += st_m.eval(p, k.dt, k.cv, t, sec + 4, dj);
         drm += dj;
         {
             const S S1 = S0m + drm;
@@ -232,61 +261,100 @@ struct FSoc {
 };
 
 // ---------------------------------------------------------------------------------------------
-// the exact line sampler
+// the exact line sampler, register-resident
+//
+// walk! (src/local.jl:1294-1443) visits the points of one side in walking order: the next
+// primary breakpoint beyond the current point is found by a scan of the breakpoint registers
+// (the same sequence as the sorted list with duplicates skipped), the secondary sign changes of
+// a segment by a scan of the secondaries, storage roots first and the other roots on the pieces
+// between them (piece_roots!, :1267-1291), ray steps beyond the last breakpoint. The points are
+// not stored: a visitor consumes them. sample_line! (:1501-1560) walks both sides accumulating
+// the segment log-masses (logseg) online, draws the side and the segment with the first uniform,
+// re-walks that side to the segment, and draws the offset (trexp_offset) with the second.
 // ---------------------------------------------------------------------------------------------
 
-// Sign changes of the non-storage secondaries on a piece (a, b) where they are linear, appended
-// in walking order (piece_roots!, src/local.jl:1267-1291). Returns false when the walk buffer
-// is full.
-template <typename S, class F>
-__device__ bool piece_roots(const F &f, S beta, S &cmin, const S *A, const S *B, S a, S b, S sgn, S *pts, S *cs,
-                            int &np) {
-    S sr[F::NSEC];
-    int ns = 0;
+// next root in walking order among the flagged entries; consumes it; false if none
+template <typename S, int NS>
+__device__ __forceinline__ bool next_root(const S *rt, bool *hr, S sgn, S &x) {
+    int jb = -1;
+    S xb = 0;
 #pragma unroll
-    for (int j = 0; j < F::NSEC; j++) {
+    for (int j = 0; j < NS; j++)
+        if (hr[j] && (jb < 0 || sgn * (xb - rt[j]) > 0)) {
+            jb = j;
+            xb = rt[j];
+        }
+    if (jb < 0) return false;
+#pragma unroll
+    for (int j = 0; j < NS; j++)
+        if (j == jb) hr[j] = false;
+    x = xb;
+    return true;
+}
+
+// visit one point: 0 go on, 2 the visitor stops the walk, -1 the side has more than PMAX points
+template <typename S, class V>
+__device__ __forceinline__ int visit_point(V &vis, S x, S cx, S &cmin, int &np) {
+    if (++np > PMAX) return -1;
+    if (vis(x, cx)) return 2;
+    cmin = fmin(cmin, cx);
+    return 0;
+}
+
+// sign changes of the non-storage secondaries on a piece (a, b) where they are linear
+template <typename S, class F, class V>
+__device__ int piece_roots(const F &f, S beta, S &cmin, const S *A, const S *B, S a, S b, S sgn, int &np, V &vis) {
+    constexpr int NS = F::NSEC;
+    S rt[NS];
+    bool hr[NS];
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        hr[j] = false;
+        rt[j] = 0;
         if (F::gmask(j)) continue;
         const S u = A[j], v = B[j];
         if ((u < 0) != (v < 0) && u != v) {
             const S x = a + (b - a) * u / (u - v);
-            if (sgn * (x - a) > 0 && sgn * (b - x) > 0) sr[ns++] = x;
+            if (sgn * (x - a) > 0 && sgn * (b - x) > 0) {
+                rt[j] = x;
+                hr[j] = true;
+            }
         }
     }
-    if (ns == 0) return true;
-    insort(sr, ns, sgn);
-    S tmp[F::NSEC];
-    for (int q = 0; q < ns; q++) {
-        const S cx = beta * f.eval(sr[q], tmp);
-        if (np >= PMAX) return false;
-        pts[np] = sr[q];
-        cs[np] = cx;
-        np++;
-        cmin = fmin(cmin, cx);
+    S x, tmp[NS];
+    while (next_root<S, NS>(rt, hr, sgn, x)) {
+        const int w = visit_point(vis, x, beta * f.eval(x, tmp), cmin, np);
+        if (w) return w;
     }
-    return true;
+    return 0;
 }
 
-// walk! (src/local.jl:1294-1443) without requirement-tracking kinks. Returns 1 (proper),
-// 0 (improper: no point maxnats above the minimum within 80 ray steps) or −1 (buffer full).
-template <typename S, class F>
-__device__ int walk(const F &f, S beta, S maxnats, const S *bp, int nb, S sgn, const S *sec0, S *pts, S *cs, int &np) {
-    constexpr int NS = F::NSEC;
+// 1 proper (stopped maxnats above the minimum), 0 improper, -1 point cap, 2 stopped by the visitor
+template <typename S, class F, class V>
+__device__ int walk(const F &f, S beta, S maxnats, const S *bp, int nbp, S sgn, const S *sec0, V &vis) {
+    constexpr int NS = F::NSEC, NB = F::NBP;
     S secprev[NS], sec[NS];
 #pragma unroll
     for (int j = 0; j < NS; j++) secprev[j] = sec0[j];
-    S cmin = 0, tp = 0, delta = 0;
-    int idx = 0, nx = 0;
-    np = 0;
+    S cmin = 0, tp = 0, delta = 0, clast = 0;
+    int nx = 0, np = 0;
     while (true) {
-        S t;
-        if (idx < nb) {
-            t = bp[idx++];
-            if (t == tp) continue;
-        } else {
-            const S cprev = np == 0 ? S(0) : cs[np - 1];
+        S t = 0;
+        bool have = false;
+#pragma unroll
+        for (int q = 0; q < NB; q++)
+            if (q < nbp) {
+                const S b = bp[q];
+                if (sgn * (b - tp) > 0 && (!have || sgn * (t - b) > 0)) {
+                    t = b;
+                    have = true;
+                }
+            }
+        if (!have) {
+            // beyond the last breakpoint: the step from the outgoing slope at tp, capped
             const S hp = S(1e-6) * fmax(S(1), fabs(tp));
             S tmp[NS];
-            const S sl = (beta * f.eval(tp + sgn * hp, tmp) - cprev) / hp;
+            const S sl = (beta * f.eval(tp + sgn * hp, tmp) - clast) / hp;
             if (sl > 0) delta = fmin(2 * maxnats / sl, S(1e3) * fmax(fabs(tp), S(1e-3)));
             else delta = delta == 0 ? fmax(fabs(tp), S(1e-6)) : 2 * delta;
             t = tp + sgn * delta;
@@ -294,119 +362,138 @@ __device__ int walk(const F &f, S beta, S maxnats, const S *bp, int nb, S sgn, c
             if (!isfinite(t)) return 0;   // the ray left the floating-point range: improper
         }
         const S c = beta * f.eval(t, sec);
-        S roots[NS], groots[NS];
-        int nr = 0, ng = 0;
+        S rt[NS];
+        bool hr[NS], hg[NS];
+        bool anyg = false;
 #pragma unroll
         for (int j = 0; j < NS; j++) {
+            hr[j] = hg[j] = false;
+            rt[j] = 0;
             const S a = secprev[j], b = sec[j];
             if ((a < 0) != (b < 0) && a != b) {
                 const S x = tp + (t - tp) * a / (a - b);
                 if (sgn * (x - tp) > 0 && sgn * (t - x) > 0) {
-                    if (F::STOR && F::gmask(j)) groots[ng++] = x;
-                    else roots[nr++] = x;
+                    rt[j] = x;
+                    if (F::STOR && F::gmask(j)) {
+                        hg[j] = true;
+                        anyg = true;
+                    } else {
+                        hr[j] = true;
+                    }
                 }
             }
         }
-        if (F::STOR && ng > 0) {
-            S secB[NS];
-            insort(groots, ng, sgn);
-            S a0 = tp;
-            for (int q = 0; q < ng; q++) {
-                const S x = groots[q];
+        int w;
+        if (F::STOR && anyg) {
+            // storage roots split the segment; the other roots are found on the pieces
+            S a0 = tp, x;
+            while (next_root<S, NS>(rt, hg, sgn, x)) {
+                S secB[NS];
                 const S cx = beta * f.eval(x, secB);
-                if (!piece_roots(f, beta, cmin, secprev, secB, a0, x, sgn, pts, cs, np)) return -1;
-                if (np >= PMAX) return -1;
-                pts[np] = x;
-                cs[np] = cx;
-                np++;
-                cmin = fmin(cmin, cx);
+                if ((w = piece_roots(f, beta, cmin, secprev, secB, a0, x, sgn, np, vis))) return w;
+                if ((w = visit_point(vis, x, cx, cmin, np))) return w;
 #pragma unroll
                 for (int j = 0; j < NS; j++) secprev[j] = secB[j];
                 a0 = x;
             }
-            if (!piece_roots(f, beta, cmin, secprev, sec, a0, t, sgn, pts, cs, np)) return -1;
+            if ((w = piece_roots(f, beta, cmin, secprev, sec, a0, t, sgn, np, vis))) return w;
 #pragma unroll
             for (int j = 0; j < NS; j++) secprev[j] = sec[j];
         } else {
 #pragma unroll
             for (int j = 0; j < NS; j++) secprev[j] = sec[j];
-            if (nr > 0) {
-                insort(roots, nr, sgn);
-                S tmp[NS];
-                for (int q = 0; q < nr; q++) {
-                    const S cx = beta * f.eval(roots[q], tmp);
-                    if (np >= PMAX) return -1;
-                    pts[np] = roots[q];
-                    cs[np] = cx;
-                    np++;
-                    cmin = fmin(cmin, cx);
-                }
-            }
+            S x, tmp[NS];
+            while (next_root<S, NS>(rt, hr, sgn, x))
+                if ((w = visit_point(vis, x, beta * f.eval(x, tmp), cmin, np))) return w;
         }
-        if (np >= PMAX) return -1;
-        pts[np] = t;
-        cs[np] = c;
-        np++;
-        cmin = fmin(cmin, c);
+        if ((w = visit_point(vis, t, c, cmin, np))) return w;
+        clast = c;
         if (c - cmin > maxnats) return 1;
         tp = t;
     }
 }
 
-// sample_line! (src/local.jl:1501-1560): status 0 drawn, 1 improper (t = 0), 2 guard (t = 0)
+template <typename S>
+__device__ __forceinline__ S seg_logmass(S ta, S ca, S tb, S cb) {
+    const S L = fabs(tb - ta);
+    return L > 0 ? -fmin(ca, cb) + logseg<S>(L, fabs(cb - ca)) : S(-INFINITY);
+}
+
+// pass 1: online log-sum-exp of the segment masses of one side
+template <typename S>
+struct MassAcc {
+    S tp = 0, cp = 0, mx = -INFINITY, s = 0;
+    __device__ bool operator()(S t, S c) {
+        const S lm = seg_logmass(tp, cp, t, c);
+        if (lm > mx) {
+            s = s * exp(mx - lm) + S(1);
+            mx = lm;
+        } else {
+            s += exp(lm - mx);
+        }
+        tp = t;
+        cp = c;
+        return false;
+    }
+};
+
+// pass 2: the segment where the cumulative mass from t = 0 outward crosses the target
+template <typename S>
+struct SegSel {
+    S tp = 0, cp = 0, c0 = 0, target = 0, MX = 0;
+    bool left = false;
+    S ta = 0, ca = 0, tb = 0, cb = 0;
+    __device__ bool operator()(S t, S c) {
+        const S lm = seg_logmass(tp, cp, t, c);
+        const S m = exp(lm - MX);
+        c0 += m;
+        if (left) {
+            ta = t; ca = c; tb = tp; cb = cp;
+        } else {
+            ta = tp; ca = cp; tb = t; cb = c;
+        }
+        tp = t;
+        cp = c;
+        return lm > S(-INFINITY) && (left ? c0 > target : target <= c0);
+    }
+};
+
+// sample_line!: status 0 drawn, 1 improper (t = 0), 2 guard (t = 0)
 template <typename S, class F>
 __device__ S sample_line(const F &f, const S *bps, int nbp, S beta, S maxnats, S useg, S uoff, int &status) {
-    S right[F::NBP], left[F::NBP];
-    int nr = 0, nl = 0;
-    for (int q = 0; q < nbp; q++) {
-        const S x = bps[q];
-        if (!isfinite(x)) continue;
-        if (x > 0) right[nr++] = x;
-        else if (x < 0) left[nl++] = x;
+    constexpr int NB = F::NBP;
+    S bp[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const S x = q < nbp ? bps[q] : S(0);
+        bp[q] = isfinite(x) ? x : S(0);   // 0 is never beyond the current point of a walk
     }
-    insort(right, nr, S(1));
-    insort(left, nl, S(-1));
     S sec0[F::NSEC];
     f.eval(S(0), sec0);
-    S pr[PMAX], cr[PMAX], pl[PMAX], cl[PMAX];
-    int npr, npl;
-    int w = walk(f, beta, maxnats, right, nr, S(1), sec0, pr, cr, npr);
+    MassAcc<S> mr, ml;
+    int w = walk(f, beta, maxnats, bp, NB, S(1), sec0, mr);
     if (w != 1) {
         status = w == 0 ? 1 : 2;
         return 0;
     }
-    w = walk(f, beta, maxnats, left, nl, S(-1), sec0, pl, cl, npl);
+    w = walk(f, beta, maxnats, bp, NB, S(-1), sec0, ml);
     if (w != 1) {
         status = w == 0 ? 1 : 2;
         return 0;
     }
+    const S MX = fmax(ml.mx, mr.mx);
+    const S Ml = ml.s * exp(ml.mx - MX), Mr = mr.s * exp(mr.mx - MX);
+    const S u = useg * (Ml + Mr);
+    SegSel<S> sel;
+    sel.MX = MX;
+    sel.left = u <= Ml;
+    sel.target = sel.left ? Ml - u : u - Ml;
+    walk(f, beta, maxnats, bp, NB, sel.left ? S(-1) : S(1), sec0, sel);
     status = 0;
-    // points in increasing order: the left walk reversed, 0, the right walk
-    const int P = npl + 1 + npr;
-    auto pt = [&](int q) -> S { return q < npl ? pl[npl - 1 - q] : (q == npl ? S(0) : pr[q - npl - 1]); };
-    auto cv = [&](int q) -> S { return q < npl ? cl[npl - 1 - q] : (q == npl ? S(0) : cr[q - npl - 1]); };
-    auto lmass = [&](int q) -> S {
-        const S L = pt(q + 1) - pt(q);
-        const S c0 = cv(q), c1 = cv(q + 1);
-        return L > 0 ? -fmin(c0, c1) + logseg<S>(L, fabs(c1 - c0)) : S(-INFINITY);
-    };
-    S mx = -INFINITY;
-    for (int q = 0; q < P - 1; q++) mx = fmax(mx, lmass(q));
-    S tot = 0;
-    for (int q = 0; q < P - 1; q++) tot += exp(lmass(q) - mx);
-    const S u = useg * tot;
-    S acc = 0;
-    int j = P - 2;
-    for (int q = 0; q < P - 1; q++) {
-        acc += exp(lmass(q) - mx);
-        if (u <= acc) {
-            j = q;
-            break;
-        }
-    }
-    const S L = pt(j + 1) - pt(j);
-    const S g = (cv(j + 1) - cv(j)) / L;
-    const S t = (g >= 0 || fabs(g * L) < S(1e-10)) ? pt(j) + trexp_offset(g, L, uoff) : pt(j + 1) - trexp_offset(-g, L, uoff);
+    const S L = sel.tb - sel.ta;
+    const S g = (sel.cb - sel.ca) / L;
+    const S t = (g >= 0 || fabs(g * L) < S(1e-10)) ? sel.ta + trexp_offset(g, L, uoff)
+                                                     : sel.tb - trexp_offset(-g, L, uoff);
     if (!isfinite(t)) {   // a non-finite draw (round-off at the range limits): the move is not made
         status = 2;
         return 0;
@@ -424,7 +511,7 @@ struct SweepArgs {
     int s;                         // storage carrier (SOC, SPAIR)
     const int *nodes;              // THETA: node list of the round
     int nnodes;
-    unsigned long long *stats;     // [4]: attempted, accepted, improper, guard
+    unsigned long long *stats;     // [5][STAT_SLOTS]: attempted, accepted, improper, guard, ΔC evaluations
 };
 
 template <typename S>
@@ -450,14 +537,19 @@ __device__ __forceinline__ S unit_open(double u) {
 }
 
 // wave-aggregated statistics
-__device__ __forceinline__ void add_stats(unsigned long long *st, bool att, bool acc, int status) {
+__device__ __forceinline__ void add_stats(unsigned long long *st, bool att, bool acc, int status, int nev) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) nev += __shfl_xor(nev, off);
     const unsigned long long ma = __ballot(att), mc = __ballot(att && acc), mi = __ballot(att && status == 1),
                              mg = __ballot(att && status == 2);
     if ((threadIdx.x & 63) == 0) {
-        if (ma) atomicAdd(st + 0, (unsigned long long)__popcll(ma));
-        if (mc) atomicAdd(st + 1, (unsigned long long)__popcll(mc));
-        if (mi) atomicAdd(st + 2, (unsigned long long)__popcll(mi));
-        if (mg) atomicAdd(st + 3, (unsigned long long)__popcll(mg));
+        // one slot per wave (mod STAT_SLOTS): same-address atomics from every wave would serialize
+        const unsigned slot = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) & (STAT_SLOTS - 1);
+        if (ma) atomicAdd(st + 0 * STAT_SLOTS + slot, (unsigned long long)__popcll(ma));
+        if (mc) atomicAdd(st + 1 * STAT_SLOTS + slot, (unsigned long long)__popcll(mc));
+        if (mi) atomicAdd(st + 2 * STAT_SLOTS + slot, (unsigned long long)__popcll(mi));
+        if (mg) atomicAdd(st + 3 * STAT_SLOTS + slot, (unsigned long long)__popcll(mg));
+        if (nev) atomicAdd(st + 4 * STAT_SLOTS + slot, (unsigned long long)nev);
     }
 }
 
@@ -466,7 +558,7 @@ __global__ void __launch_bounds__(256) k_gen(DevModel<S> m, SweepState<S> st, Sw
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nsite = (size_t)m.N * a.ntau;
     bool att = false, acc = false;
-    int status = 0;
+    int status = 0, nev = 0;
     if (g < nsite * m.R) {
         const int j = (int)(g % m.R);
         const size_t site = g / m.R;
@@ -489,6 +581,7 @@ __global__ void __launch_bounds__(256) k_gen(DevModel<S> m, SweepState<S> st, Sw
         uniforms2(a.k0, a.k1, j, (uint32_t)(n * m.T + tau), a.round, a.ctr, u1, u2);
         att = true;
         const S t = sample_line(f, bps, 2, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+        nev = f.nev;
         if (t != 0) {
             acc = true;
             S sec[1], dr;
@@ -497,7 +590,7 @@ __global__ void __launch_bounds__(256) k_gen(DevModel<S> m, SweepState<S> st, Sw
             st.r[o] += dr;
         }
     }
-    add_stats(a.stats, att, acc, status);
+    add_stats(a.stats, att, acc, status, nev);
 }
 
 template <typename S, int D>
@@ -505,7 +598,7 @@ __global__ void __launch_bounds__(256) k_theta(DevModel<S> m, SweepState<S> st, 
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nsite = (size_t)a.nnodes * a.ntau;
     bool att = false, acc = false;
-    int status = 0;
+    int status = 0, nev = 0;
     if (g < nsite * m.R) {
         const int j = (int)(g % m.R);
         const size_t site = g / m.R;
@@ -545,6 +638,7 @@ __global__ void __launch_bounds__(256) k_theta(DevModel<S> m, SweepState<S> st, 
         uniforms2(a.k0, a.k1, j, (uint32_t)(n * m.T + tau), a.round, a.ctr, u1, u2);
         att = true;
         const S t = sample_line(f, bps, 2 * f.nl, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+        nev = f.nev;
         if (t != 0) {
             acc = true;
             S sec[D + 1], drn, dro[D];
@@ -560,7 +654,7 @@ __global__ void __launch_bounds__(256) k_theta(DevModel<S> m, SweepState<S> st, 
             }
         }
     }
-    add_stats(a.stats, att, acc, status);
+    add_stats(a.stats, att, acc, status, nev);
 }
 
 template <typename S, bool GEN>
@@ -569,7 +663,7 @@ __global__ void __launch_bounds__(256) k_soc(DevModel<S> m, SweepState<S> st, Sw
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nsite = (size_t)m.N * a.ntau;
     bool att = false, acc = false;
-    int status = 0;
+    int status = 0, nev = 0;
     if (g < nsite * m.R) {
         const int j = (int)(g % m.R);
         const size_t site = g / m.R;
@@ -585,9 +679,10 @@ __global__ void __launch_bounds__(256) k_soc(DevModel<S> m, SweepState<S> st, Sw
         const size_t ot = (size_t)tau * m.R + j, om = (size_t)taum * m.R + j, o1 = (size_t)tau1 * m.R + j;
         FSoc<S, GEN> f;
         f.k = {m.dt, m.cv, m.c_shed, m.c_dump};
-        f.m = &m;
-        f.s = s;
-        f.n = n;
+        {
+            const int us = s * m.N + n;
+            f.p = {m.hours[us], m.eta_in[us], m.eta_out[us], m.qt[us], m.mc[(ESM_KSTOR + s) * m.N + n]};
+        }
         const S kst = m.kc[(ESM_KSTOR + s) * m.N + n];
         bool ok = true;
         if (GEN) {
@@ -606,11 +701,11 @@ __global__ void __launch_bounds__(256) k_soc(DevModel<S> m, SweepState<S> st, Sw
         }
         if (ok) {
             const S e_t = ye[ot], e_1 = ye[o1], e_m = ye[om];
-            S d, run;
+            S d;
             f.st_t = {e_t, e_1, S(1), S(0), kst, kst + m.kappa0, S(0), S(0)};
-            f.st_t.c0 = soc_cost(m, s, n, kst, kst + m.kappa0, e_t, e_1, d, f.st_t.j0, run);
+            f.st_t.c0 = soc_cost_p(f.p, m.dt, m.cv, kst, kst + m.kappa0, e_t, e_1, d, f.st_t.j0);
             f.st_m = {e_m, e_t, S(0), S(1), kst, kst + m.kappa0, S(0), S(0)};
-            f.st_m.c0 = soc_cost(m, s, n, kst, kst + m.kappa0, e_m, e_t, d, f.st_m.j0, run);
+            f.st_m.c0 = soc_cost_p(f.p, m.dt, m.cv, kst, kst + m.kappa0, e_m, e_t, d, f.st_m.j0);
             f.S0t = rr[ot];
             f.phit = phibal(f.S0t, m.c_shed, m.c_dump);
             f.S0m = rr[om];
@@ -632,6 +727,7 @@ __global__ void __launch_bounds__(256) k_soc(DevModel<S> m, SweepState<S> st, Sw
             uniforms2(a.k0, a.k1, j, (uint32_t)(n * m.T + tau), a.round, a.ctr, u1, u2);
             att = true;
             const S t = sample_line(f, bps, nb, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+            nev = f.nev;
             if (t != 0) {
                 acc = true;
                 S sec[8], drt, drm;
@@ -646,7 +742,7 @@ __global__ void __launch_bounds__(256) k_soc(DevModel<S> m, SweepState<S> st, Sw
             }
         }
     }
-    add_stats(a.stats, att, acc, status);
+    add_stats(a.stats, att, acc, status, nev);
 }
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -669,7 +765,7 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
         a.tau0 = 0;
         a.tstride = 1;
         a.ntau = T;
-        a.stats = h->d_stats + 0;
+        a.stats = h->d_stats + 0 * 5 * STAT_SLOTS;
         {
             ProfScope pg(h, ESM_KERN_GEN);
             hipLaunchKernelGGL((k_gen<S>), dim3(nblk((size_t)N * T * R, B)), dim3(B), 0, h->stream, m, st, a, beta, mxn);
@@ -680,7 +776,7 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
             a.round = 1000u + (uint32_t)tr.colour;
             a.nodes = h->d_theta_nodes + tr.off;
             a.nnodes = tr.count;
-            a.stats = h->d_stats + 4;
+            a.stats = h->d_stats + 1 * 5 * STAT_SLOTS;
             const unsigned nb = nblk((size_t)tr.count * T * R, B);
             ProfScope pt(h, ESM_KERN_THETA);
             switch (tr.dmax) {
@@ -700,7 +796,7 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
                     a.tstride = 2;
                     a.ntau = h->tclass[c].second;
                     a.round = (fam == 0 ? 2000u : 3000u) + (uint32_t)(s * 4 + (int)c);
-                    a.stats = h->d_stats + (fam == 0 ? 8 : 12);
+                    a.stats = h->d_stats + (fam == 0 ? 2 : 3) * 5 * STAT_SLOTS;
                     const unsigned nb = nblk((size_t)N * a.ntau * R, B);
                     ProfScope pf(h, fam == 0 ? ESM_KERN_SOC : ESM_KERN_SPAIR);
                     if (fam == 0)

@@ -141,12 +141,12 @@ FAMILY_NAMES = ("gen", "theta", "soc", "spair")
 
 def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0, maxnats: float = 40.0):
     """``operation_sweep!`` / ``local_sweep!`` (src/local.jl:1927-2062, :2401-2425) of every replica,
-    ``nsweeps`` times at fixed capacities and weather. Returns the move statistics [4 families, 4]
-    (attempted, accepted, improper, point-cap)."""
-    st = np.zeros(16, dtype=np.int64)
+    ``nsweeps`` times at fixed capacities and weather. Returns the move statistics [4 families, 5]
+    (attempted, accepted, improper, point-cap or non-finite, ΔC evaluations)."""
+    st = np.zeros(20, dtype=np.int64)
     check(R.lib.esm_sweeps(R.handle, int(nsweeps), _key(key), int(counter) & 0xFFFFFFFF, float(maxnats),
                            st.ctypes.data_as(C.POINTER(C.c_int64))), "esm_sweeps")
-    return st.reshape(4, 4)
+    return st.reshape(4, 5)
 
 
 def follow_capacities(R: Replicas, knew, kmin: float = 1.0):

@@ -179,8 +179,9 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
  *   3 storage–gas pairs (`prep_spair!`, :1708-1726), per storage carrier and parity class.
  * Each (site, replica) draws its two uniforms from Philox4x32-10 under `key` with counter
  * (replica, node·T + step, round id, counter + sweep index). The residual and flow caches are
- * updated incrementally. stats ([4][4], may be NULL): per family the moves attempted,
- * accepted (t ≠ 0), improper lines (t = 0) and walks cut at the point cap (t = 0).
+ * updated incrementally. stats ([4][5], may be NULL): per family the moves attempted,
+ * accepted (t ≠ 0), improper lines (t = 0), walks cut at the point cap or drawn non-finite
+ * (t = 0), and the ΔC(t) evaluations made (the walks, the outward re-walk to the drawn segment).
  */
 int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
                    int64_t *stats);

@@ -45,7 +45,7 @@ int orc_categorical(const double *p, int K, double u);
 void orc_regime_window_path(const esm_model *m, int *path, int a, int b, const double *us);
 
 /* Operation sweeps (oracle_sweep.c), in place on v and the caches r, F (built by orc_energy);
- * stats [4][4] as esm_sweeps; nthreads OpenMP threads over the replicas. */
+ * stats [4][5] as esm_sweeps (the evaluation count is the oracle's own); nthreads OpenMP threads over the replicas. */
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
                const uint32_t key[2], uint32_t ctr0, double maxnats, int64_t *stats, int nthreads);
 int orc_colouring(const esm_model *m, int *colour);

@@ -140,12 +140,12 @@ class Oracle:
             _, _, r, F = self.energy(v, k, x)
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         F = np.ascontiguousarray(F, dtype=np.float64).copy()
-        st = np.zeros(16, dtype=np.int64)
+        st = np.zeros(20, dtype=np.int64)
         kk = (C.c_uint32 * 2)(*key)
         rc = self.lib.orc_sweeps(C.byref(self.cm), R, _p(v), _p(k), _p(r), _p(F), int(nsweeps), kk, int(counter),
                                  float(maxnats), st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
         assert rc == 0
-        return v, r, F, st.reshape(4, 4)
+        return v, r, F, st.reshape(4, 5)
 
     def follow(self, v, k, x, knew, kmin=1.0):
         """follow_capacities! on copies: returns (v, k)."""

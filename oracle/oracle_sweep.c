@@ -304,9 +304,12 @@ typedef struct {
     int stor;
 } Block;
 
+static __thread long g_nev;   /* ΔC evaluations of the current thread (statistics) */
+
 /* mv_eval! of the block move: later step first, secondaries appended in that order */
 static double mv_eval(Block *b, double t) {
     double D = 0.0;
+    g_nev++;
     b->nsec = 0;
     Step *order[2] = {b->ws, b->wa};
     for (int q = 0; q < 2; q++) {
@@ -557,7 +560,8 @@ static void clear_step(Step *w, int tau) {
     w->ne = w->nl = w->nn = w->nstu = w->nbp = w->ng = w->nsec = 0;
 }
 
-static void stat_add(int64_t *st, int status, double t) {
+static void stat_add(int64_t *st, int status, double t, long nev) {
+    st[4] += nev;
     st[0]++;
     if (t != 0.0) st[1]++;
     if (status == 1) st[2]++;
@@ -599,9 +603,11 @@ static void do_move(Sys *S, int j, int fam, int s, int n, int tau, const uint32_
     uint32_t c4[4] = {(uint32_t)j, (uint32_t)(n * T + tau), round, ctr}, w[4];
     orc_philox(key, c4, w);
     int status;
+    g_nev = 0;
     double t = sample_line(&b, m->beta_v, maxnats, u01(w[0], w[1]), u01(w[2], w[3]), &status);
+    long nev = g_nev;
     commit(&b, t);
-    stat_add(stats, status, t);
+    stat_add(stats, status, t, nev);
 }
 
 /* greedy distance-2 colouring in node order */
@@ -683,11 +689,11 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
         ncls = 3; cfirst[0] = 0; ccount[0] = (T - 1) / 2; cfirst[1] = 1; ccount[1] = (T - 1) / 2;
         cfirst[2] = T - 1; ccount[2] = 1;
     }
-    int64_t st[16];
+    int64_t st[20];
     memset(st, 0, sizeof(st));
     if (nthreads < 1) nthreads = 1;
     /* replicas are independent: threads over replicas, each with its own statistics */
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : st[:16])
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : st[:20])
     for (int j = 0; j < R; j++) {
         Sys S = S0;
         for (int sw = 0; sw < nsweeps; sw++) {
@@ -697,7 +703,7 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
             for (int c = 0; c < nc; c++)
                 for (int n = 0; n < N; n++) {
                     if (colour[n] != c || S.adj_ptr[n + 1] == S.adj_ptr[n]) continue;
-                    for (int t = 0; t < T; t++) do_move(&S, j, 1, 0, n, t, key, 1000u + (uint32_t)c, ctr, maxnats, st + 4);
+                    for (int t = 0; t < T; t++) do_move(&S, j, 1, 0, n, t, key, 1000u + (uint32_t)c, ctr, maxnats, st + 5);
                 }
             for (int fam = 2; fam <= 3; fam++)
                 for (int s = 0; s < ESM_NSTOR; s++)
@@ -706,7 +712,7 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
                             for (int q = 0; q < ccount[c]; q++) {
                                 int t = cfirst[c] + 2 * q;
                                 do_move(&S, j, fam, s, n, t, key, (fam == 2 ? 2000u : 3000u) + (uint32_t)(s * 4 + c),
-                                        ctr, maxnats, st + 4 * fam);
+                                        ctr, maxnats, st + 5 * fam);
                             }
         }
     }

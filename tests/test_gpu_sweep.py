@@ -42,7 +42,7 @@ def test_sweep_fp64_matches_oracle(nsweeps):
     vd, _, _ = dev.download()
     rd, Fd = dev.caches()
     v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, k, nsweeps, key=(3, 4), counter=10, x=x, threads=4)
-    assert np.array_equal(st, st_)
+    assert np.array_equal(st[:, :4], st_[:, :4])
     for f, scale in ((0, np.abs(v_[0]).max()), (1, 1.0), (2, 1.0), (3, 1.0)):
         np.testing.assert_allclose(vd[f], v_[f], atol=1e-9 * scale, err_msg=f"field {f}")
     np.testing.assert_allclose(rd, r_, atol=1e-9 * np.abs(s.load).max())
@@ -64,7 +64,7 @@ def test_sweep_fp64_larger_network():
     st = api.operation_sweep(dev, 1, key=(9, 9), counter=0)
     vd, _, _ = dev.download()
     v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, k, 1, key=(9, 9), counter=0, x=x, threads=4)
-    assert np.array_equal(st, st_)
+    assert np.array_equal(st[:, :4], st_[:, :4])
     np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
     np.testing.assert_allclose(vd[0], v_[0], atol=1e-9 * np.abs(v_[0]).max())
     dev.close()
