@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--fp64", action="store_true", help="FP64 device state")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="first run a plain device copy of 1 GiB three times (the calibration kernel of the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
     return ap.parse_args()
 
 
@@ -114,6 +117,13 @@ def main():
 
     prec = FP64 if args.fp64 else FP32
     es = 8 if prec == FP64 else 4
+    if args.copy_calib:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")     # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     T, R = args.hours, args.replicas
     wl = Workload(args.nodes, args.lines, T, R, prec)
     lib, h = wl.dev.lib, wl.dev.handle
@@ -145,6 +155,7 @@ def main():
     kern_ms = tot_ms / max(cnt, 1)
     achieved = kb[dom] / (kern_ms * 1e-3) / 1e9
     breakdown = {k: round(v[0] / args.steps, 4) for k, v in prof.items()}
+    traffic = measured_traffic(dom, wl.s, T, R, prec)
     line_moves = (wl.s.N * 6) * T * R          # gas, potential, 2 SoC, 2 pairs per node and step
     out = {
         "metric": "sweeps/s (replica steps/s of the plain GPU replica sampler)",
@@ -164,7 +175,7 @@ def main():
                                f"following)", "replicas": R, "hours": T, "nodes": wl.s.N, "lines": wl.s.L,
                    "line_moves_per_step": line_moves, "line_moves_per_s": line_moves * value},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_ms": kern_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms": kern_ms,
                      "bytes_per_launch": kb[dom], "launches": cnt},
         "kernel_ms_per_step": breakdown,
         "sweep_stats": {name: {"moves": int(st[0]), "accepted": int(st[1]), "improper": int(st[2]),
@@ -176,6 +187,18 @@ def main():
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_threads)
     wl.dev.close()
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel, s, T, R, prec):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated against a plain copy), when it was measured on this exact workload."""
+    from esmsampling import FP32
+    p = ROOT / "profiles" / "hbm_traffic_replica_step_40n_8760h_64r_fp32.json"
+    if not (p.exists() and s.N == 40 and s.L == 70 and T == 8760 and R == 64 and prec == FP32):
+        return None
+    names = {"k_spair": "k_soc<float, true>", "k_soc": "k_soc<float, false>", "k_gen": "k_gen<float>",
+             "k_energy": "k_energy<float, true, false>", "k_follow": "k_follow<float>"}
+    return json.loads(p.read_text())["bytes_per_dispatch"].get(names.get(kernel, ""))
 
 
 def cpu_baseline(wl, threads):

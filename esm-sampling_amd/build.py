@@ -24,7 +24,7 @@ ARCH = os.environ.get("ESM_OFFLOAD_ARCH", "gfx950")
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-ffp-contract=off", "-Wno-unused-result",
+    "-ffp-contract=off", "-Wno-unused-result", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
 ]
 
 
