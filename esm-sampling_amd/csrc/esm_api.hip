@@ -722,6 +722,29 @@ int32_t esm_upload_weather(esm_handle *h, const int32_t *m, const double *z) {
     return ESM_OK;
 }
 
+int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t jd) {
+    if (!src || !dst) { set_error("esm_copy_replica: null handle"); return ESM_EINVAL; }
+    if (src->N != dst->N || src->L != dst->L || src->T != dst->T || src->prec != dst->prec) {
+        set_error("esm_copy_replica: the handles differ in shape or precision");
+        return ESM_EINVAL;
+    }
+    if (js < 0 || js >= src->R || jd < 0 || jd >= dst->R) { set_error("esm_copy_replica: replica index out of range"); return ESM_EINVAL; }
+    if (src == dst && js == jd) return ESM_OK;
+    ESM_HIP_CHECK(hipStreamSynchronize(src->stream));
+    const bool weather = src->has_weather && dst->has_weather && src->weather_ready;
+    int rc;
+    if ((rc = launch_copy_replica(src, js, dst, jd, weather))) return rc;
+    if (dst != src) {
+        dst->k = src->k;
+        if ((rc = upload_k(dst))) return rc;
+    }
+    if (weather && dst->R == 1) dst->weather_ready = true;
+    if ((rc = rebuild(dst))) return rc;
+    ESM_HIP_CHECK(hipStreamSynchronize(dst->stream));
+    dst->scores_valid = false;
+    return ESM_OK;
+}
+
 int32_t esm_profile(esm_handle *h, int32_t enable) {
     if (!h) { set_error("esm_profile: null handle"); return ESM_EINVAL; }
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));

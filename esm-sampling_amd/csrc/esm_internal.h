@@ -147,4 +147,8 @@ int launch_follow(esm_handle *h, const void *d_kc0, const void *d_kc1, const int
 template <typename S>
 int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0, uint32_t k1, uint32_t ctr);
 
+// Copy replica js of hs into replica jd of hd (operation, raw weather and, with `weather`, the
+// anomalies and regimes) on hd's stream. Both handles have the same shape and precision.
+int launch_copy_replica(const esm_handle *hs, int js, esm_handle *hd, int jd, bool weather);
+
 }  // namespace esm

@@ -92,6 +92,7 @@ SIGNATURES = [
                                      _ip]),
     ("esm_download_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
     ("esm_upload_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
+    ("esm_copy_replica", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     ("esm_profile", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_profile_read", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
 ]

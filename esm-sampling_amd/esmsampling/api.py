@@ -203,6 +203,12 @@ def set_weather(R: Replicas, m, z):
     check(R.lib.esm_upload_weather(R.handle, iptr(m), dptr(z)), "esm_upload_weather")
 
 
+def copy_replica(src: Replicas, js: int, dst: Replicas, jd: int):
+    """Copy replica ``js`` of ``src`` into replica ``jd`` of ``dst`` on the device (operation, weather
+    and capacities; ``dst``'s caches are rebuilt): ``copy(s)`` of the lag probe, src/replicas.jl:736."""
+    check(src.lib.esm_copy_replica(src.handle, int(js), dst.handle, int(jd)), "esm_copy_replica")
+
+
 def weather_schedule(it: int, j: int, M: int, theta: float, burst_every: int, phi: float):
     """The weather angle of replica j (0-based) at step ``it`` and whether it is a burst
     (src/replicas.jl:571-576): bursts every ``burst_every`` steps, staggered by

@@ -1,0 +1,486 @@
+"""The replica–Langevin driver (``sampler = "replicas"``) over the device path, on one GPU.
+
+Mirrors ``run_replicas`` (src/replicas.jl:841-947) and its pieces: ``ReplicaCtl``
+(:481-515), the gate (``relaxation_indicator`` :634-641, ``update_gate!`` :651-661), the drift
+weights (:303-314, :664), ``replica_step!`` (:676-705), the lag probe and the step adaptation
+(:717-773), ``init_replicas`` (:783-795) and the replica-mean monitor (:827) with the
+convergence report of the drivers (src/chains.jl:301-310). One chain is one ``api.Replicas``
+handle (its replicas share the capacities); every device call works on all replicas of the
+chain at once.
+
+Configuration: TOML with the reference's key names (src/run.jl:5-125) and its defaults where
+this path implements the feature; an unknown key raises, as ``load_config`` does
+(src/run.jl:135-150). The system is the seeded synthetic one (``synth.make_system``), set by
+the keys ``synthetic_nodes``, ``synthetic_lines``, ``snapshots``, ``resolution_hours``,
+``system_seed`` and ``precision``, which have no counterpart in the reference (it reads its
+data file). Keys of features not built here are accepted only at their "off" value:
+``replica_ffbs = false``, ``tilt_lambda = 0``, ``storage_long = 0``, ``storage_xpair = 0``.
+
+Random streams: Philox4x32-10 keys per (seed, chain, purpose) replace the MersenneTwister streams
+``restart_seed(cfg, r, 1000c + j)`` (src/replicas.jl:872-874); the Langevin normals come from a
+numpy Philox generator per chain.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import time
+
+import numpy as np
+
+from . import FP32, FP64, api, synth
+from .diagnostics import convergence_summary, warmup_end
+
+NK = synth.NK
+
+# κ_s offsets of w = log(k + κ_s) per carrier (MW), src/replicas.jl:518
+LANGEVIN_KAPPA = {"onwind": 500.0, "solar": 1000.0, "offwind": 200.0, "gas": 300.0, "battery": 100.0, "H2": 100.0}
+
+
+@dataclasses.dataclass
+class Config:
+    """The configuration keys of the replica path (reference names and defaults, src/run.jl)."""
+    seed: int = 1
+    sampler: str = "replicas"
+    out: str = ""
+    nchains: int = 4
+    chain_starts: list = dataclasses.field(default_factory=lambda: ["low", "high", "heuristic", "heuristic"])
+    replicas: int = 8
+    mlda_warmup: int = 100
+    nsamples: int = 300
+    c_shed: float = 3000.0
+    c_dump: float = 5.0
+    c_viol: float = 3000.0
+    beta_v: float = 4e-5
+    beta_k: float = 4e-8
+    rhat_max: float = 1.01
+    ess_min: float = 400.0
+    mcse_max: float = 0.05
+    rhat_screen: float = 1.05
+    replica_init_sweeps: int = 40
+    replica_inner: int = 1
+    replica_theta: float = 0.0
+    replica_burst_every: int = 40
+    replica_burst_angle: float = 0.3
+    replica_window: int = 60
+    replica_window_prob: float = 0.5
+    replica_ffbs: bool = False
+    replica_compensate: bool = True
+    replica_gate_min: int = 10
+    replica_gate_max: int = 30
+    replica_gate_min_window: int = 20
+    replica_gate_max_window: int = 60
+    replica_gate_win: int = 8
+    replica_gate_tol: float = 0.3
+    replica_probe_every: int = 20
+    replica_probe_len: int = 30
+    replica_probe_tail: int = 15
+    replica_probe_win: int = 10
+    replica_probe_avg: int = 3
+    langevin_eta: float = 5e-4
+    langevin_eta_min: float = 2e-5
+    langevin_eta_max: float = 2e-3
+    langevin_adapt: bool = True
+    langevin_gap_hi: float = 0.10
+    langevin_gap_lo: float = 0.05
+    langevin_kappa: dict = dataclasses.field(default_factory=dict)
+    langevin_kappa_default: float = 100.0
+    langevin_kmin: float = 1.0
+    langevin_clip: float = 0.02
+    langevin_box: bool = True
+    tilt_lambda: float = 0.0
+    tilt_functional: str = "resload_mean"
+    storage_long: float = 0.0
+    storage_xpair: float = 0.0
+    maxnats: float = 40.0
+    # the synthetic system (no counterpart in the reference, which reads its data file)
+    synthetic_nodes: int = 40
+    synthetic_lines: int = 70
+    snapshots: int = 8760
+    resolution_hours: float = 1.0
+    system_seed: int = 1
+    precision: str = "fp32"
+
+    def check(self):
+        if self.sampler != "replicas":
+            raise ValueError(f'sampler = "{self.sampler}": this driver runs sampler = "replicas" only')
+        if self.replica_ffbs:
+            raise ValueError("replica_ffbs = true: the centred FFBS of the regimes is not built (its default is false)")
+        if self.tilt_lambda != 0.0:
+            raise ValueError("tilt_lambda ≠ 0: the stress tilt functional is not built (λ = 0 is the untilted drift)")
+        if self.storage_long != 0.0 or self.storage_xpair != 0.0:
+            raise ValueError("storage_long / storage_xpair > 0: the long-range and cross-node storage moves are not built")
+        if self.precision not in ("fp32", "fp64"):
+            raise ValueError('precision must be "fp32" or "fp64"')
+        if self.replicas < 1 or self.nchains < 1:
+            raise ValueError("replicas and nchains must be positive")
+        for s in self.chain_starts:
+            if s not in ("low", "high", "heuristic"):
+                raise ValueError(f'chain start "{s}": "low", "high" or "heuristic"')
+        return self
+
+
+def load_config(path_or_dict) -> Config:
+    """A Config from a TOML file (or a dict of the same keys); unknown keys raise."""
+    if isinstance(path_or_dict, dict):
+        d = dict(path_or_dict)
+    else:
+        import tomli
+        with open(path_or_dict, "rb") as f:
+            d = tomli.load(f)
+    names = {f.name for f in dataclasses.fields(Config)}
+    for k in d:
+        if k not in names:
+            raise ValueError(f"unknown config key {k}")
+    return Config(**d).check()
+
+
+@dataclasses.dataclass
+class ReplicaCtl:
+    """Settings of the replica sampler (``ReplicaCtl(cfg, P)``, src/replicas.jl:524-538)."""
+    M: int
+    inner: int
+    theta: float
+    burst_every: int
+    phi: float
+    window: int
+    window_prob: float
+    comp: bool
+    gate_min: int
+    gate_max: int
+    gate_min_window: int
+    gate_max_window: int
+    gate_win: int
+    gate_tol: float
+    eta_w: np.ndarray
+    kappa_s: np.ndarray
+    kmin: float
+    clip: float
+    box: bool
+    probe_every: int
+    probe_len: int
+    probe_tail: int
+    probe_win: int
+    probe_avg: int
+    adapt: bool
+    gap_hi: float
+    gap_lo: float
+    eta_min: float
+    eta_max: float
+    shrink: float = 0.7
+    grow: float = 1.2
+
+    @classmethod
+    def from_config(cls, cfg: Config):
+        ks = np.array([cfg.langevin_kappa.get(c, LANGEVIN_KAPPA.get(c, cfg.langevin_kappa_default))
+                       for c in synth.CARRIERS], dtype=np.float64)
+        return cls(M=cfg.replicas, inner=cfg.replica_inner, theta=cfg.replica_theta, burst_every=cfg.replica_burst_every,
+                   phi=cfg.replica_burst_angle, window=cfg.replica_window, window_prob=cfg.replica_window_prob,
+                   comp=cfg.replica_compensate, gate_min=cfg.replica_gate_min, gate_max=cfg.replica_gate_max,
+                   gate_min_window=cfg.replica_gate_min_window, gate_max_window=cfg.replica_gate_max_window,
+                   gate_win=cfg.replica_gate_win, gate_tol=cfg.replica_gate_tol,
+                   eta_w=np.full(NK, cfg.langevin_eta), kappa_s=ks, kmin=cfg.langevin_kmin, clip=cfg.langevin_clip,
+                   box=cfg.langevin_box, probe_every=cfg.replica_probe_every, probe_len=cfg.replica_probe_len,
+                   probe_tail=cfg.replica_probe_tail, probe_win=cfg.replica_probe_win, probe_avg=cfg.replica_probe_avg,
+                   adapt=cfg.langevin_adapt, gap_hi=cfg.langevin_gap_hi, gap_lo=cfg.langevin_gap_lo,
+                   eta_min=cfg.langevin_eta_min, eta_max=cfg.langevin_eta_max)
+
+
+def interior_nodes(k, khi, lo: float = 50.0):
+    """Per k field the nodes with lo < k < box − 1 (src/replicas.jl:475)."""
+    return (k > lo) & (k < khi - 1.0)
+
+
+def masked_means(v, mask):
+    """Mean of every field's node vector over its masked nodes, NaN where none (src/replicas.jl:477)."""
+    out = np.full(v.shape[0], np.nan)
+    for f in range(v.shape[0]):
+        if mask[f].any():
+            out[f] = v[f][mask[f]].mean()
+    return out
+
+
+def relaxation_indicator(gbuf, win: int) -> float:
+    """Largest difference over the k fields between the mean normalised carrier score of the last
+    win/2 steps and of the win/2 before; inf until there are ``win`` steps (src/replicas.jl:634-641)."""
+    h = win // 2
+    if not (len(gbuf) >= 2 * h and h >= 1):
+        return math.inf
+    n = len(gbuf)
+    a = np.mean(gbuf[n - 2 * h:n - h], axis=0)
+    b = np.mean(gbuf[n - h:n], axis=0)
+    d = [abs(x - y) for x, y in zip(a, b) if math.isfinite(x) and math.isfinite(y)]
+    return max(d) if d else 0.0
+
+
+class ReplicaSet:
+    """One chain's replicas (a device handle) and their gate, probe windows and Langevin step
+    (``ReplicaSet``, src/replicas.jl:547-562)."""
+
+    def __init__(self, dev: api.Replicas | None, eta_w, M: int | None = None):
+        M = dev.R if dev is not None else int(M)
+        self.dev = dev
+        self.k = None
+        self.since = np.full(M, 10 ** 9, dtype=np.int64)
+        self.window = np.zeros(M, dtype=bool)
+        self.active = np.ones(M, dtype=bool)
+        self.gbuf = [[] for _ in range(M)]
+        self.hist = [[] for _ in range(M)]
+        self.eta_w = np.array(eta_w, dtype=np.float64)
+        self.gaps = []
+
+
+def update_gate(R: ReplicaSet, j: int, g, ctl: ReplicaCtl) -> bool:
+    """``update_gate!`` (src/replicas.jl:651-661)."""
+    if ctl.burst_every == 0:
+        R.active[j] = True
+        return True
+    if R.active[j]:
+        return True
+    R.gbuf[j].append(np.asarray(g, dtype=np.float64))
+    if len(R.gbuf[j]) > ctl.gate_win:
+        R.gbuf[j].pop(0)
+    gmin, gmax = (ctl.gate_min_window, ctl.gate_max_window) if R.window[j] else (ctl.gate_min, ctl.gate_max)
+    if R.since[j] < gmin:
+        return False
+    R.active[j] = bool(R.since[j] >= gmax or relaxation_indicator(R.gbuf[j], ctl.gate_win) < ctl.gate_tol)
+    if R.active[j]:
+        R.gbuf[j].clear()
+    return bool(R.active[j])
+
+
+def drift_weights(active):
+    """Drift weights without a tilt functional: 1/M_active over the active replicas (:664, :310-314)."""
+    return api.uniform_weights(active)
+
+
+class Keys:
+    """Philox keys of one chain: (seed, 1000·chain + purpose)."""
+
+    def __init__(self, seed: int, chain: int):
+        base = 1000 * (chain + 1)
+        self.sweep = (seed, base + 1)
+        self.weather = (seed, base + 2)
+        self.init = (seed, base + 3)
+        self.probe = (seed, base + 500)
+
+
+def _sweep_scores(R: ReplicaSet, nsweeps: int, key, counter: int, maxnats: float, r: float):
+    """``nsweeps`` operation sweeps, after each the energy (caches rebuilt) and the capacity
+    scores; returns (mean of r·ŝ over the sweeps [R, 6, N], fk, parts of the last sweep, stats)."""
+    acc, fk, parts, stats = None, None, None, np.zeros((4, 5), dtype=np.int64)
+    for q in range(nsweeps):
+        stats += api.operation_sweep(R.dev, 1, key=key, counter=counter + q, maxnats=maxnats)
+        parts = api.energy_cost(R.dev)
+        sk, fk = api.capacity_scores(R.dev)
+        acc = r * sk / nsweeps if acc is None else acc + r * sk / nsweeps
+    return acc, fk, parts, stats
+
+
+def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, maxnats: float = 40.0):
+    """``replica_step!`` (src/replicas.jl:676-705) of one chain: the weather step of every replica
+    (scheduled angle, regime window with a burst, compensated re-dispatch), ``inner`` operation
+    sweeps with the mean capacity score, the gates, the drift weights, one Langevin step of the
+    shared capacities and the dispatch following on every replica. Returns the step's record."""
+    M = R.dev.R
+    r = sys.beta_k / sys.beta_v
+    th = np.zeros(M)
+    wp = np.zeros(M)
+    burst = np.zeros(M, dtype=bool)
+    for j in range(M):
+        th[j], burst[j] = api.weather_schedule(it, j, M, ctl.theta, ctl.burst_every, ctl.phi)
+        regime = (burst[j] or ctl.burst_every == 0) and ctl.window > 0
+        wp[j] = ctl.window_prob if regime else 0.0
+    t0 = time.perf_counter()
+    moved = np.zeros(M, dtype=np.int32)
+    if (th > 0).any() or (wp > 0).any():
+        moved = api.weather_process_step(R.dev, th, wp, window=max(ctl.window, 1), compensate=ctl.comp,
+                                         key=keys.weather, counter=it)
+    for j in range(M):
+        if burst[j]:
+            R.since[j] = 0
+            R.window[j] = moved[j] == 2
+            R.active[j] = False
+            R.gbuf[j].clear()
+            R.hist[j].clear()
+        else:
+            R.since[j] += 1
+    Kraw = R.k.copy()
+    mask = interior_nodes(Kraw, sys.khi)
+    rsk, fk, parts, stats = _sweep_scores(R, ctl.inner, keys.sweep, it * ctl.inner, maxnats, r)
+    fkm = masked_means(fk, mask)
+    for j in range(M):
+        update_gate(R, j, masked_means(rsk[j], mask) / fkm, ctl)
+        if R.active[j]:
+            R.hist[j].append(rsk[j])
+            if len(R.hist[j]) > ctl.probe_win:
+                R.hist[j].pop(0)
+    wts = drift_weights(R.active)
+    xi = rng.standard_normal((NK, sys.N))
+    knew, nclip = api.langevin_capacities(R.dev, wts, xi, R.eta_w, ctl.kappa_s, kmin=ctl.kmin, clip=ctl.clip,
+                                          box=ctl.box)
+    R.k = api.follow_capacities(R.dev, knew, kmin=ctl.kmin)
+    tw = time.perf_counter() - t0
+    rs = masked_means(np.tensordot(wts, rsk, axes=1), mask)
+    return dict(it=it, parts=parts, wts=wts, moved=moved, nclip=nclip, Kraw=Kraw, knew=knew, rs=rs, fk=fkm, tw=tw,
+                stats=stats)
+
+
+def probe_candidate(R: ReplicaSet, ctl: ReplicaCtl):
+    """The active replica with a full in-flight window that is longest after its burst (:717-721)."""
+    cand = [j for j in range(R.dev.R) if R.active[j] and len(R.hist[j]) >= ctl.probe_win]
+    if not cand:
+        return None
+    return cand[int(np.argmax([R.since[j] for j in cand]))]
+
+
+def lag_probe(R: ReplicaSet, j: int, probe: api.Replicas, ctl: ReplicaCtl, sys, key, counter: int,
+              maxnats: float = 40.0):
+    """``lag_probe`` (src/replicas.jl:733-750): sweep a copy of replica j at frozen (k, x) and
+    compare its equilibrated score with the replica's in-flight score, per k field."""
+    r = sys.beta_k / sys.beta_v
+    api.copy_replica(R.dev, j, probe, 0)
+    parts_in = api.energy_cost(probe)[0]
+    acc, fk, parts = None, None, None
+    for q in range(ctl.probe_len):
+        api.operation_sweep(probe, 1, key=key, counter=counter + q, maxnats=maxnats)
+        if q >= ctl.probe_len - ctl.probe_tail:
+            parts = api.energy_cost(probe)[0]
+            sk, fk = api.capacity_scores(probe)
+            acc = r * sk[0] / ctl.probe_tail if acc is None else acc + r * sk[0] / ctl.probe_tail
+    inflight = np.mean(R.hist[j], axis=0)
+    mask = interior_nodes(R.k, sys.khi)
+    rs_eq, rs_in, fkm = masked_means(acc, mask), masked_means(inflight, mask), masked_means(fk, mask)
+    return dict(gap=(rs_in - rs_eq) / fkm, rs_eq=rs_eq, rs_in=rs_in, fk=fkm, op_in=parts_in[1], op_eq=parts[1],
+                shed_in=parts_in[4], shed_eq=parts[4])
+
+
+def adapt_step(R: ReplicaSet, gap, ctl: ReplicaCtl) -> bool:
+    """``adapt_step!`` (src/replicas.jl:760-773)."""
+    R.gaps.append(np.array(gap, dtype=np.float64))
+    if len(R.gaps) < ctl.probe_avg:
+        return False
+    for f in range(len(R.eta_w)):
+        v = [g[f] for g in R.gaps if math.isfinite(g[f])]
+        if not v:
+            continue
+        gm = float(np.mean(v))
+        if abs(gm) > ctl.gap_hi:
+            R.eta_w[f] = max(ctl.eta_min, R.eta_w[f] * ctl.shrink)
+        elif abs(gm) < ctl.gap_lo:
+            R.eta_w[f] = min(ctl.eta_max, R.eta_w[f] * ctl.grow)
+    R.gaps.clear()
+    return True
+
+
+MONITOR_NAMES = [f"k_{c}" for c in synth.CARRIERS] + ["capex", "opex", "total", "shed_MWh", "penalty", "co2_t"]
+
+
+def replica_monitor(R: ReplicaSet, parts):
+    """Replica means of the monitored quantities (:827): capacity totals per carrier (shared),
+    capex, opex, total = capex + opex (src/sampler.jl:34), shed, penalty, CO2."""
+    k = R.k
+    p = parts.mean(axis=0)
+    return np.concatenate([k.sum(axis=1), [p[0], p[1], p[0] + p[1], p[4], p[6], p[7]]])
+
+
+def init_replicas(sys, k0, M: int, prec: int, ctl: ReplicaCtl, keys: Keys, nsweeps: int, maxnats: float = 40.0):
+    """``init_replicas`` (src/replicas.jl:783-795): M replicas at capacities k0, each with an exact
+    prior weather draw, the heuristic operation for its weather, and ``nsweeps`` sweeps at fixed k."""
+    dev = api.Replicas(sys, M, prec)
+    api.sample_weather(dev, key=keys.weather, counter=0xFFFFFFF0)
+    _, _, x = dev.download()
+    dev.upload(v=synth.heuristic_state(sys, x, k0), k=k0)
+    R = ReplicaSet(dev, ctl.eta_w)
+    R.k = np.array(k0, dtype=np.float64)
+    if nsweeps > 0:
+        api.operation_sweep(dev, nsweeps, key=keys.init, counter=0, maxnats=maxnats)
+    return R
+
+
+def chain_start_capacities(sys, kind: str):
+    """Capacities of a chain start: ×0.5 (low), ×2 (high) or as set up (src/run.jl:77)."""
+    f = {"low": 0.5, "high": 2.0, "heuristic": 1.0}[kind]
+    return np.clip(sys.k0 * f, sys.klo, sys.khi)
+
+
+def run_replicas(cfg: Config, log=None):
+    """``run_replicas`` (src/replicas.jl:841-947) on one GPU: ``nchains`` chains of ``replicas``
+    weather replicas, ``mlda_warmup`` + ``nsamples`` replica steps, probes and step adaptation
+    during the warmup, and the convergence report of the replica-mean quantities over the
+    production window, with ESS per second of production wall time."""
+    cfg = cfg.check() if isinstance(cfg, Config) else load_config(cfg)
+    say = log or (lambda s: None)
+    t_setup = time.perf_counter()
+    sys = synth.make_system(N=cfg.synthetic_nodes, L=cfg.synthetic_lines, T=cfg.snapshots, dt=cfg.resolution_hours,
+                            seed=cfg.system_seed)
+    sys = dataclasses.replace(sys, c_shed=cfg.c_shed, c_dump=cfg.c_dump, c_viol=cfg.c_viol, beta_v=cfg.beta_v,
+                              beta_k=cfg.beta_k)
+    prec = FP64 if cfg.precision == "fp64" else FP32
+    ctl = ReplicaCtl.from_config(cfg)
+    nc, M = cfg.nchains, cfg.replicas
+    nwarm, nprod = cfg.mlda_warmup, cfg.nsamples
+    kinds = [cfg.chain_starts[c % len(cfg.chain_starts)] for c in range(nc)]
+    keys = [Keys(cfg.seed, c) for c in range(nc)]
+    rngs = [np.random.Generator(np.random.Philox(key=[cfg.seed, 1000 * (c + 1)])) for c in range(nc)]
+    sets = []
+    for c in range(nc):
+        sets.append(init_replicas(sys, chain_start_capacities(sys, kinds[c]), M, prec, ctl, keys[c],
+                                  cfg.replica_init_sweeps, cfg.maxnats))
+        say(f"chain {c} ({kinds[c]}): {M} replicas initialised ({cfg.replica_init_sweeps} sweeps each)")
+    probe = api.Replicas(sys, 1, prec) if ctl.probe_every > 0 else None
+    t_init = time.perf_counter()
+    traces = [[] for _ in range(nc)]
+    records = [[] for _ in range(nc)]
+    probes = [[] for _ in range(nc)]
+    nprobe = [0] * nc
+    t_prod0 = None
+    for it in range(1, nwarm + nprod + 1):
+        if it == nwarm + 1:
+            t_prod0 = time.perf_counter()
+        for c in range(nc):
+            R = sets[c]
+            st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats)
+            traces[c].append(replica_monitor(R, st["parts"]))
+            records[c].append(dict(it=it, k=R.k.copy(), active=R.active.copy(), weights=st["wts"], nclip=st["nclip"],
+                                   moved=st["moved"], rs=st["rs"], fk=st["fk"], eta_w=R.eta_w.copy(),
+                                   op=st["parts"][:, 1].copy(), shed=st["parts"][:, 4].copy()))
+            if ctl.probe_every > 0 and it % ctl.probe_every == 0:
+                j = probe_candidate(R, ctl)
+                if j is not None:
+                    pr = lag_probe(R, j, probe, ctl, sys, keys[c].probe, nprobe[c] * ctl.probe_len, cfg.maxnats)
+                    nprobe[c] += 1
+                    eta0 = R.eta_w.copy()
+                    adapted = ctl.adapt and it <= nwarm and adapt_step(R, pr["gap"], ctl)
+                    probes[c].append(dict(it=it, j=j, adapted=adapted, eta_before=eta0, eta_after=R.eta_w.copy(), **pr))
+    t_end = time.perf_counter()
+    if t_prod0 is None:
+        t_prod0 = t_end
+    # convergence report on the replica-mean quantities (src/chains.jl:301-310)
+    nq = len(MONITOR_NAMES)
+    qtr = [[np.array([tr[t][q] for t in range(len(tr))]) for tr in traces] for q in range(nq)]
+    good = [all(np.all(np.isfinite(c)) for c in qtr[q]) for q in range(nq)]
+    w0 = nwarm
+    if nwarm + nprod >= 20:
+        wdet = warmup_end([qtr[q] for q in range(nq) if good[q]], threshold=cfg.rhat_screen,
+                          every=max(5, (nwarm + nprod) // 40))
+        w0 = max(nwarm, wdet)
+        if w0 >= nwarm + nprod - 4:
+            w0 = nwarm
+    summ = {}
+    for q in range(nq):
+        if good[q] and nwarm + nprod - w0 >= 4:
+            summ[MONITOR_NAMES[q]] = convergence_summary([c[w0:] for c in qtr[q]], rhat_max=cfg.rhat_max,
+                                                         ess_min=cfg.ess_min, mcse_max=cfg.mcse_max)
+    prod_s = t_end - t_prod0
+    tot = summ.get("total")
+    for R in sets:
+        R.dev.close()
+    if probe is not None:
+        probe.close()
+    return dict(names=MONITOR_NAMES, traces=traces, records=records, probes=probes, summary=summ, window_start=w0,
+                kinds=kinds, wall=dict(setup=t_init - t_setup, steps=t_end - t_init, production=prod_s),
+                steps_per_s=(nwarm + nprod) / max(t_end - t_init, 1e-12),
+                ess_total=None if tot is None else tot["ess_bulk"],
+                ess_per_s=None if tot is None or prod_s <= 0 else tot["ess_bulk"] / prod_s,
+                passed=all(s["passed"] for s in summ.values()) if summ else False)

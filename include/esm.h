@@ -233,6 +233,15 @@ int32_t esm_download_weather(esm_handle *h, int32_t *m, double *z);
 int32_t esm_upload_weather(esm_handle *h, const int32_t *m, const double *z);
 
 /*
+ * Copy replica js of `src` into replica jd of `dst` (same N, L, T and precision; the handles may
+ * be the same): the operation fields, the raw weather and, when both hold a weather process,
+ * the anomalies and regimes; `dst` takes the capacities of `src` and its caches are rebuilt.
+ * Replaces `copy(s)` of one replica's State for the lag probe, which sweeps a copy of a replica
+ * at frozen (k, x) (src/replicas.jl:733-750).
+ */
+int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t jd);
+
+/*
  * Kernel timing with HIP events recorded on the handle's stream around every launch of the
  * named kernel (the stage timing of the reference, src/stages.jl:26-53, at kernel grain).
  * esm_profile(h, 1) clears the records and starts recording, esm_profile(h, 0) stops;

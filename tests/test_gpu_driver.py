@@ -1,0 +1,70 @@
+"""The replica driver on the device: the replica copy of the lag probe (esm_copy_replica) and a
+short run of run_replicas with bursts, gates, probes and the convergence report."""
+import numpy as np
+import pytest
+
+from conftest import random_state
+from esmsampling import FP64, api, synth
+from esmsampling import replicas as RP
+
+pytestmark = pytest.mark.gpu
+
+
+def test_copy_replica_matches_source():
+    print("copy replica fp64", flush=True)
+    s = synth.make_system(N=10, L=16, T=48, dt=1.0, seed=8)
+    R = 3
+    v, k, x = random_state(s, R, seed=21, spread=False)
+    src = api.Replicas(s, R, FP64)
+    api.sample_weather(src, key=(4, 4), counter=1)
+    src.upload(v=v, k=k)
+    dst = api.Replicas(s, 1, FP64)
+    api.copy_replica(src, 1, dst, 0)
+    vs, ks, xs = src.download()
+    vd, kd, xd = dst.download()
+    np.testing.assert_array_equal(vd[..., 0], vs[..., 1])
+    np.testing.assert_array_equal(xd[..., 0], xs[..., 1])
+    np.testing.assert_array_equal(kd, ks)
+    ms, zs = api.weather_state(src)
+    md, zd = api.weather_state(dst)
+    np.testing.assert_array_equal(md[:, 0], ms[:, 1])
+    np.testing.assert_array_equal(zd[..., 0], zs[..., 1])
+    ps = api.energy_cost(src)
+    pd = api.energy_cost(dst)
+    np.testing.assert_allclose(pd[0], ps[1], rtol=1e-12)
+    # the copy sweeps on its own: the source is unchanged
+    api.operation_sweep(dst, 1, key=(1, 1), counter=0)
+    np.testing.assert_array_equal(src.download()[0], vs)
+    with pytest.raises(Exception, match="out of range"):
+        api.copy_replica(src, 3, dst, 0)
+    src.close()
+    dst.close()
+
+
+def test_run_replicas_short():
+    print("run_replicas 2 chains x 4 replicas", flush=True)
+    cfg = RP.load_config(dict(nchains=2, replicas=4, mlda_warmup=10, nsamples=14, replica_init_sweeps=2,
+                              replica_burst_every=4, replica_gate_min=1, replica_gate_max=2, replica_gate_min_window=1,
+                              replica_gate_max_window=3, replica_gate_win=2, replica_probe_every=3,
+                              replica_probe_len=4, replica_probe_tail=2, replica_probe_win=1, replica_probe_avg=1,
+                              replica_window=8, synthetic_nodes=10, synthetic_lines=16, snapshots=48,
+                              resolution_hours=1.0, precision="fp64", seed=3))
+    res = RP.run_replicas(cfg)
+    assert len(res["traces"]) == 2 and len(res["traces"][0]) == 24
+    tr = np.array(res["traces"])
+    assert np.all(np.isfinite(tr))
+    # the capacities are shared within a chain and move by the Langevin step
+    k0 = res["records"][0][0]["k"]
+    assert not np.array_equal(k0, res["records"][0][-1]["k"])
+    # capex + opex = total
+    i = RP.MONITOR_NAMES.index
+    np.testing.assert_allclose(tr[..., i("total")], tr[..., i("capex")] + tr[..., i("opex")], rtol=1e-12)
+    assert "total" in res["summary"] and res["ess_total"] > 0 and res["ess_per_s"] > 0
+    assert sum(len(p) for p in res["probes"]) > 0
+    for p in res["probes"][0]:
+        assert np.all(np.isfinite(p["rs_eq"][np.isfinite(p["fk"])]))
+    # bursts reset the gates: some steps have inactive replicas
+    assert any(not r["active"].all() for r in res["records"][0])
+    for rec in res["records"][0]:
+        w = rec["weights"]
+        assert w.sum() == pytest.approx(1.0) and np.all(w[~rec["active"]] == 0) or not rec["active"].any()
