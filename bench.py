@@ -5,11 +5,13 @@
 
 A step is one replica step (``replica_step!``, src/replicas.jl:676-705, ``replica_inner = 1``)
 over all replicas of one chain: every replica's weather step (pCN bursts staggered over the
-replicas, regime windows, compensated re-dispatch), one operation sweep of exact line moves,
-the energy totals and the capacity score, one Langevin step of the shared capacities and the
-dispatch following on every replica. The workload is the 40-node / 70-line synthetic system at
-an hourly horizon (8760 steps) with 64 weather replicas; the state is FP32 with FP64 totals,
-MH-free moves and FP64 Langevin. Inputs are seeded and resident in HBM before the timed region.
+replicas, regime windows, compensated re-dispatch), one operation sweep of exact line moves
+(every atom, then round(4N) random moves per step over the plain families at the reference's
+weights), the energy totals and the capacity score, one Langevin step of the shared capacities
+and the dispatch following on every replica. The workload is the 40-node / 70-line synthetic
+system at an hourly horizon (8760 steps) with 64 weather replicas; the state is FP32 with FP64
+totals, the Jacobian test and the Langevin step in FP64. Inputs are seeded and resident in HBM
+before the timed region.
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -89,19 +91,30 @@ class Workload:
         return parts
 
 
-def kernel_bytes(s, R, es, ntheta_launch):
+def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
     """Algorithmic HBM bytes per launch of each kernel (reads + writes of state and caches that
-    the kernel's algorithm needs, each value once)."""
+    the kernel's algorithm needs, each value once). For the random families: the moves of one
+    launch (from the sweep statistics) times the values one move reads and writes."""
     N, L, T = s.N, s.L, s.T
     half = T // 2
-    return {
+    deg = 2.0 * L / N
+    out = {
         "k_energy": (es * (7 * N + N + L) + 6 * 8) * T * R,        # θ, gas, 2 SoC, 3 CF in; r, F, 6 step parts out
         "k_gen": 4 * es * N * T * R,                                 # y, r in and out
         "k_theta": (8 * L + 4 * N) * es * T * R / max(ntheta_launch, 1),   # per site 4·deg + 4 values
         "k_soc": 8 * es * N * half * R,                              # e(τm, τ, τ+1), r(τm, τ) in; e(τ), r(τm, τ) out
-        "k_spair": 12 * es * N * half * R,                           # + gas at τm and τ in and out
         "k_follow": 9 * es * N * T * R,                              # y, r, 3 CF, 2 SoC in; y, r out
     }
+    # per move: unit y, r; potential θ, deg flows, deg + 1 balances; spair 3 SoC, 2 gas, 2 r (+ writes);
+    # cell y, r of every node of the cell (mean N/3); xloc/xlocc 2 gas, ~2·deg flows and balances; xpair
+    # 2 gas, every flow and balance and angle
+    per_move = {"k_unit": 4, "k_pot": 2 * (2 * deg + 2), "k_spair": 12, "k_cell": 4 * N / 3,
+                "k_xloc": 2 * (2 + 4 * deg), "k_xlocc": 2 * (2 + 6 * deg), "k_xpair": 2 * (2 + L + 2 * N)}
+    fam = {"k_spair": 3, "k_unit": 4, "k_pot": 5, "k_cell": 6, "k_xloc": 7, "k_xlocc": 8, "k_xpair": 9}
+    for k, f in fam.items():
+        launches = max(prof.get(k, (0, 0))[1] / max(steps, 1), 1)
+        out[k] = per_move[k] * es * float(stats[f, 0]) / launches
+    return out
 
 
 def main():
@@ -113,7 +126,7 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
-    from esmsampling import FP32, FP64, _lib
+    from esmsampling import FP32, FP64, _lib, api
 
     prec = FP64 if args.fp64 else FP32
     es = 8 if prec == FP64 else 4
@@ -138,7 +151,9 @@ def main():
     t1 = time.perf_counter()
     names = {_lib.KERN_ENERGY: "k_energy", _lib.KERN_SCORE: "k_score", _lib.KERN_WEATHER: "weather",
              _lib.KERN_GEN: "k_gen", _lib.KERN_THETA: "k_theta", _lib.KERN_SOC: "k_soc", _lib.KERN_SPAIR: "k_spair",
-             _lib.KERN_FOLLOW: "k_follow", _lib.KERN_SWEEP: "sweep"}
+             _lib.KERN_FOLLOW: "k_follow", _lib.KERN_SWEEP: "sweep", _lib.KERN_UNIT: "k_unit",
+             _lib.KERN_POT: "k_pot", _lib.KERN_CELL: "k_cell", _lib.KERN_XLOC: "k_xloc", _lib.KERN_XLOCC: "k_xlocc",
+             _lib.KERN_XPAIR: "k_xpair"}
     prof = {}
     for kid, name in names.items():
         tot, cnt = C.c_double(), C.c_int64()
@@ -148,7 +163,7 @@ def main():
     ms_step = 1e3 * (t1 - t0) / args.steps
     value = args.steps / (t1 - t0)
     ntheta = prof["k_theta"][1] / max(args.steps, 1)
-    kb = kernel_bytes(wl.s, R, es, ntheta)
+    kb = kernel_bytes(wl.s, R, es, ntheta, wl.last_stats, prof, args.steps)
     cand = {k: prof[k][0] for k in kb}
     dom = max(cand, key=cand.get)
     tot_ms, cnt = prof[dom]
@@ -156,7 +171,7 @@ def main():
     achieved = kb[dom] / (kern_ms * 1e-3) / 1e9
     breakdown = {k: round(v[0] / args.steps, 4) for k, v in prof.items()}
     traffic = measured_traffic(dom, wl.s, T, R, prec)
-    line_moves = (wl.s.N * 6) * T * R          # gas, potential, 2 SoC, 2 pairs per node and step
+    line_moves = int(wl.last_stats[:, 0].sum())   # exact line moves of one sweep (atoms and random families)
     out = {
         "metric": "sweeps/s (replica steps/s of the plain GPU replica sampler)",
         "value": value,
@@ -180,7 +195,7 @@ def main():
         "kernel_ms_per_step": breakdown,
         "sweep_stats": {name: {"moves": int(st[0]), "accepted": int(st[1]), "improper": int(st[2]),
                                "cut": int(st[3]), "evals_per_move": round(float(st[4]) / max(int(st[0]), 1), 3)}
-                        for name, st in zip(("gas", "potential", "soc", "spair"), wl.last_stats)},
+                        for name, st in zip(api.FAMILY_NAMES, wl.last_stats)},
         "total_cost_mean": float(np.mean(parts[:, 0] + parts[:, 1])),
     }
     if not args.no_cpu:

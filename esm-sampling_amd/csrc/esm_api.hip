@@ -129,6 +129,52 @@ static void clear_prof(esm_handle *h) {
     }
 }
 
+// Copy the move templates to the device: the integer arrays in one buffer, the values in the
+// state precision in another.
+template <typename S>
+static int upload_move_tables(esm_handle *h) {
+    const MoveTablesHost &mt = h->mt;
+    MoveTablesDev &d = h->dmt;
+    const std::vector<const std::vector<int> *> iv = {&mt.th_off, &mt.th_node, &mt.en_off, &mt.en_node, &mt.en_div,
+                                                      &mt.kk_off, &mt.kk_node, &mt.ln_off, &mt.ln_line, &mt.nd_off,
+                                                      &mt.nd_node, &mt.nd_ent, &mt.nd_kk, &mt.nd_l, &mt.nl_pos,
+                                                      &mt.nl_sgn, &mt.xp_off, &mt.lev_off};
+    std::vector<int> ib;
+    std::vector<size_t> io;
+    for (auto *v : iv) {
+        io.push_back(ib.size());
+        ib.insert(ib.end(), v->begin(), v->end());
+        ib.push_back(0);   // keeps every array non-empty
+    }
+    const std::vector<const std::vector<double> *> vv = {&mt.th_val, &mt.en_c, &mt.kk_g, &mt.ln_df};
+    std::vector<S> vb;
+    std::vector<size_t> vo;
+    for (auto *v : vv) {
+        vo.push_back(vb.size());
+        for (double x : *v) vb.push_back((S)x);
+        vb.push_back(S(0));
+    }
+    ESM_HIP_CHECK(hipMalloc((void **)&d.ibuf, ib.size() * sizeof(int)));
+    ESM_HIP_CHECK(hipMemcpy(d.ibuf, ib.data(), ib.size() * sizeof(int), hipMemcpyHostToDevice));
+    ESM_HIP_CHECK(hipMalloc(&d.vbuf, vb.size() * sizeof(S)));
+    ESM_HIP_CHECK(hipMemcpy(d.vbuf, vb.data(), vb.size() * sizeof(S), hipMemcpyHostToDevice));
+    ESM_HIP_CHECK(hipMalloc((void **)&d.qcum, mt.qcum.size() * sizeof(double)));
+    ESM_HIP_CHECK(hipMemcpy(d.qcum, mt.qcum.data(), mt.qcum.size() * sizeof(double), hipMemcpyHostToDevice));
+    int **dst[] = {&d.th_off, &d.th_node, &d.en_off, &d.en_node, &d.en_div, &d.kk_off, &d.kk_node,
+                               &d.ln_off, &d.ln_line, &d.nd_off, &d.nd_node, &d.nd_ent, &d.nd_kk, &d.nd_l,
+                               &d.nl_pos, &d.nl_sgn, &d.xp_off, &d.lev_off};
+    for (size_t q = 0; q < iv.size(); q++) *dst[q] = d.ibuf + io[q];
+    const S *vbase = (const S *)d.vbuf;
+    d.th_val = vbase + vo[0];
+    d.en_c = vbase + vo[1];
+    d.kk_g = vbase + vo[2];
+    d.ln_df = vbase + vo[3];
+    d.xp0 = mt.xp0;
+    d.cl0 = mt.cl0;
+    d.nlev = mt.nlev;
+    return 0;
+}
+
 static void free_handle(esm_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     clear_prof(h);
@@ -136,7 +182,8 @@ static void free_handle(esm_handle *h) {
                   h->mb.load, h->mb.mc, h->mb.ef, h->mb.kc, h->mb.hours, h->mb.eta_in, h->mb.eta_out, h->mb.qt,
                   h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,
                   h->d_out, h->d_sk, h->d_fk, h->d_z, h->d_ytmp, h->d_carry, h->d_zin, h->d_mu, h->d_xbar, h->d_m,
-                  h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0, h->d_theta_nodes, h->d_stats};
+                  h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0, h->d_theta_nodes, h->d_stats,
+                  h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -181,6 +228,15 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
             return ESM_EINVAL;
         }
     }
+    if (mdl->ncountry < 0 || (mdl->ncountry > 0 && !mdl->country)) {
+        set_error("esm_create: ncountry > 0 needs the country labels");
+        return ESM_EINVAL;
+    }
+    for (int n = 0; mdl->ncountry > 0 && n < N; n++)
+        if (mdl->country[n] < 0 || mdl->country[n] >= mdl->ncountry) {
+            set_error("esm_create: country label of node " + std::to_string(n) + " out of range");
+            return ESM_EINVAL;
+        }
     esm_handle *h = new esm_handle();
     h->N = N;
     h->L = L;
@@ -206,6 +262,8 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     h->eta_out = cp(mdl->stor_eta_out, ESM_NSTOR * N);
     h->qt = cp(mdl->stor_qt, ESM_NSTOR * N);
     h->k.assign(ESM_NK * N, 0.0);
+    h->ncountry = mdl->ncountry;
+    if (mdl->ncountry > 0) h->country.assign(mdl->country, mdl->country + N);
     // CSR adjacency: lines incident to every node, in line order
     h->adj_ptr.assign(N + 1, 0);
     for (int e = 0; e < L; e++) {
@@ -285,6 +343,14 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
         }
         h->theta_nodes_h = flat;
     }
+    {
+        std::string err;
+        if (!build_move_tables(h, h->mt, err)) {
+            set_error("esm_create: " + err);
+            delete h;
+            return ESM_EINVAL;
+        }
+    }
     int rc = 0;
     if (hipGetDevice(&h->device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess) {
         set_error("esm_create: no HIP device or stream");
@@ -357,7 +423,9 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     zmalloc((void **)&h->d_out, (size_t)std::max(nrep * 8, ESM_NK * N * 2 + 64) * sizeof(double));
     zmalloc((void **)&h->d_sk, (size_t)nrep * ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
-    zmalloc((void **)&h->d_stats, 20 * STAT_SLOTS * sizeof(unsigned long long));
+    zmalloc((void **)&h->d_stats, (size_t)ESM_NFAM * ESM_NSTAT * STAT_SLOTS * sizeof(unsigned long long));
+    zmalloc((void **)&h->d_counts, (size_t)ESM_NWEIGHT * T * sizeof(int));
+    if (!rc) rc = precision == ESM_FP64 ? upload_move_tables<double>(h) : upload_move_tables<float>(h);
     // weather process (optional: a model without weather tables has none)
     h->has_weather = mdl->Ptr && mdl->pi0 && mdl->mu && mdl->xbar && mdl->K >= 1 && mdl->K <= ESM_MAXK &&
                      mdl->sigma > 0 && mdl->sigma0 > 0;
@@ -561,16 +629,17 @@ int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32
                    int64_t *stats) {
     if (!h || !key) { set_error("esm_sweeps: null argument"); return ESM_EINVAL; }
     if (nsweeps < 0 || !(maxnats > 0)) { set_error("esm_sweeps: nsweeps must be >= 0 and maxnats > 0"); return ESM_EINVAL; }
-    ESM_HIP_CHECK(hipMemsetAsync(h->d_stats, 0, 20 * STAT_SLOTS * sizeof(unsigned long long), h->stream));
+    constexpr int NST = ESM_NFAM * ESM_NSTAT;
+    ESM_HIP_CHECK(hipMemsetAsync(h->d_stats, 0, (size_t)NST * STAT_SLOTS * sizeof(unsigned long long), h->stream));
     int rc = h->prec == ESM_FP64 ? launch_sweeps<double>(h, nsweeps, key[0], key[1], counter, maxnats)
                                  : launch_sweeps<float>(h, nsweeps, key[0], key[1], counter, maxnats);
     if (rc) return rc;
     if (stats) {
-        std::vector<unsigned long long> st(20 * STAT_SLOTS);
+        std::vector<unsigned long long> st((size_t)NST * STAT_SLOTS);
         ESM_HIP_CHECK(hipMemcpyAsync(st.data(), h->d_stats, st.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                      h->stream));
         ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
-        for (int i = 0; i < 20; i++) {
+        for (int i = 0; i < NST; i++) {
             unsigned long long acc = 0;
             for (int q = 0; q < STAT_SLOTS; q++) acc += st[(size_t)i * STAT_SLOTS + q];
             stats[i] = (int64_t)acc;
@@ -578,6 +647,23 @@ int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32
     }
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     h->scores_valid = false;
+    return ESM_OK;
+}
+
+int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, int32_t atoms) {
+    if (!h) { set_error("esm_sweep_options: null handle"); return ESM_EINVAL; }
+    if (!(extra >= 0) || extra > 1e4) { set_error("esm_sweep_options: extra must be in [0, 1e4]"); return ESM_EINVAL; }
+    static const double wdef[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
+    const double *w = weights ? weights : wdef;
+    double tot = 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) {
+        if (!(w[q] >= 0) || !std::isfinite(w[q])) { set_error("esm_sweep_options: weights must be finite and >= 0"); return ESM_EINVAL; }
+        tot += w[q];
+    }
+    if (extra > 0 && !(tot > 0)) { set_error("esm_sweep_options: all weights are zero"); return ESM_EINVAL; }
+    h->sweep_extra = extra;
+    for (int q = 0; q < ESM_NWEIGHT; q++) h->sweep_w[q] = w[q];
+    h->sweep_atoms = atoms != 0;
     return ESM_OK;
 }
 

@@ -35,6 +35,47 @@ struct ModelBuf {
 
 }  // namespace esm
 
+namespace esm {
+
+// Lane-uniform templates of the table-driven random moves (cell, xloc / xlocc, xpair), built
+// from the network at esm_create. Template q lists, each in a canonical order:
+//   θ entries      th_node/th_val [th_off[q], th_off[q+1])      angle change per unit t
+//   gas entries    en_node/en_c/en_div [en_off..)               slope en_c, divided by the node's
+//                                                               gas capacity when en_div
+//   kinks          kk_node/kk_g [kk_off..)                      leak nodes and their requirement
+//                                                               change g per unit t (xlocc)
+//   lines          ln_line/ln_df [ln_off..)                     flow change per unit t (sorted)
+//   touched nodes  nd_node/nd_ent/nd_kk [nd_off..)              sorted; the node's gas entry and
+//                                                               kink (positions, −1 none); its
+//                                                               lines nl_pos/nl_sgn [nd_l[slot],
+//                                                               nd_l[slot+1]) (+1: the node is bus0)
+// Template ids: xloc (n, p) at xp_off[n] + p (p-th partner of n); xpair (n, m) at xp0 + n·N + m
+// (empty when n = m or in different components); the cells of partition level l at
+// cl0 + lev_off[l] + c.
+struct MoveTablesHost {
+    std::vector<int> th_off, th_node, en_off, en_node, en_div, kk_off, kk_node, ln_off, ln_line, nd_off, nd_node,
+        nd_ent, nd_kk, nd_l, nl_pos, nl_sgn, xp_off, lev_off;
+    std::vector<double> th_val, en_c, kk_g, ln_df, qcum;
+    int xp0 = 0, cl0 = 0, ntpl = 0, nlev = 0;
+};
+
+struct MoveTablesDev {
+    int *ibuf = nullptr;     // all integer arrays, one allocation
+    void *vbuf = nullptr;    // all value arrays in the state precision
+    double *qcum = nullptr;
+    int *th_off = nullptr, *th_node = nullptr, *en_off = nullptr, *en_node = nullptr, *en_div = nullptr,
+              *kk_off = nullptr, *kk_node = nullptr, *ln_off = nullptr, *ln_line = nullptr, *nd_off = nullptr,
+              *nd_node = nullptr, *nd_ent = nullptr, *nd_kk = nullptr, *nd_l = nullptr, *nl_pos = nullptr,
+              *nl_sgn = nullptr, *xp_off = nullptr, *lev_off = nullptr;
+    const void *th_val = nullptr, *en_c = nullptr, *kk_g = nullptr, *ln_df = nullptr;
+    int xp0 = 0, cl0 = 0, nlev = 0;
+};
+
+// Build the templates from the model (host, FP64); false with a message on a bad network.
+bool build_move_tables(const esm_handle *h, MoveTablesHost &mt, std::string &err);
+
+}  // namespace esm
+
 struct esm_handle {
     int N = 0, L = 0, T = 0, R = 0, prec = 64;
     double dt = 1.0;
@@ -48,6 +89,8 @@ struct esm_handle {
     std::vector<int> adj_ptr, adj_line, adj_other;
     std::vector<double> adj_bv;
     std::vector<double> k;   // [ESM_NK][N] raw capacities (shared)
+    int ncountry = 0;
+    std::vector<int> country;   // [N] middle partition level of the cell moves (empty: none)
     // device model
     int *d_bus0 = nullptr, *d_bus1 = nullptr, *d_adj_ptr = nullptr, *d_adj_line = nullptr, *d_adj_other = nullptr;
     esm::ModelBuf mb;
@@ -93,7 +136,14 @@ struct esm_handle {
     std::vector<int> theta_nodes_h;
     int *d_theta_nodes = nullptr;
     std::vector<std::pair<int, int>> tclass;      // time-parity classes: (first step, count), stride 2
-    unsigned long long *d_stats = nullptr;        // [4 families][5 statistics][STAT_SLOTS]
+    unsigned long long *d_stats = nullptr;        // [ESM_NFAM][ESM_NSTAT][STAT_SLOTS]
+    // random families of the sweep (esm_sweep_options) and their move templates
+    bool sweep_atoms = true;
+    double sweep_extra = 4.0;
+    double sweep_w[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
+    int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step
+    esm::MoveTablesHost mt;                       // templates (host, FP64)
+    esm::MoveTablesDev dmt;                       // the same on the device (values in the state precision)
     // kernel timing (esm_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[ESM_NKERN];
@@ -138,6 +188,10 @@ int launch_langevin(esm_handle *h, const double *d_w, const double *d_xi, const 
 // k_sweep.hip
 template <typename S>
 int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t ctr0, double maxnats);
+
+// k_random.hip: the random families of one sweep (counter ctr), class by class
+template <typename S>
+int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
 
 // k_follow.hip
 template <typename S>

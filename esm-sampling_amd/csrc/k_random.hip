@@ -1,0 +1,633 @@
+// k_random.hip — the random moves of an operation sweep (`local_sweep!`, src/local.jl:1927-2062).
+//
+// Per step τ the reference draws round(extra·N) moves over the plain families
+// (plain_families, :1912-1915) and runs them in a shuffled order together with the atoms; steps
+// of one time-parity class run in parallel. Here, per step and sweep, k_movecounts makes the same
+// draws (one uniform each, inverted on the cumulative weights) and counts them per family; one
+// kernel per family and parity class then runs, in every thread (step τ, replica j), that
+// family's moves of step τ in sequence. A move's selection (node, partner, cell) comes from a
+// counter of (family, index, step, sweep) and is the same for every replica, so the 64 lanes of a
+// wave (64 replicas of one step) run one move shape on their own states; the line draw and the
+// Jacobian test use counters of the replica.
+//
+// Families and their directions (the complete level of the synthetic system: one dispatchable
+// unit per node, so `pair` and `cellpair` are void and are drawn but not run):
+//   unit       the gas utilization of a node (`prep_random!` :unit)           site_gen
+//   potential  the angle of a node (:potential)                                site_theta
+//   spair      storage–gas pair at a node (`prep_spair!`)                      site_soc<GEN>
+//   cell       +1 on every gas utilization of a cell of a partition level      FSparse
+//   xloc       gas +1/κ_n at n, −1/κ_m at a neighbour m, the flow re-routed on the ball {n, m},
+//              the leak into the residuals of the first layer (`prep_xloc!`)   FSparse
+//   xlocc      xloc with the leak compensated pro rata by the gas unit of each leak node: a
+//              kinked entry (σ/κ)·φ(t), φ(t) = (Q + g t)₊ − Q₊, and the Jacobian test
+//              (`prorata!` :1782-1815, `logjac` :1874)                         FSparse
+//   xpair      gas ±1/κ at any two nodes of one component, the flow re-routed globally,
+//              Δθ = L⁺(e_n − e_m) (`prep_reroute!`)                            FSparse
+// FSparse evaluates ΔC(t) of a table-driven move on the fly from the caches of (τ, j) (the terms
+// at t = 0 are the cache values; nothing is written before the commit), in the arithmetic of
+// `eval_cached!` (:926-981): gas entries, kinked entries, line overloads, node balances, each
+// group summed in its listed order. The walk visits the primary breakpoints (clamp points of
+// the entries, kinks, line limits) and the sign changes of the node balances between them in
+// walking order, as `walk!` does; equal roots of several nodes are visited once.
+#include <algorithm>
+
+#include "sweep_core.h"
+
+namespace esm {
+
+// positions of the plain families in the weight list
+enum { W_UNIT = 0, W_PAIR = 1, W_XPAIR = 2, W_XLOCC = 3, W_XLOC = 4, W_POT = 5, W_CELLPAIR = 6, W_CELL = 7, W_SPAIR = 8 };
+
+struct CumW {
+    double c[ESM_NWEIGHT];
+};
+
+struct RandArgs {
+    uint32_t k0, k1, ctr;
+    int tau0, tstride, ntau;
+    int w;                       // the family's position in the weight list
+    const int *counts;           // [ESM_NWEIGHT][T]
+    unsigned long long *stats;   // [ESM_NSTAT][STAT_SLOTS] of the family
+};
+
+__device__ __forceinline__ uint32_t pick(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
+
+// the selection words of move `k` of family `w` at step τ
+__device__ __forceinline__ void selection(const RandArgs &a, int k, int tau, uint32_t *s) {
+    philox4x32(a.k0, a.k1, 0x80000000u | ((uint32_t)a.w << 20) | (uint32_t)k, (uint32_t)tau, 9001u, a.ctr, s);
+}
+
+__device__ __forceinline__ void line_uniforms(const RandArgs &a, int j, int k, int tau, double &u1, double &u2) {
+    uniforms2(a.k0, a.k1, j, (uint32_t)tau, 0x40000000u | ((uint32_t)a.w << 20) | (uint32_t)k, a.ctr, u1, u2);
+}
+
+// per step: round(extra·N) family draws, counted per family
+__global__ void k_movecounts(int T, int nextra, CumW cw, uint32_t k0, uint32_t k1, uint32_t ctr, int *counts) {
+    const int tau = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tau >= T) return;
+    int cnt[ESM_NWEIGHT];
+#pragma unroll
+    for (int q = 0; q < ESM_NWEIGHT; q++) cnt[q] = 0;
+    for (int d = 0; d < nextra; d++) {
+        uint32_t w[4];
+        philox4x32(k0, k1, 0x80000000u | (uint32_t)d, (uint32_t)tau, 9000u, ctr, w);
+        const double u = u01(w[0], w[1]);
+        int f = ESM_NWEIGHT - 1;
+#pragma unroll
+        for (int q = ESM_NWEIGHT - 1; q >= 0; q--)
+            if (cw.c[q] >= u) f = q;
+#pragma unroll
+        for (int q = 0; q < ESM_NWEIGHT; q++) cnt[q] += q == f;
+    }
+#pragma unroll
+    for (int q = 0; q < ESM_NWEIGHT; q++) counts[(size_t)q * T + tau] = cnt[q];
+}
+
+// ---------------------------------------------------------------------------------------------
+// unit, potential, spair: the atom sites at a drawn node
+// ---------------------------------------------------------------------------------------------
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_rand_unit(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    MoveOut out;
+    if (g < (size_t)a.ntau * m.R) {
+        const int j = (int)(g % m.R);
+        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
+        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        for (int k = 0; k < cnt; k++) {
+            uint32_t s[4];
+            selection(a, k, tau, s);
+            const int n = (int)pick(s[0], (uint32_t)m.N);
+            double u1, u2;
+            line_uniforms(a, j, k, tau, u1, u2);
+            site_gen(m, st, n, tau, j, u1, u2, beta, maxnats, out);
+        }
+    }
+    add_counts(a.stats, out);
+}
+
+// potential moves whose node degree falls in the bucket (DLO, D]: one kernel per bucket keeps
+// the register footprint of its degree; a step's potential moves run bucket by bucket, each
+// bucket in draw order
+template <typename S, int DLO, int D>
+__global__ void __launch_bounds__(256) k_rand_pot(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    MoveOut out;
+    if (g < (size_t)a.ntau * m.R) {
+        const int j = (int)(g % m.R);
+        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
+        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        for (int k = 0; k < cnt; k++) {
+            uint32_t s[4];
+            selection(a, k, tau, s);
+            const int n = (int)pick(s[0], (uint32_t)m.N);
+            const int deg = m.adj_ptr[n + 1] - m.adj_ptr[n];
+            if (deg <= DLO || deg > D) continue;   // another bucket (degree 0: no angle direction)
+            double u1, u2;
+            line_uniforms(a, j, k, tau, u1, u2);
+            site_theta<S, D>(m, st, n, tau, j, u1, u2, beta, maxnats, out);
+        }
+    }
+    add_counts(a.stats, out);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_rand_spair(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats,
+                                                    S kmin) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    MoveOut out;
+    if (g < (size_t)a.ntau * m.R) {
+        const int j = (int)(g % m.R);
+        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
+        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        for (int k = 0; k < cnt; k++) {
+            uint32_t s[4];
+            selection(a, k, tau, s);
+            const int n = (int)pick(s[0], (uint32_t)m.N);
+            const int sc = (int)pick(s[1], (uint32_t)ESM_NSTOR);   // the node's storage units: battery, H2
+            double u1, u2;
+            line_uniforms(a, j, k, tau, u1, u2);
+            site_soc<S, true>(m, st, sc, n, tau, j, u1, u2, beta, maxnats, kmin, out);
+        }
+    }
+    add_counts(a.stats, out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// table-driven moves (cell, xloc, xlocc, xpair)
+// ---------------------------------------------------------------------------------------------
+
+template <typename S>
+struct Tables {
+    const int *th_off, *th_node, *en_off, *en_node, *en_div, *kk_off, *kk_node, *ln_off, *ln_line, *nd_off, *nd_node,
+        *nd_ent, *nd_kk, *nd_l, *nl_pos, *nl_sgn, *xp_off, *lev_off;
+    const S *th_val, *en_c, *kk_g, *ln_df;
+    const double *qcum;
+    int xp0, cl0, nlev;
+};
+
+template <typename S>
+struct FSparse {
+    const DevModel<S> *m;
+    const Tables<S> *tb;
+    const S *yg, *r, *F;    // gas utilization, residual and flow planes
+    size_t o, TR;           // (τ, j) offset inside a [·][T][R] plane, plane stride
+    int e0, e1, q0, q1, l0, l1, n0, n1;
+    bool comp;              // the kinks of the template are active (xlocc)
+    S kmin;
+    mutable int nev = 0;
+
+    __device__ S kap(int n) const { return m->kc[ESM_KGAS * m->N + n]; }
+    // slope of gas entry e (entries below 1e-14 are zero, as `finalize!` makes them)
+    __device__ S slope(int e) const {
+        const S c = tb->en_c[e];
+        const S a = tb->en_div[e] ? c / kap(tb->en_node[e]) : c;
+        return fabs(a) < S(1e-14) ? S(0) : a;
+    }
+    __device__ S ygv(int n) const { return yg[(size_t)n * TR + o]; }
+    __device__ S rv(int n) const { return r[(size_t)n * TR + o]; }
+    __device__ S Fv(int e) const { return F[(size_t)e * TR + o]; }
+    __device__ void gen(int n, S y, S &c, S &p) const {
+        const S k = kap(n);
+        gen_cost(m->mc[ESM_KGAS * m->N + n], k, k + m->kappa0, m->dt, m->cv, y, c, p);
+    }
+    // kink q (`prorata!` of one unit): false when it is not made; Q, e, P at t = 0, ak = σ/κ
+    __device__ bool kink(int q, S &Q, S &g, S &ak, S &P, S &e, S &y0) const {
+        const int n = tb->kk_node[q];
+        const S k = kap(n);
+        if (!(fabs(k) >= kmin)) return false;
+        const S w = ygv(n);
+        P = S(0) + k * w;
+        const S Qs = S(0) + k * clampv(w, S(0), S(1));
+        if (fabs(P) < S(1e-6)) return false;
+        Q = rv(n) + Qs;
+        e = P - fmax(Q, S(0));
+        g = tb->kk_g[q];
+        const S sig = k * w / P;
+        ak = sig == S(0) ? S(0) : sig / k;
+        y0 = w;
+        return g != S(0);
+    }
+    __device__ static S kphi(S Q, S g, S t) { return fmax(Q + g * t, S(0)) - fmax(Q, S(0)); }
+
+    // balance of touched node p at t
+    __device__ S node_S(int p, S t) const {
+        const int n = tb->nd_node[p];
+        S dr = 0;
+        const int ie = tb->nd_ent[p];
+        if (ie >= 0) {
+            const S a = slope(e0 + ie);
+            if (a != S(0)) {
+                const S y0 = ygv(n);
+                S c0, p0, c1, p1;
+                gen(n, y0, c0, p0);
+                gen(n, y0 + t * a, c1, p1);
+                dr -= p1 - p0;
+            }
+        }
+        const int ik = tb->nd_kk[p];
+        if (comp && ik >= 0) {
+            S Q, g, ak, P, e, y0;
+            if (kink(q0 + ik, Q, g, ak, P, e, y0) && ak != S(0)) {
+                S c0, p0, c1, p1;
+                gen(n, y0, c0, p0);
+                gen(n, y0 + ak * kphi(Q, g, t), c1, p1);
+                dr -= p1 - p0;
+            }
+        }
+        for (int q = tb->nd_l[p]; q < tb->nd_l[p + 1]; q++) {
+            const int pos = l0 + tb->nl_pos[q];
+            const S f0 = Fv(tb->ln_line[pos]);
+            const S d = (f0 + t * tb->ln_df[pos]) - f0;
+            dr += tb->nl_sgn[q] > 0 ? d : -d;
+        }
+        return rv(n) + dr;
+    }
+
+    // ΔC(t)
+    __device__ S eval(S t) const {
+        nev++;
+        S D = 0;
+        for (int e = e0; e < e1; e++) {
+            const S a = slope(e);
+            if (a == S(0)) continue;
+            const int n = tb->en_node[e];
+            const S y0 = ygv(n);
+            S c0, p0, c1, p1;
+            gen(n, y0, c0, p0);
+            gen(n, y0 + t * a, c1, p1);
+            D += c1 - c0;
+        }
+        if (comp)
+            for (int q = q0; q < q1; q++) {
+                S Q, g, ak, P, e, y0;
+                if (!kink(q, Q, g, ak, P, e, y0) || ak == S(0)) continue;
+                const int n = tb->kk_node[q];
+                S c0, p0, c1, p1;
+                gen(n, y0, c0, p0);
+                gen(n, y0 + ak * kphi(Q, g, t), c1, p1);
+                D += c1 - c0;
+            }
+        for (int l = l0; l < l1; l++) {
+            const int e = tb->ln_line[l];
+            const S f0 = Fv(e), se = m->ls[e];
+            const S f1 = f0 + t * tb->ln_df[l];
+            D += m->cv * m->dt * (fmax(fabs(f1) - se, S(0)) - fmax(fabs(f0) - se, S(0)));
+        }
+        for (int p = n0; p < n1; p++) {
+            const S S0 = rv(tb->nd_node[p]);
+            const S S1 = node_S(p, t);
+            D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+        }
+        return D;
+    }
+
+    // the nearest primary breakpoint beyond tp in walking direction sgn
+    __device__ bool next_bp(S tp, S sgn, S &t) const {
+        bool have = false;
+        auto cand = [&](S b) {
+            if (isfinite(b) && sgn * (b - tp) > 0 && (!have || sgn * (t - b) > 0)) {
+                t = b;
+                have = true;
+            }
+        };
+        for (int e = e0; e < e1; e++) {
+            const S a = slope(e);
+            if (a == S(0)) continue;
+            const S y0 = ygv(tb->en_node[e]);
+            cand((S(0) - y0) / a);
+            cand((S(1) - y0) / a);
+        }
+        if (comp)
+            for (int q = q0; q < q1; q++) {
+                S Q, g, ak, P, e, y0;
+                if (!kink(q, Q, g, ak, P, e, y0)) continue;
+                cand(-Q / g);
+                if (ak != S(0)) {
+                    const S sl = ak * g, off = ak * Q;
+                    cand((S(0) - y0 - S(0)) / sl);
+                    cand((S(1) - y0 - S(0)) / sl);
+                    cand((S(0) - y0 - off) / sl);
+                    cand((S(1) - y0 - off) / sl);
+                }
+            }
+        for (int l = l0; l < l1; l++) {
+            const int e = tb->ln_line[l];
+            const S f0 = Fv(e), se = m->ls[e], df = tb->ln_df[l];
+            cand((se - f0) / df);
+            cand((-se - f0) / df);
+        }
+        return have;
+    }
+
+    // the next sign change of a node balance strictly between x and t (walking order) on the
+    // segment (tp, t), where the balances are linear
+    __device__ bool next_root(S tp, S t, S x, S sgn, S &xr) const {
+        bool have = false;
+        for (int p = n0; p < n1; p++) {
+            const S A = node_S(p, tp), B = node_S(p, t);
+            if ((A < 0) != (B < 0) && A != B) {
+                const S rt = tp + (t - tp) * A / (A - B);
+                if (sgn * (rt - tp) > 0 && sgn * (t - rt) > 0 && sgn * (rt - x) > 0 && (!have || sgn * (xr - rt) > 0)) {
+                    xr = rt;
+                    have = true;
+                }
+            }
+        }
+        return have;
+    }
+
+    // log Jacobian Σ (|S| − 1) log|P(t)/P| of the compensated kinks (FP64); one unit per node
+    __device__ double logjac(S t, bool &any) const {
+        double lj = 0.0;
+        any = false;
+        for (int q = q0; q < q1; q++) {
+            S Q, g, ak, P, e, y0;
+            if (!kink(q, Q, g, ak, P, e, y0)) continue;
+            any = true;
+            const double nm = 0.0;   // units of the node with |κ| ≥ κmin, minus one
+            lj += nm * log(fabs((fmax((double)Q + (double)g * (double)t, 0.0) + (double)e) / (double)P));
+        }
+        return lj;
+    }
+};
+
+// walk! of a table-driven move: 1 proper, 0 improper, -1 point cap, 2 stopped by the visitor
+template <typename S, class V>
+__device__ int walk_sparse(const FSparse<S> &f, S beta, S maxnats, S sgn, V &vis) {
+    S cmin = 0, tp = 0, delta = 0, clast = 0;
+    int nx = 0, np = 0;
+    while (true) {
+        S t = 0;
+        if (!f.next_bp(tp, sgn, t)) {
+            const S hp = S(1e-6) * fmax(S(1), fabs(tp));
+            const S sl = (beta * f.eval(tp + sgn * hp) - clast) / hp;
+            if (sl > 0) delta = fmin(2 * maxnats / sl, S(1e3) * fmax(fabs(tp), S(1e-3)));
+            else delta = delta == 0 ? fmax(fabs(tp), S(1e-6)) : 2 * delta;
+            t = tp + sgn * delta;
+            if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;
+        }
+        const S c = beta * f.eval(t);
+        S x = tp, xr = 0;
+        int w;
+        while (f.next_root(tp, t, x, sgn, xr)) {
+            if ((w = visit_point(vis, xr, beta * f.eval(xr), cmin, np))) return w;
+            x = xr;
+        }
+        if ((w = visit_point(vis, t, c, cmin, np))) return w;
+        clast = c;
+        if (c - cmin > maxnats) return 1;
+        tp = t;
+    }
+}
+
+template <typename S>
+__device__ S sample_sparse(const FSparse<S> &f, S beta, S maxnats, S useg, S uoff, int &status) {
+    MassAcc<S> mr, ml;
+    int w = walk_sparse(f, beta, maxnats, S(1), mr);
+    if (w != 1) {
+        status = w == 0 ? 1 : 2;
+        return 0;
+    }
+    w = walk_sparse(f, beta, maxnats, S(-1), ml);
+    if (w != 1) {
+        status = w == 0 ? 1 : 2;
+        return 0;
+    }
+    const S MX = fmax(ml.mx, mr.mx);
+    const S Ml = ml.s * exp(ml.mx - MX), Mr = mr.s * exp(mr.mx - MX);
+    const S u = useg * (Ml + Mr);
+    SegSel<S> sel;
+    sel.MX = MX;
+    sel.left = u <= Ml;
+    sel.target = sel.left ? Ml - u : u - Ml;
+    walk_sparse(f, beta, maxnats, sel.left ? S(-1) : S(1), sel);
+    status = 0;
+    const S L = sel.tb - sel.ta;
+    const S g = (sel.cb - sel.ca) / L;
+    const S t = (g >= 0 || fabs(g * L) < S(1e-10)) ? sel.ta + trexp_offset(g, L, uoff) : sel.tb - trexp_offset(-g, L, uoff);
+    if (!isfinite(t)) {
+        status = 2;
+        return 0;
+    }
+    return t;
+}
+
+enum { SP_CELL = 0, SP_XLOC = 1, SP_XLOCC = 2, SP_XPAIR = 3 };
+
+template <typename S, int KIND>
+__global__ void __launch_bounds__(256) k_sparse(DevModel<S> m, SweepState<S> st, RandArgs a, Tables<S> tb, S beta,
+                                                S maxnats, S kmin) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    MoveOut out;
+    if (g < (size_t)a.ntau * m.R) {
+        const int j = (int)(g % m.R);
+        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
+        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        const size_t TR = (size_t)m.T * m.R, o = (size_t)tau * m.R + j;
+        S *yg = st.v + plane(m.N, m.T, m.R);
+        for (int k = 0; k < cnt; k++) {
+            uint32_t s[4];
+            selection(a, k, tau, s);
+            int tid;
+            if (KIND == SP_CELL) {
+                const double u = u01(s[0], s[1]);
+                int lv = tb.nlev - 1;
+                for (int q = tb.nlev - 1; q >= 0; q--)
+                    if (tb.qcum[q] >= u) lv = q;
+                const int nc = tb.lev_off[lv + 1] - tb.lev_off[lv];
+                tid = tb.cl0 + tb.lev_off[lv] + (int)pick(s[2], (uint32_t)nc);
+            } else if (KIND == SP_XPAIR) {
+                const int n = (int)pick(s[0], (uint32_t)m.N), mm = (int)pick(s[1], (uint32_t)m.N);
+                if (mm == n) continue;
+                tid = tb.xp0 + n * m.N + mm;
+                if (tb.en_off[tid + 1] == tb.en_off[tid]) continue;   // another component
+            } else {
+                const int n = (int)pick(s[0], (uint32_t)m.N);
+                const int np = tb.xp_off[n + 1] - tb.xp_off[n];
+                if (np == 0) continue;
+                tid = tb.xp_off[n] + (int)pick(s[1], (uint32_t)np);
+            }
+            FSparse<S> f;
+            f.m = &m;
+            f.tb = &tb;
+            f.yg = yg;
+            f.r = st.r;
+            f.F = st.F;
+            f.o = o;
+            f.TR = TR;
+            f.e0 = tb.en_off[tid];
+            f.e1 = tb.en_off[tid + 1];
+            f.q0 = tb.kk_off[tid];
+            f.q1 = tb.kk_off[tid + 1];
+            f.l0 = tb.ln_off[tid];
+            f.l1 = tb.ln_off[tid + 1];
+            f.n0 = tb.nd_off[tid];
+            f.n1 = tb.nd_off[tid + 1];
+            f.comp = KIND == SP_XLOCC;
+            f.kmin = kmin;
+            if (KIND != SP_CELL) {   // the pair's units need |κ| ≥ κmin
+                bool ok = true;
+                for (int e = f.e0; e < f.e1; e++) ok = ok && fabs(f.kap(tb.en_node[e])) >= kmin;
+                if (!ok) continue;
+            }
+            double u1, u2;
+            line_uniforms(a, j, k, tau, u1, u2);
+            int status = 0;
+            S t = sample_sparse(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+            if (KIND == SP_XLOCC && t != 0) {
+                bool any;
+                const double lj = f.logjac(t, any);
+                if (any) {
+                    uint32_t w[4];
+                    philox4x32(a.k0, a.k1, (uint32_t)j, (uint32_t)tau, 0x60000000u | ((uint32_t)a.w << 20) | (uint32_t)k,
+                               a.ctr, w);
+                    if (!(u01(w[0], w[1]) < exp(fmin(0.0, lj)))) {
+                        t = 0;
+                        status = 2;
+                    }
+                }
+            }
+            out.add(status, t != 0, f.nev);
+            if (t != 0) {
+                // node by node: the balance and the node's entry read only that node's values (and
+                // the flows, written after all nodes)
+                for (int p = f.n0; p < f.n1; p++) {
+                    const int n = tb.nd_node[p];
+                    const S S1 = f.node_S(p, t);
+                    S ynew = 0;
+                    bool wy = false;
+                    const int ie = tb.nd_ent[p];
+                    if (ie >= 0) {
+                        const S av = f.slope(f.e0 + ie);
+                        if (av != S(0)) {
+                            ynew = f.ygv(n) + t * av;
+                            wy = true;
+                        }
+                    }
+                    const int ik = tb.nd_kk[p];
+                    if (f.comp && ik >= 0) {
+                        S Q, gk, ak, P, e, y0;
+                        if (f.kink(f.q0 + ik, Q, gk, ak, P, e, y0) && ak != S(0)) {
+                            ynew = y0 + ak * FSparse<S>::kphi(Q, gk, t);
+                            wy = true;
+                        }
+                    }
+                    st.r[(size_t)n * TR + o] = S1;
+                    if (wy) yg[(size_t)n * TR + o] = ynew;
+                }
+                for (int l = f.l0; l < f.l1; l++) {
+                    S &F1 = st.F[(size_t)tb.ln_line[l] * TR + o];
+                    F1 = F1 + t * tb.ln_df[l];
+                }
+                for (int q = tb.th_off[tid]; q < tb.th_off[tid + 1]; q++) {
+                    S &th = st.v[(size_t)tb.th_node[q] * TR + o];
+                    th = th + t * tb.th_val[q];
+                }
+            }
+        }
+    }
+    add_counts(a.stats, out);
+}
+
+static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename S>
+int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats) {
+    const int N = h->N, T = h->T, R = h->R;
+    const int nextra = (int)nearbyint(h->sweep_extra * N);
+    if (nextra <= 0) return 0;
+    CumW cw;
+    double acc = 0.0, tot = 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) tot += h->sweep_w[q];
+    for (int q = 0; q < ESM_NWEIGHT; q++) {
+        acc += h->sweep_w[q];
+        cw.c[q] = acc / tot;
+    }
+    hipLaunchKernelGGL(k_movecounts, dim3(nblk((size_t)T, 256)), dim3(256), 0, h->stream, T, nextra, cw, k0, k1, ctr,
+                       h->d_counts);
+    ESM_HIP_CHECK(hipGetLastError());
+    const DevModel<S> m = dev_model<S>(h);
+    SweepState<S> st{(S *)h->d_v, (S *)h->d_r, (S *)h->d_F, (const S *)h->d_x};
+    const S beta = (S)h->m.beta_v, mxn = (S)maxnats, kmin = S(1);
+    const MoveTablesDev &d = h->dmt;
+    Tables<S> tb{d.th_off, d.th_node, d.en_off, d.en_node, d.en_div, d.kk_off, d.kk_node, d.ln_off, d.ln_line,
+                 d.nd_off, d.nd_node, d.nd_ent, d.nd_kk, d.nd_l, d.nl_pos, d.nl_sgn, d.xp_off, d.lev_off,
+                 (const S *)d.th_val, (const S *)d.en_c, (const S *)d.kk_g, (const S *)d.ln_df, d.qcum,
+                 d.xp0, d.cl0, d.nlev};
+    // the families in the order of the weight list; pair and cellpair are void here
+    const int order[7] = {W_UNIT, W_XPAIR, W_XLOCC, W_XLOC, W_POT, W_CELL, W_SPAIR};
+    for (size_t c = 0; c < h->tclass.size(); c++) {
+        RandArgs a{k0, k1, ctr, h->tclass[c].first, 2, h->tclass[c].second, 0, h->d_counts, nullptr};
+        const unsigned nb = nblk((size_t)a.ntau * R, 256);
+        for (int w : order) {
+            if (!(h->sweep_w[w] > 0)) continue;
+            a.w = w;
+            auto stats = [&](int fam) { return h->d_stats + (size_t)fam * ESM_NSTAT * STAT_SLOTS; };
+            switch (w) {
+                case W_UNIT: {
+                    a.stats = stats(ESM_FAM_UNIT);
+                    ProfScope p(h, ESM_KERN_UNIT);
+                    hipLaunchKernelGGL((k_rand_unit<S>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                    break;
+                }
+                case W_POT: {
+                    a.stats = stats(ESM_FAM_POT);
+                    ProfScope p(h, ESM_KERN_POT);
+                    int dmax = 0;
+                    for (int n = 0; n < N; n++) dmax = std::max(dmax, h->adj_ptr[n + 1] - h->adj_ptr[n]);
+                    hipLaunchKernelGGL((k_rand_pot<S, 0, 4>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                    if (dmax > 4)
+                        hipLaunchKernelGGL((k_rand_pot<S, 4, 8>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                    if (dmax > 8)
+                        hipLaunchKernelGGL((k_rand_pot<S, 8, 16>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                    break;
+                }
+                case W_SPAIR: {
+                    a.stats = stats(ESM_FAM_SPAIR);
+                    ProfScope p(h, ESM_KERN_SPAIR);
+                    hipLaunchKernelGGL((k_rand_spair<S>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn, kmin);
+                    break;
+                }
+                case W_CELL: {
+                    a.stats = stats(ESM_FAM_CELL);
+                    ProfScope p(h, ESM_KERN_CELL);
+                    hipLaunchKernelGGL((k_sparse<S, SP_CELL>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
+                                       mxn, kmin);
+                    break;
+                }
+                case W_XLOC: {
+                    a.stats = stats(ESM_FAM_XLOC);
+                    ProfScope p(h, ESM_KERN_XLOC);
+                    hipLaunchKernelGGL((k_sparse<S, SP_XLOC>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
+                                       mxn, kmin);
+                    break;
+                }
+                case W_XLOCC: {
+                    a.stats = stats(ESM_FAM_XLOCC);
+                    ProfScope p(h, ESM_KERN_XLOCC);
+                    hipLaunchKernelGGL((k_sparse<S, SP_XLOCC>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
+                                       mxn, kmin);
+                    break;
+                }
+                case W_XPAIR: {
+                    a.stats = stats(ESM_FAM_XPAIR);
+                    ProfScope p(h, ESM_KERN_XPAIR);
+                    hipLaunchKernelGGL((k_sparse<S, SP_XPAIR>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
+                                       mxn, kmin);
+                    break;
+                }
+                default: break;
+            }
+            ESM_HIP_CHECK(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+template int launch_random<float>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
+template int launch_random<double>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
+
+}  // namespace esm

@@ -31,7 +31,7 @@ class EsmModel(C.Structure):
         ("c_shed", C.c_double), ("c_dump", C.c_double), ("c_viol", C.c_double), ("gamma_k", C.c_double),
         ("kappa0", C.c_double), ("beta_v", C.c_double), ("beta_k", C.c_double),
         ("K", C.c_int32), ("Ptr", _dp), ("pi0", _dp), ("phi", C.c_double), ("sigma", C.c_double),
-        ("sigma0", C.c_double), ("mu", _dp), ("xbar", _dp),
+        ("sigma0", C.c_double), ("mu", _dp), ("xbar", _dp), ("ncountry", C.c_int32), ("country", _ip),
     ]
 
 
@@ -56,6 +56,8 @@ def make_cmodel(sys):
         "capex": f64(sys.capex), "klo": f64(sys.klo), "khi": f64(sys.khi), "hours": f64(sys.stor_hours),
         "ein": f64(sys.stor_eta_in), "eout": f64(sys.stor_eta_out), "qt": f64(sys.stor_qt),
         "Ptr": f64(sys.Ptr), "pi0": f64(sys.pi0), "mu": f64(sys.mu), "xbar": f64(sys.xbar),
+        "country": np.ascontiguousarray(sys.country if getattr(sys, "country", None) is not None
+                                        else np.zeros(sys.N), dtype=np.int32),
     }
     m = EsmModel(
         N=sys.N, L=sys.L, T=sys.T, dt=sys.dt,
@@ -66,6 +68,7 @@ def make_cmodel(sys):
         c_shed=sys.c_shed, c_dump=sys.c_dump, c_viol=sys.c_viol, gamma_k=sys.gamma_k, kappa0=sys.kappa0,
         beta_v=sys.beta_v, beta_k=sys.beta_k, K=sys.K, Ptr=dptr(keep["Ptr"]), pi0=dptr(keep["pi0"]),
         phi=sys.phi, sigma=sys.sigma, sigma0=sys.sigma0, mu=dptr(keep["mu"]), xbar=dptr(keep["xbar"]),
+        ncountry=int(getattr(sys, "ncountry", 0) or 0), country=iptr(keep["country"]),
     )
     return m, keep
 
@@ -93,10 +96,16 @@ SIGNATURES = [
     ("esm_download_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
     ("esm_upload_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
     ("esm_copy_replica", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
+    ("esm_sweep_options", C.c_int32, [C.c_void_p, C.c_double, _dp, C.c_int32]),
     ("esm_profile", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_profile_read", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
 ]
-KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW = range(9)
+(KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW,
+ KERN_UNIT, KERN_POT, KERN_CELL, KERN_XLOC, KERN_XLOCC, KERN_XPAIR) = range(15)
+NFAM, NSTAT, NWEIGHT = 10, 5, 9
+# the plain random families and their default weights (plain_families, src/local.jl:1912-1915)
+WEIGHT_NAMES = ("unit", "pair", "xpair", "xlocc", "xloc", "potential", "cellpair", "cell", "spair")
+DEFAULT_WEIGHTS = (0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3)
 
 _lib = None
 

@@ -136,17 +136,29 @@ def _key(key):
     return k
 
 
-FAMILY_NAMES = ("gen", "theta", "soc", "spair")
+FAMILY_NAMES = ("gen", "theta", "soc", "spair", "unit", "potential", "cell", "xloc", "xlocc", "xpair")
+
+
+def sweep_options(R: Replicas, extra: float = 4.0, weights=None, atoms: bool = True):
+    """The `local_sweep!` options (src/local.jl:1927-1929): ``extra`` random moves per node and step,
+    the weights of the plain families (``_lib.WEIGHT_NAMES`` order; None: the reference's) and
+    whether the atoms run."""
+    w = None if weights is None else f64(weights)
+    if w is not None and w.shape != (_lib.NWEIGHT,):
+        raise ValueError(f"weights must have {_lib.NWEIGHT} entries")
+    check(R.lib.esm_sweep_options(R.handle, float(extra), dptr(w) if w is not None else None, int(bool(atoms))),
+          "esm_sweep_options")
 
 
 def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0, maxnats: float = 40.0):
     """``operation_sweep!`` / ``local_sweep!`` (src/local.jl:1927-2062, :2401-2425) of every replica,
-    ``nsweeps`` times at fixed capacities and weather. Returns the move statistics [4 families, 5]
-    (attempted, accepted, improper, point-cap or non-finite, ΔC evaluations)."""
-    st = np.zeros(20, dtype=np.int64)
+    ``nsweeps`` times at fixed capacities and weather: the atoms, then the random families.
+    Returns the move statistics [10 families (``FAMILY_NAMES``), 5] (attempted, accepted,
+    improper, point-cap / non-finite / Jacobian-rejected, ΔC evaluations)."""
+    st = np.zeros(_lib.NFAM * _lib.NSTAT, dtype=np.int64)
     check(R.lib.esm_sweeps(R.handle, int(nsweeps), _key(key), int(counter) & 0xFFFFFFFF, float(maxnats),
                            st.ctypes.data_as(C.POINTER(C.c_int64))), "esm_sweeps")
-    return st.reshape(4, 5)
+    return st.reshape(_lib.NFAM, _lib.NSTAT)
 
 
 def follow_capacities(R: Replicas, knew, kmin: float = 1.0):

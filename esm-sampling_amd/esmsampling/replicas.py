@@ -268,7 +268,7 @@ class Keys:
 def _sweep_scores(R: ReplicaSet, nsweeps: int, key, counter: int, maxnats: float, r: float):
     """``nsweeps`` operation sweeps, after each the energy (caches rebuilt) and the capacity
     scores; returns (mean of r·ŝ over the sweeps [R, 6, N], fk, parts of the last sweep, stats)."""
-    acc, fk, parts, stats = None, None, None, np.zeros((4, 5), dtype=np.int64)
+    acc, fk, parts, stats = None, None, None, np.zeros((len(api.FAMILY_NAMES), 5), dtype=np.int64)
     for q in range(nsweeps):
         stats += api.operation_sweep(R.dev, 1, key=key, counter=counter + q, maxnats=maxnats)
         parts = api.energy_cost(R.dev)

@@ -71,6 +71,8 @@ class System:
     xbar: np.ndarray = None    # [NREN*N, 24]
     k0: np.ndarray = None      # [NK, N] heuristic start capacities
     pos: np.ndarray = None     # [N, 2] node positions (graph construction only)
+    ncountry: int = 0          # middle partition level of the cell moves (0: none)
+    country: np.ndarray = None # [N] int32 in 0..ncountry-1
 
     def nodelines(self):
         """Lines incident to every node (``LocalModel.nodelines``, src/local.jl:242-245)."""
@@ -116,6 +118,23 @@ def _graph(rng, N, L):
     bus0 = np.array([e[0] for e in edges], dtype=np.int32)
     bus1 = np.array([e[1] for e in edges], dtype=np.int32)
     return pos, bus0, bus1
+
+
+def _countries(pos):
+    """The middle level of the partition hierarchy (the reference's countries, E.parts,
+    src/energy.jl:292): max(2, N // 5) angular sectors of the node positions around their centroid,
+    of near-equal size, labelled in order of first appearance in node order. Uses no random
+    numbers, so it leaves the rest of the system unchanged."""
+    N = len(pos)
+    C = max(2, N // 5)
+    ang = np.arctan2(pos[:, 1] - pos[:, 1].mean(), pos[:, 0] - pos[:, 0].mean())
+    order = np.argsort(ang, kind="stable")
+    sector = np.empty(N, dtype=np.int64)
+    sector[order] = (np.arange(N) * C) // N
+    relabel = {}
+    for n in range(N):
+        relabel.setdefault(int(sector[n]), len(relabel))
+    return C, np.array([relabel[int(s)] for s in sector], dtype=np.int32)
 
 
 def make_system(N: int = 40, L: int = 70, T: int = 8760, dt: float = 1.0, seed: int = 1) -> System:
@@ -197,7 +216,8 @@ def make_system(N: int = 40, L: int = 70, T: int = 8760, dt: float = 1.0, seed: 
     return System(N=N, L=L, T=T, dt=float(dt), line_bus0=bus0, line_bus1=bus1, line_b=b, line_s=s,
                   load=load, mc=mc, ef=ef, capex=capex, klo=klo, khi=khi, stor_hours=hours,
                   stor_eta_in=eta_in, stor_eta_out=eta_out, stor_qt=qt, K=K, Ptr=Ptr, pi0=pi0,
-                  phi=phi, sigma=sigma, sigma0=sigma0, mu=mu, xbar=xbar, k0=k0, pos=pos)
+                  phi=phi, sigma=sigma, sigma0=sigma0, mu=mu, xbar=xbar, k0=k0, pos=pos,
+                  ncountry=_countries(pos)[0], country=_countries(pos)[1])
 
 
 def hour_of_day(T: int, dt: float) -> np.ndarray:

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 1
+#define ESM_ABI_VERSION 2
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -98,6 +98,11 @@ typedef struct esm_model {
     double sigma0;             /* standard deviation of z at step 1                         */
     const double *mu;          /* [K][ESM_NREN*N][3] regime intercept, cos and sin coefficients */
     const double *xbar;        /* [ESM_NREN*N][24] climatological logit by hour of day      */
+    /* partitions of the graph-Haar hierarchy between the whole system and the single nodes
+     * (E.parts, src/energy.jl:292: [one cell, country, node]): the cell moves act on the cells
+     * of every level. country [N] in 0..ncountry-1; NULL (or ncountry = 0): no middle level.  */
+    int32_t ncountry;
+    const int32_t *country;
 } esm_model;
 
 typedef struct esm_handle esm_handle;
@@ -168,23 +173,57 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
                      const double *kappa_s, double kmin, double clip, int32_t box,
                      double *knew, int32_t *nclip);
 
+/* move families of the sweep statistics */
+#define ESM_FAM_GEN 0     /* gas utilization atoms                                   */
+#define ESM_FAM_THETA 1   /* voltage-angle atoms                                     */
+#define ESM_FAM_SOC 2     /* state-of-charge atoms                                   */
+#define ESM_FAM_SPAIR 3   /* random: storage–gas pair (prep_spair!)                  */
+#define ESM_FAM_UNIT 4    /* random: one unit at one node (prep_random! :unit)       */
+#define ESM_FAM_POT 5     /* random: the angle of one node (:potential)              */
+#define ESM_FAM_CELL 6    /* random: one carrier over a cell of a partition (:cell)  */
+#define ESM_FAM_XLOC 7    /* random: localized cross-node pair, leak to the residuals */
+#define ESM_FAM_XLOCC 8   /* random: localized pair, leak compensated pro rata        */
+#define ESM_FAM_XPAIR 9   /* random: cross-node pair, global re-routing               */
+#define ESM_NFAM 10
+#define ESM_NSTAT 5       /* attempted, accepted, improper, guard, ΔC evaluations     */
+/* the plain random families in the reference's order and default weights
+ * (plain_families, src/local.jl:1912-1915): unit 0.3, pair 0.25, xpair 0.03, xlocc 0.08,
+ * xloc 0.04, potential 0.1, cellpair 0.1, cell 0.1, spair 0.3                            */
+#define ESM_NWEIGHT 9
+
 /*
  * `operation_sweep!` -> `local_sweep!` (src/local.jl:1927-2062, :2401-2425) of every replica,
  * `nsweeps` times, at fixed capacities and weather: exact line-conditional moves drawn from
  * exp(−β_v ΔC(t)) (`sample_line!`, :1501-1560; `maxnats` is the walk's stop, 40 in the
- * reference). One sweep runs, in this order, conflict-free rounds of every family:
- *   0 gas utilization atoms at every (node, step);
- *   1 voltage-angle (potential) atoms, one round per colour of a distance-2 colouring;
- *   2 state-of-charge atoms, per storage carrier and time-parity class;
- *   3 storage–gas pairs (`prep_spair!`, :1708-1726), per storage carrier and parity class.
- * Each (site, replica) draws its two uniforms from Philox4x32-10 under `key` with counter
- * (replica, node·T + step, round id, counter + sweep index). The residual and flow caches are
- * updated incrementally. stats ([4][5], may be NULL): per family the moves attempted,
- * accepted (t ≠ 0), improper lines (t = 0), walks cut at the point cap or drawn non-finite
- * (t = 0), and the ΔC(t) evaluations made (the walks, the outward re-walk to the drawn segment).
+ * reference). One sweep runs, in this order:
+ *   atoms (unless switched off by esm_sweep_options), in conflict-free rounds: gas
+ *     utilization at every (node, step); voltage angles, one round per colour of a distance-2
+ *     colouring; states of charge per storage carrier and time-parity class;
+ *   random moves, per time-parity class: at every step round(extra·N) draws over the plain
+ *     families with their weights (pair and cellpair are void in a system with one
+ *     dispatchable unit per node and one injecting carrier per cell), run family by family,
+ *     the draws of a step in sequence; the selection of a move (family, node, partner, cell)
+ *     depends on the step and the sweep only, the line draw on the replica.
+ * Uniforms are Philox4x32-10 under `key`: atoms with counter (replica, node·T + step, round
+ * id, counter + sweep index); the family draws with (2^31 + draw, step, 9000, ·), the
+ * selection of the index-th move of family w (its position in the weight list) with
+ * (2^31 + w·2^20 + index, step, 9001, ·), its line draw with (replica, step,
+ * 2^30 + w·2^20 + index, ·) and the Jacobian test of the compensated pairs with (replica,
+ * step, 3·2^29 + w·2^20 + index, ·). The residual and flow caches
+ * are updated incrementally. stats ([ESM_NFAM][ESM_NSTAT], may be NULL): per family the moves
+ * attempted, accepted (t ≠ 0), improper lines (t = 0), walks cut at the point cap, drawn
+ * non-finite or rejected by the Jacobian test (t = 0), and the ΔC(t) evaluations made.
  */
 int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
                    int64_t *stats);
+
+/*
+ * Options of the operation sweep (`local_sweep!` keywords, src/local.jl:1927-1929): `extra`
+ * random moves per node and step (`local_extra`, 4 by default, src/run.jl:21), the weights of
+ * the plain families (ESM_NWEIGHT, in the order above; NULL: the reference's), and whether
+ * the atom rounds run (`atoms`). Defaults hold from esm_create on.
+ */
+int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, int32_t atoms);
 
 /*
  * `follow_capacities!` (src/replicas.jl:448-472) of every replica: move the shared capacities
@@ -254,9 +293,15 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_KERN_GEN 4     /* one round of gas atoms                     */
 #define ESM_KERN_THETA 5   /* one round of potential atoms (one bucket)  */
 #define ESM_KERN_SOC 6     /* one round of SoC atoms                     */
-#define ESM_KERN_SPAIR 7   /* one round of storage-gas pairs             */
+#define ESM_KERN_SPAIR 7   /* random storage-gas pairs of one parity class */
 #define ESM_KERN_FOLLOW 8  /* dispatch following                         */
-#define ESM_NKERN 9
+#define ESM_KERN_UNIT 9    /* random unit moves of one parity class      */
+#define ESM_KERN_POT 10    /* random potential moves                     */
+#define ESM_KERN_CELL 11   /* random cell moves                          */
+#define ESM_KERN_XLOC 12   /* random localized pairs                     */
+#define ESM_KERN_XLOCC 13  /* random compensated localized pairs         */
+#define ESM_KERN_XPAIR 14  /* random re-routed pairs                     */
+#define ESM_NKERN 15
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
 

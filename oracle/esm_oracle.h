@@ -44,10 +44,27 @@ int orc_weather_step(const esm_model *m, int R, const double *theta, const doubl
 int orc_categorical(const double *p, int K, double u);
 void orc_regime_window_path(const esm_model *m, int *path, int a, int b, const double *us);
 
-/* Operation sweeps (oracle_sweep.c), in place on v and the caches r, F (built by orc_energy);
- * stats [4][5] as esm_sweeps (the evaluation count is the oracle's own); nthreads OpenMP threads over the replicas. */
+/* Operation sweeps (oracle_sweep.c), in place on v and the caches r, F (built by orc_energy),
+ * with the options of esm_sweep_options (weights NULL: the reference's); stats [ESM_NFAM][5] as
+ * esm_sweeps (the evaluation count is the oracle's own); nthreads OpenMP threads over the replicas. */
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
-               const uint32_t key[2], uint32_t ctr0, double maxnats, int64_t *stats, int nthreads);
+               const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
+               int64_t *stats, int nthreads);
+/* Templates of the table-driven random moves: info [8] (templates, first xpair, first cell,
+ * levels, cells per level), xp_off [N + 1]; the touched and leak nodes of one template. */
+int orc_tables_info(const esm_model *m, int *info, int *xp_off);
+int orc_template_nodes(const esm_model *m, int tid, int *nodes, int *nn, int *kk, double *kg, int *nk);
+/* One table-driven move (kind 0 cell, 1 xloc, 2 xlocc, 3 xpair) of template tid at (τ, j):
+ * the draw with given uniforms (status 3: not valid), ΔC(t), and the commit at t. */
+int orc_sparse_draw(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                    int tid, int tau, double maxnats, double useg, double uoff, double *t_out, int *status);
+int orc_sparse_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                    int tid, int tau, const double *t, int nt, double *dc);
+int orc_sparse_apply(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                     int tid, int tau, double t);
+int orc_sparse_draw_many(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                         int tid, int tau, double maxnats, const double *useg, const double *uoff, int nd,
+                         double *t_out, int *status);
 int orc_colouring(const esm_model *m, int *colour);
 /* One move of family fam (0 gas, 1 potential, 2 SoC, 3 storage–gas pair) at (n, τ) of replica j
  * with given uniforms (no commit); its ΔC(t) at given t; its commit at t. */

@@ -45,8 +45,18 @@ def load():
         lib.orc_regime_window_path.argtypes = [M, _ip, C.c_int, C.c_int, _dp]
         lib.orc_regime_window_path.restype = None
         i64p = C.POINTER(C.c_int64)
-        lib.orc_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double, i64p,
-                                   C.c_int]
+        lib.orc_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double,
+                                   C.c_double, _dp, C.c_int, i64p, C.c_int]
+        lib.orc_tables_info.argtypes = [M, _ip, _ip]
+        lib.orc_template_nodes.argtypes = [M, C.c_int, _ip, _ip, _ip, _dp, _ip]
+        lib.orc_sparse_draw.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_double, _dp, _ip]
+        lib.orc_sparse_cost.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
+                                        C.c_int, _dp]
+        lib.orc_sparse_apply.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_double]
+        lib.orc_sparse_draw_many.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_double, _dp, _dp, C.c_int, _dp, _ip]
         lib.orc_colouring.argtypes = [M, _ip]
         lib.orc_move_draw.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                       C.c_double, C.c_double, C.c_double, _dp, _ip]
@@ -131,8 +141,9 @@ class Oracle:
                                          moved.ctypes.data_as(_ip)) == 0
         return m, z, x, v, moved
 
-    def sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1):
-        """Operation sweeps in place on copies; returns (v, r, F, stats [4, 4])."""
+    def sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1,
+               extra=4.0, weights=None, atoms=True):
+        """Operation sweeps in place on copies; returns (v, r, F, stats [10, 5])."""
         R = v.shape[-1]
         v = np.ascontiguousarray(v, dtype=np.float64).copy()
         k = np.ascontiguousarray(k, dtype=np.float64)
@@ -140,12 +151,68 @@ class Oracle:
             _, _, r, F = self.energy(v, k, x)
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         F = np.ascontiguousarray(F, dtype=np.float64).copy()
-        st = np.zeros(20, dtype=np.int64)
+        st = np.zeros(50, dtype=np.int64)
         kk = (C.c_uint32 * 2)(*key)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         rc = self.lib.orc_sweeps(C.byref(self.cm), R, _p(v), _p(k), _p(r), _p(F), int(nsweeps), kk, int(counter),
-                                 float(maxnats), st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
+                                 float(maxnats), float(extra), _p(w), int(bool(atoms)),
+                                 st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
         assert rc == 0
-        return v, r, F, st.reshape(4, 5)
+        return v, r, F, st.reshape(10, 5)
+
+    # -- table-driven random moves (kind 0 cell, 1 xloc, 2 xlocc, 3 xpair) ----------------------
+    def tables_info(self):
+        """dict(ntpl, xp0, cl0, nlev, cells per level, xp_off [N + 1])."""
+        info = np.zeros(8, dtype=np.int32)
+        xp = np.zeros(self.sys.N + 1, dtype=np.int32)
+        assert self.lib.orc_tables_info(C.byref(self.cm), info.ctypes.data_as(_ip), xp.ctypes.data_as(_ip)) == 0
+        return dict(ntpl=int(info[0]), xp0=int(info[1]), cl0=int(info[2]), nlev=int(info[3]),
+                    cells=[int(c) for c in info[4:4 + int(info[3])]], xp_off=xp)
+
+    def template_nodes(self, tid):
+        """(touched nodes, leak nodes, their requirement change per unit t) of template tid."""
+        N = self.sys.N
+        nodes = np.zeros(N, dtype=np.int32)
+        kk = np.zeros(N, dtype=np.int32)
+        kg = np.zeros(N)
+        nn, nk = C.c_int32(), C.c_int32()
+        assert self.lib.orc_template_nodes(C.byref(self.cm), int(tid), nodes.ctypes.data_as(_ip), C.byref(nn),
+                                           kk.ctypes.data_as(_ip), _p(kg), C.byref(nk)) == 0
+        return nodes[:nn.value].copy(), kk[:nk.value].copy(), kg[:nk.value].copy()
+
+    def sparse_draw(self, v, k, r, F, j, kind, tid, tau, useg, uoff, maxnats=40.0):
+        R = v.shape[-1]
+        t = C.c_double()
+        st = C.c_int32()
+        rc = self.lib.orc_sparse_draw(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, kind, tid, tau,
+                                      float(maxnats), float(useg), float(uoff), C.byref(t), C.byref(st))
+        assert rc == 0, rc
+        return t.value, st.value
+
+    def sparse_draw_many(self, v, k, r, F, j, kind, tid, tau, useg, uoff, maxnats=40.0):
+        R = v.shape[-1]
+        useg = np.ascontiguousarray(useg, dtype=np.float64)
+        uoff = np.ascontiguousarray(uoff, dtype=np.float64)
+        t = np.empty(len(useg))
+        st = np.empty(len(useg), dtype=np.int32)
+        rc = self.lib.orc_sparse_draw_many(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, kind, tid, tau,
+                                           float(maxnats), _p(useg), _p(uoff), len(useg), _p(t), st.ctypes.data_as(_ip))
+        assert rc == 0, rc
+        return t, st
+
+    def sparse_cost(self, v, k, r, F, j, kind, tid, tau, ts):
+        R = v.shape[-1]
+        ts = np.ascontiguousarray(ts, dtype=np.float64)
+        out = np.empty_like(ts)
+        rc = self.lib.orc_sparse_cost(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, kind, tid, tau, _p(ts),
+                                      len(ts), _p(out))
+        assert rc == 0, rc
+        return out
+
+    def sparse_apply(self, v, k, r, F, j, kind, tid, tau, t):
+        R = v.shape[-1]
+        rc = self.lib.orc_sparse_apply(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, kind, tid, tau, float(t))
+        assert rc == 0, rc
 
     def follow(self, v, k, x, knew, kmin=1.0):
         """follow_capacities! on copies: returns (v, k)."""

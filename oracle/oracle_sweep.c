@@ -464,36 +464,11 @@ static double trexp_offset(double g, double L, double u) {
     return L + log1p(-u * (-expm1(x))) / (-g);
 }
 
-/* sample_line! with the uniforms given: status 0 drawn, 1 improper, 2 point cap */
-static double sample_line(Block *b, double beta, double maxnats, double useg, double uoff, int *status) {
-    double right[2 * MAXBP], left[2 * MAXBP];
-    int nr = 0, nl = 0;
-    Step *bs[2] = {b->wa, b->ws};          /* bm.bps = wa.bps ++ ws.bps */
-    for (int q = 0; q < 2; q++)
-        for (int k = 0; k < bs[q]->nbp; k++) {
-            double x = bs[q]->bps[k];
-            if (!isfinite(x)) continue;
-            if (x > 0) right[nr++] = x;
-            else if (x < 0) left[nl++] = x;
-        }
-    insort(right, nr, 1.0);
-    insort(left, nl, -1.0);
-    storage_mask(b);
-    mv_eval(b, 0.0);
-    int ns = b->nsec;
-    double sec0[2 * MAXSEC];
-    memcpy(sec0, b->sec, sizeof(double) * ns);
-    static __thread Pts PR, PL;
-    int w = walk(b, beta, right, nr, 1.0, maxnats, sec0, ns, &PR);
-    if (w != 1) {
-        *status = w == 0 ? 1 : 2;
-        return 0.0;
-    }
-    w = walk(b, beta, left, nl, -1.0, maxnats, sec0, ns, &PL);
-    if (w != 1) {
-        *status = w == 0 ? 1 : 2;
-        return 0.0;
-    }
+/* segment log-masses of the walked points (left side reversed, t = 0, right side), the segment
+ * by inversion of the first uniform, the offset (trexp_offset) with the second (sample_line!,
+ * src/local.jl:1530-1560); status 0 drawn, 2 a non-finite draw */
+static double draw_points(const Pts *PLp, const Pts *PRp, double useg, double uoff, int *status) {
+    const Pts PL = *PLp, PR = *PRp;
     *status = 0;
     int np = PL.n + 1 + PR.n;
     double pts[2 * PMAX + 1], cs[2 * PMAX + 1], lm[2 * PMAX];
@@ -535,7 +510,40 @@ static double sample_line(Block *b, double beta, double maxnats, double useg, do
     return t;
 }
 
-/* commit_block! / apply_step! */
+/* sample_line! with the uniforms given: status 0 drawn, 1 improper, 2 point cap */
+static double sample_line(Block *b, double beta, double maxnats, double useg, double uoff, int *status) {
+    double right[2 * MAXBP], left[2 * MAXBP];
+    int nr = 0, nl = 0;
+    Step *bs[2] = {b->wa, b->ws};          /* bm.bps = wa.bps ++ ws.bps */
+    for (int q = 0; q < 2; q++)
+        for (int k = 0; k < bs[q]->nbp; k++) {
+            double x = bs[q]->bps[k];
+            if (!isfinite(x)) continue;
+            if (x > 0) right[nr++] = x;
+            else if (x < 0) left[nl++] = x;
+        }
+    insort(right, nr, 1.0);
+    insort(left, nl, -1.0);
+    storage_mask(b);
+    mv_eval(b, 0.0);
+    int ns = b->nsec;
+    double sec0[2 * MAXSEC];
+    memcpy(sec0, b->sec, sizeof(double) * ns);
+    static __thread Pts PR, PL;
+    int w = walk(b, beta, right, nr, 1.0, maxnats, sec0, ns, &PR);
+    if (w != 1) {
+        *status = w == 0 ? 1 : 2;
+        return 0.0;
+    }
+    w = walk(b, beta, left, nl, -1.0, maxnats, sec0, ns, &PL);
+    if (w != 1) {
+        *status = w == 0 ? 1 : 2;
+        return 0.0;
+    }
+    return draw_points(&PL, &PR, useg, uoff, status);
+}
+
+/* commit_block! / apply_step! *//* commit_block! / apply_step! */
 static void commit(Block *b, double t) {
     Sys *S = b->S;
     int j = b->j;
@@ -568,9 +576,10 @@ static void stat_add(int64_t *st, int status, double t, long nev) {
     if (status == 2) st[3]++;
 }
 
-/* one move of family fam at (n, τ) of replica j; s = storage carrier */
-static void do_move(Sys *S, int j, int fam, int s, int n, int tau, const uint32_t key[2], uint32_t round,
-                    uint32_t ctr, double maxnats, int64_t *stats) {
+/* one move of family fam at (n, τ) of replica j with the uniforms of its line draw; s = storage
+ * carrier. Moves that are not valid (a pair below κmin) are not made and not counted. */
+static void do_move_u(Sys *S, int j, int fam, int s, int n, int tau, double useg, double uoff, double maxnats,
+                      int64_t *stats) {
     const esm_model *m = S->m;
     int T = m->T, N = m->N;
     int taum = (tau - 1 + T) % T;
@@ -600,14 +609,786 @@ static void do_move(Sys *S, int j, int fam, int s, int n, int tau, const uint32_
     record(S, &ws, NULL, j);
     record(S, &wa, &ws, j);
     Block b = {S, &wa, &ws, j};
-    uint32_t c4[4] = {(uint32_t)j, (uint32_t)(n * T + tau), round, ctr}, w[4];
-    orc_philox(key, c4, w);
     int status;
     g_nev = 0;
-    double t = sample_line(&b, m->beta_v, maxnats, u01(w[0], w[1]), u01(w[2], w[3]), &status);
+    double t = sample_line(&b, m->beta_v, maxnats, useg, uoff, &status);
     long nev = g_nev;
     commit(&b, t);
     stat_add(stats, status, t, nev);
+}
+
+/* an atom move: uniforms from counter (replica, n·T + τ, round, sweep counter) */
+static void do_move(Sys *S, int j, int fam, int s, int n, int tau, const uint32_t key[2], uint32_t round,
+                    uint32_t ctr, double maxnats, int64_t *stats) {
+    uint32_t c4[4] = {(uint32_t)j, (uint32_t)(n * S->m->T + tau), round, ctr}, w[4];
+    orc_philox(key, c4, w);
+    do_move_u(S, j, fam, s, n, tau, u01(w[0], w[1]), u01(w[2], w[3]), maxnats, stats);
+}
+
+
+/* ============================================================================================
+ * The random families (local_sweep! :1952-2044, plain_families :1912-1915) and the templates of
+ * the table-driven ones.
+ *
+ * Templates (network only): xloc / xlocc after xloc_directions (:122-191) with one-hop partners
+ * and ball radius 0, xpair after prep_reroute! / reroute! (:44-110, :592-601) grounded at the
+ * first node of the component, cells of every partition level (E.parts, src/energy.jl:292;
+ * LocalModel :265-283). A direction is finished as finalize! does (:656-757): angle entries
+ * below 1e-14 are zero, lines of the angle entries with |Δf| > 1e-14 are kept, the touched nodes
+ * are those of the gas entries, the kinks and the kept lines; lines and nodes are listed in
+ * increasing order (the order of the evaluation sums).
+ * ============================================================================================ */
+
+typedef struct {
+    int *a;
+    int n, cap;
+} IVec;
+typedef struct {
+    double *a;
+    int n, cap;
+} DVec;
+static void ipush(IVec *v, int x) {
+    if (v->n == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 64;
+        v->a = (int *)realloc(v->a, sizeof(int) * v->cap);
+    }
+    v->a[v->n++] = x;
+}
+static void dpush(DVec *v, double x) {
+    if (v->n == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 64;
+        v->a = (double *)realloc(v->a, sizeof(double) * v->cap);
+    }
+    v->a[v->n++] = x;
+}
+
+typedef struct {
+    IVec th_off, th_node, en_off, en_node, en_div, kk_off, kk_node, ln_off, ln_line, nd_off, nd_node, nd_ent, nd_kk,
+        nd_l, nl_pos, nl_sgn, xp_off, lev_off;
+    DVec th_val, en_c, kk_g, ln_df, qcum;
+    int xp0, cl0, ntpl, nlev;
+} Tables;
+
+static void tables_free(Tables *tb) {
+    IVec *iv[] = {&tb->th_off, &tb->th_node, &tb->en_off, &tb->en_node, &tb->en_div, &tb->kk_off, &tb->kk_node,
+                  &tb->ln_off, &tb->ln_line, &tb->nd_off, &tb->nd_node, &tb->nd_ent, &tb->nd_kk, &tb->nd_l,
+                  &tb->nl_pos, &tb->nl_sgn, &tb->xp_off, &tb->lev_off};
+    DVec *dv[] = {&tb->th_val, &tb->en_c, &tb->kk_g, &tb->ln_df, &tb->qcum};
+    for (size_t q = 0; q < sizeof(iv) / sizeof(iv[0]); q++) free(iv[q]->a);
+    for (size_t q = 0; q < sizeof(dv) / sizeof(dv[0]); q++) free(dv[q]->a);
+}
+
+/* a direction before finalize: angle entries, gas entries (slope c, /κ when div), kinks */
+typedef struct {
+    int nth, ne, nk;
+    int th_node[256], en_node[256], en_div[256], kk_node[256];
+    double th_val[256], en_c[256], kk_g[256];
+} Dir;
+
+typedef struct {
+    const esm_model *m;
+    int N, L;
+    int *lptr, *lines;      /* lines at each node, line order */
+    int *nptr, *nbr;        /* neighbours (one per line end, line order) */
+    int *comp, ncomp;
+    int *cptr, *cnodes;     /* nodes of each component, increasing */
+    int *loc;               /* position among the non-ground nodes of the component (0: ground) */
+    double **G;             /* grounded Laplacian factor per component */
+} Net;
+
+/* A = G Gᵀ in place (row-major lower); 0 if not positive definite */
+static int chol(double *A, int n) {
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j <= i; j++) {
+            double s = A[(size_t)i * n + j];
+            for (int k = 0; k < j; k++) s -= A[(size_t)i * n + k] * A[(size_t)j * n + k];
+            if (i == j) {
+                if (!(s > 0)) return 0;
+                A[(size_t)i * n + i] = sqrt(s);
+            } else {
+                A[(size_t)i * n + j] = s / A[(size_t)j * n + j];
+            }
+        }
+        for (int j = i + 1; j < n; j++) A[(size_t)i * n + j] = 0.0;
+    }
+    return 1;
+}
+static void chol_solve(const double *G, int n, double *b) {
+    for (int i = 0; i < n; i++) {
+        double s = b[i];
+        for (int k = 0; k < i; k++) s -= G[(size_t)i * n + k] * b[k];
+        b[i] = s / G[(size_t)i * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int k = i + 1; k < n; k++) s -= G[(size_t)k * n + i] * b[k];
+        b[i] = s / G[(size_t)i * n + i];
+    }
+}
+
+static int cmp_int(const void *a, const void *b) { return (*(const int *)a > *(const int *)b) - (*(const int *)a < *(const int *)b); }
+
+static void emit(Tables *tb, const Net *net, const Dir *d) {
+    const esm_model *m = net->m;
+    int N = net->N;
+    double *dth = (double *)calloc(N, sizeof(double));
+    int *lmark = (int *)calloc(net->L, sizeof(int)), *nmark = (int *)calloc(N, sizeof(int));
+    for (int q = 0; q < d->nth; q++) {
+        if (fabs(d->th_val[q]) < 1e-14) continue;
+        dth[d->th_node[q]] = d->th_val[q];
+        ipush(&tb->th_node, d->th_node[q]);
+        dpush(&tb->th_val, d->th_val[q]);
+        for (int p = net->lptr[d->th_node[q]]; p < net->lptr[d->th_node[q] + 1]; p++) lmark[net->lines[p]] = 1;
+    }
+    int nl = 0, *ll = (int *)malloc(sizeof(int) * (net->L + 1));
+    double *lf = (double *)malloc(sizeof(double) * (net->L + 1));
+    for (int e = 0; e < net->L; e++) {
+        if (!lmark[e]) continue;
+        double df = m->line_b[e] * (dth[m->line_bus0[e]] - dth[m->line_bus1[e]]);
+        if (fabs(df) <= 1e-14) continue;
+        ll[nl] = e;
+        lf[nl] = df;
+        nl++;
+        nmark[m->line_bus0[e]] = nmark[m->line_bus1[e]] = 1;
+    }
+    for (int q = 0; q < d->ne; q++) nmark[d->en_node[q]] = 1;
+    for (int q = 0; q < d->nk; q++) nmark[d->kk_node[q]] = 1;
+    for (int q = 0; q < d->ne; q++) {
+        ipush(&tb->en_node, d->en_node[q]);
+        dpush(&tb->en_c, d->en_c[q]);
+        ipush(&tb->en_div, d->en_div[q]);
+    }
+    for (int q = 0; q < d->nk; q++) {
+        ipush(&tb->kk_node, d->kk_node[q]);
+        dpush(&tb->kk_g, d->kk_g[q]);
+    }
+    for (int l = 0; l < nl; l++) {
+        ipush(&tb->ln_line, ll[l]);
+        dpush(&tb->ln_df, lf[l]);
+    }
+    for (int n = 0; n < N; n++) {
+        if (!nmark[n]) continue;
+        int ent = -1, kk = -1;
+        for (int q = 0; q < d->ne; q++)
+            if (d->en_node[q] == n) ent = q;
+        for (int q = 0; q < d->nk; q++)
+            if (d->kk_node[q] == n) kk = q;
+        ipush(&tb->nd_node, n);
+        ipush(&tb->nd_ent, ent);
+        ipush(&tb->nd_kk, kk);
+        for (int l = 0; l < nl; l++) {
+            int e = ll[l];
+            if (m->line_bus0[e] == n || m->line_bus1[e] == n) {
+                ipush(&tb->nl_pos, l);
+                ipush(&tb->nl_sgn, m->line_bus0[e] == n ? 1 : -1);
+            }
+        }
+        ipush(&tb->nd_l, tb->nl_pos.n);
+    }
+    ipush(&tb->th_off, tb->th_node.n);
+    ipush(&tb->en_off, tb->en_node.n);
+    ipush(&tb->kk_off, tb->kk_node.n);
+    ipush(&tb->ln_off, tb->ln_line.n);
+    ipush(&tb->nd_off, tb->nd_node.n);
+    tb->ntpl++;
+    free(dth); free(lmark); free(nmark); free(ll); free(lf);
+}
+
+/* Δθ with L Δθ = e_n − e_m on the component of n, grounded at its first node (reroute!) */
+static void reroute(const Net *net, int n, int mm, double *x) {
+    int c = net->comp[n], k = net->cptr[c + 1] - net->cptr[c] - 1;
+    double *rhs = (double *)calloc(k > 0 ? k : 1, sizeof(double));
+    if (net->loc[n] > 0) rhs[net->loc[n] - 1] += 1.0;
+    if (net->loc[mm] > 0) rhs[net->loc[mm] - 1] -= 1.0;
+    if (k > 0) chol_solve(net->G[c], k, rhs);
+    for (int i = 0; i < net->N; i++) x[i] = 0.0;
+    for (int q = net->cptr[c]; q < net->cptr[c + 1]; q++) {
+        int i = net->cnodes[q];
+        x[i] = net->loc[i] > 0 ? rhs[net->loc[i] - 1] : 0.0;
+    }
+    free(rhs);
+}
+
+static int build_tables(const esm_model *m, Tables *tb) {
+    int N = m->N, L = m->L;
+    memset(tb, 0, sizeof(*tb));
+    if (N > 256) return -1;
+    Net net;
+    memset(&net, 0, sizeof(net));
+    net.m = m;
+    net.N = N;
+    net.L = L;
+    net.lptr = (int *)calloc(N + 1, sizeof(int));
+    net.lines = (int *)malloc(sizeof(int) * 2 * L);
+    net.nptr = (int *)calloc(N + 1, sizeof(int));
+    net.nbr = (int *)malloc(sizeof(int) * 2 * L);
+    for (int e = 0; e < L; e++) {
+        net.lptr[m->line_bus0[e] + 1]++;
+        net.lptr[m->line_bus1[e] + 1]++;
+    }
+    for (int n = 0; n < N; n++) net.lptr[n + 1] += net.lptr[n];
+    memcpy(net.nptr, net.lptr, sizeof(int) * (N + 1));
+    {
+        int *f1 = (int *)malloc(sizeof(int) * N), *f2 = (int *)malloc(sizeof(int) * N);
+        for (int n = 0; n < N; n++) f1[n] = f2[n] = net.lptr[n];
+        for (int e = 0; e < L; e++) {
+            int a = m->line_bus0[e], b = m->line_bus1[e];
+            net.lines[f1[a]++] = e;
+            net.lines[f1[b]++] = e;
+            net.nbr[f2[a]++] = b;
+            net.nbr[f2[b]++] = a;
+        }
+        free(f1); free(f2);
+    }
+    /* components, nodes increasing */
+    net.comp = (int *)malloc(sizeof(int) * N);
+    for (int n = 0; n < N; n++) net.comp[n] = -1;
+    int *stack = (int *)malloc(sizeof(int) * N), *order = (int *)malloc(sizeof(int) * N), no = 0;
+    net.cptr = (int *)calloc(N + 1, sizeof(int));
+    for (int s = 0; s < N; s++) {
+        if (net.comp[s] >= 0) continue;
+        int c = net.ncomp++, sp = 0, c0 = no;
+        stack[sp++] = s;
+        net.comp[s] = c;
+        while (sp) {
+            int i = stack[--sp];
+            order[no++] = i;
+            for (int q = net.nptr[i]; q < net.nptr[i + 1]; q++) {
+                int jn = net.nbr[q];
+                if (net.comp[jn] < 0) {
+                    net.comp[jn] = c;
+                    stack[sp++] = jn;
+                }
+            }
+        }
+        qsort(order + c0, no - c0, sizeof(int), cmp_int);
+        net.cptr[c + 1] = no;
+    }
+    net.cnodes = order;
+    net.loc = (int *)calloc(N, sizeof(int));
+    net.G = (double **)calloc(net.ncomp, sizeof(double *));
+    for (int c = 0; c < net.ncomp; c++) {
+        int c0 = net.cptr[c], k = net.cptr[c + 1] - c0 - 1;
+        for (int q = 1; q <= k; q++) net.loc[net.cnodes[c0 + q]] = q;
+        net.loc[net.cnodes[c0]] = 0;
+        if (k == 0) continue;
+        double *A = (double *)calloc((size_t)k * k, sizeof(double));
+        for (int e = 0; e < L; e++) {
+            int a = m->line_bus0[e], b = m->line_bus1[e];
+            if (net.comp[a] != c) continue;
+            int ia = net.loc[a], ib = net.loc[b];
+            double w = m->line_b[e];
+            if (ia > 0) A[(size_t)(ia - 1) * k + (ia - 1)] += w;
+            if (ib > 0) A[(size_t)(ib - 1) * k + (ib - 1)] += w;
+            if (ia > 0 && ib > 0) {
+                A[(size_t)(ia - 1) * k + (ib - 1)] -= w;
+                A[(size_t)(ib - 1) * k + (ia - 1)] -= w;
+            }
+        }
+        if (!chol(A, k)) return -2;
+        net.G[c] = A;
+    }
+    ipush(&tb->th_off, 0);
+    ipush(&tb->en_off, 0);
+    ipush(&tb->kk_off, 0);
+    ipush(&tb->ln_off, 0);
+    ipush(&tb->nd_off, 0);
+    ipush(&tb->nd_l, 0);
+    ipush(&tb->xp_off, 0);
+    double *x = (double *)malloc(sizeof(double) * N);
+    static __thread Dir d;
+    /* xloc / xlocc: partners = neighbours (one hop), ball B = {n, m} */
+    for (int n = 0; n < N; n++) {
+        int np = 0, part[256];
+        for (int q = net.nptr[n]; q < net.nptr[n + 1]; q++) {
+            int mm = net.nbr[q], seen = mm == n;
+            for (int r = 0; r < np; r++) seen |= part[r] == mm;
+            if (!seen) part[np++] = mm;
+        }
+        qsort(part, np, sizeof(int), cmp_int);
+        for (int pi = 0; pi < np; pi++) {
+            int mm = part[pi];
+            d.nth = d.ne = d.nk = 0;
+            d.en_node[d.ne] = n; d.en_c[d.ne] = 1.0; d.en_div[d.ne++] = 1;
+            d.en_node[d.ne] = mm; d.en_c[d.ne] = -1.0; d.en_div[d.ne++] = 1;
+            int B[2] = {n < mm ? n : mm, n < mm ? mm : n};
+            int csize = net.cptr[net.comp[n] + 1] - net.cptr[net.comp[n]];
+            if (2 >= csize) {
+                reroute(&net, n, mm, x);
+                for (int q = net.cptr[net.comp[n]]; q < net.cptr[net.comp[n] + 1]; q++) {
+                    int i = net.cnodes[q];
+                    if (fabs(x[i]) >= 1e-15) { d.th_node[d.nth] = i; d.th_val[d.nth++] = x[i]; }
+                }
+            } else {
+                double A[4] = {0, 0, 0, 0};
+                for (int q = 0; q < 2; q++)
+                    for (int p2 = net.lptr[B[q]]; p2 < net.lptr[B[q] + 1]; p2++) {
+                        int e = net.lines[p2], a = m->line_bus0[e], b = m->line_bus1[e];
+                        int o = a == B[q] ? b : a;
+                        A[q * 2 + q] += m->line_b[e];
+                        int po = o == B[0] ? 0 : (o == B[1] ? 1 : -1);
+                        if (po >= 0) A[q * 2 + po] -= m->line_b[e];
+                    }
+                if (!chol(A, 2)) return -2;
+                double th[2] = {0, 0};
+                th[n == B[0] ? 0 : 1] += 1.0;
+                th[mm == B[0] ? 0 : 1] -= 1.0;
+                chol_solve(A, 2, th);
+                for (int q = 0; q < 2; q++)
+                    if (fabs(th[q]) >= 1e-15) { d.th_node[d.nth] = B[q]; d.th_val[d.nth++] = th[q]; }
+                /* leak −Σ b Δθ_i into the first layer outside B, sorted by node */
+                int lkn[256], nlk = 0;
+                double lkg[256];
+                for (int q = 0; q < 2; q++)
+                    for (int p2 = net.lptr[B[q]]; p2 < net.lptr[B[q] + 1]; p2++) {
+                        int e = net.lines[p2], a = m->line_bus0[e], b = m->line_bus1[e];
+                        int o = a == B[q] ? b : a;
+                        if (o == B[0] || o == B[1]) continue;
+                        int jl = 0;
+                        while (jl < nlk && lkn[jl] != o) jl++;
+                        if (jl == nlk) { lkn[nlk] = o; lkg[nlk] = 0.0; nlk++; }
+                        lkg[jl] -= m->line_b[e] * th[q];
+                    }
+                for (int a = 1; a < nlk; a++)      /* insertion sort by node */
+                    for (int b = a; b > 0 && lkn[b - 1] > lkn[b]; b--) {
+                        int tn = lkn[b]; lkn[b] = lkn[b - 1]; lkn[b - 1] = tn;
+                        double tg = lkg[b]; lkg[b] = lkg[b - 1]; lkg[b - 1] = tg;
+                    }
+                for (int q = 0; q < nlk; q++)
+                    if (lkg[q] != 0.0) { d.kk_node[d.nk] = lkn[q]; d.kk_g[d.nk++] = lkg[q]; }
+            }
+            emit(tb, &net, &d);
+        }
+        ipush(&tb->xp_off, tb->ntpl);
+    }
+    /* xpair: every ordered pair of one component */
+    tb->xp0 = tb->ntpl;
+    for (int n = 0; n < N; n++)
+        for (int mm = 0; mm < N; mm++) {
+            d.nth = d.ne = d.nk = 0;
+            if (mm != n && net.comp[mm] == net.comp[n]) {
+                d.en_node[d.ne] = n; d.en_c[d.ne] = 1.0; d.en_div[d.ne++] = 1;
+                d.en_node[d.ne] = mm; d.en_c[d.ne] = -1.0; d.en_div[d.ne++] = 1;
+                reroute(&net, n, mm, x);
+                for (int q = net.cptr[net.comp[n]]; q < net.cptr[net.comp[n] + 1]; q++) {
+                    int i = net.cnodes[q];
+                    if (fabs(x[i]) >= 1e-15) { d.th_node[d.nth] = i; d.th_val[d.nth++] = x[i]; }
+                }
+            }
+            emit(tb, &net, &d);
+        }
+    /* cells: the whole system, the countries, the nodes (cells in order of first appearance) */
+    tb->cl0 = tb->ntpl;
+    ipush(&tb->lev_off, 0);
+    int nlev = m->ncountry > 0 ? 3 : 2;
+    double acc = 0.0;
+    for (int lv = 0; lv < nlev; lv++) {
+        int keys[256], nkey = 0;
+        for (int n = 0; n < N; n++) {
+            int lab = lv == 0 ? 0 : (lv == nlev - 1 ? n : m->country[n]), seen = 0;
+            for (int q = 0; q < nkey; q++) seen |= keys[q] == lab;
+            if (!seen) keys[nkey++] = lab;
+        }
+        for (int q = 0; q < nkey; q++) {
+            d.nth = d.ne = d.nk = 0;
+            for (int n = 0; n < N; n++) {
+                int lab = lv == 0 ? 0 : (lv == nlev - 1 ? n : m->country[n]);
+                if (lab == keys[q]) { d.en_node[d.ne] = n; d.en_c[d.ne] = 1.0; d.en_div[d.ne++] = 0; }
+            }
+            emit(tb, &net, &d);
+        }
+        ipush(&tb->lev_off, tb->lev_off.a[tb->lev_off.n - 1] + nkey);
+        acc += (double)nkey;
+        dpush(&tb->qcum, acc);
+    }
+    for (int q = 0; q < tb->qcum.n; q++) tb->qcum.a[q] /= acc;
+    tb->nlev = nlev;
+    free(x);
+    for (int c = 0; c < net.ncomp; c++) free(net.G[c]);
+    free(net.G); free(net.lptr); free(net.lines); free(net.nptr); free(net.nbr); free(net.comp); free(stack);
+    free(net.cnodes); free(net.cptr); free(net.loc);
+    return 0;
+}
+
+/* ---- a table-driven move of replica j at τ, evaluated on the fly from the caches ---- */
+enum { SP_CELL = 0, SP_XLOC = 1, SP_XLOCC = 2, SP_XPAIR = 3 };
+
+typedef struct {
+    Sys *S;
+    const Tables *tb;
+    int j, tau, tid, comp;
+    int e0, e1, q0, q1, l0, l1, n0, n1;
+    double kmin;
+} SMove;
+
+static void sm_init(SMove *f, Sys *S, const Tables *tb, int j, int tau, int tid, int comp) {
+    f->S = S; f->tb = tb; f->j = j; f->tau = tau; f->tid = tid; f->comp = comp; f->kmin = 1.0;
+    f->e0 = tb->en_off.a[tid]; f->e1 = tb->en_off.a[tid + 1];
+    f->q0 = tb->kk_off.a[tid]; f->q1 = tb->kk_off.a[tid + 1];
+    f->l0 = tb->ln_off.a[tid]; f->l1 = tb->ln_off.a[tid + 1];
+    f->n0 = tb->nd_off.a[tid]; f->n1 = tb->nd_off.a[tid + 1];
+}
+#define SMV(arr, n) (f->S->arr)[((size_t)(n) * f->S->m->T + f->tau) * f->S->R + f->j]
+static double sm_kap(const SMove *f, int n) { return f->S->kc[ESM_KGAS * f->S->m->N + n]; }
+static double sm_slope(const SMove *f, int e) {
+    double c = f->tb->en_c.a[e];
+    double a = f->tb->en_div.a[e] ? c / sm_kap(f, f->tb->en_node.a[e]) : c;
+    return fabs(a) < 1e-14 ? 0.0 : a;
+}
+static void sm_gen(const SMove *f, int n, double y, double *c, double *p) {
+    const esm_model *m = f->S->m;
+    double k = sm_kap(f, n);
+    gen_cost(m->mc[ESM_KGAS * m->N + n], 1.0, k, k + m->kappa0, m->dt, m->c_viol, y, c, p);
+}
+/* prorata! of the single unit of a leak node: 0 when no kink is made */
+static int sm_kink(const SMove *f, int q, double *Q, double *g, double *ak, double *P, double *e, double *y0) {
+    int n = f->tb->kk_node.a[q];
+    double k = sm_kap(f, n);
+    if (!(fabs(k) >= f->kmin)) return 0;
+    double w = SMV(yg, n);
+    *P = 0.0 + k * w;
+    double Qs = 0.0 + k * clampd(w, 0.0, 1.0);
+    if (fabs(*P) < 1e-6) return 0;
+    *Q = SMV(r, n) + Qs;
+    *e = *P - fmax(*Q, 0.0);
+    *g = f->tb->kk_g.a[q];
+    double sig = k * w / *P;
+    *ak = sig == 0.0 ? 0.0 : sig / k;
+    *y0 = w;
+    return *g != 0.0;
+}
+static double sm_phi(double Q, double g, double t) { return fmax(Q + g * t, 0.0) - fmax(Q, 0.0); }
+
+static double sm_node_S(const SMove *f, int p, double t) {
+    const Tables *tb = f->tb;
+    int n = tb->nd_node.a[p];
+    double dr = 0.0;
+    int ie = tb->nd_ent.a[p];
+    if (ie >= 0) {
+        double a = sm_slope(f, f->e0 + ie);
+        if (a != 0.0) {
+            double y0 = SMV(yg, n), c0, p0, c1, p1;
+            sm_gen(f, n, y0, &c0, &p0);
+            sm_gen(f, n, y0 + t * a, &c1, &p1);
+            dr -= p1 - p0;
+        }
+    }
+    int ik = tb->nd_kk.a[p];
+    if (f->comp && ik >= 0) {
+        double Q, g, ak, P, e, y0;
+        if (sm_kink(f, f->q0 + ik, &Q, &g, &ak, &P, &e, &y0) && ak != 0.0) {
+            double c0, p0, c1, p1;
+            sm_gen(f, n, y0, &c0, &p0);
+            sm_gen(f, n, y0 + ak * sm_phi(Q, g, t), &c1, &p1);
+            dr -= p1 - p0;
+        }
+    }
+    for (int q = tb->nd_l.a[p]; q < tb->nd_l.a[p + 1]; q++) {
+        int pos = f->l0 + tb->nl_pos.a[q];
+        double f0 = SMV(F, tb->ln_line.a[pos]);
+        double dd = (f0 + t * tb->ln_df.a[pos]) - f0;
+        dr += tb->nl_sgn.a[q] > 0 ? dd : -dd;
+    }
+    return SMV(r, n) + dr;
+}
+
+static double sm_eval(const SMove *f, double t) {
+    const Tables *tb = f->tb;
+    const esm_model *m = f->S->m;
+    double D = 0.0;
+    g_nev++;
+    for (int e = f->e0; e < f->e1; e++) {
+        double a = sm_slope(f, e);
+        if (a == 0.0) continue;
+        int n = tb->en_node.a[e];
+        double y0 = SMV(yg, n), c0, p0, c1, p1;
+        sm_gen(f, n, y0, &c0, &p0);
+        sm_gen(f, n, y0 + t * a, &c1, &p1);
+        D += c1 - c0;
+    }
+    if (f->comp)
+        for (int q = f->q0; q < f->q1; q++) {
+            double Q, g, ak, P, e, y0, c0, p0, c1, p1;
+            if (!sm_kink(f, q, &Q, &g, &ak, &P, &e, &y0) || ak == 0.0) continue;
+            int n = tb->kk_node.a[q];
+            sm_gen(f, n, y0, &c0, &p0);
+            sm_gen(f, n, y0 + ak * sm_phi(Q, g, t), &c1, &p1);
+            D += c1 - c0;
+        }
+    for (int l = f->l0; l < f->l1; l++) {
+        int e = tb->ln_line.a[l];
+        double f0 = SMV(F, e), se = m->line_s[e];
+        double f1 = f0 + t * tb->ln_df.a[l];
+        D += m->c_viol * m->dt * (fmax(fabs(f1) - se, 0.0) - fmax(fabs(f0) - se, 0.0));
+    }
+    for (int p = f->n0; p < f->n1; p++) {
+        double S0 = SMV(r, tb->nd_node.a[p]);
+        double S1 = sm_node_S(f, p, t);
+        D += m->dt * (phibal(m, S1) - phibal(m, S0));
+    }
+    return D;
+}
+
+static int sm_next_bp(const SMove *f, double tp, double sgn, double *t) {
+    const Tables *tb = f->tb;
+    const esm_model *m = f->S->m;
+    int have = 0;
+#define CAND(bv)                                                                   \
+    do {                                                                           \
+        double b_ = (bv);                                                          \
+        if (isfinite(b_) && sgn * (b_ - tp) > 0 && (!have || sgn * (*t - b_) > 0)) { \
+            *t = b_;                                                               \
+            have = 1;                                                              \
+        }                                                                          \
+    } while (0)
+    for (int e = f->e0; e < f->e1; e++) {
+        double a = sm_slope(f, e);
+        if (a == 0.0) continue;
+        double y0 = SMV(yg, tb->en_node.a[e]);
+        CAND((0.0 - y0) / a);
+        CAND((1.0 - y0) / a);
+    }
+    if (f->comp)
+        for (int q = f->q0; q < f->q1; q++) {
+            double Q, g, ak, P, e, y0;
+            if (!sm_kink(f, q, &Q, &g, &ak, &P, &e, &y0)) continue;
+            CAND(-Q / g);
+            if (ak != 0.0) {
+                double sl = ak * g, off = ak * Q;
+                CAND((0.0 - y0 - 0.0) / sl);
+                CAND((1.0 - y0 - 0.0) / sl);
+                CAND((0.0 - y0 - off) / sl);
+                CAND((1.0 - y0 - off) / sl);
+            }
+        }
+    for (int l = f->l0; l < f->l1; l++) {
+        int e = tb->ln_line.a[l];
+        double f0 = SMV(F, e), se = m->line_s[e], df = tb->ln_df.a[l];
+        CAND((se - f0) / df);
+        CAND((-se - f0) / df);
+    }
+#undef CAND
+    return have;
+}
+
+static int sm_next_root(const SMove *f, double tp, double t, double x, double sgn, double *xr) {
+    int have = 0;
+    for (int p = f->n0; p < f->n1; p++) {
+        double A = sm_node_S(f, p, tp), B = sm_node_S(f, p, t);
+        if ((A < 0) != (B < 0) && A != B) {
+            double rt = tp + (t - tp) * A / (A - B);
+            if (sgn * (rt - tp) > 0 && sgn * (t - rt) > 0 && sgn * (rt - x) > 0 && (!have || sgn * (*xr - rt) > 0)) {
+                *xr = rt;
+                have = 1;
+            }
+        }
+    }
+    return have;
+}
+
+/* walk! of a table-driven move: 1 proper, 0 improper, -1 point cap */
+static int sm_walk(const SMove *f, double beta, double sgn, double maxnats, Pts *P) {
+    double cmin = 0.0, tp = 0.0, delta = 0.0;
+    int nx = 0;
+    P->n = 0;
+    P->full = 0;
+    for (;;) {
+        double t;
+        if (!sm_next_bp(f, tp, sgn, &t)) {
+            double cprev = P->n == 0 ? 0.0 : P->c[P->n - 1];
+            double hp = 1e-6 * fmax(1.0, fabs(tp));
+            double sl = (beta * sm_eval(f, tp + sgn * hp) - cprev) / hp;
+            if (sl > 0) delta = fmin(2 * maxnats / sl, 1e3 * fmax(fabs(tp), 1e-3));
+            else delta = delta == 0.0 ? fmax(fabs(tp), 1e-6) : 2 * delta;
+            t = tp + sgn * delta;
+            if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;
+        }
+        double c = beta * sm_eval(f, t);
+        double x = tp, xr = 0.0;
+        while (sm_next_root(f, tp, t, x, sgn, &xr)) {
+            double cx = beta * sm_eval(f, xr);
+            push_pt(P, xr, cx);
+            cmin = fmin(cmin, cx);
+            x = xr;
+        }
+        push_pt(P, t, c);
+        if (P->full) return -1;
+        cmin = fmin(cmin, c);
+        if (c - cmin > maxnats) return 1;
+        tp = t;
+    }
+}
+
+static double sm_sample(const SMove *f, double beta, double maxnats, double useg, double uoff, int *status) {
+    static __thread Pts PR, PL;
+    int w = sm_walk(f, beta, 1.0, maxnats, &PR);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    w = sm_walk(f, beta, -1.0, maxnats, &PL);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    return draw_points(&PL, &PR, useg, uoff, status);
+}
+
+/* logjac (:1874): Σ (|S| − 1) log|P(t)/P| over the kinks made; one unit per node here */
+static double sm_logjac(const SMove *f, double t, int *any) {
+    double lj = 0.0;
+    *any = 0;
+    for (int q = f->q0; q < f->q1; q++) {
+        double Q, g, ak, P, e, y0;
+        if (!sm_kink(f, q, &Q, &g, &ak, &P, &e, &y0)) continue;
+        *any = 1;
+        double nm = 0.0;
+        lj += nm * log(fabs((fmax(Q + g * t, 0.0) + e) / P));
+    }
+    return lj;
+}
+
+/* commit_block! of a table-driven move: node by node, then flows, then angles */
+static void sm_commit(const SMove *f, double t) {
+    const Tables *tb = f->tb;
+    if (t == 0.0) return;
+    for (int p = f->n0; p < f->n1; p++) {
+        int n = tb->nd_node.a[p];
+        double S1 = sm_node_S(f, p, t), ynew = 0.0;
+        int wy = 0, ie = tb->nd_ent.a[p], ik = tb->nd_kk.a[p];
+        if (ie >= 0) {
+            double a = sm_slope(f, f->e0 + ie);
+            if (a != 0.0) { ynew = SMV(yg, n) + t * a; wy = 1; }
+        }
+        if (f->comp && ik >= 0) {
+            double Q, g, ak, P, e, y0;
+            if (sm_kink(f, f->q0 + ik, &Q, &g, &ak, &P, &e, &y0) && ak != 0.0) { ynew = y0 + ak * sm_phi(Q, g, t); wy = 1; }
+        }
+        SMV(r, n) = S1;
+        if (wy) SMV(yg, n) = ynew;
+    }
+    for (int l = f->l0; l < f->l1; l++) {
+        double *F1 = &SMV(F, tb->ln_line.a[l]);
+        *F1 = *F1 + t * tb->ln_df.a[l];
+    }
+    for (int q = tb->th_off.a[f->tid]; q < tb->th_off.a[f->tid + 1]; q++) {
+        double *th = &SMV(th, tb->th_node.a[q]);
+        *th = *th + t * tb->th_val.a[q];
+    }
+}
+
+/* whether the pair's units have |κ| ≥ κmin (cells have no such condition) */
+static int sm_valid(const SMove *f, int kind) {
+    if (kind == SP_CELL) return 1;
+    if (f->e0 == f->e1) return 0;
+    for (int e = f->e0; e < f->e1; e++)
+        if (!(fabs(sm_kap(f, f->tb->en_node.a[e])) >= f->kmin)) return 0;
+    return 1;
+}
+
+/* one table-driven move with given uniforms (umh: the Jacobian test's); returns t, counts it */
+static double sm_move(Sys *S, const Tables *tb, int j, int tau, int tid, int kind, double useg, double uoff, double umh,
+                      double maxnats, int64_t *stats) {
+    SMove f;
+    sm_init(&f, S, tb, j, tau, tid, kind == SP_XLOCC);
+    if (!sm_valid(&f, kind)) return 0.0;
+    int status;
+    g_nev = 0;
+    double t = sm_sample(&f, S->m->beta_v, maxnats, useg, uoff, &status);
+    if (kind == SP_XLOCC && t != 0.0) {
+        int any;
+        double lj = sm_logjac(&f, t, &any);
+        if (any && !(umh < exp(fmin(0.0, lj)))) {
+            t = 0.0;
+            status = 2;
+        }
+    }
+    long nev = g_nev;
+    sm_commit(&f, t);
+    if (stats) stat_add(stats, status, t, nev);
+    return t;
+}
+
+static uint32_t pick(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
+
+/* family counts of every step: round(extra·N) draws over the cumulative weights */
+static void move_counts(const esm_model *m, double extra, const double *wts, const uint32_t key[2], uint32_t ctr,
+                        int *counts) {
+    int T = m->T, nextra = (int)nearbyint(extra * m->N);
+    double cw[ESM_NWEIGHT], acc = 0.0, tot = 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) tot += wts[q];
+    for (int q = 0; q < ESM_NWEIGHT; q++) { acc += wts[q]; cw[q] = acc / tot; }
+    for (int tau = 0; tau < T; tau++) {
+        int cnt[ESM_NWEIGHT] = {0};
+        for (int d = 0; d < nextra; d++) {
+            uint32_t c4[4] = {0x80000000u | (uint32_t)d, (uint32_t)tau, 9000u, ctr}, w[4];
+            orc_philox(key, c4, w);
+            double u = u01(w[0], w[1]);
+            int f = ESM_NWEIGHT - 1;
+            for (int q = ESM_NWEIGHT - 1; q >= 0; q--)
+                if (cw[q] >= u) f = q;
+            cnt[f]++;
+        }
+        for (int q = 0; q < ESM_NWEIGHT; q++) counts[(size_t)q * T + tau] = nextra > 0 ? cnt[q] : 0;
+    }
+}
+
+enum { W_UNIT = 0, W_PAIR = 1, W_XPAIR = 2, W_XLOCC = 3, W_XLOC = 4, W_POT = 5, W_CELLPAIR = 6, W_CELL = 7, W_SPAIR = 8 };
+
+/* the random moves of step τ for replica j, family by family (pair and cellpair are void) */
+static void random_step(Sys *S, const Tables *tb, int j, int tau, const int *counts, const double *wts,
+                        const uint32_t key[2], uint32_t ctr, double maxnats, int64_t *st) {
+    const esm_model *m = S->m;
+    int N = m->N, T = m->T;
+    static const int order[7] = {W_UNIT, W_XPAIR, W_XLOCC, W_XLOC, W_POT, W_CELL, W_SPAIR};
+    for (int oi = 0; oi < 7; oi++) {
+        int w = order[oi];
+        if (!(wts[w] > 0)) continue;
+        int cnt = counts[(size_t)w * T + tau];
+        int nbucket = w == W_POT ? 3 : 1;
+        for (int bk = 0; bk < nbucket; bk++)
+            for (int k = 0; k < cnt; k++) {
+                uint32_t c4[4] = {0x80000000u | ((uint32_t)w << 20) | (uint32_t)k, (uint32_t)tau, 9001u, ctr}, s[4];
+                orc_philox(key, c4, s);
+                uint32_t cl[4] = {(uint32_t)j, (uint32_t)tau, 0x40000000u | ((uint32_t)w << 20) | (uint32_t)k, ctr}, wl[4];
+                orc_philox(key, cl, wl);
+                double useg = u01(wl[0], wl[1]), uoff = u01(wl[2], wl[3]);
+                int n = (int)pick(s[0], (uint32_t)N);
+                if (w == W_UNIT) {
+                    do_move_u(S, j, 0, 0, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_UNIT);
+                } else if (w == W_POT) {
+                    int deg = S->adj_ptr[n + 1] - S->adj_ptr[n];
+                    int lo = bk == 0 ? 0 : (bk == 1 ? 4 : 8), hi = bk == 0 ? 4 : (bk == 1 ? 8 : 16);
+                    if (deg <= lo || deg > hi) continue;
+                    do_move_u(S, j, 1, 0, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_POT);
+                } else if (w == W_SPAIR) {
+                    int sc = (int)pick(s[1], (uint32_t)ESM_NSTOR);
+                    do_move_u(S, j, 3, sc, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_SPAIR);
+                } else {
+                    int kind, tid, fam;
+                    if (w == W_CELL) {
+                        double u = u01(s[0], s[1]);
+                        int lv = tb->nlev - 1;
+                        for (int q = tb->nlev - 1; q >= 0; q--)
+                            if (tb->qcum.a[q] >= u) lv = q;
+                        int nc = tb->lev_off.a[lv + 1] - tb->lev_off.a[lv];
+                        tid = tb->cl0 + tb->lev_off.a[lv] + (int)pick(s[2], (uint32_t)nc);
+                        kind = SP_CELL;
+                        fam = ESM_FAM_CELL;
+                    } else if (w == W_XPAIR) {
+                        int mm = (int)pick(s[1], (uint32_t)N);
+                        if (mm == n) continue;
+                        tid = tb->xp0 + n * N + mm;
+                        kind = SP_XPAIR;
+                        fam = ESM_FAM_XPAIR;
+                    } else {
+                        int np = tb->xp_off.a[n + 1] - tb->xp_off.a[n];
+                        if (np == 0) continue;
+                        tid = tb->xp_off.a[n] + (int)pick(s[1], (uint32_t)np);
+                        kind = w == W_XLOCC ? SP_XLOCC : SP_XLOC;
+                        fam = w == W_XLOCC ? ESM_FAM_XLOCC : ESM_FAM_XLOC;
+                    }
+                    uint32_t cm[4] = {(uint32_t)j, (uint32_t)tau, 0x60000000u | ((uint32_t)w << 20) | (uint32_t)k, ctr}, wm[4];
+                    orc_philox(key, cm, wm);
+                    sm_move(S, tb, j, tau, tid, kind, useg, uoff, u01(wm[0], wm[1]), maxnats, st + 5 * fam);
+                }
+            }
+    }
 }
 
 /* greedy distance-2 colouring in node order */
@@ -653,9 +1434,12 @@ int orc_colouring(const esm_model *m, int *colour) {
 }
 
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
-               const uint32_t key[2], uint32_t ctr0, double maxnats, int64_t *stats, int nthreads) {
+               const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
+               int64_t *stats, int nthreads) {
     int N = m->N, L = m->L, T = m->T;
     if (N > 256) return -1;
+    static const double wdef[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
+    const double *wts = weights ? weights : wdef;
     size_t NTR = (size_t)N * T * R;
     Sys S0;
     S0.m = m;
@@ -681,6 +1465,8 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
         S0.adj_line[fill[m->line_bus1[e]]++] = e;
     }
     int nc = colouring(m, S0.adj_ptr, S0.adj_line, colour);
+    Tables tb;
+    if (build_tables(m, &tb)) return -2;
     /* time-parity classes */
     int cfirst[3], ccount[3], ncls;
     if (T % 2 == 0) {
@@ -689,35 +1475,44 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
         ncls = 3; cfirst[0] = 0; ccount[0] = (T - 1) / 2; cfirst[1] = 1; ccount[1] = (T - 1) / 2;
         cfirst[2] = T - 1; ccount[2] = 1;
     }
-    int64_t st[20];
+    int *counts = (int *)malloc(sizeof(int) * ESM_NWEIGHT * T * (nsweeps > 0 ? nsweeps : 1));
+    for (int sw = 0; sw < nsweeps; sw++) move_counts(m, extra, wts, key, ctr0 + (uint32_t)sw, counts + (size_t)sw * ESM_NWEIGHT * T);
+    int64_t st[ESM_NFAM * ESM_NSTAT];
     memset(st, 0, sizeof(st));
     if (nthreads < 1) nthreads = 1;
     /* replicas are independent: threads over replicas, each with its own statistics */
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : st[:20])
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : st[:ESM_NFAM * ESM_NSTAT])
     for (int j = 0; j < R; j++) {
         Sys S = S0;
         for (int sw = 0; sw < nsweeps; sw++) {
             uint32_t ctr = ctr0 + (uint32_t)sw;
-            for (int n = 0; n < N; n++)
-                for (int t = 0; t < T; t++) do_move(&S, j, 0, 0, n, t, key, 0u, ctr, maxnats, st + 0);
-            for (int c = 0; c < nc; c++)
-                for (int n = 0; n < N; n++) {
-                    if (colour[n] != c || S.adj_ptr[n + 1] == S.adj_ptr[n]) continue;
-                    for (int t = 0; t < T; t++) do_move(&S, j, 1, 0, n, t, key, 1000u + (uint32_t)c, ctr, maxnats, st + 5);
-                }
-            for (int fam = 2; fam <= 3; fam++)
+            if (atoms) {
+                for (int n = 0; n < N; n++)
+                    for (int t = 0; t < T; t++) do_move(&S, j, 0, 0, n, t, key, 0u, ctr, maxnats, st + 5 * ESM_FAM_GEN);
+                for (int c = 0; c < nc; c++)
+                    for (int n = 0; n < N; n++) {
+                        if (colour[n] != c || S.adj_ptr[n + 1] == S.adj_ptr[n]) continue;
+                        for (int t = 0; t < T; t++)
+                            do_move(&S, j, 1, 0, n, t, key, 1000u + (uint32_t)c, ctr, maxnats, st + 5 * ESM_FAM_THETA);
+                    }
                 for (int s = 0; s < ESM_NSTOR; s++)
                     for (int c = 0; c < ncls; c++)
                         for (int n = 0; n < N; n++)
                             for (int q = 0; q < ccount[c]; q++) {
                                 int t = cfirst[c] + 2 * q;
-                                do_move(&S, j, fam, s, n, t, key, (fam == 2 ? 2000u : 3000u) + (uint32_t)(s * 4 + c),
-                                        ctr, maxnats, st + 5 * fam);
+                                do_move(&S, j, 2, s, n, t, key, 2000u + (uint32_t)(s * 4 + c), ctr, maxnats,
+                                        st + 5 * ESM_FAM_SOC);
                             }
+            }
+            const int *cnt = counts + (size_t)sw * ESM_NWEIGHT * T;
+            for (int c = 0; c < ncls; c++)
+                for (int q = 0; q < ccount[c]; q++)
+                    random_step(&S, &tb, j, cfirst[c] + 2 * q, cnt, wts, key, ctr, maxnats, st);
         }
     }
     if (stats) memcpy(stats, st, sizeof(st));
-    free(S0.adj_ptr); free(S0.adj_line); free(fill); free(colour);
+    free(S0.adj_ptr); free(S0.adj_line); free(fill); free(colour); free(counts);
+    tables_free(&tb);
     return 0;
 }
 
@@ -819,3 +1614,118 @@ int orc_move_apply(const esm_model *m, int R, double *v, const double *k, double
 
 double orc_logseg(double L, double x) { return logseg(L, x); }
 double orc_trexp_offset(double g, double L, double u) { return trexp_offset(g, L, u); }
+
+/* ---- explicit-uniform table-driven moves and the template layout, for the pins ---- */
+
+/* info [8]: templates, first xpair, first cell, levels, then the cells of levels 0, 1, 2 (0
+ * beyond); xp_off [N + 1] (may be NULL): the xloc templates of each node */
+int orc_tables_info(const esm_model *m, int *info, int *xp_off) {
+    Tables tb;
+    if (build_tables(m, &tb)) return -1;
+    info[0] = tb.ntpl;
+    info[1] = tb.xp0;
+    info[2] = tb.cl0;
+    info[3] = tb.nlev;
+    for (int q = 0; q < 3; q++) info[4 + q] = q < tb.nlev ? tb.lev_off.a[q + 1] - tb.lev_off.a[q] : 0;
+    info[7] = 0;
+    if (xp_off) memcpy(xp_off, tb.xp_off.a, sizeof(int) * (m->N + 1));
+    tables_free(&tb);
+    return 0;
+}
+
+/* the touched nodes and leak nodes of template tid: nodes [≤ N] and kink nodes with their g */
+int orc_template_nodes(const esm_model *m, int tid, int *nodes, int *nn, int *kk, double *kg, int *nk) {
+    Tables tb;
+    if (build_tables(m, &tb)) return -1;
+    if (tid < 0 || tid >= tb.ntpl) { tables_free(&tb); return -2; }
+    *nn = 0;
+    for (int p = tb.nd_off.a[tid]; p < tb.nd_off.a[tid + 1]; p++) nodes[(*nn)++] = tb.nd_node.a[p];
+    *nk = 0;
+    for (int q = tb.kk_off.a[tid]; q < tb.kk_off.a[tid + 1]; q++) {
+        kk[*nk] = tb.kk_node.a[q];
+        kg[(*nk)++] = tb.kk_g.a[q];
+    }
+    tables_free(&tb);
+    return 0;
+}
+
+static int sparse_setup(Sys *S, const esm_model *m, int R, double *v, const double *k, double *r, double *F, int *ptr,
+                        int *lines, Tables *tb) {
+    if (m->N > 256 || m->L > 2048) return -1;
+    sys_view(S, m, R, v, k, r, F, ptr, lines);
+    if (build_tables(m, tb)) return -2;
+    return 0;
+}
+
+/* draw of template tid (kind: 0 cell, 1 xloc, 2 xlocc, 3 xpair) at (τ, replica j) with given
+ * uniforms, no commit; t = 0 and status 3 when the move is not valid */
+int orc_sparse_draw(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                    int tid, int tau, double maxnats, double useg, double uoff, double *t_out, int *status) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    Tables tb;
+    int rc = sparse_setup(&S, m, R, v, k, r, F, ptr, lines, &tb);
+    if (rc) return rc;
+    SMove f;
+    sm_init(&f, &S, &tb, j, tau, tid, kind == SP_XLOCC);
+    if (!sm_valid(&f, kind)) {
+        *t_out = 0.0;
+        *status = 3;
+    } else {
+        *t_out = sm_sample(&f, m->beta_v, maxnats, useg, uoff, status);
+    }
+    tables_free(&tb);
+    return 0;
+}
+
+int orc_sparse_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                    int tid, int tau, const double *t, int nt, double *dc) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    Tables tb;
+    int rc = sparse_setup(&S, m, R, v, k, r, F, ptr, lines, &tb);
+    if (rc) return rc;
+    SMove f;
+    sm_init(&f, &S, &tb, j, tau, tid, kind == SP_XLOCC);
+    for (int q = 0; q < nt; q++) dc[q] = sm_eval(&f, t[q]);
+    tables_free(&tb);
+    return 0;
+}
+
+int orc_sparse_apply(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                     int tid, int tau, double t) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    Tables tb;
+    int rc = sparse_setup(&S, m, R, v, k, r, F, ptr, lines, &tb);
+    if (rc) return rc;
+    SMove f;
+    sm_init(&f, &S, &tb, j, tau, tid, kind == SP_XLOCC);
+    sm_commit(&f, t);
+    tables_free(&tb);
+    return 0;
+}
+
+/* nd draws of one table-driven move with given uniform pairs, the tables built once, no commit */
+int orc_sparse_draw_many(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int kind,
+                         int tid, int tau, double maxnats, const double *useg, const double *uoff, int nd,
+                         double *t_out, int *status) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    Tables tb;
+    int rc = sparse_setup(&S, m, R, v, k, r, F, ptr, lines, &tb);
+    if (rc) return rc;
+    SMove f;
+    sm_init(&f, &S, &tb, j, tau, tid, kind == SP_XLOCC);
+    int ok = sm_valid(&f, kind);
+    for (int q = 0; q < nd; q++) {
+        if (!ok) {
+            t_out[q] = 0.0;
+            status[q] = 3;
+            continue;
+        }
+        t_out[q] = sm_sample(&f, m->beta_v, maxnats, useg[q], uoff[q], &status[q]);
+    }
+    tables_free(&tb);
+    return 0;
+}
