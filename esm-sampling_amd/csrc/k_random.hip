@@ -52,6 +52,18 @@ struct RandArgs {
 
 __device__ __forceinline__ uint32_t pick(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
 
+// With UNI (the replica count a multiple of 64, so every wave holds the replicas of one step) the
+// step, the move counts and the selection words are the same in all lanes of a wave; saying so
+// lets the template and model-table reads become scalar loads.
+template <bool UNI>
+__device__ __forceinline__ int uniform(int x) { return UNI ? __builtin_amdgcn_readfirstlane(x) : x; }
+template <bool UNI>
+__device__ __forceinline__ void uniform4(uint32_t *s) {
+    if (UNI)
+#pragma unroll
+        for (int i = 0; i < 4; i++) s[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)s[i]);
+}
+
 // the selection words of move `k` of family `w` at step τ
 __device__ __forceinline__ void selection(const RandArgs &a, int k, int tau, uint32_t *s) {
     philox4x32(a.k0, a.k1, 0x80000000u | ((uint32_t)a.w << 20) | (uint32_t)k, (uint32_t)tau, 9001u, a.ctr, s);
@@ -87,17 +99,18 @@ __global__ void k_movecounts(int T, int nextra, CumW cw, uint32_t k0, uint32_t k
 // unit, potential, spair: the atom sites at a drawn node
 // ---------------------------------------------------------------------------------------------
 
-template <typename S>
+template <typename S, bool UNI>
 __global__ void __launch_bounds__(256) k_rand_unit(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
     if (g < (size_t)a.ntau * m.R) {
         const int j = (int)(g % m.R);
-        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
-        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
+        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
         for (int k = 0; k < cnt; k++) {
             uint32_t s[4];
             selection(a, k, tau, s);
+            uniform4<UNI>(s);
             const int n = (int)pick(s[0], (uint32_t)m.N);
             double u1, u2;
             line_uniforms(a, j, k, tau, u1, u2);
@@ -110,17 +123,18 @@ __global__ void __launch_bounds__(256) k_rand_unit(DevModel<S> m, SweepState<S> 
 // potential moves whose node degree falls in the bucket (DLO, D]: one kernel per bucket keeps
 // the register footprint of its degree; a step's potential moves run bucket by bucket, each
 // bucket in draw order
-template <typename S, int DLO, int D>
+template <typename S, int DLO, int D, bool UNI>
 __global__ void __launch_bounds__(256) k_rand_pot(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
     if (g < (size_t)a.ntau * m.R) {
         const int j = (int)(g % m.R);
-        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
-        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
+        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
         for (int k = 0; k < cnt; k++) {
             uint32_t s[4];
             selection(a, k, tau, s);
+            uniform4<UNI>(s);
             const int n = (int)pick(s[0], (uint32_t)m.N);
             const int deg = m.adj_ptr[n + 1] - m.adj_ptr[n];
             if (deg <= DLO || deg > D) continue;   // another bucket (degree 0: no angle direction)
@@ -132,18 +146,19 @@ __global__ void __launch_bounds__(256) k_rand_pot(DevModel<S> m, SweepState<S> s
     add_counts(a.stats, out);
 }
 
-template <typename S>
+template <typename S, bool UNI>
 __global__ void __launch_bounds__(256) k_rand_spair(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats,
                                                     S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
     if (g < (size_t)a.ntau * m.R) {
         const int j = (int)(g % m.R);
-        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
-        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
+        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
         for (int k = 0; k < cnt; k++) {
             uint32_t s[4];
             selection(a, k, tau, s);
+            uniform4<UNI>(s);
             const int n = (int)pick(s[0], (uint32_t)m.N);
             const int sc = (int)pick(s[1], (uint32_t)ESM_NSTOR);   // the node's storage units: battery, H2
             double u1, u2;
@@ -243,6 +258,149 @@ struct FSparse {
             dr += tb->nl_sgn[q] > 0 ? d : -d;
         }
         return rv(n) + dr;
+    }
+
+    // balance of touched node p at ta and at tb (one read of the node's values)
+    __device__ void node_S2(int p, S ta, S tb, S &A, S &B) const {
+        const int n = tb_()->nd_node[p];
+        S da = 0, db = 0;
+        const int ie = tb_()->nd_ent[p];
+        if (ie >= 0) {
+            const S a = slope(e0 + ie);
+            if (a != S(0)) {
+                const S y0 = ygv(n);
+                S c0, p0, c1, p1, c2, p2;
+                gen(n, y0, c0, p0);
+                gen(n, y0 + ta * a, c1, p1);
+                gen(n, y0 + tb * a, c2, p2);
+                da -= p1 - p0;
+                db -= p2 - p0;
+            }
+        }
+        const int ik = tb_()->nd_kk[p];
+        if (comp && ik >= 0) {
+            S Q, g, ak, P, e, y0;
+            if (kink(q0 + ik, Q, g, ak, P, e, y0) && ak != S(0)) {
+                S c0, p0, c1, p1, c2, p2;
+                gen(n, y0, c0, p0);
+                gen(n, y0 + ak * kphi(Q, g, ta), c1, p1);
+                gen(n, y0 + ak * kphi(Q, g, tb), c2, p2);
+                da -= p1 - p0;
+                db -= p2 - p0;
+            }
+        }
+        for (int q = tb_()->nd_l[p]; q < tb_()->nd_l[p + 1]; q++) {
+            const int pos = l0 + tb_()->nl_pos[q];
+            const S f0 = Fv(tb_()->ln_line[pos]), df = tb_()->ln_df[pos];
+            const S d1 = (f0 + ta * df) - f0, d2 = (f0 + tb * df) - f0;
+            const bool b0 = tb_()->nl_sgn[q] > 0;
+            da += b0 ? d1 : -d1;
+            db += b0 ? d2 : -d2;
+        }
+        const S S0 = rv(n);
+        A = S0 + da;
+        B = S0 + db;
+    }
+    __device__ const Tables<S> *tb_() const { return tb; }
+
+    // One pass at a primary point t reached from tp: ΔC(t); the sign changes of the node balances
+    // strictly inside (tp, t) in walking order (at most KR kept; `more` when some were left out);
+    // the nearest primary breakpoint beyond t. With nodes = false only the breakpoint.
+    static constexpr int KR = 4;
+    struct Pass {
+        S c = 0;
+        int nr = 0;
+        S rt[KR];
+        bool more = false, hb = false;
+        S nb = 0;
+    };
+    __device__ void pass(S tp, S t, S sgn, bool nodes, Pass &P) const {
+        if (nodes) nev++;
+        S D = 0;
+        auto cand = [&](S b) {
+            if (isfinite(b) && sgn * (b - t) > 0 && (!P.hb || sgn * (P.nb - b) > 0)) {
+                P.nb = b;
+                P.hb = true;
+            }
+        };
+        for (int e = e0; e < e1; e++) {
+            const S a = slope(e);
+            if (a == S(0)) continue;
+            const int n = tb->en_node[e];
+            const S y0 = ygv(n);
+            if (nodes) {
+                S c0, p0, c1, p1;
+                gen(n, y0, c0, p0);
+                gen(n, y0 + t * a, c1, p1);
+                D += c1 - c0;
+            }
+            cand((S(0) - y0) / a);
+            cand((S(1) - y0) / a);
+        }
+        if (comp)
+            for (int q = q0; q < q1; q++) {
+                S Q, g, ak, P_, e, y0;
+                if (!kink(q, Q, g, ak, P_, e, y0)) continue;
+                cand(-Q / g);
+                if (ak != S(0)) {
+                    if (nodes) {
+                        const int n = tb->kk_node[q];
+                        S c0, p0, c1, p1;
+                        gen(n, y0, c0, p0);
+                        gen(n, y0 + ak * kphi(Q, g, t), c1, p1);
+                        D += c1 - c0;
+                    }
+                    const S sl = ak * g, off = ak * Q;
+                    cand((S(0) - y0 - S(0)) / sl);
+                    cand((S(1) - y0 - S(0)) / sl);
+                    cand((S(0) - y0 - off) / sl);
+                    cand((S(1) - y0 - off) / sl);
+                }
+            }
+        for (int l = l0; l < l1; l++) {
+            const int e = tb->ln_line[l];
+            const S f0 = Fv(e), se = m->ls[e], df = tb->ln_df[l];
+            if (nodes) {
+                const S f1 = f0 + t * df;
+                D += m->cv * m->dt * (fmax(fabs(f1) - se, S(0)) - fmax(fabs(f0) - se, S(0)));
+            }
+            cand((se - f0) / df);
+            cand((-se - f0) / df);
+        }
+        if (nodes)
+            for (int p = n0; p < n1; p++) {
+                S A, B;
+                node_S2(p, tp, t, A, B);
+                const S S0 = rv(tb->nd_node[p]);
+                D += m->dt * (phibal(B, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+                if ((A < 0) != (B < 0) && A != B) {
+                    S x = tp + (t - tp) * A / (A - B);
+                    if (sgn * (x - tp) > 0 && sgn * (t - x) > 0) {
+                        bool dup = false;
+#pragma unroll
+                        for (int i = 0; i < KR; i++) dup = dup || (i < P.nr && P.rt[i] == x);
+                        if (!dup) {
+                            bool carry = true;
+#pragma unroll
+                            for (int i = 0; i < KR; i++) {
+                                if (i < P.nr) {
+                                    if (sgn * (x - P.rt[i]) < 0) {
+                                        const S tmp = P.rt[i];
+                                        P.rt[i] = x;
+                                        x = tmp;
+                                    }
+                                } else if (i == P.nr && carry) {
+                                    P.rt[i] = x;
+                                    carry = false;
+                                }
+                            }
+                            if (carry) P.more = true;
+                            else P.nr++;
+                        }
+                    }
+                }
+            }
+        P.c = D;
     }
 
     // ΔC(t)
@@ -353,14 +511,22 @@ struct FSparse {
     }
 };
 
-// walk! of a table-driven move: 1 proper, 0 improper, -1 point cap, 2 stopped by the visitor
+// walk! of a table-driven move: 1 proper, 0 improper, -1 point cap, 2 stopped by the visitor.
+// One pass per primary point (cost, roots of the segment, next breakpoint); roots beyond the
+// KR kept by a pass are found by scans.
 template <typename S, class V>
 __device__ int walk_sparse(const FSparse<S> &f, S beta, S maxnats, S sgn, V &vis) {
     S cmin = 0, tp = 0, delta = 0, clast = 0;
     int nx = 0, np = 0;
+    typename FSparse<S>::Pass P0;
+    f.pass(S(0), S(0), sgn, false, P0);
+    bool hb = P0.hb;
+    S nb = P0.nb;
     while (true) {
         S t = 0;
-        if (!f.next_bp(tp, sgn, t)) {
+        if (hb) {
+            t = nb;
+        } else {
             const S hp = S(1e-6) * fmax(S(1), fabs(tp));
             const S sl = (beta * f.eval(tp + sgn * hp) - clast) / hp;
             if (sl > 0) delta = fmin(2 * maxnats / sl, S(1e3) * fmax(fabs(tp), S(1e-3)));
@@ -369,12 +535,22 @@ __device__ int walk_sparse(const FSparse<S> &f, S beta, S maxnats, S sgn, V &vis
             if (++nx > 80) return 0;
             if (!isfinite(t)) return 0;
         }
-        const S c = beta * f.eval(t);
-        S x = tp, xr = 0;
+        typename FSparse<S>::Pass P;
+        f.pass(tp, t, sgn, true, P);
+        const S c = beta * P.c;
+        hb = P.hb;
+        nb = P.nb;
         int w;
-        while (f.next_root(tp, t, x, sgn, xr)) {
-            if ((w = visit_point(vis, xr, beta * f.eval(xr), cmin, np))) return w;
-            x = xr;
+#pragma unroll
+        for (int i = 0; i < FSparse<S>::KR; i++)
+            if (i < P.nr)
+                if ((w = visit_point(vis, P.rt[i], beta * f.eval(P.rt[i]), cmin, np))) return w;
+        if (P.more) {
+            S x = P.rt[FSparse<S>::KR - 1], xr = 0;
+            while (f.next_root(tp, t, x, sgn, xr)) {
+                if ((w = visit_point(vis, xr, beta * f.eval(xr), cmin, np))) return w;
+                x = xr;
+            }
         }
         if ((w = visit_point(vis, t, c, cmin, np))) return w;
         clast = c;
@@ -383,28 +559,50 @@ __device__ int walk_sparse(const FSparse<S> &f, S beta, S maxnats, S sgn, V &vis
     }
 }
 
+// the two visitors of the sampler in one type (mode 0: segment masses of one side; mode 1: the
+// segment of the target mass), so that the three walks share one call site
+template <typename S>
+struct Vis2 {
+    int mode = 0;
+    MassAcc<S> ma;
+    SegSel<S> ss;
+    __device__ bool operator()(S t, S c) { return mode == 0 ? ma(t, c) : ss(t, c); }
+};
+
 template <typename S>
 __device__ S sample_sparse(const FSparse<S> &f, S beta, S maxnats, S useg, S uoff, int &status) {
-    MassAcc<S> mr, ml;
-    int w = walk_sparse(f, beta, maxnats, S(1), mr);
-    if (w != 1) {
-        status = w == 0 ? 1 : 2;
-        return 0;
+    Vis2<S> vis;
+    S Mx[2] = {S(-INFINITY), S(-INFINITY)}, Ms[2] = {S(0), S(0)};
+    S sgn = 1;
+#pragma nounroll
+    for (int k = 0; k < 3; k++) {
+        if (k == 2) {
+            const S MX = fmax(Mx[0], Mx[1]);
+            const S Mr = Ms[0] * exp(Mx[0] - MX), Ml = Ms[1] * exp(Mx[1] - MX);
+            const S u = useg * (Ml + Mr);
+            vis.mode = 1;
+            vis.ss = SegSel<S>();
+            vis.ss.MX = MX;
+            vis.ss.left = u <= Ml;
+            vis.ss.target = vis.ss.left ? Ml - u : u - Ml;
+            sgn = vis.ss.left ? S(-1) : S(1);
+        } else {
+            vis.mode = 0;
+            vis.ma = MassAcc<S>();
+            sgn = k == 0 ? S(1) : S(-1);
+        }
+        const int w = walk_sparse(f, beta, maxnats, sgn, vis);
+        if (k < 2) {
+            if (w != 1) {
+                status = w == 0 ? 1 : 2;
+                return 0;
+            }
+            Mx[k] = vis.ma.mx;
+            Ms[k] = vis.ma.s;
+        }
     }
-    w = walk_sparse(f, beta, maxnats, S(-1), ml);
-    if (w != 1) {
-        status = w == 0 ? 1 : 2;
-        return 0;
-    }
-    const S MX = fmax(ml.mx, mr.mx);
-    const S Ml = ml.s * exp(ml.mx - MX), Mr = mr.s * exp(mr.mx - MX);
-    const S u = useg * (Ml + Mr);
-    SegSel<S> sel;
-    sel.MX = MX;
-    sel.left = u <= Ml;
-    sel.target = sel.left ? Ml - u : u - Ml;
-    walk_sparse(f, beta, maxnats, sel.left ? S(-1) : S(1), sel);
     status = 0;
+    const SegSel<S> &sel = vis.ss;
     const S L = sel.tb - sel.ta;
     const S g = (sel.cb - sel.ca) / L;
     const S t = (g >= 0 || fabs(g * L) < S(1e-10)) ? sel.ta + trexp_offset(g, L, uoff) : sel.tb - trexp_offset(-g, L, uoff);
@@ -417,20 +615,21 @@ __device__ S sample_sparse(const FSparse<S> &f, S beta, S maxnats, S useg, S uof
 
 enum { SP_CELL = 0, SP_XLOC = 1, SP_XLOCC = 2, SP_XPAIR = 3 };
 
-template <typename S, int KIND>
+template <typename S, int KIND, bool UNI>
 __global__ void __launch_bounds__(256) k_sparse(DevModel<S> m, SweepState<S> st, RandArgs a, Tables<S> tb, S beta,
                                                 S maxnats, S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
     if (g < (size_t)a.ntau * m.R) {
         const int j = (int)(g % m.R);
-        const int tau = a.tau0 + (int)(g / m.R) * a.tstride;
-        const int cnt = a.counts[(size_t)a.w * m.T + tau];
+        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
+        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
         const size_t TR = (size_t)m.T * m.R, o = (size_t)tau * m.R + j;
         S *yg = st.v + plane(m.N, m.T, m.R);
         for (int k = 0; k < cnt; k++) {
             uint32_t s[4];
             selection(a, k, tau, s);
+            uniform4<UNI>(s);
             int tid;
             if (KIND == SP_CELL) {
                 const double u = u01(s[0], s[1]);
@@ -559,6 +758,12 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                  d.xp0, d.cl0, d.nlev};
     // the families in the order of the weight list; pair and cellpair are void here
     const int order[7] = {W_UNIT, W_XPAIR, W_XLOCC, W_XLOC, W_POT, W_CELL, W_SPAIR};
+    const bool uni = R % 64 == 0;
+#define RLAUNCH(KU, KD, ...)                                                                           \
+    do {                                                                                               \
+        if (uni) hipLaunchKernelGGL(KU, dim3(nb), dim3(256), 0, h->stream, __VA_ARGS__);                \
+        else hipLaunchKernelGGL(KD, dim3(nb), dim3(256), 0, h->stream, __VA_ARGS__);                    \
+    } while (0)
     for (size_t c = 0; c < h->tclass.size(); c++) {
         RandArgs a{k0, k1, ctr, h->tclass[c].first, 2, h->tclass[c].second, 0, h->d_counts, nullptr};
         const unsigned nb = nblk((size_t)a.ntau * R, 256);
@@ -570,7 +775,7 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 case W_UNIT: {
                     a.stats = stats(ESM_FAM_UNIT);
                     ProfScope p(h, ESM_KERN_UNIT);
-                    hipLaunchKernelGGL((k_rand_unit<S>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                    RLAUNCH((k_rand_unit<S, true>), (k_rand_unit<S, false>), m, st, a, beta, mxn);
                     break;
                 }
                 case W_POT: {
@@ -578,45 +783,41 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                     ProfScope p(h, ESM_KERN_POT);
                     int dmax = 0;
                     for (int n = 0; n < N; n++) dmax = std::max(dmax, h->adj_ptr[n + 1] - h->adj_ptr[n]);
-                    hipLaunchKernelGGL((k_rand_pot<S, 0, 4>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                    RLAUNCH((k_rand_pot<S, 0, 4, true>), (k_rand_pot<S, 0, 4, false>), m, st, a, beta, mxn);
                     if (dmax > 4)
-                        hipLaunchKernelGGL((k_rand_pot<S, 4, 8>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                        RLAUNCH((k_rand_pot<S, 4, 8, true>), (k_rand_pot<S, 4, 8, false>), m, st, a, beta, mxn);
                     if (dmax > 8)
-                        hipLaunchKernelGGL((k_rand_pot<S, 8, 16>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn);
+                        RLAUNCH((k_rand_pot<S, 8, 16, true>), (k_rand_pot<S, 8, 16, false>), m, st, a, beta, mxn);
                     break;
                 }
                 case W_SPAIR: {
                     a.stats = stats(ESM_FAM_SPAIR);
                     ProfScope p(h, ESM_KERN_SPAIR);
-                    hipLaunchKernelGGL((k_rand_spair<S>), dim3(nb), dim3(256), 0, h->stream, m, st, a, beta, mxn, kmin);
+                    RLAUNCH((k_rand_spair<S, true>), (k_rand_spair<S, false>), m, st, a, beta, mxn, kmin);
                     break;
                 }
                 case W_CELL: {
                     a.stats = stats(ESM_FAM_CELL);
                     ProfScope p(h, ESM_KERN_CELL);
-                    hipLaunchKernelGGL((k_sparse<S, SP_CELL>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
-                                       mxn, kmin);
+                    RLAUNCH((k_sparse<S, SP_CELL, true>), (k_sparse<S, SP_CELL, false>), m, st, a, tb, beta, mxn, kmin);
                     break;
                 }
                 case W_XLOC: {
                     a.stats = stats(ESM_FAM_XLOC);
                     ProfScope p(h, ESM_KERN_XLOC);
-                    hipLaunchKernelGGL((k_sparse<S, SP_XLOC>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
-                                       mxn, kmin);
+                    RLAUNCH((k_sparse<S, SP_XLOC, true>), (k_sparse<S, SP_XLOC, false>), m, st, a, tb, beta, mxn, kmin);
                     break;
                 }
                 case W_XLOCC: {
                     a.stats = stats(ESM_FAM_XLOCC);
                     ProfScope p(h, ESM_KERN_XLOCC);
-                    hipLaunchKernelGGL((k_sparse<S, SP_XLOCC>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
-                                       mxn, kmin);
+                    RLAUNCH((k_sparse<S, SP_XLOCC, true>), (k_sparse<S, SP_XLOCC, false>), m, st, a, tb, beta, mxn, kmin);
                     break;
                 }
                 case W_XPAIR: {
                     a.stats = stats(ESM_FAM_XPAIR);
                     ProfScope p(h, ESM_KERN_XPAIR);
-                    hipLaunchKernelGGL((k_sparse<S, SP_XPAIR>), dim3(nb), dim3(256), 0, h->stream, m, st, a, tb, beta,
-                                       mxn, kmin);
+                    RLAUNCH((k_sparse<S, SP_XPAIR, true>), (k_sparse<S, SP_XPAIR, false>), m, st, a, tb, beta, mxn, kmin);
                     break;
                 }
                 default: break;
@@ -626,6 +827,8 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     }
     return 0;
 }
+
+#undef RLAUNCH
 
 template int launch_random<float>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
 template int launch_random<double>(esm_handle *, uint32_t, uint32_t, uint32_t, double);

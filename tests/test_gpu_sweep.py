@@ -137,3 +137,26 @@ def test_full_sweep_fp64_matches_oracle_40_nodes():
     np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
     np.testing.assert_allclose(vd[0], v_[0], atol=1e-9 * np.abs(v_[0]).max())
     dev.close()
+
+
+def test_sweep_fp64_matches_oracle_64_replicas():
+    """64 replicas: every wave holds one step's replicas (the wave-uniform kernels)."""
+    print("full sweep fp64 64 replicas", flush=True)
+    s = synth.make_system(N=10, L=16, T=10, dt=1.0, seed=6)
+    R = 64
+    rng = np.random.default_rng(8)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    v[0] = rng.normal(0, 300.0, (s.N, s.T, R))
+    dev = api.Replicas(s, R, FP64)
+    dev.upload(v=v, k=s.k0, x=x)
+    st = api.operation_sweep(dev, 2, key=(5, 6), counter=3)
+    vd, _, _ = dev.download()
+    rd, Fd = dev.caches()
+    v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 2, key=(5, 6), counter=3, x=x, threads=8)
+    assert np.all(st_[:, 0] > 0)
+    assert np.array_equal(st[:, :4], st_[:, :4])
+    np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
+    np.testing.assert_allclose(vd[0], v_[0], atol=1e-9 * np.abs(v_[0]).max())
+    np.testing.assert_allclose(rd, r_, atol=1e-9 * np.abs(s.load).max())
+    dev.close()
