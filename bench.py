@@ -153,7 +153,7 @@ def main():
              _lib.KERN_GEN: "k_gen", _lib.KERN_THETA: "k_theta", _lib.KERN_SOC: "k_soc", _lib.KERN_SPAIR: "k_spair",
              _lib.KERN_FOLLOW: "k_follow", _lib.KERN_SWEEP: "sweep", _lib.KERN_UNIT: "k_unit",
              _lib.KERN_POT: "k_pot", _lib.KERN_CELL: "k_cell", _lib.KERN_XLOC: "k_xloc", _lib.KERN_XLOCC: "k_xlocc",
-             _lib.KERN_XPAIR: "k_xpair"}
+             _lib.KERN_XPAIR: "k_xpair", _lib.KERN_SMS: "k_storage_ms"}
     prof = {}
     for kid, name in names.items():
         tot, cnt = C.c_double(), C.c_int64()

@@ -138,7 +138,8 @@ static int upload_move_tables(esm_handle *h) {
     const std::vector<const std::vector<int> *> iv = {&mt.th_off, &mt.th_node, &mt.en_off, &mt.en_node, &mt.en_div,
                                                       &mt.kk_off, &mt.kk_node, &mt.ln_off, &mt.ln_line, &mt.nd_off,
                                                       &mt.nd_node, &mt.nd_ent, &mt.nd_kk, &mt.nd_l, &mt.nl_pos,
-                                                      &mt.nl_sgn, &mt.xp_off, &mt.lev_off};
+                                                      &mt.nl_sgn, &mt.xp_off, &mt.lev_off, &mt.sms_units,
+                                                      &mt.grp_off, &mt.grp_node, &mt.grp_si};
     std::vector<int> ib;
     std::vector<size_t> io;
     for (auto *v : iv) {
@@ -162,7 +163,8 @@ static int upload_move_tables(esm_handle *h) {
     ESM_HIP_CHECK(hipMemcpy(d.qcum, mt.qcum.data(), mt.qcum.size() * sizeof(double), hipMemcpyHostToDevice));
     int **dst[] = {&d.th_off, &d.th_node, &d.en_off, &d.en_node, &d.en_div, &d.kk_off, &d.kk_node,
                                &d.ln_off, &d.ln_line, &d.nd_off, &d.nd_node, &d.nd_ent, &d.nd_kk, &d.nd_l,
-                               &d.nl_pos, &d.nl_sgn, &d.xp_off, &d.lev_off};
+                               &d.nl_pos, &d.nl_sgn, &d.xp_off, &d.lev_off, &d.sms_units, &d.grp_off,
+                               &d.grp_node, &d.grp_si};
     for (size_t q = 0; q < iv.size(); q++) *dst[q] = d.ibuf + io[q];
     const S *vbase = (const S *)d.vbuf;
     d.th_val = vbase + vo[0];
@@ -650,7 +652,7 @@ int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32
     return ESM_OK;
 }
 
-int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, int32_t atoms) {
+int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, int32_t atoms, double p_storage_ms) {
     if (!h) { set_error("esm_sweep_options: null handle"); return ESM_EINVAL; }
     if (!(extra >= 0) || extra > 1e4) { set_error("esm_sweep_options: extra must be in [0, 1e4]"); return ESM_EINVAL; }
     static const double wdef[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
@@ -661,9 +663,11 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
         tot += w[q];
     }
     if (extra > 0 && !(tot > 0)) { set_error("esm_sweep_options: all weights are zero"); return ESM_EINVAL; }
+    if (!(p_storage_ms >= 0 && p_storage_ms <= 1)) { set_error("esm_sweep_options: p_storage_ms must be in [0, 1]"); return ESM_EINVAL; }
     h->sweep_extra = extra;
     for (int q = 0; q < ESM_NWEIGHT; q++) h->sweep_w[q] = w[q];
     h->sweep_atoms = atoms != 0;
+    h->sweep_pms = p_storage_ms;
     return ESM_OK;
 }
 

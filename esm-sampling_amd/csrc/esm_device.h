@@ -82,13 +82,18 @@ __device__ __forceinline__ S soc_cost(const DevModel<S> &m, int s, int n, S k, S
     return c;
 }
 
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).
-__device__ __forceinline__ void philox4x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2,
-                                           uint32_t c3, uint32_t out[4]) {
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
+    return (uint32_t)(((uint64_t)a * b) >> 32);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11); also used by the host for the per-sweep
+// schedule draws.
+__host__ __device__ __forceinline__ void philox4x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2,
+                                                    uint32_t c3, uint32_t out[4]) {
 #pragma unroll
     for (int i = 0; i < 10; i++) {
-        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = mulhi32(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = mulhi32(0xCD9E8D57u, c2);
         const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
         c1 = lo1;
@@ -104,7 +109,7 @@ __device__ __forceinline__ void philox4x32(uint32_t k0, uint32_t k1, uint32_t c0
 }
 
 // Uniform in [0, 1) from two 32-bit words (53 bits).
-__device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) {
+__host__ __device__ __forceinline__ double u01(uint32_t hi, uint32_t lo) {
     const uint64_t x = ((uint64_t)hi << 32) | lo;
     return (double)(x >> 11) * 0x1.0p-53;
 }

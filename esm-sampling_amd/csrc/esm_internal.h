@@ -57,6 +57,9 @@ struct MoveTablesHost {
         nd_ent, nd_kk, nd_l, nl_pos, nl_sgn, xp_off, lev_off;
     std::vector<double> th_val, en_c, kk_g, ln_df, qcum;
     int xp0 = 0, cl0 = 0, ntpl = 0, nlev = 0;
+    // storage multiscale moves: every storage unit (carrier·N + node, carrier-major) and the
+    // country–carrier groups (sorted by country, then carrier; nodes increasing)
+    std::vector<int> sms_units, grp_off, grp_node, grp_si;
 };
 
 struct MoveTablesDev {
@@ -69,6 +72,7 @@ struct MoveTablesDev {
               *nl_sgn = nullptr, *xp_off = nullptr, *lev_off = nullptr;
     const void *th_val = nullptr, *en_c = nullptr, *kk_g = nullptr, *ln_df = nullptr;
     int xp0 = 0, cl0 = 0, nlev = 0;
+    int *sms_units = nullptr, *grp_off = nullptr, *grp_node = nullptr, *grp_si = nullptr;
 };
 
 // Build the templates from the model (host, FP64); false with a message on a bad network.
@@ -141,6 +145,9 @@ struct esm_handle {
     bool sweep_atoms = true;
     double sweep_extra = 4.0;
     double sweep_w[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
+    double sweep_pms = 0.5;      // probability of the storage multiscale sweep per operation sweep
+    int sweep_ms_moves = 8;      // moves per block
+    double sweep_ms_pagg = 0.5;  // probability of a country–carrier group
     int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step
     esm::MoveTablesHost mt;                       // templates (host, FP64)
     esm::MoveTablesDev dmt;                       // the same on the device (values in the state precision)
@@ -192,6 +199,10 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
 // k_random.hip: the random families of one sweep (counter ctr), class by class
 template <typename S>
 int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
+
+// k_storage.hip: the storage multiscale sweep (with its probability) of one operation sweep
+template <typename S>
+int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
 
 // k_follow.hip
 template <typename S>

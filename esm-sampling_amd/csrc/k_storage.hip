@@ -1,0 +1,579 @@
+// k_storage.hip — the storage multiscale sweep (`storage_ms_sweep!`, src/flowmoves.jl:255-324;
+// moves `prep_storage_ms!`, :187-244), which `operation_sweep!` runs after the local sweep with
+// probability 1/2 (src/local.jl:2414-2418).
+//
+// For every dyadic block length L < T/2 (2, 4, …, 128; random order, random offset o, blocks
+// o + iL without wrap, the two parity classes of blocks in random order), 8 moves per block: a
+// plateau on the block for one storage unit or, with probability 1/2, for all units of one
+// carrier in one country (the same change t of the state of charge at the instants b0+1 … b0+L);
+// the gas unit of each node follows pro rata at the block's ends (a requirement change of +κs at
+// step b0 and −κs at step b0+L, κs = k h/Δt, shifted by `soc_qadj`), with the Jacobian test. The
+// flow basis' shapes do not exist in this basis, so every move is a plateau.
+//
+// MI355X layout: one wave per (block, group of replicas). A segment of W lanes (W the smallest
+// power of two ≥ L + 1, at most 64) owns one replica: lane `sub` of the segment owns the block
+// steps q ≡ sub (mod W) and evaluates their terms (kinked gas at the block ends, the storage terms
+// of the moved units, the node balances) from values staged in LDS. ΔC(t) is the butterfly sum of
+// the lanes' partial sums over the segment (every lane of a segment gets the same bits, so the
+// segment walks as one); breakpoints and the sign changes of the secondaries are segment minima.
+// The walk follows `walk!` (src/local.jl:1294-1443) with storage secondaries: the roots of the
+// storage secondaries on (tp, t) split the segment, the node balances' roots are found on the
+// pieces; equal roots are visited once.
+#include "sweep_core.h"
+
+namespace esm {
+
+constexpr int SMS_NU = 8;          // units per move (largest country–carrier group)
+constexpr int SMS_SP = 3;          // block steps per lane (L + 1 ≤ 3·64)
+constexpr int SMS_PMAX = 4096;     // walk points per side
+
+struct SmsArgs {
+    uint32_t k0, k1, ctr;
+    int L, base, nblk, nrg;        // block b0 = base + 2L·blk, blk < nblk; nrg replica groups
+    int nmoves;
+    double p_agg;
+    const int *units;              // [2N] unit u: storage carrier u / N, node u % N
+    int nunits;
+    const int *grp_off, *grp_node, *grp_si;
+    int ngroups;
+    unsigned long long *stats;     // [ESM_NSTAT][STAT_SLOTS]
+};
+
+template <typename S, int W>
+__device__ __forceinline__ S seg_sum(S v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// segment minimum in walking order of a candidate (have = false: none)
+template <typename S, int W>
+__device__ __forceinline__ bool seg_best(bool have, S x, S sgn, S &out) {
+    S key = have ? sgn * x : S(INFINITY);
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) key = fmin(key, __shfl_xor(key, off));
+    if (!(key < S(INFINITY))) return false;
+    out = sgn * key;
+    return true;
+}
+
+template <typename S>
+struct SmsKink {
+    S Q, g, ak, y0, P, e, valid;
+};
+
+// the moved unit u of a segment: its storage parameters, clamped capacity, κs, node
+template <typename S>
+struct SmsUnit {
+    StoPar<S> par;
+    S kst, ks;
+    int node;
+};
+
+// one replica's move, seen by one lane of its segment; the values live in the segment's LDS
+template <typename S, int W>
+struct FStor {
+    const DevModel<S> *m;
+    int L, sub, nu;
+    const S *ye_l, *ye1_l, *S0_l;      // [q][u]
+    const SmsKink<S> *kk_l;            // [2][u]: the kinks at steps 0 and L
+    const SmsUnit<S> *un_l;            // [u]
+    mutable int nev = 0;
+
+    __device__ S gen_c(int n, S y, S &p) const {
+        S c;
+        const S k = m->kc[ESM_KGAS * m->N + n];
+        gen_cost(m->mc[ESM_KGAS * m->N + n], k, k + m->kappa0, m->dt, m->cv, y, c, p);
+        return c;
+    }
+    __device__ static S kphi(S Q, S g, S t) { return fmax(Q + g * t, S(0)) - fmax(Q, S(0)); }
+
+    // the terms of unit u at block step q at t, in this order: its node's kinked gas (block
+    // ends), its storage term, its node's balance; the storage secondaries (sec3) and the balance
+    __device__ S unit_step(int q, int u, S t, S *sec3, S &S1) const {
+        const SmsUnit<S> U = un_l[u];
+        S D = 0, dr = 0;
+        const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
+        if (kidx >= 0) {
+            const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+            if (kk.valid != S(0) && kk.ak != S(0)) {
+                S p0, p1;
+                const S c0 = gen_c(U.node, kk.y0, p0);
+                const S c1 = gen_c(U.node, kk.y0 + kk.ak * kphi(kk.Q, kk.g, t), p1);
+                D += c1 - c0;
+                dr -= p1 - p0;
+            }
+        }
+        const S dye = q >= 1 ? S(1) : S(0), dye1 = q <= L - 1 ? S(1) : S(0);
+        const S ye = ye_l[q * SMS_NU + u], ye1 = ye1_l[q * SMS_NU + u];
+        const S w = U.kst + m->kappa0;
+        S d0, j0, d1, j1;
+        const S c0 = soc_cost_p(U.par, m->dt, m->cv, U.kst, w, ye, ye1, d0, j0);
+        const S c1 = soc_cost_p(U.par, m->dt, m->cv, U.kst, w, ye + t * dye, ye1 + t * dye1, d1, j1);
+        D += c1 - c0;
+        dr += j1 - j0;
+        if (sec3) {
+            sec3[0] = d1;
+            sec3[1] = U.par.ein > 0 ? d1 - U.par.ein : S(1);
+            sec3[2] = -d1 * U.par.eout - S(1);
+        }
+        const S S0 = S0_l[q * SMS_NU + u];
+        S1 = S0 + dr;
+        D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+        return D;
+    }
+
+    __device__ S step(int q, S t) const {
+        S D = 0, S1;
+        for (int u = 0; u < nu; u++) D += unit_step(q, u, t, nullptr, S1);
+        return D;
+    }
+
+    // ΔC(t): the lanes' partial sums (own steps in increasing order), summed over the segment
+    __device__ S eval(S t) const {
+        nev++;
+        S D = 0;
+#pragma unroll
+        for (int i = 0; i < SMS_SP; i++) {
+            const int q = sub + W * i;
+            if (q <= L) D += step(q, t);
+        }
+        return seg_sum<S, W>(D);
+    }
+
+    // the nearest primary breakpoint beyond tp (segment minimum)
+    __device__ bool next_bp(S tp, S sgn, S &t) const {
+        bool have = false;
+        S best = 0;
+        auto cand = [&](S b) {
+            if (isfinite(b) && sgn * (b - tp) > 0 && (!have || sgn * (best - b) > 0)) {
+                best = b;
+                have = true;
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < SMS_SP; i++) {
+            const int q = sub + W * i;
+            if (q > L) continue;
+            const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
+            for (int u = 0; u < nu; u++) {
+                if (kidx >= 0) {
+                    const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+                    if (kk.valid != S(0)) {
+                        cand(-kk.Q / kk.g);
+                        if (kk.ak != S(0)) {
+                            const S sl = kk.ak * kk.g, off = kk.ak * kk.Q;
+                            cand((S(0) - kk.y0 - S(0)) / sl);
+                            cand((S(1) - kk.y0 - S(0)) / sl);
+                            cand((S(0) - kk.y0 - off) / sl);
+                            cand((S(1) - kk.y0 - off) / sl);
+                        }
+                    }
+                }
+                if (q >= 1) {
+                    const S ye = ye_l[q * SMS_NU + u];
+                    cand((S(0) - ye) / S(1));
+                    cand((S(1) - ye) / S(1));
+                }
+            }
+        }
+        return seg_best<S, W>(have, best, sgn, t);
+    }
+
+    // the nearest sign change beyond x, strictly inside (a, b), of the storage secondaries
+    // (stor) or of the node balances, from their values at a and b (linear in between)
+    __device__ bool next_root(bool stor, S a, S b, S x, S sgn, S &xr) const {
+        bool have = false;
+        S best = 0;
+#pragma unroll
+        for (int i = 0; i < SMS_SP; i++) {
+            const int q = sub + W * i;
+            if (q > L) continue;
+            for (int u = 0; u < nu; u++) {
+                S sa[3], sb[3], Ba, Bb;
+                unit_step(q, u, a, sa, Ba);
+                unit_step(q, u, b, sb, Bb);
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    if (!stor && k > 0) break;
+                    const S A = stor ? sa[k] : Ba, B = stor ? sb[k] : Bb;
+                    if ((A < 0) != (B < 0) && A != B) {
+                        const S r = a + (b - a) * A / (A - B);
+                        if (sgn * (r - a) > 0 && sgn * (b - r) > 0 && sgn * (r - x) > 0 &&
+                            (!have || sgn * (best - r) > 0)) {
+                            best = r;
+                            have = true;
+                        }
+                    }
+                }
+            }
+        }
+        return seg_best<S, W>(have, best, sgn, xr);
+    }
+
+    // log Jacobian of the compensated kinks (one unit per node: |S| − 1 = 0), segment sum
+    __device__ double logjac(S t, bool &any) const {
+        double lj = 0.0;
+        int cnt = 0;
+#pragma unroll
+        for (int i = 0; i < SMS_SP; i++) {
+            const int q = sub + W * i;
+            const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
+            if (q > L || kidx < 0) continue;
+            for (int u = 0; u < nu; u++) {
+                const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+                if (kk.valid == S(0)) continue;
+                cnt++;
+                const double nm = 0.0;
+                lj += nm * log(fabs((fmax((double)kk.Q + (double)kk.g * (double)t, 0.0) + (double)kk.e) / (double)kk.P));
+            }
+        }
+        any = seg_sum<double, W>((double)cnt) > 0;
+        return seg_sum<double, W>(lj);
+    }
+};
+
+// words (of S) of one segment's LDS: the staged values, the kinks and the units (padded to S)
+template <typename S>
+__host__ __device__ constexpr size_t sms_seg_words(int nstep) {
+    return 3 * (size_t)nstep * SMS_NU + 7 * 2 * SMS_NU + (sizeof(SmsUnit<S>) + sizeof(S) - 1) / sizeof(S) * SMS_NU;
+}
+
+template <class V, typename S>
+__device__ __forceinline__ int sms_visit(V &vis, S x, S cx, S &cmin, int &np) {
+    if (++np > SMS_PMAX) return -1;
+    if (vis(x, cx)) return 2;
+    cmin = fmin(cmin, cx);
+    return 0;
+}
+
+// walk! with storage secondaries: 1 proper, 0 improper, -1 point cap, 2 stopped by the visitor
+template <typename S, int W, class V>
+__device__ int walk_stor(const FStor<S, W> &f, S beta, S maxnats, S sgn, V &vis) {
+    S cmin = 0, tp = 0, delta = 0, clast = 0;
+    int nx = 0, np = 0, w;
+    while (true) {
+        S t = 0;
+        if (!f.next_bp(tp, sgn, t)) {
+            const S hp = S(1e-6) * fmax(S(1), fabs(tp));
+            const S sl = (beta * f.eval(tp + sgn * hp) - clast) / hp;
+            if (sl > 0) delta = fmin(2 * maxnats / sl, S(1e3) * fmax(fabs(tp), S(1e-3)));
+            else delta = delta == 0 ? fmax(fabs(tp), S(1e-6)) : 2 * delta;
+            t = tp + sgn * delta;
+            if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;
+        }
+        const S c = beta * f.eval(t);
+        S a0 = tp, xs = tp, x = 0;
+        while (f.next_root(true, tp, t, xs, sgn, x)) {
+            const S cx = beta * f.eval(x);
+            S xb = a0, y = 0;
+            while (f.next_root(false, a0, x, xb, sgn, y)) {
+                if ((w = sms_visit(vis, y, beta * f.eval(y), cmin, np))) return w;
+                xb = y;
+            }
+            if ((w = sms_visit(vis, x, cx, cmin, np))) return w;
+            a0 = x;
+            xs = x;
+        }
+        S xb = a0, y = 0;
+        while (f.next_root(false, a0, t, xb, sgn, y)) {
+            if ((w = sms_visit(vis, y, beta * f.eval(y), cmin, np))) return w;
+            xb = y;
+        }
+        if ((w = sms_visit(vis, t, c, cmin, np))) return w;
+        clast = c;
+        if (c - cmin > maxnats) return 1;
+        tp = t;
+    }
+}
+
+template <typename S>
+struct VisS {
+    int mode = 0;
+    MassAcc<S> ma;
+    SegSel<S> ss;
+    __device__ bool operator()(S t, S c) { return mode == 0 ? ma(t, c) : ss(t, c); }
+};
+
+template <typename S, int W>
+__device__ S sample_stor(const FStor<S, W> &f, S beta, S maxnats, S useg, S uoff, int &status) {
+    VisS<S> vis;
+    S Mx[2] = {S(-INFINITY), S(-INFINITY)}, Ms[2] = {S(0), S(0)};
+#pragma nounroll
+    for (int k = 0; k < 3; k++) {
+        S sgn;
+        if (k == 2) {
+            const S MX = fmax(Mx[0], Mx[1]);
+            const S Mr = Ms[0] * exp(Mx[0] - MX), Ml = Ms[1] * exp(Mx[1] - MX);
+            const S u = useg * (Ml + Mr);
+            vis.mode = 1;
+            vis.ss = SegSel<S>();
+            vis.ss.MX = MX;
+            vis.ss.left = u <= Ml;
+            vis.ss.target = vis.ss.left ? Ml - u : u - Ml;
+            sgn = vis.ss.left ? S(-1) : S(1);
+        } else {
+            vis.mode = 0;
+            vis.ma = MassAcc<S>();
+            sgn = k == 0 ? S(1) : S(-1);
+        }
+        const int w = walk_stor(f, beta, maxnats, sgn, vis);
+        if (k < 2) {
+            if (w != 1) {
+                status = w == 0 ? 1 : 2;
+                return 0;
+            }
+            Mx[k] = vis.ma.mx;
+            Ms[k] = vis.ma.s;
+        }
+    }
+    status = 0;
+    const SegSel<S> &sel = vis.ss;
+    const S L = sel.tb - sel.ta;
+    const S g = (sel.cb - sel.ca) / L;
+    const S t = (g >= 0 || fabs(g * L) < S(1e-10)) ? sel.ta + trexp_offset(g, L, uoff) : sel.tb - trexp_offset(-g, L, uoff);
+    if (!isfinite(t)) {
+        status = 2;
+        return 0;
+    }
+    return t;
+}
+
+template <typename S, int W>
+__global__ void __launch_bounds__(64) k_storage_ms(DevModel<S> m, SweepState<S> st, SmsArgs a, S beta, S maxnats,
+                                                  S kmin) {
+    extern __shared__ unsigned char sms_lds[];
+    constexpr int SEGS = 64 / W;
+    const int lane = threadIdx.x;
+    const int seg = lane / W, sub = lane % W;
+    const int blk = blockIdx.x / a.nrg, rg = blockIdx.x % a.nrg;
+    const int L = a.L, b0 = a.base + 2 * L * blk;
+    const int j = rg * SEGS + seg;
+    const bool valid = j < m.R;
+    const int nstep = L + 1;
+    const size_t TR = (size_t)m.T * m.R;
+    // LDS of this segment: ye, ye1, S0 [nstep][SMS_NU]; kinks [2][SMS_NU]; units [SMS_NU]
+    S *base = (S *)sms_lds + (size_t)seg * sms_seg_words<S>(nstep);
+    S *ye_l = base, *ye1_l = base + nstep * SMS_NU, *S0_l = base + 2 * nstep * SMS_NU;
+    SmsKink<S> *kk_l = (SmsKink<S> *)(base + 3 * nstep * SMS_NU);
+    SmsUnit<S> *un_l = (SmsUnit<S> *)(base + 3 * nstep * SMS_NU + 7 * 2 * SMS_NU);
+    MoveOut out;
+    for (int mv = 0; mv < a.nmoves; mv++) {
+        uint32_t s[4];
+        philox4x32(a.k0, a.k1, (uint32_t)mv, (uint32_t)b0, 9200u + (uint32_t)L, a.ctr, s);
+        // the units: a country–carrier group (probability p_agg) or one unit
+        int si, nu, node[SMS_NU];
+        if (u01(s[0], s[1]) < a.p_agg) {
+            const int g = (int)mulhi32(s[2], (uint32_t)a.ngroups);
+            si = a.grp_si[g];
+            nu = a.grp_off[g + 1] - a.grp_off[g];
+#pragma unroll
+            for (int u = 0; u < SMS_NU; u++) node[u] = u < nu ? a.grp_node[a.grp_off[g] + u] : 0;
+        } else {
+            const int un = a.units[(int)mulhi32(s[3], (uint32_t)a.nunits)];
+            si = un / m.N;
+            nu = 1;
+            node[0] = un % m.N;
+#pragma unroll
+            for (int u = 1; u < SMS_NU; u++) node[u] = 0;
+        }
+        if (!valid) continue;
+        FStor<S, W> f;
+        f.m = &m;
+        f.L = L;
+        f.sub = sub;
+        f.nu = nu;
+        f.ye_l = ye_l;
+        f.ye1_l = ye1_l;
+        f.S0_l = S0_l;
+        f.kk_l = kk_l;
+        f.un_l = un_l;
+        const S *ye_g = st.v + plane((2 + si) * m.N, m.T, m.R);
+        const S *yg_g = st.v + plane(m.N, m.T, m.R);
+        if (sub == 0)
+            for (int u = 0; u < nu; u++) {
+                const int us = si * m.N + node[u];
+                SmsUnit<S> U;
+                U.par = {m.hours[us], m.eta_in[us], m.eta_out[us], m.qt[us], m.mc[(ESM_KSTOR + si) * m.N + node[u]]};
+                U.kst = m.kc[(ESM_KSTOR + si) * m.N + node[u]];
+                U.ks = U.kst * m.hours[us] / m.dt;
+                U.node = node[u];
+                un_l[u] = U;
+            }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // stage the block's values of the segment's replica (each lane its own steps)
+#pragma unroll
+        for (int i = 0; i < SMS_SP; i++) {
+            const int q = sub + W * i;
+            if (q > L) continue;
+            const int tau = b0 + q, tau1 = tau + 1 == m.T ? 0 : tau + 1;
+            for (int u = 0; u < nu; u++) {
+                const size_t on = (size_t)node[u] * TR;
+                ye_l[q * SMS_NU + u] = ye_g[on + (size_t)tau * m.R + j];
+                ye1_l[q * SMS_NU + u] = ye_g[on + (size_t)tau1 * m.R + j];
+                S0_l[q * SMS_NU + u] = st.r[on + (size_t)tau * m.R + j];
+            }
+            // the pro-rata kinks at the block's ends (`prorata!` with `soc_qadj`)
+            const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
+            if (kidx >= 0)
+                for (int u = 0; u < nu; u++) {
+                    const SmsUnit<S> U = un_l[u];
+                    SmsKink<S> kk{0, 0, 0, 0, 0, 0, 0};
+                    const int n = node[u];
+                    const S k = m.kc[ESM_KGAS * m.N + n];
+                    const S g = kidx == 0 ? U.ks * (S(1) - S(0)) : U.ks * (S(0) - S(1));
+                    if (fabs(k) >= kmin && g != S(0)) {
+                        const S w = yg_g[(size_t)n * TR + (size_t)tau * m.R + j];
+                        const S P = S(0) + k * w;
+                        const S Qs = S(0) + k * clampv(w, S(0), S(1));
+                        if (!(fabs(P) < S(1e-6))) {
+                            // soc_qadj: nominal minus actual injection of the moved unit at (n, τ)
+                            const S Y = ye_l[q * SMS_NU + u], Y1 = ye1_l[q * SMS_NU + u];
+                            const S ec = clampv(Y, S(0), S(1)), ec1 = clampv(Y1, S(0), S(1));
+                            const S d = U.par.h * (ec1 - ec) / m.dt - U.par.qt;
+                            S pc, mcp;
+                            if (d >= 0) {
+                                pc = U.par.ein > 0 ? d / U.par.ein : S(0);
+                                mcp = 0;
+                            } else {
+                                pc = 0;
+                                mcp = fmin(-d * U.par.eout, S(1));
+                            }
+                            const S qadj = S(0) + (U.kst * U.par.h * (Y1 - Y) / m.dt - U.kst * (pc - mcp));
+                            kk.Q = st.r[(size_t)n * TR + (size_t)tau * m.R + j] + Qs + qadj;
+                            kk.g = g;
+                            kk.e = P - fmax(kk.Q, S(0));
+                            kk.P = P;
+                            const S sig = k * w / P;
+                            kk.ak = sig == S(0) ? S(0) : sig / k;
+                            kk.y0 = w;
+                            kk.valid = 1;
+                        }
+                    }
+                    kk_l[kidx * SMS_NU + u] = kk;
+                }
+        }
+        double u1, u2;
+        uniforms2(a.k0, a.k1, j, (uint32_t)b0, 0x50000000u + (uint32_t)(L * 16 + mv), a.ctr, u1, u2);
+        int status = 0;
+        S t = sample_stor<S, W>(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+        if (t != 0) {
+            bool any;
+            const double lj = f.logjac(t, any);
+            if (any) {
+                uint32_t w4[4];
+                philox4x32(a.k0, a.k1, (uint32_t)j, (uint32_t)b0, 0x70000000u + (uint32_t)(L * 16 + mv), a.ctr, w4);
+                if (!(u01(w4[0], w4[1]) < exp(fmin(0.0, lj)))) {
+                    t = 0;
+                    status = 2;
+                }
+            }
+        }
+        if (sub == 0) out.add(status, t != 0, f.nev);
+        if (t != 0) {
+            S *ye_w = st.v + plane((2 + si) * m.N, m.T, m.R);
+            S *yg_w = st.v + plane(m.N, m.T, m.R);
+#pragma unroll
+            for (int i = 0; i < SMS_SP; i++) {
+                const int q = sub + W * i;
+                if (q > L) continue;
+                const int tau = b0 + q;
+                const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
+                for (int u = 0; u < nu; u++) {
+                    S S1;
+                    f.unit_step(q, u, t, nullptr, S1);
+                    const size_t o = (size_t)node[u] * TR + (size_t)tau * m.R + j;
+                    st.r[o] = S1;
+                    if (q >= 1) ye_w[o] = ye_l[q * SMS_NU + u] + t;
+                    if (kidx >= 0) {
+                        const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+                        if (kk.valid != S(0) && kk.ak != S(0)) yg_w[o] = kk.y0 + kk.ak * FStor<S, W>::kphi(kk.Q, kk.g, t);
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    add_counts(a.stats, out);
+}
+
+static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// the schedule of one storage sweep (drawn on the host from the sweep's Philox counter)
+struct SmsSchedule {
+    bool run = false;
+    std::vector<int> lengths;      // in sweep order
+    std::vector<int> offset;       // per length in that order
+    std::vector<int> first;        // class drawn first (0: even blocks, 1: odd)
+};
+
+static SmsSchedule sms_schedule(int T, double p, uint32_t k0, uint32_t k1, uint32_t ctr) {
+    SmsSchedule sc;
+    uint32_t w[4];
+    philox4x32(k0, k1, 0u, 0u, 9300u, ctr, w);
+    sc.run = u01(w[0], w[1]) < p;
+    if (!sc.run) return sc;
+    for (int L = 2; L <= 128; L *= 2)
+        if (L < T / 2) sc.lengths.push_back(L);
+    for (int i = (int)sc.lengths.size() - 1; i >= 1; i--) {   // Fisher–Yates
+        philox4x32(k0, k1, (uint32_t)i, 0u, 9301u, ctr, w);
+        const int jx = (int)mulhi32(w[0], (uint32_t)(i + 1));
+        std::swap(sc.lengths[i], sc.lengths[jx]);
+    }
+    for (int L : sc.lengths) {
+        philox4x32(k0, k1, (uint32_t)L, 0u, 9302u, ctr, w);
+        sc.offset.push_back((int)mulhi32(w[0], (uint32_t)L));
+        sc.first.push_back(u01(w[2], w[3]) < 0.5 ? 0 : 1);
+    }
+    return sc;
+}
+
+template <typename S>
+int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats) {
+    const int T = h->T, R = h->R;
+    if (!(h->sweep_pms > 0) || h->mt.sms_units.empty()) return 0;
+    const SmsSchedule sc = sms_schedule(T, h->sweep_pms, k0, k1, ctr);
+    if (!sc.run) return 0;
+    const DevModel<S> m = dev_model<S>(h);
+    SweepState<S> st{(S *)h->d_v, (S *)h->d_r, (S *)h->d_F, (const S *)h->d_x};
+    const S beta = (S)h->m.beta_v, mxn = (S)maxnats, kmin = S(1);
+    const MoveTablesDev &d = h->dmt;
+    for (size_t li = 0; li < sc.lengths.size(); li++) {
+        const int L = sc.lengths[li], o = sc.offset[li];
+        if (o > T - L - 1) continue;
+        const int nb = (T - L - 1 - o) / L + 1;
+        if (nb < 2) continue;
+        int W = 4;
+        while (W < L + 1 && W < 64) W *= 2;
+        const int segs = 64 / W;
+        const int nrg = (R + segs - 1) / segs;
+        for (int ci = 0; ci < 2; ci++) {
+            const int cls = ci == 0 ? sc.first[li] : 1 - sc.first[li];
+            const int nbc = (nb - cls + 1) / 2;
+            if (nbc <= 0) continue;
+            SmsArgs a{k0, k1, ctr, L, o + cls * L, nbc, nrg, h->sweep_ms_moves, h->sweep_ms_pagg,
+                      d.sms_units, (int)h->mt.sms_units.size(), d.grp_off, d.grp_node, d.grp_si,
+                      (int)h->mt.grp_si.size(), h->d_stats + (size_t)ESM_FAM_SMS * ESM_NSTAT * STAT_SLOTS};
+            const size_t lds = (size_t)segs * sms_seg_words<S>(L + 1) * sizeof(S);
+            const dim3 grid((unsigned)nbc * (unsigned)nrg), blk(64);
+            ProfScope p(h, ESM_KERN_SMS);
+            switch (W) {
+                case 4: hipLaunchKernelGGL((k_storage_ms<S, 4>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                case 8: hipLaunchKernelGGL((k_storage_ms<S, 8>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                case 16: hipLaunchKernelGGL((k_storage_ms<S, 16>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                case 32: hipLaunchKernelGGL((k_storage_ms<S, 32>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                default: hipLaunchKernelGGL((k_storage_ms<S, 64>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+            }
+            ESM_HIP_CHECK(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+template int launch_storage_ms<float>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
+template int launch_storage_ms<double>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
+
+}  // namespace esm

@@ -138,9 +138,11 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
                 }
             }
         }
-        // the random families, per time-parity class
+        // the random families, per time-parity class, then (with its probability) the storage
+        // multiscale sweep (`operation_sweep!`, src/local.jl:2413-2418)
         int rc = launch_random<S>(h, k0, k1, a.ctr, maxnats);
         if (rc) return rc;
+        if ((rc = launch_storage_ms<S>(h, k0, k1, a.ctr, maxnats))) return rc;
     }
     ESM_HIP_CHECK(hipGetLastError());
     return 0;

@@ -357,6 +357,31 @@ bool build_move_tables(const esm_handle *h, MoveTablesHost &mt, std::string &err
         mt.qcum.push_back(acc);
     }
     for (double &q : mt.qcum) q /= acc;
+    // storage multiscale moves (`storage_ms_sweep!`, src/flowmoves.jl:266-277): the units, and
+    // the groups of one carrier in one country (cellof = E.parts[2]; single nodes without countries)
+    for (int s = 0; s < ESM_NSTOR; s++)
+        for (int n = 0; n < N; n++) mt.sms_units.push_back(s * N + n);
+    std::vector<int> cellof(N);
+    for (int n = 0; n < N; n++) cellof[n] = h->ncountry > 0 ? h->country[n] : n;
+    std::vector<int> cells(cellof);
+    std::sort(cells.begin(), cells.end());
+    cells.erase(std::unique(cells.begin(), cells.end()), cells.end());
+    mt.grp_off.push_back(0);
+    for (int c : cells)
+        for (int s = 0; s < ESM_NSTOR; s++) {
+            int cnt = 0;
+            for (int n = 0; n < N; n++)
+                if (cellof[n] == c) {
+                    mt.grp_node.push_back(n);
+                    cnt++;
+                }
+            if (cnt > 8) {
+                err = "move tables: a country holds more than 8 nodes (the storage group limit)";
+                return false;
+            }
+            mt.grp_si.push_back(s);
+            mt.grp_off.push_back((int)mt.grp_node.size());
+        }
     return true;
 }
 

@@ -136,24 +136,27 @@ def _key(key):
     return k
 
 
-FAMILY_NAMES = ("gen", "theta", "soc", "spair", "unit", "potential", "cell", "xloc", "xlocc", "xpair")
+FAMILY_NAMES = ("gen", "theta", "soc", "spair", "unit", "potential", "cell", "xloc", "xlocc", "xpair",
+                "storage_ms")
 
 
-def sweep_options(R: Replicas, extra: float = 4.0, weights=None, atoms: bool = True):
+def sweep_options(R: Replicas, extra: float = 4.0, weights=None, atoms: bool = True, p_storage_ms: float = 0.5):
     """The `local_sweep!` options (src/local.jl:1927-1929): ``extra`` random moves per node and step,
-    the weights of the plain families (``_lib.WEIGHT_NAMES`` order; None: the reference's) and
-    whether the atoms run."""
+    the weights of the plain families (``_lib.WEIGHT_NAMES`` order; None: the reference's),
+    whether the atoms run, and the probability of the storage multiscale sweep that
+    `operation_sweep!` runs after them (src/local.jl:2414-2418: 1/2)."""
     w = None if weights is None else f64(weights)
     if w is not None and w.shape != (_lib.NWEIGHT,):
         raise ValueError(f"weights must have {_lib.NWEIGHT} entries")
-    check(R.lib.esm_sweep_options(R.handle, float(extra), dptr(w) if w is not None else None, int(bool(atoms))),
-          "esm_sweep_options")
+    check(R.lib.esm_sweep_options(R.handle, float(extra), dptr(w) if w is not None else None, int(bool(atoms)),
+                                  float(p_storage_ms)), "esm_sweep_options")
 
 
 def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0, maxnats: float = 40.0):
     """``operation_sweep!`` / ``local_sweep!`` (src/local.jl:1927-2062, :2401-2425) of every replica,
-    ``nsweeps`` times at fixed capacities and weather: the atoms, then the random families.
-    Returns the move statistics [10 families (``FAMILY_NAMES``), 5] (attempted, accepted,
+    ``nsweeps`` times at fixed capacities and weather: the atoms, the random families, then (with
+    its probability) the storage multiscale sweep. Returns the move statistics [11 families
+    (``FAMILY_NAMES``), 5] (attempted, accepted,
     improper, point-cap / non-finite / Jacobian-rejected, ΔC evaluations)."""
     st = np.zeros(_lib.NFAM * _lib.NSTAT, dtype=np.int64)
     check(R.lib.esm_sweeps(R.handle, int(nsweeps), _key(key), int(counter) & 0xFFFFFFFF, float(maxnats),

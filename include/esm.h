@@ -184,7 +184,8 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
 #define ESM_FAM_XLOC 7    /* random: localized cross-node pair, leak to the residuals */
 #define ESM_FAM_XLOCC 8   /* random: localized pair, leak compensated pro rata        */
 #define ESM_FAM_XPAIR 9   /* random: cross-node pair, global re-routing               */
-#define ESM_NFAM 10
+#define ESM_FAM_SMS 10    /* storage multiscale: block plateaus (storage_ms_sweep!)   */
+#define ESM_NFAM 11
 #define ESM_NSTAT 5       /* attempted, accepted, improper, guard, ΔC evaluations     */
 /* the plain random families in the reference's order and default weights
  * (plain_families, src/local.jl:1912-1915): unit 0.3, pair 0.25, xpair 0.03, xlocc 0.08,
@@ -220,10 +221,21 @@ int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32
 /*
  * Options of the operation sweep (`local_sweep!` keywords, src/local.jl:1927-1929): `extra`
  * random moves per node and step (`local_extra`, 4 by default, src/run.jl:21), the weights of
- * the plain families (ESM_NWEIGHT, in the order above; NULL: the reference's), and whether
- * the atom rounds run (`atoms`). Defaults hold from esm_create on.
+ * the plain families (ESM_NWEIGHT, in the order above; NULL: the reference's), whether the
+ * atom rounds run (`atoms`), and the probability of the storage multiscale sweep after them
+ * (`operation_sweep!`, src/local.jl:2414-2418: 1/2). Defaults hold from esm_create on.
+ *
+ * The storage multiscale sweep (`storage_ms_sweep!`, src/flowmoves.jl:255-324): for every block
+ * length L in {2, 4, …, 128} below T/2, in an order drawn per sweep, with an offset o drawn in
+ * [0, L), the blocks b0 = o + iL (steps b0 … b0 + L, no wrap) in two parity classes drawn in
+ * either order; 8 plateau moves per block on one storage unit or (probability 1/2) one carrier's
+ * units in one country, compensated pro rata at the block ends, with the Jacobian test. Draws
+ * (Philox under the sweep's key and counter c): whether it runs (0, 0, 9300, c); the length
+ * order (i, 0, 9301, c), Fisher–Yates; offset and class order (L, 0, 9302, c); the units of
+ * move m of the block at b0 (m, b0, 9200 + L, c); its line draw (replica, b0,
+ * 0x50000000 + 16L + m, c) and Jacobian test (replica, b0, 0x70000000 + 16L + m, c).
  */
-int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, int32_t atoms);
+int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, int32_t atoms, double p_storage_ms);
 
 /*
  * `follow_capacities!` (src/replicas.jl:448-472) of every replica: move the shared capacities
@@ -301,7 +313,8 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_KERN_XLOC 12   /* random localized pairs                     */
 #define ESM_KERN_XLOCC 13  /* random compensated localized pairs         */
 #define ESM_KERN_XPAIR 14  /* random re-routed pairs                     */
-#define ESM_NKERN 15
+#define ESM_KERN_SMS 15    /* storage multiscale moves of one block length and class */
+#define ESM_NKERN 16
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
 

@@ -1433,89 +1433,6 @@ int orc_colouring(const esm_model *m, int *colour) {
     return nc;
 }
 
-int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
-               const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
-               int64_t *stats, int nthreads) {
-    int N = m->N, L = m->L, T = m->T;
-    if (N > 256) return -1;
-    static const double wdef[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
-    const double *wts = weights ? weights : wdef;
-    size_t NTR = (size_t)N * T * R;
-    Sys S0;
-    S0.m = m;
-    S0.R = R;
-    S0.th = v;
-    S0.yg = v + NTR;
-    S0.ye[0] = v + 2 * NTR;
-    S0.ye[1] = v + 3 * NTR;
-    S0.r = r;
-    S0.F = F;
-    for (int u = 0; u < ESM_NK * N; u++) S0.kc[u] = clampd(k[u], m->klo[u], m->khi[u]);
-    S0.adj_ptr = (int *)calloc(N + 1, sizeof(int));
-    S0.adj_line = (int *)malloc(sizeof(int) * 2 * L);
-    int *fill = (int *)malloc(sizeof(int) * N), *colour = (int *)malloc(sizeof(int) * N);
-    for (int e = 0; e < L; e++) {
-        S0.adj_ptr[m->line_bus0[e] + 1]++;
-        S0.adj_ptr[m->line_bus1[e] + 1]++;
-    }
-    for (int n = 0; n < N; n++) S0.adj_ptr[n + 1] += S0.adj_ptr[n];
-    for (int n = 0; n < N; n++) fill[n] = S0.adj_ptr[n];
-    for (int e = 0; e < L; e++) {
-        S0.adj_line[fill[m->line_bus0[e]]++] = e;
-        S0.adj_line[fill[m->line_bus1[e]]++] = e;
-    }
-    int nc = colouring(m, S0.adj_ptr, S0.adj_line, colour);
-    Tables tb;
-    if (build_tables(m, &tb)) return -2;
-    /* time-parity classes */
-    int cfirst[3], ccount[3], ncls;
-    if (T % 2 == 0) {
-        ncls = 2; cfirst[0] = 0; ccount[0] = T / 2; cfirst[1] = 1; ccount[1] = T / 2;
-    } else {
-        ncls = 3; cfirst[0] = 0; ccount[0] = (T - 1) / 2; cfirst[1] = 1; ccount[1] = (T - 1) / 2;
-        cfirst[2] = T - 1; ccount[2] = 1;
-    }
-    int *counts = (int *)malloc(sizeof(int) * ESM_NWEIGHT * T * (nsweeps > 0 ? nsweeps : 1));
-    for (int sw = 0; sw < nsweeps; sw++) move_counts(m, extra, wts, key, ctr0 + (uint32_t)sw, counts + (size_t)sw * ESM_NWEIGHT * T);
-    int64_t st[ESM_NFAM * ESM_NSTAT];
-    memset(st, 0, sizeof(st));
-    if (nthreads < 1) nthreads = 1;
-    /* replicas are independent: threads over replicas, each with its own statistics */
-#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : st[:ESM_NFAM * ESM_NSTAT])
-    for (int j = 0; j < R; j++) {
-        Sys S = S0;
-        for (int sw = 0; sw < nsweeps; sw++) {
-            uint32_t ctr = ctr0 + (uint32_t)sw;
-            if (atoms) {
-                for (int n = 0; n < N; n++)
-                    for (int t = 0; t < T; t++) do_move(&S, j, 0, 0, n, t, key, 0u, ctr, maxnats, st + 5 * ESM_FAM_GEN);
-                for (int c = 0; c < nc; c++)
-                    for (int n = 0; n < N; n++) {
-                        if (colour[n] != c || S.adj_ptr[n + 1] == S.adj_ptr[n]) continue;
-                        for (int t = 0; t < T; t++)
-                            do_move(&S, j, 1, 0, n, t, key, 1000u + (uint32_t)c, ctr, maxnats, st + 5 * ESM_FAM_THETA);
-                    }
-                for (int s = 0; s < ESM_NSTOR; s++)
-                    for (int c = 0; c < ncls; c++)
-                        for (int n = 0; n < N; n++)
-                            for (int q = 0; q < ccount[c]; q++) {
-                                int t = cfirst[c] + 2 * q;
-                                do_move(&S, j, 2, s, n, t, key, 2000u + (uint32_t)(s * 4 + c), ctr, maxnats,
-                                        st + 5 * ESM_FAM_SOC);
-                            }
-            }
-            const int *cnt = counts + (size_t)sw * ESM_NWEIGHT * T;
-            for (int c = 0; c < ncls; c++)
-                for (int q = 0; q < ccount[c]; q++)
-                    random_step(&S, &tb, j, cfirst[c] + 2 * q, cnt, wts, key, ctr, maxnats, st);
-        }
-    }
-    if (stats) memcpy(stats, st, sizeof(st));
-    free(S0.adj_ptr); free(S0.adj_line); free(fill); free(colour); free(counts);
-    tables_free(&tb);
-    return 0;
-}
-
 /* Explicit-uniform single move (no commit) and its cost along the line, for the pins of the
  * line sampler: the state arrays are read only. fam 0 gas, 1 potential, 2 SoC, 3 pair. */
 static int prepare_single(Sys *S, int j, int fam, int s, int n, int tau, Step *wa, Step *ws) {
@@ -1729,3 +1646,611 @@ int orc_sparse_draw_many(const esm_model *m, int R, double *v, const double *k, 
     tables_free(&tb);
     return 0;
 }
+
+/* ============================================================================================
+ * The storage multiscale sweep (storage_ms_sweep!, src/flowmoves.jl:255-324; prep_storage_ms!
+ * :187-244), plateau moves (no flow-basis shapes in this basis), with the arithmetic of the
+ * device's segment evaluation: block step q belongs to lane q mod W of a W-lane segment (W the
+ * smallest power of two ≥ L + 1, at most 64); a lane sums its steps in increasing order, each
+ * step its units in order (unit u: its node's kinked gas at the block ends, its storage term,
+ * its node's balance); the lanes' sums are combined by the butterfly x_l + x_{l xor o},
+ * o = W/2 … 1. The walk splits each primary segment at the storage secondaries' roots and finds
+ * the balances' roots on the pieces (walk!, src/local.jl:1294-1443); equal roots are visited once.
+ * ============================================================================================ */
+
+#define SMS_NU 8
+#define SMS_PMAX 4096
+
+typedef struct {
+    double Q, g, ak, y0, P, e;
+    int valid;
+} SmsKink;
+
+typedef struct {
+    Sys *S;
+    int j, L, b0, W, si, nu;
+    int node[SMS_NU];
+    double kst[SMS_NU], ks[SMS_NU];
+    double *ye, *ye1, *S0;          /* [(L + 1)][SMS_NU] */
+    SmsKink kk[2][SMS_NU];
+} SmsMove;
+
+static double sms_gen(const SmsMove *f, int n, double y, double *p) {
+    const esm_model *m = f->S->m;
+    double c, k = f->S->kc[ESM_KGAS * m->N + n];
+    gen_cost(m->mc[ESM_KGAS * m->N + n], 1.0, k, k + m->kappa0, m->dt, m->c_viol, y, &c, p);
+    return c;
+}
+
+static double sms_unit_step(const SmsMove *f, int q, int u, double t, double *sec3, double *S1) {
+    const esm_model *m = f->S->m;
+    int N = m->N, n = f->node[u], us = f->si * N + n;
+    double D = 0.0, dr = 0.0;
+    int kidx = q == 0 ? 0 : (q == f->L ? 1 : -1);
+    if (kidx >= 0) {
+        const SmsKink *kk = &f->kk[kidx][u];
+        if (kk->valid && kk->ak != 0.0) {
+            double p0, p1;
+            double c0 = sms_gen(f, n, kk->y0, &p0);
+            double c1 = sms_gen(f, n, kk->y0 + kk->ak * sm_phi(kk->Q, kk->g, t), &p1);
+            D += c1 - c0;
+            dr -= p1 - p0;
+        }
+    }
+    double dye = q >= 1 ? 1.0 : 0.0, dye1 = q <= f->L - 1 ? 1.0 : 0.0;
+    double ye = f->ye[q * SMS_NU + u], ye1 = f->ye1[q * SMS_NU + u];
+    double w = f->kst[u] + m->kappa0, d0, j0, d1, j1;
+    double c0 = soc_cost(m, f->si, n, f->kst[u], w, m->dt, m->c_viol, ye, ye1, &d0, &j0);
+    double c1 = soc_cost(m, f->si, n, f->kst[u], w, m->dt, m->c_viol, ye + t * dye, ye1 + t * dye1, &d1, &j1);
+    D += c1 - c0;
+    dr += j1 - j0;
+    if (sec3) {
+        sec3[0] = d1;
+        sec3[1] = m->stor_eta_in[us] > 0 ? d1 - m->stor_eta_in[us] : 1.0;
+        sec3[2] = -d1 * m->stor_eta_out[us] - 1.0;
+    }
+    double S0 = f->S0[q * SMS_NU + u];
+    *S1 = S0 + dr;
+    D += m->dt * (phibal(m, *S1) - phibal(m, S0));
+    return D;
+}
+
+static double sms_eval(const SmsMove *f, double t) {
+    double part[64];
+    g_nev++;
+    for (int l = 0; l < f->W; l++) {
+        double D = 0.0;
+        for (int q = l; q <= f->L; q += f->W) {
+            double Dq = 0.0, S1;
+            for (int u = 0; u < f->nu; u++) Dq += sms_unit_step(f, q, u, t, NULL, &S1);
+            D += Dq;
+        }
+        part[l] = D;
+    }
+    for (int off = f->W / 2; off > 0; off >>= 1) {
+        double nx[64];
+        for (int l = 0; l < f->W; l++) nx[l] = part[l] + part[l ^ off];
+        memcpy(part, nx, sizeof(double) * f->W);
+    }
+    return part[0];
+}
+
+static int sms_next_bp(const SmsMove *f, double tp, double sgn, double *t) {
+    int have = 0;
+#define SCAND(bv)                                                                    \
+    do {                                                                             \
+        double b_ = (bv);                                                            \
+        if (isfinite(b_) && sgn * (b_ - tp) > 0 && (!have || sgn * (*t - b_) > 0)) { \
+            *t = b_;                                                                 \
+            have = 1;                                                                \
+        }                                                                            \
+    } while (0)
+    for (int q = 0; q <= f->L; q++) {
+        int kidx = q == 0 ? 0 : (q == f->L ? 1 : -1);
+        for (int u = 0; u < f->nu; u++) {
+            if (kidx >= 0) {
+                const SmsKink *kk = &f->kk[kidx][u];
+                if (kk->valid) {
+                    SCAND(-kk->Q / kk->g);
+                    if (kk->ak != 0.0) {
+                        double sl = kk->ak * kk->g, off = kk->ak * kk->Q;
+                        SCAND((0.0 - kk->y0 - 0.0) / sl);
+                        SCAND((1.0 - kk->y0 - 0.0) / sl);
+                        SCAND((0.0 - kk->y0 - off) / sl);
+                        SCAND((1.0 - kk->y0 - off) / sl);
+                    }
+                }
+            }
+            if (q >= 1) {
+                double ye = f->ye[q * SMS_NU + u];
+                SCAND((0.0 - ye) / 1.0);
+                SCAND((1.0 - ye) / 1.0);
+            }
+        }
+    }
+#undef SCAND
+    return have;
+}
+
+static int sms_next_root(const SmsMove *f, int stor, double a, double b, double x, double sgn, double *xr) {
+    int have = 0;
+    for (int q = 0; q <= f->L; q++)
+        for (int u = 0; u < f->nu; u++) {
+            double sa[3], sb[3], Ba, Bb;
+            sms_unit_step(f, q, u, a, sa, &Ba);
+            sms_unit_step(f, q, u, b, sb, &Bb);
+            for (int k = 0; k < (stor ? 3 : 1); k++) {
+                double A = stor ? sa[k] : Ba, B = stor ? sb[k] : Bb;
+                if ((A < 0) != (B < 0) && A != B) {
+                    double r = a + (b - a) * A / (A - B);
+                    if (sgn * (r - a) > 0 && sgn * (b - r) > 0 && sgn * (r - x) > 0 && (!have || sgn * (*xr - r) > 0)) {
+                        *xr = r;
+                        have = 1;
+                    }
+                }
+            }
+        }
+    return have;
+}
+
+typedef struct {
+    double *p, *c;
+    int n, full;
+} BigPts;
+
+static void bpush(BigPts *P, double x, double c) {
+    if (P->n >= SMS_PMAX) {
+        P->full = 1;
+        return;
+    }
+    P->p[P->n] = x;
+    P->c[P->n] = c;
+    P->n++;
+}
+
+static int sms_walk(const SmsMove *f, double beta, double sgn, double maxnats, BigPts *P) {
+    double cmin = 0.0, tp = 0.0, delta = 0.0;
+    int nx = 0;
+    P->n = 0;
+    P->full = 0;
+    for (;;) {
+        double t;
+        if (!sms_next_bp(f, tp, sgn, &t)) {
+            double cprev = P->n == 0 ? 0.0 : P->c[P->n - 1];
+            double hp = 1e-6 * fmax(1.0, fabs(tp));
+            double sl = (beta * sms_eval(f, tp + sgn * hp) - cprev) / hp;
+            if (sl > 0) delta = fmin(2 * maxnats / sl, 1e3 * fmax(fabs(tp), 1e-3));
+            else delta = delta == 0.0 ? fmax(fabs(tp), 1e-6) : 2 * delta;
+            t = tp + sgn * delta;
+            if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;
+        }
+        double c = beta * sms_eval(f, t);
+        double a0 = tp, xs = tp, x = 0.0;
+        while (sms_next_root(f, 1, tp, t, xs, sgn, &x)) {
+            double cx = beta * sms_eval(f, x);
+            double xb = a0, y = 0.0;
+            while (sms_next_root(f, 0, a0, x, xb, sgn, &y)) {
+                double cy = beta * sms_eval(f, y);
+                bpush(P, y, cy);
+                cmin = fmin(cmin, cy);
+                xb = y;
+            }
+            bpush(P, x, cx);
+            cmin = fmin(cmin, cx);
+            a0 = x;
+            xs = x;
+        }
+        double xb = a0, y = 0.0;
+        while (sms_next_root(f, 0, a0, t, xb, sgn, &y)) {
+            double cy = beta * sms_eval(f, y);
+            bpush(P, y, cy);
+            cmin = fmin(cmin, cy);
+            xb = y;
+        }
+        bpush(P, t, c);
+        if (P->full) return -1;
+        cmin = fmin(cmin, c);
+        if (c - cmin > maxnats) return 1;
+        tp = t;
+    }
+}
+
+/* the draw from two walked point lists of any length (the draw_points arithmetic) */
+static double draw_points_big(const BigPts *PL, const BigPts *PR, double useg, double uoff, int *status) {
+    int np = PL->n + 1 + PR->n;
+    double *pts = (double *)malloc(sizeof(double) * np), *cs = (double *)malloc(sizeof(double) * np);
+    double *lm = (double *)malloc(sizeof(double) * (np > 1 ? np - 1 : 1));
+    for (int q = 0; q < PL->n; q++) {
+        pts[q] = PL->p[PL->n - 1 - q];
+        cs[q] = PL->c[PL->n - 1 - q];
+    }
+    pts[PL->n] = 0.0;
+    cs[PL->n] = 0.0;
+    for (int q = 0; q < PR->n; q++) {
+        pts[PL->n + 1 + q] = PR->p[q];
+        cs[PL->n + 1 + q] = PR->c[q];
+    }
+    int nseg = np - 1;
+    double mx = -INFINITY;
+    for (int q = 0; q < nseg; q++) {
+        double L = pts[q + 1] - pts[q];
+        lm[q] = L > 0 ? -fmin(cs[q], cs[q + 1]) + logseg(L, fabs(cs[q + 1] - cs[q])) : -INFINITY;
+        mx = fmax(mx, lm[q]);
+    }
+    double tot = 0.0;
+    for (int q = 0; q < nseg; q++) tot += exp(lm[q] - mx);
+    double u = useg * tot, acc = 0.0;
+    int jj = nseg - 1;
+    for (int q = 0; q < nseg; q++) {
+        acc += exp(lm[q] - mx);
+        if (u <= acc) {
+            jj = q;
+            break;
+        }
+    }
+    double L = pts[jj + 1] - pts[jj];
+    double g = (cs[jj + 1] - cs[jj]) / L;
+    double t = (g >= 0 || fabs(g * L) < 1e-10) ? pts[jj] + trexp_offset(g, L, uoff) : pts[jj + 1] - trexp_offset(-g, L, uoff);
+    *status = 0;
+    if (!isfinite(t)) {
+        *status = 2;
+        t = 0.0;
+    }
+    free(pts); free(cs); free(lm);
+    return t;
+}
+
+static double sms_sample(const SmsMove *f, double beta, double maxnats, double useg, double uoff, int *status) {
+    static __thread double bufs[4][SMS_PMAX];
+    BigPts PR = {bufs[0], bufs[1], 0, 0}, PL = {bufs[2], bufs[3], 0, 0};
+    int w = sms_walk(f, beta, 1.0, maxnats, &PR);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    w = sms_walk(f, beta, -1.0, maxnats, &PL);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    return draw_points_big(&PL, &PR, useg, uoff, status);
+}
+
+/* stage the block values of replica j and the kinks at the block's ends (prorata! with
+ * soc_qadj, src/local.jl:1782-1840) */
+static void sms_prepare(SmsMove *f, Sys *S, int j, int L, int b0, int si, const int *nodes, int nu) {
+    const esm_model *m = S->m;
+    int N = m->N, T = m->T;
+    f->S = S; f->j = j; f->L = L; f->b0 = b0; f->si = si; f->nu = nu;
+    f->W = 4;
+    while (f->W < L + 1 && f->W < 64) f->W *= 2;
+    for (int u = 0; u < nu; u++) {
+        f->node[u] = nodes[u];
+        f->kst[u] = S->kc[(ESM_KSTOR + si) * N + nodes[u]];
+        f->ks[u] = f->kst[u] * m->stor_hours[si * N + nodes[u]] / m->dt;
+    }
+    double *ye_a = S->ye[si];
+    for (int q = 0; q <= L; q++) {
+        int tau = b0 + q, tau1 = tau + 1 == T ? 0 : tau + 1;
+        for (int u = 0; u < nu; u++) {
+            int n = nodes[u];
+            f->ye[q * SMS_NU + u] = AT(ye_a, n, tau);
+            f->ye1[q * SMS_NU + u] = AT(ye_a, n, tau1);
+            f->S0[q * SMS_NU + u] = AT(S->r, n, tau);
+        }
+    }
+    for (int kidx = 0; kidx < 2; kidx++) {
+        int q = kidx == 0 ? 0 : L, tau = b0 + q;
+        for (int u = 0; u < nu; u++) {
+            SmsKink *kk = &f->kk[kidx][u];
+            memset(kk, 0, sizeof(*kk));
+            int n = nodes[u], us = si * N + n;
+            double k = S->kc[ESM_KGAS * N + n];
+            double g = kidx == 0 ? f->ks[u] * (1.0 - 0.0) : f->ks[u] * (0.0 - 1.0);
+            if (!(fabs(k) >= 1.0) || g == 0.0) continue;
+            double w = AT(S->yg, n, tau);
+            double P = 0.0 + k * w, Qs = 0.0 + k * clampd(w, 0.0, 1.0);
+            if (fabs(P) < 1e-6) continue;
+            double Y = f->ye[q * SMS_NU + u], Y1 = f->ye1[q * SMS_NU + u];
+            double ec = clampd(Y, 0.0, 1.0), ec1 = clampd(Y1, 0.0, 1.0);
+            double d = m->stor_hours[us] * (ec1 - ec) / m->dt - m->stor_qt[us], pc, mcp;
+            if (d >= 0) { pc = m->stor_eta_in[us] > 0 ? d / m->stor_eta_in[us] : 0.0; mcp = 0.0; }
+            else { pc = 0.0; mcp = fmin(-d * m->stor_eta_out[us], 1.0); }
+            double qadj = 0.0 + (f->kst[u] * m->stor_hours[us] * (Y1 - Y) / m->dt - f->kst[u] * (pc - mcp));
+            kk->Q = AT(S->r, n, tau) + Qs + qadj;
+            kk->g = g;
+            kk->e = P - fmax(kk->Q, 0.0);
+            kk->P = P;
+            double sig = k * w / P;
+            kk->ak = sig == 0.0 ? 0.0 : sig / k;
+            kk->y0 = w;
+            kk->valid = 1;
+        }
+    }
+}
+
+static void sms_commit(const SmsMove *f, double t) {
+    Sys *S = f->S;
+    int j = f->j;
+    if (t == 0.0) return;
+    for (int q = 0; q <= f->L; q++) {
+        int tau = f->b0 + q, kidx = q == 0 ? 0 : (q == f->L ? 1 : -1);
+        for (int u = 0; u < f->nu; u++) {
+            int n = f->node[u];
+            double S1;
+            sms_unit_step(f, q, u, t, NULL, &S1);
+            AT(S->r, n, tau) = S1;
+            if (q >= 1) AT(S->ye[f->si], n, tau) = f->ye[q * SMS_NU + u] + t;
+            if (kidx >= 0) {
+                const SmsKink *kk = &f->kk[kidx][u];
+                if (kk->valid && kk->ak != 0.0) AT(S->yg, n, tau) = kk->y0 + kk->ak * sm_phi(kk->Q, kk->g, t);
+            }
+        }
+    }
+}
+
+static double sms_logjac(const SmsMove *f, double t, int *any) {
+    double part[64] = {0};
+    int cnt = 0;
+    for (int kidx = 0; kidx < 2; kidx++) {
+        int q = kidx == 0 ? 0 : f->L, lane = q % f->W;
+        for (int u = 0; u < f->nu; u++) {
+            const SmsKink *kk = &f->kk[kidx][u];
+            if (!kk->valid) continue;
+            cnt++;
+            double nm = 0.0;
+            part[lane] += nm * log(fabs((fmax(kk->Q + kk->g * t, 0.0) + kk->e) / kk->P));
+        }
+    }
+    *any = cnt > 0;
+    for (int off = f->W / 2; off > 0; off >>= 1) {
+        double nx[64];
+        for (int l = 0; l < f->W; l++) nx[l] = part[l] + part[l ^ off];
+        memcpy(part, nx, sizeof(double) * f->W);
+    }
+    return part[0];
+}
+
+/* units and country–carrier groups (cells sorted, then carrier; nodes increasing) */
+typedef struct {
+    int nunits, ngroups;
+    int grp_off[2 * 256 + 1], grp_node[2 * 256], grp_si[2 * 256];
+} SmsTables;
+
+static int sms_tables(const esm_model *m, SmsTables *st) {
+    int N = m->N;
+    st->nunits = ESM_NSTOR * N;
+    int cells[256], nc = 0;
+    for (int n = 0; n < N; n++) {
+        int c = m->ncountry > 0 ? m->country[n] : n, seen = 0;
+        for (int q = 0; q < nc; q++) seen |= cells[q] == c;
+        if (!seen) cells[nc++] = c;
+    }
+    qsort(cells, nc, sizeof(int), cmp_int);
+    st->ngroups = 0;
+    int off = 0;
+    st->grp_off[0] = 0;
+    for (int q = 0; q < nc; q++)
+        for (int s = 0; s < ESM_NSTOR; s++) {
+            int cnt = 0;
+            for (int n = 0; n < N; n++)
+                if ((m->ncountry > 0 ? m->country[n] : n) == cells[q]) { st->grp_node[off++] = n; cnt++; }
+            if (cnt > SMS_NU) return -1;
+            st->grp_si[st->ngroups] = s;
+            st->grp_off[++st->ngroups] = off;
+        }
+    return 0;
+}
+
+/* one storage multiscale move of replica j; returns t and counts it */
+static double sms_move(Sys *S, int j, int L, int b0, int si, const int *nodes, int nu, double useg, double uoff,
+                       double umh, double maxnats, int64_t *stats) {
+    static __thread double ye[129 * SMS_NU], ye1[129 * SMS_NU], S0[129 * SMS_NU];
+    SmsMove f;
+    f.ye = ye; f.ye1 = ye1; f.S0 = S0;
+    sms_prepare(&f, S, j, L, b0, si, nodes, nu);
+    int status;
+    g_nev = 0;
+    double t = sms_sample(&f, S->m->beta_v, maxnats, useg, uoff, &status);
+    if (t != 0.0) {
+        int any;
+        double lj = sms_logjac(&f, t, &any);
+        if (any && !(umh < exp(fmin(0.0, lj)))) { t = 0.0; status = 2; }
+    }
+    long nev = g_nev;
+    sms_commit(&f, t);
+    if (stats) stat_add(stats, status, t, nev);
+    return t;
+}
+
+/* the schedule of one storage sweep: whether it runs, lengths in order, offsets, first class */
+static int sms_schedule(int T, double p, const uint32_t key[2], uint32_t ctr, int *lengths, int *offs, int *first) {
+    uint32_t c[4] = {0u, 0u, 9300u, ctr}, w[4];
+    orc_philox(key, c, w);
+    if (!(u01(w[0], w[1]) < p)) return 0;
+    int n = 0;
+    for (int L = 2; L <= 128; L *= 2)
+        if (L < T / 2) lengths[n++] = L;
+    for (int i = n - 1; i >= 1; i--) {
+        uint32_t c2[4] = {(uint32_t)i, 0u, 9301u, ctr};
+        orc_philox(key, c2, w);
+        int jx = (int)(((uint64_t)w[0] * (uint32_t)(i + 1)) >> 32);
+        int tmp = lengths[i]; lengths[i] = lengths[jx]; lengths[jx] = tmp;
+    }
+    for (int q = 0; q < n; q++) {
+        uint32_t c3[4] = {(uint32_t)lengths[q], 0u, 9302u, ctr};
+        orc_philox(key, c3, w);
+        offs[q] = (int)(((uint64_t)w[0] * (uint32_t)lengths[q]) >> 32);
+        first[q] = u01(w[2], w[3]) < 0.5 ? 0 : 1;
+    }
+    return n;
+}
+
+/* the storage sweep of replica j (nmoves per block, group probability p_agg) */
+static void sms_sweep(Sys *S, const SmsTables *st, int j, double pms, const uint32_t key[2], uint32_t ctr,
+                      double maxnats, int64_t *stats) {
+    const esm_model *m = S->m;
+    int T = m->T, N = m->N;
+    int lengths[8], offs[8], first[8];
+    int nlen = sms_schedule(T, pms, key, ctr, lengths, offs, first);
+    for (int li = 0; li < nlen; li++) {
+        int L = lengths[li], o = offs[li];
+        if (o > T - L - 1) continue;
+        int nb = (T - L - 1 - o) / L + 1;
+        if (nb < 2) continue;
+        for (int ci = 0; ci < 2; ci++) {
+            int cls = ci == 0 ? first[li] : 1 - first[li];
+            for (int bi = cls; bi < nb; bi += 2) {
+                int b0 = o + L * bi;
+                for (int mv = 0; mv < 8; mv++) {
+                    uint32_t c4[4] = {(uint32_t)mv, (uint32_t)b0, 9200u + (uint32_t)L, ctr}, s[4];
+                    orc_philox(key, c4, s);
+                    int si, nu, nodes[SMS_NU];
+                    if (u01(s[0], s[1]) < 0.5) {
+                        int g = (int)(((uint64_t)s[2] * (uint32_t)st->ngroups) >> 32);
+                        si = st->grp_si[g];
+                        nu = st->grp_off[g + 1] - st->grp_off[g];
+                        for (int u = 0; u < nu; u++) nodes[u] = st->grp_node[st->grp_off[g] + u];
+                    } else {
+                        int un = (int)(((uint64_t)s[3] * (uint32_t)st->nunits) >> 32);
+                        si = un / N;
+                        nu = 1;
+                        nodes[0] = un % N;
+                    }
+                    uint32_t cl[4] = {(uint32_t)j, (uint32_t)b0, 0x50000000u + (uint32_t)(L * 16 + mv), ctr}, wl[4];
+                    orc_philox(key, cl, wl);
+                    uint32_t cm[4] = {(uint32_t)j, (uint32_t)b0, 0x70000000u + (uint32_t)(L * 16 + mv), ctr}, wm[4];
+                    orc_philox(key, cm, wm);
+                    sms_move(S, j, L, b0, si, nodes, nu, u01(wl[0], wl[1]), u01(wl[2], wl[3]), u01(wm[0], wm[1]),
+                             maxnats, stats);
+                }
+            }
+        }
+    }
+}
+
+/* ---- explicit storage multiscale moves, for the pins ---- */
+int orc_sms_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int L, int b0,
+                 int si, const int *nodes, int nu, const double *t, int nt, double *dc) {
+    static __thread Sys S;
+    static __thread double ye[129 * SMS_NU], ye1[129 * SMS_NU], S0[129 * SMS_NU];
+    int ptr[257], lines[2 * 2048];
+    if (m->N > 256 || m->L > 2048 || L > 128 || nu > SMS_NU || b0 + L >= m->T) return -1;
+    sys_view(&S, m, R, v, k, r, F, ptr, lines);
+    SmsMove f;
+    f.ye = ye; f.ye1 = ye1; f.S0 = S0;
+    sms_prepare(&f, &S, j, L, b0, si, nodes, nu);
+    for (int q = 0; q < nt; q++) dc[q] = sms_eval(&f, t[q]);
+    return 0;
+}
+
+int orc_sms_draw_many(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int L,
+                      int b0, int si, const int *nodes, int nu, double maxnats, const double *useg, const double *uoff,
+                      int nd, double *t_out, int *status) {
+    static __thread Sys S;
+    static __thread double ye[129 * SMS_NU], ye1[129 * SMS_NU], S0[129 * SMS_NU];
+    int ptr[257], lines[2 * 2048];
+    if (m->N > 256 || m->L > 2048 || L > 128 || nu > SMS_NU || b0 + L >= m->T) return -1;
+    sys_view(&S, m, R, v, k, r, F, ptr, lines);
+    SmsMove f;
+    f.ye = ye; f.ye1 = ye1; f.S0 = S0;
+    sms_prepare(&f, &S, j, L, b0, si, nodes, nu);
+    for (int q = 0; q < nd; q++) t_out[q] = sms_sample(&f, m->beta_v, maxnats, useg[q], uoff[q], &status[q]);
+    return 0;
+}
+
+int orc_sms_apply(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int L, int b0,
+                  int si, const int *nodes, int nu, double t) {
+    static __thread Sys S;
+    static __thread double ye[129 * SMS_NU], ye1[129 * SMS_NU], S0[129 * SMS_NU];
+    int ptr[257], lines[2 * 2048];
+    if (m->N > 256 || m->L > 2048 || L > 128 || nu > SMS_NU || b0 + L >= m->T) return -1;
+    sys_view(&S, m, R, v, k, r, F, ptr, lines);
+    SmsMove f;
+    f.ye = ye; f.ye1 = ye1; f.S0 = S0;
+    sms_prepare(&f, &S, j, L, b0, si, nodes, nu);
+    sms_commit(&f, t);
+    return 0;
+}
+
+int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
+               const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
+               double pms, int64_t *stats, int nthreads) {
+    int N = m->N, L = m->L, T = m->T;
+    if (N > 256) return -1;
+    SmsTables smst;
+    if (sms_tables(m, &smst)) return -3;
+    static const double wdef[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
+    const double *wts = weights ? weights : wdef;
+    size_t NTR = (size_t)N * T * R;
+    Sys S0;
+    S0.m = m;
+    S0.R = R;
+    S0.th = v;
+    S0.yg = v + NTR;
+    S0.ye[0] = v + 2 * NTR;
+    S0.ye[1] = v + 3 * NTR;
+    S0.r = r;
+    S0.F = F;
+    for (int u = 0; u < ESM_NK * N; u++) S0.kc[u] = clampd(k[u], m->klo[u], m->khi[u]);
+    S0.adj_ptr = (int *)calloc(N + 1, sizeof(int));
+    S0.adj_line = (int *)malloc(sizeof(int) * 2 * L);
+    int *fill = (int *)malloc(sizeof(int) * N), *colour = (int *)malloc(sizeof(int) * N);
+    for (int e = 0; e < L; e++) {
+        S0.adj_ptr[m->line_bus0[e] + 1]++;
+        S0.adj_ptr[m->line_bus1[e] + 1]++;
+    }
+    for (int n = 0; n < N; n++) S0.adj_ptr[n + 1] += S0.adj_ptr[n];
+    for (int n = 0; n < N; n++) fill[n] = S0.adj_ptr[n];
+    for (int e = 0; e < L; e++) {
+        S0.adj_line[fill[m->line_bus0[e]]++] = e;
+        S0.adj_line[fill[m->line_bus1[e]]++] = e;
+    }
+    int nc = colouring(m, S0.adj_ptr, S0.adj_line, colour);
+    Tables tb;
+    if (build_tables(m, &tb)) return -2;
+    /* time-parity classes */
+    int cfirst[3], ccount[3], ncls;
+    if (T % 2 == 0) {
+        ncls = 2; cfirst[0] = 0; ccount[0] = T / 2; cfirst[1] = 1; ccount[1] = T / 2;
+    } else {
+        ncls = 3; cfirst[0] = 0; ccount[0] = (T - 1) / 2; cfirst[1] = 1; ccount[1] = (T - 1) / 2;
+        cfirst[2] = T - 1; ccount[2] = 1;
+    }
+    int *counts = (int *)malloc(sizeof(int) * ESM_NWEIGHT * T * (nsweeps > 0 ? nsweeps : 1));
+    for (int sw = 0; sw < nsweeps; sw++) move_counts(m, extra, wts, key, ctr0 + (uint32_t)sw, counts + (size_t)sw * ESM_NWEIGHT * T);
+    int64_t st[ESM_NFAM * ESM_NSTAT];
+    memset(st, 0, sizeof(st));
+    if (nthreads < 1) nthreads = 1;
+    /* replicas are independent: threads over replicas, each with its own statistics */
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : st[:ESM_NFAM * ESM_NSTAT])
+    for (int j = 0; j < R; j++) {
+        Sys S = S0;
+        for (int sw = 0; sw < nsweeps; sw++) {
+            uint32_t ctr = ctr0 + (uint32_t)sw;
+            if (atoms) {
+                for (int n = 0; n < N; n++)
+                    for (int t = 0; t < T; t++) do_move(&S, j, 0, 0, n, t, key, 0u, ctr, maxnats, st + 5 * ESM_FAM_GEN);
+                for (int c = 0; c < nc; c++)
+                    for (int n = 0; n < N; n++) {
+                        if (colour[n] != c || S.adj_ptr[n + 1] == S.adj_ptr[n]) continue;
+                        for (int t = 0; t < T; t++)
+                            do_move(&S, j, 1, 0, n, t, key, 1000u + (uint32_t)c, ctr, maxnats, st + 5 * ESM_FAM_THETA);
+                    }
+                for (int s = 0; s < ESM_NSTOR; s++)
+                    for (int c = 0; c < ncls; c++)
+                        for (int n = 0; n < N; n++)
+                            for (int q = 0; q < ccount[c]; q++) {
+                                int t = cfirst[c] + 2 * q;
+                                do_move(&S, j, 2, s, n, t, key, 2000u + (uint32_t)(s * 4 + c), ctr, maxnats,
+                                        st + 5 * ESM_FAM_SOC);
+                            }
+            }
+            const int *cnt = counts + (size_t)sw * ESM_NWEIGHT * T;
+            for (int c = 0; c < ncls; c++)
+                for (int q = 0; q < ccount[c]; q++)
+                    random_step(&S, &tb, j, cfirst[c] + 2 * q, cnt, wts, key, ctr, maxnats, st);
+            if (pms > 0) sms_sweep(&S, &smst, j, pms, key, ctr, maxnats, st + 5 * ESM_FAM_SMS);
+        }
+    }
+    if (stats) memcpy(stats, st, sizeof(st));
+    free(S0.adj_ptr); free(S0.adj_line); free(fill); free(colour); free(counts);
+    tables_free(&tb);
+    return 0;
+}
+

@@ -103,12 +103,12 @@ def test_random_family_fp64_matches_oracle(family):
     w[WEIGHT_NAMES.index("pair")] = 0.5           # void draws take their share
     dev = api.Replicas(s, R, FP64)
     dev.upload(v=v, k=k, x=x)
-    api.sweep_options(dev, extra=2.0, weights=w, atoms=False)
+    api.sweep_options(dev, extra=2.0, weights=w, atoms=False, p_storage_ms=0.0)
     st = api.operation_sweep(dev, 2, key=(8, 1), counter=5)
     vd, _, _ = dev.download()
     rd, Fd = dev.caches()
     v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, k, 2, key=(8, 1), counter=5, x=x, threads=4, extra=2.0, weights=w,
-                                               atoms=False)
+                                               atoms=False, p_storage_ms=0.0)
     assert st_[:, 0].sum() > 0
     assert np.array_equal(st[:, :4], st_[:, :4]), (st[:, :4], st_[:, :4])
     for f, scale in ((0, max(np.abs(v_[0]).max(), 1.0)), (1, 1.0), (2, 1.0), (3, 1.0)):
@@ -132,7 +132,7 @@ def test_full_sweep_fp64_matches_oracle_40_nodes():
     st = api.operation_sweep(dev, 1, key=(2, 3), counter=1)
     vd, _, _ = dev.download()
     v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 1, key=(2, 3), counter=1, x=x, threads=4)
-    assert np.all(st_[:, 0] > 0)
+    assert np.all(st_[:10, 0] > 0)              # the storage sweep runs with probability 1/2
     assert np.array_equal(st[:, :4], st_[:, :4])
     np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
     np.testing.assert_allclose(vd[0], v_[0], atol=1e-9 * np.abs(v_[0]).max())
@@ -154,9 +154,55 @@ def test_sweep_fp64_matches_oracle_64_replicas():
     vd, _, _ = dev.download()
     rd, Fd = dev.caches()
     v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 2, key=(5, 6), counter=3, x=x, threads=8)
-    assert np.all(st_[:, 0] > 0)
+    assert np.all(st_[:10, 0] > 0)
     assert np.array_equal(st[:, :4], st_[:, :4])
     np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
     np.testing.assert_allclose(vd[0], v_[0], atol=1e-9 * np.abs(v_[0]).max())
     np.testing.assert_allclose(rd, r_, atol=1e-9 * np.abs(s.load).max())
+    dev.close()
+
+
+@pytest.mark.parametrize("R,T", [(4, 47), (64, 20), (3, 300)])
+def test_storage_ms_fp64_matches_oracle(R, T):
+    """The storage multiscale sweep alone (every block length below T/2, segments of 4 … 64 lanes)."""
+    print(f"storage_ms fp64 R={R} T={T}", flush=True)
+    s = synth.make_system(N=10, L=16, T=T, dt=1.0, seed=12)
+    rng = np.random.default_rng(R + T)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    v[2:] = rng.uniform(0.1, 0.9, v[2:].shape)
+    dev = api.Replicas(s, R, FP64)
+    dev.upload(v=v, k=s.k0, x=x)
+    api.sweep_options(dev, extra=0.0, atoms=False, p_storage_ms=1.0)
+    st = api.operation_sweep(dev, 1, key=(4, 9), counter=2)
+    vd, _, _ = dev.download()
+    rd, Fd = dev.caches()
+    v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 1, key=(4, 9), counter=2, x=x, threads=4, extra=0.0,
+                                               atoms=False, p_storage_ms=1.0)
+    assert st_[10, 0] > 0 and np.all(st_[:10, 0] == 0)
+    assert np.array_equal(st[:, :4], st_[:, :4]), (st[10], st_[10])
+    np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
+    np.testing.assert_allclose(rd, r_, atol=1e-9 * np.abs(s.load).max())
+    dev.close()
+
+
+def test_storage_ms_fp32_consistent():
+    print("storage_ms fp32", flush=True)
+    s = synth.make_system(N=10, L=16, T=300, dt=1.0, seed=13)
+    R = 64
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R)).astype(np.float32).astype(np.float64)
+    v = synth.heuristic_state(s, x)
+    v[2:] = rng.uniform(0.1, 0.9, v[2:].shape)
+    v = v.astype(np.float32).astype(np.float64)
+    dev = api.Replicas(s, R, FP32)
+    dev.upload(v=v, k=s.k0, x=x)
+    api.sweep_options(dev, extra=0.0, atoms=False, p_storage_ms=1.0)
+    st = api.operation_sweep(dev, 2, key=(6, 6), counter=0)
+    _, _, drift = api.LocalCost(dev)
+    v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 2, key=(6, 6), counter=0, x=x, threads=8, extra=0.0,
+                                               atoms=False, p_storage_ms=1.0)
+    assert st[10, 0] == st_[10, 0] and st[10, 2] == 0 and st[10, 3] == 0
+    assert abs(st[10, 1] - st_[10, 1]) <= max(3, 0.02 * st_[10, 1])
+    assert np.all(drift < 1e-3 * np.abs(s.load).max())
     dev.close()
