@@ -30,7 +30,7 @@ constexpr int SMS_PMAX = 4096;     // walk points per side
 struct SmsArgs {
     uint32_t k0, k1, ctr;
     int L, base, nblk, nrg;        // block b0 = base + 2L·blk, blk < nblk; nrg replica groups
-    int nmoves;
+    int nmoves, nus;               // nus: units of the largest group (LDS plane stride)
     double p_agg;
     const int *units;              // [2N] unit u: storage carrier u / N, node u % N
     int nunits;
@@ -60,7 +60,13 @@ __device__ __forceinline__ bool seg_best(bool have, S x, S sgn, S &out) {
 template <typename S>
 struct SmsKink {
     S Q, g, ak, y0, P, e, valid;
+    S gc0, gp0;                    // gas cost and output at t = 0
 };
+
+// per-(step, unit) LDS planes of a segment, each [L + 1][nus]: the staged values and the terms
+// at t = 0, then the walk's cached secondaries (d at the interval ends tp and t, balances at the
+// piece ends a0 and x); every plane entry is written and read by the lane that owns the step only
+enum SmsPlane { P_YE, P_YE1, P_S0, P_C0, P_J0, P_PH0, P_DA, P_DB, P_BA, P_BX, SMS_NPLANE };
 
 // the moved unit u of a segment: its storage parameters, clamped capacity, κs, node
 template <typename S>
@@ -74,12 +80,15 @@ struct SmsUnit {
 template <typename S, int W>
 struct FStor {
     const DevModel<S> *m;
-    int L, sub, nu;
-    const S *ye_l, *ye1_l, *S0_l;      // [q][u]
+    int L, sub, nu, nus;                // nus: the plane stride (the largest group)
+    S *pl;                             // SMS_NPLANE planes [q][u] of stride (L + 1)·nus
     const SmsKink<S> *kk_l;            // [2][u]: the kinks at steps 0 and L
     const SmsUnit<S> *un_l;            // [u]
     mutable int nev = 0;
+    enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
+    static constexpr int SP = W < 64 ? 1 : SMS_SP;     // block steps per lane (L + 1 ≤ W below 64)
 
+    __device__ S *plane(int k) const { return pl + (size_t)k * (L + 1) * nus; }
     __device__ S gen_c(int n, S y, S &p) const {
         S c;
         const S k = m->kc[ESM_KGAS * m->N + n];
@@ -87,58 +96,76 @@ struct FStor {
         return c;
     }
     __device__ static S kphi(S Q, S g, S t) { return fmax(Q + g * t, S(0)) - fmax(Q, S(0)); }
+    // the storage secondaries of a unit from its d: d, d − η_in (1 without charging), −d η_out − 1
+    __device__ static S sec(int k, S d, const StoPar<S> &p) {
+        return k == 0 ? d : (k == 1 ? (p.ein > 0 ? d - p.ein : S(1)) : -d * p.eout - S(1));
+    }
 
     // the terms of unit u at block step q at t, in this order: its node's kinked gas (block
-    // ends), its storage term, its node's balance; the storage secondaries (sec3) and the balance
-    __device__ S unit_step(int q, int u, S t, S *sec3, S &S1) const {
+    // ends), its storage term, its node's balance; its d (storage secondaries) and balance
+    __device__ S unit_step(int q, int u, S t, S &d1, S &S1) const {
         const SmsUnit<S> U = un_l[u];
+        const int i = q * nus + u;
         S D = 0, dr = 0;
         const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
         if (kidx >= 0) {
-            const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+            const SmsKink<S> kk = kk_l[kidx * nus + u];
             if (kk.valid != S(0) && kk.ak != S(0)) {
-                S p0, p1;
-                const S c0 = gen_c(U.node, kk.y0, p0);
+                S p1;
                 const S c1 = gen_c(U.node, kk.y0 + kk.ak * kphi(kk.Q, kk.g, t), p1);
-                D += c1 - c0;
-                dr -= p1 - p0;
+                D += c1 - kk.gc0;
+                dr -= p1 - kk.gp0;
             }
         }
         const S dye = q >= 1 ? S(1) : S(0), dye1 = q <= L - 1 ? S(1) : S(0);
-        const S ye = ye_l[q * SMS_NU + u], ye1 = ye1_l[q * SMS_NU + u];
         const S w = U.kst + m->kappa0;
-        S d0, j0, d1, j1;
-        const S c0 = soc_cost_p(U.par, m->dt, m->cv, U.kst, w, ye, ye1, d0, j0);
-        const S c1 = soc_cost_p(U.par, m->dt, m->cv, U.kst, w, ye + t * dye, ye1 + t * dye1, d1, j1);
-        D += c1 - c0;
-        dr += j1 - j0;
-        if (sec3) {
-            sec3[0] = d1;
-            sec3[1] = U.par.ein > 0 ? d1 - U.par.ein : S(1);
-            sec3[2] = -d1 * U.par.eout - S(1);
-        }
-        const S S0 = S0_l[q * SMS_NU + u];
-        S1 = S0 + dr;
-        D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+        S j1;
+        const S c1 = soc_cost_p(U.par, m->dt, m->cv, U.kst, w, plane(P_YE)[i] + t * dye, plane(P_YE1)[i] + t * dye1,
+                                d1, j1);
+        D += c1 - plane(P_C0)[i];
+        dr += j1 - plane(P_J0)[i];
+        S1 = plane(P_S0)[i] + dr;
+        D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - plane(P_PH0)[i]);
         return D;
     }
 
-    __device__ S step(int q, S t) const {
-        S D = 0, S1;
-        for (int u = 0; u < nu; u++) D += unit_step(q, u, t, nullptr, S1);
-        return D;
-    }
-
-    // ΔC(t): the lanes' partial sums (own steps in increasing order), summed over the segment
-    __device__ S eval(S t) const {
-        nev++;
+    // ΔC(t): the lanes' partial sums (own steps in increasing order), summed over the segment.
+    // Recording: REC_B d and the balance at t (an interval's end), REC_X the balance (a storage
+    // root); uncounted (no evaluation of the reference's walk): REC_A d and the balance at the
+    // walk's start, REC_T the balance at an interval's end after a split
+    __device__ S eval(S t, int rec) const {
+        if (rec <= REC_X) nev++;
         S D = 0;
-#pragma unroll
-        for (int i = 0; i < SMS_SP; i++) {
+#pragma nounroll
+        for (int i = 0; i < SP; i++) {
             const int q = sub + W * i;
-            if (q <= L) D += step(q, t);
+            if (q > L) continue;
+            S Dq = 0;
+            for (int u = 0; u < nu; u++) {
+                S d1, S1;
+                Dq += unit_step(q, u, t, d1, S1);
+                const int k = q * nus + u;
+                if (rec == REC_B) plane(P_DB)[k] = d1;
+                if (rec == REC_A) plane(P_DA)[k] = d1;
+                if (rec != REC_NONE) plane(rec == REC_A ? P_BA : P_BX)[k] = S1;
+            }
+            D += Dq;
         }
         return seg_sum<S, W>(D);
+    }
+
+    // the piece's end becomes the next piece's start (balances; with d: the interval's end)
+    __device__ void shift(bool d) const {
+#pragma nounroll
+        for (int i = 0; i < SP; i++) {
+            const int q = sub + W * i;
+            if (q > L) continue;
+            for (int u = 0; u < nu; u++) {
+                const int k = q * nus + u;
+                if (d) plane(P_DA)[k] = plane(P_DB)[k];
+                plane(P_BA)[k] = plane(P_BX)[k];
+            }
+        }
     }
 
     // the nearest primary breakpoint beyond tp (segment minimum)
@@ -151,14 +178,14 @@ struct FStor {
                 have = true;
             }
         };
-#pragma unroll
-        for (int i = 0; i < SMS_SP; i++) {
+#pragma nounroll
+        for (int i = 0; i < SP; i++) {
             const int q = sub + W * i;
             if (q > L) continue;
             const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
             for (int u = 0; u < nu; u++) {
                 if (kidx >= 0) {
-                    const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+                    const SmsKink<S> kk = kk_l[kidx * nus + u];
                     if (kk.valid != S(0)) {
                         cand(-kk.Q / kk.g);
                         if (kk.ak != S(0)) {
@@ -171,7 +198,7 @@ struct FStor {
                     }
                 }
                 if (q >= 1) {
-                    const S ye = ye_l[q * SMS_NU + u];
+                    const S ye = plane(P_YE)[q * nus + u];
                     cand((S(0) - ye) / S(1));
                     cand((S(1) - ye) / S(1));
                 }
@@ -181,22 +208,24 @@ struct FStor {
     }
 
     // the nearest sign change beyond x, strictly inside (a, b), of the storage secondaries
-    // (stor) or of the node balances, from their values at a and b (linear in between)
+    // (stor: from d at the interval's ends, P_DA and P_DB) or of the node balances (from the
+    // piece's ends, P_BA and P_BX), linear in between
     __device__ bool next_root(bool stor, S a, S b, S x, S sgn, S &xr) const {
         bool have = false;
         S best = 0;
-#pragma unroll
-        for (int i = 0; i < SMS_SP; i++) {
+#pragma nounroll
+        for (int i = 0; i < SP; i++) {
             const int q = sub + W * i;
             if (q > L) continue;
             for (int u = 0; u < nu; u++) {
-                S sa[3], sb[3], Ba, Bb;
-                unit_step(q, u, a, sa, Ba);
-                unit_step(q, u, b, sb, Bb);
+                const int k0 = q * nus + u;
+                const StoPar<S> par = un_l[u].par;
+                const S da = plane(P_DA)[k0], db = plane(P_DB)[k0];
+                const S ba = plane(P_BA)[k0], bb = plane(P_BX)[k0];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     if (!stor && k > 0) break;
-                    const S A = stor ? sa[k] : Ba, B = stor ? sb[k] : Bb;
+                    const S A = stor ? sec(k, da, par) : ba, B = stor ? sec(k, db, par) : bb;
                     if ((A < 0) != (B < 0) && A != B) {
                         const S r = a + (b - a) * A / (A - B);
                         if (sgn * (r - a) > 0 && sgn * (b - r) > 0 && sgn * (r - x) > 0 &&
@@ -215,13 +244,13 @@ struct FStor {
     __device__ double logjac(S t, bool &any) const {
         double lj = 0.0;
         int cnt = 0;
-#pragma unroll
-        for (int i = 0; i < SMS_SP; i++) {
+#pragma nounroll
+        for (int i = 0; i < SP; i++) {
             const int q = sub + W * i;
             const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
             if (q > L || kidx < 0) continue;
             for (int u = 0; u < nu; u++) {
-                const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+                const SmsKink<S> kk = kk_l[kidx * nus + u];
                 if (kk.valid == S(0)) continue;
                 cnt++;
                 const double nm = 0.0;
@@ -233,10 +262,15 @@ struct FStor {
     }
 };
 
-// words (of S) of one segment's LDS: the staged values, the kinks and the units (padded to S)
+// words (of S) of one segment's LDS: the planes and the kinks
 template <typename S>
-__host__ __device__ constexpr size_t sms_seg_words(int nstep) {
-    return 3 * (size_t)nstep * SMS_NU + 7 * 2 * SMS_NU + (sizeof(SmsUnit<S>) + sizeof(S) - 1) / sizeof(S) * SMS_NU;
+__host__ __device__ constexpr size_t sms_seg_words(int nstep, int nus) {
+    return SMS_NPLANE * (size_t)nstep * nus + sizeof(SmsKink<S>) / sizeof(S) * 2 * nus;
+}
+// words (of S) of the wave's moved units, shared by its segments (the move is the wave's)
+template <typename S>
+__host__ __device__ constexpr size_t sms_unit_words() {
+    return (sizeof(SmsUnit<S>) + sizeof(S) - 1) / sizeof(S) * SMS_NU;
 }
 
 template <class V, typename S>
@@ -247,44 +281,107 @@ __device__ __forceinline__ int sms_visit(V &vis, S x, S cx, S &cmin, int &np) {
     return 0;
 }
 
-// walk! with storage secondaries: 1 proper, 0 improper, -1 point cap, 2 stopped by the visitor
+// walk! with storage secondaries: 1 proper, 0 improper, -1 point cap, 2 stopped by the visitor.
+// The secondaries at the ends of each interval and piece are the ones recorded by the evaluations
+// there (the walk's start records t = 0), so the root searches evaluate nothing. Each pass of the
+// outer loop evaluates ΔC at exactly one point per walking segment: the inner loop runs the cheap
+// part (breakpoint and root searches, visits of points whose cost is known) until the next point
+// to evaluate, so the segments of a wave stay in step at the one evaluation site.
+//   BP   next primary breakpoint t (else a ray probe, then T); evaluate t
+//   T    evaluate the interval's end t
+//   STOR next storage root x on (tp, t): evaluate x, then BAL on (a0, x); none: BAL on (a0, t)
+//   BAL  next balance root y on the piece: evaluate and visit y; none: visit the piece's end
 template <typename S, int W, class V>
 __device__ int walk_stor(const FStor<S, W> &f, S beta, S maxnats, S sgn, V &vis) {
-    S cmin = 0, tp = 0, delta = 0, clast = 0;
+    using F = FStor<S, W>;
+    enum { ST_START, ST_BP, ST_RAY, ST_T, ST_STOR, ST_BAL };
+    S cmin = 0, tp = 0, delta = 0, clast = 0, t = 0, c = 0, a0 = 0, xs = 0, x = 0, cx = 0, xb = 0, hp = 0;
     int nx = 0, np = 0, w;
+    bool split = false, inx = false;
+    int st = ST_START;
     while (true) {
-        S t = 0;
-        if (!f.next_bp(tp, sgn, t)) {
-            const S hp = S(1e-6) * fmax(S(1), fabs(tp));
-            const S sl = (beta * f.eval(tp + sgn * hp) - clast) / hp;
+        S qt = 0, r = 0;
+        int qm = F::REC_NONE;
+        while (true) {
+            if (st == ST_START) {
+                qm = F::REC_A;
+                break;
+            }
+            if (st == ST_BP) {
+                if (f.next_bp(tp, sgn, t)) {
+                    st = ST_T;
+                } else {
+                    hp = S(1e-6) * fmax(S(1), fabs(tp));
+                    qt = tp + sgn * hp;
+                    st = ST_RAY;
+                    break;
+                }
+            }
+            if (st == ST_T) {
+                qt = t;
+                qm = F::REC_B;
+                break;
+            }
+            const bool stor = st == ST_STOR;
+            if (f.next_root(stor, stor ? tp : a0, stor ? t : (inx ? x : t), stor ? xs : xb, sgn, r)) {
+                qt = r;
+                qm = stor ? F::REC_X : F::REC_NONE;
+                break;
+            }
+            if (stor) {
+                inx = false;
+                xb = a0;
+                st = ST_BAL;
+                if (split) {            // the balance at t, overwritten by the storage roots
+                    qt = t;
+                    qm = F::REC_T;
+                    break;
+                }
+                continue;
+            }
+            // the piece is done: visit its end (a storage root or the interval's end)
+            if ((w = sms_visit(vis, inx ? x : t, inx ? cx : c, cmin, np))) return w;
+            if (inx) {
+                f.shift(false);
+                a0 = x;
+                xs = x;
+                st = ST_STOR;
+            } else {
+                clast = c;
+                if (c - cmin > maxnats) return 1;
+                tp = t;
+                f.shift(true);
+                st = ST_BP;
+            }
+        }
+        const S val = f.eval(qt, qm);
+        if (st == ST_START) {
+            st = ST_BP;
+        } else if (st == ST_RAY) {
+            const S sl = (beta * val - clast) / hp;
             if (sl > 0) delta = fmin(2 * maxnats / sl, S(1e3) * fmax(fabs(tp), S(1e-3)));
             else delta = delta == 0 ? fmax(fabs(tp), S(1e-6)) : 2 * delta;
             t = tp + sgn * delta;
             if (++nx > 80) return 0;
             if (!isfinite(t)) return 0;
-        }
-        const S c = beta * f.eval(t);
-        S a0 = tp, xs = tp, x = 0;
-        while (f.next_root(true, tp, t, xs, sgn, x)) {
-            const S cx = beta * f.eval(x);
-            S xb = a0, y = 0;
-            while (f.next_root(false, a0, x, xb, sgn, y)) {
-                if ((w = sms_visit(vis, y, beta * f.eval(y), cmin, np))) return w;
-                xb = y;
-            }
-            if ((w = sms_visit(vis, x, cx, cmin, np))) return w;
-            a0 = x;
-            xs = x;
-        }
-        S xb = a0, y = 0;
-        while (f.next_root(false, a0, t, xb, sgn, y)) {
-            if ((w = sms_visit(vis, y, beta * f.eval(y), cmin, np))) return w;
-            xb = y;
-        }
-        if ((w = sms_visit(vis, t, c, cmin, np))) return w;
-        clast = c;
-        if (c - cmin > maxnats) return 1;
-        tp = t;
+            st = ST_T;
+        } else if (st == ST_T) {
+            c = beta * val;
+            a0 = tp;
+            xs = tp;
+            split = false;
+            st = ST_STOR;
+        } else if (st == ST_STOR) {             // a storage root
+            x = r;
+            cx = beta * val;
+            split = true;
+            inx = true;
+            xb = a0;
+            st = ST_BAL;
+        } else if (qm == F::REC_NONE) {         // ST_BAL: a balance root
+            if ((w = sms_visit(vis, r, beta * val, cmin, np))) return w;
+            xb = r;
+        }                                       // ST_BAL after REC_T: search the last piece
     }
 }
 
@@ -345,84 +442,93 @@ __global__ void __launch_bounds__(64) k_storage_ms(DevModel<S> m, SweepState<S> 
                                                   S kmin) {
     extern __shared__ unsigned char sms_lds[];
     constexpr int SEGS = 64 / W;
+    constexpr int SP = FStor<S, W>::SP;
     const int lane = threadIdx.x;
     const int seg = lane / W, sub = lane % W;
     const int blk = blockIdx.x / a.nrg, rg = blockIdx.x % a.nrg;
     const int L = a.L, b0 = a.base + 2 * L * blk;
     const int j = rg * SEGS + seg;
     const bool valid = j < m.R;
-    const int nstep = L + 1;
+    const int nstep = L + 1, nus = a.nus;
     const size_t TR = (size_t)m.T * m.R;
-    // LDS of this segment: ye, ye1, S0 [nstep][SMS_NU]; kinks [2][SMS_NU]; units [SMS_NU]
-    S *base = (S *)sms_lds + (size_t)seg * sms_seg_words<S>(nstep);
-    S *ye_l = base, *ye1_l = base + nstep * SMS_NU, *S0_l = base + 2 * nstep * SMS_NU;
-    SmsKink<S> *kk_l = (SmsKink<S> *)(base + 3 * nstep * SMS_NU);
-    SmsUnit<S> *un_l = (SmsUnit<S> *)(base + 3 * nstep * SMS_NU + 7 * 2 * SMS_NU);
+    // LDS: the wave's units [SMS_NU], then per segment the planes [SMS_NPLANE][nstep][nus] and
+    // the kinks [2][nus]
+    SmsUnit<S> *un_l = (SmsUnit<S> *)sms_lds;
+    S *base = (S *)sms_lds + sms_unit_words<S>() + (size_t)seg * sms_seg_words<S>(nstep, nus);
+    S *ye_l = base + P_YE * nstep * nus, *ye1_l = base + P_YE1 * nstep * nus;
+    S *S0_l = base + P_S0 * nstep * nus, *c0_l = base + P_C0 * nstep * nus;
+    S *j0_l = base + P_J0 * nstep * nus, *ph0_l = base + P_PH0 * nstep * nus;
+    SmsKink<S> *kk_l = (SmsKink<S> *)(base + SMS_NPLANE * nstep * nus);
     MoveOut out;
     for (int mv = 0; mv < a.nmoves; mv++) {
         uint32_t s[4];
         philox4x32(a.k0, a.k1, (uint32_t)mv, (uint32_t)b0, 9200u + (uint32_t)L, a.ctr, s);
-        // the units: a country–carrier group (probability p_agg) or one unit
-        int si, nu, node[SMS_NU];
+        // the units: a country–carrier group (probability p_agg) or one unit; lane u < nu of the
+        // wave stages unit u
+        int si, nu, node;
         if (u01(s[0], s[1]) < a.p_agg) {
             const int g = (int)mulhi32(s[2], (uint32_t)a.ngroups);
             si = a.grp_si[g];
             nu = a.grp_off[g + 1] - a.grp_off[g];
-#pragma unroll
-            for (int u = 0; u < SMS_NU; u++) node[u] = u < nu ? a.grp_node[a.grp_off[g] + u] : 0;
+            node = lane < nu ? a.grp_node[a.grp_off[g] + lane] : 0;
         } else {
             const int un = a.units[(int)mulhi32(s[3], (uint32_t)a.nunits)];
             si = un / m.N;
             nu = 1;
-            node[0] = un % m.N;
-#pragma unroll
-            for (int u = 1; u < SMS_NU; u++) node[u] = 0;
+            node = un % m.N;
         }
+        if (lane < nu) {
+            const int us = si * m.N + node;
+            SmsUnit<S> U;
+            U.par = {m.hours[us], m.eta_in[us], m.eta_out[us], m.qt[us], m.mc[(ESM_KSTOR + si) * m.N + node]};
+            U.kst = m.kc[(ESM_KSTOR + si) * m.N + node];
+            U.ks = U.kst * m.hours[us] / m.dt;
+            U.node = node;
+            un_l[lane] = U;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (!valid) continue;
         FStor<S, W> f;
         f.m = &m;
         f.L = L;
         f.sub = sub;
         f.nu = nu;
-        f.ye_l = ye_l;
-        f.ye1_l = ye1_l;
-        f.S0_l = S0_l;
+        f.nus = nus;
+        f.pl = base;
         f.kk_l = kk_l;
         f.un_l = un_l;
         const S *ye_g = st.v + plane((2 + si) * m.N, m.T, m.R);
         const S *yg_g = st.v + plane(m.N, m.T, m.R);
-        if (sub == 0)
-            for (int u = 0; u < nu; u++) {
-                const int us = si * m.N + node[u];
-                SmsUnit<S> U;
-                U.par = {m.hours[us], m.eta_in[us], m.eta_out[us], m.qt[us], m.mc[(ESM_KSTOR + si) * m.N + node[u]]};
-                U.kst = m.kc[(ESM_KSTOR + si) * m.N + node[u]];
-                U.ks = U.kst * m.hours[us] / m.dt;
-                U.node = node[u];
-                un_l[u] = U;
-            }
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // stage the block's values of the segment's replica (each lane its own steps)
-#pragma unroll
-        for (int i = 0; i < SMS_SP; i++) {
+#pragma nounroll
+        for (int i = 0; i < SP; i++) {
             const int q = sub + W * i;
             if (q > L) continue;
             const int tau = b0 + q, tau1 = tau + 1 == m.T ? 0 : tau + 1;
             for (int u = 0; u < nu; u++) {
-                const size_t on = (size_t)node[u] * TR;
-                ye_l[q * SMS_NU + u] = ye_g[on + (size_t)tau * m.R + j];
-                ye1_l[q * SMS_NU + u] = ye_g[on + (size_t)tau1 * m.R + j];
-                S0_l[q * SMS_NU + u] = st.r[on + (size_t)tau * m.R + j];
+                const SmsUnit<S> U = un_l[u];
+                const size_t on = (size_t)U.node * TR;
+                const int i0 = q * nus + u;
+                const S ye = ye_g[on + (size_t)tau * m.R + j], ye1 = ye_g[on + (size_t)tau1 * m.R + j];
+                const S S0 = st.r[on + (size_t)tau * m.R + j];
+                ye_l[i0] = ye;
+                ye1_l[i0] = ye1;
+                S0_l[i0] = S0;
+                // the unit's storage term and its node's balance at t = 0
+                S d0, j0;
+                c0_l[i0] = soc_cost_p(U.par, m.dt, m.cv, U.kst, U.kst + m.kappa0, ye, ye1, d0, j0);
+                j0_l[i0] = j0;
+                ph0_l[i0] = phibal(S0, m.c_shed, m.c_dump);
             }
             // the pro-rata kinks at the block's ends (`prorata!` with `soc_qadj`)
             const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
             if (kidx >= 0)
                 for (int u = 0; u < nu; u++) {
                     const SmsUnit<S> U = un_l[u];
-                    SmsKink<S> kk{0, 0, 0, 0, 0, 0, 0};
-                    const int n = node[u];
+                    SmsKink<S> kk{0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    const int n = U.node;
                     const S k = m.kc[ESM_KGAS * m.N + n];
                     const S g = kidx == 0 ? U.ks * (S(1) - S(0)) : U.ks * (S(0) - S(1));
                     if (fabs(k) >= kmin && g != S(0)) {
@@ -431,7 +537,7 @@ __global__ void __launch_bounds__(64) k_storage_ms(DevModel<S> m, SweepState<S> 
                         const S Qs = S(0) + k * clampv(w, S(0), S(1));
                         if (!(fabs(P) < S(1e-6))) {
                             // soc_qadj: nominal minus actual injection of the moved unit at (n, τ)
-                            const S Y = ye_l[q * SMS_NU + u], Y1 = ye1_l[q * SMS_NU + u];
+                            const S Y = ye_l[q * nus + u], Y1 = ye1_l[q * nus + u];
                             const S ec = clampv(Y, S(0), S(1)), ec1 = clampv(Y1, S(0), S(1));
                             const S d = U.par.h * (ec1 - ec) / m.dt - U.par.qt;
                             S pc, mcp;
@@ -451,9 +557,10 @@ __global__ void __launch_bounds__(64) k_storage_ms(DevModel<S> m, SweepState<S> 
                             kk.ak = sig == S(0) ? S(0) : sig / k;
                             kk.y0 = w;
                             kk.valid = 1;
+                            if (kk.ak != S(0)) kk.gc0 = f.gen_c(n, w, kk.gp0);
                         }
                     }
-                    kk_l[kidx * SMS_NU + u] = kk;
+                    kk_l[kidx * nus + u] = kk;
                 }
         }
         double u1, u2;
@@ -476,20 +583,20 @@ __global__ void __launch_bounds__(64) k_storage_ms(DevModel<S> m, SweepState<S> 
         if (t != 0) {
             S *ye_w = st.v + plane((2 + si) * m.N, m.T, m.R);
             S *yg_w = st.v + plane(m.N, m.T, m.R);
-#pragma unroll
-            for (int i = 0; i < SMS_SP; i++) {
+#pragma nounroll
+            for (int i = 0; i < SP; i++) {
                 const int q = sub + W * i;
                 if (q > L) continue;
                 const int tau = b0 + q;
                 const int kidx = q == 0 ? 0 : (q == L ? 1 : -1);
                 for (int u = 0; u < nu; u++) {
-                    S S1;
-                    f.unit_step(q, u, t, nullptr, S1);
-                    const size_t o = (size_t)node[u] * TR + (size_t)tau * m.R + j;
+                    S d1, S1;
+                    f.unit_step(q, u, t, d1, S1);
+                    const size_t o = (size_t)un_l[u].node * TR + (size_t)tau * m.R + j;
                     st.r[o] = S1;
-                    if (q >= 1) ye_w[o] = ye_l[q * SMS_NU + u] + t;
+                    if (q >= 1) ye_w[o] = ye_l[q * nus + u] + t;
                     if (kidx >= 0) {
-                        const SmsKink<S> kk = kk_l[kidx * SMS_NU + u];
+                        const SmsKink<S> kk = kk_l[kidx * nus + u];
                         if (kk.valid != S(0) && kk.ak != S(0)) yg_w[o] = kk.y0 + kk.ak * FStor<S, W>::kphi(kk.Q, kk.g, t);
                     }
                 }
@@ -541,6 +648,9 @@ int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, dou
     SweepState<S> st{(S *)h->d_v, (S *)h->d_r, (S *)h->d_F, (const S *)h->d_x};
     const S beta = (S)h->m.beta_v, mxn = (S)maxnats, kmin = S(1);
     const MoveTablesDev &d = h->dmt;
+    int lds_max = 0, nus = 1;
+    for (size_t g = 0; g + 1 < h->mt.grp_off.size(); g++) nus = std::max(nus, h->mt.grp_off[g + 1] - h->mt.grp_off[g]);
+    ESM_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
     for (size_t li = 0; li < sc.lengths.size(); li++) {
         const int L = sc.lengths[li], o = sc.offset[li];
         if (o > T - L - 1) continue;
@@ -554,10 +664,15 @@ int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, dou
             const int cls = ci == 0 ? sc.first[li] : 1 - sc.first[li];
             const int nbc = (nb - cls + 1) / 2;
             if (nbc <= 0) continue;
-            SmsArgs a{k0, k1, ctr, L, o + cls * L, nbc, nrg, h->sweep_ms_moves, h->sweep_ms_pagg,
+            SmsArgs a{k0, k1, ctr, L, o + cls * L, nbc, nrg, h->sweep_ms_moves, nus, h->sweep_ms_pagg,
                       d.sms_units, (int)h->mt.sms_units.size(), d.grp_off, d.grp_node, d.grp_si,
                       (int)h->mt.grp_si.size(), h->d_stats + (size_t)ESM_FAM_SMS * ESM_NSTAT * STAT_SLOTS};
-            const size_t lds = (size_t)segs * sms_seg_words<S>(L + 1) * sizeof(S);
+            const size_t lds = (sms_unit_words<S>() + (size_t)segs * sms_seg_words<S>(L + 1, nus)) * sizeof(S);
+            if (lds > (size_t)lds_max) {
+                set_error("storage multiscale sweep: block length " + std::to_string(L) + " needs " +
+                          std::to_string(lds) + " B of LDS per workgroup, the device allows " + std::to_string(lds_max));
+                return ESM_EINVAL;
+            }
             const dim3 grid((unsigned)nbc * (unsigned)nrg), blk(64);
             ProfScope p(h, ESM_KERN_SMS);
             switch (W) {
