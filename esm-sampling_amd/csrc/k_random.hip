@@ -313,6 +313,29 @@ struct FSparse {
         S rt[KR];
         bool more = false, hb = false;
         S nb = 0;
+        // keep root x among the KR nearest distinct roots in walking order (more: one was dropped)
+        __device__ void insert(S x, S sgn) {
+            bool dup = false;
+#pragma unroll
+            for (int i = 0; i < KR; i++) dup = dup || (i < nr && rt[i] == x);
+            if (dup) return;
+            bool carry = true;
+#pragma unroll
+            for (int i = 0; i < KR; i++) {
+                if (i < nr) {
+                    if (sgn * (x - rt[i]) < 0) {
+                        const S tmp = rt[i];
+                        rt[i] = x;
+                        x = tmp;
+                    }
+                } else if (i == nr && carry) {
+                    rt[i] = x;
+                    carry = false;
+                }
+            }
+            if (carry) more = true;
+            else nr++;
+        }
     };
     __device__ void pass(S tp, S t, S sgn, bool nodes, Pass &P) const {
         if (nodes) nev++;
@@ -374,30 +397,8 @@ struct FSparse {
                 const S S0 = rv(tb->nd_node[p]);
                 D += m->dt * (phibal(B, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
                 if ((A < 0) != (B < 0) && A != B) {
-                    S x = tp + (t - tp) * A / (A - B);
-                    if (sgn * (x - tp) > 0 && sgn * (t - x) > 0) {
-                        bool dup = false;
-#pragma unroll
-                        for (int i = 0; i < KR; i++) dup = dup || (i < P.nr && P.rt[i] == x);
-                        if (!dup) {
-                            bool carry = true;
-#pragma unroll
-                            for (int i = 0; i < KR; i++) {
-                                if (i < P.nr) {
-                                    if (sgn * (x - P.rt[i]) < 0) {
-                                        const S tmp = P.rt[i];
-                                        P.rt[i] = x;
-                                        x = tmp;
-                                    }
-                                } else if (i == P.nr && carry) {
-                                    P.rt[i] = x;
-                                    carry = false;
-                                }
-                            }
-                            if (carry) P.more = true;
-                            else P.nr++;
-                        }
-                    }
+                    const S x = tp + (t - tp) * A / (A - B);
+                    if (sgn * (x - tp) > 0 && sgn * (t - x) > 0) P.insert(x, sgn);
                 }
             }
         P.c = D;
@@ -479,21 +480,17 @@ struct FSparse {
         return have;
     }
 
-    // the next sign change of a node balance strictly between x and t (walking order) on the
-    // segment (tp, t), where the balances are linear
-    __device__ bool next_root(S tp, S t, S x, S sgn, S &xr) const {
-        bool have = false;
+    // the next KR distinct sign changes of the node balances strictly between x and t (walking
+    // order) on the segment (tp, t), where the balances are linear; more: others remain
+    __device__ void roots_beyond(S tp, S t, S x, S sgn, Pass &P) const {
         for (int p = n0; p < n1; p++) {
-            const S A = node_S(p, tp), B = node_S(p, t);
+            S A, B;
+            node_S2(p, tp, t, A, B);
             if ((A < 0) != (B < 0) && A != B) {
                 const S rt = tp + (t - tp) * A / (A - B);
-                if (sgn * (rt - tp) > 0 && sgn * (t - rt) > 0 && sgn * (rt - x) > 0 && (!have || sgn * (xr - rt) > 0)) {
-                    xr = rt;
-                    have = true;
-                }
+                if (sgn * (rt - tp) > 0 && sgn * (t - rt) > 0 && sgn * (rt - x) > 0) P.insert(rt, sgn);
             }
         }
-        return have;
     }
 
     // log Jacobian Σ (|S| − 1) log|P(t)/P| of the compensated kinks (FP64); one unit per node
@@ -541,16 +538,28 @@ __device__ int walk_sparse(const FSparse<S> &f, S beta, S maxnats, S sgn, V &vis
         hb = P.hb;
         nb = P.nb;
         int w;
+        // the roots of the segment in walking order: the ones the pass kept, then KR per scan
+        int nr = P.nr;
+        bool more = P.more;
+        S rt[FSparse<S>::KR];
 #pragma unroll
-        for (int i = 0; i < FSparse<S>::KR; i++)
-            if (i < P.nr)
-                if ((w = visit_point(vis, P.rt[i], beta * f.eval(P.rt[i]), cmin, np))) return w;
-        if (P.more) {
-            S x = P.rt[FSparse<S>::KR - 1], xr = 0;
-            while (f.next_root(tp, t, x, sgn, xr)) {
+        for (int k = 0; k < FSparse<S>::KR; k++) rt[k] = P.rt[k];
+        while (true) {
+#pragma nounroll
+            for (int i = 0; i < nr; i++) {
+                S xr = rt[0];
+#pragma unroll
+                for (int k = 1; k < FSparse<S>::KR; k++)
+                    if (k == i) xr = rt[k];
                 if ((w = visit_point(vis, xr, beta * f.eval(xr), cmin, np))) return w;
-                x = xr;
             }
+            if (!more) break;
+            typename FSparse<S>::Pass Q;
+            f.roots_beyond(tp, t, rt[FSparse<S>::KR - 1], sgn, Q);
+            nr = Q.nr;
+            more = Q.more;
+#pragma unroll
+            for (int k = 0; k < FSparse<S>::KR; k++) rt[k] = Q.rt[k];
         }
         if ((w = visit_point(vis, t, c, cmin, np))) return w;
         clast = c;
