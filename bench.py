@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--fp64", action="store_true", help="FP64 device state")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
+    ap.add_argument("--p-storage-ms", type=float, default=0.5,
+                    help="probability of the storage multiscale sweep per operation sweep (the reference's 1/2)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a plain device copy of 1 GiB three times (the calibration kernel of the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -95,6 +97,7 @@ def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
     """Algorithmic HBM bytes per launch of each kernel (reads + writes of state and caches that
     the kernel's algorithm needs, each value once). For the random families: the moves of one
     launch (from the sweep statistics) times the values one move reads and writes."""
+    import numpy as np
     N, L, T = s.N, s.L, s.T
     half = T // 2
     deg = 2.0 * L / N
@@ -114,6 +117,15 @@ def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
     for k, f in fam.items():
         launches = max(prof.get(k, (0, 0))[1] / max(steps, 1), 1)
         out[k] = per_move[k] * es * float(stats[f, 0]) / launches
+    # storage multiscale sweep (when it runs: 2 launches per block length L = 2, 4, …, 128 < T/2): a
+    # move on a block of L + 1 steps reads ye, ye1 and r of every step and the gas at both ends of
+    # each of its nu units, and writes r of every step, ye of the L inner instants and the gas at the
+    # ends: (5L + 8)·nu values; nu = 1/2 + 1/2·(mean country size); 8 moves per block
+    lengths = [L for L in (2, 4, 8, 16, 32, 64, 128) if L < T // 2]
+    if lengths:
+        nu = 0.5 + 0.5 * float(np.bincount(s.country).mean()) if s.ncountry > 0 else 1.0
+        tot = sum(8 * (T - L) / L * (5 * L + 8) * nu * es * R for L in lengths)
+        out["k_storage_ms"] = tot / (2 * len(lengths))
     return out
 
 
@@ -139,6 +151,7 @@ def main():
         del src, dst
     T, R = args.hours, args.replicas
     wl = Workload(args.nodes, args.lines, T, R, prec)
+    api.sweep_options(wl.dev, p_storage_ms=args.p_storage_ms)
     lib, h = wl.dev.lib, wl.dev.handle
     for it in range(args.warmup):
         wl.step(it)
@@ -211,8 +224,11 @@ def measured_traffic(kernel, s, T, R, prec):
     p = ROOT / "profiles" / "hbm_traffic_replica_step_40n_8760h_64r_fp32.json"
     if not (p.exists() and s.N == 40 and s.L == 70 and T == 8760 and R == 64 and prec == FP32):
         return None
-    names = {"k_spair": "k_soc<float, true>", "k_soc": "k_soc<float, false>", "k_gen": "k_gen<float>",
-             "k_energy": "k_energy<float, true, false>", "k_follow": "k_follow<float>"}
+    names = {"k_spair": "k_rand_spair<float, true>", "k_soc": "k_soc<float>", "k_gen": "k_gen<float>",
+             "k_energy": "k_energy<float, true, false>", "k_follow": "k_follow<float>",
+             "k_unit": "k_rand_unit<float, true>", "k_cell": "k_sparse<float, 0, true>",
+             "k_xloc": "k_sparse<float, 1, true>", "k_xlocc": "k_sparse<float, 2, true>",
+             "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms"}
     return json.loads(p.read_text())["bytes_per_dispatch"].get(names.get(kernel, ""))
 
 

@@ -12,15 +12,16 @@
 // ΔC is evaluated from the terms recorded at t = 0, in the arithmetic of `eval_cached!`
 // (src/local.jl:926-981): generator terms, line overloads, storage terms, node balances.
 //
-// Families (the complete level of the synthetic system, one dispatchable unit per node):
-//   GEN    gas utilization atom at (n, τ)                      (atoms / `unit`)
-//   THETA  voltage angle of node n at τ                        (`potential`)
-//   SOC    state of charge of storage s at instant τ           (atoms of the SoC fields)
-//   SPAIR  storage s at τ with the gas unit compensating at τm and τ (`prep_spair!`, :1708-1726)
+// Atom families (the complete level of the synthetic system, one dispatchable unit per node):
+//   GEN    gas utilization atom at (n, τ)
+//   THETA  voltage angle of node n at τ
+//   SOC    state of charge of storage s at instant τ
 // A round is a set of sites whose terms are disjoint, so every site of a round reads the state
 // before the round and the order of sites inside a round does not matter: GEN all (n, τ);
-// THETA the nodes of one colour of a distance-2 colouring, every τ; SOC and SPAIR one storage
-// carrier and one time-parity class (τ and τ ± 1 share a storage term).
+// THETA the nodes of one colour of a distance-2 colouring, every τ; SOC one storage carrier and
+// one time-parity class (τ and τ ± 1 share a storage term). After the atoms the sweep runs the
+// random plain families (k_random.hip) and, with probability 1/2, the storage multiscale sweep
+// (k_storage.hip).
 //
 // Random numbers: per (site, replica) two uniforms from Philox4x32-10 with counter
 // (replica, n·T + τ, round id, sweep counter).
