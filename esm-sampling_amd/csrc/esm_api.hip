@@ -671,6 +671,17 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
     return ESM_OK;
 }
 
+int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double hmin) {
+    if (!h) { set_error("esm_storage_long_options: null handle"); return ESM_EINVAL; }
+    if (!(p >= 0 && p <= 1)) { set_error("esm_storage_long_options: p must be in [0, 1]"); return ESM_EINVAL; }
+    if (nmoves < 0 || nmoves > 256) { set_error("esm_storage_long_options: nmoves must be in [0, 256]"); return ESM_EINVAL; }
+    if (!(hmin >= 0) || !std::isfinite(hmin)) { set_error("esm_storage_long_options: hmin must be finite and >= 0"); return ESM_EINVAL; }
+    h->sweep_plong = p;
+    h->sweep_long_moves = nmoves;
+    h->sweep_long_hours = hmin;
+    return ESM_OK;
+}
+
 int32_t esm_follow_capacities(esm_handle *h, const double *knew, double kmin, double *kout) {
     if (!h || !knew) { set_error("esm_follow_capacities: null argument"); return ESM_EINVAL; }
     const int N = h->N, n = ESM_NK * N;

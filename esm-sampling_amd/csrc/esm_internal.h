@@ -148,6 +148,9 @@ struct esm_handle {
     double sweep_pms = 0.5;      // probability of the storage multiscale sweep per operation sweep
     int sweep_ms_moves = 8;      // moves per block
     double sweep_ms_pagg = 0.5;  // probability of a country–carrier group
+    double sweep_plong = 1.0;       // probability of the long-range storage sweep per operation sweep
+    int sweep_long_moves = 32;      // long-range plateaus per storage unit and sweep
+    double sweep_long_hours = 24.0; // ... for the units with at least this many hours of storage
     int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step
     esm::MoveTablesHost mt;                       // templates (host, FP64)
     esm::MoveTablesDev dmt;                       // the same on the device (values in the state precision)
@@ -203,6 +206,8 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
 // k_storage.hip: the storage multiscale sweep (with its probability) of one operation sweep
 template <typename S>
 int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
+template <typename S>
+int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
 
 // k_follow.hip
 template <typename S>

@@ -86,6 +86,7 @@ struct FStor {
     const SmsUnit<S> *un_l;            // [u]
     mutable int nev = 0;
     enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
+    static constexpr bool CACHED = true;               // root searches read the recorded secondaries
     static constexpr int SP = W < 64 ? 1 : SMS_SP;     // block steps per lane (L + 1 ≤ W below 64)
 
     __device__ S *plane(int k) const { return pl + (size_t)k * (L + 1) * nus; }
@@ -291,9 +292,8 @@ __device__ __forceinline__ int sms_visit(V &vis, S x, S cx, S &cmin, int &np) {
 //   T    evaluate the interval's end t
 //   STOR next storage root x on (tp, t): evaluate x, then BAL on (a0, x); none: BAL on (a0, t)
 //   BAL  next balance root y on the piece: evaluate and visit y; none: visit the piece's end
-template <typename S, int W, class V>
-__device__ int walk_stor(const FStor<S, W> &f, S beta, S maxnats, S sgn, V &vis) {
-    using F = FStor<S, W>;
+template <class F, typename S, class V>
+__device__ int walk_stor(const F &f, S beta, S maxnats, S sgn, V &vis) {
     enum { ST_START, ST_BP, ST_RAY, ST_T, ST_STOR, ST_BAL };
     S cmin = 0, tp = 0, delta = 0, clast = 0, t = 0, c = 0, a0 = 0, xs = 0, x = 0, cx = 0, xb = 0, hp = 0;
     int nx = 0, np = 0, w;
@@ -304,6 +304,10 @@ __device__ int walk_stor(const FStor<S, W> &f, S beta, S maxnats, S sgn, V &vis)
         int qm = F::REC_NONE;
         while (true) {
             if (st == ST_START) {
+                if (!F::CACHED) {               // nothing to record at t = 0
+                    st = ST_BP;
+                    continue;
+                }
                 qm = F::REC_A;
                 break;
             }
@@ -332,7 +336,7 @@ __device__ int walk_stor(const FStor<S, W> &f, S beta, S maxnats, S sgn, V &vis)
                 inx = false;
                 xb = a0;
                 st = ST_BAL;
-                if (split) {            // the balance at t, overwritten by the storage roots
+                if (F::CACHED && split) {   // the balance at t, overwritten by the storage roots
                     qt = t;
                     qm = F::REC_T;
                     break;
@@ -393,8 +397,8 @@ struct VisS {
     __device__ bool operator()(S t, S c) { return mode == 0 ? ma(t, c) : ss(t, c); }
 };
 
-template <typename S, int W>
-__device__ S sample_stor(const FStor<S, W> &f, S beta, S maxnats, S useg, S uoff, int &status) {
+template <class F, typename S>
+__device__ S sample_stor(const F &f, S beta, S maxnats, S useg, S uoff, int &status) {
     VisS<S> vis;
     S Mx[2] = {S(-INFINITY), S(-INFINITY)}, Ms[2] = {S(0), S(0)};
 #pragma nounroll
@@ -566,7 +570,7 @@ __global__ void __launch_bounds__(64) k_storage_ms(DevModel<S> m, SweepState<S> 
         double u1, u2;
         uniforms2(a.k0, a.k1, j, (uint32_t)b0, 0x50000000u + (uint32_t)(L * 16 + mv), a.ctr, u1, u2);
         int status = 0;
-        S t = sample_stor<S, W>(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+        S t = sample_stor(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
         if (t != 0) {
             bool any;
             const double lj = f.logjac(t, any);
@@ -690,5 +694,460 @@ int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, dou
 
 template int launch_storage_ms<float>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
 template int launch_storage_ms<double>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
+
+
+// ---------------------------------------------------------------------------------------------
+// The long-range storage sweep (`storage_long_sweep!`, src/storage_long.jl:245-298), which
+// `operation_sweep!` runs after the multiscale sweep with probability `storage_long` (1 by
+// default, src/local.jl:2419-2423). Per cell (node at the complete level), for every storage unit
+// with at least `hmin` hours and κs ≥ 1: a level move (every instant of the year rises by t) and
+// `nmoves` long-range plateaus (`prep_slong!`, :77-146: the state of charge rises at the cyclic
+// instants a+1 … a+L, L uniform on 1 … T−1; the node's gas unit follows pro rata at both end
+// steps), shuffled. Interior steps whose two instants stay in [0, 1] change nothing (their cost
+// change is exactly zero, their secondaries keep their values at t = 0: `mv_eval!`, :148-188).
+//
+// MI355X layout: one wave per (node, replica), running the node's move list in order (the
+// cells are independent). The unit's state-of-charge trajectory is staged in LDS (T values; in
+// global memory when T does not fit); lane l owns the move's steps q ≡ l (mod 64) and keeps the
+// window of t over which its interior steps stay in the box, so an evaluation inside it costs the
+// two end steps only. The walk is the multiscale sweep's, with W = 64 and the secondaries
+// recomputed at the interval ends (only end steps and out-of-box interior steps vary). The move
+// list is drawn from the sweep's counter by every wave of a node alike (Philox; see esm.h).
+// ---------------------------------------------------------------------------------------------
+
+constexpr int SLG_MAXM = 2 * 257;      // moves per node: per unit one level move and ≤ 256 plateaus
+
+struct SlgArgs {
+    uint32_t k0, k1, ctr;
+    int nmoves;                        // plateaus per unit
+    uint32_t elig[ESM_NSTOR][8];       // bit n of unit (s, n): takes part (hours ≥ hmin, κs ≥ 1)
+    unsigned long long *st_long, *st_level;
+};
+
+template <typename S, bool YL>
+struct FLong {
+    enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
+    static constexpr bool CACHED = false;
+    const DevModel<S> *m;
+    const S *yv;          // the unit's state of charge: LDS [T] (YL) or the global plane at (n, ·, j)
+    const S *rv;          // the node's residual plane at (n, ·, j), stride R
+    const S *gv;          // the node's gas plane at (n, ·, j), stride R
+    int T, R, lane, L, level, a0, n;
+    StoPar<S> par;
+    S kst, w;
+    const SmsKink<S> *kk; // [2] in LDS: the kinks at steps a and b
+    S llo, lhi;           // t in [llo, lhi]: every interior step of the lane stays in the box
+    mutable int nev = 0;
+
+    __device__ S y(int tau) const { return YL ? yv[tau] : yv[(size_t)tau * R]; }
+    __device__ S r0(int tau) const { return rv[(size_t)tau * R]; }
+    __device__ int tau(int q) const {
+        const int x = a0 + q;
+        return x >= T ? x - T : x;
+    }
+    __device__ int next(int tau) const { return tau + 1 == T ? 0 : tau + 1; }
+    // the lane's first step at or after q0
+    __device__ int first(int q0) const { return lane >= q0 ? lane : lane + 64 * ((q0 - lane + 63) / 64); }
+    __device__ bool owns_a() const { return !level && lane == 0; }
+    __device__ bool owns_b() const { return !level && lane == L % 64; }
+    __device__ bool inside(int q, S t) const {
+        const int ta = tau(q);
+        const S ye = y(ta), ye1 = y(next(ta));
+        return S(0) <= ye && ye <= S(1) && S(0) <= ye1 && ye1 <= S(1) && fmax(-ye, -ye1) <= t &&
+               t <= fmin(S(1) - ye, S(1) - ye1);
+    }
+    __device__ S gen_c(S yy, S &p) const {
+        S c;
+        const S k = m->kc[ESM_KGAS * m->N + n];
+        gen_cost(m->mc[ESM_KGAS * m->N + n], k, k + m->kappa0, m->dt, m->cv, yy, c, p);
+        return c;
+    }
+    // an end step (q = 0: step a, its end instant rises; q = L: step b, its start instant rises)
+    __device__ S end_term(int q, S t, S &d1, S &S1) const {
+        const int ta = tau(q);
+        const S ye = y(ta), ye1 = y(next(ta)), S0 = r0(ta);
+        S D = 0, dr = 0;
+        const SmsKink<S> k = kk[q == 0 ? 0 : 1];
+        if (k.valid != S(0) && k.ak != S(0)) {
+            S p1;
+            const S c1 = gen_c(k.y0 + k.ak * FStor<S, 64>::kphi(k.Q, k.g, t), p1);
+            D += c1 - k.gc0;
+            dr -= p1 - k.gp0;
+        }
+        const S dye = q >= 1 ? S(1) : S(0), dye1 = q <= L - 1 ? S(1) : S(0);
+        S d0, j0, j1;
+        const S c0 = soc_cost_p(par, m->dt, m->cv, kst, w, ye, ye1, d0, j0);
+        const S c1 = soc_cost_p(par, m->dt, m->cv, kst, w, ye + t * dye, ye1 + t * dye1, d1, j1);
+        D += c1 - c0;
+        dr += j1 - j0;
+        S1 = S0 + dr;
+        D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+        return D;
+    }
+    // an interior step (both instants rise): its term, d and balance (unchanged inside the box)
+    __device__ S int_term(int q, S t, S &d1, S &S1) const {
+        const int ta = tau(q);
+        const S ye = y(ta), ye1 = y(next(ta)), S0 = r0(ta);
+        S d0, j0, j1;
+        const S c0 = soc_cost_p(par, m->dt, m->cv, kst, w, ye, ye1, d0, j0);
+        if (S(0) <= ye && ye <= S(1) && S(0) <= ye1 && ye1 <= S(1) && fmax(-ye, -ye1) <= t &&
+            t <= fmin(S(1) - ye, S(1) - ye1)) {
+            d1 = d0;
+            S1 = S0;
+            return S(0);
+        }
+        const S c1 = soc_cost_p(par, m->dt, m->cv, kst, w, ye + t, ye1 + t, d1, j1);
+        S1 = S0 + (j1 - j0);
+        return (c1 - c0) + m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+    }
+    __device__ int qi0() const { return 1; }
+    __device__ int qi1() const { return level ? L : L - 1; }   // interior steps qi0 … qi1
+
+    // ΔC(t): the lane's steps in increasing q (end a, the interior steps out of the box, end b),
+    // summed over the wave
+    __device__ S eval(S t, int rec) const {
+        if (rec <= REC_X) nev++;
+        S D = 0, d1, S1;
+        if (owns_a()) D += end_term(0, t, d1, S1);
+        if (!(llo <= t && t <= lhi))
+            for (int q = first(qi0()); q <= qi1(); q += 64)
+                if (!inside(q, t)) D += int_term(q, t, d1, S1);
+        if (owns_b()) D += end_term(L, t, d1, S1);
+        return seg_sum<S, 64>(D);
+    }
+    __device__ void shift(bool) const {}
+
+    __device__ bool next_bp(S tp, S sgn, S &t) const {
+        bool have = false;
+        S best = 0;
+        auto cand = [&](S b) {
+            if (isfinite(b) && sgn * (b - tp) > 0 && (!have || sgn * (best - b) > 0)) {
+                best = b;
+                have = true;
+            }
+        };
+        for (int e = 0; e < 2; e++) {
+            if (!(e == 0 ? owns_a() : owns_b())) continue;
+            const SmsKink<S> k = kk[e];
+            if (k.valid == S(0)) continue;
+            cand(-k.Q / k.g);
+            if (k.ak != S(0)) {
+                const S sl = k.ak * k.g, off = k.ak * k.Q;
+                cand((S(0) - k.y0 - S(0)) / sl);
+                cand((S(1) - k.y0 - S(0)) / sl);
+                cand((S(0) - k.y0 - off) / sl);
+                cand((S(1) - k.y0 - off) / sl);
+            }
+        }
+        for (int q = first(1); q <= L; q += 64) {     // the clamp points of the rising instants
+            const S ye = y(tau(q));
+            cand((S(0) - ye) / S(1));
+            cand((S(1) - ye) / S(1));
+        }
+        return seg_best<S, 64>(have, best, sgn, t);
+    }
+
+    // the nearest sign change beyond x, strictly inside (a, b), of the storage secondaries (stor)
+    // or of the balances, from their values at a and b (linear in between)
+    __device__ bool next_root(bool stor, S a, S b, S x, S sgn, S &xr) const {
+        bool have = false;
+        S best = 0;
+        auto test = [&](S da, S db, S Ba, S Bb) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                if (!stor && k > 0) break;
+                const S A = stor ? FStor<S, 64>::sec(k, da, par) : Ba, B = stor ? FStor<S, 64>::sec(k, db, par) : Bb;
+                if ((A < 0) != (B < 0) && A != B) {
+                    const S r = a + (b - a) * A / (A - B);
+                    if (sgn * (r - a) > 0 && sgn * (b - r) > 0 && sgn * (r - x) > 0 && (!have || sgn * (best - r) > 0)) {
+                        best = r;
+                        have = true;
+                    }
+                }
+            }
+        };
+        S da, db, Ba, Bb;
+        if (owns_a()) {
+            end_term(0, a, da, Ba);
+            end_term(0, b, db, Bb);
+            test(da, db, Ba, Bb);
+        }
+        if (!(llo <= a && a <= lhi && llo <= b && b <= lhi))
+            for (int q = first(qi0()); q <= qi1(); q += 64) {
+                if (inside(q, a) && inside(q, b)) continue;   // constant secondaries
+                int_term(q, a, da, Ba);
+                int_term(q, b, db, Bb);
+                test(da, db, Ba, Bb);
+            }
+        if (owns_b()) {
+            end_term(L, a, da, Ba);
+            end_term(L, b, db, Bb);
+            test(da, db, Ba, Bb);
+        }
+        return seg_best<S, 64>(have, best, sgn, xr);
+    }
+
+    __device__ double logjac(S t, bool &any) const {
+        double lj = 0.0;
+        int cnt = 0;
+        for (int e = 0; e < 2; e++) {
+            if (!(e == 0 ? owns_a() : owns_b())) continue;
+            const SmsKink<S> k = kk[e];
+            if (k.valid == S(0)) continue;
+            cnt++;
+            const double nm = 0.0;
+            lj += nm * log(fabs((fmax((double)k.Q + (double)k.g * (double)t, 0.0) + (double)k.e) / (double)k.P));
+        }
+        any = seg_sum<double, 64>((double)cnt) > 0;
+        return seg_sum<double, 64>(lj);
+    }
+};
+
+// words (of S) of the LDS of one wave: the kinks, then the trajectory (YL)
+template <typename S>
+__host__ __device__ constexpr size_t slg_words(int T, bool yl) {
+    return sizeof(SmsKink<S>) / sizeof(S) * 2 + (yl ? (size_t)T : 0);
+}
+
+template <typename S, bool YL>
+__global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S> st, SlgArgs a, S beta, S maxnats,
+                                                    S kmin) {
+    extern __shared__ unsigned char slg_lds[];
+    __shared__ int msi[SLG_MAXM], ma[SLG_MAXM], mL[SLG_MAXM];
+    const int lane = threadIdx.x;
+    const int n = blockIdx.x / m.R, j = blockIdx.x % m.R;
+    const int N = m.N, T = m.T, R = m.R;
+    // the node's move list (the same in every wave of the node): per unit taking part a level
+    // move and `nmoves` plateaus, then a Fisher–Yates shuffle
+    int cnt = 0, u = 0;
+    for (int si = 0; si < ESM_NSTOR; si++) {
+        if (!((a.elig[si][n >> 5] >> (n & 31)) & 1u)) continue;
+        uint32_t w[4];
+        philox4x32(a.k0, a.k1, (uint32_t)n, (uint32_t)u, 9411u, a.ctr, w);
+        if (u01(w[0], w[1]) < 1.0) {
+            if (lane == 0) {
+                msi[cnt] = si;
+                ma[cnt] = 0;
+                mL[cnt] = T;
+            }
+            cnt++;
+        }
+        for (int mm = 0; mm < a.nmoves; mm++) {
+            philox4x32(a.k0, a.k1, (uint32_t)n, 65536u * (uint32_t)u + (uint32_t)mm, 9412u, a.ctr, w);
+            if (lane == 0) {
+                msi[cnt] = si;
+                mL[cnt] = 1 + (int)mulhi32(w[0], (uint32_t)(T - 1));
+                ma[cnt] = (int)mulhi32(w[1], (uint32_t)T);
+            }
+            cnt++;
+        }
+        u++;
+    }
+    if (cnt == 0) return;
+    if (lane == 0)
+        for (int i = cnt - 1; i >= 1; i--) {
+            uint32_t w[4];
+            philox4x32(a.k0, a.k1, (uint32_t)n, (uint32_t)i, 9413u, a.ctr, w);
+            const int jx = (int)mulhi32(w[0], (uint32_t)(i + 1));
+            int tmp = msi[i]; msi[i] = msi[jx]; msi[jx] = tmp;
+            tmp = ma[i]; ma[i] = ma[jx]; ma[jx] = tmp;
+            tmp = mL[i]; mL[i] = mL[jx]; mL[jx] = tmp;
+        }
+    __syncthreads();
+    S *lds = (S *)slg_lds;
+    SmsKink<S> *kk_l = (SmsKink<S> *)lds;
+    S *ys = lds + sizeof(SmsKink<S>) / sizeof(S) * 2;
+    const size_t TR = (size_t)T * R;
+    const S *rv = st.r + (size_t)n * TR + j;
+    S *gv = st.v + plane(N, T, R) + (size_t)n * TR + j;
+    MoveOut out_long, out_level;
+    int cur = -1;
+    for (int i = 0; i < cnt; i++) {
+        const int si = msi[i], L = mL[i], level = L >= T;
+        S *yg = st.v + plane((2 + si) * N, T, R) + (size_t)n * TR + j;
+        if (YL && si != cur) {          // stage the unit's trajectory
+            for (int tau = lane; tau < T; tau += 64) ys[tau] = yg[(size_t)tau * R];
+            cur = si;
+        }
+        FLong<S, YL> f;
+        f.m = &m;
+        f.yv = YL ? ys : yg;
+        f.rv = rv;
+        f.gv = gv;
+        f.T = T;
+        f.R = R;
+        f.lane = lane;
+        f.L = level ? T : L;
+        f.level = level;
+        f.a0 = level ? T - 1 : ma[i];
+        f.n = n;
+        const int us = si * N + n;
+        f.par = {m.hours[us], m.eta_in[us], m.eta_out[us], m.qt[us], m.mc[(ESM_KSTOR + si) * N + n]};
+        f.kst = m.kc[(ESM_KSTOR + si) * N + n];
+        f.w = f.kst + m.kappa0;
+        f.kk = kk_l;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        // the kinks at the end steps (`prorata!` with `soc_qadj`), by the lanes owning them
+        if (!level)
+            for (int e = 0; e < 2; e++) {
+                if (!(e == 0 ? f.owns_a() : f.owns_b())) continue;
+                const int tau = f.tau(e == 0 ? 0 : f.L), tau1 = f.next(tau);
+                SmsKink<S> k{0, 0, 0, 0, 0, 0, 0, 0, 0};
+                const S ks = f.kst * m.hours[us] / m.dt;
+                const S kg = m.kc[ESM_KGAS * N + n];
+                const S g = e == 0 ? ks * (S(1) - S(0)) : ks * (S(0) - S(1));
+                if (fabs(kg) >= kmin && g != S(0)) {
+                    const S w = gv[(size_t)tau * R];
+                    const S P = S(0) + kg * w;
+                    const S Qs = S(0) + kg * clampv(w, S(0), S(1));
+                    if (!(fabs(P) < S(1e-6))) {
+                        const S Y = f.y(tau), Y1 = f.y(tau1);
+                        const S ec = clampv(Y, S(0), S(1)), ec1 = clampv(Y1, S(0), S(1));
+                        const S d = f.par.h * (ec1 - ec) / m.dt - f.par.qt;
+                        S pc, mcp;
+                        if (d >= 0) {
+                            pc = f.par.ein > 0 ? d / f.par.ein : S(0);
+                            mcp = 0;
+                        } else {
+                            pc = 0;
+                            mcp = fmin(-d * f.par.eout, S(1));
+                        }
+                        const S qadj = S(0) + (f.kst * f.par.h * (Y1 - Y) / m.dt - f.kst * (pc - mcp));
+                        k.Q = rv[(size_t)tau * R] + Qs + qadj;
+                        k.g = g;
+                        k.e = P - fmax(k.Q, S(0));
+                        k.P = P;
+                        const S sig = kg * w / P;
+                        k.ak = sig == S(0) ? S(0) : sig / kg;
+                        k.y0 = w;
+                        k.valid = 1;
+                        if (k.ak != S(0)) k.gc0 = f.gen_c(w, k.gp0);
+                    }
+                }
+                kk_l[e] = k;
+            }
+        // the lane's window: t over which every one of its interior steps stays in the box
+        S llo = -INFINITY, lhi = INFINITY;
+        for (int q = f.first(f.qi0()); q <= f.qi1(); q += 64) {
+            const int tau = f.tau(q);
+            const S ye = f.y(tau), ye1 = f.y(f.next(tau));
+            if (S(0) <= ye && ye <= S(1) && S(0) <= ye1 && ye1 <= S(1)) {
+                llo = fmax(llo, fmax(-ye, -ye1));
+                lhi = fmin(lhi, fmin(S(1) - ye, S(1) - ye1));
+            } else {
+                llo = INFINITY;
+                lhi = -INFINITY;
+            }
+        }
+        f.llo = llo;
+        f.lhi = lhi;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        double u1, u2;
+        uniforms2(a.k0, a.k1, j, (uint32_t)n, 0x54000000u + (uint32_t)i, a.ctr, u1, u2);
+        int status = 0;
+        S t = sample_stor(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+        if (t != 0) {
+            bool any;
+            const double lj = f.logjac(t, any);
+            if (any) {
+                uint32_t w4[4];
+                philox4x32(a.k0, a.k1, (uint32_t)j, (uint32_t)n, 0x74000000u + (uint32_t)i, a.ctr, w4);
+                if (!(u01(w4[0], w4[1]) < exp(fmin(0.0, lj)))) {
+                    t = 0;
+                    status = 2;
+                }
+            }
+        }
+        if (lane == 0) (level ? out_level : out_long).add(status, t != 0, f.nev);
+        if (t != 0) {
+            // from the state before the move: the residuals (and the gas at the end steps) ...
+            S *rw = st.r + (size_t)n * TR + j;
+            S d1, S1;
+            for (int e = 0; e < 2; e++) {
+                if (!(e == 0 ? f.owns_a() : f.owns_b())) continue;
+                const int q = e == 0 ? 0 : f.L, tau = f.tau(q);
+                f.end_term(q, t, d1, S1);
+                rw[(size_t)tau * R] = S1;
+                const SmsKink<S> k = kk_l[e];
+                if (k.valid != S(0) && k.ak != S(0)) gv[(size_t)tau * R] = k.y0 + k.ak * FStor<S, 64>::kphi(k.Q, k.g, t);
+            }
+            for (int q = f.first(f.qi0()); q <= f.qi1(); q += 64)
+                if (!f.inside(q, t)) {
+                    f.int_term(q, t, d1, S1);
+                    rw[(size_t)f.tau(q) * R] = S1;
+                }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            // ... then the rising instants a+1 … a+L
+            for (int q = f.first(1); q <= f.L; q += 64) {
+                const int tau = f.tau(q);
+                const S nv = f.y(tau) + t;
+                yg[(size_t)tau * R] = nv;
+                if (YL) ys[tau] = nv;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+    add_counts(a.st_long, out_long);
+    add_counts(a.st_level, out_level);
+}
+
+template <typename S>
+int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats) {
+    const int T = h->T, R = h->R, N = h->N;
+    if (!(h->sweep_plong > 0) || T < 2) return 0;
+    uint32_t w[4];
+    philox4x32(k0, k1, 0u, 0u, 9410u, ctr, w);
+    if (!(u01(w[0], w[1]) < h->sweep_plong)) return 0;
+    if (N > 256 || h->sweep_long_moves > 256) {
+        set_error("long-range storage sweep: at most 256 nodes and 256 plateaus per unit");
+        return ESM_EINVAL;
+    }
+    SlgArgs a{};
+    a.k0 = k0;
+    a.k1 = k1;
+    a.ctr = ctr;
+    a.nmoves = h->sweep_long_moves;
+    bool any = false;
+    for (int si = 0; si < ESM_NSTOR; si++)
+        for (int n = 0; n < N; n++) {
+            const int us = si * N + n, ku = (ESM_KSTOR + si) * N + n;
+            const double kst = std::min(std::max(h->k[ku], h->klo[ku]), h->khi[ku]);
+            if (h->hours[us] >= h->sweep_long_hours && kst * h->hours[us] / h->dt >= 1.0) {
+                a.elig[si][n >> 5] |= 1u << (n & 31);
+                any = true;
+            }
+        }
+    if (!any) return 0;
+    a.st_long = h->d_stats + (size_t)ESM_FAM_SLONG * ESM_NSTAT * STAT_SLOTS;
+    a.st_level = h->d_stats + (size_t)ESM_FAM_SLEVEL * ESM_NSTAT * STAT_SLOTS;
+    int lds_max = 0;
+    ESM_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+    const size_t stat_lds = 3 * sizeof(int) * SLG_MAXM;
+    const bool yl = slg_words<S>(T, true) * sizeof(S) + stat_lds <= (size_t)lds_max;
+    const size_t lds = slg_words<S>(T, yl) * sizeof(S);
+    const DevModel<S> m = dev_model<S>(h);
+    SweepState<S> st{(S *)h->d_v, (S *)h->d_r, (S *)h->d_F, (const S *)h->d_x};
+    const dim3 grid((unsigned)(N * R)), blk(64);
+    ProfScope p(h, ESM_KERN_SLONG);
+    if (yl)
+        hipLaunchKernelGGL((k_storage_long<S, true>), grid, blk, lds, h->stream, m, st, a, (S)h->m.beta_v, (S)maxnats,
+                           S(1));
+    else
+        hipLaunchKernelGGL((k_storage_long<S, false>), grid, blk, lds, h->stream, m, st, a, (S)h->m.beta_v,
+                           (S)maxnats, S(1));
+    ESM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template int launch_storage_long<float>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
+template int launch_storage_long<double>(esm_handle *, uint32_t, uint32_t, uint32_t, double);
 
 }  // namespace esm

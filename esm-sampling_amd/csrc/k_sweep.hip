@@ -20,8 +20,8 @@
 // before the round and the order of sites inside a round does not matter: GEN all (n, τ);
 // THETA the nodes of one colour of a distance-2 colouring, every τ; SOC one storage carrier and
 // one time-parity class (τ and τ ± 1 share a storage term). After the atoms the sweep runs the
-// random plain families (k_random.hip) and, with probability 1/2, the storage multiscale sweep
-// (k_storage.hip).
+// random plain families (k_random.hip), with probability 1/2 the storage multiscale sweep and
+// with probability `storage_long` the long-range storage sweep (k_storage.hip).
 //
 // Random numbers: per (site, replica) two uniforms from Philox4x32-10 with counter
 // (replica, n·T + τ, round id, sweep counter).
@@ -139,11 +139,12 @@ int launch_sweeps(esm_handle *h, int nsweeps, uint32_t k0, uint32_t k1, uint32_t
                 }
             }
         }
-        // the random families, per time-parity class, then (with its probability) the storage
-        // multiscale sweep (`operation_sweep!`, src/local.jl:2413-2418)
+        // the random families, per time-parity class, then (each with its probability) the storage
+        // multiscale sweep and the long-range storage sweep (`operation_sweep!`, src/local.jl:2413-2423)
         int rc = launch_random<S>(h, k0, k1, a.ctr, maxnats);
         if (rc) return rc;
         if ((rc = launch_storage_ms<S>(h, k0, k1, a.ctr, maxnats))) return rc;
+        if ((rc = launch_storage_long<S>(h, k0, k1, a.ctr, maxnats))) return rc;
     }
     ESM_HIP_CHECK(hipGetLastError());
     return 0;

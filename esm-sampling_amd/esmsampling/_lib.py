@@ -97,12 +97,13 @@ SIGNATURES = [
     ("esm_upload_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
     ("esm_copy_replica", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     ("esm_sweep_options", C.c_int32, [C.c_void_p, C.c_double, _dp, C.c_int32, C.c_double]),
+    ("esm_storage_long_options", C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_double]),
     ("esm_profile", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_profile_read", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
 ]
 (KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW,
- KERN_UNIT, KERN_POT, KERN_CELL, KERN_XLOC, KERN_XLOCC, KERN_XPAIR, KERN_SMS) = range(16)
-NFAM, NSTAT, NWEIGHT = 11, 5, 9
+ KERN_UNIT, KERN_POT, KERN_CELL, KERN_XLOC, KERN_XLOCC, KERN_XPAIR, KERN_SMS, KERN_SLONG) = range(17)
+NFAM, NSTAT, NWEIGHT = 13, 5, 9
 # the plain random families and their default weights (plain_families, src/local.jl:1912-1915)
 WEIGHT_NAMES = ("unit", "pair", "xpair", "xlocc", "xloc", "potential", "cellpair", "cell", "spair")
 DEFAULT_WEIGHTS = (0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3)

@@ -137,7 +137,7 @@ def _key(key):
 
 
 FAMILY_NAMES = ("gen", "theta", "soc", "spair", "unit", "potential", "cell", "xloc", "xlocc", "xpair",
-                "storage_ms")
+                "storage_ms", "slong", "slevel")
 
 
 def sweep_options(R: Replicas, extra: float = 4.0, weights=None, atoms: bool = True, p_storage_ms: float = 0.5):
@@ -152,10 +152,19 @@ def sweep_options(R: Replicas, extra: float = 4.0, weights=None, atoms: bool = T
                                   float(p_storage_ms)), "esm_sweep_options")
 
 
+def storage_long_options(R: Replicas, p: float = 1.0, nmoves: int = 32, hmin: float = 24.0):
+    """The long-range storage sweep (`storage_long_sweep!`, src/storage_long.jl:245-298) that
+    `operation_sweep!` runs with probability ``p`` (``storage_long``, src/run.jl:27-29): a level
+    move and ``nmoves`` long-range plateaus (``storage_long_moves``) per storage unit with at least
+    ``hmin`` hours (``storage_long_hours``)."""
+    check(R.lib.esm_storage_long_options(R.handle, float(p), int(nmoves), float(hmin)), "esm_storage_long_options")
+
+
 def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0, maxnats: float = 40.0):
     """``operation_sweep!`` / ``local_sweep!`` (src/local.jl:1927-2062, :2401-2425) of every replica,
-    ``nsweeps`` times at fixed capacities and weather: the atoms, the random families, then (with
-    its probability) the storage multiscale sweep. Returns the move statistics [11 families
+    ``nsweeps`` times at fixed capacities and weather: the atoms, the random families, then (each
+    with its probability) the storage multiscale and long-range storage sweeps. Returns the move
+    statistics [13 families
     (``FAMILY_NAMES``), 5] (attempted, accepted,
     improper, point-cap / non-finite / Jacobian-rejected, ΔC evaluations)."""
     st = np.zeros(_lib.NFAM * _lib.NSTAT, dtype=np.int64)

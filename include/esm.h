@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 2
+#define ESM_ABI_VERSION 3
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -185,7 +185,9 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
 #define ESM_FAM_XLOCC 8   /* random: localized pair, leak compensated pro rata        */
 #define ESM_FAM_XPAIR 9   /* random: cross-node pair, global re-routing               */
 #define ESM_FAM_SMS 10    /* storage multiscale: block plateaus (storage_ms_sweep!)   */
-#define ESM_NFAM 11
+#define ESM_FAM_SLONG 11  /* long-range storage plateau (slong, storage_long_sweep!)  */
+#define ESM_FAM_SLEVEL 12 /* level move of a storage trajectory (slevel)              */
+#define ESM_NFAM 13
 #define ESM_NSTAT 5       /* attempted, accepted, improper, guard, ΔC evaluations     */
 /* the plain random families in the reference's order and default weights
  * (plain_families, src/local.jl:1912-1915): unit 0.3, pair 0.25, xpair 0.03, xlocc 0.08,
@@ -236,6 +238,22 @@ int32_t esm_sweeps(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32
  * 0x50000000 + 16L + m, c) and Jacobian test (replica, b0, 0x70000000 + 16L + m, c).
  */
 int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, int32_t atoms, double p_storage_ms);
+
+/*
+ * Options of the long-range storage sweep (`storage_long_sweep!`, src/storage_long.jl:245-298),
+ * which the operation sweep runs after the multiscale sweep with probability `p`
+ * (`storage_long`, 1 by default, src/run.jl:27-29): for every storage unit with at least `hmin`
+ * hours (`storage_long_hours`, 24) and κs = k h/Δt ≥ 1, a level move (every instant of the year
+ * rises by t) and `nmoves` (`storage_long_moves`, 32; at most 256) long-range plateaus (the state
+ * of charge rises at the cyclic instants a+1 … a+L, L uniform on 1 … T−1, a uniform; the node's
+ * gas unit follows pro rata at both end steps, with the Jacobian test), per node in a shuffled
+ * order. Draws (Philox under the sweep's key and counter c): whether it runs (0, 0, 9410, c);
+ * for node n and its u-th unit taking part the level move (n, u, 9411, c) and plateau m's L and a
+ * (n, 65536u + m, 9412, c); the node's list shuffled by Fisher–Yates with (n, i, 9413, c); move
+ * i's line draw (replica, n, 0x54000000 + i, c) and Jacobian test (replica, n,
+ * 0x74000000 + i, c). Statistics: families ESM_FAM_SLONG (plateaus) and ESM_FAM_SLEVEL.
+ */
+int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double hmin);
 
 /*
  * `follow_capacities!` (src/replicas.jl:448-472) of every replica: move the shared capacities
@@ -314,7 +332,8 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_KERN_XLOCC 13  /* random compensated localized pairs         */
 #define ESM_KERN_XPAIR 14  /* random re-routed pairs                     */
 #define ESM_KERN_SMS 15    /* storage multiscale moves of one block length and class */
-#define ESM_NKERN 16
+#define ESM_KERN_SLONG 16  /* long-range storage moves (one launch per sweep)         */
+#define ESM_NKERN 17
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
 

@@ -49,7 +49,8 @@ void orc_regime_window_path(const esm_model *m, int *path, int a, int b, const d
  * esm_sweeps (the evaluation count is the oracle's own); nthreads OpenMP threads over the replicas. */
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
                const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
-               double p_storage_ms, int64_t *stats, int nthreads);
+               double p_storage_ms, double p_storage_long, int long_moves, double long_hours, int64_t *stats,
+               int nthreads);
 /* One storage multiscale plateau (oracle_sweep.c) on the block b0 … b0 + L of replica j for the
  * units of carrier si at nodes[nu]: ΔC(t), draws with given uniforms (no commit), the commit. */
 int orc_sms_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int L, int b0,
@@ -59,6 +60,15 @@ int orc_sms_draw_many(const esm_model *m, int R, double *v, const double *k, dou
                       int nd, double *t_out, int *status);
 int orc_sms_apply(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int L, int b0,
                   int si, const int *nodes, int nu, double t);
+/* long-range storage moves of unit (si, n) of replica j: a plateau on instants a+1 … a+L (cyclic,
+ * a 0-based), L = T the level move (storage_long.jl) */
+int orc_slong_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int si, int n,
+                   int a, int L, const double *t, int nt, double *dc);
+int orc_slong_draw_many(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int si,
+                        int n, int a, int L, double maxnats, const double *useg, const double *uoff, int nd,
+                        double *t_out, int *status);
+int orc_slong_apply(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int si, int n,
+                    int a, int L, double t);
 /* Templates of the table-driven random moves: info [8] (templates, first xpair, first cell,
  * levels, cells per level), xp_off [N + 1]; the touched and leak nodes of one template. */
 int orc_tables_info(const esm_model *m, int *info, int *xp_off);

@@ -46,7 +46,14 @@ def load():
         lib.orc_regime_window_path.restype = None
         i64p = C.POINTER(C.c_int64)
         lib.orc_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double,
-                                   C.c_double, _dp, C.c_int, C.c_double, i64p, C.c_int]
+                                   C.c_double, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, i64p,
+                                   C.c_int]
+        lib.orc_slong_cost.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       _dp, C.c_int, _dp]
+        lib.orc_slong_draw_many.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.c_int, C.c_double, _dp, _dp, C.c_int, _dp, _ip]
+        lib.orc_slong_apply.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_double]
         lib.orc_sms_cost.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _ip,
                                      C.c_int, _dp, C.c_int, _dp]
         lib.orc_sms_draw_many.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, _ip,
@@ -148,8 +155,9 @@ class Oracle:
         return m, z, x, v, moved
 
     def sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1,
-               extra=4.0, weights=None, atoms=True, p_storage_ms=0.5):
-        """Operation sweeps in place on copies; returns (v, r, F, stats [11, 5])."""
+               extra=4.0, weights=None, atoms=True, p_storage_ms=0.5, p_storage_long=1.0, long_moves=32,
+               long_hours=24.0):
+        """Operation sweeps in place on copies; returns (v, r, F, stats [13, 5])."""
         R = v.shape[-1]
         v = np.ascontiguousarray(v, dtype=np.float64).copy()
         k = np.ascontiguousarray(k, dtype=np.float64)
@@ -157,14 +165,15 @@ class Oracle:
             _, _, r, F = self.energy(v, k, x)
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         F = np.ascontiguousarray(F, dtype=np.float64).copy()
-        st = np.zeros(55, dtype=np.int64)
+        st = np.zeros(65, dtype=np.int64)
         kk = (C.c_uint32 * 2)(*key)
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         rc = self.lib.orc_sweeps(C.byref(self.cm), R, _p(v), _p(k), _p(r), _p(F), int(nsweeps), kk, int(counter),
                                  float(maxnats), float(extra), _p(w), int(bool(atoms)), float(p_storage_ms),
+                                 float(p_storage_long), int(long_moves), float(long_hours),
                                  st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
         assert rc == 0, rc
-        return v, r, F, st.reshape(11, 5)
+        return v, r, F, st.reshape(13, 5)
 
     # -- storage multiscale plateaus -----------------------------------------------------------
     def sms_cost(self, v, k, r, F, j, L, b0, si, nodes, ts):
@@ -195,6 +204,32 @@ class Oracle:
         nd = np.ascontiguousarray(nodes, dtype=np.int32)
         rc = self.lib.orc_sms_apply(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, L, b0, si,
                                     nd.ctypes.data_as(_ip), len(nd), float(t))
+        assert rc == 0, rc
+
+    # -- long-range storage moves (L = T: the level move) -----------------------------------------
+    def slong_cost(self, v, k, r, F, j, si, n, a, L, ts):
+        R = v.shape[-1]
+        ts = np.ascontiguousarray(ts, dtype=np.float64)
+        out = np.empty_like(ts)
+        rc = self.lib.orc_slong_cost(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, si, n, a, L, _p(ts), len(ts),
+                                     _p(out))
+        assert rc == 0, rc
+        return out
+
+    def slong_draw_many(self, v, k, r, F, j, si, n, a, L, useg, uoff, maxnats=40.0):
+        R = v.shape[-1]
+        useg = np.ascontiguousarray(useg, dtype=np.float64)
+        uoff = np.ascontiguousarray(uoff, dtype=np.float64)
+        t = np.empty(len(useg))
+        st = np.empty(len(useg), dtype=np.int32)
+        rc = self.lib.orc_slong_draw_many(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, si, n, a, L,
+                                          float(maxnats), _p(useg), _p(uoff), len(useg), _p(t), st.ctypes.data_as(_ip))
+        assert rc == 0, rc
+        return t, st
+
+    def slong_apply(self, v, k, r, F, j, si, n, a, L, t):
+        R = v.shape[-1]
+        rc = self.lib.orc_slong_apply(C.byref(self.cm), R, *self._mv_args(v, k, r, F), j, si, n, a, L, float(t))
         assert rc == 0, rc
 
     # -- table-driven random moves (kind 0 cell, 1 xloc, 2 xlocc, 3 xpair) ----------------------

@@ -2168,9 +2168,423 @@ int orc_sms_apply(const esm_model *m, int R, double *v, const double *k, double 
     return 0;
 }
 
+/* ============================================================================================
+ * The long-range storage sweep (storage_long_sweep!, src/storage_long.jl:245-298): per cell, for
+ * every storage unit with at least `hmin` hours and κs = k h/Δt ≥ 1, a level move (L = T: every
+ * instant rises) and `nmoves` long-range plateaus (the state of charge rises by t at the cyclic
+ * instants a+1 … a+L, L uniform on 1 … T−1, a uniform; prep_slong! :77-146), in a shuffled
+ * order; the node's gas unit compensates pro rata at both end steps (prorata! with soc_qadj),
+ * with the Jacobian test. Interior steps whose two instants stay in [0, 1] change nothing: their
+ * cost change is exactly zero and their secondaries keep their values at t = 0 (mv_eval!
+ * :148-188). The arithmetic is the device's: step q of the move (q = 0 … L, or 1 … T for the
+ * level move; step q is τ = a + q mod T) belongs to lane q mod 64; a lane sums its steps' terms
+ * in increasing q, the lanes' sums are combined by the 64-lane butterfly. Selection (shared by
+ * the replicas, Philox under the sweep's key and counter c): whether the sweep runs
+ * (0, 0, 9410, c); per cell (its node n) and unit u of the cell: the level move (n, u, 9411, c);
+ * plateau m: L and a from (n, 65536u + m, 9412, c); the cell's move list shuffled by
+ * Fisher–Yates with (n, i, 9413, c); move i's line draw (replica, n, 0x54000000 + i, c) and
+ * Jacobian test (replica, n, 0x74000000 + i, c).
+ * ============================================================================================ */
+
+typedef struct {
+    Sys *S;
+    int j, si, n, a0, L, level;
+    double kst, ks;
+    SmsKink kk[2];                 /* the kinks at step a (q = 0) and step b (q = L) */
+} SlgMove;
+
+static int slg_tau(const SlgMove *f, int q) { return (f->a0 + q) % f->S->m->T; }
+static int slg_q0(const SlgMove *f) { return f->level ? 1 : 0; }
+
+/* whether interior step q keeps both its instants in [0, 1] at t (its terms do not change) */
+static int slg_inside(const SlgMove *f, int q, double t) {
+    Sys *S = f->S;
+    int j = f->j, T = S->m->T;
+    if (!f->level && (q == 0 || q == f->L)) return 0;
+    int tau = slg_tau(f, q), tau1 = tau + 1 == T ? 0 : tau + 1;
+    double ye = AT(S->ye[f->si], f->n, tau), ye1 = AT(S->ye[f->si], f->n, tau1);
+    return 0.0 <= ye && ye <= 1.0 && 0.0 <= ye1 && ye1 <= 1.0 && fmax(-ye, -ye1) <= t && t <= fmin(1.0 - ye, 1.0 - ye1);
+}
+
+/* term of step q at t; *d1 the storage quantity d, *S1 the balance, *moved 0 when an interior
+ * step keeps both instants in [0, 1] (no change: d and the balance keep their values at t = 0) */
+static double slg_step(const SlgMove *f, int q, double t, double *d1, double *S1, int *moved) {
+    Sys *S = f->S;
+    const esm_model *m = S->m;
+    int j = f->j, N = m->N, T = m->T, n = f->n;
+    int tau = slg_tau(f, q), tau1 = tau + 1 == T ? 0 : tau + 1;
+    double ye = AT(S->ye[f->si], n, tau), ye1 = AT(S->ye[f->si], n, tau1), S0 = AT(S->r, n, tau);
+    double w = f->kst + m->kappa0, d0, j0, j1;
+    int end = !f->level && (q == 0 || q == f->L);
+    double c0 = soc_cost(m, f->si, n, f->kst, w, m->dt, m->c_viol, ye, ye1, &d0, &j0);
+    if (!end) {
+        if (slg_inside(f, q, t)) {
+            *d1 = d0;
+            *S1 = S0;
+            *moved = 0;
+            return 0.0;
+        }
+        *moved = 1;
+        double c1 = soc_cost(m, f->si, n, f->kst, w, m->dt, m->c_viol, ye + t, ye1 + t, d1, &j1);
+        *S1 = S0 + (j1 - j0);
+        return (c1 - c0) + m->dt * (phibal(m, *S1) - phibal(m, S0));
+    }
+    *moved = 1;
+    double D = 0.0, dr = 0.0;
+    const SmsKink *kk = &f->kk[q == 0 ? 0 : 1];
+    if (kk->valid && kk->ak != 0.0) {
+        double p0, p1, c = S->kc[ESM_KGAS * N + n], cg0, cg1;
+        gen_cost(m->mc[ESM_KGAS * N + n], 1.0, c, c + m->kappa0, m->dt, m->c_viol, kk->y0, &cg0, &p0);
+        gen_cost(m->mc[ESM_KGAS * N + n], 1.0, c, c + m->kappa0, m->dt, m->c_viol, kk->y0 + kk->ak * sm_phi(kk->Q, kk->g, t),
+                 &cg1, &p1);
+        D += cg1 - cg0;
+        dr -= p1 - p0;
+    }
+    double dye = q >= 1 ? 1.0 : 0.0, dye1 = q <= f->L - 1 ? 1.0 : 0.0;
+    double c1 = soc_cost(m, f->si, n, f->kst, w, m->dt, m->c_viol, ye + t * dye, ye1 + t * dye1, d1, &j1);
+    D += c1 - c0;
+    dr += j1 - j0;
+    *S1 = S0 + dr;
+    D += m->dt * (phibal(m, *S1) - phibal(m, S0));
+    return D;
+}
+
+static double slg_eval(const SlgMove *f, double t) {
+    double part[64] = {0};
+    g_nev++;
+    for (int q = slg_q0(f); q <= f->L; q++) {
+        if (slg_inside(f, q, t)) continue;
+        double d1, S1;
+        int moved;
+        double x = slg_step(f, q, t, &d1, &S1, &moved);
+        part[q % 64] += x;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        double nx[64];
+        for (int l = 0; l < 64; l++) nx[l] = part[l] + part[l ^ off];
+        memcpy(part, nx, sizeof(part));
+    }
+    return part[0];
+}
+
+static int slg_next_bp(const SlgMove *f, double tp, double sgn, double *t) {
+    Sys *S = f->S;
+    int j = f->j, have = 0;
+#define SCAND(bv)                                                                    \
+    do {                                                                             \
+        double b_ = (bv);                                                            \
+        if (isfinite(b_) && sgn * (b_ - tp) > 0 && (!have || sgn * (*t - b_) > 0)) { \
+            *t = b_;                                                                 \
+            have = 1;                                                                \
+        }                                                                            \
+    } while (0)
+    if (!f->level)
+        for (int kidx = 0; kidx < 2; kidx++) {
+            const SmsKink *kk = &f->kk[kidx];
+            if (!kk->valid) continue;
+            SCAND(-kk->Q / kk->g);
+            if (kk->ak != 0.0) {
+                double sl = kk->ak * kk->g, off = kk->ak * kk->Q;
+                SCAND((0.0 - kk->y0 - 0.0) / sl);
+                SCAND((1.0 - kk->y0 - 0.0) / sl);
+                SCAND((0.0 - kk->y0 - off) / sl);
+                SCAND((1.0 - kk->y0 - off) / sl);
+            }
+        }
+    for (int q = 1; q <= f->L; q++) {
+        double ye = AT(S->ye[f->si], f->n, slg_tau(f, q));
+        SCAND((0.0 - ye) / 1.0);
+        SCAND((1.0 - ye) / 1.0);
+    }
+#undef SCAND
+    return have;
+}
+
+static int slg_next_root(const SlgMove *f, int stor, double a, double b, double x, double sgn, double *xr) {
+    const esm_model *m = f->S->m;
+    int us = f->si * m->N + f->n, have = 0;
+    double ein = m->stor_eta_in[us], eout = m->stor_eta_out[us];
+    for (int q = slg_q0(f); q <= f->L; q++) {
+        if (slg_inside(f, q, a) && slg_inside(f, q, b)) continue;   /* constant secondaries */
+        double da, db, Ba, Bb;
+        int mv;
+        slg_step(f, q, a, &da, &Ba, &mv);
+        slg_step(f, q, b, &db, &Bb, &mv);
+        for (int k = 0; k < (stor ? 3 : 1); k++) {
+            double A, B;
+            if (!stor) { A = Ba; B = Bb; }
+            else if (k == 0) { A = da; B = db; }
+            else if (k == 1) { A = ein > 0 ? da - ein : 1.0; B = ein > 0 ? db - ein : 1.0; }
+            else { A = -da * eout - 1.0; B = -db * eout - 1.0; }
+            if ((A < 0) != (B < 0) && A != B) {
+                double r = a + (b - a) * A / (A - B);
+                if (sgn * (r - a) > 0 && sgn * (b - r) > 0 && sgn * (r - x) > 0 && (!have || sgn * (*xr - r) > 0)) {
+                    *xr = r;
+                    have = 1;
+                }
+            }
+        }
+    }
+    return have;
+}
+
+static int slg_walk(const SlgMove *f, double beta, double sgn, double maxnats, BigPts *P) {
+    double cmin = 0.0, tp = 0.0, delta = 0.0, cprev = 0.0;
+    int nx = 0;
+    while (1) {
+        double t = 0.0;
+        if (!slg_next_bp(f, tp, sgn, &t)) {
+            double hp = 1e-6 * fmax(1.0, fabs(tp));
+            double sl = (beta * slg_eval(f, tp + sgn * hp) - cprev) / hp;
+            if (sl > 0) delta = fmin(2 * maxnats / sl, 1e3 * fmax(fabs(tp), 1e-3));
+            else delta = delta == 0.0 ? fmax(fabs(tp), 1e-6) : 2 * delta;
+            t = tp + sgn * delta;
+            if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;
+        }
+        double c = beta * slg_eval(f, t);
+        double a0 = tp, xs = tp, x = 0.0;
+        while (slg_next_root(f, 1, tp, t, xs, sgn, &x)) {
+            double cx = beta * slg_eval(f, x);
+            double xb = a0, y = 0.0;
+            while (slg_next_root(f, 0, a0, x, xb, sgn, &y)) {
+                double cy = beta * slg_eval(f, y);
+                bpush(P, y, cy);
+                cmin = fmin(cmin, cy);
+                xb = y;
+            }
+            bpush(P, x, cx);
+            cmin = fmin(cmin, cx);
+            a0 = x;
+            xs = x;
+        }
+        double xb = a0, y = 0.0;
+        while (slg_next_root(f, 0, a0, t, xb, sgn, &y)) {
+            double cy = beta * slg_eval(f, y);
+            bpush(P, y, cy);
+            cmin = fmin(cmin, cy);
+            xb = y;
+        }
+        bpush(P, t, c);
+        if (P->full) return -1;
+        cmin = fmin(cmin, c);
+        cprev = c;
+        if (c - cmin > maxnats) return 1;
+        tp = t;
+    }
+}
+
+static double slg_sample(const SlgMove *f, double beta, double maxnats, double useg, double uoff, int *status) {
+    static __thread double bufs[4][SMS_PMAX];
+    BigPts PR = {bufs[0], bufs[1], 0, 0}, PL = {bufs[2], bufs[3], 0, 0};
+    int w = slg_walk(f, beta, 1.0, maxnats, &PR);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    w = slg_walk(f, beta, -1.0, maxnats, &PL);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    return draw_points_big(&PL, &PR, useg, uoff, status);
+}
+
+/* a move of unit (si, n) of replica j: L < T a plateau from step a (0-based) to a + L, L = T the
+ * level move; the kinks at both end steps (prorata! with soc_qadj, as the multiscale blocks) */
+static void slg_prepare(SlgMove *f, Sys *S, int j, int si, int n, int a, int L) {
+    const esm_model *m = S->m;
+    int N = m->N, T = m->T, us = si * N + n;
+    memset(f, 0, sizeof(*f));
+    f->S = S; f->j = j; f->si = si; f->n = n; f->L = L;
+    f->level = L >= T;
+    f->a0 = f->level ? T - 1 : a;
+    if (f->level) f->L = T;
+    f->kst = S->kc[(ESM_KSTOR + si) * N + n];
+    f->ks = f->kst * m->stor_hours[us] / m->dt;
+    if (f->level) return;
+    for (int kidx = 0; kidx < 2; kidx++) {
+        SmsKink *kk = &f->kk[kidx];
+        int tau = slg_tau(f, kidx == 0 ? 0 : f->L), tau1 = tau + 1 == T ? 0 : tau + 1;
+        double k = S->kc[ESM_KGAS * N + n];
+        double g = kidx == 0 ? f->ks * (1.0 - 0.0) : f->ks * (0.0 - 1.0);
+        if (!(fabs(k) >= 1.0) || g == 0.0) continue;
+        double w = AT(S->yg, n, tau);
+        double P = 0.0 + k * w, Qs = 0.0 + k * clampd(w, 0.0, 1.0);
+        if (fabs(P) < 1e-6) continue;
+        double Y = AT(S->ye[si], n, tau), Y1 = AT(S->ye[si], n, tau1);
+        double ec = clampd(Y, 0.0, 1.0), ec1 = clampd(Y1, 0.0, 1.0);
+        double d = m->stor_hours[us] * (ec1 - ec) / m->dt - m->stor_qt[us], pc, mcp;
+        if (d >= 0) { pc = m->stor_eta_in[us] > 0 ? d / m->stor_eta_in[us] : 0.0; mcp = 0.0; }
+        else { pc = 0.0; mcp = fmin(-d * m->stor_eta_out[us], 1.0); }
+        double qadj = 0.0 + (f->kst * m->stor_hours[us] * (Y1 - Y) / m->dt - f->kst * (pc - mcp));
+        kk->Q = AT(S->r, n, tau) + Qs + qadj;
+        kk->g = g;
+        kk->e = P - fmax(kk->Q, 0.0);
+        kk->P = P;
+        double sig = k * w / P;
+        kk->ak = sig == 0.0 ? 0.0 : sig / k;
+        kk->y0 = w;
+        kk->valid = 1;
+    }
+}
+
+static void slg_commit(const SlgMove *f, double t) {
+    Sys *S = f->S;
+    int j = f->j, n = f->n;
+    if (t == 0.0) return;
+    double *nr = (double *)malloc(sizeof(double) * (f->L + 1));
+    int *mv = (int *)malloc(sizeof(int) * (f->L + 1));
+    for (int q = slg_q0(f); q <= f->L; q++) {   /* every term from the state before the move */
+        double d1;
+        slg_step(f, q, t, &d1, &nr[q], &mv[q]);
+    }
+    for (int q = slg_q0(f); q <= f->L; q++) {
+        int tau = slg_tau(f, q);
+        if (mv[q]) AT(S->r, n, tau) = nr[q];
+        if (!f->level && (q == 0 || q == f->L)) {
+            const SmsKink *kk = &f->kk[q == 0 ? 0 : 1];
+            if (kk->valid && kk->ak != 0.0) AT(S->yg, n, tau) = kk->y0 + kk->ak * sm_phi(kk->Q, kk->g, t);
+        }
+    }
+    for (int q = 1; q <= f->L; q++) AT(S->ye[f->si], n, slg_tau(f, q)) += t;
+    free(nr); free(mv);
+}
+
+static double slg_logjac(const SlgMove *f, double t, int *any) {
+    double part[64] = {0};
+    int cnt = 0;
+    if (!f->level)
+        for (int kidx = 0; kidx < 2; kidx++) {
+            const SmsKink *kk = &f->kk[kidx];
+            if (!kk->valid) continue;
+            cnt++;
+            double nm = 0.0;
+            part[(kidx == 0 ? 0 : f->L) % 64] += nm * log(fabs((fmax(kk->Q + kk->g * t, 0.0) + kk->e) / kk->P));
+        }
+    *any = cnt > 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        double nx[64];
+        for (int l = 0; l < 64; l++) nx[l] = part[l] + part[l ^ off];
+        memcpy(part, nx, sizeof(part));
+    }
+    return part[0];
+}
+
+static double slg_move(Sys *S, int j, int si, int n, int a, int L, double useg, double uoff, double umh,
+                       double maxnats, int64_t *stats) {
+    SlgMove f;
+    slg_prepare(&f, S, j, si, n, a, L);
+    int status;
+    g_nev = 0;
+    double t = slg_sample(&f, S->m->beta_v, maxnats, useg, uoff, &status);
+    if (t != 0.0) {
+        int any;
+        double lj = slg_logjac(&f, t, &any);
+        if (any && !(umh < exp(fmin(0.0, lj)))) { t = 0.0; status = 2; }
+    }
+    long nev = g_nev;
+    slg_commit(&f, t);
+    if (stats) stat_add(stats, status, t, nev);
+    return t;
+}
+
+/* the move list of the cell of node n (units of the node with hours ≥ hmin and κs ≥ 1, in
+ * carrier order): level moves and plateaus, shuffled; entry = (unit carrier, a, L) */
+static int slg_moves(const esm_model *m, const double *kc, int n, int nmoves, double hmin, const uint32_t key[2],
+                     uint32_t ctr, int *msi, int *ma, int *mL) {
+    int N = m->N, T = m->T, cnt = 0, u = 0;
+    for (int si = 0; si < ESM_NSTOR; si++) {
+        int us = si * N + n;
+        double kst = kc[(ESM_KSTOR + si) * N + n];
+        if (!(m->stor_hours[us] >= hmin && kst * m->stor_hours[us] / m->dt >= 1.0)) continue;
+        uint32_t c1[4] = {(uint32_t)n, (uint32_t)u, 9411u, ctr}, w[4];
+        orc_philox(key, c1, w);
+        if (u01(w[0], w[1]) < 1.0) { msi[cnt] = si; ma[cnt] = 0; mL[cnt] = T; cnt++; }
+        for (int mm = 0; mm < nmoves; mm++) {
+            uint32_t c2[4] = {(uint32_t)n, (uint32_t)(65536u * (uint32_t)u + (uint32_t)mm), 9412u, ctr};
+            orc_philox(key, c2, w);
+            msi[cnt] = si;
+            mL[cnt] = 1 + (int)(((uint64_t)w[0] * (uint32_t)(T - 1)) >> 32);
+            ma[cnt] = (int)(((uint64_t)w[1] * (uint32_t)T) >> 32);
+            cnt++;
+        }
+        u++;
+    }
+    for (int i = cnt - 1; i >= 1; i--) {
+        uint32_t c3[4] = {(uint32_t)n, (uint32_t)i, 9413u, ctr}, w[4];
+        orc_philox(key, c3, w);
+        int jx = (int)(((uint64_t)w[0] * (uint32_t)(i + 1)) >> 32), tmp;
+        tmp = msi[i]; msi[i] = msi[jx]; msi[jx] = tmp;
+        tmp = ma[i]; ma[i] = ma[jx]; ma[jx] = tmp;
+        tmp = mL[i]; mL[i] = mL[jx]; mL[jx] = tmp;
+    }
+    return cnt;
+}
+
+static void slg_sweep(Sys *S, int j, double plong, int nmoves, double hmin, const uint32_t key[2], uint32_t ctr,
+                      double maxnats, int64_t *st_long, int64_t *st_level) {
+    const esm_model *m = S->m;
+    uint32_t c0[4] = {0u, 0u, 9410u, ctr}, w[4];
+    orc_philox(key, c0, w);
+    if (!(u01(w[0], w[1]) < plong)) return;
+    int cap = ESM_NSTOR * (nmoves + 1);
+    int *msi = (int *)malloc(sizeof(int) * cap), *ma = (int *)malloc(sizeof(int) * cap), *mL = (int *)malloc(sizeof(int) * cap);
+    for (int n = 0; n < m->N; n++) {
+        int cnt = slg_moves(m, S->kc, n, nmoves, hmin, key, ctr, msi, ma, mL);
+        for (int i = 0; i < cnt; i++) {
+            uint32_t cl[4] = {(uint32_t)j, (uint32_t)n, 0x54000000u + (uint32_t)i, ctr}, wl[4];
+            orc_philox(key, cl, wl);
+            uint32_t cm[4] = {(uint32_t)j, (uint32_t)n, 0x74000000u + (uint32_t)i, ctr}, wm[4];
+            orc_philox(key, cm, wm);
+            slg_move(S, j, msi[i], n, ma[i], mL[i], u01(wl[0], wl[1]), u01(wl[2], wl[3]), u01(wm[0], wm[1]), maxnats,
+                     mL[i] >= m->T ? st_level : st_long);
+        }
+    }
+    free(msi); free(ma); free(mL);
+}
+
+/* ---- explicit long-range moves, for the pins ---- */
+int orc_slong_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int si, int n,
+                   int a, int L, const double *t, int nt, double *dc) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    if (m->N > 256 || m->L > 2048 || si < 0 || si >= ESM_NSTOR || n < 0 || n >= m->N || a < 0 || a >= m->T || L < 1 ||
+        L > m->T)
+        return -1;
+    sys_view(&S, m, R, v, k, r, F, ptr, lines);
+    SlgMove f;
+    slg_prepare(&f, &S, j, si, n, a, L);
+    for (int q = 0; q < nt; q++) dc[q] = slg_eval(&f, t[q]);
+    return 0;
+}
+
+int orc_slong_draw_many(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int si,
+                        int n, int a, int L, double maxnats, const double *useg, const double *uoff, int nd,
+                        double *t_out, int *status) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    if (m->N > 256 || m->L > 2048 || si < 0 || si >= ESM_NSTOR || n < 0 || n >= m->N || a < 0 || a >= m->T || L < 1 ||
+        L > m->T)
+        return -1;
+    sys_view(&S, m, R, v, k, r, F, ptr, lines);
+    SlgMove f;
+    slg_prepare(&f, &S, j, si, n, a, L);
+    for (int q = 0; q < nd; q++) t_out[q] = slg_sample(&f, m->beta_v, maxnats, useg[q], uoff[q], &status[q]);
+    return 0;
+}
+
+int orc_slong_apply(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int si, int n,
+                    int a, int L, double t) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    if (m->N > 256 || m->L > 2048 || si < 0 || si >= ESM_NSTOR || n < 0 || n >= m->N || a < 0 || a >= m->T || L < 1 ||
+        L > m->T)
+        return -1;
+    sys_view(&S, m, R, v, k, r, F, ptr, lines);
+    SlgMove f;
+    slg_prepare(&f, &S, j, si, n, a, L);
+    slg_commit(&f, t);
+    return 0;
+}
+
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
                const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
-               double pms, int64_t *stats, int nthreads) {
+               double pms, double plong, int long_moves, double long_hours, int64_t *stats, int nthreads) {
     int N = m->N, L = m->L, T = m->T;
     if (N > 256) return -1;
     SmsTables smst;
@@ -2246,6 +2660,9 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
                 for (int q = 0; q < ccount[c]; q++)
                     random_step(&S, &tb, j, cfirst[c] + 2 * q, cnt, wts, key, ctr, maxnats, st);
             if (pms > 0) sms_sweep(&S, &smst, j, pms, key, ctr, maxnats, st + 5 * ESM_FAM_SMS);
+            if (plong > 0)
+                slg_sweep(&S, j, plong, long_moves, long_hours, key, ctr, maxnats, st + 5 * ESM_FAM_SLONG,
+                          st + 5 * ESM_FAM_SLEVEL);
         }
     }
     if (stats) memcpy(stats, st, sizeof(st));

@@ -206,3 +206,78 @@ def test_storage_ms_fp32_consistent():
     assert abs(st[10, 1] - st_[10, 1]) <= max(3, 0.02 * st_[10, 1])
     assert np.all(drift < 1e-3 * np.abs(s.load).max())
     dev.close()
+
+
+@pytest.mark.parametrize("R,T,hmin", [(3, 47, 24.0), (64, 30, 24.0), (2, 400, 24.0), (3, 47, 0.0)])
+def test_storage_long_fp64_matches_oracle(R, T, hmin):
+    """The long-range storage sweep alone: level moves and plateaus of every unit with at least
+    hmin hours (hmin = 0: the batteries too, two units per node in one shuffled list)."""
+    print(f"storage_long fp64 R={R} T={T} hmin={hmin}", flush=True)
+    s = synth.make_system(N=10, L=16, T=T, dt=1.0, seed=14)
+    rng = np.random.default_rng(R + T)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    v[2:] = rng.uniform(0.1, 0.9, v[2:].shape)
+    dev = api.Replicas(s, R, FP64)
+    dev.upload(v=v, k=s.k0, x=x)
+    api.sweep_options(dev, extra=0.0, atoms=False, p_storage_ms=0.0)
+    api.storage_long_options(dev, p=1.0, nmoves=6, hmin=hmin)
+    st = api.operation_sweep(dev, 1, key=(5, 2), counter=7)
+    vd, _, _ = dev.download()
+    rd, Fd = dev.caches()
+    v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 1, key=(5, 2), counter=7, x=x, threads=4, extra=0.0,
+                                               atoms=False, p_storage_ms=0.0, p_storage_long=1.0, long_moves=6,
+                                               long_hours=hmin)
+    assert st_[11, 0] > 0 and st_[12, 0] > 0 and np.all(st_[:11, 0] == 0)
+    assert np.array_equal(st[:, :4], st_[:, :4]), (st[11:], st_[11:])
+    np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
+    np.testing.assert_allclose(rd, r_, atol=1e-9 * np.abs(s.load).max())
+    dev.close()
+
+
+def test_storage_long_fp64_trajectory_in_global_memory():
+    """A horizon whose trajectory does not fit the LDS (FP64, 20500 steps): the kernel reads it
+    from global memory."""
+    print("storage_long fp64 T=20500", flush=True)
+    s = synth.make_system(N=4, L=4, T=20500, dt=1.0, seed=15)
+    R = 2
+    rng = np.random.default_rng(4)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    v[2:] = rng.uniform(0.1, 0.9, v[2:].shape)
+    dev = api.Replicas(s, R, FP64)
+    dev.upload(v=v, k=s.k0, x=x)
+    api.sweep_options(dev, extra=0.0, atoms=False, p_storage_ms=0.0)
+    api.storage_long_options(dev, p=1.0, nmoves=2, hmin=24.0)
+    st = api.operation_sweep(dev, 1, key=(5, 3), counter=1)
+    vd, _, _ = dev.download()
+    v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 1, key=(5, 3), counter=1, x=x, threads=2, extra=0.0,
+                                               atoms=False, p_storage_ms=0.0, p_storage_long=1.0, long_moves=2)
+    assert st_[11, 0] == 2 * s.N * R and st_[12, 0] == s.N * R
+    assert np.array_equal(st[:, :4], st_[:, :4]), (st[11:], st_[11:])
+    np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
+    dev.close()
+
+
+def test_storage_long_fp32_consistent():
+    print("storage_long fp32", flush=True)
+    s = synth.make_system(N=10, L=16, T=300, dt=1.0, seed=16)
+    R = 64
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R)).astype(np.float32).astype(np.float64)
+    v = synth.heuristic_state(s, x)
+    v[2:] = rng.uniform(0.1, 0.9, v[2:].shape)
+    v = v.astype(np.float32).astype(np.float64)
+    dev = api.Replicas(s, R, FP32)
+    dev.upload(v=v, k=s.k0, x=x)
+    api.sweep_options(dev, extra=0.0, atoms=False, p_storage_ms=0.0)
+    api.storage_long_options(dev, p=1.0, nmoves=8)
+    st = api.operation_sweep(dev, 2, key=(6, 7), counter=0)
+    _, _, drift = api.LocalCost(dev)
+    v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 2, key=(6, 7), counter=0, x=x, threads=8, extra=0.0,
+                                               atoms=False, p_storage_ms=0.0, p_storage_long=1.0, long_moves=8)
+    for f in (11, 12):
+        assert st[f, 0] == st_[f, 0] and st[f, 2] == 0 and st[f, 3] == 0
+        assert abs(st[f, 1] - st_[f, 1]) <= max(3, 0.02 * st_[f, 1])
+    assert np.all(drift < 1e-3 * np.abs(s.load).max())
+    dev.close()
