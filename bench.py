@@ -6,8 +6,9 @@
 A step is one replica step (``replica_step!``, src/replicas.jl:676-705, ``replica_inner = 1``)
 over all replicas of one chain: every replica's weather step (pCN bursts staggered over the
 replicas, regime windows, compensated re-dispatch), one operation sweep of exact line moves
-(every atom, then round(4N) random moves per step over the plain families at the reference's
-weights), the energy totals and the capacity score, one Langevin step of the shared capacities
+(every atom, round(4N) random moves per step over the plain families at the reference's
+weights, the storage multiscale sweep with probability 1/2 and the long-range storage sweep),
+the energy totals and the capacity score, one Langevin step of the shared capacities
 and the dispatch following on every replica. The workload is the 40-node / 70-line synthetic
 system at an hourly horizon (8760 steps) with 64 weather replicas; the state is FP32 with FP64
 totals, the Jacobian test and the Langevin step in FP64. Inputs are seeded and resident in HBM
@@ -126,6 +127,12 @@ def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
         nu = 0.5 + 0.5 * float(np.bincount(s.country).mean()) if s.ncountry > 0 else 1.0
         tot = sum(8 * (T - L) / L * (5 * L + 8) * nu * es * R for L in lengths)
         out["k_storage_ms"] = tot / (2 * len(lengths))
+    # long-range storage sweep (one launch per step): a plateau reads the L + 1 instants of its unit's
+    # state of charge, r and the gas at both end steps, and writes the L rising instants, r and the
+    # gas at the ends: (2L + 9)·es with L = T/2 on average; a level move reads and writes T instants
+    nm, nl = float(stats[11, 0]), float(stats[12, 0])
+    if nm + nl > 0:
+        out["k_storage_long"] = es * (nm * (2 * (T / 2) + 9) + nl * 2 * T)
     return out
 
 
@@ -166,7 +173,7 @@ def main():
              _lib.KERN_GEN: "k_gen", _lib.KERN_THETA: "k_theta", _lib.KERN_SOC: "k_soc", _lib.KERN_SPAIR: "k_spair",
              _lib.KERN_FOLLOW: "k_follow", _lib.KERN_SWEEP: "sweep", _lib.KERN_UNIT: "k_unit",
              _lib.KERN_POT: "k_pot", _lib.KERN_CELL: "k_cell", _lib.KERN_XLOC: "k_xloc", _lib.KERN_XLOCC: "k_xlocc",
-             _lib.KERN_XPAIR: "k_xpair", _lib.KERN_SMS: "k_storage_ms"}
+             _lib.KERN_XPAIR: "k_xpair", _lib.KERN_SMS: "k_storage_ms", _lib.KERN_SLONG: "k_storage_long"}
     prof = {}
     for kid, name in names.items():
         tot, cnt = C.c_double(), C.c_int64()
@@ -228,7 +235,8 @@ def measured_traffic(kernel, s, T, R, prec):
              "k_energy": "k_energy<float, true, false>", "k_follow": "k_follow<float>",
              "k_unit": "k_rand_unit<float, true>", "k_cell": "k_sparse<float, 0, true>",
              "k_xloc": "k_sparse<float, 1, true>", "k_xlocc": "k_sparse<float, 2, true>",
-             "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms"}
+             "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms",
+             "k_storage_long": "k_storage_long<float, true>"}
     return json.loads(p.read_text())["bytes_per_dispatch"].get(names.get(kernel, ""))
 
 

@@ -710,12 +710,30 @@ template int launch_storage_ms<double>(esm_handle *, uint32_t, uint32_t, uint32_
 // cells are independent). The unit's state-of-charge trajectory is staged in LDS (T values; in
 // global memory when T does not fit); lane l owns the move's steps q ≡ l (mod 64) and keeps the
 // window of t over which its interior steps stay in the box, so an evaluation inside it costs the
-// two end steps only. The walk is the multiscale sweep's, with W = 64 and the secondaries
-// recomputed at the interval ends (only end steps and out-of-box interior steps vary). The move
+// two end steps only. The walk is the multiscale sweep's, with W = 64: the end steps'
+// secondaries are recorded by the evaluations at the interval and piece ends, the out-of-box
+// interior steps' ones are recomputed there (the others do not vary). The move
 // list is drawn from the sweep's counter by every wave of a node alike (Philox; see esm.h).
 // ---------------------------------------------------------------------------------------------
 
 constexpr int SLG_MAXM = 2 * 257;      // moves per node: per unit one level move and ≤ 256 plateaus
+constexpr int SLG_MAXCH = 64;          // window chunks per lane
+
+// interior steps of a lane per window chunk (at least 8), and the chunks, over a horizon of T
+__host__ __device__ constexpr int slg_ch(int T) {
+    return ((T + 63) / 64 + SLG_MAXCH - 1) / SLG_MAXCH > 8 ? ((T + 63) / 64 + SLG_MAXCH - 1) / SLG_MAXCH : 8;
+}
+__host__ __device__ constexpr int slg_nch(int T) { return ((T + 63) / 64 + slg_ch(T) - 1) / slg_ch(T); }
+
+// float bounds of a window rounded inward (a conservative filter of the exact test)
+__device__ __forceinline__ float inward_lo(double x) {
+    float f = (float)x;
+    return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+__device__ __forceinline__ float inward_hi(double x) {
+    float f = (float)x;
+    return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
 
 struct SlgArgs {
     uint32_t k0, k1, ctr;
@@ -727,7 +745,7 @@ struct SlgArgs {
 template <typename S, bool YL>
 struct FLong {
     enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
-    static constexpr bool CACHED = false;
+    static constexpr bool CACHED = true;    // the end steps' secondaries are recorded (interior ones recomputed)
     const DevModel<S> *m;
     const S *yv;          // the unit's state of charge: LDS [T] (YL) or the global plane at (n, ·, j)
     const S *rv;          // the node's residual plane at (n, ·, j), stride R
@@ -737,7 +755,23 @@ struct FLong {
     S kst, w;
     const SmsKink<S> *kk; // [2] in LDS: the kinks at steps a and b
     S llo, lhi;           // t in [llo, lhi]: every interior step of the lane stays in the box
+    // the same per chunk of `ch` consecutive interior steps of the lane (LDS [chunk][lane], bounds
+    // rounded inward to float: a conservative filter, every step is still tested exactly)
+    const float *cwl, *cwh;
+    int ch;
     mutable int nev = 0;
+    // the lane's nearest clamp point of its rising instants beyond the walk's last point `bp_tp`
+    // in direction `bp_sgn` (the points are fixed during a move and a walk only moves forward, so
+    // the lane scans its instants again only when the walk passes it or a new walk starts)
+    mutable S bp_tp = 0, bp_sgn = 0, bp_next = 0;
+    mutable bool bp_have = false;
+    // the end steps' d at the interval ends (A: tp, B: t) and balances at the piece ends (A: a0,
+    // X: x or t), recorded by the evaluations there; slot 0 step a, slot 1 step b
+    mutable S edA[2] = {0, 0}, edB[2] = {0, 0}, ebA[2] = {0, 0}, ebX[2] = {0, 0};
+    __device__ void record(int e, int rec, S d1, S S1) const {
+        if (rec == REC_B || rec == REC_A) (rec == REC_B ? edB : edA)[e] = d1;
+        if (rec != REC_NONE) (rec == REC_A ? ebA : ebX)[e] = S1;
+    }
 
     __device__ S y(int tau) const { return YL ? yv[tau] : yv[(size_t)tau * R]; }
     __device__ S r0(int tau) const { return rv[(size_t)tau * R]; }
@@ -784,21 +818,26 @@ struct FLong {
         D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
         return D;
     }
-    // an interior step (both instants rise): its term, d and balance (unchanged inside the box)
-    __device__ S int_term(int q, S t, S &d1, S &S1) const {
+    // an interior step (both instants rise) at t: false when both instants stay in the box (no
+    // change; with `sec`, d and the balance keep their values at t = 0: d1, S1), else its term x
+    __device__ bool int_term(int q, S t, bool sec, S &x, S &d1, S &S1) const {
         const int ta = tau(q);
-        const S ye = y(ta), ye1 = y(next(ta)), S0 = r0(ta);
+        const S ye = y(ta), ye1 = y(next(ta));
+        const bool in = S(0) <= ye && ye <= S(1) && S(0) <= ye1 && ye1 <= S(1) && fmax(-ye, -ye1) <= t &&
+                        t <= fmin(S(1) - ye, S(1) - ye1);
+        if (in && !sec) return false;
+        const S S0 = r0(ta);
         S d0, j0, j1;
         const S c0 = soc_cost_p(par, m->dt, m->cv, kst, w, ye, ye1, d0, j0);
-        if (S(0) <= ye && ye <= S(1) && S(0) <= ye1 && ye1 <= S(1) && fmax(-ye, -ye1) <= t &&
-            t <= fmin(S(1) - ye, S(1) - ye1)) {
+        if (in) {
             d1 = d0;
             S1 = S0;
-            return S(0);
+            return false;
         }
         const S c1 = soc_cost_p(par, m->dt, m->cv, kst, w, ye + t, ye1 + t, d1, j1);
         S1 = S0 + (j1 - j0);
-        return (c1 - c0) + m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+        x = (c1 - c0) + m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
+        return true;
     }
     __device__ int qi0() const { return 1; }
     __device__ int qi1() const { return level ? L : L - 1; }   // interior steps qi0 … qi1
@@ -807,15 +846,40 @@ struct FLong {
     // summed over the wave
     __device__ S eval(S t, int rec) const {
         if (rec <= REC_X) nev++;
-        S D = 0, d1, S1;
-        if (owns_a()) D += end_term(0, t, d1, S1);
+        S d1, S1, xa = 0, xb = 0;
+        const bool oa = owns_a(), ob = owns_b();
+        // the end terms in one pass of the two owning lanes (lane 0: step a, lane L mod 64: step
+        // b), a second pass for step b when lane 0 owns both (L ≡ 0 mod 64)
+        if (oa || ob) {
+            const int e = oa ? 0 : 1;
+            const S x = end_term(e == 0 ? 0 : L, t, d1, S1);
+            if (e == 0) xa = x;
+            else xb = x;
+            record(e, rec, d1, S1);
+        }
+        if (oa && ob) {
+            xb = end_term(L, t, d1, S1);
+            record(1, rec, d1, S1);
+        }
+        S D = 0;
+        if (oa) D += xa;
         if (!(llo <= t && t <= lhi))
-            for (int q = first(qi0()); q <= qi1(); q += 64)
-                if (!inside(q, t)) D += int_term(q, t, d1, S1);
-        if (owns_b()) D += end_term(L, t, d1, S1);
+            for (int q0 = first(qi0()), c = 0; q0 <= qi1(); q0 += 64 * ch, c++) {
+                if (cwl[c * 64 + lane] <= t && t <= cwh[c * 64 + lane]) continue;
+                for (int q = q0; q <= qi1() && q < q0 + 64 * ch; q += 64) {
+                    S x;
+                    if (int_term(q, t, false, x, d1, S1)) D += x;
+                }
+            }
+        if (ob) D += xb;
         return seg_sum<S, 64>(D);
     }
-    __device__ void shift(bool) const {}
+    __device__ void shift(bool d) const {
+        for (int e = 0; e < 2; e++) {
+            if (d) edA[e] = edB[e];
+            ebA[e] = ebX[e];
+        }
+    }
 
     __device__ bool next_bp(S tp, S sgn, S &t) const {
         bool have = false;
@@ -839,11 +903,27 @@ struct FLong {
                 cand((S(1) - k.y0 - off) / sl);
             }
         }
-        for (int q = first(1); q <= L; q += 64) {     // the clamp points of the rising instants
-            const S ye = y(tau(q));
-            cand((S(0) - ye) / S(1));
-            cand((S(1) - ye) / S(1));
+        // the clamp points of the rising instants (cached while the walk moves forward)
+        if (!(bp_sgn == sgn && sgn * (tp - bp_tp) >= 0 && (!bp_have || sgn * (bp_next - tp) > 0))) {
+            bool h2 = false;
+            S b2 = 0;
+            auto cand2 = [&](S b) {
+                if (isfinite(b) && sgn * (b - tp) > 0 && (!h2 || sgn * (b2 - b) > 0)) {
+                    b2 = b;
+                    h2 = true;
+                }
+            };
+            for (int q = first(1); q <= L; q += 64) {
+                const S ye = y(tau(q));
+                cand2((S(0) - ye) / S(1));
+                cand2((S(1) - ye) / S(1));
+            }
+            bp_have = h2;
+            bp_next = b2;
+            bp_sgn = sgn;
         }
+        bp_tp = tp;
+        if (bp_have) cand(bp_next);
         return seg_best<S, 64>(have, best, sgn, t);
     }
 
@@ -867,23 +947,20 @@ struct FLong {
             }
         };
         S da, db, Ba, Bb;
-        if (owns_a()) {
-            end_term(0, a, da, Ba);
-            end_term(0, b, db, Bb);
-            test(da, db, Ba, Bb);
-        }
+        if (owns_a()) test(edA[0], edB[0], ebA[0], ebX[0]);
         if (!(llo <= a && a <= lhi && llo <= b && b <= lhi))
-            for (int q = first(qi0()); q <= qi1(); q += 64) {
-                if (inside(q, a) && inside(q, b)) continue;   // constant secondaries
-                int_term(q, a, da, Ba);
-                int_term(q, b, db, Bb);
-                test(da, db, Ba, Bb);
+            for (int q0 = first(qi0()), c = 0; q0 <= qi1(); q0 += 64 * ch, c++) {
+                const float lo = cwl[c * 64 + lane], hi = cwh[c * 64 + lane];
+                if (lo <= a && a <= hi && lo <= b && b <= hi) continue;
+                for (int q = q0; q <= qi1() && q < q0 + 64 * ch; q += 64) {
+                    if (inside(q, a) && inside(q, b)) continue;   // constant secondaries
+                    S x;
+                    int_term(q, a, true, x, da, Ba);
+                    int_term(q, b, true, x, db, Bb);
+                    test(da, db, Ba, Bb);
+                }
             }
-        if (owns_b()) {
-            end_term(L, a, da, Ba);
-            end_term(L, b, db, Bb);
-            test(da, db, Ba, Bb);
-        }
+        if (owns_b()) test(edA[1], edB[1], ebA[1], ebX[1]);
         return seg_best<S, 64>(have, best, sgn, xr);
     }
 
@@ -903,10 +980,10 @@ struct FLong {
     }
 };
 
-// words (of S) of the LDS of one wave: the kinks, then the trajectory (YL)
+// bytes of the LDS of one wave: the kinks, the chunk windows, then the trajectory (YL)
 template <typename S>
-__host__ __device__ constexpr size_t slg_words(int T, bool yl) {
-    return sizeof(SmsKink<S>) / sizeof(S) * 2 + (yl ? (size_t)T : 0);
+__host__ __device__ constexpr size_t slg_bytes(int T, bool yl) {
+    return sizeof(SmsKink<S>) * 2 + sizeof(float) * 2 * 64 * (size_t)slg_nch(T) + (yl ? sizeof(S) * (size_t)T : 0);
 }
 
 template <typename S, bool YL>
@@ -954,9 +1031,10 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
             tmp = mL[i]; mL[i] = mL[jx]; mL[jx] = tmp;
         }
     __syncthreads();
-    S *lds = (S *)slg_lds;
-    SmsKink<S> *kk_l = (SmsKink<S> *)lds;
-    S *ys = lds + sizeof(SmsKink<S>) / sizeof(S) * 2;
+    SmsKink<S> *kk_l = (SmsKink<S> *)slg_lds;
+    const int nch = slg_nch(T);
+    float *cwl = (float *)(slg_lds + sizeof(SmsKink<S>) * 2), *cwh = cwl + 64 * nch;
+    S *ys = (S *)(cwh + 64 * nch);
     const size_t TR = (size_t)T * R;
     const S *rv = st.r + (size_t)n * TR + j;
     S *gv = st.v + plane(N, T, R) + (size_t)n * TR + j;
@@ -986,6 +1064,9 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
         f.kst = m.kc[(ESM_KSTOR + si) * N + n];
         f.w = f.kst + m.kappa0;
         f.kk = kk_l;
+        f.cwl = cwl;
+        f.cwh = cwh;
+        f.ch = slg_ch(T);
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -1030,16 +1111,23 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
             }
         // the lane's window: t over which every one of its interior steps stays in the box
         S llo = -INFINITY, lhi = INFINITY;
-        for (int q = f.first(f.qi0()); q <= f.qi1(); q += 64) {
-            const int tau = f.tau(q);
-            const S ye = f.y(tau), ye1 = f.y(f.next(tau));
-            if (S(0) <= ye && ye <= S(1) && S(0) <= ye1 && ye1 <= S(1)) {
-                llo = fmax(llo, fmax(-ye, -ye1));
-                lhi = fmin(lhi, fmin(S(1) - ye, S(1) - ye1));
-            } else {
-                llo = INFINITY;
-                lhi = -INFINITY;
+        for (int q0 = f.first(f.qi0()), c = 0; q0 <= f.qi1(); q0 += 64 * f.ch, c++) {
+            S clo = -INFINITY, chi = INFINITY;
+            for (int q = q0; q <= f.qi1() && q < q0 + 64 * f.ch; q += 64) {
+                const int tau = f.tau(q);
+                const S ye = f.y(tau), ye1 = f.y(f.next(tau));
+                if (S(0) <= ye && ye <= S(1) && S(0) <= ye1 && ye1 <= S(1)) {
+                    clo = fmax(clo, fmax(-ye, -ye1));
+                    chi = fmin(chi, fmin(S(1) - ye, S(1) - ye1));
+                } else {
+                    clo = INFINITY;
+                    chi = -INFINITY;
+                }
             }
+            cwl[c * 64 + lane] = inward_lo((double)clo);
+            cwh[c * 64 + lane] = inward_hi((double)chi);
+            llo = fmax(llo, clo);
+            lhi = fmin(lhi, chi);
         }
         f.llo = llo;
         f.lhi = lhi;
@@ -1075,11 +1163,10 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
                 const SmsKink<S> k = kk_l[e];
                 if (k.valid != S(0) && k.ak != S(0)) gv[(size_t)tau * R] = k.y0 + k.ak * FStor<S, 64>::kphi(k.Q, k.g, t);
             }
-            for (int q = f.first(f.qi0()); q <= f.qi1(); q += 64)
-                if (!f.inside(q, t)) {
-                    f.int_term(q, t, d1, S1);
-                    rw[(size_t)f.tau(q) * R] = S1;
-                }
+            for (int q = f.first(f.qi0()); q <= f.qi1(); q += 64) {
+                S x;
+                if (f.int_term(q, t, false, x, d1, S1)) rw[(size_t)f.tau(q) * R] = S1;
+            }
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -1131,8 +1218,12 @@ int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, d
     int lds_max = 0;
     ESM_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
     const size_t stat_lds = 3 * sizeof(int) * SLG_MAXM;
-    const bool yl = slg_words<S>(T, true) * sizeof(S) + stat_lds <= (size_t)lds_max;
-    const size_t lds = slg_words<S>(T, yl) * sizeof(S);
+    const bool yl = slg_bytes<S>(T, true) + stat_lds <= (size_t)lds_max;
+    const size_t lds = slg_bytes<S>(T, yl);
+    if (lds + stat_lds > (size_t)lds_max) {
+        set_error("long-range storage sweep: the horizon needs more LDS than the device has");
+        return ESM_EINVAL;
+    }
     const DevModel<S> m = dev_model<S>(h);
     SweepState<S> st{(S *)h->d_v, (S *)h->d_r, (S *)h->d_F, (const S *)h->d_x};
     const dim3 grid((unsigned)(N * R)), blk(64);

@@ -13,8 +13,9 @@ this path implements the feature; an unknown key raises, as ``load_config`` does
 (src/run.jl:135-150). The system is the seeded synthetic one (``synth.make_system``), set by
 the keys ``synthetic_nodes``, ``synthetic_lines``, ``snapshots``, ``resolution_hours``,
 ``system_seed`` and ``precision``, which have no counterpart in the reference (it reads its
-data file). Keys of features not built here are accepted only at their "off" value:
-``replica_ffbs = false``, ``tilt_lambda = 0``, ``storage_long = 0``, ``storage_xpair = 0``.
+data file). Keys of features not built here are accepted only at their "off" value (and default
+to it): ``replica_ffbs = false``, ``tilt_lambda = 0``, ``storage_xpair = 0``, ``storage_xlong = 0``
+(the cross-node storage pairs of storage_cross.jl).
 
 Random streams: Philox4x32-10 keys per (seed, chain, purpose) replace the MersenneTwister streams
 ``restart_seed(cfg, r, 1000c + j)`` (src/replicas.jl:872-874); the Langevin normals come from a
@@ -90,8 +91,11 @@ class Config:
     langevin_box: bool = True
     tilt_lambda: float = 0.0
     tilt_functional: str = "resload_mean"
-    storage_long: float = 0.0
+    storage_long: float = 1.0
+    storage_long_moves: int = 32
+    storage_long_hours: float = 24.0
     storage_xpair: float = 0.0
+    storage_xlong: int = 0
     maxnats: float = 40.0
     # the synthetic system (no counterpart in the reference, which reads its data file)
     synthetic_nodes: int = 40
@@ -108,8 +112,10 @@ class Config:
             raise ValueError("replica_ffbs = true: the centred FFBS of the regimes is not built (its default is false)")
         if self.tilt_lambda != 0.0:
             raise ValueError("tilt_lambda ≠ 0: the stress tilt functional is not built (λ = 0 is the untilted drift)")
-        if self.storage_long != 0.0 or self.storage_xpair != 0.0:
-            raise ValueError("storage_long / storage_xpair > 0: the long-range and cross-node storage moves are not built")
+        if self.storage_xpair != 0.0 or self.storage_xlong != 0:
+            raise ValueError("storage_xpair / storage_xlong > 0: the cross-node storage pairs (sxpair, sxlong) are not built")
+        if not (0.0 <= self.storage_long <= 1.0) or not (0 <= self.storage_long_moves <= 256) or self.storage_long_hours < 0:
+            raise ValueError("storage_long in [0, 1], storage_long_moves in [0, 256], storage_long_hours >= 0")
         if self.precision not in ("fp32", "fp64"):
             raise ValueError('precision must be "fp32" or "fp64"')
         if self.replicas < 1 or self.nchains < 1:
@@ -167,6 +173,9 @@ class ReplicaCtl:
     gap_lo: float
     eta_min: float
     eta_max: float
+    plong: float = 1.0
+    long_moves: int = 32
+    long_hours: float = 24.0
     shrink: float = 0.7
     grow: float = 1.2
 
@@ -183,7 +192,12 @@ class ReplicaCtl:
                    box=cfg.langevin_box, probe_every=cfg.replica_probe_every, probe_len=cfg.replica_probe_len,
                    probe_tail=cfg.replica_probe_tail, probe_win=cfg.replica_probe_win, probe_avg=cfg.replica_probe_avg,
                    adapt=cfg.langevin_adapt, gap_hi=cfg.langevin_gap_hi, gap_lo=cfg.langevin_gap_lo,
-                   eta_min=cfg.langevin_eta_min, eta_max=cfg.langevin_eta_max)
+                   eta_min=cfg.langevin_eta_min, eta_max=cfg.langevin_eta_max, plong=cfg.storage_long,
+                   long_moves=cfg.storage_long_moves, long_hours=cfg.storage_long_hours)
+
+    def sweep_options(self, dev):
+        """The operation sweep's options of a replica set (the long-range storage sweep's keys)."""
+        api.storage_long_options(dev, p=self.plong, nmoves=self.long_moves, hmin=self.long_hours)
 
 
 def interior_nodes(k, khi, lo: float = 50.0):
@@ -388,6 +402,7 @@ def init_replicas(sys, k0, M: int, prec: int, ctl: ReplicaCtl, keys: Keys, nswee
     """``init_replicas`` (src/replicas.jl:783-795): M replicas at capacities k0, each with an exact
     prior weather draw, the heuristic operation for its weather, and ``nsweeps`` sweeps at fixed k."""
     dev = api.Replicas(sys, M, prec)
+    ctl.sweep_options(dev)
     api.sample_weather(dev, key=keys.weather, counter=0xFFFFFFF0)
     _, _, x = dev.download()
     dev.upload(v=synth.heuristic_state(sys, x, k0), k=k0)
@@ -429,6 +444,8 @@ def run_replicas(cfg: Config, log=None):
                                   cfg.replica_init_sweeps, cfg.maxnats))
         say(f"chain {c} ({kinds[c]}): {M} replicas initialised ({cfg.replica_init_sweeps} sweeps each)")
     probe = api.Replicas(sys, 1, prec) if ctl.probe_every > 0 else None
+    if probe is not None:
+        ctl.sweep_options(probe)
     t_init = time.perf_counter()
     traces = [[] for _ in range(nc)]
     records = [[] for _ in range(nc)]

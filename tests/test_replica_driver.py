@@ -93,8 +93,17 @@ def test_config_keys():
         RP.load_config({"replica_typo": 1})
     with pytest.raises(ValueError, match="FFBS"):
         RP.load_config({"replica_ffbs": True})
-    with pytest.raises(ValueError, match="storage"):
-        RP.load_config({"storage_long": 1.0})
+    c = RP.load_config({"storage_long": 0.5, "storage_long_moves": 8, "storage_long_hours": 100.0})
+    ctl = RP.ReplicaCtl.from_config(c)
+    assert (ctl.plong, ctl.long_moves, ctl.long_hours) == (0.5, 8, 100.0)
+    d = RP.ReplicaCtl.from_config(RP.load_config({}))
+    assert (d.plong, d.long_moves, d.long_hours) == (1.0, 32, 24.0)     # src/run.jl:27-29
+    with pytest.raises(ValueError, match="sxpair"):
+        RP.load_config({"storage_xpair": 0.1})
+    with pytest.raises(ValueError, match="sxpair"):
+        RP.load_config({"storage_xlong": 8})
+    with pytest.raises(ValueError, match="storage_long"):
+        RP.load_config({"storage_long_moves": 300})
 
 
 def test_config_toml(tmp_path):
