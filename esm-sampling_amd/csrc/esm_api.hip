@@ -180,6 +180,7 @@ static int upload_move_tables(esm_handle *h) {
 static void free_handle(esm_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     clear_prof(h);
+    mlp_free(h);
     void *ps[] = {h->d_bus0, h->d_bus1, h->d_adj_ptr, h->d_adj_line, h->d_adj_other, h->mb.lb, h->mb.ls, h->mb.adj_b,
                   h->mb.load, h->mb.mc, h->mb.ef, h->mb.kc, h->mb.hours, h->mb.eta_in, h->mb.eta_out, h->mb.qt,
                   h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,

@@ -80,6 +80,10 @@ bool build_move_tables(const esm_handle *h, MoveTablesHost &mt, std::string &err
 
 }  // namespace esm
 
+namespace esm {
+struct MlpState;   // k_mlp.hip
+}
+
 struct esm_handle {
     int N = 0, L = 0, T = 0, R = 0, prec = 64;
     double dt = 1.0;
@@ -154,6 +158,8 @@ struct esm_handle {
     int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step
     esm::MoveTablesHost mt;                       // templates (host, FP64)
     esm::MoveTablesDev dmt;                       // the same on the device (values in the state precision)
+    // the learned dispatch-block proposal (esm_mlp_*; null until esm_mlp_init)
+    struct esm::MlpState *mlp = nullptr;
     // kernel timing (esm_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[ESM_NKERN];
@@ -206,6 +212,9 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
 // k_storage.hip: the storage multiscale sweep (with its probability) of one operation sweep
 template <typename S>
 int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
+
+// k_mlp.hip: release the learned proposal's network and buffers
+void mlp_free(esm_handle *h);
 template <typename S>
 int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
 

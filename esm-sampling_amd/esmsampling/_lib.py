@@ -18,6 +18,7 @@ FP64, FP32 = 64, 32
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
+_u32p = C.POINTER(C.c_uint32)
 
 
 class EsmModel(C.Structure):
@@ -98,6 +99,13 @@ SIGNATURES = [
     ("esm_copy_replica", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     ("esm_sweep_options", C.c_int32, [C.c_void_p, C.c_double, _dp, C.c_int32, C.c_double]),
     ("esm_storage_long_options", C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_double]),
+    ("esm_mlp_init", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _u32p]),
+    ("esm_mlp_dims", C.c_int32, [C.c_void_p, _ip]),
+    ("esm_mlp_params", C.c_int32, [C.c_void_p, _dp, _dp]),
+    ("esm_mlp_forward", C.c_int32, [C.c_void_p, _dp, C.c_int64, _dp]),
+    ("esm_mlp_gradient", C.c_int32, [C.c_void_p, _dp, _dp, _dp, C.c_int64, _dp, _dp]),
+    ("esm_mlp_train", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, _u32p, C.c_uint32, _dp]),
+    ("esm_mlp_propose", C.c_int32, [C.c_void_p, _u32p, C.c_uint32, C.POINTER(C.c_int64), _dp, _dp, _dp]),
     ("esm_profile", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_profile_read", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
 ]

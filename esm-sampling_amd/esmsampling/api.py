@@ -264,3 +264,78 @@ def tilt_weights(lam: float, phis, active):
     x = np.where(a, lam * np.asarray(phis, dtype=np.float64), -np.inf)
     w = np.exp(x - x.max())
     return w / w.sum()
+
+
+class LearnedProposal:
+    """The learned dispatch-block proposal of a replica set (``esm_mlp_*``): an MLP proposes the
+    gas utilization of one node over a block of ``block_hours`` hours from the block's context
+    (the requirement the unit meets, the node's capacity factors and capacities); every proposal
+    is accepted or rejected by the exact Metropolis–Hastings ratio (FP64), so the target is
+    unchanged and the plain line moves stay the judge. Trained on the sampler's own states by
+    maximum likelihood (Adam). No counterpart in the reference."""
+
+    def __init__(self, R: Replicas, block_hours: int = 24, ncomp: int = 4, key=(7, 7)):
+        self.R = R
+        check(R.lib.esm_mlp_init(R.handle, int(block_hours), int(ncomp), _key(key)), "esm_mlp_init")
+        d = np.zeros(7, dtype=np.int32)
+        check(R.lib.esm_mlp_dims(R.handle, d.ctypes.data_as(C.POINTER(C.c_int32))), "esm_mlp_dims")
+        self.B, self.K, self.din, self.dh, self.dout, self.nb, self.nparam = (int(q) for q in d)
+        self.n_inst = R.sys.N * self.nb * R.R
+
+    def params(self, set_to=None):
+        """The FP32 master parameters (per layer: Wt [out][in], then the bias); with ``set_to``, set them first."""
+        out = np.zeros(self.nparam)
+        src = None if set_to is None else f64(set_to)
+        if src is not None and src.shape != (self.nparam,):
+            raise ValueError(f"{self.nparam} parameters expected")
+        check(self.R.lib.esm_mlp_params(self.R.handle, dptr(out), dptr(src) if src is not None else None), "esm_mlp_params")
+        return out
+
+    def layers(self, p=None):
+        """[(Wt [out][in], b [out])] of the four layers, from ``p`` (or the current parameters)."""
+        p = self.params() if p is None else p
+        dims = [(self.din, self.dh), (self.dh, self.dh), (self.dh, self.dh), (self.dh, self.dout)]
+        out, o = [], 0
+        for fi, fo in dims:
+            W = p[o:o + fi * fo].reshape(fo, fi)
+            o += fi * fo
+            b = p[o:o + fo]
+            o += fo
+            out.append((W, b))
+        return out
+
+    def forward(self, x):
+        x = f64(x)
+        out = np.zeros((x.shape[0], self.dout))
+        check(self.R.lib.esm_mlp_forward(self.R.handle, dptr(x), x.shape[0], dptr(out)), "esm_mlp_forward")
+        return out
+
+    def gradient(self, x, y, mu0):
+        x, y, mu0 = f64(x), f64(y), f64(mu0)
+        g = np.zeros(self.nparam)
+        loss = C.c_double()
+        check(self.R.lib.esm_mlp_gradient(self.R.handle, dptr(x), dptr(y), dptr(mu0), x.shape[0], dptr(g),
+                                          C.byref(loss)), "esm_mlp_gradient")
+        return float(loss.value), g
+
+    def train(self, steps: int, batch: int = 16384, lr: float = 1e-3, key=(9, 9), counter: int = 0):
+        """``steps`` Adam steps on minibatches of the current states; the mean loss of every step."""
+        loss = np.zeros(max(int(steps), 1))
+        check(self.R.lib.esm_mlp_train(self.R.handle, int(steps), int(batch), float(lr), _key(key),
+                                       int(counter) & 0xFFFFFFFF, dptr(loss)), "esm_mlp_train")
+        return loss[:int(steps)]
+
+    def propose(self, key=(11, 11), counter: int = 0, trace: bool = False):
+        """One learned move of every (node, block, replica); returns the statistics [5] (attempted,
+        accepted, 0, 0, network evaluations) and, with ``trace``, the network outputs, proposals and
+        MH log-ratios per instance ((node·nb + block)·R + replica)."""
+        st = np.zeros(5, dtype=np.int64)
+        out = yp = lr = None
+        if trace:
+            out = np.zeros((self.n_inst, self.dout))
+            yp = np.zeros((self.n_inst, self.B))
+            lr = np.zeros(self.n_inst)
+        check(self.R.lib.esm_mlp_propose(self.R.handle, _key(key), int(counter) & 0xFFFFFFFF,
+                                         st.ctypes.data_as(C.POINTER(C.c_int64)), dptr(out) if trace else None,
+                                         dptr(yp) if trace else None, dptr(lr) if trace else None), "esm_mlp_propose")
+        return (st, out, yp, lr) if trace else st

@@ -256,6 +256,49 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
 int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double hmin);
 
 /*
+ * The learned dispatch-block proposal (no counterpart in the reference; the extension that the
+ * plain line-conditional moves judge): a move replaces the gas utilization of node n over the B
+ * hours of block d (steps dB … dB+B−1, T/B blocks per node) of one replica by a draw from
+ * q(·|c), a product over the hours of mixtures of K Gaussians given by an MLP of the block's
+ * context c (the
+ * requirement r + κ cl(y) the unit meets, per hour over κ; the node's three capacity factors per
+ * hour; its six capacities over the system means; a constant — nothing that depends on the
+ * block's y), accepted with probability min(1, exp(−β_v ΔC(y') + log q(y|c) − log q(y'|c)))
+ * (ΔC: the block's gas terms and node balances, exact; FP64 ratio), so the target is unchanged.
+ * Network: 128 inputs, three hidden layers of 256 (ReLU), dout = 3KB outputs rounded up to a
+ * multiple of 64 ([logits K·B][means K·B around cl(Q/κ)][log sd K·B, clamped to [−7, 2]],
+ * index k·B + b; the weights of hour b are the softmax over k), BF16 weights and activations
+ * with FP32 accumulation (MFMA), FP32 master weights, Adam.
+ *
+ * esm_mlp_init: a new network for blocks of B hours (1 … min(24, T)) with K components (1 … 8),
+ * weights from Philox under `key`. esm_mlp_dims: [B, K, din, dh, dout, blocks per node, number
+ * of parameters]. esm_mlp_params: get and/or set the FP32 master parameters (per layer
+ * l = 1 … 4: Wt [out][in] then the bias [out]); either pointer may be NULL.
+ * esm_mlp_forward: the outputs [M][dout] of M ≤ 65536 explicit input rows x [M][128] (rounded
+ * to BF16). esm_mlp_gradient: for explicit rows x, the blocks' y [M][B] and the centres mu0
+ * [M][B], the loss Σ −log q(y|c) and the gradient of its mean over the rows (parameter layout).
+ * esm_mlp_train: `steps` Adam steps (β = 0.9, 0.999) with learning rate lr on minibatches of
+ * `batch` ≤ 65536 instances drawn from the current state of every replica (instance i of step s:
+ * Philox (i, s, 0x5c000000, counter)); loss [steps] (may be NULL): each step's mean loss.
+ * esm_mlp_propose: one learned move of every (node, block, replica), uniforms Philox (replica,
+ * node·nb + block, ·, counter): 0x5a000000 + b/2 for the normals of hours b, b+1, 0x5e000000 +
+ * b/2 for their components, 0x5b000000 (words 2–3) for the acceptance; stats [ESM_NSTAT]
+ * (attempted, accepted, 0, 0, network evaluations); out [n_inst][dout], yprop [n_inst][B],
+ * logratio [n_inst] (may be NULL) receive the network outputs, proposals and MH log-ratios,
+ * instance i = (node·nb + block)·R + replica.
+ */
+int32_t esm_mlp_init(esm_handle *h, int32_t block_hours, int32_t ncomp, const uint32_t key[2]);
+int32_t esm_mlp_dims(esm_handle *h, int32_t *dims);
+int32_t esm_mlp_params(esm_handle *h, double *get, const double *set);
+int32_t esm_mlp_forward(esm_handle *h, const double *x, int64_t M, double *out);
+int32_t esm_mlp_gradient(esm_handle *h, const double *x, const double *y, const double *mu0, int64_t M, double *grad,
+                         double *loss);
+int32_t esm_mlp_train(esm_handle *h, int32_t steps, int32_t batch, double lr, const uint32_t key[2], uint32_t counter,
+                      double *loss);
+int32_t esm_mlp_propose(esm_handle *h, const uint32_t key[2], uint32_t counter, int64_t *stats, double *out,
+                        double *yprop, double *logratio);
+
+/*
  * `follow_capacities!` (src/replicas.jl:448-472) of every replica: move the shared capacities
  * from the current ones to knew [ESM_NK][N] with the dispatch following: gas at fixed output
  * (utilizations × c(k0)/c(k1), :412-435), then every renewable and storage unit with
