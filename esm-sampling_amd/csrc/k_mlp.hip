@@ -15,7 +15,7 @@
 // min(1, exp(−β_v ΔC(y') + log q(y|c) − log q(y'|c))) (FP64). Instances of different (node,
 // block, replica) touch disjoint values, so all of them run in one launch.
 //
-// Network: din = 128 inputs (103 used), three hidden layers of 256 with ReLU, dout = 3KB outputs
+// Network: din = 160 inputs (6B + 7 used), three hidden layers of 256 with ReLU, dout = 3KB outputs
 // rounded up to a multiple of 64, BF16 weights and activations, FP32 accumulation, FP32 master
 // weights and Adam moments. The GEMMs are one LDS-tiled kernel on v_mfma_f32_32x32x16_bf16 (64 × 64 tile per
 // workgroup, four waves of 32 × 32), with transposed operands for the backward pass.
@@ -154,7 +154,8 @@ __device__ __forceinline__ double group_sum(double v) {
 // features of instances idx[t] (or m0 + t without idx), t < cnt, into X [cnt][din] (BF16), the
 // block's current y into Y [cnt][B] and the centres cl(Q/κ) into MU0 [cnt][B]: features
 // [Q/κ per hour (B)][capacity factors per carrier and hour (3B)][capacities over the system
-// means (6)][1][0 …]
+// means (6)][1][the node's net line inflow over κ per hour (B)][the largest loading |F|/s of its
+// lines per hour (B)][0 …]
 template <typename S>
 __global__ void __launch_bounds__(256) k_mlp_features(DevModel<S> m, SweepState<S> st, const double *kraw, MlpDims D,
                                                       long long m0, const long long *idx, int cnt, uint16_t *X, float *Y,
@@ -186,6 +187,19 @@ __global__ void __launch_bounds__(256) k_mlp_features(DevModel<S> m, SweepState<
             val = mean > 0 ? (float)(kraw[c * m.N + n] / mean) : 0.0f;
         } else if (f == 4 * B + ESM_NK) {
             val = 1.0f;
+        } else if (f < 6 * B + ESM_NK + 1) {
+            const int b = (f - 4 * B - ESM_NK - 1) % B;
+            const bool inflow = f < 5 * B + ESM_NK + 1;
+            const size_t o = (size_t)(d * B + b) * m.R + j;
+            S acc = S(0);
+            for (int e = m.adj_ptr[n]; e < m.adj_ptr[n + 1]; e++) {
+                const int l = m.adj_line[e];
+                const S F = st.F[(size_t)l * TR + o];
+                if (inflow) acc += m.bus1[l] == n ? F : -F;      // F flows from bus0 to bus1
+                else acc = fmax(acc, fabs(F) / m.ls[l]);
+            }
+            const S kap = m.kc[ESM_KGAS * m.N + n];
+            val = inflow ? (kap > S(0) ? (float)(acc / kap) : 0.0f) : (float)acc;
         }
         x[f] = f2bf(val);
     }
@@ -485,7 +499,7 @@ int32_t esm_mlp_init(esm_handle *h, int32_t block_hours, int32_t ncomp, const ui
         MlpDims &D = s.D;
         D.B = block_hours;
         D.K = ncomp;
-        D.din = 128;
+        D.din = 160;
         D.dh = 256;
         D.dout = (3 * D.K * D.B + 63) / 64 * 64;
         D.nb = h->T / D.B;

@@ -13,7 +13,8 @@ this path implements the feature; an unknown key raises, as ``load_config`` does
 (src/run.jl:135-150). The system is the seeded synthetic one (``synth.make_system``), set by
 the keys ``synthetic_nodes``, ``synthetic_lines``, ``snapshots``, ``resolution_hours``,
 ``system_seed`` and ``precision``, which have no counterpart in the reference (it reads its
-data file). Keys of features not built here are accepted only at their "off" value (and default
+data file), nor do the ``learned_*`` keys of the learned dispatch-block proposal (off by
+default). Keys of features not built here are accepted only at their "off" value (and default
 to it): ``replica_ffbs = false``, ``tilt_lambda = 0``, ``storage_xpair = 0``, ``storage_xlong = 0``
 (the cross-node storage pairs of storage_cross.jl).
 
@@ -97,6 +98,21 @@ class Config:
     storage_xpair: float = 0.0
     storage_xlong: int = 0
     maxnats: float = 40.0
+    # the learned dispatch-block proposal (no counterpart in the reference): after every
+    # operation sweep, `learned_moves` learned moves of every (node, block, replica), exactly MH
+    # corrected; the network is trained by maximum likelihood on the chain's own states,
+    # `learned_train_steps` Adam steps before the first replica step and `learned_train_more`
+    # every `learned_train_every` warmup steps (never in the production window, whose kernel
+    # stays fixed)
+    learned_proposal: bool = False
+    learned_block_hours: int = 24
+    learned_components: int = 8
+    learned_moves: int = 1
+    learned_train_steps: int = 1000
+    learned_train_every: int = 0
+    learned_train_more: int = 200
+    learned_batch: int = 4096
+    learned_lr: float = 2e-4
     # the synthetic system (no counterpart in the reference, which reads its data file)
     synthetic_nodes: int = 40
     synthetic_lines: int = 70
@@ -116,6 +132,13 @@ class Config:
             raise ValueError("storage_xpair / storage_xlong > 0: the cross-node storage pairs (sxpair, sxlong) are not built")
         if not (0.0 <= self.storage_long <= 1.0) or not (0 <= self.storage_long_moves <= 256) or self.storage_long_hours < 0:
             raise ValueError("storage_long in [0, 1], storage_long_moves in [0, 256], storage_long_hours >= 0")
+        if self.learned_proposal and not (1 <= self.learned_block_hours <= min(24, self.snapshots)
+                                          and 1 <= self.learned_components <= 8 and self.learned_moves >= 0
+                                          and self.learned_train_steps >= 0 and self.learned_train_every >= 0
+                                          and self.learned_train_more >= 0 and 1 <= self.learned_batch <= 65536
+                                          and self.learned_lr > 0):
+            raise ValueError("learned_*: block hours in [1, min(24, snapshots)], components in [1, 8], batch in "
+                             "[1, 65536], lr > 0, counts >= 0")
         if self.precision not in ("fp32", "fp64"):
             raise ValueError('precision must be "fp32" or "fp64"')
         if self.replicas < 1 or self.nchains < 1:
@@ -276,22 +299,63 @@ class Keys:
         self.sweep = (seed, base + 1)
         self.weather = (seed, base + 2)
         self.init = (seed, base + 3)
+        self.learned = (seed, base + 4)
+        self.train = (seed, base + 5)
+        self.net = (seed, base + 6)
         self.probe = (seed, base + 500)
 
 
-def _sweep_scores(R: ReplicaSet, nsweeps: int, key, counter: int, maxnats: float, r: float):
-    """``nsweeps`` operation sweeps, after each the energy (caches rebuilt) and the capacity
-    scores; returns (mean of r·ŝ over the sweeps [R, 6, N], fk, parts of the last sweep, stats)."""
+class Learned:
+    """The learned proposal of one chain's replica set: the network, its schedule and counts."""
+
+    def __init__(self, dev: api.Replicas, cfg: Config, keys: Keys):
+        self.net = api.LearnedProposal(dev, block_hours=cfg.learned_block_hours, ncomp=cfg.learned_components,
+                                       key=keys.net)
+        self.cfg, self.keys = cfg, keys
+        self.moves = cfg.learned_moves
+        self.att = self.acc = 0
+        self.ntrain = 0
+        self.loss = []
+        self.wall = 0.0          # seconds spent training
+
+    def train(self, steps: int):
+        if steps > 0:
+            t0 = time.perf_counter()
+            self.loss.extend(self.net.train(steps, batch=self.cfg.learned_batch, lr=self.cfg.learned_lr,
+                                            key=self.keys.train, counter=self.ntrain))
+            self.ntrain += 1
+            self.wall += time.perf_counter() - t0
+
+    def sweep(self, counter: int):
+        """``moves`` learned moves of every (node, block, replica); returns (attempted, accepted)."""
+        att = acc = 0
+        for q in range(self.moves):
+            st = self.net.propose(key=self.keys.learned, counter=counter * max(self.moves, 1) + q)
+            att += int(st[0])
+            acc += int(st[1])
+        self.att += att
+        self.acc += acc
+        return att, acc
+
+
+def _sweep_scores(R: ReplicaSet, nsweeps: int, key, counter: int, maxnats: float, r: float, learned=None):
+    """``nsweeps`` operation sweeps (each followed by the learned moves, when there are), after
+    each the energy (caches rebuilt) and the capacity scores; returns (mean of r·ŝ over the sweeps
+    [R, 6, N], fk, parts of the last sweep, stats, learned (attempted, accepted))."""
     acc, fk, parts, stats = None, None, None, np.zeros((len(api.FAMILY_NAMES), 5), dtype=np.int64)
+    lst = np.zeros(2, dtype=np.int64)
     for q in range(nsweeps):
         stats += api.operation_sweep(R.dev, 1, key=key, counter=counter + q, maxnats=maxnats)
+        if learned is not None:
+            lst += learned.sweep(counter + q)
         parts = api.energy_cost(R.dev)
         sk, fk = api.capacity_scores(R.dev)
         acc = r * sk / nsweeps if acc is None else acc + r * sk / nsweeps
-    return acc, fk, parts, stats
+    return acc, fk, parts, stats, lst
 
 
-def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, maxnats: float = 40.0):
+def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, maxnats: float = 40.0,
+                 learned: Learned | None = None):
     """``replica_step!`` (src/replicas.jl:676-705) of one chain: the weather step of every replica
     (scheduled angle, regime window with a burst, compensated re-dispatch), ``inner`` operation
     sweeps with the mean capacity score, the gates, the drift weights, one Langevin step of the
@@ -321,7 +385,7 @@ def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, 
             R.since[j] += 1
     Kraw = R.k.copy()
     mask = interior_nodes(Kraw, sys.khi)
-    rsk, fk, parts, stats = _sweep_scores(R, ctl.inner, keys.sweep, it * ctl.inner, maxnats, r)
+    rsk, fk, parts, stats, lst = _sweep_scores(R, ctl.inner, keys.sweep, it * ctl.inner, maxnats, r, learned)
     fkm = masked_means(fk, mask)
     for j in range(M):
         update_gate(R, j, masked_means(rsk[j], mask) / fkm, ctl)
@@ -337,7 +401,7 @@ def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, 
     tw = time.perf_counter() - t0
     rs = masked_means(np.tensordot(wts, rsk, axes=1), mask)
     return dict(it=it, parts=parts, wts=wts, moved=moved, nclip=nclip, Kraw=Kraw, knew=knew, rs=rs, fk=fkm, tw=tw,
-                stats=stats)
+                stats=stats, learned=lst)
 
 
 def probe_candidate(R: ReplicaSet, ctl: ReplicaCtl):
@@ -443,6 +507,12 @@ def run_replicas(cfg: Config, log=None):
         sets.append(init_replicas(sys, chain_start_capacities(sys, kinds[c]), M, prec, ctl, keys[c],
                                   cfg.replica_init_sweeps, cfg.maxnats))
         say(f"chain {c} ({kinds[c]}): {M} replicas initialised ({cfg.replica_init_sweeps} sweeps each)")
+    learned = [None] * nc
+    if cfg.learned_proposal:
+        for c in range(nc):
+            learned[c] = Learned(sets[c].dev, cfg, keys[c])
+            learned[c].train(cfg.learned_train_steps)
+        say(f"learned proposal: {cfg.learned_train_steps} training steps per chain")
     probe = api.Replicas(sys, 1, prec) if ctl.probe_every > 0 else None
     if probe is not None:
         ctl.sweep_options(probe)
@@ -457,11 +527,17 @@ def run_replicas(cfg: Config, log=None):
             t_prod0 = time.perf_counter()
         for c in range(nc):
             R = sets[c]
-            st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats)
+            if (learned[c] is not None and cfg.learned_train_every > 0 and it <= nwarm
+                    and it % cfg.learned_train_every == 0):
+                learned[c].train(cfg.learned_train_more)
+            st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats, learned[c])
             traces[c].append(replica_monitor(R, st["parts"]))
             records[c].append(dict(it=it, k=R.k.copy(), active=R.active.copy(), weights=st["wts"], nclip=st["nclip"],
                                    moved=st["moved"], rs=st["rs"], fk=st["fk"], eta_w=R.eta_w.copy(),
-                                   op=st["parts"][:, 1].copy(), shed=st["parts"][:, 4].copy()))
+                                   op=st["parts"][:, 1].copy(), shed=st["parts"][:, 4].copy(),
+                                   learned=st["learned"].copy()))
+            if it % 25 == 0 or it == nwarm + nprod:
+                say(f"chain {c} step {it}/{nwarm + nprod}: total {traces[c][-1][MONITOR_NAMES.index('total')]:.6g}")
             if ctl.probe_every > 0 and it % ctl.probe_every == 0:
                 j = probe_candidate(R, ctl)
                 if j is not None:
@@ -495,9 +571,20 @@ def run_replicas(cfg: Config, log=None):
         R.dev.close()
     if probe is not None:
         probe.close()
+    lsum = None
+    if cfg.learned_proposal:
+        lsum = dict(attempted=sum(q.att for q in learned), accepted=sum(q.acc for q in learned),
+                    train_steps=sum(len(q.loss) for q in learned),
+                    loss_first=[q.loss[0] if q.loss else None for q in learned],
+                    loss_last=[q.loss[-1] if q.loss else None for q in learned],
+                    train_s=sum(q.wall for q in learned))
     return dict(names=MONITOR_NAMES, traces=traces, records=records, probes=probes, summary=summ, window_start=w0,
+                learned=lsum,
                 kinds=kinds, wall=dict(setup=t_init - t_setup, steps=t_end - t_init, production=prod_s),
                 steps_per_s=(nwarm + nprod) / max(t_end - t_init, 1e-12),
                 ess_total=None if tot is None else tot["ess_bulk"],
                 ess_per_s=None if tot is None or prod_s <= 0 else tot["ess_bulk"] / prod_s,
+                # with the learned proposal, the training time counts as production time too
+                ess_per_s_with_training=None if tot is None or prod_s <= 0 else
+                tot["ess_bulk"] / (prod_s + (lsum["train_s"] if lsum else 0.0)),
                 passed=all(s["passed"] for s in summ.values()) if summ else False)

@@ -262,10 +262,11 @@ int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double
  * q(·|c), a product over the hours of mixtures of K Gaussians given by an MLP of the block's
  * context c (the
  * requirement r + κ cl(y) the unit meets, per hour over κ; the node's three capacity factors per
- * hour; its six capacities over the system means; a constant — nothing that depends on the
+ * hour; its six capacities over the system means; a constant; per hour the node's net line
+ * inflow over κ and the largest loading |F|/s of its lines — nothing that depends on the
  * block's y), accepted with probability min(1, exp(−β_v ΔC(y') + log q(y|c) − log q(y'|c)))
  * (ΔC: the block's gas terms and node balances, exact; FP64 ratio), so the target is unchanged.
- * Network: 128 inputs, three hidden layers of 256 (ReLU), dout = 3KB outputs rounded up to a
+ * Network: 160 inputs (6B + 7 used), three hidden layers of 256 (ReLU), dout = 3KB outputs rounded up to a
  * multiple of 64 ([logits K·B][means K·B around cl(Q/κ)][log sd K·B, clamped to [−7, 2]],
  * index k·B + b; the weights of hour b are the softmax over k), BF16 weights and activations
  * with FP32 accumulation (MFMA), FP32 master weights, Adam.
@@ -274,7 +275,7 @@ int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double
  * weights from Philox under `key`. esm_mlp_dims: [B, K, din, dh, dout, blocks per node, number
  * of parameters]. esm_mlp_params: get and/or set the FP32 master parameters (per layer
  * l = 1 … 4: Wt [out][in] then the bias [out]); either pointer may be NULL.
- * esm_mlp_forward: the outputs [M][dout] of M ≤ 65536 explicit input rows x [M][128] (rounded
+ * esm_mlp_forward: the outputs [M][dout] of M ≤ 65536 explicit input rows x [M][160] (rounded
  * to BF16). esm_mlp_gradient: for explicit rows x, the blocks' y [M][B] and the centres mu0
  * [M][B], the loss Σ −log q(y|c) and the gradient of its mean over the rows (parameter layout).
  * esm_mlp_train: `steps` Adam steps (β = 0.9, 0.999) with learning rate lr on minibatches of

@@ -115,30 +115,39 @@ def loss_fp64(layers, X, y, mu0, K, B):
     return -logq(h, mu0, y, K, B)[0].mean()
 
 
-def features(sys, v, k, x, r, B, R):
-    """X [n_inst, 128] (BF16 values), Y and MU0 [n_inst, B] of every instance
+def features(sys, v, k, x, r, F, B, R, din=160):
+    """X [n_inst, din] (BF16 values), Y and MU0 [n_inst, B] of every instance
     ((node·nb + block)·R + replica) of the state (v [4][N][T][R], raw capacities k [6][N], raw
-    capacity factors x [3][N][T][R], residual r [N][T][R])."""
+    capacity factors x [3][N][T][R], residual r [N][T][R], flows F [L][T][R], positive from bus0
+    to bus1). Features: [Q/κ per hour][capacity factors per carrier and hour][capacities over the
+    system means][1][net line inflow over κ per hour][largest loading |F|/s of the node's lines
+    per hour][0 …]."""
     N, T = sys.N, sys.T
     nb = T // B
     kc = np.clip(k, sys.klo, sys.khi)
     kap = kc[3]
-    rows = []
-    Ys, M0 = [], []
     kmean = k.mean(axis=1)
+    rows, Ys, M0 = [], [], []
     for n in range(N):
+        out_l = np.flatnonzero(sys.line_bus0 == n)
+        in_l = np.flatnonzero(sys.line_bus1 == n)
+        inc = np.concatenate([out_l, in_l])
         for d in range(nb):
+            tau = np.arange(d * B, d * B + B)
             for j in range(R):
-                tau = np.arange(d * B, d * B + B)
                 y = v[1, n, tau, j]
                 Q = r[n, tau, j] + kap[n] * np.clip(y, 0, 1)
                 q = f32(Q / kap[n]) if kap[n] > 0 else np.zeros(B)
-                feat = np.zeros(128)
+                feat = np.zeros(din)
                 feat[:B] = q
                 for c in range(3):
                     feat[B + c * B:B + (c + 1) * B] = x[c, n, tau, j]
                 feat[4 * B:4 * B + 6] = np.where(kmean > 0, k[:, n] / np.where(kmean > 0, kmean, 1.0), 0.0)
                 feat[4 * B + 6] = 1.0
+                inflow = F[in_l][:, tau, j].sum(axis=0) - F[out_l][:, tau, j].sum(axis=0)
+                feat[4 * B + 7:5 * B + 7] = inflow / kap[n] if kap[n] > 0 else 0.0
+                load = np.abs(F[inc][:, tau, j]) / sys.line_s[inc][:, None]
+                feat[5 * B + 7:6 * B + 7] = load.max(axis=0) if len(inc) else 0.0
                 rows.append(bf16(feat))
                 Ys.append(y)
                 M0.append(np.clip(q, 0.0, 1.0))

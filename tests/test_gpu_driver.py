@@ -68,3 +68,26 @@ def test_run_replicas_short():
     for rec in res["records"][0]:
         w = rec["weights"]
         assert w.sum() == pytest.approx(1.0) and np.all(w[~rec["active"]] == 0) or not rec["active"].any()
+
+
+def test_run_replicas_learned_proposal():
+    """The driver with the learned proposal: trained before the first step and during the warmup
+    (never in the production window), one learned sweep after every operation sweep, its counts
+    in the records, and the report with the training time counted."""
+    print("run_replicas learned 2 chains x 8 replicas", flush=True)
+    cfg = RP.load_config(dict(nchains=2, replicas=8, mlda_warmup=10, nsamples=10, replica_init_sweeps=4,
+                              replica_probe_every=0, synthetic_nodes=10, synthetic_lines=16, snapshots=96,
+                              resolution_hours=1.0, precision="fp32", seed=5, learned_proposal=True,
+                              learned_train_steps=300, learned_train_every=5, learned_train_more=50,
+                              learned_components=4))
+    res = RP.run_replicas(cfg)
+    ls = res["learned"]
+    per_step = 10 * (96 // 24) * 8
+    assert ls["attempted"] == 2 * 20 * per_step
+    assert 0 < ls["accepted"] < ls["attempted"]
+    assert ls["train_steps"] == 2 * (300 + 2 * 50)          # warmup steps 5 and 10
+    assert all(b < a for a, b in zip(ls["loss_first"], ls["loss_last"]))
+    for rec in res["records"][0]:
+        assert rec["learned"][0] == per_step
+    assert np.all(np.isfinite(np.array(res["traces"])))
+    assert res["ess_per_s_with_training"] < res["ess_per_s"]

@@ -89,7 +89,7 @@ def test_learned_move_ratio_and_decisions_fp64(B):
     st, out, yp, lr = net.propose(key=key, counter=ctr, trace=True)
     v1, _, _ = dev.download()
     K, nb = net.K, net.nb
-    X, Y, MU0 = mo.features(s, v0, k0, x0, r0, B, R)
+    X, Y, MU0 = mo.features(s, v0, k0, x0, r0, F0, B, R)
     ref, _ = mo.forward(layers, X)
     assert np.max(np.abs(out - ref)) <= 2e-2 * np.abs(ref).max()
     lq_old = mo.logq(out, mo.f32(MU0), Y, K, B)[0]      # the state's own y

@@ -104,6 +104,12 @@ def test_config_keys():
         RP.load_config({"storage_xlong": 8})
     with pytest.raises(ValueError, match="storage_long"):
         RP.load_config({"storage_long_moves": 300})
+    c = RP.load_config({"learned_proposal": True, "learned_block_hours": 12, "learned_components": 4})
+    assert c.learned_block_hours == 12 and c.learned_components == 4 and c.learned_moves == 1
+    assert RP.load_config({}).learned_proposal is False
+    for bad in ({"learned_block_hours": 25}, {"learned_components": 9}, {"learned_lr": 0.0}, {"learned_batch": 70000}):
+        with pytest.raises(ValueError, match="learned_"):
+            RP.load_config({"learned_proposal": True, **bad})
 
 
 def test_config_toml(tmp_path):
