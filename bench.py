@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--lines", type=int, default=70)
     ap.add_argument("--fp64", action="store_true", help="FP64 device state")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
     ap.add_argument("--p-storage-ms", type=float, default=0.5,
                     help="probability of the storage multiscale sweep per operation sweep (the reference's 1/2)")
@@ -221,7 +222,67 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_threads)
     wl.dev.close()
+    if not args.no_learned:
+        out["learned"] = learned_leg(args, prec)
     print(json.dumps(out), flush=True)
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
+
+
+def learned_leg(args, prec, R=128, K=8, B=24, train_steps=300, sweeps=3):
+    """The learned dispatch-block proposal on the 40-node / 8760 h / 128-replica workload: states
+    of the plain sampler (two operation sweeps), maximum-likelihood training (Adam, batch 4096),
+    then learned sweeps (every (node, 24 h block, replica) once: features, the MLP's forward pass
+    on MFMA, draw, exact MH ratio, commit) beside one plain operation sweep of the same replicas.
+    The forward pass's BF16 MFMA rate is its algorithmic flops (the network's shape) over its
+    HIP-event time."""
+    import numpy as np
+    import torch
+    from esmsampling import _lib, api, synth
+
+    s = synth.make_system(N=args.nodes, L=args.lines, T=args.hours, dt=1.0, seed=1)
+    dev = api.Replicas(s, R, prec)
+    api.sample_weather(dev, key=(5, 1))
+    dev.upload(v=synth.heuristic_state(s, dev.download()[2]), k=s.k0)
+    api.operation_sweep(dev, 1, key=(5, 2), counter=0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    api.operation_sweep(dev, 1, key=(5, 2), counter=1)
+    plain_ms = 1e3 * (time.perf_counter() - t0)
+    net = api.LearnedProposal(dev, block_hours=B, ncomp=K, key=(5, 3))
+    lib, h = dev.lib, dev.handle
+    lib.esm_profile(h, 1)
+    t0 = time.perf_counter()
+    loss = net.train(train_steps, batch=4096, lr=2e-4, key=(5, 4))
+    train_ms = 1e3 * (time.perf_counter() - t0) / train_steps
+    net.propose(key=(5, 5), counter=0)             # warm
+    att = acc = 0
+    t0 = time.perf_counter()
+    for c in range(sweeps):
+        st = net.propose(key=(5, 5), counter=1 + c)
+        att += int(st[0])
+        acc += int(st[1])
+    sweep_ms = 1e3 * (time.perf_counter() - t0) / sweeps
+    prof = {}
+    for kid, name in ((_lib.KERN_MLPF, "forward"), (_lib.KERN_MLPP, "propose"), (_lib.KERN_MLPT, "train_step")):
+        tot, cnt = C.c_double(), C.c_int64()
+        lib.esm_profile_read(h, kid, C.byref(tot), C.byref(cnt))
+        prof[name] = (tot.value, cnt.value)
+    lib.esm_profile(h, 0)
+    dev.close()
+    rows = net.n_inst * (sweeps + 1)                # forward rows timed (the warm sweep included)
+    flops_row = 2.0 * (net.din * net.dh + 2 * net.dh * net.dh + net.dh * net.dout)
+    fwd_ms = prof["forward"][0]
+    tflops = flops_row * rows / (fwd_ms * 1e-3) / 1e12
+    return {"workload": f"{s.N} nodes, {s.T} h, {R} replicas, {B} h blocks, {K} components per hour",
+            "blocks_per_sweep": net.n_inst, "learned_sweep_ms": sweep_ms, "plain_operation_sweep_ms": plain_ms,
+            "acceptance": acc / max(att, 1), "train_steps": train_steps, "train_step_ms": train_ms,
+            "train_loss_first_last": [float(loss[0]), float(loss[-1])],
+            "forward_ms_per_sweep": fwd_ms / (sweeps + 1), "propose_kernel_ms_per_sweep": prof["propose"][0] / (sweeps + 1),
+            "mfma": {"kernel": "k_gemm (forward pass, 4 layers)", "bound": "mfma", "achieved": tflops,
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+                     "flops_per_row": flops_row, "rows": rows}}
 
 
 def measured_traffic(kernel, s, T, R, prec):

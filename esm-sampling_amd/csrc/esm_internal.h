@@ -172,8 +172,8 @@ struct ProfScope {
     esm_handle *h;
     int id;
     hipEvent_t b = nullptr;
-    ProfScope(esm_handle *h_, int id_) : h(h_), id(id_) {
-        if (h->prof) {
+    ProfScope(esm_handle *h_, int id_) : h(h_), id(id_) {   // id < 0: not timed
+        if (h->prof && id >= 0) {
             hipEvent_t a;
             if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
                 (void)hipEventRecord(a, h->stream);

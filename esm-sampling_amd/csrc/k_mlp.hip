@@ -113,6 +113,105 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kc, const
     }
 }
 
+// The forward GEMM C[i][j] = Σ_k A[i][k] B[j][k] (+ bias, ReLU to BF16 or FP32) with both operands
+// K-contiguous (activations [M][K], weights Wt [N][K]): 128 × 128 tile per workgroup, four waves
+// of 64 × 64 (2 × 2 MFMA 32 × 32 × 16 tiles each), K in steps of 32 staged in LDS with 16-byte
+// global loads and ds_read_b128 fragment reads, the next K step prefetched into registers while
+// the current one is multiplied. K must be a multiple of 32 and lda, ldb multiples of 8.
+constexpr int GN_M = 128, GN_N = 128, GN_K = 32, GN_LD = GN_K + 8;
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_nt(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
+                                                 const uint16_t *__restrict__ B, int ldb, const float *__restrict__ bias,
+                                                 void *__restrict__ C, int ldc) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[GN_M][GN_LD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[GN_N][GN_LD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i0 = blockIdx.y * GN_M, j0 = blockIdx.x * GN_N;
+    const int wr = wave >> 1, wc = wave & 1;
+    // global → register staging: 2 chunks of 8 values per operand per thread (row = c / 4, k = 8 (c % 4))
+    uint4 ra[2], rb[2];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int c = tid + 256 * e, r = c >> 2, kc = (c & 3) * 8;
+            const int i = i0 + r, j = j0 + r;
+            ra[e] = i < M ? *(const uint4 *)(A + (size_t)i * lda + k0 + kc) : make_uint4(0, 0, 0, 0);
+            rb[e] = j < N ? *(const uint4 *)(B + (size_t)j * ldb + k0 + kc) : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int c = tid + 256 * e, r = c >> 2, kc = (c & 3) * 8;
+            *(uint4 *)&As[r][kc] = ra[e];
+            *(uint4 *)&Bs[r][kc] = rb[e];
+        }
+    };
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x16{0};
+    load(0);
+    store();
+    __syncthreads();
+    for (int k0 = 0; k0 < K; k0 += GN_K) {
+        const bool more = k0 + GN_K < K;
+        if (more) load(k0 + GN_K);
+#pragma unroll
+        for (int kk = 0; kk < GN_K; kk += 16) {
+            const int ko = kk + 8 * (lane >> 5);
+            bf16x8 fa[2], fb[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                fa[q] = *(const bf16x8 *)&As[wr * 64 + q * 32 + (lane & 31)][ko];
+                fb[q] = *(const bf16x8 *)&Bs[wc * 64 + q * 32 + (lane & 31)][ko];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+        if (more) {
+            store();
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int j = j0 + wc * 64 + b * 32 + (lane & 31);
+            if (j >= N) continue;
+            const float bj = bias ? bias[j] : 0.0f;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int i = i0 + wr * 64 + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                if (i >= M) continue;
+                const float x = acc[a][b][v] + bj;
+                if (EPI == EPI_F32) ((float *)C)[(size_t)i * ldc + j] = x;
+                else ((uint16_t *)C)[(size_t)i * ldc + j] = f2bf(fmaxf(x, 0.0f));
+            }
+        }
+}
+
+template <int EPI>
+static int gemm_nt(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                   const float *bias, void *C, int ldc) {
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
+    if (K % GN_K || lda % 8 || ldb % 8 || ((uintptr_t)A & 15) || ((uintptr_t)B & 15)) {
+        set_error("gemm_nt: K must be a multiple of 32, operands 16-byte aligned");
+        return ESM_EINVAL;
+    }
+    const dim3 grid((unsigned)((N + GN_N - 1) / GN_N), (unsigned)((M + GN_M - 1) / GN_M));
+    hipLaunchKernelGGL((k_gemm_nt<EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, C, ldc);
+    ESM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <bool TA, bool TB, int EPI>
 static int gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
                 const float *bias, const uint16_t *H, void *C, int ldc, int ksplit = 1) {
@@ -210,13 +309,15 @@ __global__ void __launch_bounds__(256) k_mlp_features(DevModel<S> m, SweepState<
 // standard deviations (clamped to [−7, 2]); layout [logits K·B][means K·B][log sd K·B], index
 // k·B + b. (With flows and every other unit fixed, the target of the block is a product over its
 // hours, so the family is one of products too.)
-__device__ __forceinline__ float mix_logw(const float *o, int K, int B, int k, int b) {
+// log Σ_k exp(logit_kb): the normalizer of hour b's weights
+__device__ __forceinline__ float mix_lse(const float *o, int K, int B, int b) {
     float mx = -INFINITY;
     for (int q = 0; q < K; q++) mx = fmaxf(mx, o[q * B + b]);
     float s = 0.0f;
     for (int q = 0; q < K; q++) s += expf(o[q * B + b] - mx);
-    return o[k * B + b] - mx - logf(s);
+    return mx + logf(s);
 }
+__device__ __forceinline__ float mix_logw(const float *o, int B, int k, int b, float lse) { return o[k * B + b] - lse; }
 __device__ __forceinline__ float mix_mu(const float *o, const float *mu0, int K, int B, int k, int b) {
     return mu0[b] + o[K * B + k * B + b];
 }
@@ -225,11 +326,12 @@ __device__ __forceinline__ float mix_ls(const float *o, int K, int B, int k, int
 }
 
 // log q_b(y_b) of hour b in FP64 and the responsibilities of its components
-__device__ double mix_logq_hour(const float *o, const float *mu0, double y, int K, int B, int b, double *resp) {
+__device__ double mix_logq_hour(const float *o, const float *mu0, double y, int K, int B, int b, float lse,
+                                double *resp) {
     double lk[8], mx = -INFINITY;
     for (int k = 0; k < K; k++) {
         const double ls = mix_ls(o, K, B, k, b), z = (y - (double)mix_mu(o, mu0, K, B, k, b)) * exp(-ls);
-        lk[k] = (double)mix_logw(o, K, B, k, b) - 0.5 * z * z - ls - 0.91893853320467274178;
+        lk[k] = (double)mix_logw(o, B, k, b, lse) - 0.5 * z * z - ls - 0.91893853320467274178;
         mx = fmax(mx, lk[k]);
     }
     double tot = 0.0;
@@ -252,11 +354,12 @@ __global__ void __launch_bounds__(256) k_mlp_head_grad(const float *OUT, const f
         if (b < B) {
             double resp[8];
             const double y = (double)Y[(size_t)t * B + b];
-            l = -mix_logq_hour(o, mu0, y, K, B, b, resp);
+            const float lse = mix_lse(o, K, B, b);
+            l = -mix_logq_hour(o, mu0, y, K, B, b, lse, resp);
             for (int k = 0; k < K; k++) {
                 const int q = k * B + b;
                 // d(−log q_b)/d logit = softmax − resp
-                g[q] = f2bf(scale * (float)(exp((double)mix_logw(o, K, B, k, b)) - resp[k]));
+                g[q] = f2bf(scale * (float)(exp((double)mix_logw(o, B, k, b, lse)) - resp[k]));
                 const float ls = mix_ls(o, K, B, k, b), raw = o[2 * K * B + q];
                 const double sd = exp((double)ls), z = (y - (double)mix_mu(o, mu0, K, B, k, b)) / sd;
                 g[K * B + q] = f2bf(scale * (float)(-resp[k] * z / sd));                     // d/dμ
@@ -306,10 +409,11 @@ __global__ void __launch_bounds__(256) k_mlp_propose(DevModel<S> m, SweepState<S
         const double rad = sqrt(-2.0 * log(u1));
         const double z = (b & 1) ? rad * sin(6.283185307179586477 * u2) : rad * cos(6.283185307179586477 * u2);
         const double uc = (b & 1) ? u01(wc[2], wc[3]) : u01(wc[0], wc[1]);
+        const float lse = mix_lse(o, K, B, b);
         int kc = K - 1;
         double acc = 0.0;
         for (int k = 0; k < K; k++) {
-            acc += exp((double)mix_logw(o, K, B, k, b));
+            acc += exp((double)mix_logw(o, B, k, b, lse));
             if (uc < acc) {
                 kc = k;
                 break;
@@ -319,8 +423,8 @@ __global__ void __launch_bounds__(256) k_mlp_propose(DevModel<S> m, SweepState<S
         // the reverse density at the state's own y (not its FP32 copy)
         y0 = st.v[plane(m.N, m.T, m.R) + o_];
         y1 = (S)yp;
-        lqo = mix_logq_hour(o, mu0, (double)y0, K, B, b, nullptr);
-        lqn = mix_logq_hour(o, mu0, (double)y1, K, B, b, nullptr);
+        lqo = mix_logq_hour(o, mu0, (double)y0, K, B, b, lse, nullptr);
+        lqn = mix_logq_hour(o, mu0, (double)y1, K, B, b, lse, nullptr);
         // ΔC of the hour: the gas terms and the node's balance
         const S kap = m.kc[ESM_KGAS * m.N + n], mc = m.mc[ESM_KGAS * m.N + n], S0 = st.r[o_];
         S c0, p0, c1, p1;
@@ -431,19 +535,20 @@ static int mlp_refresh_bf16(esm_handle *h) {
 }
 
 // forward of cnt rows of s.X into s.OUT
-static int mlp_forward(esm_handle *h, int cnt) {
+static int mlp_forward(esm_handle *h, int cnt, bool prof = false) {
     MlpState &s = *h->mlp;
+    ProfScope ps(h, prof ? ESM_KERN_MLPF : -1);
     const MlpDims &D = s.D;
     int rc;
     const uint16_t *in = s.X;
     for (int l = 0; l < 3; l++) {
-        if ((rc = gemm<false, true, EPI_RELU>(h->stream, cnt, D.dh, s.lin[l], in, s.lin[l], s.wt16 + s.ow[l], s.lin[l],
-                                               s.p + s.ob[l], nullptr, s.H[l], D.dh)))
+        if ((rc = gemm_nt<EPI_RELU>(h->stream, cnt, D.dh, s.lin[l], in, s.lin[l], s.wt16 + s.ow[l], s.lin[l],
+                                     s.p + s.ob[l], s.H[l], D.dh)))
             return rc;
         in = s.H[l];
     }
-    return gemm<false, true, EPI_F32>(h->stream, cnt, D.dout, D.dh, in, D.dh, s.wt16 + s.ow[3], D.dh, s.p + s.ob[3],
-                                      nullptr, s.OUT, D.dout);
+    return gemm_nt<EPI_F32>(h->stream, cnt, D.dout, D.dh, in, D.dh, s.wt16 + s.ow[3], D.dh, s.p + s.ob[3], s.OUT,
+                            D.dout);
 }
 
 // backward of cnt rows from s.DG[3] = dL/dOUT into s.g (added)
@@ -659,6 +764,7 @@ int32_t esm_mlp_train(esm_handle *h, int32_t steps, int32_t batch, double lr, co
         const int c1 = std::min(steps, c0 + MLP_LOSS_CHUNK);
         ESM_HIP_CHECK(hipMemsetAsync(s.dloss, 0, (size_t)(c1 - c0) * 8, h->stream));
         for (int it = c0; it < c1; it++) {
+            ProfScope pt(h, ESM_KERN_MLPT);
             hipLaunchKernelGGL(k_batch_idx, dim3((unsigned)((batch + 255) / 256)), dim3(256), 0, h->stream, key[0], key[1],
                                counter, (uint32_t)it, D.n_inst, batch, s.idx);
             ESM_HIP_CHECK(hipGetLastError());
@@ -704,8 +810,9 @@ int32_t esm_mlp_propose(esm_handle *h, const uint32_t key[2], uint32_t counter, 
         const int cnt = (int)std::min<long long>(s.cap, D.n_inst - m0);
         rc = h->prec == ESM_FP64 ? mlp_features<double>(h, m0, nullptr, cnt) : mlp_features<float>(h, m0, nullptr, cnt);
         if (rc) return rc;
-        if ((rc = mlp_forward(h, cnt))) return rc;
+        if ((rc = mlp_forward(h, cnt, true))) return rc;
         const dim3 grid((unsigned)((cnt + MLP_GPB - 1) / MLP_GPB)), blk(256);
+        ProfScope pp(h, ESM_KERN_MLPP);
         if (h->prec == ESM_FP64) {
             const DevModel<double> m = dev_model<double>(h);
             SweepState<double> st{(double *)h->d_v, (double *)h->d_r, (double *)h->d_F, (const double *)h->d_x};

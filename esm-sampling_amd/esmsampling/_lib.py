@@ -110,7 +110,8 @@ SIGNATURES = [
     ("esm_profile_read", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
 ]
 (KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW,
- KERN_UNIT, KERN_POT, KERN_CELL, KERN_XLOC, KERN_XLOCC, KERN_XPAIR, KERN_SMS, KERN_SLONG) = range(17)
+ KERN_UNIT, KERN_POT, KERN_CELL, KERN_XLOC, KERN_XLOCC, KERN_XPAIR, KERN_SMS, KERN_SLONG, KERN_MLPF, KERN_MLPP,
+ KERN_MLPT) = range(20)
 NFAM, NSTAT, NWEIGHT = 13, 5, 9
 # the plain random families and their default weights (plain_families, src/local.jl:1912-1915)
 WEIGHT_NAMES = ("unit", "pair", "xpair", "xlocc", "xloc", "potential", "cellpair", "cell", "spair")

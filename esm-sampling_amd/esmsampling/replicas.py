@@ -27,6 +27,7 @@ from __future__ import annotations
 import dataclasses
 import math
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -513,39 +514,48 @@ def run_replicas(cfg: Config, log=None):
             learned[c] = Learned(sets[c].dev, cfg, keys[c])
             learned[c].train(cfg.learned_train_steps)
         say(f"learned proposal: {cfg.learned_train_steps} training steps per chain")
-    probe = api.Replicas(sys, 1, prec) if ctl.probe_every > 0 else None
-    if probe is not None:
-        ctl.sweep_options(probe)
+    # one probe handle per chain (src/replicas.jl:733: a probe slot per chain), so chains share nothing
+    probes_h = [api.Replicas(sys, 1, prec) if ctl.probe_every > 0 else None for _ in range(nc)]
+    for q in probes_h:
+        if q is not None:
+            ctl.sweep_options(q)
     t_init = time.perf_counter()
     traces = [[] for _ in range(nc)]
     records = [[] for _ in range(nc)]
     probes = [[] for _ in range(nc)]
     nprobe = [0] * nc
+
+    def chain_step(c, it):
+        R = sets[c]
+        if (learned[c] is not None and cfg.learned_train_every > 0 and it <= nwarm
+                and it % cfg.learned_train_every == 0):
+            learned[c].train(cfg.learned_train_more)
+        st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats, learned[c])
+        traces[c].append(replica_monitor(R, st["parts"]))
+        records[c].append(dict(it=it, k=R.k.copy(), active=R.active.copy(), weights=st["wts"], nclip=st["nclip"],
+                               moved=st["moved"], rs=st["rs"], fk=st["fk"], eta_w=R.eta_w.copy(),
+                               op=st["parts"][:, 1].copy(), shed=st["parts"][:, 4].copy(),
+                               learned=st["learned"].copy()))
+        if it % 25 == 0 or it == nwarm + nprod:
+            say(f"chain {c} step {it}/{nwarm + nprod}: total {traces[c][-1][MONITOR_NAMES.index('total')]:.6g}")
+        if ctl.probe_every > 0 and it % ctl.probe_every == 0:
+            j = probe_candidate(R, ctl)
+            if j is not None:
+                pr = lag_probe(R, j, probes_h[c], ctl, sys, keys[c].probe, nprobe[c] * ctl.probe_len, cfg.maxnats)
+                nprobe[c] += 1
+                eta0 = R.eta_w.copy()
+                adapted = ctl.adapt and it <= nwarm and adapt_step(R, pr["gap"], ctl)
+                probes[c].append(dict(it=it, j=j, adapted=adapted, eta_before=eta0, eta_after=R.eta_w.copy(), **pr))
+
+    # the chains are independent: one host thread each, so their device work (one HIP stream per
+    # handle) overlaps; every chain's results are those of a sequential run (its own keys)
     t_prod0 = None
-    for it in range(1, nwarm + nprod + 1):
-        if it == nwarm + 1:
-            t_prod0 = time.perf_counter()
-        for c in range(nc):
-            R = sets[c]
-            if (learned[c] is not None and cfg.learned_train_every > 0 and it <= nwarm
-                    and it % cfg.learned_train_every == 0):
-                learned[c].train(cfg.learned_train_more)
-            st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats, learned[c])
-            traces[c].append(replica_monitor(R, st["parts"]))
-            records[c].append(dict(it=it, k=R.k.copy(), active=R.active.copy(), weights=st["wts"], nclip=st["nclip"],
-                                   moved=st["moved"], rs=st["rs"], fk=st["fk"], eta_w=R.eta_w.copy(),
-                                   op=st["parts"][:, 1].copy(), shed=st["parts"][:, 4].copy(),
-                                   learned=st["learned"].copy()))
-            if it % 25 == 0 or it == nwarm + nprod:
-                say(f"chain {c} step {it}/{nwarm + nprod}: total {traces[c][-1][MONITOR_NAMES.index('total')]:.6g}")
-            if ctl.probe_every > 0 and it % ctl.probe_every == 0:
-                j = probe_candidate(R, ctl)
-                if j is not None:
-                    pr = lag_probe(R, j, probe, ctl, sys, keys[c].probe, nprobe[c] * ctl.probe_len, cfg.maxnats)
-                    nprobe[c] += 1
-                    eta0 = R.eta_w.copy()
-                    adapted = ctl.adapt and it <= nwarm and adapt_step(R, pr["gap"], ctl)
-                    probes[c].append(dict(it=it, j=j, adapted=adapted, eta_before=eta0, eta_after=R.eta_w.copy(), **pr))
+    with ThreadPoolExecutor(max_workers=nc) as ex:
+        for it in range(1, nwarm + nprod + 1):
+            if it == nwarm + 1:
+                t_prod0 = time.perf_counter()
+            for f in [ex.submit(chain_step, c, it) for c in range(nc)]:
+                f.result()
     t_end = time.perf_counter()
     if t_prod0 is None:
         t_prod0 = t_end
@@ -569,8 +579,9 @@ def run_replicas(cfg: Config, log=None):
     tot = summ.get("total")
     for R in sets:
         R.dev.close()
-    if probe is not None:
-        probe.close()
+    for q in probes_h:
+        if q is not None:
+            q.close()
     lsum = None
     if cfg.learned_proposal:
         lsum = dict(attempted=sum(q.att for q in learned), accepted=sum(q.acc for q in learned),

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 3
+#define ESM_ABI_VERSION 4
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -377,7 +377,10 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_KERN_XPAIR 14  /* random re-routed pairs                     */
 #define ESM_KERN_SMS 15    /* storage multiscale moves of one block length and class */
 #define ESM_KERN_SLONG 16  /* long-range storage moves (one launch per sweep)         */
-#define ESM_NKERN 17
+#define ESM_KERN_MLPF 17   /* the learned proposal's forward pass of one chunk (4 MFMA GEMMs) */
+#define ESM_KERN_MLPP 18   /* the learned moves of one chunk (draw, MH ratio, commit)         */
+#define ESM_KERN_MLPT 19   /* one training step (features, forward, head, backward, Adam)     */
+#define ESM_NKERN 20
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
 

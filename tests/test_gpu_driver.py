@@ -91,3 +91,19 @@ def test_run_replicas_learned_proposal():
         assert rec["learned"][0] == per_step
     assert np.all(np.isfinite(np.array(res["traces"])))
     assert res["ess_per_s_with_training"] < res["ess_per_s"]
+
+
+def test_chains_in_threads_match_a_chain_alone():
+    """The chains run in host threads, each on its own handle and stream: chain 0 of a three-chain
+    run is bitwise the chain of a one-chain run with the same seed (its own keys, no shared state)."""
+    print("run_replicas threads vs alone", flush=True)
+    base = dict(replicas=4, mlda_warmup=6, nsamples=6, replica_init_sweeps=2, replica_burst_every=4,
+                replica_probe_every=3, replica_probe_len=3, replica_probe_tail=1, replica_probe_win=1,
+                replica_probe_avg=1, replica_gate_min=1, replica_gate_max=2, replica_gate_min_window=1,
+                replica_gate_max_window=3, replica_gate_win=2, synthetic_nodes=10, synthetic_lines=16, snapshots=48,
+                precision="fp64", seed=11, learned_proposal=True, learned_train_steps=20, learned_components=2)
+    a = RP.run_replicas(RP.load_config(dict(base, nchains=3)))
+    b = RP.run_replicas(RP.load_config(dict(base, nchains=1)))
+    assert np.array_equal(np.array(a["traces"][0]), np.array(b["traces"][0]))
+    for ra, rb in zip(a["records"][0], b["records"][0]):
+        assert np.array_equal(ra["k"], rb["k"]) and np.array_equal(ra["learned"], rb["learned"])
