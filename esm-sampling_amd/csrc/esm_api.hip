@@ -119,13 +119,13 @@ static int rebuild(esm_handle *h) {
     return h->prec == ESM_FP64 ? launch_energy<double>(h, false, false) : launch_energy<float>(h, false, false);
 }
 
-static void clear_prof(esm_handle *h) {
-    for (int i = 0; i < ESM_NKERN; i++) {
-        for (auto &e : h->prof_ev[i]) {
-            (void)hipEventDestroy(e.first);
-            (void)hipEventDestroy(e.second);
-        }
-        h->prof_ev[i].clear();
+// forget the recorded intervals (their events return to the pool); with `destroy`, free the pool
+static void clear_prof(esm_handle *h, bool destroy = false) {
+    for (int i = 0; i < ESM_NKERN; i++) h->prof_ev[i].clear();
+    h->prof_next = 0;
+    if (destroy) {
+        for (hipEvent_t e : h->prof_pool) (void)hipEventDestroy(e);
+        h->prof_pool.clear();
     }
 }
 
@@ -179,7 +179,7 @@ static int upload_move_tables(esm_handle *h) {
 
 static void free_handle(esm_handle *h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    clear_prof(h);
+    clear_prof(h, true);
     mlp_free(h);
     void *ps[] = {h->d_bus0, h->d_bus1, h->d_adj_ptr, h->d_adj_line, h->d_adj_other, h->mb.lb, h->mb.ls, h->mb.adj_b,
                   h->mb.load, h->mb.mc, h->mb.ef, h->mb.kc, h->mb.hours, h->mb.eta_in, h->mb.eta_out, h->mb.qt,
@@ -850,7 +850,17 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 int32_t esm_profile(esm_handle *h, int32_t enable) {
     if (!h) { set_error("esm_profile: null handle"); return ESM_EINVAL; }
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
-    if (enable) clear_prof(h);
+    if (enable) {
+        // intervals of earlier profiles are dropped; the pool holds enough events for a replica
+        // step's launches, so the timed work creates none (creating them on the way would keep the
+        // device waiting for the host between launches)
+        clear_prof(h);
+        while (h->prof_pool.size() < 8192) {
+            hipEvent_t e = nullptr;
+            ESM_HIP_CHECK(hipEventCreate(&e));
+            h->prof_pool.push_back(e);
+        }
+    }
     h->prof = enable != 0;
     return ESM_OK;
 }

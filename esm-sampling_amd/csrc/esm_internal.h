@@ -163,6 +163,16 @@ struct esm_handle {
     // kernel timing (esm_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[ESM_NKERN];
+    std::vector<hipEvent_t> prof_pool;   // events created ahead of the timed work (handed out in order)
+    size_t prof_next = 0;
+    hipEvent_t prof_event() {            // an event from the pool (a new one when it is used up)
+        if (prof_next < prof_pool.size()) return prof_pool[prof_next++];
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        prof_pool.push_back(e);
+        prof_next = prof_pool.size();
+        return e;
+    }
 };
 
 namespace esm {
@@ -174,12 +184,11 @@ struct ProfScope {
     hipEvent_t b = nullptr;
     ProfScope(esm_handle *h_, int id_) : h(h_), id(id_) {   // id < 0: not timed
         if (h->prof && id >= 0) {
-            hipEvent_t a;
-            if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+            hipEvent_t a = h->prof_event();
+            b = a ? h->prof_event() : nullptr;
+            if (b) {
                 (void)hipEventRecord(a, h->stream);
                 h->prof_ev[id].push_back({a, b});
-            } else {
-                b = nullptr;
             }
         }
     }

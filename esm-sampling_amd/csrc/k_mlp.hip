@@ -499,6 +499,9 @@ __global__ void k_batch_idx(uint32_t k0, uint32_t k1, uint32_t ctr, uint32_t ste
 }
 
 constexpr int MLP_LOSS_CHUNK = 1024;
+// rows per chunk of instances (and the largest explicit batch): a chunk's buffers take ≈ 0.5 GB;
+// larger chunks ran the GEMMs at the same rate per row
+constexpr int MLP_CAP = 65536;
 
 struct MlpState {
     MlpDims D{};
@@ -507,7 +510,7 @@ struct MlpState {
     float *p = nullptr, *g = nullptr, *am = nullptr, *av = nullptr;   // master params, gradient, Adam moments
     uint16_t *wt16 = nullptr;                                         // BF16 Wt of every layer (offsets ow)
     long long adam_t = 0;
-    int cap = 65536;
+    int cap = MLP_CAP;
     uint16_t *X = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *DG[4] = {nullptr, nullptr, nullptr, nullptr};
     float *OUT = nullptr, *Y = nullptr, *MU0 = nullptr, *YP = nullptr;
     double *LR = nullptr, *dloss = nullptr;   // loss sums of up to MLP_LOSS_CHUNK training steps
@@ -534,19 +537,20 @@ static int mlp_refresh_bf16(esm_handle *h) {
     return 0;
 }
 
-// forward of cnt rows of s.X into s.OUT
+// forward of cnt rows of s.X into s.OUT (with prof: every GEMM launch timed on its own)
 static int mlp_forward(esm_handle *h, int cnt, bool prof = false) {
     MlpState &s = *h->mlp;
-    ProfScope ps(h, prof ? ESM_KERN_MLPF : -1);
     const MlpDims &D = s.D;
     int rc;
     const uint16_t *in = s.X;
     for (int l = 0; l < 3; l++) {
+        ProfScope ps(h, prof ? ESM_KERN_MLPF : -1);
         if ((rc = gemm_nt<EPI_RELU>(h->stream, cnt, D.dh, s.lin[l], in, s.lin[l], s.wt16 + s.ow[l], s.lin[l],
                                      s.p + s.ob[l], s.H[l], D.dh)))
             return rc;
         in = s.H[l];
     }
+    ProfScope ps(h, prof ? ESM_KERN_MLPF : -1);
     return gemm_nt<EPI_F32>(h->stream, cnt, D.dout, D.dh, in, D.dh, s.wt16 + s.ow[3], D.dh, s.p + s.ob[3], s.OUT,
                             D.dout);
 }

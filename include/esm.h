@@ -377,7 +377,7 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_KERN_XPAIR 14  /* random re-routed pairs                     */
 #define ESM_KERN_SMS 15    /* storage multiscale moves of one block length and class */
 #define ESM_KERN_SLONG 16  /* long-range storage moves (one launch per sweep)         */
-#define ESM_KERN_MLPF 17   /* the learned proposal's forward pass of one chunk (4 MFMA GEMMs) */
+#define ESM_KERN_MLPF 17   /* one MFMA GEMM of the learned proposal's forward pass (4 per chunk) */
 #define ESM_KERN_MLPP 18   /* the learned moves of one chunk (draw, MH ratio, commit)         */
 #define ESM_KERN_MLPT 19   /* one training step (features, forward, head, backward, Adam)     */
 #define ESM_NKERN 20
