@@ -427,7 +427,7 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     zmalloc((void **)&h->d_sk, (size_t)nrep * ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_stats, (size_t)ESM_NFAM * ESM_NSTAT * STAT_SLOTS * sizeof(unsigned long long));
-    zmalloc((void **)&h->d_counts, (size_t)ESM_NWEIGHT * T * sizeof(int));
+    zmalloc((void **)&h->d_counts, (size_t)(ESM_NWEIGHT + 1) * T * sizeof(int));
     if (!rc) rc = precision == ESM_FP64 ? upload_move_tables<double>(h) : upload_move_tables<float>(h);
     // weather process (optional: a model without weather tables has none)
     h->has_weather = mdl->Ptr && mdl->pi0 && mdl->mu && mdl->xbar && mdl->K >= 1 && mdl->K <= ESM_MAXK &&
@@ -669,6 +669,12 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
     for (int q = 0; q < ESM_NWEIGHT; q++) h->sweep_w[q] = w[q];
     h->sweep_atoms = atoms != 0;
     h->sweep_pms = p_storage_ms;
+    return ESM_OK;
+}
+
+int32_t esm_sweep_launch(esm_handle *h, int32_t merged) {
+    if (!h) { set_error("esm_sweep_launch: null handle"); return ESM_EINVAL; }
+    h->sweep_merged = merged != 0;
     return ESM_OK;
 }
 

@@ -155,7 +155,9 @@ struct esm_handle {
     double sweep_plong = 1.0;       // probability of the long-range storage sweep per operation sweep
     int sweep_long_moves = 32;      // long-range plateaus per storage unit and sweep
     double sweep_long_hours = 24.0; // ... for the units with at least this many hours of storage
-    int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step
+    bool sweep_merged = true;       // the random families of a parity class in one launch
+    int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step, then T ints
+                                                  // (a parity class's steps in launch order)
     esm::MoveTablesHost mt;                       // templates (host, FP64)
     esm::MoveTablesDev dmt;                       // the same on the device (values in the state precision)
     // the learned dispatch-block proposal (esm_mlp_*; null until esm_mlp_init)

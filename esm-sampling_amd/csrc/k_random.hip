@@ -3,9 +3,11 @@
 // Per step τ the reference draws round(extra·N) moves over the plain families
 // (plain_families, :1912-1915) and runs them in a shuffled order together with the atoms; steps
 // of one time-parity class run in parallel. Here, per step and sweep, k_movecounts makes the same
-// draws (one uniform each, inverted on the cumulative weights) and counts them per family; one
-// kernel per family and parity class then runs, in every thread (step τ, replica j), that
-// family's moves of step τ in sequence. A move's selection (node, partner, cell) comes from a
+// draws (one uniform each, inverted on the cumulative weights) and counts them per family; per
+// parity class the families then run in a fixed order (unit, potential, spair, xpair, xlocc,
+// xloc, cell), every thread (step τ, replica j) running a family's moves of step τ in sequence:
+// one kernel per atom-site family and one for the four table-driven families together (or one
+// per family, esm_sweep_launch). A move's selection (node, partner, cell) comes from a
 // counter of (family, index, step, sweep) and is the same for every replica, so the 64 lanes of a
 // wave (64 replicas of one step) run one move shape on their own states; the line draw and the
 // Jacobian test use counters of the replica.
@@ -48,7 +50,9 @@ struct RandArgs {
     int w;                       // the family's position in the weight list
     const int *counts;           // [ESM_NWEIGHT][T]
     unsigned long long *stats;   // [ESM_NSTAT][STAT_SLOTS] of the family
+    const int *perm;             // the class's steps in launch order (null: in order)
 };
+
 
 __device__ __forceinline__ uint32_t pick(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
 
@@ -62,6 +66,13 @@ __device__ __forceinline__ void uniform4(uint32_t *s) {
     if (UNI)
 #pragma unroll
         for (int i = 0; i < 4; i++) s[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)s[i]);
+}
+
+// the step of the wave that holds element g (replica fastest)
+template <bool UNI>
+__device__ __forceinline__ int step_of(const RandArgs &a, size_t g, int R) {
+    const int i = (int)(g / R);
+    return uniform<UNI>(a.perm ? a.perm[i] : a.tau0 + i * a.tstride);
 }
 
 // the selection words of move `k` of family `w` at step τ
@@ -100,23 +111,26 @@ __global__ void k_movecounts(int T, int nextra, CumW cw, uint32_t k0, uint32_t k
 // ---------------------------------------------------------------------------------------------
 
 template <typename S, bool UNI>
+__device__ __forceinline__ void run_unit(const DevModel<S> &m, SweepState<S> &st, const RandArgs &a, int tau, int j,
+                                         S beta, S maxnats, MoveOut &out) {
+    const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
+    for (int k = 0; k < cnt; k++) {
+        uint32_t s[4];
+        selection(a, k, tau, s);
+        uniform4<UNI>(s);
+        const int n = (int)pick(s[0], (uint32_t)m.N);
+        double u1, u2;
+        line_uniforms(a, j, k, tau, u1, u2);
+        site_gen(m, st, n, tau, j, u1, u2, beta, maxnats, out);
+    }
+}
+
+template <typename S, bool UNI>
 __global__ void __launch_bounds__(256) k_rand_unit(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R) {
-        const int j = (int)(g % m.R);
-        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
-        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
-        for (int k = 0; k < cnt; k++) {
-            uint32_t s[4];
-            selection(a, k, tau, s);
-            uniform4<UNI>(s);
-            const int n = (int)pick(s[0], (uint32_t)m.N);
-            double u1, u2;
-            line_uniforms(a, j, k, tau, u1, u2);
-            site_gen(m, st, n, tau, j, u1, u2, beta, maxnats, out);
-        }
-    }
+    if (g < (size_t)a.ntau * m.R)
+        run_unit<S, UNI>(m, st, a, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, out);
     add_counts(a.stats, out);
 }
 
@@ -124,26 +138,45 @@ __global__ void __launch_bounds__(256) k_rand_unit(DevModel<S> m, SweepState<S> 
 // the register footprint of its degree; a step's potential moves run bucket by bucket, each
 // bucket in draw order
 template <typename S, int DLO, int D, bool UNI>
+__device__ __forceinline__ void run_pot(const DevModel<S> &m, SweepState<S> &st, const RandArgs &a, int tau, int j,
+                                        S beta, S maxnats, MoveOut &out) {
+    const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
+    for (int k = 0; k < cnt; k++) {
+        uint32_t s[4];
+        selection(a, k, tau, s);
+        uniform4<UNI>(s);
+        const int n = (int)pick(s[0], (uint32_t)m.N);
+        const int deg = m.adj_ptr[n + 1] - m.adj_ptr[n];
+        if (deg <= DLO || deg > D) continue;   // another bucket (degree 0: no angle direction)
+        double u1, u2;
+        line_uniforms(a, j, k, tau, u1, u2);
+        site_theta<S, D>(m, st, n, tau, j, u1, u2, beta, maxnats, out);
+    }
+}
+
+template <typename S, int DLO, int D, bool UNI>
 __global__ void __launch_bounds__(256) k_rand_pot(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R) {
-        const int j = (int)(g % m.R);
-        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
-        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
-        for (int k = 0; k < cnt; k++) {
-            uint32_t s[4];
-            selection(a, k, tau, s);
-            uniform4<UNI>(s);
-            const int n = (int)pick(s[0], (uint32_t)m.N);
-            const int deg = m.adj_ptr[n + 1] - m.adj_ptr[n];
-            if (deg <= DLO || deg > D) continue;   // another bucket (degree 0: no angle direction)
-            double u1, u2;
-            line_uniforms(a, j, k, tau, u1, u2);
-            site_theta<S, D>(m, st, n, tau, j, u1, u2, beta, maxnats, out);
-        }
-    }
+    if (g < (size_t)a.ntau * m.R)
+        run_pot<S, DLO, D, UNI>(m, st, a, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, out);
     add_counts(a.stats, out);
+}
+
+template <typename S, bool UNI>
+__device__ __forceinline__ void run_spair(const DevModel<S> &m, SweepState<S> &st, const RandArgs &a, int tau, int j,
+                                          S beta, S maxnats, S kmin, MoveOut &out) {
+    const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
+    for (int k = 0; k < cnt; k++) {
+        uint32_t s[4];
+        selection(a, k, tau, s);
+        uniform4<UNI>(s);
+        const int n = (int)pick(s[0], (uint32_t)m.N);
+        const int sc = (int)pick(s[1], (uint32_t)ESM_NSTOR);   // the node's storage units: battery, H2
+        double u1, u2;
+        line_uniforms(a, j, k, tau, u1, u2);
+        site_soc<S, true>(m, st, sc, n, tau, j, u1, u2, beta, maxnats, kmin, out);
+    }
 }
 
 template <typename S, bool UNI>
@@ -151,21 +184,8 @@ __global__ void __launch_bounds__(256) k_rand_spair(DevModel<S> m, SweepState<S>
                                                     S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R) {
-        const int j = (int)(g % m.R);
-        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
-        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
-        for (int k = 0; k < cnt; k++) {
-            uint32_t s[4];
-            selection(a, k, tau, s);
-            uniform4<UNI>(s);
-            const int n = (int)pick(s[0], (uint32_t)m.N);
-            const int sc = (int)pick(s[1], (uint32_t)ESM_NSTOR);   // the node's storage units: battery, H2
-            double u1, u2;
-            line_uniforms(a, j, k, tau, u1, u2);
-            site_soc<S, true>(m, st, sc, n, tau, j, u1, u2, beta, maxnats, kmin, out);
-        }
-    }
+    if (g < (size_t)a.ntau * m.R)
+        run_spair<S, UNI>(m, st, a, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, kmin, out);
     add_counts(a.stats, out);
 }
 
@@ -625,13 +645,9 @@ __device__ S sample_sparse(const FSparse<S> &f, S beta, S maxnats, S useg, S uof
 enum { SP_CELL = 0, SP_XLOC = 1, SP_XLOCC = 2, SP_XPAIR = 3 };
 
 template <typename S, int KIND, bool UNI>
-__global__ void __launch_bounds__(256) k_sparse(DevModel<S> m, SweepState<S> st, RandArgs a, Tables<S> tb, S beta,
-                                                S maxnats, S kmin) {
-    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    MoveOut out;
-    if (g < (size_t)a.ntau * m.R) {
-        const int j = (int)(g % m.R);
-        const int tau = uniform<UNI>(a.tau0 + (int)(g / m.R) * a.tstride);
+__device__ __forceinline__ void run_sparse(const DevModel<S> &m, SweepState<S> &st, const RandArgs &a, const Tables<S> &tb,
+                                           int tau, int j, S beta, S maxnats, S kmin, MoveOut &out) {
+    {
         const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
         const size_t TR = (size_t)m.T * m.R, o = (size_t)tau * m.R + j;
         S *yg = st.v + plane(m.N, m.T, m.R);
@@ -737,7 +753,95 @@ __global__ void __launch_bounds__(256) k_sparse(DevModel<S> m, SweepState<S> st,
             }
         }
     }
+}
+
+template <typename S, int KIND, bool UNI>
+__global__ void __launch_bounds__(256) k_sparse(DevModel<S> m, SweepState<S> st, RandArgs a, Tables<S> tb, S beta,
+                                                S maxnats, S kmin) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    MoveOut out;
+    if (g < (size_t)a.ntau * m.R)
+        run_sparse<S, KIND, UNI>(m, st, a, tb, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, kmin, out);
     add_counts(a.stats, out);
+}
+
+// The table-driven families of a step (xpair, xlocc, xloc, cell: the last four of the family
+// order) in one thread (step τ, replica j), in that order. Steps of one parity class share no
+// values, so states and statistics equal those of one launch per family; a launch then lasts as
+// long as its slowest step's whole list instead of the sum over the families of each family's
+// slowest step. (unit, potential and spair keep their own launches: their register footprints,
+// up to 256 VGPRs for the potential moves of degree > 8, would cut the merged kernel's occupancy.)
+constexpr int NSPARSE = 4;
+struct SparseAll {
+    unsigned long long *stats[NSPARSE];   // xpair, xlocc, xloc, cell
+    uint32_t mask;                        // families with a positive weight
+};
+
+// The steps of a parity class, heaviest first by Σ_f wk[f] · (moves of family f at the step): a
+// launch hands its first waves the longest lists, so the waves that do not fit in the first round
+// are the shortest. Which wave runs a step does not change any result. One block per class: a
+// counting sort over SORT_BINS keys.
+constexpr int SORT_BINS = 2048;
+struct StepWeights {
+    int wk[ESM_NWEIGHT];
+};
+__global__ void __launch_bounds__(1024) k_step_order(const int *counts, int T, int tau0, int tstride, int ntau,
+                                                     StepWeights sw, int *perm) {
+    __shared__ int hist[SORT_BINS];
+    for (int b = threadIdx.x; b < SORT_BINS; b += blockDim.x) hist[b] = 0;
+    __syncthreads();
+    auto key = [&](int tau) {
+        int w = 0;
+#pragma unroll
+        for (int q = 0; q < ESM_NWEIGHT; q++) w += sw.wk[q] * counts[(size_t)q * T + tau];
+        return SORT_BINS - 1 - min(w, SORT_BINS - 1);   // descending weight
+    };
+    for (int i = threadIdx.x; i < ntau; i += blockDim.x) atomicAdd(&hist[key(tau0 + i * tstride)], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int b = 0; b < SORT_BINS; b++) {
+            const int c = hist[b];
+            hist[b] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ntau; i += blockDim.x) {
+        const int tau = tau0 + i * tstride;
+        perm[atomicAdd(&hist[key(tau)], 1)] = tau;
+    }
+}
+
+template <typename S, bool UNI>
+__global__ void __launch_bounds__(256, 4) k_sparse_all(DevModel<S> m, SweepState<S> st, RandArgs a, SparseAll sa,
+                                                       Tables<S> tb, S beta, S maxnats, S kmin) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    MoveOut o[NSPARSE];
+    if (g < (size_t)a.ntau * m.R) {
+        const int j = (int)(g % m.R);
+        const int tau = step_of<UNI>(a, g, m.R);
+        RandArgs b = a;
+        if (sa.mask & 1u) {
+            b.w = W_XPAIR;
+            run_sparse<S, SP_XPAIR, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[0]);
+        }
+        if (sa.mask & 2u) {
+            b.w = W_XLOCC;
+            run_sparse<S, SP_XLOCC, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[1]);
+        }
+        if (sa.mask & 4u) {
+            b.w = W_XLOC;
+            run_sparse<S, SP_XLOC, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[2]);
+        }
+        if (sa.mask & 8u) {
+            b.w = W_CELL;
+            run_sparse<S, SP_CELL, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[3]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NSPARSE; q++)
+        if (sa.mask >> q & 1u) add_counts(sa.stats[q], o[q]);
 }
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
@@ -765,19 +869,50 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                  d.nd_off, d.nd_node, d.nd_ent, d.nd_kk, d.nd_l, d.nl_pos, d.nl_sgn, d.xp_off, d.lev_off,
                  (const S *)d.th_val, (const S *)d.en_c, (const S *)d.kk_g, (const S *)d.ln_df, d.qcum,
                  d.xp0, d.cl0, d.nlev};
-    // the families in the order of the weight list; pair and cellpair are void here
-    const int order[7] = {W_UNIT, W_XPAIR, W_XLOCC, W_XLOC, W_POT, W_CELL, W_SPAIR};
+    // the family order (pair and cellpair are void here): the atom-site families, then the
+    // table-driven ones
+    const int order[7] = {W_UNIT, W_POT, W_SPAIR, W_XPAIR, W_XLOCC, W_XLOC, W_CELL};
     const bool uni = R % 64 == 0;
+    int dmax = 0;
+    for (int n = 0; n < N; n++) dmax = std::max(dmax, h->adj_ptr[n + 1] - h->adj_ptr[n]);
+    SparseAll sa{};
+    {
+        const int sw[NSPARSE] = {W_XPAIR, W_XLOCC, W_XLOC, W_CELL};
+        const int sf[NSPARSE] = {ESM_FAM_XPAIR, ESM_FAM_XLOCC, ESM_FAM_XLOC, ESM_FAM_CELL};
+        for (int q = 0; q < NSPARSE; q++) {
+            sa.stats[q] = h->d_stats + (size_t)sf[q] * ESM_NSTAT * STAT_SLOTS;
+            if (h->sweep_w[sw[q]] > 0) sa.mask |= 1u << q;
+        }
+    }
 #define RLAUNCH(KU, KD, ...)                                                                           \
     do {                                                                                               \
         if (uni) hipLaunchKernelGGL(KU, dim3(nb), dim3(256), 0, h->stream, __VA_ARGS__);                \
         else hipLaunchKernelGGL(KD, dim3(nb), dim3(256), 0, h->stream, __VA_ARGS__);                    \
     } while (0)
+    // the steps of a class heaviest first (when a wave holds one step's replicas: several steps per
+    // wave would put the heaviest together in the same waves, measured slower at 16 replicas) for
+    // the launches that need more than one round of waves: spair (194 VGPRs) and the merged
+    // table-driven families (an xpair evaluation reads every line and node, cells and compensated
+    // pairs about half as much)
+    int *perm = h->d_counts + (size_t)ESM_NWEIGHT * T;   // T ints after the counts
+    auto order_steps = [&](const RandArgs &a, StepWeights sw) -> const int * {
+        if (!uni) return nullptr;
+        hipLaunchKernelGGL(k_step_order, dim3(1), dim3(1024), 0, h->stream, h->d_counts, T, a.tau0, a.tstride, a.ntau,
+                           sw, perm);
+        return perm;
+    };
+    StepWeights w_spair{}, w_sparse{};
+    w_spair.wk[W_SPAIR] = 1;
+    w_sparse.wk[W_XPAIR] = 4;
+    w_sparse.wk[W_XLOCC] = 2;
+    w_sparse.wk[W_XLOC] = 1;
+    w_sparse.wk[W_CELL] = 2;
     for (size_t c = 0; c < h->tclass.size(); c++) {
-        RandArgs a{k0, k1, ctr, h->tclass[c].first, 2, h->tclass[c].second, 0, h->d_counts, nullptr};
+        RandArgs a{k0, k1, ctr, h->tclass[c].first, 2, h->tclass[c].second, 0, h->d_counts, nullptr, nullptr};
         const unsigned nb = nblk((size_t)a.ntau * R, 256);
         for (int w : order) {
             if (!(h->sweep_w[w] > 0)) continue;
+            if (h->sweep_merged && (w == W_XPAIR || w == W_XLOCC || w == W_XLOC || w == W_CELL)) continue;
             a.w = w;
             auto stats = [&](int fam) { return h->d_stats + (size_t)fam * ESM_NSTAT * STAT_SLOTS; };
             switch (w) {
@@ -790,8 +925,6 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 case W_POT: {
                     a.stats = stats(ESM_FAM_POT);
                     ProfScope p(h, ESM_KERN_POT);
-                    int dmax = 0;
-                    for (int n = 0; n < N; n++) dmax = std::max(dmax, h->adj_ptr[n + 1] - h->adj_ptr[n]);
                     RLAUNCH((k_rand_pot<S, 0, 4, true>), (k_rand_pot<S, 0, 4, false>), m, st, a, beta, mxn);
                     if (dmax > 4)
                         RLAUNCH((k_rand_pot<S, 4, 8, true>), (k_rand_pot<S, 4, 8, false>), m, st, a, beta, mxn);
@@ -802,7 +935,9 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 case W_SPAIR: {
                     a.stats = stats(ESM_FAM_SPAIR);
                     ProfScope p(h, ESM_KERN_SPAIR);
-                    RLAUNCH((k_rand_spair<S, true>), (k_rand_spair<S, false>), m, st, a, beta, mxn, kmin);
+                    RandArgs b = a;
+                    b.perm = order_steps(a, w_spair);
+                    RLAUNCH((k_rand_spair<S, true>), (k_rand_spair<S, false>), m, st, b, beta, mxn, kmin);
                     break;
                 }
                 case W_CELL: {
@@ -831,6 +966,13 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 }
                 default: break;
             }
+            ESM_HIP_CHECK(hipGetLastError());
+        }
+        if (h->sweep_merged && sa.mask) {
+            ProfScope p(h, ESM_KERN_RANDOM);
+            RandArgs b = a;
+            b.perm = order_steps(a, w_sparse);
+            RLAUNCH((k_sparse_all<S, true>), (k_sparse_all<S, false>), m, st, b, sa, tb, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());
         }
     }

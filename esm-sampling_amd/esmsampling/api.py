@@ -160,6 +160,12 @@ def storage_long_options(R: Replicas, p: float = 1.0, nmoves: int = 32, hmin: fl
     check(R.lib.esm_storage_long_options(R.handle, float(p), int(nmoves), float(hmin)), "esm_storage_long_options")
 
 
+def sweep_launch(R: Replicas, merged: bool = True):
+    """The table-driven random families of a parity class in one launch (default) or one launch
+    per family (per-family kernel times); states and statistics are identical."""
+    check(R.lib.esm_sweep_launch(R.handle, 1 if merged else 0), "esm_sweep_launch")
+
+
 def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0, maxnats: float = 40.0):
     """``operation_sweep!`` / ``local_sweep!`` (src/local.jl:1927-2062, :2401-2425) of every replica,
     ``nsweeps`` times at fixed capacities and weather: the atoms, the random families, then (each

@@ -256,6 +256,15 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
 int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double hmin);
 
 /*
+ * How the random families of an operation sweep are launched (no counterpart in the reference,
+ * which runs them on CPU threads): merged = 1 (the default) runs the table-driven families of a
+ * step (xpair, xlocc, xloc, cell) in one launch per parity class; merged = 0 launches one kernel
+ * per family and parity class (per-family kernel times for esm_profile_read). States and
+ * statistics are identical.
+ */
+int32_t esm_sweep_launch(esm_handle *h, int32_t merged);
+
+/*
  * The learned dispatch-block proposal (no counterpart in the reference; the extension that the
  * plain line-conditional moves judge): a move replaces the gas utilization of node n over the B
  * hours of block d (steps dB … dB+B−1, T/B blocks per node) of one replica by a draw from
@@ -380,7 +389,8 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_KERN_MLPF 17   /* one MFMA GEMM of the learned proposal's forward pass (4 per chunk) */
 #define ESM_KERN_MLPP 18   /* the learned moves of one chunk (draw, MH ratio, commit)         */
 #define ESM_KERN_MLPT 19   /* one training step (features, forward, head, backward, Adam)     */
-#define ESM_NKERN 20
+#define ESM_KERN_RANDOM 20 /* the table-driven random families of one parity class, merged    */
+#define ESM_NKERN 21
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
 

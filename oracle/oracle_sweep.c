@@ -1335,7 +1335,7 @@ static void random_step(Sys *S, const Tables *tb, int j, int tau, const int *cou
                         const uint32_t key[2], uint32_t ctr, double maxnats, int64_t *st) {
     const esm_model *m = S->m;
     int N = m->N, T = m->T;
-    static const int order[7] = {W_UNIT, W_XPAIR, W_XLOCC, W_XLOC, W_POT, W_CELL, W_SPAIR};
+    static const int order[7] = {W_UNIT, W_POT, W_SPAIR, W_XPAIR, W_XLOCC, W_XLOC, W_CELL};
     for (int oi = 0; oi < 7; oi++) {
         int w = order[oi];
         if (!(wts[w] > 0)) continue;

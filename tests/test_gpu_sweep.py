@@ -162,6 +162,30 @@ def test_sweep_fp64_matches_oracle_64_replicas():
     dev.close()
 
 
+@pytest.mark.parametrize("prec,R", [(FP64, 3), (FP32, 64)])
+def test_merged_random_launch_equals_one_launch_per_family(prec, R):
+    """The table-driven families of a parity class in one launch (the default) leave states, caches
+    and statistics bitwise equal to one launch per family (steps of a class share no values)."""
+    print(f"merged random launch prec={prec} R={R}", flush=True)
+    s = synth.make_system(N=40, L=70, T=48, dt=1.0, seed=2)
+    rng = np.random.default_rng(9)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    v[0] = rng.normal(0, 300.0, (s.N, s.T, R))
+    out = []
+    for merged in (True, False):
+        dev = api.Replicas(s, R, prec)
+        dev.upload(v=v, k=s.k0, x=x)
+        api.sweep_launch(dev, merged)
+        st = api.operation_sweep(dev, 2, key=(4, 4), counter=5)
+        out.append((st, *dev.download(), *dev.caches()))
+        dev.close()
+    (sa, va, _, _, ra, Fa), (sb, vb, _, _, rb, Fb) = out
+    assert np.all(sa[3:10, 0] > 0)
+    assert np.array_equal(sa, sb)
+    assert np.array_equal(va, vb) and np.array_equal(ra, rb) and np.array_equal(Fa, Fb)
+
+
 @pytest.mark.parametrize("R,T", [(4, 47), (64, 20), (3, 300)])
 def test_storage_ms_fp64_matches_oracle(R, T):
     """The storage multiscale sweep alone (every block length below T/2, segments of 4 … 64 lanes)."""
