@@ -119,6 +119,13 @@ def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
     for k, f in fam.items():
         launches = max(prof.get(k, (0, 0))[1] / max(steps, 1), 1)
         out[k] = per_move[k] * es * float(stats[f, 0]) / launches
+    # the merged launch of the table-driven families (one per parity class): their moves together
+    merged = ("k_cell", "k_xloc", "k_xlocc", "k_xpair")
+    launches = max(prof.get("k_random", (0, 0))[1] / max(steps, 1), 1)
+    out["k_random"] = sum(per_move[k] * es * float(stats[fam[k], 0]) for k in merged) / launches
+    for k in merged:
+        if prof.get(k, (0, 0))[1] == 0:
+            out.pop(k)
     # storage multiscale sweep (when it runs: 2 launches per block length L = 2, 4, …, 128 < T/2): a
     # move on a block of L + 1 steps reads ye, ye1 and r of every step and the gas at both ends of
     # each of its nu units, and writes r of every step, ye of the L inner instants and the gas at the
@@ -174,7 +181,8 @@ def main():
              _lib.KERN_GEN: "k_gen", _lib.KERN_THETA: "k_theta", _lib.KERN_SOC: "k_soc", _lib.KERN_SPAIR: "k_spair",
              _lib.KERN_FOLLOW: "k_follow", _lib.KERN_SWEEP: "sweep", _lib.KERN_UNIT: "k_unit",
              _lib.KERN_POT: "k_pot", _lib.KERN_CELL: "k_cell", _lib.KERN_XLOC: "k_xloc", _lib.KERN_XLOCC: "k_xlocc",
-             _lib.KERN_XPAIR: "k_xpair", _lib.KERN_SMS: "k_storage_ms", _lib.KERN_SLONG: "k_storage_long"}
+             _lib.KERN_XPAIR: "k_xpair", _lib.KERN_SMS: "k_storage_ms", _lib.KERN_SLONG: "k_storage_long",
+             _lib.KERN_RANDOM: "k_random"}
     prof = {}
     for kid, name in names.items():
         tot, cnt = C.c_double(), C.c_int64()
@@ -298,7 +306,7 @@ def measured_traffic(kernel, s, T, R, prec):
              "k_unit": "k_rand_unit<float, true>", "k_cell": "k_sparse<float, 0, true>",
              "k_xloc": "k_sparse<float, 1, true>", "k_xlocc": "k_sparse<float, 2, true>",
              "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms",
-             "k_storage_long": "k_storage_long<float, true>"}
+             "k_storage_long": "k_storage_long<float, true>", "k_random": "k_sparse_all<float, true>"}
     return json.loads(p.read_text())["bytes_per_dispatch"].get(names.get(kernel, ""))
 
 

@@ -293,7 +293,7 @@ __device__ __forceinline__ int sms_visit(V &vis, S x, S cx, S &cmin, int &np) {
 //   STOR next storage root x on (tp, t): evaluate x, then BAL on (a0, x); none: BAL on (a0, t)
 //   BAL  next balance root y on the piece: evaluate and visit y; none: visit the piece's end
 template <class F, typename S, class V>
-__device__ int walk_stor(const F &f, S beta, S maxnats, S sgn, V &vis) {
+__device__ __forceinline__ int walk_stor(const F &f, S beta, S maxnats, S sgn, V &vis) {
     enum { ST_START, ST_BP, ST_RAY, ST_T, ST_STOR, ST_BAL };
     S cmin = 0, tp = 0, delta = 0, clast = 0, t = 0, c = 0, a0 = 0, xs = 0, x = 0, cx = 0, xb = 0, hp = 0;
     int nx = 0, np = 0, w;
@@ -398,15 +398,15 @@ struct VisS {
 };
 
 template <class F, typename S>
-__device__ S sample_stor(const F &f, S beta, S maxnats, S useg, S uoff, int &status) {
+__device__ __forceinline__ S sample_stor(const F &f, S beta, S maxnats, S useg, S uoff, int &status) {
     VisS<S> vis;
-    S Mx[2] = {S(-INFINITY), S(-INFINITY)}, Ms[2] = {S(0), S(0)};
+    S Mx0 = S(-INFINITY), Mx1 = S(-INFINITY), Ms0 = 0, Ms1 = 0;   // the right (0) and left (1) walks
 #pragma nounroll
     for (int k = 0; k < 3; k++) {
         S sgn;
         if (k == 2) {
-            const S MX = fmax(Mx[0], Mx[1]);
-            const S Mr = Ms[0] * exp(Mx[0] - MX), Ml = Ms[1] * exp(Mx[1] - MX);
+            const S MX = fmax(Mx0, Mx1);
+            const S Mr = Ms0 * exp(Mx0 - MX), Ml = Ms1 * exp(Mx1 - MX);
             const S u = useg * (Ml + Mr);
             vis.mode = 1;
             vis.ss = SegSel<S>();
@@ -425,8 +425,13 @@ __device__ S sample_stor(const F &f, S beta, S maxnats, S useg, S uoff, int &sta
                 status = w == 0 ? 1 : 2;
                 return 0;
             }
-            Mx[k] = vis.ma.mx;
-            Ms[k] = vis.ma.s;
+            if (k == 0) {
+                Mx0 = vis.ma.mx;
+                Ms0 = vis.ma.s;
+            } else {
+                Mx1 = vis.ma.mx;
+                Ms1 = vis.ma.s;
+            }
         }
     }
     status = 0;
@@ -767,10 +772,24 @@ struct FLong {
     mutable bool bp_have = false;
     // the end steps' d at the interval ends (A: tp, B: t) and balances at the piece ends (A: a0,
     // X: x or t), recorded by the evaluations there; slot 0 step a, slot 1 step b
-    mutable S edA[2] = {0, 0}, edB[2] = {0, 0}, ebA[2] = {0, 0}, ebX[2] = {0, 0};
+    // (scalars written by branches: an array indexed by a run-time slot, or a store through a
+    // selected address, would keep the whole walker in scratch memory)
+    mutable S edA0 = 0, edA1 = 0, edB0 = 0, edB1 = 0, ebA0 = 0, ebA1 = 0, ebX0 = 0, ebX1 = 0;
     __device__ void record(int e, int rec, S d1, S S1) const {
-        if (rec == REC_B || rec == REC_A) (rec == REC_B ? edB : edA)[e] = d1;
-        if (rec != REC_NONE) (rec == REC_A ? ebA : ebX)[e] = S1;
+        if (rec == REC_B) {
+            if (e) edB1 = d1;
+            else edB0 = d1;
+        } else if (rec == REC_A) {
+            if (e) edA1 = d1;
+            else edA0 = d1;
+        }
+        if (rec == REC_A) {
+            if (e) ebA1 = S1;
+            else ebA0 = S1;
+        } else if (rec != REC_NONE) {
+            if (e) ebX1 = S1;
+            else ebX0 = S1;
+        }
     }
 
     __device__ S y(int tau) const { return YL ? yv[tau] : yv[(size_t)tau * R]; }
@@ -875,10 +894,12 @@ struct FLong {
         return seg_sum<S, 64>(D);
     }
     __device__ void shift(bool d) const {
-        for (int e = 0; e < 2; e++) {
-            if (d) edA[e] = edB[e];
-            ebA[e] = ebX[e];
+        if (d) {
+            edA0 = edB0;
+            edA1 = edB1;
         }
+        ebA0 = ebX0;
+        ebA1 = ebX1;
     }
 
     __device__ bool next_bp(S tp, S sgn, S &t) const {
@@ -947,7 +968,7 @@ struct FLong {
             }
         };
         S da, db, Ba, Bb;
-        if (owns_a()) test(edA[0], edB[0], ebA[0], ebX[0]);
+        if (owns_a()) test(edA0, edB0, ebA0, ebX0);
         if (!(llo <= a && a <= lhi && llo <= b && b <= lhi))
             for (int q0 = first(qi0()), c = 0; q0 <= qi1(); q0 += 64 * ch, c++) {
                 const float lo = cwl[c * 64 + lane], hi = cwh[c * 64 + lane];
@@ -960,7 +981,7 @@ struct FLong {
                     test(da, db, Ba, Bb);
                 }
             }
-        if (owns_b()) test(edA[1], edB[1], ebA[1], ebX[1]);
+        if (owns_b()) test(edA1, edB1, ebA1, ebX1);
         return seg_best<S, 64>(have, best, sgn, xr);
     }
 
@@ -1150,7 +1171,10 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
                 }
             }
         }
-        if (lane == 0) (level ? out_level : out_long).add(status, t != 0, f.nev);
+        if (lane == 0) {
+            if (level) out_level.add(status, t != 0, f.nev);
+            else out_long.add(status, t != 0, f.nev);
+        }
         if (t != 0) {
             // from the state before the move: the residuals (and the gas at the end steps) ...
             S *rw = st.r + (size_t)n * TR + j;
