@@ -26,12 +26,13 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 import time
 from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import FP32, FP64, api, synth
+from . import FP32, FP64, api, records as rec_io, synth
 from .diagnostics import convergence_summary, warmup_end
 
 NK = synth.NK
@@ -99,6 +100,12 @@ class Config:
     storage_xpair: float = 0.0
     storage_xlong: int = 0
     maxnats: float = 40.0
+    # snapshots (src/run.jl:12-14): the state snapshot file (resumed from if it exists; "": off),
+    # a snapshot every this many minutes at a step boundary, and the stop file (if it exists at a
+    # step boundary: snapshot, remove it, stop with exit code 75)
+    checkpoint: str = ""
+    checkpoint_minutes: float = 30.0
+    stop_file: str = ""
     # the learned dispatch-block proposal (no counterpart in the reference): after every
     # operation sweep, `learned_moves` learned moves of every (node, block, replica), exactly MH
     # corrected; the network is trained by maximum likelihood on the chain's own states,
@@ -508,34 +515,68 @@ def run_replicas(cfg: Config, log=None):
         sets.append(init_replicas(sys, chain_start_capacities(sys, kinds[c]), M, prec, ctl, keys[c],
                                   cfg.replica_init_sweeps, cfg.maxnats))
         say(f"chain {c} ({kinds[c]}): {M} replicas initialised ({cfg.replica_init_sweeps} sweeps each)")
+    snap = rec_io.read_snapshot(cfg.checkpoint, cfg)
     learned = [None] * nc
     if cfg.learned_proposal:
         for c in range(nc):
             learned[c] = Learned(sets[c].dev, cfg, keys[c])
-            learned[c].train(cfg.learned_train_steps)
-        say(f"learned proposal: {cfg.learned_train_steps} training steps per chain")
+            if snap is None:
+                learned[c].train(cfg.learned_train_steps)
+        if snap is None:
+            say(f"learned proposal: {cfg.learned_train_steps} training steps per chain")
     # one probe handle per chain (src/replicas.jl:733: a probe slot per chain), so chains share nothing
     probes_h = [api.Replicas(sys, 1, prec) if ctl.probe_every > 0 else None for _ in range(nc)]
     for q in probes_h:
         if q is not None:
             ctl.sweep_options(q)
-    t_init = time.perf_counter()
     traces = [[] for _ in range(nc)]
     records = [[] for _ in range(nc)]
     probes = [[] for _ in range(nc)]
     nprobe = [0] * nc
+    it0, restarts, prod_before = 0, 0, 0.0
+    if snap is not None:
+        # resume (src/replicas.jl:1133-1167): the replicas' states, the driver's gates, buffers,
+        # Langevin steps and generator states, the records so far
+        it0, restarts = int(snap["it"]), int(snap["restarts"]) + 1
+        prod_before = float(snap.get("run/production_s", 0.0))
+        for c in range(nc):
+            g = f"chains/{c + 1}"
+            if str(snap[f"{g}/start"]) != kinds[c] or int(snap[f"{g}/replicas"]) != M:
+                raise ValueError(f"snapshot {cfg.checkpoint}: chain {c + 1} does not fit the configuration")
+            R = sets[c]
+            api.set_weather(R.dev, snap[f"{g}/state/m"], snap[f"{g}/state/z"])
+            R.dev.upload(v=snap[f"{g}/state/v"], k=snap[f"{g}/state/k"], x=snap[f"{g}/state/x"])
+            R.k = snap[f"{g}/state/kraw"].copy()
+            R.eta_w = snap[f"{g}/eta_w"].copy()
+            R.since = snap[f"{g}/since"].astype(np.int64)
+            R.window = snap[f"{g}/window"].astype(bool)
+            R.active = snap[f"{g}/active"].astype(bool)
+            R.gbuf, R.hist = rec_io.restore_driver_lists(snap, c)
+            rngs[c].bit_generator.state = rec_io.rng_state_from_json(str(snap[f"{g}/driver/rng"]))
+            nprobe[c] = int(snap[f"{g}/driver/nprobe"])
+            traces[c], records[c], probes[c] = rec_io.chain_records(snap, c)
+            if learned[c] is not None and f"{g}/learned/params" in snap:
+                L = learned[c]
+                L.net.params(set_to=snap[f"{g}/learned/params"])
+                L.ntrain = int(snap[f"{g}/learned/ntrain"])
+                L.att, L.acc = (int(q) for q in snap[f"{g}/learned/counts"])
+                L.loss = list(snap[f"{g}/learned/loss"])
+                L.wall = float(snap[f"{g}/learned/wall"])
+        say(f"resumed from {cfg.checkpoint} after step {it0} (restart {restarts})")
+    t_init = time.perf_counter()
 
     def chain_step(c, it):
         R = sets[c]
         if (learned[c] is not None and cfg.learned_train_every > 0 and it <= nwarm
                 and it % cfg.learned_train_every == 0):
             learned[c].train(cfg.learned_train_more)
+        t_step = time.perf_counter()
         st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats, learned[c])
         traces[c].append(replica_monitor(R, st["parts"]))
-        records[c].append(dict(it=it, k=R.k.copy(), active=R.active.copy(), weights=st["wts"], nclip=st["nclip"],
-                               moved=st["moved"], rs=st["rs"], fk=st["fk"], eta_w=R.eta_w.copy(),
-                               op=st["parts"][:, 1].copy(), shed=st["parts"][:, 4].copy(),
-                               learned=st["learned"].copy()))
+        rec = rec_io.replica_step_record(R.k, st["parts"], st["wts"], R.active, R.since, st["moved"], st["rs"], st["fk"],
+                                         R.eta_w, st["nclip"], time.perf_counter() - t_step, st["tw"], st["learned"])
+        rec.update(it=it, k=R.k.copy(), nclip=int(st["nclip"]))
+        records[c].append(rec)
         if it % 25 == 0 or it == nwarm + nprod:
             say(f"chain {c} step {it}/{nwarm + nprod}: total {traces[c][-1][MONITOR_NAMES.index('total')]:.6g}")
         if ctl.probe_every > 0 and it % ctl.probe_every == 0:
@@ -545,17 +586,41 @@ def run_replicas(cfg: Config, log=None):
                 nprobe[c] += 1
                 eta0 = R.eta_w.copy()
                 adapted = ctl.adapt and it <= nwarm and adapt_step(R, pr["gap"], ctl)
-                probes[c].append(dict(it=it, j=j, adapted=adapted, eta_before=eta0, eta_after=R.eta_w.copy(), **pr))
+                probes[c].append(dict(rec_io.probe_record(it, j, adapted, pr, eta0, R.eta_w), j=j, adapted=adapted,
+                                      **{k: v for k, v in pr.items() if k not in ("gap", "rs_eq", "rs_in", "fk")}))
 
     # the chains are independent: one host thread each, so their device work (one HIP stream per
     # handle) overlaps; every chain's results are those of a sequential run (its own keys)
-    t_prod0 = None
+    def snapshot(it, prod_s):
+        rec_io.write_snapshot(cfg.checkpoint, cfg, it, restarts, kinds, sets, traces, records, probes, rngs, nprobe,
+                              learned, prod_s=prod_s)
+
+    t_prod0 = time.perf_counter() if it0 >= nwarm else None    # resumed inside the production window
+    status = 0
+    it_last = it0
+    t_ckpt = time.perf_counter()
     with ThreadPoolExecutor(max_workers=nc) as ex:
-        for it in range(1, nwarm + nprod + 1):
+        for it in range(it0 + 1, nwarm + nprod + 1):
             if it == nwarm + 1:
                 t_prod0 = time.perf_counter()
             for f in [ex.submit(chain_step, c, it) for c in range(nc)]:
                 f.result()
+            it_last = it
+            if it == nwarm + nprod:
+                break
+            # step boundary: the stop file (snapshot, remove it, exit code 75) and timed snapshots
+            prod_s = prod_before + (time.perf_counter() - t_prod0 if t_prod0 is not None else 0.0)
+            if cfg.stop_file and os.path.exists(cfg.stop_file):
+                if not cfg.checkpoint:
+                    raise ValueError("stop_file needs checkpoint: the snapshot to resume from")
+                snapshot(it, prod_s)
+                os.remove(cfg.stop_file)
+                say(f"stopped after step {it}: snapshot {cfg.checkpoint}")
+                status = rec_io.EXIT_CHECKPOINTED
+                break
+            if cfg.checkpoint and time.perf_counter() - t_ckpt >= 60.0 * cfg.checkpoint_minutes:
+                snapshot(it, prod_s)
+                t_ckpt = time.perf_counter()
     t_end = time.perf_counter()
     if t_prod0 is None:
         t_prod0 = t_end
@@ -575,13 +640,18 @@ def run_replicas(cfg: Config, log=None):
         if good[q] and nwarm + nprod - w0 >= 4:
             summ[MONITOR_NAMES[q]] = convergence_summary([c[w0:] for c in qtr[q]], rhat_max=cfg.rhat_max,
                                                          ess_min=cfg.ess_min, mcse_max=cfg.mcse_max)
-    prod_s = t_end - t_prod0
+    prod_s = prod_before + (t_end - t_prod0)
     tot = summ.get("total")
     for R in sets:
         R.dev.close()
     for q in probes_h:
         if q is not None:
             q.close()
+    passed = all(q["passed"] for q in summ.values()) if summ else False
+    if cfg.out and status == 0:
+        rec_io.write_replica_outputs(cfg.out, cfg, sys, ctl, records, probes, restarts, it0 if snap is not None else -1)
+        rec_io.write_replica_summary(rec_io.summary_path(cfg.out), ctl, records, probes, kinds, nwarm, passed,
+                                     t_end - t_setup)
     lsum = None
     if cfg.learned_proposal:
         lsum = dict(attempted=sum(q.att for q in learned), accepted=sum(q.acc for q in learned),
@@ -590,12 +660,30 @@ def run_replicas(cfg: Config, log=None):
                     loss_last=[q.loss[-1] if q.loss else None for q in learned],
                     train_s=sum(q.wall for q in learned))
     return dict(names=MONITOR_NAMES, traces=traces, records=records, probes=probes, summary=summ, window_start=w0,
-                learned=lsum,
+                learned=lsum, status=status, restarts=restarts, resumed_at=it0 if snap is not None else -1,
                 kinds=kinds, wall=dict(setup=t_init - t_setup, steps=t_end - t_init, production=prod_s),
-                steps_per_s=(nwarm + nprod) / max(t_end - t_init, 1e-12),
+                steps_per_s=(it_last - it0) / max(t_end - t_init, 1e-12), steps_done=it_last,
                 ess_total=None if tot is None else tot["ess_bulk"],
                 ess_per_s=None if tot is None or prod_s <= 0 else tot["ess_bulk"] / prod_s,
                 # with the learned proposal, the training time counts as production time too
                 ess_per_s_with_training=None if tot is None or prod_s <= 0 else
                 tot["ess_bulk"] / (prod_s + (lsum["train_s"] if lsum else 0.0)),
                 passed=all(s["passed"] for s in summ.values()) if summ else False)
+
+
+def main(argv=None):
+    """``julia run.jl config.toml`` for this driver: run the replica sampler of a TOML
+    configuration and exit with 0 (finished) or 75 (stopped by the stop file after a snapshot)."""
+    import sys as _sys
+    args = _sys.argv[1:] if argv is None else argv
+    if len(args) != 1:
+        print("usage: python -m esmsampling.replicas config.toml", file=_sys.stderr)
+        return 2
+    res = run_replicas(load_config(args[0]), log=lambda m: print(m, flush=True))
+    print(f"steps {res['steps_done']}, steps/s {res['steps_per_s']:.3f}, ESS/s of total cost {res['ess_per_s']}, "
+          f"convergence {'passed' if res['passed'] else 'not passed'}", flush=True)
+    return int(res["status"])
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

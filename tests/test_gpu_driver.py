@@ -107,3 +107,38 @@ def test_chains_in_threads_match_a_chain_alone():
     assert np.array_equal(np.array(a["traces"][0]), np.array(b["traces"][0]))
     for ra, rb in zip(a["records"][0], b["records"][0]):
         assert np.array_equal(ra["k"], rb["k"]) and np.array_equal(ra["learned"], rb["learned"])
+
+
+def test_stop_file_snapshot_and_resume(tmp_path):
+    """A run stopped by the stop file in the production window (snapshot, exit code 75) and
+    resumed from its snapshot continues as the run without a stop: the same traces, capacities
+    and Langevin steps (FP64), the records of both parts, the outputs of the finished run."""
+    print("stop file, snapshot, resume", flush=True)
+    base = dict(nchains=2, replicas=4, mlda_warmup=10, nsamples=20, replica_init_sweeps=2, replica_burst_every=4,
+                replica_probe_every=6, replica_probe_len=3, replica_probe_tail=1, replica_probe_win=1,
+                replica_probe_avg=1, replica_gate_min=1, replica_gate_max=2, replica_gate_min_window=1,
+                replica_gate_max_window=3, replica_gate_win=2, synthetic_nodes=10, synthetic_lines=16, snapshots=48,
+                precision="fp64", seed=13)
+    a = RP.run_replicas(RP.load_config(dict(base, out=str(tmp_path / "a" / "out.npz"))))
+    ck, stop = tmp_path / "ck.npz", tmp_path / "STOP"
+
+    def log(msg):
+        if "step 25/" in msg:
+            stop.write_text("")
+
+    cfg = RP.load_config(dict(base, checkpoint=str(ck), stop_file=str(stop), out=str(tmp_path / "b" / "out.npz")))
+    b1 = RP.run_replicas(cfg, log=log)
+    assert b1["status"] == 75 and b1["steps_done"] == 25 and ck.exists() and not stop.exists()
+    b2 = RP.run_replicas(cfg)
+    assert b2["status"] == 0 and b2["restarts"] == 1 and b2["resumed_at"] == 25 and b2["steps_done"] == 30
+    for c in range(2):
+        np.testing.assert_allclose(np.array(b2["traces"][c]), np.array(a["traces"][c]), rtol=1e-12)
+        for ra, rb in zip(a["records"][c], b2["records"][c]):
+            assert ra["it"] == rb["it"]
+            np.testing.assert_allclose(rb["k"], ra["k"], rtol=1e-12)
+            np.testing.assert_allclose(rb["eta_w"], ra["eta_w"], rtol=1e-12)
+        assert len(b2["probes"][c]) == len(a["probes"][c])
+    with np.load(tmp_path / "b" / "out.npz", allow_pickle=False) as f:
+        assert f["replicas/1/ktot"].shape == (6, 30) and int(f["run/restarts"]) == 1
+        assert int(f["run/resumed_at_sweep"]) == 25
+    assert (tmp_path / "b" / "summary_replicas.txt").exists()
