@@ -44,6 +44,10 @@ def parse():
     ap.add_argument("--fp64", action="store_true", help="FP64 device state")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--ess", action="store_true",
+                    help="also run the replica driver with and without the learned proposal and report the ESS/s "
+                         "of total system cost of both (minutes)")
+    ap.add_argument("--ess-steps", type=int, default=240, help="replica steps of each --ess run (1/6 warmup)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
     ap.add_argument("--p-storage-ms", type=float, default=0.5,
                     help="probability of the storage multiscale sweep per operation sweep (the reference's 1/2)")
@@ -232,7 +236,39 @@ def main():
     wl.dev.close()
     if not args.no_learned:
         out["learned"] = learned_leg(args, prec)
+    if args.ess:
+        out["ess"] = ess_leg(args)
     print(json.dumps(out), flush=True)
+
+
+def ess_leg(args, nchains=2, R=16, T=2920):
+    """ESS/s of total system cost (``ess_bulk`` of the replica-mean total, src/convergence.jl:111,
+    over the production steps per second of production wall time) of the replica driver, plain and
+    with the learned proposal (training before the first step and in the warmup; its time counted
+    in ``ess_per_s_with_training``), on 2 chains × 16 replicas of the 40-node system at T hours."""
+    from esmsampling import replicas as RP
+    nw = max(args.ess_steps // 6, 2)
+    base = dict(nchains=nchains, replicas=R, mlda_warmup=nw, nsamples=args.ess_steps - nw, replica_init_sweeps=10,
+                synthetic_nodes=args.nodes, synthetic_lines=args.lines, snapshots=T, resolution_hours=1.0,
+                precision="fp32", seed=7)
+    # a short run first (discarded): the first run of a process pays one-time costs (code objects
+    # loaded at their first launch, clocks) that would otherwise fall on the plain run alone
+    RP.run_replicas(RP.load_config({**base, "mlda_warmup": 2, "nsamples": 8, "learned_proposal": True,
+                                    "learned_train_steps": 10}))
+    res = {}
+    for name, extra in (("plain", {}), ("learned", dict(learned_proposal=True, learned_train_steps=1000,
+                                                         learned_train_every=10, learned_train_more=100))):
+        r = RP.run_replicas(RP.load_config({**base, **extra}),
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
+        tot = r["summary"].get("total", {})
+        res[name] = {"ess_total": r["ess_total"], "ess_per_s": r["ess_per_s"],
+                     "ess_per_s_with_training": r["ess_per_s_with_training"], "rhat": tot.get("rhat"),
+                     "steps_per_s": r["steps_per_s"], "production_s": r["wall"]["production"],
+                     "learned": r["learned"]}
+    p, l = res["plain"]["ess_per_s"], res["learned"]["ess_per_s_with_training"]
+    res["ratio_learned_vs_plain"] = None if not p or l is None else l / p
+    res["config"] = {"nchains": nchains, "replicas": R, "hours": T, "steps": args.ess_steps, "warmup": nw}
+    return res
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
