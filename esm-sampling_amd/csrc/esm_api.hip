@@ -674,7 +674,16 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
 
 int32_t esm_sweep_launch(esm_handle *h, int32_t merged) {
     if (!h) { set_error("esm_sweep_launch: null handle"); return ESM_EINVAL; }
+    if (merged != 0 && merged != 1 && merged != 2 && merged != 4 && merged != 8 && merged != 16) {
+        set_error("esm_sweep_launch: 0 (one launch per family), 1 (merged), or 2, 4, 8, 16 (merged, waves per step)");
+        return ESM_EINVAL;
+    }
+    if (merged > 1 && h->R % 64) {
+        set_error("esm_sweep_launch: waves per step need a replica count that is a multiple of 64");
+        return ESM_EINVAL;
+    }
     h->sweep_merged = merged != 0;
+    h->sweep_waves = merged > 1 ? merged : 0;
     return ESM_OK;
 }
 

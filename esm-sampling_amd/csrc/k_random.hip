@@ -212,6 +212,11 @@ struct FSparse {
     bool comp;              // the kinks of the template are active (xlocc)
     S kmin;
     mutable int nev = 0;
+    // split over the nw waves of a workgroup (wave wv takes every nw-th entry, node and line; the
+    // partial results are combined through `red`, LDS [nw][RED][64], in wave order); nw = 1: none
+    int wv = 0, nw = 1;
+    S *red = nullptr;
+    static constexpr int RED = 9;   // c, hb, nb, nr, more, rt[KR]
 
     __device__ S kap(int n) const { return m->kc[ESM_KGAS * m->N + n]; }
     // slope of gas entry e (entries below 1e-14 are zero, as `finalize!` makes them)
@@ -366,7 +371,7 @@ struct FSparse {
                 P.hb = true;
             }
         };
-        for (int e = e0; e < e1; e++) {
+        for (int e = e0 + wv; e < e1; e += nw) {
             const S a = slope(e);
             if (a == S(0)) continue;
             const int n = tb->en_node[e];
@@ -381,7 +386,7 @@ struct FSparse {
             cand((S(1) - y0) / a);
         }
         if (comp)
-            for (int q = q0; q < q1; q++) {
+            for (int q = q0 + wv; q < q1; q += nw) {
                 S Q, g, ak, P_, e, y0;
                 if (!kink(q, Q, g, ak, P_, e, y0)) continue;
                 cand(-Q / g);
@@ -400,7 +405,7 @@ struct FSparse {
                     cand((S(1) - y0 - off) / sl);
                 }
             }
-        for (int l = l0; l < l1; l++) {
+        for (int l = l0 + wv; l < l1; l += nw) {
             const int e = tb->ln_line[l];
             const S f0 = Fv(e), se = m->ls[e], df = tb->ln_df[l];
             if (nodes) {
@@ -411,7 +416,7 @@ struct FSparse {
             cand((-se - f0) / df);
         }
         if (nodes)
-            for (int p = n0; p < n1; p++) {
+            for (int p = n0 + wv; p < n1; p += nw) {
                 S A, B;
                 node_S2(p, tp, t, A, B);
                 const S S0 = rv(tb->nd_node[p]);
@@ -422,13 +427,14 @@ struct FSparse {
                 }
             }
         P.c = D;
+        combine(P, sgn, nodes);
     }
 
     // ΔC(t)
     __device__ S eval(S t) const {
         nev++;
         S D = 0;
-        for (int e = e0; e < e1; e++) {
+        for (int e = e0 + wv; e < e1; e += nw) {
             const S a = slope(e);
             if (a == S(0)) continue;
             const int n = tb->en_node[e];
@@ -439,7 +445,7 @@ struct FSparse {
             D += c1 - c0;
         }
         if (comp)
-            for (int q = q0; q < q1; q++) {
+            for (int q = q0 + wv; q < q1; q += nw) {
                 S Q, g, ak, P, e, y0;
                 if (!kink(q, Q, g, ak, P, e, y0) || ak == S(0)) continue;
                 const int n = tb->kk_node[q];
@@ -448,18 +454,18 @@ struct FSparse {
                 gen(n, y0 + ak * kphi(Q, g, t), c1, p1);
                 D += c1 - c0;
             }
-        for (int l = l0; l < l1; l++) {
+        for (int l = l0 + wv; l < l1; l += nw) {
             const int e = tb->ln_line[l];
             const S f0 = Fv(e), se = m->ls[e];
             const S f1 = f0 + t * tb->ln_df[l];
             D += m->cv * m->dt * (fmax(fabs(f1) - se, S(0)) - fmax(fabs(f0) - se, S(0)));
         }
-        for (int p = n0; p < n1; p++) {
+        for (int p = n0 + wv; p < n1; p += nw) {
             const S S0 = rv(tb->nd_node[p]);
             const S S1 = node_S(p, t);
             D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
         }
-        return D;
+        return sum_waves(D);
     }
 
     // the nearest primary breakpoint beyond tp in walking direction sgn
@@ -471,7 +477,7 @@ struct FSparse {
                 have = true;
             }
         };
-        for (int e = e0; e < e1; e++) {
+        for (int e = e0 + wv; e < e1; e += nw) {
             const S a = slope(e);
             if (a == S(0)) continue;
             const S y0 = ygv(tb->en_node[e]);
@@ -491,7 +497,7 @@ struct FSparse {
                     cand((S(1) - y0 - off) / sl);
                 }
             }
-        for (int l = l0; l < l1; l++) {
+        for (int l = l0 + wv; l < l1; l += nw) {
             const int e = tb->ln_line[l];
             const S f0 = Fv(e), se = m->ls[e], df = tb->ln_df[l];
             cand((se - f0) / df);
@@ -503,7 +509,7 @@ struct FSparse {
     // the next KR distinct sign changes of the node balances strictly between x and t (walking
     // order) on the segment (tp, t), where the balances are linear; more: others remain
     __device__ void roots_beyond(S tp, S t, S x, S sgn, Pass &P) const {
-        for (int p = n0; p < n1; p++) {
+        for (int p = n0 + wv; p < n1; p += nw) {
             S A, B;
             node_S2(p, tp, t, A, B);
             if ((A < 0) != (B < 0) && A != B) {
@@ -511,20 +517,69 @@ struct FSparse {
                 if (sgn * (rt - tp) > 0 && sgn * (t - rt) > 0 && sgn * (rt - x) > 0) P.insert(rt, sgn);
             }
         }
+        combine(P, sgn, true);
     }
 
     // log Jacobian Σ (|S| − 1) log|P(t)/P| of the compensated kinks (FP64); one unit per node
     __device__ double logjac(S t, bool &any) const {
         double lj = 0.0;
         any = false;
-        for (int q = q0; q < q1; q++) {
+        for (int q = q0 + wv; q < q1; q += nw) {
             S Q, g, ak, P, e, y0;
             if (!kink(q, Q, g, ak, P, e, y0)) continue;
             any = true;
             const double nm = 0.0;   // units of the node with |κ| ≥ κmin, minus one
             lj += nm * log(fabs((fmax((double)Q + (double)g * (double)t, 0.0) + (double)e) / (double)P));
         }
+        if (nw > 1) {   // (lj is exactly 0 here: one unit per node, |S| − 1 = 0)
+            any = sum_waves(any ? S(1) : S(0)) > S(0);
+            lj = (double)sum_waves((S)lj);
+        }
         return lj;
+    }
+
+    // the sum over the waves of a per-lane partial value (in wave order)
+    __device__ S sum_waves(S x) const {
+        if (nw == 1) return x;
+        const int lane = threadIdx.x & 63;
+        red[(size_t)wv * RED * 64 + lane] = x;
+        __syncthreads();
+        S s = 0;
+        for (int w = 0; w < nw; w++) s += red[(size_t)w * RED * 64 + lane];
+        __syncthreads();
+        return s;
+    }
+
+    // the waves' partial passes combined: the costs summed in wave order, the nearest breakpoint,
+    // the KR nearest distinct roots (more: a wave had more, or the merge dropped one)
+    __device__ void combine(Pass &P, S sgn, bool roots) const {
+        if (nw == 1) return;
+        const int lane = threadIdx.x & 63;
+        S *o = red + (size_t)wv * RED * 64 + lane;
+        o[0] = P.c;
+        o[64] = P.hb ? S(1) : S(0);
+        o[128] = P.nb;
+        o[192] = (S)P.nr;
+        o[256] = P.more ? S(1) : S(0);
+#pragma unroll
+        for (int k = 0; k < KR; k++) o[(5 + k) * 64] = P.rt[k];
+        __syncthreads();
+        Pass Q;
+        for (int w = 0; w < nw; w++) {
+            const S *x = red + (size_t)w * RED * 64 + lane;
+            Q.c += x[0];
+            if (x[64] != S(0) && (!Q.hb || sgn * (Q.nb - x[128]) > 0)) {
+                Q.nb = x[128];
+                Q.hb = true;
+            }
+            if (roots) {
+                const int nr = (int)x[192];
+                for (int k = 0; k < nr; k++) Q.insert(x[(5 + k) * 64], sgn);
+                if (x[256] != S(0)) Q.more = true;
+            }
+        }
+        __syncthreads();
+        P = Q;
     }
 };
 
@@ -646,7 +701,8 @@ enum { SP_CELL = 0, SP_XLOC = 1, SP_XLOCC = 2, SP_XPAIR = 3 };
 
 template <typename S, int KIND, bool UNI>
 __device__ __forceinline__ void run_sparse(const DevModel<S> &m, SweepState<S> &st, const RandArgs &a, const Tables<S> &tb,
-                                           int tau, int j, S beta, S maxnats, S kmin, MoveOut &out) {
+                                           int tau, int j, S beta, S maxnats, S kmin, MoveOut &out, int wv = 0,
+                                           int nw = 1, S *red = nullptr) {
     {
         const int cnt = uniform<UNI>(a.counts[(size_t)a.w * m.T + tau]);
         const size_t TR = (size_t)m.T * m.R, o = (size_t)tau * m.R + j;
@@ -692,6 +748,9 @@ __device__ __forceinline__ void run_sparse(const DevModel<S> &m, SweepState<S> &
             f.n1 = tb.nd_off[tid + 1];
             f.comp = KIND == SP_XLOCC;
             f.kmin = kmin;
+            f.wv = wv;
+            f.nw = nw;
+            f.red = red;
             if (KIND != SP_CELL) {   // the pair's units need |κ| ≥ κmin
                 bool ok = true;
                 for (int e = f.e0; e < f.e1; e++) ok = ok && fabs(f.kap(tb.en_node[e])) >= kmin;
@@ -714,11 +773,11 @@ __device__ __forceinline__ void run_sparse(const DevModel<S> &m, SweepState<S> &
                     }
                 }
             }
-            out.add(status, t != 0, f.nev);
+            if (wv == 0) out.add(status, t != 0, f.nev);
             if (t != 0) {
                 // node by node: the balance and the node's entry read only that node's values (and
-                // the flows, written after all nodes)
-                for (int p = f.n0; p < f.n1; p++) {
+                // the flows, written after all nodes; with nw waves, each writes every nw-th node)
+                for (int p = f.n0 + wv; p < f.n1; p += nw) {
                     const int n = tb.nd_node[p];
                     const S S1 = f.node_S(p, t);
                     S ynew = 0;
@@ -742,15 +801,19 @@ __device__ __forceinline__ void run_sparse(const DevModel<S> &m, SweepState<S> &
                     st.r[(size_t)n * TR + o] = S1;
                     if (wy) yg[(size_t)n * TR + o] = ynew;
                 }
-                for (int l = f.l0; l < f.l1; l++) {
+            }
+            if (nw > 1) __syncthreads();   // every wave's balances read the flows before the update
+            if (t != 0) {
+                for (int l = f.l0 + wv; l < f.l1; l += nw) {
                     S &F1 = st.F[(size_t)tb.ln_line[l] * TR + o];
                     F1 = F1 + t * tb.ln_df[l];
                 }
-                for (int q = tb.th_off[tid]; q < tb.th_off[tid + 1]; q++) {
+                for (int q = tb.th_off[tid] + wv; q < tb.th_off[tid + 1]; q += nw) {
                     S &th = st.v[(size_t)tb.th_node[q] * TR + o];
                     th = th + t * tb.th_val[q];
                 }
             }
+            if (nw > 1) __syncthreads();   // the next move reads the new state
         }
     }
 }
@@ -813,10 +876,23 @@ __global__ void __launch_bounds__(1024) k_step_order(const int *counts, int T, i
     }
 }
 
-template <typename S, bool UNI>
-__global__ void __launch_bounds__(256, 4) k_sparse_all(DevModel<S> m, SweepState<S> st, RandArgs a, SparseAll sa,
-                                                       Tables<S> tb, S beta, S maxnats, S kmin) {
-    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// nw = 1: a thread per (step, replica), four steps per workgroup. nw > 1 (R a multiple of 64): a
+// workgroup of nw waves per (step, 64 replicas), lane = replica, the waves splitting each move's
+// template entries (FSparse's wave split); every wave runs the same walk on the combined values.
+template <typename S, bool UNI, bool SPLIT>
+__global__ void __launch_bounds__(1024) k_sparse_all(DevModel<S> m, SweepState<S> st, RandArgs a, SparseAll sa,
+                                                     Tables<S> tb, S beta, S maxnats, S kmin, int nw_) {
+    extern __shared__ unsigned char sp_lds[];
+    const int nw = SPLIT ? nw_ : 1;   // a compile-time 1 keeps the combining code out of the one-wave form
+    size_t g;
+    int wv = 0;
+    if (!SPLIT) {
+        g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    } else {
+        wv = threadIdx.x >> 6;
+        g = (size_t)blockIdx.x * 64 + (threadIdx.x & 63);
+    }
+    S *red = nw > 1 ? (S *)sp_lds : nullptr;
     MoveOut o[NSPARSE];
     if (g < (size_t)a.ntau * m.R) {
         const int j = (int)(g % m.R);
@@ -824,19 +900,19 @@ __global__ void __launch_bounds__(256, 4) k_sparse_all(DevModel<S> m, SweepState
         RandArgs b = a;
         if (sa.mask & 1u) {
             b.w = W_XPAIR;
-            run_sparse<S, SP_XPAIR, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[0]);
+            run_sparse<S, SP_XPAIR, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[0], wv, nw, red);
         }
         if (sa.mask & 2u) {
             b.w = W_XLOCC;
-            run_sparse<S, SP_XLOCC, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[1]);
+            run_sparse<S, SP_XLOCC, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[1], wv, nw, red);
         }
         if (sa.mask & 4u) {
             b.w = W_XLOC;
-            run_sparse<S, SP_XLOC, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[2]);
+            run_sparse<S, SP_XLOC, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[2], wv, nw, red);
         }
         if (sa.mask & 8u) {
             b.w = W_CELL;
-            run_sparse<S, SP_CELL, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[3]);
+            run_sparse<S, SP_CELL, UNI>(m, st, b, tb, tau, j, beta, maxnats, kmin, o[3], wv, nw, red);
         }
     }
 #pragma unroll
@@ -845,6 +921,14 @@ __global__ void __launch_bounds__(256, 4) k_sparse_all(DevModel<S> m, SweepState
 }
 
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+// waves per (step, 64 replicas) of the merged table-driven launch: set by esm_sweep_launch, else
+// one. Measured (FP32, storage sweeps off, ms per sweep of the merged launch for 1 / 2 / 4 / 8 / 16
+// waves): 40 nodes × 8760 h × 64 replicas 57 / 59 / 55 / 75 / 152; 256 nodes × 2920 h × 64
+// replicas 753 / 1041 / 853 / 780 / 797. The launch is VALU-bound over the whole GPU at 256
+// nodes (every walk pass rescans all ~700 template entries), so more waves add barriers, not
+// throughput.
+static int sparse_waves(const esm_handle *h) { return h->sweep_waves > 0 ? h->sweep_waves : 1; }
 
 template <typename S>
 int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats) {
@@ -972,7 +1056,15 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
             ProfScope p(h, ESM_KERN_RANDOM);
             RandArgs b = a;
             b.perm = order_steps(a, w_sparse);
-            RLAUNCH((k_sparse_all<S, true>), (k_sparse_all<S, false>), m, st, b, sa, tb, beta, mxn, kmin);
+            const int nw = uni ? sparse_waves(h) : 1;
+            if (nw == 1) {
+                RLAUNCH((k_sparse_all<S, true, false>), (k_sparse_all<S, false, false>), m, st, b, sa, tb, beta, mxn, kmin,
+                        1);
+            } else {
+                const size_t lds = (size_t)nw * FSparse<S>::RED * 64 * sizeof(S);
+                hipLaunchKernelGGL((k_sparse_all<S, true, true>), dim3((unsigned)((size_t)a.ntau * R / 64)), dim3(64 * nw),
+                                   lds, h->stream, m, st, b, sa, tb, beta, mxn, kmin, nw);
+            }
             ESM_HIP_CHECK(hipGetLastError());
         }
     }

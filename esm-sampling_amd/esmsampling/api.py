@@ -160,10 +160,13 @@ def storage_long_options(R: Replicas, p: float = 1.0, nmoves: int = 32, hmin: fl
     check(R.lib.esm_storage_long_options(R.handle, float(p), int(nmoves), float(hmin)), "esm_storage_long_options")
 
 
-def sweep_launch(R: Replicas, merged: bool = True):
-    """The table-driven random families of a parity class in one launch (default) or one launch
-    per family (per-family kernel times); states and statistics are identical."""
-    check(R.lib.esm_sweep_launch(R.handle, 1 if merged else 0), "esm_sweep_launch")
+def sweep_launch(R: Replicas, merged=True):
+    """The table-driven random families of a parity class in one launch (True or 1, the default)
+    or one launch per family (False or 0: per-family kernel times; the same states and
+    statistics), or merged with 2, 4, 8 or 16 waves per (step, 64 replicas) splitting every
+    move's template entries (replica counts that are multiples of 64)."""
+    mode = int(merged) if not isinstance(merged, bool) else (1 if merged else 0)
+    check(R.lib.esm_sweep_launch(R.handle, mode), "esm_sweep_launch")
 
 
 def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0, maxnats: float = 40.0):

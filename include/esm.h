@@ -259,8 +259,11 @@ int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double
  * How the random families of an operation sweep are launched (no counterpart in the reference,
  * which runs them on CPU threads): merged = 1 (the default) runs the table-driven families of a
  * step (xpair, xlocc, xloc, cell) in one launch per parity class; merged = 0 launches one kernel
- * per family and parity class (per-family kernel times for esm_profile_read). States and
- * statistics are identical.
+ * per family and parity class (per-family kernel times for esm_profile_read); merged = 2, 4, 8 or
+ * 16 runs the merged launch with that many waves per (step, 64 replicas), which split every
+ * move's template entries (replica counts that are multiples of 64; merged = 1 uses one wave).
+ * States and statistics of 0 and 1 are identical; more waves sum the terms of ΔC in another
+ * order.
  */
 int32_t esm_sweep_launch(esm_handle *h, int32_t merged);
 

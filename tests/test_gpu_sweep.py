@@ -186,6 +186,40 @@ def test_merged_random_launch_equals_one_launch_per_family(prec, R):
     assert np.array_equal(va, vb) and np.array_equal(ra, rb) and np.array_equal(Fa, Fb)
 
 
+@pytest.mark.parametrize("waves", [2, 4])
+def test_sweep_fp64_waves_per_step_matches_oracle(waves):
+    """The merged table-driven launch with several waves per (step, 64 replicas), each taking every
+    waves-th template entry, against the oracle (FP64): the same move statistics, states to 1e-9
+    (the terms of ΔC are summed in another order)."""
+    print(f"sweep fp64 waves per step {waves}", flush=True)
+    s = synth.make_system(N=40, L=70, T=8, dt=1.0, seed=3)
+    R = 64
+    rng = np.random.default_rng(11)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    v[0] = rng.normal(0, 300.0, (s.N, s.T, R))
+    dev = api.Replicas(s, R, FP64)
+    dev.upload(v=v, k=s.k0, x=x)
+    api.sweep_launch(dev, waves)
+    st = api.operation_sweep(dev, 1, key=(7, 3), counter=2)
+    vd, _, _ = dev.download()
+    rd, Fd = dev.caches()
+    v_, r_, F_, st_ = oracle.Oracle(s).sweeps(v, s.k0, 1, key=(7, 3), counter=2, x=x, threads=8)
+    assert np.all(st_[6:10, 0] > 0)
+    assert np.array_equal(st[:, :4], st_[:, :4])
+    np.testing.assert_allclose(vd[1:], v_[1:], atol=1e-9)
+    np.testing.assert_allclose(vd[0], v_[0], atol=1e-9 * np.abs(v_[0]).max())
+    np.testing.assert_allclose(rd, r_, atol=1e-9 * np.abs(s.load).max())
+    np.testing.assert_allclose(Fd, F_, atol=1e-9 * np.abs(F_).max())
+    with pytest.raises(Exception, match="multiple of 64"):
+        small = api.Replicas(s, 3, FP64)
+        try:
+            api.sweep_launch(small, 4)
+        finally:
+            small.close()
+    dev.close()
+
+
 @pytest.mark.parametrize("R,T", [(4, 47), (64, 20), (3, 300)])
 def test_storage_ms_fp64_matches_oracle(R, T):
     """The storage multiscale sweep alone (every block length below T/2, segments of 4 … 64 lanes)."""
