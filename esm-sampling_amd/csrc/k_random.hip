@@ -180,7 +180,7 @@ __device__ __forceinline__ void run_spair(const DevModel<S> &m, SweepState<S> &s
 }
 
 template <typename S, bool UNI>
-__global__ void __launch_bounds__(256) k_rand_spair(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats,
+__global__ void __launch_bounds__(256, 3) k_rand_spair(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats,
                                                     S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
