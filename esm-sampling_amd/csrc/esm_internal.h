@@ -16,6 +16,14 @@ constexpr int ESM_MAXK = 4;       // largest regime count of the weather model
 constexpr int ESM_MAXWIN = 256;   // longest regime window
 constexpr int STAT_SLOTS = 256;   // sweep counters are spread over this many addresses per statistic
 
+// Minimum waves per SIMD asked of a latency-bound sweep kernel (second __launch_bounds__
+// argument). The FP32 instances take the occupancy measured fastest, which caps their registers
+// and accepts a few spilled bytes per lane; the FP64 instances need about twice the registers and
+// would spill hundreds of bytes per lane under the same cap, so they keep the compiler's own
+// allocation.
+template <typename S>
+constexpr int min_waves(int fp32_waves) { return sizeof(S) == 4 ? fp32_waves : 1; }
+
 void set_error(const std::string &msg);
 
 #define ESM_HIP_CHECK(expr)                                                                   \

@@ -180,7 +180,7 @@ __device__ __forceinline__ void run_spair(const DevModel<S> &m, SweepState<S> &s
 }
 
 template <typename S, bool UNI>
-__global__ void __launch_bounds__(256, 3) k_rand_spair(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats,
+__global__ void __launch_bounds__(256, min_waves<S>(3)) k_rand_spair(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats,
                                                     S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
@@ -880,8 +880,9 @@ __global__ void __launch_bounds__(1024) k_step_order(const int *counts, int T, i
 // workgroup of nw waves per (step, 64 replicas), lane = replica, the waves splitting each move's
 // template entries (FSparse's wave split); every wave runs the same walk on the combined values.
 template <typename S, bool UNI, bool SPLIT>
-__global__ void __launch_bounds__(1024) k_sparse_all(DevModel<S> m, SweepState<S> st, RandArgs a, SparseAll sa,
-                                                     Tables<S> tb, S beta, S maxnats, S kmin, int nw_) {
+__global__ void __launch_bounds__(SPLIT ? 1024 : 256, SPLIT ? 1 : min_waves<S>(4))
+    k_sparse_all(DevModel<S> m, SweepState<S> st, RandArgs a, SparseAll sa, Tables<S> tb, S beta, S maxnats, S kmin,
+                 int nw_) {
     extern __shared__ unsigned char sp_lds[];
     const int nw = SPLIT ? nw_ : 1;   // a compile-time 1 keeps the combining code out of the one-wave form
     size_t g;
