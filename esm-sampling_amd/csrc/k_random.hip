@@ -958,6 +958,7 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     // table-driven ones
     const int order[7] = {W_UNIT, W_POT, W_SPAIR, W_XPAIR, W_XLOCC, W_XLOC, W_CELL};
     const bool uni = R % 64 == 0;
+    const int rb = h->rand_block;   // every kernel of RLAUNCH is bounded to 256 threads and counts per wave
     int dmax = 0;
     for (int n = 0; n < N; n++) dmax = std::max(dmax, h->adj_ptr[n + 1] - h->adj_ptr[n]);
     SparseAll sa{};
@@ -971,8 +972,8 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     }
 #define RLAUNCH(KU, KD, ...)                                                                           \
     do {                                                                                               \
-        if (uni) hipLaunchKernelGGL(KU, dim3(nb), dim3(256), 0, h->stream, __VA_ARGS__);                \
-        else hipLaunchKernelGGL(KD, dim3(nb), dim3(256), 0, h->stream, __VA_ARGS__);                    \
+        if (uni) hipLaunchKernelGGL(KU, dim3(nb), dim3(rb), 0, h->stream, __VA_ARGS__);                 \
+        else hipLaunchKernelGGL(KD, dim3(nb), dim3(rb), 0, h->stream, __VA_ARGS__);                     \
     } while (0)
     // the steps of a class heaviest first (when a wave holds one step's replicas: several steps per
     // wave would put the heaviest together in the same waves, measured slower at 16 replicas) for
@@ -994,7 +995,7 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     w_sparse.wk[W_CELL] = 2;
     for (size_t c = 0; c < h->tclass.size(); c++) {
         RandArgs a{k0, k1, ctr, h->tclass[c].first, 2, h->tclass[c].second, 0, h->d_counts, nullptr, nullptr};
-        const unsigned nb = nblk((size_t)a.ntau * R, 256);
+        const unsigned nb = nblk((size_t)a.ntau * R, (unsigned)rb);
         for (int w : order) {
             if (!(h->sweep_w[w] > 0)) continue;
             if (h->sweep_merged && (w == W_XPAIR || w == W_XLOCC || w == W_XLOC || w == W_CELL)) continue;

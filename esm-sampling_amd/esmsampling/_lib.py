@@ -109,11 +109,18 @@ SIGNATURES = [
     ("esm_mlp_propose", C.c_int32, [C.c_void_p, _u32p, C.c_uint32, C.POINTER(C.c_int64), _dp, _dp, _dp]),
     ("esm_profile", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_profile_read", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.POINTER(C.c_int64)]),
+    ("esm_batch_create", C.c_int32, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]),
+    ("esm_batch_destroy", C.c_int32, [C.c_void_p]),
+    ("esm_batch_sweeps", C.c_int32, [C.c_void_p, C.c_int32, _u32p, _u32p, C.c_double, C.POINTER(C.c_int64)]),
+    ("esm_batch_counts", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
+    ("esm_sweep_block", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("esm_autotune", C.c_int32, [C.c_void_p, C.c_int32, _u32p, C.c_uint32, C.c_double, _ip, _dp, _ip]),
 ]
 (KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW,
  KERN_UNIT, KERN_POT, KERN_CELL, KERN_XLOC, KERN_XLOCC, KERN_XPAIR, KERN_SMS, KERN_SLONG, KERN_MLPF, KERN_MLPP,
  KERN_MLPT, KERN_RANDOM) = range(21)
 NFAM, NSTAT, NWEIGHT = 13, 5, 9
+BATCH_MAX = 64
 # the plain random families and their default weights (plain_families, src/local.jl:1912-1915)
 WEIGHT_NAMES = ("unit", "pair", "xpair", "xlocc", "xloc", "potential", "cellpair", "cell", "spair")
 DEFAULT_WEIGHTS = (0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3)

@@ -182,6 +182,106 @@ def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0,
     return st.reshape(_lib.NFAM, _lib.NSTAT)
 
 
+def sweep_block(R: Replicas, block: int = 256):
+    """Workgroup size of the one-wave random launches (64, 128 or 256); states and statistics do
+    not depend on it."""
+    check(R.lib.esm_sweep_block(R.handle, int(block)), "esm_sweep_block")
+
+
+AUTOTUNE_FORMS = (0, 1, 2, 4)
+AUTOTUNE_BLOCKS = (64, 128, 256)
+
+
+def _shape_key(R: Replicas) -> str:
+    return f"N{R.N}_L{R.L}_T{R.T}_R{R.R}_fp{R.precision}"
+
+
+def autotune(R: Replicas, nsweeps: int = 1, key=(7, 7), counter: int = 0, maxnats: float = 40.0,
+             cache: str | None = None):
+    """Pick the launch form of the random families (``sweep_launch``) and their workgroup size
+    (``sweep_block``) for this shape by timing every candidate on the device (``esm_autotune``:
+    each candidate runs ``nsweeps`` sweeps from the current state, which is restored afterwards,
+    so the chain does not move). With ``cache`` (a JSON file), a shape tuned before takes the
+    stored choice without timing, and a new shape's choice is added to the file. Returns
+    {"form", "block", "ms": {(form, block): ms per sweep}, "cached"}."""
+    import json
+    import os
+    sk = _shape_key(R)
+    table = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            table = json.load(f)
+        if sk in table:
+            form, block = int(table[sk]["form"]), int(table[sk]["block"])
+            sweep_launch(R, form)
+            sweep_block(R, block)
+            return {"form": form, "block": block, "ms": {}, "cached": True}
+    best = (C.c_int32 * 2)()
+    ms = (C.c_double * 12)()
+    nc = C.c_int32()
+    check(R.lib.esm_autotune(R.handle, int(nsweeps), _key(key), int(counter) & 0xFFFFFFFF, float(maxnats), best, ms,
+                             C.byref(nc)), "esm_autotune")
+    times = {(AUTOTUNE_FORMS[c // 3], AUTOTUNE_BLOCKS[c % 3]): float(ms[c]) for c in range(nc.value)}
+    out = {"form": int(best[0]), "block": int(best[1]), "ms": times, "cached": False}
+    if cache:
+        table[sk] = {"form": out["form"], "block": out["block"],
+                     "ms": {f"{f}/{b}": t for (f, b), t in times.items()}}
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, cache)
+    return out
+
+
+class Batch:
+    """Replica sets of uneven size (network, horizon, replica count, precision) swept together as
+    one HIP graph launch (``esm_batch_sweeps``): the same kernels, arguments and results as
+    ``operation_sweep`` on each set, run as concurrent branches of one graph. The sets must not
+    be used by other threads during a batched sweep; kernel profiling must be off on them."""
+
+    def __init__(self, sets):
+        self.sets = list(sets)
+        if not 1 <= len(self.sets) <= _lib.BATCH_MAX:
+            raise ValueError(f"a batch holds 1 to {_lib.BATCH_MAX} replica sets")
+        self.lib = self.sets[0].lib
+        hs = (C.c_void_p * len(self.sets))(*[r.handle.value for r in self.sets])
+        b = C.c_void_p()
+        check(self.lib.esm_batch_create(hs, len(self.sets), C.byref(b)), "esm_batch_create")
+        self._b = b
+
+    def sweeps(self, nsweeps: int = 1, keys=None, counters=None, maxnats: float = 40.0):
+        """``operation_sweep`` of every set (set i under keys[i] and counters[i]); returns the
+        statistics [n sets, 13 families, 5]."""
+        n = len(self.sets)
+        keys = [(1, 0)] * n if keys is None else list(keys)
+        counters = [0] * n if counters is None else list(counters)
+        if len(keys) != n or len(counters) != n:
+            raise ValueError("one key and one counter per replica set")
+        kk = (C.c_uint32 * (2 * n))(*[int(x) & 0xFFFFFFFF for k in keys for x in k])
+        cc = (C.c_uint32 * n)(*[int(c) & 0xFFFFFFFF for c in counters])
+        st = np.zeros(n * _lib.NFAM * _lib.NSTAT, dtype=np.int64)
+        check(self.lib.esm_batch_sweeps(self._b, int(nsweeps), kk, cc, float(maxnats),
+                                        st.ctypes.data_as(C.POINTER(C.c_int64))), "esm_batch_sweeps")
+        return st.reshape(n, _lib.NFAM, _lib.NSTAT)
+
+    def counts(self):
+        """(graph launches, instantiations, in-place updates of the executable graph)."""
+        c = (C.c_int64 * 3)()
+        check(self.lib.esm_batch_counts(self._b, c), "esm_batch_counts")
+        return tuple(int(x) for x in c)
+
+    def close(self):
+        if self._b:
+            self.lib.esm_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def follow_capacities(R: Replicas, knew, kmin: float = 1.0):
     """``follow_capacities!`` (src/replicas.jl:448-472): move the shared capacities to ``knew`` [6, N]
     with the dispatch following on every replica; returns the committed capacities [6, N]."""

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 4
+#define ESM_ABI_VERSION 5
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -396,6 +396,45 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_NKERN 21
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
+
+/*
+ * Batched sweeps and the launch autotuner (no counterpart in the reference, whose chains run on
+ * CPU threads; the batching extension of this library).
+ *
+ * A batch holds up to ESM_BATCH_MAX handles on one device, of any network size, horizon,
+ * replica count and precision. esm_batch_sweeps runs esm_sweeps on every handle, with its own
+ * key (keys[2i], keys[2i+1]) and counter (counters[i]), as ONE HIP graph: the launches of each
+ * handle are captured on its own stream, forked from and joined into the first handle's stream,
+ * so the replica sets run as concurrent branches of one graph launch. The states, caches and
+ * statistics are exactly those of esm_sweeps on each handle (the same kernels, arguments and
+ * order). stats ([n][ESM_NFAM][ESM_NSTAT], may be NULL) as esm_sweeps. The executable graph is
+ * kept and updated in place on later calls (instantiated again when the launch sequence
+ * changes); esm_batch_counts gives (graph launches, instantiations, in-place updates). The
+ * handles must not be used by other threads during the call, and kernel profiling
+ * (esm_profile) must be off on them. Destroying a batch leaves its handles alone.
+ *
+ * esm_sweep_block sets the workgroup size of the one-wave random launches (64, 128 or 256, the
+ * default); states and statistics do not depend on it.
+ *
+ * esm_autotune picks the launch form of the random families for this handle's shape by timing:
+ * candidates c = 3·f + b over the forms f of esm_sweep_launch (0, 1, and 2, 4 when R is a multiple
+ * of 64) and the blocks b (64, 128, 256), each running `nsweeps` sweeps (1 … 64) under `key` and
+ * `counter` from the handle's current state (after one untimed sweep), timed with HIP events.
+ * The state is restored afterwards: the call leaves v, the caches and the chain unchanged (the
+ * statistics of esm_sweeps are overwritten). Sets the fastest candidate on the handle and returns
+ * it in best[0] (form) and best[1] (block); ms[c] (may be NULL; 12 entries suffice) the mean ms
+ * per sweep of each candidate, *ncand (may be NULL) their number.
+ */
+#define ESM_BATCH_MAX 64
+typedef struct esm_batch esm_batch;
+int32_t esm_batch_create(esm_handle *const *handles, int32_t n, esm_batch **out);
+int32_t esm_batch_destroy(esm_batch *b);
+int32_t esm_batch_sweeps(esm_batch *b, int32_t nsweeps, const uint32_t *keys, const uint32_t *counters, double maxnats,
+                         int64_t *stats);
+int32_t esm_batch_counts(const esm_batch *b, int64_t *counts);
+int32_t esm_sweep_block(esm_handle *h, int32_t block);
+int32_t esm_autotune(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
+                     int32_t *best, double *ms, int32_t *ncand);
 
 #ifdef __cplusplus
 }

@@ -32,8 +32,21 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_without_gpu_work():
     from esmsampling import _lib
     lib = _lib.load()
-    assert lib.esm_version() == 4
+    assert lib.esm_version() == 5
     # an invalid argument is rejected on the host, before any device call
     h = C.c_void_p()
     assert lib.esm_create(None, 1, 64, C.byref(h)) == -1
     assert b"null" in lib.esm_last_error()
+
+
+def test_batch_and_autotune_reject_null_arguments_without_gpu_work():
+    from esmsampling import _lib
+    lib = _lib.load()
+    out = C.c_void_p()
+    assert lib.esm_batch_create(None, 1, C.byref(out)) == -1 and b"null" in lib.esm_last_error()
+    assert lib.esm_batch_sweeps(None, 1, None, None, 40.0, None) == -1
+    assert lib.esm_batch_counts(None, None) == -1
+    assert lib.esm_batch_destroy(None) == 0
+    assert lib.esm_sweep_block(None, 128) == -1
+    assert lib.esm_autotune(None, 1, None, 0, 40.0, None, None, None) == -1
+    assert _lib.BATCH_MAX == 64

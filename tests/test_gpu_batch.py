@@ -1,0 +1,119 @@
+"""Batched sweeps (one HIP graph over replica sets of uneven network, horizon, replica count and
+precision) against the same sweeps run eagerly on twin sets, replay after replay; and the launch
+autotuner: it leaves the chain where it was, and its choice gives the states of that launch form."""
+import json
+
+import numpy as np
+import pytest
+
+from esmsampling import FP32, FP64, api, synth
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [  # (N, L, T, R, precision)
+    (6, 8, 10, 3, FP64),
+    (5, 6, 12, 2, FP64),
+    (8, 12, 8, 64, FP32),
+]
+
+
+def _make(N, L, T, R, prec, seed):
+    s = synth.make_system(N=N, L=L, T=T, dt=1.0, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    v[0] = rng.normal(0, 200.0, (s.N, s.T, R))
+    pair = []
+    for _ in range(2):
+        d = api.Replicas(s, R, prec)
+        d.upload(v=v, k=s.k0, x=x)
+        pair.append(d)
+    return pair
+
+
+def _same(a, b):
+    va, _, _ = a.download()
+    vb, _, _ = b.download()
+    ra, Fa = a.caches()
+    rb, Fb = b.caches()
+    return np.array_equal(va, vb) and np.array_equal(ra, rb) and np.array_equal(Fa, Fb)
+
+
+def test_batch_graph_matches_eager_sweeps():
+    print("batch graph vs eager, 3 uneven sets", flush=True)
+    sets = [_make(*sh, seed=11 + i) for i, sh in enumerate(SHAPES)]
+    batched = [p[0] for p in sets]
+    eager = [p[1] for p in sets]
+    for d in batched + eager:
+        api.sweep_options(d, p_storage_ms=1.0)
+    b = api.Batch(batched)
+    keys = [(3, 4), (5, 6), (7, 8)]
+    # replay 1: every storage sweep on; 2: set 0 without the multiscale sweep (another launch
+    # sequence: a new instantiation); 3 and 4: new counters on the same sequence (in-place updates)
+    plan = [(1.0, 0), (0.0, 1), (0.0, 2), (0.0, 3)]
+    for rep, (p0, ctr) in enumerate(plan):
+        api.sweep_options(batched[0], p_storage_ms=p0)
+        api.sweep_options(eager[0], p_storage_ms=p0)
+        counters = [10 * ctr + i for i in range(3)]
+        st = b.sweeps(1, keys=keys, counters=counters)
+        for i, d in enumerate(eager):
+            st_e = api.operation_sweep(d, 1, key=keys[i], counter=counters[i])
+            assert np.array_equal(st[i], st_e), f"replay {rep}, set {i}: statistics"
+            assert _same(batched[i], d), f"replay {rep}, set {i}: states or caches"
+        # eager work of the program between the replays (on both twins: the energy rebuilds the
+        # caches of set 1 from the state)
+        api.energy_cost(batched[1])
+        api.energy_cost(eager[1])
+        batched[2].download()
+    launches, inst, upd = b.counts()
+    assert launches == len(plan) and inst >= 2 and upd >= 1 and inst + upd == launches
+    b.close()
+    for d in batched + eager:
+        d.close()
+
+
+def test_batch_rejects_bad_use():
+    print("batch argument checks", flush=True)
+    (a, a2), = [_make(5, 6, 8, 2, FP64, seed=3)]
+    with pytest.raises(Exception, match="twice"):
+        api.Batch([a, a])
+    b = api.Batch([a, a2])
+    with pytest.raises(ValueError):
+        b.sweeps(1, keys=[(1, 1)], counters=[0, 0])
+    a.lib.esm_profile(a.handle, 1)
+    with pytest.raises(Exception, match="profiling"):
+        b.sweeps(1)
+    a.lib.esm_profile(a.handle, 0)
+    b.sweeps(1)
+    b.close()
+    a.close()
+    a2.close()
+
+
+@pytest.mark.parametrize("R,prec", [(64, FP32), (3, FP64)])
+def test_autotune_keeps_the_chain_and_its_choice_reproduces(R, prec, tmp_path):
+    print(f"autotune R={R} prec={prec}", flush=True)
+    d, twin = _make(8, 12, 8, R, prec, seed=21)
+    api.sweep_options(d, p_storage_ms=1.0)
+    api.sweep_options(twin, p_storage_ms=1.0)
+    v0, _, _ = d.download()
+    r0, F0 = d.caches()
+    cache = str(tmp_path / "tune.json")
+    res = api.autotune(d, nsweeps=1, key=(9, 9), counter=5, cache=cache)
+    ncand = 12 if R % 64 == 0 else 6
+    assert len(res["ms"]) == ncand and all(t > 0 for t in res["ms"].values())
+    assert (res["form"], res["block"]) == min(res["ms"], key=res["ms"].get)
+    v1, _, _ = d.download()
+    r1, F1 = d.caches()
+    assert np.array_equal(v0, v1) and np.array_equal(r0, r1) and np.array_equal(F0, F1)
+    # the twin runs the chosen form with the default block: the block does not change results
+    api.sweep_launch(twin, res["form"])
+    st = api.operation_sweep(d, 1, key=(1, 2), counter=0)
+    st_t = api.operation_sweep(twin, 1, key=(1, 2), counter=0)
+    assert np.array_equal(st, st_t) and _same(d, twin)
+    # the cache gives the same choice without timing
+    again = api.autotune(twin, cache=cache)
+    assert again["cached"] and (again["form"], again["block"]) == (res["form"], res["block"])
+    assert list(json.load(open(cache))) == [api._shape_key(d)]
+    d.close()
+    twin.close()
