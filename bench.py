@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
     ap.add_argument("--p-storage-ms", type=float, default=0.5,
                     help="probability of the storage multiscale sweep per operation sweep (the reference's 1/2)")
+    ap.add_argument("--tune", choices=("table", "live", "off"), default="table",
+                    help="launch form of the random families: from the committed tuning table when it "
+                         "holds this shape (default), tuned on the device now, or the library default")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="first run a plain device copy of 1 GiB three times (the calibration kernel of the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -171,6 +175,7 @@ def main():
     T, R = args.hours, args.replicas
     wl = Workload(args.nodes, args.lines, T, R, prec)
     api.sweep_options(wl.dev, p_storage_ms=args.p_storage_ms)
+    launch = apply_tuning(api, wl.dev, args.tune)
     lib, h = wl.dev.lib, wl.dev.handle
     for it in range(args.warmup):
         wl.step(it)
@@ -203,7 +208,7 @@ def main():
     kern_ms = tot_ms / max(cnt, 1)
     achieved = kb[dom] / (kern_ms * 1e-3) / 1e9
     breakdown = {k: round(v[0] / args.steps, 4) for k, v in prof.items()}
-    traffic = measured_traffic(dom, wl.s, T, R, prec)
+    traffic = measured_traffic(dom, wl.s, T, R, prec, launch["form"])
     line_moves = int(wl.last_stats[:, 0].sum())   # exact line moves of one sweep (atoms and random families)
     out = {
         "metric": "sweeps/s (replica steps/s of the plain GPU replica sampler)",
@@ -221,7 +226,8 @@ def main():
         "config": {"workload": f"replica sampler step, {wl.s.N} nodes / {wl.s.L} lines, {T} hourly steps, "
                                f"{R} weather replicas (weather + operation sweep + energy + score + Langevin + "
                                f"following)", "replicas": R, "hours": T, "nodes": wl.s.N, "lines": wl.s.L,
-                   "line_moves_per_step": line_moves, "line_moves_per_s": line_moves * value},
+                   "line_moves_per_step": line_moves, "line_moves_per_s": line_moves * value,
+                   "launch": launch},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms": kern_ms,
                      "bytes_per_launch": kb[dom], "launches": cnt},
@@ -236,6 +242,8 @@ def main():
     wl.dev.close()
     if not args.no_learned:
         out["learned"] = learned_leg(args, prec)
+    if not args.no_batch:
+        out["batch"] = batch_leg(prec)
     if args.ess:
         out["ess"] = ess_leg(args)
     print(json.dumps(out), flush=True)
@@ -330,7 +338,68 @@ def learned_leg(args, prec, R=128, K=8, B=24, train_steps=300, sweeps=3):
                      "kernel_ms_mean": fwd_ms / max(launches, 1)}}
 
 
-def measured_traffic(kernel, s, T, R, prec):
+def apply_tuning(api, dev, mode):
+    """The launch form and workgroup size of the random families: from the committed tuning table
+    (``esmsampling/tuning.json``, written by ``python -m esmsampling.tune`` on the GPU) when it
+    holds this shape, tuned now on the device (``live``), or the library default."""
+    from esmsampling import tune
+    if mode == "table" and tune.TABLE.exists():
+        table = json.loads(tune.TABLE.read_text())
+        if api._shape_key(dev) in table:
+            r = api.autotune(dev, cache=str(tune.TABLE))
+            return {"form": r["form"], "block": r["block"], "source": "tuning table"}
+    if mode == "live":
+        r = api.autotune(dev, nsweeps=1)
+        return {"form": r["form"], "block": r["block"], "source": "tuned now",
+                "ms_per_sweep": {f"{f}/{b}": round(t, 2) for (f, b), t in r["ms"].items()}}
+    return {"form": 1, "block": 256, "source": "library default"}
+
+
+def batch_leg(prec, reps=2):
+    """Uneven batch (40 and 256 nodes, 2920 and 8760 h, 256 replicas in all): one operation sweep
+    of every replica set as one captured HIP graph (``api.Batch``) against the same sweeps run
+    one set after the other, each set at its tuned launch form."""
+    from esmsampling import api, synth
+    shapes = [(40, 70, 2920, 96), (40, 70, 8760, 64), (256, 450, 2920, 64), (256, 450, 2920, 32)]
+    sets = []
+    for i, (N, L, T, R) in enumerate(shapes):
+        s = synth.make_system(N=N, L=L, T=T, dt=1.0, seed=1 + i)
+        d = api.Replicas(s, R, prec)
+        api.sample_weather(d, key=(1, i))
+        d.upload(v=synth.heuristic_state(s, d.download()[2]), k=s.k0)
+        api.sweep_options(d, p_storage_ms=0.0)
+        apply_tuning(api, d, "table")
+        sets.append(d)
+    keys = [(2, i) for i in range(len(sets))]
+    b = api.Batch(sets)
+
+    def one_by_one(c):
+        for i, d in enumerate(sets):
+            api.operation_sweep(d, 1, key=keys[i], counter=c)
+
+    def graph(c):
+        b.sweeps(1, keys=keys, counters=[c] * len(sets))
+
+    res = {}
+    for k, (name, f) in enumerate((("one_by_one", one_by_one), ("graph", graph))):
+        f(100 * k)
+        t0 = time.perf_counter()
+        for c in range(reps):
+            f(100 * k + 1 + c)
+        res[name] = 1e3 * (time.perf_counter() - t0) / reps
+        print(f"batch leg: {name} {res[name]:.1f} ms per batched sweep", file=sys.stderr, flush=True)
+    counts = b.counts()
+    b.close()
+    for d in sets:
+        d.close()
+    return {"workload": "one operation sweep of 4 replica sets: 40 n x 2920 h x 96 r, 40 n x 8760 h x 64 r, "
+                        "256 n x 2920 h x 64 r, 256 n x 2920 h x 32 r (storage multiscale sweep off, "
+                        "long-range storage sweep on)",
+            "ms_one_by_one": res["one_by_one"], "ms_graph": res["graph"],
+            "speedup": res["one_by_one"] / res["graph"], "graph_launches_instantiations_updates": counts}
+
+
+def measured_traffic(kernel, s, T, R, prec, form=1):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (calibrated against a plain copy), when it was measured on this exact workload."""
     from esmsampling import FP32
@@ -342,7 +411,8 @@ def measured_traffic(kernel, s, T, R, prec):
              "k_unit": "k_rand_unit<float, true>", "k_cell": "k_sparse<float, 0, true>",
              "k_xloc": "k_sparse<float, 1, true>", "k_xlocc": "k_sparse<float, 2, true>",
              "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms",
-             "k_storage_long": "k_storage_long<float, true>", "k_random": "k_sparse_all<float, true>"}
+             "k_storage_long": "k_storage_long<float, true>",
+             "k_random": f"k_sparse_all<float, true, {'true' if form > 1 else 'false'}>"}
     return json.loads(p.read_text())["bytes_per_dispatch"].get(names.get(kernel, ""))
 
 

@@ -197,12 +197,12 @@ def _shape_key(R: Replicas) -> str:
 
 
 def autotune(R: Replicas, nsweeps: int = 1, key=(7, 7), counter: int = 0, maxnats: float = 40.0,
-             cache: str | None = None):
+             cache: str | None = None, retune: bool = False):
     """Pick the launch form of the random families (``sweep_launch``) and their workgroup size
     (``sweep_block``) for this shape by timing every candidate on the device (``esm_autotune``:
     each candidate runs ``nsweeps`` sweeps from the current state, which is restored afterwards,
     so the chain does not move). With ``cache`` (a JSON file), a shape tuned before takes the
-    stored choice without timing, and a new shape's choice is added to the file. Returns
+    stored choice without timing (unless ``retune``), and a new choice is written to the file. Returns
     {"form", "block", "ms": {(form, block): ms per sweep}, "cached"}."""
     import json
     import os
@@ -211,7 +211,7 @@ def autotune(R: Replicas, nsweeps: int = 1, key=(7, 7), counter: int = 0, maxnat
     if cache and os.path.exists(cache):
         with open(cache) as f:
             table = json.load(f)
-        if sk in table:
+        if sk in table and not retune:
             form, block = int(table[sk]["form"]), int(table[sk]["block"])
             sweep_launch(R, form)
             sweep_block(R, block)

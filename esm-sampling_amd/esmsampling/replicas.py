@@ -25,6 +25,7 @@ numpy Philox generator per chain.
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 import os
 import time
@@ -128,6 +129,10 @@ class Config:
     resolution_hours: float = 1.0
     system_seed: int = 1
     precision: str = "fp32"
+    # launch form of the random families (no counterpart in the reference): "table" takes the
+    # committed tuning table's choice for the replica sets' shape (esmsampling/tuning.json),
+    # "off" the library default
+    launch_tuning: str = "table"
 
     def check(self):
         if self.sampler != "replicas":
@@ -147,6 +152,8 @@ class Config:
                                           and self.learned_lr > 0):
             raise ValueError("learned_*: block hours in [1, min(24, snapshots)], components in [1, 8], batch in "
                              "[1, 65536], lr > 0, counts >= 0")
+        if self.launch_tuning not in ("table", "off"):
+            raise ValueError("launch_tuning must be 'table' or 'off'")
         if self.precision not in ("fp32", "fp64"):
             raise ValueError('precision must be "fp32" or "fp64"')
         if self.replicas < 1 or self.nchains < 1:
@@ -207,6 +214,7 @@ class ReplicaCtl:
     plong: float = 1.0
     long_moves: int = 32
     long_hours: float = 24.0
+    tuning: str = "table"
     shrink: float = 0.7
     grow: float = 1.2
 
@@ -224,11 +232,28 @@ class ReplicaCtl:
                    probe_tail=cfg.replica_probe_tail, probe_win=cfg.replica_probe_win, probe_avg=cfg.replica_probe_avg,
                    adapt=cfg.langevin_adapt, gap_hi=cfg.langevin_gap_hi, gap_lo=cfg.langevin_gap_lo,
                    eta_min=cfg.langevin_eta_min, eta_max=cfg.langevin_eta_max, plong=cfg.storage_long,
-                   long_moves=cfg.storage_long_moves, long_hours=cfg.storage_long_hours)
+                   long_moves=cfg.storage_long_moves, long_hours=cfg.storage_long_hours,
+                   tuning=cfg.launch_tuning)
 
     def sweep_options(self, dev):
-        """The operation sweep's options of a replica set (the long-range storage sweep's keys)."""
+        """The operation sweep's options of a replica set (the long-range storage sweep's keys) and
+        its launch form (from the tuning table when it holds the set's shape)."""
         api.storage_long_options(dev, p=self.plong, nmoves=self.long_moves, hmin=self.long_hours)
+        if self.tuning == "table":
+            from . import tune
+            table = _tuning_table(str(tune.TABLE))
+            if api._shape_key(dev) in table:
+                api.autotune(dev, cache=str(tune.TABLE))
+
+
+@functools.lru_cache(maxsize=4)
+def _tuning_table(path: str) -> dict:
+    import json
+    import os
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
 
 
 def interior_nodes(k, khi, lo: float = 50.0):
