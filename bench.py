@@ -48,6 +48,8 @@ def parse():
                     help="also run the replica driver with and without the learned proposal and report the ESS/s "
                          "of total system cost of both (minutes)")
     ap.add_argument("--ess-steps", type=int, default=240, help="replica steps of each --ess run (1/6 warmup)")
+    ap.add_argument("--ess-batch", choices=("threads", "graph"), default="graph",
+                    help="how the learned run of --ess runs its chains' sweeps (the plain run: threads)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
     ap.add_argument("--p-storage-ms", type=float, default=0.5,
                     help="probability of the storage multiscale sweep per operation sweep (the reference's 1/2)")
@@ -253,7 +255,9 @@ def ess_leg(args, nchains=2, R=16, T=2920):
     """ESS/s of total system cost (``ess_bulk`` of the replica-mean total, src/convergence.jl:111,
     over the production steps per second of production wall time) of the replica driver, plain and
     with the learned proposal (training before the first step and in the warmup; its time counted
-    in ``ess_per_s_with_training``), on 2 chains × 16 replicas of the 40-node system at T hours."""
+    in ``ess_per_s_with_training``), on 2 chains × 16 replicas of the 40-node system at T hours.
+    The plain run sweeps its chains from host threads; the learned run as ``--ess-batch`` says
+    (by default one captured graph per step: the learned + batched sampler)."""
     from esmsampling import replicas as RP
     nw = max(args.ess_steps // 6, 2)
     base = dict(nchains=nchains, replicas=R, mlda_warmup=nw, nsamples=args.ess_steps - nw, replica_init_sweeps=10,
@@ -265,7 +269,8 @@ def ess_leg(args, nchains=2, R=16, T=2920):
                                     "learned_train_steps": 10}))
     res = {}
     for name, extra in (("plain", {}), ("learned", dict(learned_proposal=True, learned_train_steps=1000,
-                                                         learned_train_every=10, learned_train_more=100))):
+                                                         learned_train_every=10, learned_train_more=100,
+                                                         sweep_batch=args.ess_batch))):
         r = RP.run_replicas(RP.load_config({**base, **extra}),
                             log=lambda m: print(m, file=sys.stderr, flush=True))
         tot = r["summary"].get("total", {})
@@ -275,7 +280,8 @@ def ess_leg(args, nchains=2, R=16, T=2920):
                      "learned": r["learned"]}
     p, l = res["plain"]["ess_per_s"], res["learned"]["ess_per_s_with_training"]
     res["ratio_learned_vs_plain"] = None if not p or l is None else l / p
-    res["config"] = {"nchains": nchains, "replicas": R, "hours": T, "steps": args.ess_steps, "warmup": nw}
+    res["config"] = {"nchains": nchains, "replicas": R, "hours": T, "steps": args.ess_steps, "warmup": nw,
+                     "learned_sweep_batch": args.ess_batch}
     return res
 
 

@@ -6,9 +6,10 @@
 // launches on its own stream (the statistics cleared by a kernel, not a memset node), and a
 // join of every stream back into the first: the graph holds one branch per replica set, whose
 // kernels are exactly those of esm_sweeps, in the same order, with the same arguments. The graph
-// is instantiated once and later sweeps update the executable graph in place
-// (hipGraphExecUpdate: same topology, new kernel arguments); a sweep whose launch sequence
-// differs (the storage sweeps run with their own probabilities) re-instantiates it.
+// is instantiated once; a later sweep whose kernel nodes have the same functions, grids,
+// workgroups and LDS sizes in the same order updates the executable graph in place
+// (hipGraphExecUpdate: new kernel arguments only), any other sweep (the storage sweeps run with
+// their own probabilities and draw their block lengths' order per sweep) instantiates it again.
 //
 // esm_autotune times the launch forms of the random families on the handle's own state: it saves
 // v and the caches, runs each candidate's sweeps from that saved state with the same key and
@@ -51,12 +52,36 @@ int sweeps_on_stream(esm_handle *h, int nsweeps, const uint32_t key[2], uint32_t
 
 }  // namespace
 
+// The launch shape of every node of a captured graph, in node order: type, and for kernels the
+// function, grid, workgroup and dynamic LDS size. Two graphs with equal signatures differ only in
+// kernel arguments.
+static bool graph_signature(hipGraph_t g, std::vector<uintptr_t> &sig) {
+    sig.clear();
+    size_t n = 0;
+    if (hipGraphGetNodes(g, nullptr, &n) != hipSuccess) return false;
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n && hipGraphGetNodes(g, nodes.data(), &n) != hipSuccess) return false;
+    for (size_t i = 0; i < n; i++) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nodes[i], &t) != hipSuccess) return false;
+        sig.push_back((uintptr_t)t);
+        if (t == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams kp{};
+            if (hipGraphKernelNodeGetParams(nodes[i], &kp) != hipSuccess) return false;
+            sig.insert(sig.end(), {(uintptr_t)kp.func, kp.gridDim.x, kp.gridDim.y, kp.gridDim.z, kp.blockDim.x,
+                                   kp.blockDim.y, kp.blockDim.z, kp.sharedMemBytes});
+        }
+    }
+    return true;
+}
+
 struct esm_batch {
     std::vector<esm_handle *> hs;
     int device = 0;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
     hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> sig;   // the signature of the graph the executable was made from
     int64_t counts[3] = {0, 0, 0};   // graph launches, instantiations, in-place updates
 };
 
@@ -133,11 +158,14 @@ int32_t esm_batch_sweeps(esm_batch *b, int32_t nsweeps, const uint32_t *keys, co
         if (!rc) set_error(std::string("esm_batch_sweeps: hipStreamEndCapture: ") + hipGetErrorString(ec));
         return rc ? rc : ESM_EHIP;
     }
+    std::vector<uintptr_t> sig;
+    const bool have_sig = graph_signature(g, sig);
     bool updated = false;
     if (b->exec) {
         hipGraphNode_t bad = nullptr;
         hipGraphExecUpdateResult res;
-        updated = hipGraphExecUpdate(b->exec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        updated = have_sig && sig == b->sig && hipGraphExecUpdate(b->exec, g, &bad, &res) == hipSuccess &&
+                  res == hipGraphExecUpdateSuccess;
         if (!updated) {
             (void)hipGetLastError();
             (void)hipGraphExecDestroy(b->exec);
@@ -152,6 +180,7 @@ int32_t esm_batch_sweeps(esm_batch *b, int32_t nsweeps, const uint32_t *keys, co
             set_error(std::string("esm_batch_sweeps: hipGraphInstantiate: ") + hipGetErrorString(ei));
             return ESM_EHIP;
         }
+        b->sig = have_sig ? sig : std::vector<uintptr_t>();
         b->counts[1]++;
     } else {
         b->counts[2]++;

@@ -133,6 +133,10 @@ class Config:
     # committed tuning table's choice for the replica sets' shape (esmsampling/tuning.json),
     # "off" the library default
     launch_tuning: str = "table"
+    # how the chains' operation sweeps run (no counterpart in the reference): "threads", each
+    # chain's sweep from its own host thread, or "graph", the sweeps of all chains of a step as
+    # one captured HIP graph (api.Batch; the same states)
+    sweep_batch: str = "threads"
 
     def check(self):
         if self.sampler != "replicas":
@@ -152,6 +156,8 @@ class Config:
                                           and self.learned_lr > 0):
             raise ValueError("learned_*: block hours in [1, min(24, snapshots)], components in [1, 8], batch in "
                              "[1, 65536], lr > 0, counts >= 0")
+        if self.sweep_batch not in ("threads", "graph"):
+            raise ValueError("sweep_batch must be 'threads' or 'graph'")
         if self.launch_tuning not in ("table", "off"):
             raise ValueError("launch_tuning must be 'table' or 'off'")
         if self.precision not in ("fp32", "fp64"):
@@ -371,14 +377,20 @@ class Learned:
         return att, acc
 
 
-def _sweep_scores(R: ReplicaSet, nsweeps: int, key, counter: int, maxnats: float, r: float, learned=None):
+def _sweep_scores(R: ReplicaSet, nsweeps: int, key, counter: int, maxnats: float, r: float, learned=None,
+                  sweeper=None):
     """``nsweeps`` operation sweeps (each followed by the learned moves, when there are), after
     each the energy (caches rebuilt) and the capacity scores; returns (mean of r·ŝ over the sweeps
-    [R, 6, N], fk, parts of the last sweep, stats, learned (attempted, accepted))."""
+    [R, 6, N], fk, parts of the last sweep, stats, learned (attempted, accepted)). ``sweeper(dev,
+    key, counter, maxnats)`` runs one sweep and returns its statistics (``api.operation_sweep``
+    when None)."""
     acc, fk, parts, stats = None, None, None, np.zeros((len(api.FAMILY_NAMES), 5), dtype=np.int64)
     lst = np.zeros(2, dtype=np.int64)
     for q in range(nsweeps):
-        stats += api.operation_sweep(R.dev, 1, key=key, counter=counter + q, maxnats=maxnats)
+        if sweeper is None:
+            stats += api.operation_sweep(R.dev, 1, key=key, counter=counter + q, maxnats=maxnats)
+        else:
+            stats += sweeper(R.dev, key, counter + q, maxnats)
         if learned is not None:
             lst += learned.sweep(counter + q)
         parts = api.energy_cost(R.dev)
@@ -388,7 +400,7 @@ def _sweep_scores(R: ReplicaSet, nsweeps: int, key, counter: int, maxnats: float
 
 
 def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, maxnats: float = 40.0,
-                 learned: Learned | None = None):
+                 learned: Learned | None = None, sweeper=None):
     """``replica_step!`` (src/replicas.jl:676-705) of one chain: the weather step of every replica
     (scheduled angle, regime window with a burst, compensated re-dispatch), ``inner`` operation
     sweeps with the mean capacity score, the gates, the drift weights, one Langevin step of the
@@ -418,7 +430,8 @@ def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, 
             R.since[j] += 1
     Kraw = R.k.copy()
     mask = interior_nodes(Kraw, sys.khi)
-    rsk, fk, parts, stats, lst = _sweep_scores(R, ctl.inner, keys.sweep, it * ctl.inner, maxnats, r, learned)
+    rsk, fk, parts, stats, lst = _sweep_scores(R, ctl.inner, keys.sweep, it * ctl.inner, maxnats, r, learned,
+                                               sweeper)
     fkm = masked_means(fk, mask)
     for j in range(M):
         update_gate(R, j, masked_means(rsk[j], mask) / fkm, ctl)
@@ -516,6 +529,43 @@ def chain_start_capacities(sys, kind: str):
     return np.clip(sys.k0 * f, sys.klo, sys.khi)
 
 
+class BatchedSweeps:
+    """The operation sweeps of all chains of a step as one HIP graph (``api.Batch``): every chain's
+    thread hands its sweep (key, counter) to ``sweep`` and waits at a barrier; the last one to
+    arrive runs the batched sweep for all, while the others wait, so no handle is used by two
+    threads at once. A chain that fails aborts the barrier, which ends the other chains' waits."""
+
+    def __init__(self, devs):
+        import threading
+        self.devs = list(devs)
+        self.index = {id(d): c for c, d in enumerate(self.devs)}
+        self.batch = api.Batch(self.devs)
+        self.req = [None] * len(self.devs)
+        self.out = [None] * len(self.devs)
+        self.barrier = threading.Barrier(len(self.devs), action=self._run)
+
+    def _run(self):
+        keys = [r[0] for r in self.req]
+        counters = [r[1] for r in self.req]
+        if len({r[2] for r in self.req}) != 1:
+            raise ValueError("the chains of a batched sweep use one maxnats")
+        st = self.batch.sweeps(1, keys=keys, counters=counters, maxnats=self.req[0][2])
+        for c in range(len(self.devs)):
+            self.out[c] = st[c]
+
+    def sweep(self, dev, key, counter, maxnats):
+        c = self.index[id(dev)]
+        self.req[c] = (tuple(key), int(counter), float(maxnats))
+        self.barrier.wait()
+        return self.out[c]
+
+    def abort(self):
+        self.barrier.abort()
+
+    def close(self):
+        self.batch.close()
+
+
 def run_replicas(cfg: Config, log=None):
     """``run_replicas`` (src/replicas.jl:841-947) on one GPU: ``nchains`` chains of ``replicas``
     weather replicas, ``mlda_warmup`` + ``nsamples`` replica steps, probes and step adaptation
@@ -589,14 +639,24 @@ def run_replicas(cfg: Config, log=None):
                 L.wall = float(snap[f"{g}/learned/wall"])
         say(f"resumed from {cfg.checkpoint} after step {it0} (restart {restarts})")
     t_init = time.perf_counter()
+    batched = BatchedSweeps([R.dev for R in sets]) if cfg.sweep_batch == "graph" else None
+    sweeper = batched.sweep if batched is not None else None
 
     def chain_step(c, it):
+        try:
+            chain_step_(c, it)
+        except BaseException:
+            if batched is not None:
+                batched.abort()
+            raise
+
+    def chain_step_(c, it):
         R = sets[c]
         if (learned[c] is not None and cfg.learned_train_every > 0 and it <= nwarm
                 and it % cfg.learned_train_every == 0):
             learned[c].train(cfg.learned_train_more)
         t_step = time.perf_counter()
-        st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats, learned[c])
+        st = replica_step(R, ctl, it, keys[c], rngs[c], sys, cfg.maxnats, learned[c], sweeper)
         traces[c].append(replica_monitor(R, st["parts"]))
         rec = rec_io.replica_step_record(R.k, st["parts"], st["wts"], R.active, R.since, st["moved"], st["rs"], st["fk"],
                                          R.eta_w, st["nclip"], time.perf_counter() - t_step, st["tw"], st["learned"])
@@ -667,6 +727,8 @@ def run_replicas(cfg: Config, log=None):
                                                          ess_min=cfg.ess_min, mcse_max=cfg.mcse_max)
     prod_s = prod_before + (t_end - t_prod0)
     tot = summ.get("total")
+    if batched is not None:
+        batched.close()
     for R in sets:
         R.dev.close()
     for q in probes_h:

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [  # (N, L, T, R, precision)
     (6, 8, 10, 3, FP64),
-    (5, 6, 12, 2, FP64),
+    (5, 6, 40, 2, FP64),     # block lengths 2, 4, 8, 16: a new order of kernels every sweep
     (8, 12, 8, 64, FP32),
 ]
 
@@ -44,16 +44,15 @@ def test_batch_graph_matches_eager_sweeps():
     sets = [_make(*sh, seed=11 + i) for i, sh in enumerate(SHAPES)]
     batched = [p[0] for p in sets]
     eager = [p[1] for p in sets]
-    for d in batched + eager:
-        api.sweep_options(d, p_storage_ms=1.0)
     b = api.Batch(batched)
     keys = [(3, 4), (5, 6), (7, 8)]
-    # replay 1: every storage sweep on; 2: set 0 without the multiscale sweep (another launch
-    # sequence: a new instantiation); 3 and 4: new counters on the same sequence (in-place updates)
-    plan = [(1.0, 0), (0.0, 1), (0.0, 2), (0.0, 3)]
-    for rep, (p0, ctr) in enumerate(plan):
-        api.sweep_options(batched[0], p_storage_ms=p0)
-        api.sweep_options(eager[0], p_storage_ms=p0)
+    # replays 0-3: every storage sweep on, whose block lengths run in a new order each sweep (a
+    # new kernel sequence: instantiated again); 4-6: the multiscale sweep off, the same kernel
+    # sequence with new counters (updated in place after the first)
+    plan = [(1.0, c) for c in range(4)] + [(0.0, c) for c in range(4, 7)]
+    for rep, (pms, ctr) in enumerate(plan):
+        for d in batched + eager:
+            api.sweep_options(d, p_storage_ms=pms)
         counters = [10 * ctr + i for i in range(3)]
         st = b.sweeps(1, keys=keys, counters=counters)
         for i, d in enumerate(eager):
@@ -66,7 +65,8 @@ def test_batch_graph_matches_eager_sweeps():
         api.energy_cost(eager[1])
         batched[2].download()
     launches, inst, upd = b.counts()
-    assert launches == len(plan) and inst >= 2 and upd >= 1 and inst + upd == launches
+    assert launches == len(plan) and inst + upd == launches
+    assert inst >= 2 and upd >= 1, (inst, upd)
     b.close()
     for d in batched + eager:
         d.close()

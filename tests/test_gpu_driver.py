@@ -107,6 +107,13 @@ def test_chains_in_threads_match_a_chain_alone():
     assert np.array_equal(np.array(a["traces"][0]), np.array(b["traces"][0]))
     for ra, rb in zip(a["records"][0], b["records"][0]):
         assert np.array_equal(ra["k"], rb["k"]) and np.array_equal(ra["learned"], rb["learned"])
+    # the chains' sweeps as one captured graph per step: the same run, bitwise
+    print("run_replicas graph-batched sweeps vs threads", flush=True)
+    g = RP.run_replicas(RP.load_config(dict(base, nchains=3, sweep_batch="graph")))
+    for c in range(3):
+        assert np.array_equal(np.array(a["traces"][c]), np.array(g["traces"][c]))
+        for ra, rg in zip(a["records"][c], g["records"][c]):
+            assert np.array_equal(ra["k"], rg["k"]) and np.array_equal(ra["learned"], rg["learned"])
 
 
 def test_stop_file_snapshot_and_resume(tmp_path):

@@ -27,3 +27,10 @@ def test_launch_tuning_key():
     assert replicas.load_config({"sampler": "replicas", "launch_tuning": "off"}).launch_tuning == "off"
     with pytest.raises(ValueError, match="launch_tuning"):
         replicas.load_config({"sampler": "replicas", "launch_tuning": "live"})
+
+
+def test_sweep_batch_key():
+    assert replicas.load_config({"sampler": "replicas"}).sweep_batch == "threads"
+    assert replicas.load_config({"sampler": "replicas", "sweep_batch": "graph"}).sweep_batch == "graph"
+    with pytest.raises(ValueError, match="sweep_batch"):
+        replicas.load_config({"sampler": "replicas", "sweep_batch": "persistent"})
