@@ -353,12 +353,12 @@ def apply_tuning(api, dev, mode):
         table = json.loads(tune.TABLE.read_text())
         if api._shape_key(dev) in table:
             r = api.autotune(dev, cache=str(tune.TABLE))
-            return {"form": r["form"], "block": r["block"], "source": "tuning table"}
+            return {"form": r["form"], "block": r["block"], "pad": r["pad"], "source": "tuning table"}
     if mode == "live":
         r = api.autotune(dev, nsweeps=1)
-        return {"form": r["form"], "block": r["block"], "source": "tuned now",
-                "ms_per_sweep": {f"{f}/{b}": round(t, 2) for (f, b), t in r["ms"].items()}}
-    return {"form": 1, "block": 256, "source": "library default"}
+        return {"form": r["form"], "block": r["block"], "pad": r["pad"], "source": "tuned now",
+                "ms_per_sweep": {f"{f}/{b}/{p}": round(t, 2) for (f, b, p), t in r["ms"].items()}}
+    return {"form": 1, "block": 256, "pad": 1, "source": "library default"}
 
 
 def batch_leg(prec, reps=2):

@@ -211,16 +211,27 @@ int32_t esm_sweep_block(esm_handle *h, int32_t block) {
     return ESM_OK;
 }
 
+int32_t esm_sweep_pad(esm_handle *h, int32_t pad) {
+    if (!h) { set_error("esm_sweep_pad: null handle"); return ESM_EINVAL; }
+    if (pad != 0 && pad != 1) { set_error("esm_sweep_pad: pad must be 0 or 1"); return ESM_EINVAL; }
+    h->rand_pad = pad != 0;
+    return ESM_OK;
+}
+
 int32_t esm_autotune(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
                      int32_t *best, double *ms, int32_t *ncand) {
     if (!h || !key || !best) { set_error("esm_autotune: null argument"); return ESM_EINVAL; }
     if (nsweeps < 1 || nsweeps > 64 || !(maxnats > 0)) { set_error("esm_autotune: nsweeps must be in [1, 64] and maxnats > 0"); return ESM_EINVAL; }
     if (h->prof) { set_error("esm_autotune: kernel profiling is on (esm_profile)"); return ESM_EINVAL; }
-    // candidates, form-major: forms 0 (a launch per family), 1 (merged), 2 and 4 (merged, waves per
-    // step; replica counts that are multiples of 64), each with random-launch blocks 64, 128, 256
+    // candidates c: with R a multiple of 64, c = 3·f + b over the forms 0 (a launch per family),
+    // 1 (merged), 2 and 4 (merged, waves per step) and the one-wave launches' blocks 64, 128, 256;
+    // otherwise c = 6·p + 3·f + b over padding p (0: R threads per step, 1: whole waves per
+    // step), forms 0 and 1 and the blocks
     static const int forms[4] = {0, 1, 2, 4}, blocks[3] = {64, 128, 256};
-    const int nform = h->R % 64 == 0 ? 4 : 2;
-    const int nc = nform * 3;
+    const bool r64 = h->R % 64 == 0;
+    const int nc = 12;
+    auto form_of = [&](int c) { return r64 ? forms[c / 3] : forms[(c % 6) / 3]; };
+    auto pad_of = [&](int c) { return r64 ? 0 : c / 6; };
     const size_t TR = (size_t)h->T * h->R;
     const size_t nv = (size_t)ESM_NV * h->N * TR * h->es, nr = (size_t)h->N * TR * h->es, nF = (size_t)h->L * TR * h->es;
     void *save = nullptr;
@@ -236,7 +247,7 @@ int32_t esm_autotune(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint
         }
         return hipSuccess;
     };
-    const bool merged0 = h->sweep_merged;
+    const bool merged0 = h->sweep_merged, pad0 = h->rand_pad;
     const int waves0 = h->sweep_waves, block0 = h->rand_block;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = 0;
@@ -249,10 +260,11 @@ int32_t esm_autotune(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint
     if (snap(true) != hipSuccess || hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         fail("saving the state");
     for (int c = 0; c < nc && !rc; c++) {
-        const int f = forms[c / 3];
+        const int f = form_of(c);
         h->sweep_merged = f != 0;
         h->sweep_waves = f > 1 ? f : 0;
         h->rand_block = blocks[c % 3];
+        h->rand_pad = pad_of(c) != 0;
         float t = 0.f;
         // one untimed sweep (code objects, caches), then the timed ones from the same state
         if ((rc = sweeps_on_stream(h, 1, key, counter, maxnats))) break;
@@ -279,14 +291,17 @@ int32_t esm_autotune(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint
         h->sweep_merged = merged0;
         h->sweep_waves = waves0;
         h->rand_block = block0;
+        h->rand_pad = pad0;
         return rc;
     }
-    const int f = forms[ibest / 3];
+    const int f = form_of(ibest);
     h->sweep_merged = f != 0;
     h->sweep_waves = f > 1 ? f : 0;
     h->rand_block = blocks[ibest % 3];
+    h->rand_pad = pad_of(ibest) != 0;
     best[0] = f;
     best[1] = h->rand_block;
+    best[2] = pad_of(ibest);
     if (ncand) *ncand = nc;
     return ESM_OK;
 }

@@ -51,6 +51,7 @@ struct RandArgs {
     const int *counts;           // [ESM_NWEIGHT][T]
     unsigned long long *stats;   // [ESM_NSTAT][STAT_SLOTS] of the family
     const int *perm;             // the class's steps in launch order (null: in order)
+    int rp;                      // threads per step: R, or R rounded up to a multiple of 64 (padded)
 };
 
 
@@ -73,6 +74,13 @@ template <bool UNI>
 __device__ __forceinline__ int step_of(const RandArgs &a, size_t g, int R) {
     const int i = (int)(g / R);
     return uniform<UNI>(a.perm ? a.perm[i] : a.tau0 + i * a.tstride);
+}
+
+// thread g of a one-wave launch over a class's steps: its replica j (threads j ≥ R of a padded
+// step are idle) and whether it has work
+__device__ __forceinline__ bool slot_of(const RandArgs &a, size_t g, int R, int &j) {
+    j = (int)(g % (size_t)a.rp);
+    return g < (size_t)a.ntau * a.rp && j < R;
 }
 
 // the selection words of move `k` of family `w` at step τ
@@ -129,8 +137,8 @@ template <typename S, bool UNI>
 __global__ void __launch_bounds__(256) k_rand_unit(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R)
-        run_unit<S, UNI>(m, st, a, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, out);
+    int j;
+    if (slot_of(a, g, m.R, j)) run_unit<S, UNI>(m, st, a, step_of<UNI>(a, g, a.rp), j, beta, maxnats, out);
     add_counts(a.stats, out);
 }
 
@@ -158,8 +166,8 @@ template <typename S, int DLO, int D, bool UNI>
 __global__ void __launch_bounds__(256) k_rand_pot(DevModel<S> m, SweepState<S> st, RandArgs a, S beta, S maxnats) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R)
-        run_pot<S, DLO, D, UNI>(m, st, a, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, out);
+    int j;
+    if (slot_of(a, g, m.R, j)) run_pot<S, DLO, D, UNI>(m, st, a, step_of<UNI>(a, g, a.rp), j, beta, maxnats, out);
     add_counts(a.stats, out);
 }
 
@@ -184,8 +192,8 @@ __global__ void __launch_bounds__(256, min_waves<S>(3)) k_rand_spair(DevModel<S>
                                                     S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R)
-        run_spair<S, UNI>(m, st, a, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, kmin, out);
+    int j;
+    if (slot_of(a, g, m.R, j)) run_spair<S, UNI>(m, st, a, step_of<UNI>(a, g, a.rp), j, beta, maxnats, kmin, out);
     add_counts(a.stats, out);
 }
 
@@ -823,8 +831,9 @@ __global__ void __launch_bounds__(256) k_sparse(DevModel<S> m, SweepState<S> st,
                                                 S maxnats, S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R)
-        run_sparse<S, KIND, UNI>(m, st, a, tb, step_of<UNI>(a, g, m.R), (int)(g % m.R), beta, maxnats, kmin, out);
+    int j;
+    if (slot_of(a, g, m.R, j))
+        run_sparse<S, KIND, UNI>(m, st, a, tb, step_of<UNI>(a, g, a.rp), j, beta, maxnats, kmin, out);
     add_counts(a.stats, out);
 }
 
@@ -895,9 +904,9 @@ __global__ void __launch_bounds__(SPLIT ? 1024 : 256, SPLIT ? 1 : min_waves<S>(4
     }
     S *red = nw > 1 ? (S *)sp_lds : nullptr;
     MoveOut o[NSPARSE];
-    if (g < (size_t)a.ntau * m.R) {
-        const int j = (int)(g % m.R);
-        const int tau = step_of<UNI>(a, g, m.R);
+    int j;
+    if (slot_of(a, g, m.R, j)) {   // the wave-split form runs only with R a multiple of 64 (rp = R, no idle lanes)
+        const int tau = step_of<UNI>(a, g, a.rp);
         RandArgs b = a;
         if (sa.mask & 1u) {
             b.w = W_XPAIR;
@@ -957,7 +966,10 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     // the family order (pair and cellpair are void here): the atom-site families, then the
     // table-driven ones
     const int order[7] = {W_UNIT, W_POT, W_SPAIR, W_XPAIR, W_XLOCC, W_XLOC, W_CELL};
-    const bool uni = R % 64 == 0;
+    // padded: every step gets whole waves (R rounded up to a multiple of 64, the extra lanes
+    // idle), so that a wave holds one step and its selection is wave-uniform
+    const int rp = h->rand_pad && R % 64 ? (R + 63) / 64 * 64 : R;
+    const bool uni = rp % 64 == 0;
     const int rb = h->rand_block;   // every kernel of RLAUNCH is bounded to 256 threads and counts per wave
     int dmax = 0;
     for (int n = 0; n < N; n++) dmax = std::max(dmax, h->adj_ptr[n + 1] - h->adj_ptr[n]);
@@ -994,8 +1006,8 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     w_sparse.wk[W_XLOC] = 1;
     w_sparse.wk[W_CELL] = 2;
     for (size_t c = 0; c < h->tclass.size(); c++) {
-        RandArgs a{k0, k1, ctr, h->tclass[c].first, 2, h->tclass[c].second, 0, h->d_counts, nullptr, nullptr};
-        const unsigned nb = nblk((size_t)a.ntau * R, (unsigned)rb);
+        RandArgs a{k0, k1, ctr, h->tclass[c].first, 2, h->tclass[c].second, 0, h->d_counts, nullptr, nullptr, rp};
+        const unsigned nb = nblk((size_t)a.ntau * rp, (unsigned)rb);
         for (int w : order) {
             if (!(h->sweep_w[w] > 0)) continue;
             if (h->sweep_merged && (w == W_XPAIR || w == W_XLOCC || w == W_XLOC || w == W_CELL)) continue;
@@ -1058,7 +1070,7 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
             ProfScope p(h, ESM_KERN_RANDOM);
             RandArgs b = a;
             b.perm = order_steps(a, w_sparse);
-            const int nw = uni ? sparse_waves(h) : 1;
+            const int nw = R % 64 == 0 ? sparse_waves(h) : 1;
             if (nw == 1) {
                 RLAUNCH((k_sparse_all<S, true, false>), (k_sparse_all<S, false, false>), m, st, b, sa, tb, beta, mxn, kmin,
                         1);

@@ -114,6 +114,7 @@ SIGNATURES = [
     ("esm_batch_sweeps", C.c_int32, [C.c_void_p, C.c_int32, _u32p, _u32p, C.c_double, C.POINTER(C.c_int64)]),
     ("esm_batch_counts", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("esm_sweep_block", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("esm_sweep_pad", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_autotune", C.c_int32, [C.c_void_p, C.c_int32, _u32p, C.c_uint32, C.c_double, _ip, _dp, _ip]),
 ]
 (KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW,

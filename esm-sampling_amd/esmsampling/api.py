@@ -188,8 +188,22 @@ def sweep_block(R: Replicas, block: int = 256):
     check(R.lib.esm_sweep_block(R.handle, int(block)), "esm_sweep_block")
 
 
+def sweep_pad(R: Replicas, pad: bool = True):
+    """Every step of the one-wave random launches on whole waves (R rounded up to a multiple of 64,
+    the extra lanes idle: one step per wave; the default) or R threads per step; states and
+    statistics do not depend on it."""
+    check(R.lib.esm_sweep_pad(R.handle, int(bool(pad))), "esm_sweep_pad")
+
+
 AUTOTUNE_FORMS = (0, 1, 2, 4)
 AUTOTUNE_BLOCKS = (64, 128, 256)
+
+
+def autotune_candidates(nrep: int):
+    """The (form, block, pad) of the autotuner's 12 candidates, in the library's order."""
+    if nrep % 64 == 0:
+        return [(AUTOTUNE_FORMS[c // 3], AUTOTUNE_BLOCKS[c % 3], 0) for c in range(12)]
+    return [(AUTOTUNE_FORMS[(c % 6) // 3], AUTOTUNE_BLOCKS[c % 3], c // 6) for c in range(12)]
 
 
 def _shape_key(R: Replicas) -> str:
@@ -198,12 +212,13 @@ def _shape_key(R: Replicas) -> str:
 
 def autotune(R: Replicas, nsweeps: int = 1, key=(7, 7), counter: int = 0, maxnats: float = 40.0,
              cache: str | None = None, retune: bool = False):
-    """Pick the launch form of the random families (``sweep_launch``) and their workgroup size
-    (``sweep_block``) for this shape by timing every candidate on the device (``esm_autotune``:
+    """Pick the launch form of the random families (``sweep_launch``), their workgroup size
+    (``sweep_block``) and padding (``sweep_pad``) for this shape by timing every candidate on the
+    device (``esm_autotune``:
     each candidate runs ``nsweeps`` sweeps from the current state, which is restored afterwards,
     so the chain does not move). With ``cache`` (a JSON file), a shape tuned before takes the
     stored choice without timing (unless ``retune``), and a new choice is written to the file. Returns
-    {"form", "block", "ms": {(form, block): ms per sweep}, "cached"}."""
+    {"form", "block", "pad", "ms": {(form, block, pad): ms per sweep}, "cached"}."""
     import json
     import os
     sk = _shape_key(R)
@@ -212,20 +227,22 @@ def autotune(R: Replicas, nsweeps: int = 1, key=(7, 7), counter: int = 0, maxnat
         with open(cache) as f:
             table = json.load(f)
         if sk in table and not retune:
-            form, block = int(table[sk]["form"]), int(table[sk]["block"])
+            form, block, pad = int(table[sk]["form"]), int(table[sk]["block"]), int(table[sk].get("pad", 0))
             sweep_launch(R, form)
             sweep_block(R, block)
-            return {"form": form, "block": block, "ms": {}, "cached": True}
-    best = (C.c_int32 * 2)()
+            sweep_pad(R, pad)
+            return {"form": form, "block": block, "pad": pad, "ms": {}, "cached": True}
+    best = (C.c_int32 * 3)()
     ms = (C.c_double * 12)()
     nc = C.c_int32()
     check(R.lib.esm_autotune(R.handle, int(nsweeps), _key(key), int(counter) & 0xFFFFFFFF, float(maxnats), best, ms,
                              C.byref(nc)), "esm_autotune")
-    times = {(AUTOTUNE_FORMS[c // 3], AUTOTUNE_BLOCKS[c % 3]): float(ms[c]) for c in range(nc.value)}
-    out = {"form": int(best[0]), "block": int(best[1]), "ms": times, "cached": False}
+    cands = autotune_candidates(R.R)
+    times = {cands[c]: float(ms[c]) for c in range(nc.value)}
+    out = {"form": int(best[0]), "block": int(best[1]), "pad": int(best[2]), "ms": times, "cached": False}
     if cache:
-        table[sk] = {"form": out["form"], "block": out["block"],
-                     "ms": {f"{f}/{b}": t for (f, b), t in times.items()}}
+        table[sk] = {"form": out["form"], "block": out["block"], "pad": out["pad"],
+                     "ms": {f"{f}/{b}/{p}": t for (f, b, p), t in times.items()}}
         tmp = cache + ".tmp"
         with open(tmp, "w") as f:
             json.dump(table, f, indent=1, sort_keys=True)

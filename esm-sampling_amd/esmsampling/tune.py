@@ -35,8 +35,8 @@ def tune_shape(N, L, T, R, prec, table, warm=4, seed=1):
             api.operation_sweep(d, 1, key=(2, 2), counter=c)
         t0 = time.perf_counter()
         res = api.autotune(d, nsweeps=2, key=(3, 3), counter=0, cache=str(table), retune=True)
-        ms = {f"{f}/{b}": round(t, 2) for (f, b), t in res["ms"].items()}
-        print(f"{api._shape_key(d)}: form {res['form']} block {res['block']} "
+        ms = {f"{f}/{b}/{p}": round(t, 2) for (f, b, p), t in res["ms"].items()}
+        print(f"{api._shape_key(d)}: form {res['form']} block {res['block']} pad {res['pad']} "
               f"({time.perf_counter() - t0:.1f} s) {ms}", flush=True)
         return res
     finally:

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 5
+#define ESM_ABI_VERSION 6
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -414,16 +414,20 @@ int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_
  * (esm_profile) must be off on them. Destroying a batch leaves its handles alone.
  *
  * esm_sweep_block sets the workgroup size of the one-wave random launches (64, 128 or 256, the
- * default); states and statistics do not depend on it.
+ * default); esm_sweep_pad(h, 1) gives every step of those launches whole waves (R rounded up to a
+ * multiple of 64, the extra lanes idle), so that a wave holds the replicas of one step (the
+ * wave-uniform selection and the heaviest-first order of steps) when R is not a multiple of 64
+ * (the default; 0: R threads per step). States and statistics depend on neither.
  *
- * esm_autotune picks the launch form of the random families for this handle's shape by timing:
- * candidates c = 3·f + b over the forms f of esm_sweep_launch (0, 1, and 2, 4 when R is a multiple
- * of 64) and the blocks b (64, 128, 256), each running `nsweeps` sweeps (1 … 64) under `key` and
- * `counter` from the handle's current state (after one untimed sweep), timed with HIP events.
- * The state is restored afterwards: the call leaves v, the caches and the chain unchanged (the
- * statistics of esm_sweeps are overwritten). Sets the fastest candidate on the handle and returns
- * it in best[0] (form) and best[1] (block); ms[c] (may be NULL; 12 entries suffice) the mean ms
- * per sweep of each candidate, *ncand (may be NULL) their number.
+ * esm_autotune picks the launch of the random families for this handle's shape by timing 12
+ * candidates: with R a multiple of 64, c = 3·f + b over the forms f of esm_sweep_launch (0, 1, 2,
+ * 4) and the blocks b (64, 128, 256); otherwise c = 6·p + 3·f + b over the padding p (0, 1), the
+ * forms 0 and 1 and the blocks. Each runs `nsweeps` sweeps (1 … 64) under `key` and `counter` from
+ * the handle's current state (after one untimed sweep), timed with HIP events. The state is
+ * restored afterwards: the call leaves v, the caches and the chain unchanged (the statistics of
+ * esm_sweeps are overwritten). Sets the fastest candidate on the handle and returns it in best[0]
+ * (form), best[1] (block) and best[2] (padding); ms[c] (may be NULL; 12 entries) the mean ms per
+ * sweep of each candidate, *ncand (may be NULL) their number.
  */
 #define ESM_BATCH_MAX 64
 typedef struct esm_batch esm_batch;
@@ -433,6 +437,7 @@ int32_t esm_batch_sweeps(esm_batch *b, int32_t nsweeps, const uint32_t *keys, co
                          int64_t *stats);
 int32_t esm_batch_counts(const esm_batch *b, int64_t *counts);
 int32_t esm_sweep_block(esm_handle *h, int32_t block);
+int32_t esm_sweep_pad(esm_handle *h, int32_t pad);
 int32_t esm_autotune(esm_handle *h, int32_t nsweeps, const uint32_t key[2], uint32_t counter, double maxnats,
                      int32_t *best, double *ms, int32_t *ncand);
 

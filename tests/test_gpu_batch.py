@@ -100,20 +100,43 @@ def test_autotune_keeps_the_chain_and_its_choice_reproduces(R, prec, tmp_path):
     r0, F0 = d.caches()
     cache = str(tmp_path / "tune.json")
     res = api.autotune(d, nsweeps=1, key=(9, 9), counter=5, cache=cache)
-    ncand = 12 if R % 64 == 0 else 6
-    assert len(res["ms"]) == ncand and all(t > 0 for t in res["ms"].values())
-    assert (res["form"], res["block"]) == min(res["ms"], key=res["ms"].get)
+    assert list(res["ms"]) == api.autotune_candidates(R) and all(t > 0 for t in res["ms"].values())
+    assert (res["form"], res["block"], res["pad"]) == min(res["ms"], key=res["ms"].get)
     v1, _, _ = d.download()
     r1, F1 = d.caches()
     assert np.array_equal(v0, v1) and np.array_equal(r0, r1) and np.array_equal(F0, F1)
-    # the twin runs the chosen form with the default block: the block does not change results
+    # the twin runs the chosen form with the default block and the other padding: neither changes
+    # results
     api.sweep_launch(twin, res["form"])
+    api.sweep_pad(twin, not res["pad"])
     st = api.operation_sweep(d, 1, key=(1, 2), counter=0)
     st_t = api.operation_sweep(twin, 1, key=(1, 2), counter=0)
     assert np.array_equal(st, st_t) and _same(d, twin)
     # the cache gives the same choice without timing
     again = api.autotune(twin, cache=cache)
-    assert again["cached"] and (again["form"], again["block"]) == (res["form"], res["block"])
+    assert again["cached"] and (again["form"], again["block"], again["pad"]) == (res["form"], res["block"], res["pad"])
     assert list(json.load(open(cache))) == [api._shape_key(d)]
     d.close()
     twin.close()
+
+
+@pytest.mark.parametrize("R,prec", [(3, FP64), (16, FP32), (70, FP64)])
+def test_padded_steps_change_nothing(R, prec):
+    """Whole waves per step (R rounded up to a multiple of 64) against R threads per step, merged
+    and per family: bitwise the same states, caches and statistics."""
+    print(f"padded random launches R={R} prec={prec}", flush=True)
+    a, b = _make(7, 10, 12, R, prec, seed=31)
+    for merged in (True, False):
+        for d in (a, b):
+            api.sweep_options(d, p_storage_ms=1.0)
+            api.sweep_launch(d, merged)
+        api.sweep_pad(a, False)
+        api.sweep_pad(b, True)
+        api.sweep_block(b, 64)
+        for c in range(2):
+            sa = api.operation_sweep(a, 1, key=(4, 5), counter=c + (10 if merged else 0))
+            sb = api.operation_sweep(b, 1, key=(4, 5), counter=c + (10 if merged else 0))
+            assert np.array_equal(sa, sb) and sa[:, 0].sum() > 0
+            assert _same(a, b)
+    a.close()
+    b.close()

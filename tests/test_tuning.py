@@ -17,9 +17,11 @@ def test_tuning_table_entries_are_valid_launch_forms():
         R = int(m.group(4))
         assert e["form"] in ((0, 1, 2, 4) if R % 64 == 0 else (0, 1)), key
         assert e["block"] in (64, 128, 256), key
+        assert e["pad"] in ((0,) if R % 64 == 0 else (0, 1)), key
         # the stored choice is the fastest of the timed candidates
         ms = {k: float(v) for k, v in e["ms"].items()}
-        assert min(ms, key=ms.get) == f"{e['form']}/{e['block']}", key
+        assert len(ms) == 12
+        assert min(ms, key=ms.get) == f"{e['form']}/{e['block']}/{e['pad']}", key
 
 
 def test_launch_tuning_key():
