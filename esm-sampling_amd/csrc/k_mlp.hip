@@ -348,52 +348,74 @@ __device__ __forceinline__ double group_sum(double v) {
 // [Q/κ per hour (B)][capacity factors per carrier and hour (3B)][capacities over the system
 // means (6)][1][the node's net line inflow over κ per hour (B)][the largest loading |F|/s of its
 // lines per hour (B)][0 …]
+// A block builds the features of a tile of MLP_FT consecutive instances (consecutive replicas of
+// one node and block, the replica being the fastest index): thread (i, h) takes instance i of the
+// tile and the hours b ≡ h (mod 256 / MLP_FT), so every state read of a warp covers consecutive
+// replicas of one hour (coalesced); the tile's rows are assembled in LDS and written out as
+// contiguous 16-byte stores.
+constexpr int MLP_FT = 32;
+constexpr int MLP_DIN = 160;
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_mlp_features(DevModel<S> m, SweepState<S> st, const double *kraw, MlpDims D,
                                                       long long m0, const long long *idx, int cnt, uint16_t *X, float *Y,
                                                       float *MU0) {
-    const int t = blockIdx.x * MLP_GPB + (threadIdx.x >> 5), lane = threadIdx.x & (MLP_G - 1);
-    if (t >= cnt) return;
-    int n, d, j;
-    inst_of(idx ? idx[t] : m0 + t, D.nb, m.R, n, d, j);
-    const int B = D.B;
-    const size_t TR = (size_t)m.T * m.R;
-    uint16_t *x = X + (size_t)t * D.din;
-    for (int f = lane; f < D.din; f += MLP_G) {
-        float val = 0.0f;
-        if (f < B) {
-            const size_t o = (size_t)n * TR + (size_t)(d * B + f) * m.R + j;
-            const S kap = m.kc[ESM_KGAS * m.N + n], y = st.v[plane(m.N, m.T, m.R) + o];
-            const S Q = st.r[o] + kap * clampv(y, S(0), S(1));
-            val = kap > S(0) ? (float)(Q / kap) : 0.0f;
-            Y[(size_t)t * B + f] = (float)y;
-            MU0[(size_t)t * B + f] = fminf(fmaxf(val, 0.0f), 1.0f);
-        } else if (f < 4 * B) {
-            const int c = (f - B) / B, b = (f - B) % B;
-            val = (float)st.x[((size_t)c * m.N + n) * TR + (size_t)(d * B + b) * m.R + j];
-        } else if (f < 4 * B + ESM_NK) {
-            const int c = f - 4 * B;
-            double mean = 0.0;
-            for (int q = 0; q < m.N; q++) mean += kraw[c * m.N + q];
-            mean /= m.N;
-            val = mean > 0 ? (float)(kraw[c * m.N + n] / mean) : 0.0f;
-        } else if (f == 4 * B + ESM_NK) {
-            val = 1.0f;
-        } else if (f < 6 * B + ESM_NK + 1) {
-            const int b = (f - 4 * B - ESM_NK - 1) % B;
-            const bool inflow = f < 5 * B + ESM_NK + 1;
+    __shared__ __attribute__((aligned(16))) uint16_t xs[MLP_FT * MLP_DIN];
+    __shared__ float ys[MLP_FT * 24], mus[MLP_FT * 24];
+    __shared__ double kmean[ESM_NK];
+    const int tid = threadIdx.x, ti = tid % MLP_FT, hp = tid / MLP_FT;
+    constexpr int NHP = 256 / MLP_FT;
+    const long long t0 = (long long)blockIdx.x * MLP_FT;
+    const int nin = (int)min((long long)MLP_FT, (long long)cnt - t0);
+    const int B = D.B, din = D.din;
+    if (tid < ESM_NK) {
+        double mean = 0.0;
+        for (int q = 0; q < m.N; q++) mean += kraw[tid * m.N + q];
+        kmean[tid] = mean / m.N;
+    }
+    __syncthreads();
+    if (ti < nin) {
+        int n, d, j;
+        inst_of(idx ? idx[t0 + ti] : m0 + t0 + ti, D.nb, m.R, n, d, j);
+        const size_t TR = (size_t)m.T * m.R;
+        uint16_t *x = xs + ti * din;
+        const S kap = m.kc[ESM_KGAS * m.N + n];
+        for (int b = hp; b < B; b += NHP) {
             const size_t o = (size_t)(d * B + b) * m.R + j;
-            S acc = S(0);
+            const S y = st.v[plane(m.N, m.T, m.R) + (size_t)n * TR + o];
+            const S Q = st.r[(size_t)n * TR + o] + kap * clampv(y, S(0), S(1));
+            const float q = kap > S(0) ? (float)(Q / kap) : 0.0f;
+            x[b] = f2bf(q);
+            ys[ti * B + b] = (float)y;
+            mus[ti * B + b] = fminf(fmaxf(q, 0.0f), 1.0f);
+#pragma unroll
+            for (int c = 0; c < 3; c++) x[B + c * B + b] = f2bf((float)st.x[((size_t)c * m.N + n) * TR + o]);
+            // net line inflow / κ and the largest incident loading |F| / s
+            S in = S(0), mx = S(0);
             for (int e = m.adj_ptr[n]; e < m.adj_ptr[n + 1]; e++) {
                 const int l = m.adj_line[e];
                 const S F = st.F[(size_t)l * TR + o];
-                if (inflow) acc += m.bus1[l] == n ? F : -F;      // F flows from bus0 to bus1
-                else acc = fmax(acc, fabs(F) / m.ls[l]);
+                in += m.bus1[l] == n ? F : -F;      // F flows from bus0 to bus1
+                mx = fmax(mx, fabs(F) / m.ls[l]);
             }
-            const S kap = m.kc[ESM_KGAS * m.N + n];
-            val = inflow ? (kap > S(0) ? (float)(acc / kap) : 0.0f) : (float)acc;
+            x[4 * B + ESM_NK + 1 + b] = f2bf(kap > S(0) ? (float)(in / kap) : 0.0f);
+            x[5 * B + ESM_NK + 1 + b] = f2bf((float)mx);
         }
-        x[f] = f2bf(val);
+        if (hp == 0) {
+            for (int c = 0; c < ESM_NK; c++)
+                x[4 * B + c] = f2bf(kmean[c] > 0 ? (float)(kraw[c * m.N + n] / kmean[c]) : 0.0f);
+            x[4 * B + ESM_NK] = f2bf(1.0f);
+        }
+        for (int f = 6 * B + ESM_NK + 1 + hp; f < din; f += NHP) x[f] = f2bf(0.0f);
+    }
+    __syncthreads();
+    // the tile's rows are contiguous in X, Y and MU0
+    const uint4 *src = (const uint4 *)xs;
+    uint4 *dst = (uint4 *)(X + t0 * din);
+    for (int i = tid; i < nin * din / 8; i += 256) dst[i] = src[i];
+    for (int i = tid; i < nin * B; i += 256) {
+        Y[t0 * B + i] = ys[i];
+        MU0[t0 * B + i] = mus[i];
     }
 }
 
@@ -676,7 +698,11 @@ static int mlp_features(esm_handle *h, long long m0, const long long *idx, int c
     MlpState &s = *h->mlp;
     const DevModel<S> m = dev_model<S>(h);
     SweepState<S> st{(S *)h->d_v, (S *)h->d_r, (S *)h->d_F, (const S *)h->d_x};
-    hipLaunchKernelGGL(k_mlp_features<S>, dim3((unsigned)((cnt + MLP_GPB - 1) / MLP_GPB)), dim3(256), 0, h->stream, m, st,
+    if (s.D.din != MLP_DIN || s.D.B > 24 || (uintptr_t)s.X % 16) {
+        set_error("mlp_features: 160 inputs, at most 24 hours per block, 16-byte aligned rows");
+        return ESM_EINVAL;
+    }
+    hipLaunchKernelGGL(k_mlp_features<S>, dim3((unsigned)((cnt + MLP_FT - 1) / MLP_FT)), dim3(256), 0, h->stream, m, st,
                        (const double *)h->d_k, s.D, m0, idx, cnt, s.X, s.Y, s.MU0);
     ESM_HIP_CHECK(hipGetLastError());
     return 0;
