@@ -68,12 +68,14 @@ struct SmsKink {
 // piece ends a0 and x); every plane entry is written and read by the lane that owns the step only
 enum SmsPlane { P_YE, P_YE1, P_S0, P_C0, P_J0, P_PH0, P_DA, P_DB, P_BA, P_BX, SMS_NPLANE };
 
-// the moved unit u of a segment: its storage parameters, clamped capacity, κs, node
+// the moved unit u of a segment: its storage parameters, clamped capacity, κs, node, and the
+// node's gas capacity and cost (the kinked gas terms at the block's ends)
 template <typename S>
 struct SmsUnit {
     StoPar<S> par;
     S kst, ks;
     int node;
+    S gk, gmc;
 };
 
 // one replica's move, seen by one lane of its segment; the values live in the segment's LDS
@@ -96,6 +98,11 @@ struct FStor {
         gen_cost(m->mc[ESM_KGAS * m->N + n], k, k + m->kappa0, m->dt, m->cv, y, c, p);
         return c;
     }
+    __device__ S gen_cu(const SmsUnit<S> &U, S y, S &p) const {   // gen_c from the staged unit
+        S c;
+        gen_cost(U.gmc, U.gk, U.gk + m->kappa0, m->dt, m->cv, y, c, p);
+        return c;
+    }
     __device__ static S kphi(S Q, S g, S t) { return fmax(Q + g * t, S(0)) - fmax(Q, S(0)); }
     // the storage secondaries of a unit from its d: d, d − η_in (1 without charging), −d η_out − 1
     __device__ static S sec(int k, S d, const StoPar<S> &p) {
@@ -113,7 +120,7 @@ struct FStor {
             const SmsKink<S> kk = kk_l[kidx * nus + u];
             if (kk.valid != S(0) && kk.ak != S(0)) {
                 S p1;
-                const S c1 = gen_c(U.node, kk.y0 + kk.ak * kphi(kk.Q, kk.g, t), p1);
+                const S c1 = gen_cu(U, kk.y0 + kk.ak * kphi(kk.Q, kk.g, t), p1);
                 D += c1 - kk.gc0;
                 dr -= p1 - kk.gp0;
             }
@@ -493,6 +500,8 @@ __global__ void __launch_bounds__(64, min_waves<S>(4)) k_storage_ms(DevModel<S> 
             U.kst = m.kc[(ESM_KSTOR + si) * m.N + node];
             U.ks = U.kst * m.hours[us] / m.dt;
             U.node = node;
+            U.gk = m.kc[ESM_KGAS * m.N + node];
+            U.gmc = m.mc[ESM_KGAS * m.N + node];
             un_l[lane] = U;
         }
         __builtin_amdgcn_wave_barrier();
