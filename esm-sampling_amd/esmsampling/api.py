@@ -254,7 +254,8 @@ class Batch:
     """Replica sets of uneven size (network, horizon, replica count, precision) swept together as
     one HIP graph launch (``esm_batch_sweeps``): the same kernels, arguments and results as
     ``operation_sweep`` on each set, run as concurrent branches of one graph. The sets must not
-    be used by other threads during a batched sweep; kernel profiling must be off on them."""
+    be used by other threads during a batched sweep; kernel profiling must be off on them. The
+    batch refers to the sets' handles: close it before closing them."""
 
     def __init__(self, sets):
         self.sets = list(sets)

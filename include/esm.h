@@ -411,7 +411,8 @@ int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_
  * kept and updated in place on later calls (instantiated again when the launch sequence
  * changes); esm_batch_counts gives (graph launches, instantiations, in-place updates). The
  * handles must not be used by other threads during the call, and kernel profiling
- * (esm_profile) must be off on them. Destroying a batch leaves its handles alone.
+ * (esm_profile) must be off on them. Destroying a batch leaves its handles alone; destroy it
+ * before any of them.
  *
  * esm_sweep_block sets the workgroup size of the one-wave random launches (64, 128 or 256, the
  * default); esm_sweep_pad(h, 1) gives every step of those launches whole waves (R rounded up to a
