@@ -17,12 +17,12 @@ constexpr int ESM_MAXWIN = 256;   // longest regime window
 constexpr int STAT_SLOTS = 256;   // sweep counters are spread over this many addresses per statistic
 
 // Minimum waves per SIMD asked of a latency-bound sweep kernel (second __launch_bounds__
-// argument). The FP32 instances take the occupancy measured fastest, which caps their registers
-// and accepts a few spilled bytes per lane; the FP64 instances need about twice the registers and
-// would spill hundreds of bytes per lane under the same cap, so they keep the compiler's own
-// allocation.
+// argument), per precision, each the occupancy measured fastest: the caps limit the registers
+// and accept a few spilled bytes per lane. The FP64 instances need about twice the registers;
+// most keep the compiler's own allocation (1), which the FP32 caps would turn into hundreds of
+// spilled bytes per lane.
 template <typename S>
-constexpr int min_waves(int fp32_waves) { return sizeof(S) == 4 ? fp32_waves : 1; }
+constexpr int min_waves(int fp32_waves, int fp64_waves = 1) { return sizeof(S) == 4 ? fp32_waves : fp64_waves; }
 
 void set_error(const std::string &msg);
 

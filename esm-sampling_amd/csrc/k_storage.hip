@@ -454,7 +454,7 @@ __device__ __forceinline__ S sample_stor(const F &f, S beta, S maxnats, S useg, 
 }
 
 template <typename S, int W>
-__global__ void __launch_bounds__(64, min_waves<S>(4)) k_storage_ms(DevModel<S> m, SweepState<S> st, SmsArgs a, S beta, S maxnats,
+__global__ void __launch_bounds__(64, min_waves<S>(4, 2)) k_storage_ms(DevModel<S> m, SweepState<S> st, SmsArgs a, S beta, S maxnats,
                                                   S kmin) {
     extern __shared__ unsigned char sms_lds[];
     constexpr int SEGS = 64 / W;

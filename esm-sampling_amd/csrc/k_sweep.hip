@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256, min_waves<S>(D <= 8 ? 3 : 1)) k_theta(Dev
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256, min_waves<S>(4)) k_soc(DevModel<S> m, SweepState<S> st, SweepArgs a, S beta, S maxnats, S kmin) {
+__global__ void __launch_bounds__(256, min_waves<S>(4, 2)) k_soc(DevModel<S> m, SweepState<S> st, SweepArgs a, S beta, S maxnats, S kmin) {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t nsite = (size_t)m.N * a.ntau;
     MoveOut out;
