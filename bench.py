@@ -268,7 +268,8 @@ def ess_leg(args, nchains=2, R=16, T=2920):
     RP.run_replicas(RP.load_config({**base, "mlda_warmup": 2, "nsamples": 8, "learned_proposal": True,
                                     "learned_train_steps": 10}))
     res = {}
-    for name, extra in (("plain", {}), ("learned", dict(learned_proposal=True, learned_train_steps=1000,
+    for name, extra in (("plain", dict(sweep_batch="threads")), ("learned", dict(learned_proposal=True,
+                                                                                learned_train_steps=1000,
                                                          learned_train_every=10, learned_train_more=100,
                                                          sweep_batch=args.ess_batch))):
         r = RP.run_replicas(RP.load_config({**base, **extra}),

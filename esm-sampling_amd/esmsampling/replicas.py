@@ -133,10 +133,10 @@ class Config:
     # committed tuning table's choice for the replica sets' shape (esmsampling/tuning.json),
     # "off" the library default
     launch_tuning: str = "table"
-    # how the chains' operation sweeps run (no counterpart in the reference): "threads", each
-    # chain's sweep from its own host thread, or "graph", the sweeps of all chains of a step as
-    # one captured HIP graph (api.Batch; the same states)
-    sweep_batch: str = "threads"
+    # how the chains' operation sweeps run (no counterpart in the reference): "graph", the sweeps
+    # of all chains of a step as one captured HIP graph (api.Batch), or "threads", each chain's
+    # sweep from its own host thread; the states are the same
+    sweep_batch: str = "graph"
 
     def check(self):
         if self.sampler != "replicas":

@@ -102,8 +102,8 @@ def test_chains_in_threads_match_a_chain_alone():
                 replica_probe_avg=1, replica_gate_min=1, replica_gate_max=2, replica_gate_min_window=1,
                 replica_gate_max_window=3, replica_gate_win=2, synthetic_nodes=10, synthetic_lines=16, snapshots=48,
                 precision="fp64", seed=11, learned_proposal=True, learned_train_steps=20, learned_components=2)
-    a = RP.run_replicas(RP.load_config(dict(base, nchains=3)))
-    b = RP.run_replicas(RP.load_config(dict(base, nchains=1)))
+    a = RP.run_replicas(RP.load_config(dict(base, nchains=3, sweep_batch="threads")))
+    b = RP.run_replicas(RP.load_config(dict(base, nchains=1, sweep_batch="threads")))
     assert np.array_equal(np.array(a["traces"][0]), np.array(b["traces"][0]))
     for ra, rb in zip(a["records"][0], b["records"][0]):
         assert np.array_equal(ra["k"], rb["k"]) and np.array_equal(ra["learned"], rb["learned"])

@@ -32,7 +32,7 @@ def test_launch_tuning_key():
 
 
 def test_sweep_batch_key():
-    assert replicas.load_config({"sampler": "replicas"}).sweep_batch == "threads"
-    assert replicas.load_config({"sampler": "replicas", "sweep_batch": "graph"}).sweep_batch == "graph"
+    assert replicas.load_config({"sampler": "replicas"}).sweep_batch == "graph"
+    assert replicas.load_config({"sampler": "replicas", "sweep_batch": "threads"}).sweep_batch == "threads"
     with pytest.raises(ValueError, match="sweep_batch"):
         replicas.load_config({"sampler": "replicas", "sweep_batch": "persistent"})
