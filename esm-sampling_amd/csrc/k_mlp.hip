@@ -198,22 +198,24 @@ __global__ void __launch_bounds__(256) k_gemm_nt(int M, int N, int K, const uint
         }
 }
 
-// The wide form for N ≥ 256: 128 × 256 tile per workgroup (the activations of a row block read
-// once per layer), four waves of 64 × 128 (2 × 4 MFMA tiles: each A fragment feeds four MFMAs),
-// K in steps of 32 through a double-buffered LDS ring, one barrier per step.
+// The wide form for N ≥ 192: a 128 × 64·NB tile per workgroup (NB = 4: 256 columns; NB = 3: 192,
+// which the 576 outputs of 8 components × 24 hours fill exactly), the activations of a row block
+// read once per layer; four waves of 64 × 32·NB (2 × NB MFMA tiles: each A fragment feeds NB
+// MFMAs), K in steps of 32 through a double-buffered LDS ring, one barrier per step.
 constexpr int GW_N = 256;
 
-template <int EPI>
+template <int EPI, int NB>
 __global__ void __launch_bounds__(256, 2) k_gemm_nt_w(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
                                                    const uint16_t *__restrict__ B, int ldb, const float *__restrict__ bias,
                                                    void *__restrict__ C, int ldc) {
     __shared__ __attribute__((aligned(16))) uint16_t As[2][GN_M][GN_LD];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][GW_N][GN_LD];
+    constexpr int TN = 64 * NB;
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][TN][GN_LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int i0 = blockIdx.y * GN_M, j0 = blockIdx.x * GW_N;
+    const int i0 = blockIdx.y * GN_M, j0 = blockIdx.x * TN;
     const int wr = wave >> 1, wc = wave & 1;
-    // staging: A 128 rows × 4 chunks (2 per thread), B 256 rows × 4 chunks (4 per thread)
-    uint4 ra[2], rb[4];
+    // staging: A 128 rows × 4 chunks (2 per thread), B 64·NB rows × 4 chunks (NB per thread)
+    uint4 ra[2], rb[NB];
     auto load = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 2; e++) {
@@ -221,7 +223,7 @@ __global__ void __launch_bounds__(256, 2) k_gemm_nt_w(int M, int N, int K, const
             ra[e] = i < M ? *(const uint4 *)(A + (size_t)i * lda + k0 + kc) : make_uint4(0, 0, 0, 0);
         }
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
+        for (int e = 0; e < NB; e++) {
             const int c = tid + 256 * e, r = c >> 2, kc = (c & 3) * 8, j = j0 + r;
             rb[e] = j < N ? *(const uint4 *)(B + (size_t)j * ldb + k0 + kc) : make_uint4(0, 0, 0, 0);
         }
@@ -233,16 +235,16 @@ __global__ void __launch_bounds__(256, 2) k_gemm_nt_w(int M, int N, int K, const
             *(uint4 *)&As[buf][c >> 2][(c & 3) * 8] = ra[e];
         }
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
+        for (int e = 0; e < NB; e++) {
             const int c = tid + 256 * e;
             *(uint4 *)&Bs[buf][c >> 2][(c & 3) * 8] = rb[e];
         }
     };
-    f32x16 acc[2][4];
+    f32x16 acc[2][NB];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = f32x16{0};
+        for (int b = 0; b < NB; b++) acc[a][b] = f32x16{0};
     load(0);
     store(0);
     __syncthreads();
@@ -253,15 +255,15 @@ __global__ void __launch_bounds__(256, 2) k_gemm_nt_w(int M, int N, int K, const
 #pragma unroll
         for (int kk = 0; kk < GN_K; kk += 16) {
             const int ko = kk + 8 * (lane >> 5);
-            bf16x8 fa[2], fb[4];
+            bf16x8 fa[2], fb[NB];
 #pragma unroll
             for (int q = 0; q < 2; q++) fa[q] = *(const bf16x8 *)&As[cur][wr * 64 + q * 32 + (lane & 31)][ko];
 #pragma unroll
-            for (int q = 0; q < 4; q++) fb[q] = *(const bf16x8 *)&Bs[cur][wc * 128 + q * 32 + (lane & 31)][ko];
+            for (int q = 0; q < NB; q++) fb[q] = *(const bf16x8 *)&Bs[cur][wc * 32 * NB + q * 32 + (lane & 31)][ko];
 #pragma unroll
             for (int a = 0; a < 2; a++)
 #pragma unroll
-                for (int b = 0; b < 4; b++)
+                for (int b = 0; b < NB; b++)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
         }
         if (more) store(cur ^ 1);   // the other buffer: its last readers passed the previous barrier
@@ -271,8 +273,8 @@ __global__ void __launch_bounds__(256, 2) k_gemm_nt_w(int M, int N, int K, const
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const int j = j0 + wc * 128 + b * 32 + (lane & 31);
+        for (int b = 0; b < NB; b++) {
+            const int j = j0 + wc * 32 * NB + b * 32 + (lane & 31);
             if (j >= N) continue;
             const float bj = bias ? bias[j] : 0.0f;
 #pragma unroll
@@ -294,9 +296,12 @@ static int gemm_nt(hipStream_t s, int M, int N, int K, const uint16_t *A, int ld
         set_error("gemm_nt: K must be a multiple of 32, operands 16-byte aligned");
         return ESM_EINVAL;
     }
-    if (N >= GW_N) {
+    if (N % 192 == 0 && N % GW_N != 0) {   // 192-wide tiles fill N exactly
+        const dim3 grid((unsigned)(N / 192), (unsigned)((M + GN_M - 1) / GN_M));
+        hipLaunchKernelGGL((k_gemm_nt_w<EPI, 3>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, C, ldc);
+    } else if (N >= GW_N) {
         const dim3 grid((unsigned)((N + GW_N - 1) / GW_N), (unsigned)((M + GN_M - 1) / GN_M));
-        hipLaunchKernelGGL((k_gemm_nt_w<EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, C, ldc);
+        hipLaunchKernelGGL((k_gemm_nt_w<EPI, 4>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, C, ldc);
     } else {
         const dim3 grid((unsigned)((N + GN_N - 1) / GN_N), (unsigned)((M + GN_M - 1) / GN_M));
         hipLaunchKernelGGL((k_gemm_nt<EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, C, ldc);

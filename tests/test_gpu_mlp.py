@@ -31,11 +31,12 @@ def _perturbed(net, seed, scale=0.05):
     return net.layers(net.params())
 
 
-def test_forward_matches_oracle():
-    print("mlp forward", flush=True)
+@pytest.mark.parametrize("ncomp", [4, 8])   # 320 outputs (256-wide tiles), 576 (192-wide tiles)
+def test_forward_matches_oracle(ncomp):
+    print(f"mlp forward ncomp={ncomp}", flush=True)
     s = synth.make_system(N=6, L=8, T=48, dt=1.0, seed=3)
     dev = api.Replicas(s, 2, FP64)
-    net = api.LearnedProposal(dev, block_hours=24, ncomp=4, key=(3, 5))
+    net = api.LearnedProposal(dev, block_hours=24, ncomp=ncomp, key=(3, 5))
     layers = _perturbed(net, 1, scale=2.0)
     X = np.random.default_rng(2).normal(0, 1, (700, net.din))
     out = net.forward(X)
