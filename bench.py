@@ -10,9 +10,13 @@ replicas, regime windows, compensated re-dispatch), one operation sweep of exact
 weights, the storage multiscale sweep with probability 1/2 and the long-range storage sweep),
 the energy totals and the capacity score, one Langevin step of the shared capacities
 and the dispatch following on every replica. The workload is the 40-node / 70-line synthetic
-system at an hourly horizon (8760 steps) with 64 weather replicas; the state is FP32 with FP64
-totals, the Jacobian test and the Langevin step in FP64. Inputs are seeded and resident in HBM
-before the timed region.
+system at an hourly horizon (8760 steps) with 128 weather replicas (the configuration of the
+learned-proposal target); the state is FP32 with FP64 totals, the Jacobian test and the Langevin
+step in FP64. Inputs are seeded and resident in HBM before the timed region.
+Beside the timed steps: the FP64 oracle on the host cores from the device's own state (CPU
+baseline), the learned proposal, the uneven batch, and the ESS/s of total system cost (replica
+driver, plain against learned + batched) and of the operational cost at frozen capacities and
+weather (plain sweeps against plain + learned sweeps).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -37,17 +41,21 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--replicas", type=int, default=128)
     ap.add_argument("--hours", type=int, default=8760)
     ap.add_argument("--nodes", type=int, default=40)
     ap.add_argument("--lines", type=int, default=70)
     ap.add_argument("--fp64", action="store_true", help="FP64 device state")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal leg")
-    ap.add_argument("--ess", action="store_true",
-                    help="also run the replica driver with and without the learned proposal and report the ESS/s "
-                         "of total system cost of both (minutes)")
-    ap.add_argument("--ess-steps", type=int, default=240, help="replica steps of each --ess run (1/6 warmup)")
+    ap.add_argument("--no-ess", action="store_true",
+                    help="skip the ESS legs (the replica driver with and without the learned proposal, ESS/s of "
+                         "total system cost; the operational cost at frozen capacities and weather)")
+    ap.add_argument("--ess-steps", type=int, default=48, help="replica steps of each driver run of the ESS leg "
+                                                               "(1/6 warmup)")
+    ap.add_argument("--ess-chains", type=int, default=2, help="chains of each driver run of the ESS leg")
+    ap.add_argument("--ess-sweeps", type=int, default=40,
+                    help="production sweeps of each run of the frozen-(k, x) dispatch ESS leg")
     ap.add_argument("--ess-batch", choices=("threads", "graph"), default="graph",
                     help="how the learned run of --ess runs its chains' sweeps (the plain run: threads)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
@@ -105,6 +113,14 @@ class Workload:
         return parts
 
 
+def sweep_unit_bytes(s, es):
+    """Algorithmic bytes of an operation sweep per (replica, hour): the v coordinates (4N − 1 of
+    them carry information: one angle is the gauge), r and F read and written once, and the
+    weather inputs (3 capacity factors and the load per node) read once: 2 (4N − 1 + N + L) + 4N
+    values (2,792 B in FP32 at 40 nodes / 70 lines)."""
+    return (2 * (4 * s.N - 1 + s.N + s.L) + 4 * s.N) * es
+
+
 def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
     """Algorithmic HBM bytes per launch of each kernel (reads + writes of state and caches that
     the kernel's algorithm needs, each value once). For the random families: the moves of one
@@ -129,10 +145,13 @@ def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
     for k, f in fam.items():
         launches = max(prof.get(k, (0, 0))[1] / max(steps, 1), 1)
         out[k] = per_move[k] * es * float(stats[f, 0]) / launches
-    # the merged launch of the table-driven families (one per parity class): their moves together
+    # the merged launch of the table-driven families (one per parity class) on the per-(replica, hour)
+    # basis of a sweep: every operation coordinate (θ, gas, two SoC: 4N), the residual and flow
+    # caches read and written once, the weather inputs (3 CF + load: 4N) read once, over the T/2
+    # steps of the class. A per-move sum would count the values that several moves of a step touch
+    # again for every move.
     merged = ("k_cell", "k_xloc", "k_xlocc", "k_xpair")
-    launches = max(prof.get("k_random", (0, 0))[1] / max(steps, 1), 1)
-    out["k_random"] = sum(per_move[k] * es * float(stats[fam[k], 0]) for k in merged) / launches
+    out["k_random"] = sweep_unit_bytes(s, es) * (T // 2) * R
     for k in merged:
         if prof.get(k, (0, 0))[1] == 0:
             out.pop(k)
@@ -211,6 +230,7 @@ def main():
     achieved = kb[dom] / (kern_ms * 1e-3) / 1e9
     breakdown = {k: round(v[0] / args.steps, 4) for k, v in prof.items()}
     traffic = measured_traffic(dom, wl.s, T, R, prec, launch["form"])
+    say(f"timed: {value:.3f} steps/s, {ms_step:.1f} ms per step; dominant kernel {dom} {kern_ms:.2f} ms")
     line_moves = int(wl.last_stats[:, 0].sum())   # exact line moves of one sweep (atoms and random families)
     out = {
         "metric": "sweeps/s (replica steps/s of the plain GPU replica sampler)",
@@ -232,7 +252,13 @@ def main():
                    "launch": launch},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel_ms": kern_ms,
-                     "bytes_per_launch": kb[dom], "launches": cnt},
+                     "bytes_per_launch": kb[dom], "launches": cnt,
+                     "basis": (f"{sweep_unit_bytes(wl.s, es)} B per (replica, hour) x T/2 hours of the parity "
+                               f"class x R" if dom == "k_random" else "per-kernel algorithmic bytes"),
+                     "sweep": {"bytes_per_sweep": sweep_unit_bytes(wl.s, es) * T * R,
+                               "ms_per_sweep": prof["sweep"][0] / max(prof["sweep"][1], 1),
+                               "achieved": sweep_unit_bytes(wl.s, es) * T * R /
+                                           (1e-3 * prof["sweep"][0] / max(prof["sweep"][1], 1)) / 1e9}},
         "kernel_ms_per_step": breakdown,
         "sweep_stats": {name: {"moves": int(st[0]), "accepted": int(st[1]), "improper": int(st[2]),
                                "cut": int(st[3]), "evals_per_move": round(float(st[4]) / max(int(st[0]), 1), 3)}
@@ -240,32 +266,41 @@ def main():
         "total_cost_mean": float(np.mean(parts[:, 0] + parts[:, 1])),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_threads, args.warmup + args.steps)
+        say(f"cpu baseline: {out['cpu_baseline']['value']:.4g} steps/s")
     wl.dev.close()
     if not args.no_learned:
         out["learned"] = learned_leg(args, prec)
+        say(f"learned leg: {out['learned']['learned_sweep_ms']:.1f} ms per learned sweep")
     if not args.no_batch:
         out["batch"] = batch_leg(prec)
-    if args.ess:
+    if not args.no_ess:
         out["ess"] = ess_leg(args)
+        out["ess_dispatch"] = dispatch_ess_leg(args, prec)
     print(json.dumps(out), flush=True)
 
 
-def ess_leg(args, nchains=2, R=16, T=2920):
+def say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def ess_leg(args):
     """ESS/s of total system cost (``ess_bulk`` of the replica-mean total, src/convergence.jl:111,
     over the production steps per second of production wall time) of the replica driver, plain and
     with the learned proposal (training before the first step and in the warmup; its time counted
-    in ``ess_per_s_with_training``), on 2 chains × 16 replicas of the 40-node system at T hours.
-    The plain run sweeps its chains from host threads; the learned run as ``--ess-batch`` says
-    (by default one captured graph per step: the learned + batched sampler)."""
+    in ``ess_per_s_with_training``), on ``--ess-chains`` chains of the bench's replica set (40 nodes,
+    8760 h, 128 replicas by default). The plain run sweeps its chains from host threads; the
+    learned run as ``--ess-batch`` says (by default one captured graph per step: the learned +
+    batched sampler). R-hat is the rank-normalized split R-hat of the same traces."""
     from esmsampling import replicas as RP
     nw = max(args.ess_steps // 6, 2)
-    base = dict(nchains=nchains, replicas=R, mlda_warmup=nw, nsamples=args.ess_steps - nw, replica_init_sweeps=10,
-                synthetic_nodes=args.nodes, synthetic_lines=args.lines, snapshots=T, resolution_hours=1.0,
-                precision="fp32", seed=7)
+    base = dict(nchains=args.ess_chains, replicas=args.replicas, mlda_warmup=nw, nsamples=args.ess_steps - nw,
+                replica_init_sweeps=10, synthetic_nodes=args.nodes, synthetic_lines=args.lines,
+                snapshots=args.hours, resolution_hours=8760.0 / args.hours, precision="fp32", seed=7)
     # a short run first (discarded): the first run of a process pays one-time costs (code objects
     # loaded at their first launch, clocks) that would otherwise fall on the plain run alone
-    RP.run_replicas(RP.load_config({**base, "mlda_warmup": 2, "nsamples": 8, "learned_proposal": True,
+    RP.run_replicas(RP.load_config({**base, "replicas": 16, "snapshots": 2920, "resolution_hours": 3.0,
+                                    "mlda_warmup": 2, "nsamples": 4, "learned_proposal": True,
                                     "learned_train_steps": 10}))
     res = {}
     for name, extra in (("plain", dict(sweep_batch="threads")), ("learned", dict(learned_proposal=True,
@@ -279,10 +314,67 @@ def ess_leg(args, nchains=2, R=16, T=2920):
                      "ess_per_s_with_training": r["ess_per_s_with_training"], "rhat": tot.get("rhat"),
                      "steps_per_s": r["steps_per_s"], "production_s": r["wall"]["production"],
                      "learned": r["learned"]}
+        say(f"ess leg: {name} ESS {r['ess_total']} in {r['wall']['production']:.1f} s, R-hat {tot.get('rhat')}")
     p, l = res["plain"]["ess_per_s"], res["learned"]["ess_per_s_with_training"]
     res["ratio_learned_vs_plain"] = None if not p or l is None else l / p
-    res["config"] = {"nchains": nchains, "replicas": R, "hours": T, "steps": args.ess_steps, "warmup": nw,
-                     "learned_sweep_batch": args.ess_batch}
+    res["config"] = {"nchains": args.ess_chains, "replicas": args.replicas, "hours": args.hours,
+                     "steps": args.ess_steps, "warmup": nw, "learned_sweep_batch": args.ess_batch}
+    return res
+
+
+def dispatch_ess_leg(args, prec, K=8, B=24, train_steps=1000, warm=20):
+    """ESS/s of the operational cost at frozen capacities and weather: the dispatch observable that
+    the learned proposal acts on. One replica set (the bench's shape); every replica is its own
+    chain of its own target (its weather). After ``warm`` plain sweeps from the heuristic start,
+    two runs from the same state: ``--ess-sweeps`` plain operation sweeps, and as many plain
+    sweeps each followed by a learned sweep (the network trained first, its time counted). Per
+    replica the bulk ESS of its op-cost trace (split chain, src/convergence.jl:111); the sum over
+    the replicas per second of wall time, and the z-score of the two runs' replica-mean op cost
+    (matched marginals: both runs sample the same target)."""
+    import numpy as np
+    import torch
+    from esmsampling import api, synth
+    from esmsampling.diagnostics import ess_bulk
+    s = synth.make_system(N=args.nodes, L=args.lines, T=args.hours, dt=8760.0 / args.hours, seed=3)
+    R = args.replicas
+    dev = api.Replicas(s, R, prec)
+    apply_tuning(api, dev, args.tune)
+    api.sample_weather(dev, key=(8, 1))
+    dev.upload(v=synth.heuristic_state(s, dev.download()[2]), k=s.k0)
+    api.operation_sweep(dev, warm, key=(8, 2), counter=0)
+    v0, _, _ = dev.download()
+    res = {}
+    for name in ("plain", "learned"):
+        dev.upload(v=v0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        net = None
+        if name == "learned":
+            net = api.LearnedProposal(dev, block_hours=B, ncomp=K, key=(8, 3))
+            net.train(train_steps, batch=4096, lr=2e-4, key=(8, 4))
+        tr = np.empty((args.ess_sweeps, R))
+        att = acc = 0
+        for q in range(args.ess_sweeps):
+            api.operation_sweep(dev, 1, key=(8, 5), counter=1000 + q)
+            if net is not None:
+                st = net.propose(key=(8, 6), counter=q)
+                att += int(st[0])
+                acc += int(st[1])
+            tr[q] = api.energy_cost(dev)[:, 1]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ess = np.array([ess_bulk([tr[:, j]]) for j in range(R)])
+        res[name] = {"ess_sum": float(ess.sum()), "ess_median": float(np.median(ess)), "wall_s": wall,
+                     "ess_per_s": float(ess.sum()) / wall, "op_mean": float(tr.mean()),
+                     "op_se": float(np.sqrt(np.sum(tr.var(axis=0, ddof=1) / np.maximum(ess, 1.0))) / R),
+                     "learned_acceptance": acc / att if att else None}
+        say(f"dispatch ess: {name} ESS {ess.sum():.1f} over {R} replicas in {wall:.1f} s")
+    dev.close()
+    p, l = res["plain"], res["learned"]
+    res["ratio_learned_vs_plain"] = l["ess_per_s"] / p["ess_per_s"] if p["ess_per_s"] > 0 else None
+    res["z_op_mean"] = (l["op_mean"] - p["op_mean"]) / math.sqrt(l["op_se"] ** 2 + p["op_se"] ** 2)
+    res["config"] = {"nodes": s.N, "hours": s.T, "replicas": R, "sweeps": args.ess_sweeps, "warm_sweeps": warm,
+                     "learned": {"block_hours": B, "components": K, "train_steps": train_steps}}
     return res
 
 
@@ -406,12 +498,18 @@ def batch_leg(prec, reps=2):
             "speedup": res["one_by_one"] / res["graph"], "graph_launches_instantiations_updates": counts}
 
 
+def traffic_profile(s, T, R, prec):
+    """The committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload, if there is one."""
+    from esmsampling import FP32
+    p = ROOT / "profiles" / f"hbm_traffic_replica_step_{s.N}n_{T}h_{R}r_{'fp32' if prec == FP32 else 'fp64'}.json"
+    return p if p.exists() else None
+
+
 def measured_traffic(kernel, s, T, R, prec, form=1):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (calibrated against a plain copy), when it was measured on this exact workload."""
-    from esmsampling import FP32
-    p = ROOT / "profiles" / "hbm_traffic_replica_step_40n_8760h_64r_fp32.json"
-    if not (p.exists() and s.N == 40 and s.L == 70 and T == 8760 and R == 64 and prec == FP32):
+    p = traffic_profile(s, T, R, prec)
+    if p is None:
         return None
     names = {"k_spair": "k_rand_spair<float, true>", "k_soc": "k_soc<float>", "k_gen": "k_gen<float>",
              "k_energy": "k_energy<float, true, false>", "k_follow": "k_follow<float>",
@@ -423,24 +521,27 @@ def measured_traffic(kernel, s, T, R, prec, form=1):
     return json.loads(p.read_text())["bytes_per_dispatch"].get(names.get(kernel, ""))
 
 
-def cpu_baseline(wl, threads):
+def cpu_baseline(wl, threads, it):
     """The FP64 C oracle on the host cores: one full replica step (weather step, operation sweep,
-    energy, score, Langevin, following) on a bounded sample of replicas, scaled to all replicas."""
+    energy, score, Langevin, following) of a bounded sample of the replicas, started from the
+    device's own state after the timed steps (the same operation, weather, regimes and
+    capacities, so the walks are as long as the GPU's), scaled to all replicas."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import numpy as np
     import oracle
+    from esmsampling import api
     nth = threads or os.cpu_count() or 1
     s, R = wl.s, wl.R
     rs = max(1, min(R, nth))
     o = oracle.Oracle(s)
-    m, z, x = o.weather_init(rs, key=(1, 7), counter=0)
-    from esmsampling import synth
-    v = synth.heuristic_state(s, x)
-    k = s.k0.copy()
-    th, wp = wl.controls(0)
+    v, k, x = wl.dev.download()
+    m, z = api.weather_state(wl.dev)
+    v, x, m, z = (np.ascontiguousarray(a[..., :rs]) for a in (v, x, m, z))
+    th, wp = wl.controls(it)
     t0 = time.perf_counter()
-    m, z, x, v, _ = o.weather_step(m, z, x, v, k, th[:rs], wp[:rs], window=60, compensate=True, key=(1, 11), counter=0)
-    v, r, F, _ = o.sweeps(v, k, 1, key=(1, 13), counter=0, x=x, threads=nth)
+    m, z, x, v, _ = o.weather_step(m, z, x, v, k, th[:rs], wp[:rs], window=60, compensate=True, key=(1, 11),
+                                   counter=it)
+    v, r, F, _ = o.sweeps(v, k, 1, key=(1, 13), counter=it, x=x, threads=nth)
     o.energy(v, k, x, want_caches=False)
     sk, fk, _ = o.scores(v, k, x)
     knew, _ = o.langevin(k, sk, fk, np.full(rs, 1.0 / rs), np.zeros((6, s.N)), wl.eta, wl.ks)
@@ -448,8 +549,8 @@ def cpu_baseline(wl, threads):
     dt = time.perf_counter() - t0
     per_step = dt * R / rs
     return {"value": 1.0 / per_step, "unit": "steps/s", "cores": nth, "kind": "port",
-            "sample": f"one full replica step of {rs} replicas x {s.T} steps (oracle, OpenMP over replicas in the "
-                      f"sweep), {dt:.1f} s, scaled x{R / rs:g} to {R} replicas",
+            "sample": f"one full replica step of {rs} of the {R} replicas x {s.T} steps from the device state after "
+                      f"the timed steps (oracle, OpenMP over replicas in the sweep), {dt:.1f} s, scaled x{R / rs:g}",
             "cpu_model": _cpu_model(), "nproc": os.cpu_count()}
 
 

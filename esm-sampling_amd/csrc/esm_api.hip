@@ -589,6 +589,17 @@ int32_t esm_scores(esm_handle *h, double *sk, double *fk) {
     return ESM_OK;
 }
 
+int32_t esm_score_measure(esm_handle *h, int32_t measure) {
+    if (!h) { set_error("esm_score_measure: null handle"); return ESM_EINVAL; }
+    if (measure != ESM_MEASURE_UTILIZATION && measure != ESM_MEASURE_PHYSICAL) {
+        set_error("esm_score_measure: measure must be ESM_MEASURE_UTILIZATION or ESM_MEASURE_PHYSICAL");
+        return ESM_EINVAL;
+    }
+    h->measure = measure;
+    h->scores_valid = false;
+    return ESM_OK;
+}
+
 int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, const double *eta,
                      const double *kappa_s, double kmin, double clip, int32_t box, double *knew, int32_t *nclip) {
     if (!h || !weights || !xi || !eta || !kappa_s || !knew) {

@@ -125,6 +125,7 @@ struct esm_handle {
     double *d_sk = nullptr;       // [R][ESM_NK][N]
     double *d_fk = nullptr;       // [ESM_NK][N]
     bool scores_valid = false;
+    int measure = ESM_MEASURE_UTILIZATION;   // reference measure of the capacity score (esm_score_measure)
     // weather process
     bool has_weather = false, weather_ready = false;
     int wx_chunk = 128;

@@ -202,11 +202,14 @@ __global__ void __launch_bounds__(256) k_score_part(DevModel<S> m, const S *__re
     for (int f = 0; f < ESM_NK; f++) part[(((size_t)c * ESM_NK + f) * m.N + n) * m.R + j] = acc[f];
 }
 
-// sk[j][f][n] = −β_v mask Σ_c part[c][f][n][j]; fk[f][n] = β_k (capex + γ_k capex sign(k − kc)).
+// sk[j][f][n] = −β_v mask Σ_c part[c][f][n][j] + ∂_k log J; fk[f][n] = β_k (capex + γ_k capex sign(k − kc)).
+// With the physical reference measure (nmeas = T, src/energy.jl:811-829) the dispatchable unit
+// (gas) and the two storage units of a node add n_u/(k + κ0), n_u = T utilization coordinates
+// each, where the capacity is inside its box (mask); renewables keep the utilization measure.
 __global__ void k_score_final(const double *__restrict__ part, const double *__restrict__ k, const double *__restrict__ klo,
                               const double *__restrict__ khi, const double *__restrict__ capex, double beta_v,
-                              double beta_k, double gamma_k, int N, int R, int nchunk, double *__restrict__ sk,
-                              double *__restrict__ fk) {
+                              double beta_k, double gamma_k, double kappa0, double nmeas, int N, int R, int nchunk,
+                              double *__restrict__ sk, double *__restrict__ fk) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= R * ESM_NK * N) return;
     const int j = g / (ESM_NK * N);
@@ -220,7 +223,8 @@ __global__ void k_score_final(const double *__restrict__ part, const double *__r
     }
     s += comp;
     const double mask = (klo[u] < k[u] && k[u] < khi[u]) ? 1.0 : 0.0;
-    sk[(size_t)j * ESM_NK * N + u] = -beta_v * (mask * s);
+    const double gj = (u >= ESM_KGAS * N && nmeas > 0) ? mask * (nmeas / (k[u] + kappa0)) : 0.0;
+    sk[(size_t)j * ESM_NK * N + u] = -beta_v * (mask * s) + gj;
     if (j == 0) {
         const double kc = fmin(fmax(k[u], klo[u]), khi[u]);
         const double ex = k[u] - kc;
@@ -319,7 +323,9 @@ int launch_scores(esm_handle *h) {
     }
     hipLaunchKernelGGL(k_score_final, dim3(nblk((size_t)h->R * ESM_NK * h->N, 256)), dim3(256), 0, h->stream,
                        h->d_part2, h->d_k, h->d_klo, h->d_khi,
-                       h->d_capex, h->m.beta_v, h->m.beta_k, h->m.gamma_k, h->N, h->R, nchunk, h->d_sk, h->d_fk);
+                       h->d_capex, h->m.beta_v, h->m.beta_k, h->m.gamma_k, h->m.kappa0,
+                       h->measure == ESM_MEASURE_PHYSICAL ? (double)h->T : 0.0, h->N, h->R, nchunk, h->d_sk,
+                       h->d_fk);
     ESM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -86,6 +86,7 @@ SIGNATURES = [
     ("esm_rebuild_residual", C.c_int32, [C.c_void_p, _dp]),
     ("esm_download_caches", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("esm_scores", C.c_int32, [C.c_void_p, _dp, _dp]),
+    ("esm_score_measure", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_langevin", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32, _dp, _ip]),
     ("esm_sweeps", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32), C.c_uint32, C.c_double,
                                C.POINTER(C.c_int64)]),
@@ -122,6 +123,7 @@ SIGNATURES = [
  KERN_MLPT, KERN_RANDOM) = range(21)
 NFAM, NSTAT, NWEIGHT = 13, 5, 9
 BATCH_MAX = 64
+MEASURES = {"utilization": 0, "physical": 1}   # ESM_MEASURE_*
 # the plain random families and their default weights (plain_families, src/local.jl:1912-1915)
 WEIGHT_NAMES = ("unit", "pair", "xpair", "xlocc", "xloc", "potential", "cellpair", "cell", "spair")
 DEFAULT_WEIGHTS = (0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3)

@@ -115,6 +115,15 @@ def capacity_scores(R: Replicas):
     return sk, fk
 
 
+def score_measure(R: Replicas, measure: str = "utilization"):
+    """The reference measure of the capacity score (``measure``, src/run.jl:36): "utilization"
+    (ŝ_k = −β_v ∂C/∂k) or "physical" (the dispatch of gas and storage units in MW: ŝ_k gains
+    ``log_measure_grad`` = T/(k + κ0) inside the box, src/energy.jl:811-829)."""
+    if measure not in _lib.MEASURES:
+        raise ValueError(f'measure = "{measure}": "utilization" or "physical"')
+    check(R.lib.esm_score_measure(R.handle, _lib.MEASURES[measure]), "esm_score_measure")
+
+
 def langevin_capacities(R: Replicas, weights, xi, eta, kappa_s, kmin: float = 1.0, clip: float = 0.02,
                         box: bool = True):
     """One Langevin step of the shared capacities from the last scores: ``(knew [6, N], nclip)``."""

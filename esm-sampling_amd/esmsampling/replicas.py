@@ -8,15 +8,15 @@ convergence report of the drivers (src/chains.jl:301-310). One chain is one ``ap
 handle (its replicas share the capacities); every device call works on all replicas of the
 chain at once.
 
-Configuration: TOML with the reference's key names (src/run.jl:5-125) and its defaults where
-this path implements the feature; an unknown key raises, as ``load_config`` does
-(src/run.jl:135-150). The system is the seeded synthetic one (``synth.make_system``), set by
-the keys ``synthetic_nodes``, ``synthetic_lines``, ``snapshots``, ``resolution_hours``,
-``system_seed`` and ``precision``, which have no counterpart in the reference (it reads its
-data file), nor do the ``learned_*`` keys of the learned dispatch-block proposal (off by
-default). Keys of features not built here are accepted only at their "off" value (and default
-to it): ``replica_ffbs = false``, ``tilt_lambda = 0``, ``storage_xpair = 0``, ``storage_xlong = 0``
-(the cross-node storage pairs of storage_cross.jl).
+Configuration: TOML with every key of the reference's ``Config`` (src/run.jl:7-127) and its
+defaults; an unknown key raises, as ``load_config`` does (src/run.jl:135-156). Keys that the
+replica path uses are implemented, or accepted at the values this build supports (others are
+refused); keys that the replica path never reads are accepted and have no effect. The system is
+the seeded synthetic one (``synth.make_system``): its nodes and hours per step come from the
+finest level of ``levels`` (or the keys ``synthetic_nodes``, ``resolution_hours``), its regimes
+from ``K``, its countries from the coarsest level's cells; ``synthetic_lines``, ``snapshots``,
+``system_seed`` and ``precision`` have no counterpart in the reference (it reads its data file),
+nor do the ``learned_*`` keys of the learned dispatch-block proposal (off by default).
 
 Random streams: Philox4x32-10 keys per (seed, chain, purpose) replace the MersenneTwister streams
 ``restart_seed(cfg, r, 1000c + j)`` (src/replicas.jl:872-874); the Langevin normals come from a
@@ -42,26 +42,98 @@ NK = synth.NK
 LANGEVIN_KAPPA = {"onwind": 500.0, "solar": 1000.0, "offwind": 200.0, "gas": 300.0, "battery": 100.0, "H2": 100.0}
 
 
+# Default lines of a synthetic network of N nodes (the BASELINE networks: 40 nodes / 70 lines and
+# 256 nodes / 450 lines; otherwise 1.75 lines per node, at least a spanning tree).
+def default_lines(N: int) -> int:
+    return {40: 70, 256: 450}.get(N, max(N - 1, int(round(1.75 * N))))
+
+
 @dataclasses.dataclass
 class Config:
-    """The configuration keys of the replica path (reference names and defaults, src/run.jl)."""
-    seed: int = 1
-    sampler: str = "replicas"
+    """Every key of the reference's ``Config`` (src/run.jl:7-127) with its default, plus this
+    driver's own keys (the synthetic system, the learned proposal, the launch form and the sweep
+    batching). ``check`` sorts the reference keys into three kinds: those the replica path uses
+    (implemented here), those it uses at values this build supports (others are refused with the
+    reference's message style) and those the replica path never reads (accepted, no effect:
+    level-0 warmup, bias learning, the chains' fine step, the device driver's capacity moves)."""
+    # -- data and outputs (src/run.jl:8-14) --
+    data: str = "data/processed/model.h5"
     out: str = ""
-    nchains: int = 4
-    chain_starts: list = dataclasses.field(default_factory=lambda: ["low", "high", "heuristic", "heuristic"])
-    replicas: int = 8
-    mlda_warmup: int = 100
-    nsamples: int = 300
+    cache_dir: str = ""
+    cache_mirror: str = ""
+    checkpoint: str = ""
+    checkpoint_minutes: float = 30.0
+    stop_file: str = ""
+    # -- model and levels --
+    levels: list = dataclasses.field(default_factory=lambda: [(10, 24), (40, 24)])
+    seed: int = 1
+    K: int = 4
     c_shed: float = 3000.0
     c_dump: float = 5.0
     c_viol: float = 3000.0
+    local_extra: float = 4.0
+    fine_kernel: str = "auto"
+    cell_level_power: float = 1.0
+    xloc_partner_hops: int = 1
+    xloc_ball_hops: int = 0
+    kop: bool = False
+    storage_long: float = 1.0
+    storage_long_moves: int = 32
+    storage_long_hours: float = 24.0
+    storage_xpair: float = 0.0
+    storage_xlong: int = 0
+    storage_xpair_hops: int = 1
+    storage_xpair_ball: int = 1
+    measure: str = "utilization"
+    storage: str = "soc"
+    renewables: str = "load"
+    flow_theta: str = "nested"
+    co2_price: float = math.nan
+    co2_cap: float = 0.0
+    brownfield: bool = False
+    losses: bool = False
+    loss_segments: int = 4
+    thermal: str = "hard"
+    thermal_band: float = 0.1
+    link_losses: bool = False
+    uc: list = dataclasses.field(default_factory=list)
+    fine_weather: float = 0.0
+    fine_weather_blocks: float = 1.0
+    fine_weather_block_steps: int = 600
+    init: str = "lp"
+    basis: str = "haar"
+    use_bias: bool = True
     beta_v: float = 4e-5
     beta_k: float = 4e-8
+    calibrate_beta: bool = False
+    eps_v: float = 0.01
+    eps_k: float = 0.05
+    warmup0: int = 1500
+    bias_start: int = 150
+    bias_every: int = 25
+    bias_samples: int = 16
+    bias_sample_steps: int = 4096
+    bias_table_res: int = 100
+    bias_warmstart: float = 20.0
+    descent_iters: int = 4000
+    mlda_warmup: int = 100
+    nsamples: int = 300
+    bias_monotone: bool = True
+    bias_online: bool = True
+    bias_online_every: int = 1
+    bias_online_min: int = 16
+    bias_trust: float = 1.0
+    bias_post: bool = True
+    # -- chains and convergence diagnostics --
+    nchains: int = 4
+    chain_starts: list = dataclasses.field(default_factory=lambda: ["low", "high", "heuristic", "heuristic"])
     rhat_max: float = 1.01
     ess_min: float = 400.0
     mcse_max: float = 0.05
     rhat_screen: float = 1.05
+    # -- the replica–Langevin sampler --
+    sampler: str = "chains"
+    replicas: int = 8
     replica_init_sweeps: int = 40
     replica_inner: int = 1
     replica_theta: float = 0.0
@@ -93,26 +165,24 @@ class Config:
     langevin_kmin: float = 1.0
     langevin_clip: float = 0.02
     langevin_box: bool = True
-    tilt_lambda: float = 0.0
     tilt_functional: str = "resload_mean"
-    storage_long: float = 1.0
-    storage_long_moves: int = 32
-    storage_long_hours: float = 24.0
-    storage_xpair: float = 0.0
-    storage_xlong: int = 0
+    tilt_lambda: float = 0.0
+    tilt_samples: int = 200
+    # -- the device driver (device/): its batch sizes and capacity moves --
+    device_backend: str = "gpu"
+    device_sweeps: int = 10
+    kmove_nprop: int = 2
+    kmove_sigma: float = 0.3
+    kpeak_sigma: float = 0.05
+    device_sampler: str = "mh"
+    verbose: bool = True
+    # ---- keys of this driver (no counterpart in the reference) ----
     maxnats: float = 40.0
-    # snapshots (src/run.jl:12-14): the state snapshot file (resumed from if it exists; "": off),
-    # a snapshot every this many minutes at a step boundary, and the stop file (if it exists at a
-    # step boundary: snapshot, remove it, stop with exit code 75)
-    checkpoint: str = ""
-    checkpoint_minutes: float = 30.0
-    stop_file: str = ""
-    # the learned dispatch-block proposal (no counterpart in the reference): after every
-    # operation sweep, `learned_moves` learned moves of every (node, block, replica), exactly MH
-    # corrected; the network is trained by maximum likelihood on the chain's own states,
-    # `learned_train_steps` Adam steps before the first replica step and `learned_train_more`
-    # every `learned_train_every` warmup steps (never in the production window, whose kernel
-    # stays fixed)
+    # the learned dispatch-block proposal: after every operation sweep, `learned_moves` learned
+    # moves of every (node, block, replica), exactly MH corrected; the network is trained by
+    # maximum likelihood on the chain's own states, `learned_train_steps` Adam steps before the
+    # first replica step and `learned_train_more` every `learned_train_every` warmup steps (never
+    # in the production window, whose kernel stays fixed)
     learned_proposal: bool = False
     learned_block_hours: int = 24
     learned_components: int = 8
@@ -122,34 +192,130 @@ class Config:
     learned_train_more: int = 200
     learned_batch: int = 4096
     learned_lr: float = 2e-4
-    # the synthetic system (no counterpart in the reference, which reads its data file)
-    synthetic_nodes: int = 40
-    synthetic_lines: int = 70
-    snapshots: int = 8760
-    resolution_hours: float = 1.0
+    # the synthetic system that stands in for the data file: nodes and hours per step default to
+    # the finest level (levels[-1] = (nodes, hours)), the horizon to one year, the lines to
+    # default_lines(nodes)
+    synthetic_nodes: int | None = None
+    synthetic_lines: int | None = None
+    snapshots: int | None = None
+    resolution_hours: float | None = None
     system_seed: int = 1
     precision: str = "fp32"
-    # launch form of the random families (no counterpart in the reference): "table" takes the
-    # committed tuning table's choice for the replica sets' shape (esmsampling/tuning.json),
-    # "off" the library default
+    # launch form of the random families: "table" takes the committed tuning table's choice for
+    # the replica sets' shape (esmsampling/tuning.json), "off" the library default
     launch_tuning: str = "table"
-    # how the chains' operation sweeps run (no counterpart in the reference): "graph", the sweeps
-    # of all chains of a step as one captured HIP graph (api.Batch), or "threads", each chain's
-    # sweep from its own host thread; the states are the same
+    # how the chains' operation sweeps run: "graph", the sweeps of all chains of a step as one
+    # captured HIP graph (api.Batch), or "threads", each chain's sweep from its own host thread;
+    # the states are the same
     sweep_batch: str = "graph"
 
+    # the reference keys that the replica path never reads (src/replicas.jl:841-955 and the
+    # kernels it builds): accepted with any value, no effect
+    UNREAD = ("cache_dir", "cache_mirror", "fine_kernel", "kop", "flow_theta", "loss_segments", "thermal_band",
+              "fine_weather", "fine_weather_blocks", "fine_weather_block_steps", "eps_v", "eps_k", "warmup0",
+              "bias_start", "bias_every", "bias_samples", "bias_sample_steps", "bias_table_res", "bias_warmstart",
+              "descent_iters", "bias_monotone", "bias_online", "bias_online_every", "bias_online_min", "bias_trust",
+              "bias_post", "device_backend", "device_sweeps", "kmove_nprop", "kmove_sigma", "kpeak_sigma",
+              "tilt_samples")
+
+    @property
+    def nodes(self) -> int:
+        return int(self.synthetic_nodes if self.synthetic_nodes is not None else self.levels[-1][0])
+
+    @property
+    def lines(self) -> int:
+        return int(self.synthetic_lines if self.synthetic_lines is not None else default_lines(self.nodes))
+
+    @property
+    def hours(self) -> float:
+        return float(self.resolution_hours if self.resolution_hours is not None else self.levels[-1][1])
+
+    @property
+    def steps(self) -> int:
+        return int(self.snapshots if self.snapshots is not None else round(8760 / self.hours))
+
     def check(self):
-        if self.sampler != "replicas":
-            raise ValueError(f'sampler = "{self.sampler}": this driver runs sampler = "replicas" only')
+        """``check_config`` (src/run.jl:158-169) and the replica path's limits."""
+        def refuse(key, msg):
+            raise ValueError(f"{key} = {getattr(self, key)!r}: {msg}")
+        # choice keys of the reference (its own messages)
+        if self.storage != "soc":
+            refuse("storage", 'only the state-of-charge form "soc" exists')
+        if self.renewables != "load":
+            refuse("renewables", 'only renewables as load "load" exist')
+        if self.basis not in ("haar", "flow"):
+            refuse("basis", '"haar" or "flow"')
+        if self.init not in ("lp", "heuristic"):
+            refuse("init", '"lp" or "heuristic"')
+        if self.measure not in ("utilization", "physical"):
+            refuse("measure", '"utilization" or "physical"')
+        if self.fine_kernel not in ("auto", "local", "flow", "adaptive"):
+            refuse("fine_kernel", '"auto", "local", "flow" or "adaptive"')
+        if self.sampler not in ("chains", "replicas"):
+            refuse("sampler", '"chains" or "replicas"')
+        if self.device_sampler not in ("mh", "replicas"):
+            refuse("device_sampler", '"mh" or "replicas"')
+        if self.tilt_functional not in ("none", "resload_mean"):
+            refuse("tilt_functional", '"none" or "resload_mean"')
+        if self.tilt_lambda != 0 and self.tilt_functional == "none":
+            raise ValueError("tilt_lambda ≠ 0 needs a tilt_functional")
+        # values the replica path uses that this build does not implement
+        if self.basis != "haar":
+            refuse("basis", 'the flow basis and its transport families are not built here; use "haar"')
+        if self.init != "heuristic":
+            refuse("init", 'the reference LP start is not built here (no LP solver); use "heuristic"')
+        if self.data != Config.data:
+            refuse("data", "this driver runs the seeded synthetic system (synthetic_* keys); no data file is read")
+        if self.calibrate_beta:
+            refuse("calibrate_beta", "β from the LP needs the LP start; give beta_v and beta_k")
+        if self.co2_cap > 0:
+            refuse("co2_cap", "tuning the CO2 price to a cap is not built here (0: off)")
+        if self.brownfield:
+            refuse("brownfield", "brownfield floors and fleets are not built here")
+        if self.losses:
+            refuse("losses", "line losses are not built here (hard limits without losses)")
+        if self.thermal != "hard":
+            refuse("thermal", 'only hard line limits "hard" are built here')
+        if self.link_losses:
+            refuse("link_losses", "the synthetic system has no DC links")
+        if self.uc:
+            refuse("uc", "unit commitment is not built here")
+        if self.cell_level_power != 1.0:
+            refuse("cell_level_power", "the cell level is drawn ∝ its number of cells (1.0) here")
+        if self.xloc_partner_hops != 1 or self.xloc_ball_hops != 0:
+            raise ValueError("xloc_partner_hops = 1 and xloc_ball_hops = 0 (partners are neighbours, the flow is "
+                             "re-routed on the pair) are the localized pair moves built here")
+        if self.storage_xpair_hops != 1 or self.storage_xpair_ball != 1:
+            raise ValueError("storage_xpair_hops = 1 and storage_xpair_ball = 1 are the storage pairs built here")
         if self.replica_ffbs:
-            raise ValueError("replica_ffbs = true: the centred FFBS of the regimes is not built (its default is false)")
+            refuse("replica_ffbs", "the centred FFBS of the regimes is not built (its default is false)")
         if self.tilt_lambda != 0.0:
-            raise ValueError("tilt_lambda ≠ 0: the stress tilt functional is not built (λ = 0 is the untilted drift)")
+            refuse("tilt_lambda", "the stress tilt functional is not built (λ = 0 is the untilted drift)")
         if self.storage_xpair != 0.0 or self.storage_xlong != 0:
-            raise ValueError("storage_xpair / storage_xlong > 0: the cross-node storage pairs (sxpair, sxlong) are not built")
+            raise ValueError("storage_xpair / storage_xlong > 0: the cross-node storage pairs (sxpair, sxlong) are not "
+                             "built")
+        # levels: (cells, hours per step), coarse to fine (check_levels, src/run.jl:173-183); the
+        # replica sampler runs at the finest level
+        try:
+            self.levels = [(int(a), int(b)) for a, b in self.levels]
+        except (TypeError, ValueError):
+            raise ValueError(f"levels = {self.levels!r}: a list of [cells, hours] pairs") from None
+        if not self.levels:
+            raise ValueError("levels: at least one level")
+        for (M0, h0), (M1, h1) in zip(self.levels, self.levels[1:]):
+            if M1 < M0 or h1 > h0 or h0 % h1 != 0:
+                raise ValueError(f"levels {self.levels}: cells must not decrease and each level's hours must divide "
+                                 f"the coarser level's")
+        if self.hours <= 0 or self.steps < 2 or self.nodes < 2 or self.lines < self.nodes - 1:
+            raise ValueError(f"synthetic system: {self.nodes} nodes, {self.lines} lines, {self.steps} steps of "
+                             f"{self.hours} h")
+        if not 2 <= self.K <= 4:
+            refuse("K", "2 to 4 weather regimes are built here")
+        if not self.local_extra >= 0:
+            refuse("local_extra", "must be >= 0")
         if not (0.0 <= self.storage_long <= 1.0) or not (0 <= self.storage_long_moves <= 256) or self.storage_long_hours < 0:
             raise ValueError("storage_long in [0, 1], storage_long_moves in [0, 256], storage_long_hours >= 0")
-        if self.learned_proposal and not (1 <= self.learned_block_hours <= min(24, self.snapshots)
+        if self.learned_proposal and not (1 <= self.learned_block_hours <= min(24, self.steps)
                                           and 1 <= self.learned_components <= 8 and self.learned_moves >= 0
                                           and self.learned_train_steps >= 0 and self.learned_train_every >= 0
                                           and self.learned_train_more >= 0 and 1 <= self.learned_batch <= 65536
@@ -169,9 +335,25 @@ class Config:
                 raise ValueError(f'chain start "{s}": "low", "high" or "heuristic"')
         return self
 
+    def notes(self):
+        """What this configuration leaves aside (logged at the start of a run, as the reference
+        logs its ignored keys)."""
+        out = []
+        if self.use_bias:
+            out.append("sampler = replicas: the learned bias is not used (use_bias ignored)")
+        if self.data == Config.data:
+            out.append(f"synthetic system: {self.nodes} nodes / {self.lines} lines, {self.steps} steps of "
+                       f"{self.hours:g} h (seed {self.system_seed}) in place of {self.data}")
+        return out
+
+
+# Keys of old configuration files with the only value that matches the code (src/run.jl:133)
+REMOVED_KEYS = {"kernel": "local", "p_fine": 1.0, "alpha0": False, "regularized": True}
+
 
 def load_config(path_or_dict) -> Config:
-    """A Config from a TOML file (or a dict of the same keys); unknown keys raise."""
+    """``load_config`` (src/run.jl:135-156): a Config from a TOML file (or a dict of the same
+    keys); an unknown key raises, a removed key must hold its one value, then ``check``."""
     if isinstance(path_or_dict, dict):
         d = dict(path_or_dict)
     else:
@@ -179,8 +361,13 @@ def load_config(path_or_dict) -> Config:
         with open(path_or_dict, "rb") as f:
             d = tomli.load(f)
     names = {f.name for f in dataclasses.fields(Config)}
-    for k in d:
-        if k not in names:
+    for k in list(d):
+        if k in REMOVED_KEYS:
+            if d[k] != REMOVED_KEYS[k]:
+                raise ValueError(f"config key {k} = {d[k]} is no longer supported (the code always uses "
+                                 f"{REMOVED_KEYS[k]})")
+            del d[k]
+        elif k not in names:
             raise ValueError(f"unknown config key {k}")
     return Config(**d).check()
 
@@ -221,6 +408,8 @@ class ReplicaCtl:
     long_moves: int = 32
     long_hours: float = 24.0
     tuning: str = "table"
+    extra: float = 4.0
+    measure: str = "utilization"
     shrink: float = 0.7
     grow: float = 1.2
 
@@ -239,12 +428,15 @@ class ReplicaCtl:
                    adapt=cfg.langevin_adapt, gap_hi=cfg.langevin_gap_hi, gap_lo=cfg.langevin_gap_lo,
                    eta_min=cfg.langevin_eta_min, eta_max=cfg.langevin_eta_max, plong=cfg.storage_long,
                    long_moves=cfg.storage_long_moves, long_hours=cfg.storage_long_hours,
-                   tuning=cfg.launch_tuning)
+                   tuning=cfg.launch_tuning, extra=cfg.local_extra, measure=cfg.measure)
 
     def sweep_options(self, dev):
-        """The operation sweep's options of a replica set (the long-range storage sweep's keys) and
-        its launch form (from the tuning table when it holds the set's shape)."""
+        """The operation sweep's options of a replica set (``local_extra``, the long-range storage
+        sweep's keys), the reference measure of its scores and its launch form (from the tuning
+        table when it holds the set's shape)."""
+        api.sweep_options(dev, extra=self.extra)
         api.storage_long_options(dev, p=self.plong, nmoves=self.long_moves, hmin=self.long_hours)
+        api.score_measure(dev, self.measure)
         if self.tuning == "table":
             from . import tune
             table = _tuning_table(str(tune.TABLE))
@@ -566,6 +758,22 @@ class BatchedSweeps:
         self.batch.close()
 
 
+def system_of(cfg: Config):
+    """The synthetic system of a configuration: the finest level's nodes and hours per step, the
+    coarsest level's cells as countries (when it is coarser than the nodes), K regimes, the cost
+    keys and temperatures; with ``co2_price`` = μ the gas unit's marginal cost gains μ·ef (the
+    synthetic marginal costs contain no CO2 price)."""
+    M0 = cfg.levels[0][0]
+    sys = synth.make_system(N=cfg.nodes, L=cfg.lines, T=cfg.steps, dt=cfg.hours, seed=cfg.system_seed, K=cfg.K,
+                            ncountry=M0 if 2 <= M0 < cfg.nodes else None)
+    mc = sys.mc
+    if not math.isnan(cfg.co2_price):
+        mc = mc.copy()
+        mc[synth.KGAS] = mc[synth.KGAS] + cfg.co2_price * sys.ef
+    return dataclasses.replace(sys, c_shed=cfg.c_shed, c_dump=cfg.c_dump, c_viol=cfg.c_viol, beta_v=cfg.beta_v,
+                               beta_k=cfg.beta_k, mc=mc)
+
+
 def run_replicas(cfg: Config, log=None):
     """``run_replicas`` (src/replicas.jl:841-947) on one GPU: ``nchains`` chains of ``replicas``
     weather replicas, ``mlda_warmup`` + ``nsamples`` replica steps, probes and step adaptation
@@ -574,10 +782,9 @@ def run_replicas(cfg: Config, log=None):
     cfg = cfg.check() if isinstance(cfg, Config) else load_config(cfg)
     say = log or (lambda s: None)
     t_setup = time.perf_counter()
-    sys = synth.make_system(N=cfg.synthetic_nodes, L=cfg.synthetic_lines, T=cfg.snapshots, dt=cfg.resolution_hours,
-                            seed=cfg.system_seed)
-    sys = dataclasses.replace(sys, c_shed=cfg.c_shed, c_dump=cfg.c_dump, c_viol=cfg.c_viol, beta_v=cfg.beta_v,
-                              beta_k=cfg.beta_k)
+    sys = system_of(cfg)
+    for note in cfg.notes():
+        say(note)
     prec = FP64 if cfg.precision == "fp64" else FP32
     ctl = ReplicaCtl.from_config(cfg)
     nc, M = cfg.nchains, cfg.replicas

@@ -120,13 +120,13 @@ def _graph(rng, N, L):
     return pos, bus0, bus1
 
 
-def _countries(pos):
+def _countries(pos, C=None):
     """The middle level of the partition hierarchy (the reference's countries, E.parts,
-    src/energy.jl:292): max(2, N // 5) angular sectors of the node positions around their centroid,
-    of near-equal size, labelled in order of first appearance in node order. Uses no random
-    numbers, so it leaves the rest of the system unchanged."""
+    src/energy.jl:292): C angular sectors (max(2, N // 5) by default) of the node positions around
+    their centroid, of near-equal size, labelled in order of first appearance in node order. Uses
+    no random numbers, so it leaves the rest of the system unchanged."""
     N = len(pos)
-    C = max(2, N // 5)
+    C = max(2, N // 5) if C is None else int(C)
     ang = np.arctan2(pos[:, 1] - pos[:, 1].mean(), pos[:, 0] - pos[:, 0].mean())
     order = np.argsort(ang, kind="stable")
     sector = np.empty(N, dtype=np.int64)
@@ -137,8 +137,17 @@ def _countries(pos):
     return C, np.array([relabel[int(s)] for s in sector], dtype=np.int32)
 
 
-def make_system(N: int = 40, L: int = 70, T: int = 8760, dt: float = 1.0, seed: int = 1) -> System:
-    """The synthetic system with N nodes, L lines and T steps of dt hours (T·dt = 8760 for one year)."""
+def make_system(N: int = 40, L: int = 70, T: int = 8760, dt: float = 1.0, seed: int = 1, K: int = 2,
+                ncountry: int | None = None) -> System:
+    """The synthetic system with N nodes, L lines and T steps of dt hours (T·dt = 8760 for one year),
+    K weather regimes (switch probability 0.02 per step, spread evenly over the other regimes;
+    regime intercepts evenly spaced in [−a, a]) and ``ncountry`` countries (max(2, N // 5) by
+    default). K and ncountry use no random numbers: at K = 2 the weather is the BASELINE
+    2-regime process."""
+    if not 2 <= K <= 4:
+        raise ValueError("K: 2 to 4 regimes")
+    if ncountry is not None and not 2 <= ncountry <= N:
+        raise ValueError("ncountry: 2 to N countries")
     rng = np.random.default_rng(seed)
     pos, bus0, bus1 = _graph(rng, N, L)
     x = np.exp(rng.uniform(math.log(0.01), math.log(0.2), L))
@@ -183,17 +192,17 @@ def make_system(N: int = 40, L: int = 70, T: int = 8760, dt: float = 1.0, seed: 
     qt = np.zeros((NSTOR, N))
 
     # weather: series i = c*N + n
-    K = 2
-    Ptr = np.array([[0.98, 0.02], [0.02, 0.98]])
-    pi0 = np.array([0.5, 0.5])
+    Ptr = np.full((K, K), 0.02 / (K - 1))
+    np.fill_diagonal(Ptr, 0.98)
+    pi0 = np.full(K, 1.0 / K)
     phi, sigma = 0.9, 0.3
     sigma0 = sigma / math.sqrt(1.0 - phi * phi)
     nser = NREN * N
     mu = np.zeros((K, nser, 3))
     amp = np.repeat(np.array([0.02, 0.05, 0.05]), N)          # regime intercept contrast per carrier
     seas = np.repeat(np.array([-0.03, 0.03, 0.03]), N)        # annual harmonic (solar peaks in summer)
-    mu[0, :, 0] = -amp
-    mu[1, :, 0] = amp
+    for q in range(K):
+        mu[q, :, 0] = amp * (2.0 * q / (K - 1) - 1.0)
     mu[:, :, 1] = seas[None, :]
     mu[:, :, 2] = 0.005 * rng.standard_normal(nser)[None, :]
     xbar = np.zeros((nser, 24))
@@ -217,7 +226,7 @@ def make_system(N: int = 40, L: int = 70, T: int = 8760, dt: float = 1.0, seed: 
                   load=load, mc=mc, ef=ef, capex=capex, klo=klo, khi=khi, stor_hours=hours,
                   stor_eta_in=eta_in, stor_eta_out=eta_out, stor_qt=qt, K=K, Ptr=Ptr, pi0=pi0,
                   phi=phi, sigma=sigma, sigma0=sigma0, mu=mu, xbar=xbar, k0=k0, pos=pos,
-                  ncountry=_countries(pos)[0], country=_countries(pos)[1])
+                  ncountry=_countries(pos, ncountry)[0], country=_countries(pos, ncountry)[1])
 
 
 def hour_of_day(T: int, dt: float) -> np.ndarray:

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 6
+#define ESM_ABI_VERSION 7
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -158,6 +158,19 @@ int32_t esm_download_caches(esm_handle *h, double *r, double *F);
  * esm_energy or esm_rebuild_residual first when the caches may have drifted.
  */
 int32_t esm_scores(esm_handle *h, double *sk, double *fk);
+
+/*
+ * The reference measure of the capacity score (`measure`, src/run.jl:36; `log_measure_grad`,
+ * src/energy.jl:811-829, added to the score by `capacity_scores`, src/replicas.jl:334):
+ * ESM_MEASURE_UTILIZATION (the default) leaves ŝ_k = −β_v ∂C/∂k; ESM_MEASURE_PHYSICAL measures
+ * the dispatch of the dispatchable unit and of the storage units in MW (dp = k dω), which adds
+ * ∂_k log J = T/(k + κ0) to their fields (gas, battery, H2) where the capacity is strictly inside
+ * its box (T utilization coordinates per unit; renewables keep the utilization measure). Applies
+ * to the following esm_scores calls.
+ */
+#define ESM_MEASURE_UTILIZATION 0
+#define ESM_MEASURE_PHYSICAL 1
+int32_t esm_score_measure(esm_handle *h, int32_t measure);
 
 /*
  * `langevin_capacities` / `langevin_update` (src/replicas.jl:368-403) of the shared

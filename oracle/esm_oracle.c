@@ -293,6 +293,32 @@ int orc_scores(const esm_model *m, int R, const double *v, const double *k, cons
     return 0;
 }
 
+/* log_measure, src/energy.jl:790-806: with the physical reference measure, n_u log(kc_u + κ0) per
+ * dispatchable unit without weather availability (gas: n_u = T, no unit commitment) and per
+ * storage unit (n_u = T), kc the capacity clamped to its box; renewables (weather-dependent
+ * availability) keep the utilization measure. 0 for the utilization measure. */
+double orc_log_measure(const esm_model *m, const double *k, int physical) {
+    if (!physical) return 0.0;
+    double val = 0.0;
+    for (int f = ESM_KGAS; f < ESM_NK; f++)
+        for (int n = 0; n < m->N; n++) {
+            int u = f * m->N + n;
+            val += (double)m->T * log(clampd(k[u], m->klo[u], m->khi[u]) + m->kappa0);
+        }
+    return val;
+}
+
+/* log_measure_grad, src/energy.jl:811-829: n_u/(k + κ0) where lo < k < hi, else 0, into g [NK][N]. */
+void orc_log_measure_grad(const esm_model *m, const double *k, int physical, double *g) {
+    for (int u = 0; u < ESM_NK * m->N; u++) g[u] = 0.0;
+    if (!physical) return;
+    for (int f = ESM_KGAS; f < ESM_NK; f++)
+        for (int n = 0; n < m->N; n++) {
+            int u = f * m->N + n;
+            if (m->klo[u] < k[u] && k[u] < m->khi[u]) g[u] = (double)m->T / (k[u] + m->kappa0);
+        }
+}
+
 /* langevin_update / langevin_capacities, src/replicas.jl:357-403 */
 int orc_langevin(const esm_model *m, int R, const double *k, const double *sk, const double *fk,
                  const double *weights, const double *xi, const double *eta, const double *kappa_s,

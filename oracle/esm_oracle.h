@@ -28,6 +28,9 @@ int orc_scores(const esm_model *m, int R, const double *v, const double *k, cons
                double *sk, double *fk, double *sk_abs);
 
 /* langevin_update of every k field with eta[f] > 0 (see esm_langevin). */
+/* log_measure / log_measure_grad (src/energy.jl:790-829); physical = 0: the utilization measure */
+double orc_log_measure(const esm_model *m, const double *k, int physical);
+void orc_log_measure_grad(const esm_model *m, const double *k, int physical, double *g);
 int orc_langevin(const esm_model *m, int R, const double *k, const double *sk, const double *fk,
                  const double *weights, const double *xi, const double *eta, const double *kappa_s,
                  double kmin, double clip, int box, double *knew, int *nclip);

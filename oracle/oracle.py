@@ -35,6 +35,10 @@ def load():
         lib.orc_scores.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
         lib.orc_langevin.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
                                      _dp, _ip]
+        lib.orc_log_measure.argtypes = [M, _dp, C.c_int]
+        lib.orc_log_measure.restype = C.c_double
+        lib.orc_log_measure_grad.argtypes = [M, _dp, C.c_int, _dp]
+        lib.orc_log_measure_grad.restype = None
         lib.orc_philox.argtypes = [_u32p, _u32p, _u32p]
         lib.orc_philox.restype = None
         lib.orc_weather_init.argtypes = [M, C.c_int, _u32p, C.c_uint32, _ip, _dp, _dp]
@@ -108,7 +112,9 @@ class Oracle:
         assert rc == 0
         return parts, ph, r, F
 
-    def scores(self, v, k, x):
+    def scores(self, v, k, x, measure="utilization"):
+        """(sk [R, NK, N], fk [NK, N], |terms| of sk): ``capacity_scores`` with the reference
+        measure ``measure`` ("utilization" or "physical": sk gains ``log_measure_grad``)."""
         s = self.sys
         R = v.shape[-1]
         v, k, x = (np.ascontiguousarray(a, dtype=np.float64) for a in (v, k, x))
@@ -116,7 +122,18 @@ class Oracle:
         fk = np.empty((NK, s.N))
         sa = np.empty((R, NK, s.N))
         assert self.lib.orc_scores(C.byref(self.cm), R, _p(v), _p(k), _p(x), _p(sk), _p(fk), _p(sa)) == 0
-        return sk, fk, sa
+        g = self.log_measure_grad(k, measure)
+        return sk + g[None], fk, sa + np.abs(g)[None]
+
+    def log_measure(self, k, measure="physical"):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        return self.lib.orc_log_measure(C.byref(self.cm), _p(k), int(measure == "physical"))
+
+    def log_measure_grad(self, k, measure="physical"):
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        g = np.empty((NK, self.sys.N))
+        self.lib.orc_log_measure_grad(C.byref(self.cm), _p(k), int(measure == "physical"), _p(g))
+        return g
 
     def langevin(self, k, sk, fk, weights, xi, eta, kappa_s, kmin=1.0, clip=0.02, box=True):
         R = sk.shape[0]

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_without_gpu_work():
     from esmsampling import _lib
     lib = _lib.load()
-    assert lib.esm_version() == 6
+    assert lib.esm_version() == 7
     # an invalid argument is rejected on the host, before any device call
     h = C.c_void_p()
     assert lib.esm_create(None, 1, 64, C.byref(h)) == -1
@@ -50,4 +50,5 @@ def test_batch_and_autotune_reject_null_arguments_without_gpu_work():
     assert lib.esm_sweep_block(None, 128) == -1
     assert lib.esm_sweep_pad(None, 1) == -1
     assert lib.esm_autotune(None, 1, None, 0, 40.0, None, None, None) == -1
+    assert lib.esm_score_measure(None, 1) == -1 and b"null" in lib.esm_last_error()
     assert _lib.BATCH_MAX == 64

@@ -43,7 +43,7 @@ def test_copy_replica_matches_source():
 
 def test_run_replicas_short():
     print("run_replicas 2 chains x 4 replicas", flush=True)
-    cfg = RP.load_config(dict(nchains=2, replicas=4, mlda_warmup=10, nsamples=14, replica_init_sweeps=2,
+    cfg = RP.load_config(dict(init="heuristic", nchains=2, replicas=4, mlda_warmup=10, nsamples=14, replica_init_sweeps=2,
                               replica_burst_every=4, replica_gate_min=1, replica_gate_max=2, replica_gate_min_window=1,
                               replica_gate_max_window=3, replica_gate_win=2, replica_probe_every=3,
                               replica_probe_len=4, replica_probe_tail=2, replica_probe_win=1, replica_probe_avg=1,
@@ -75,7 +75,7 @@ def test_run_replicas_learned_proposal():
     (never in the production window), one learned sweep after every operation sweep, its counts
     in the records, and the report with the training time counted."""
     print("run_replicas learned 2 chains x 8 replicas", flush=True)
-    cfg = RP.load_config(dict(nchains=2, replicas=8, mlda_warmup=10, nsamples=10, replica_init_sweeps=4,
+    cfg = RP.load_config(dict(init="heuristic", nchains=2, replicas=8, mlda_warmup=10, nsamples=10, replica_init_sweeps=4,
                               replica_probe_every=0, synthetic_nodes=10, synthetic_lines=16, snapshots=96,
                               resolution_hours=1.0, precision="fp32", seed=5, learned_proposal=True,
                               learned_train_steps=300, learned_train_every=5, learned_train_more=50,
@@ -97,7 +97,7 @@ def test_chains_in_threads_match_a_chain_alone():
     """The chains run in host threads, each on its own handle and stream: chain 0 of a three-chain
     run is bitwise the chain of a one-chain run with the same seed (its own keys, no shared state)."""
     print("run_replicas threads vs alone", flush=True)
-    base = dict(replicas=4, mlda_warmup=6, nsamples=6, replica_init_sweeps=2, replica_burst_every=4,
+    base = dict(init="heuristic", resolution_hours=1.0, replicas=4, mlda_warmup=6, nsamples=6, replica_init_sweeps=2, replica_burst_every=4,
                 replica_probe_every=3, replica_probe_len=3, replica_probe_tail=1, replica_probe_win=1,
                 replica_probe_avg=1, replica_gate_min=1, replica_gate_max=2, replica_gate_min_window=1,
                 replica_gate_max_window=3, replica_gate_win=2, synthetic_nodes=10, synthetic_lines=16, snapshots=48,
@@ -121,7 +121,7 @@ def test_stop_file_snapshot_and_resume(tmp_path):
     resumed from its snapshot continues as the run without a stop: the same traces, capacities
     and Langevin steps (FP64), the records of both parts, the outputs of the finished run."""
     print("stop file, snapshot, resume", flush=True)
-    base = dict(nchains=2, replicas=4, mlda_warmup=10, nsamples=20, replica_init_sweeps=2, replica_burst_every=4,
+    base = dict(init="heuristic", resolution_hours=1.0, nchains=2, replicas=4, mlda_warmup=10, nsamples=20, replica_init_sweeps=2, replica_burst_every=4,
                 replica_probe_every=6, replica_probe_len=3, replica_probe_tail=1, replica_probe_win=1,
                 replica_probe_avg=1, replica_gate_min=1, replica_gate_max=2, replica_gate_min_window=1,
                 replica_gate_max_window=3, replica_gate_win=2, synthetic_nodes=10, synthetic_lines=16, snapshots=48,

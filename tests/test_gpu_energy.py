@@ -61,7 +61,9 @@ def test_rebuild_drift_zero(sys40, prec):
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-def test_scores(sys40, prec):
+@pytest.mark.parametrize("measure", ["utilization", "physical"])
+def test_scores(sys40, prec, measure):
+    print(f"scores parity prec={prec} measure={measure}", flush=True)
     s = sys40
     R = 4
     v, k, x = random_state(s, R, seed=9)
@@ -69,8 +71,9 @@ def test_scores(sys40, prec):
         v, x = _round32(v, x)
     dev = api.Replicas(s, R, prec)
     dev.upload(v=v, k=k, x=x)
+    api.score_measure(dev, measure)
     sk, fk = api.capacity_scores(dev)
-    sk_, fk_, sa = oracle.Oracle(s).scores(v, k, x)
+    sk_, fk_, sa = oracle.Oracle(s).scores(v, k, x, measure=measure)
     np.testing.assert_allclose(fk, fk_, rtol=1e-14)
     tol = 1e-12 if prec == FP64 else 1e-5
     # each entry is a sum of terms of both signs: the round-off scale is the sum of their magnitudes

@@ -11,9 +11,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-def test_weather_init_matches_oracle(prec):
-    print(f"weather init prec={prec}", flush=True)
-    s = synth.make_system(N=8, L=12, T=500, seed=2)
+@pytest.mark.parametrize("K", [2, 4])
+def test_weather_init_matches_oracle(prec, K):
+    print(f"weather init prec={prec} K={K}", flush=True)
+    s = synth.make_system(N=8, L=12, T=500, seed=2, K=K)
     R = 6
     dev = api.Replicas(s, R, prec)
     api.sample_weather(dev, key=(11, 5), counter=7)
@@ -28,10 +29,10 @@ def test_weather_init_matches_oracle(prec):
 
 
 @pytest.mark.parametrize("prec", [FP64, FP32])
-@pytest.mark.parametrize("compensate", [True, False])
-def test_weather_step_matches_oracle(prec, compensate):
-    print(f"weather step prec={prec} compensate={compensate}", flush=True)
-    s = synth.make_system(N=8, L=12, T=700, seed=3)
+@pytest.mark.parametrize("compensate,K", [(True, 2), (False, 2), (True, 4)])
+def test_weather_step_matches_oracle(prec, compensate, K):
+    print(f"weather step prec={prec} compensate={compensate} K={K}", flush=True)
+    s = synth.make_system(N=8, L=12, T=700, seed=3, K=K)
     R = 8
     o = oracle.Oracle(s)
     m, z, x = o.weather_init(R, key=(2, 2), counter=0)

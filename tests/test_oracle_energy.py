@@ -188,6 +188,58 @@ def test_scores_against_central_differences(small_system):
     np.testing.assert_allclose(fk, sys.beta_k * sys.capex * (1 + sys.gamma_k * np.sign(k - kc)), rtol=1e-14)
 
 
+def test_physical_measure_scores_against_logtarget_differences(small_system):
+    """With measure = "physical": −fk + ŝ_k is the gradient of the log target at fixed (v, x),
+    logtarget = −β_k K − β_v op + log J (src/hierarchy.jl:90; the reference's pin
+    test/test_replicas.jl:179-203 with its replica config's measure), against central
+    differences of the oracle's capex, op and log J in every node capacity, kinks skipped.
+    log J itself: zero for the utilization measure, invariant under moving k beyond its box
+    (it uses the clamped capacity), and additive over units: lowering one gas capacity from k to
+    k' changes it by exactly T log((k' + κ0)/(k + κ0))."""
+    sys = small_system
+    R = 2
+    v, k, x = random_state(sys, R, seed=23)
+    o = oracle.Oracle(sys)
+    assert o.log_measure(k, "utilization") == 0.0
+    k2 = k.copy()
+    k2[synth.KGAS, 0] = 0.5 * k[synth.KGAS, 0] + 0.5 * sys.klo[synth.KGAS, 0]
+    assert o.log_measure(k2) - o.log_measure(k) == pytest.approx(
+        sys.T * np.log((k2[synth.KGAS, 0] + sys.kappa0) / (k[synth.KGAS, 0] + sys.kappa0)), rel=1e-12)
+    k3 = k.copy()
+    k3[:, 1] = sys.khi[:, 1] + 5.0
+    k4 = k.copy()
+    k4[:, 1] = sys.khi[:, 1]
+    assert o.log_measure(k3) == pytest.approx(o.log_measure(k4), rel=1e-15)
+    sk, fk, sa = o.scores(v, k, x, measure="physical")
+    sk_u, _, _ = o.scores(v, k, x)
+    assert np.all(sk[:, :synth.KGAS] == sk_u[:, :synth.KGAS])      # renewables: the utilization measure
+
+    def logtarget(kk):
+        p = o.energy(v, kk, x, want_caches=False)[0]
+        return -sys.beta_k * p[:, 0] - sys.beta_v * p[:, 1] + o.log_measure(kk)
+
+    checked = 0
+    L0 = logtarget(k)
+    for f in range(6):
+        for n in range(sys.N):
+            if not (sys.klo[f, n] < k[f, n] < sys.khi[f, n]):
+                continue
+            hstep = 1e-4 * max(1.0, k[f, n])
+            kp, km = k.copy(), k.copy()
+            kp[f, n] += hstep
+            km[f, n] -= hstep
+            Lp, Lm = logtarget(kp), logtarget(km)
+            fd = (Lp - Lm) / (2 * hstep)
+            right, left = (Lp - L0) / hstep, (L0 - Lm) / hstep
+            smooth = np.abs(right - left) <= 1e-6 * (np.abs(right) + np.abs(left) + 1e-9)
+            for j in range(R):
+                if smooth[j]:
+                    scale = sa[j, f, n] + abs(fk[f, n]) + 1e-12
+                    assert -fk[f, n] + sk[j, f, n] == pytest.approx(fd[j], rel=1e-5, abs=1e-9 * scale)
+                    checked += 1
+    assert checked >= 10 * R
+
+
 def _langevin_one_field(sys, k, g, xi, eta=1e-3, ks=100.0, kmin=1.0, clip=np.inf, box=False, hi=None):
     """The oracle's Langevin step on field 0 only, with the drift g set through fk = −g, sk = 0."""
     import copy

@@ -25,7 +25,7 @@ def _recs(rng, steps, M, N):
 
 def test_outputs_layout_and_summary(tmp_path):
     rng = np.random.default_rng(1)
-    cfg = RP.load_config({"replicas": 3, "nchains": 2, "mlda_warmup": 2, "nsamples": 3})
+    cfg = RP.load_config({"init": "heuristic", "replicas": 3, "nchains": 2, "mlda_warmup": 2, "nsamples": 3})
     ctl = RP.ReplicaCtl.from_config(cfg)
     s = synth.make_system(N=5, L=6, T=24, dt=1.0, seed=2)
     recs = [_recs(rng, 5, 3, s.N) for _ in range(2)]
@@ -71,7 +71,7 @@ def test_rng_state_round_trip():
 
 
 def test_snapshot_checks(tmp_path):
-    cfg = RP.load_config({"seed": 4, "nchains": 2})
+    cfg = RP.load_config({"init": "heuristic", "seed": 4, "nchains": 2})
     assert rio.read_snapshot(str(tmp_path / "none.npz"), cfg) is None
     p = tmp_path / "ck.npz"
     base = {"version": np.array(1), "driver": np.array("replicas"), "config/seed": np.array(4), "nchains": np.array(2)}
@@ -90,6 +90,6 @@ def test_snapshot_checks(tmp_path):
 
 
 def test_snapshot_config_keys():
-    c = RP.load_config({"stop_file": "/tmp/x", "checkpoint_minutes": 5.0})
+    c = RP.load_config({"init": "heuristic", "stop_file": "/tmp/x", "checkpoint_minutes": 5.0})
     assert c.checkpoint == "" and c.stop_file == "/tmp/x" and c.checkpoint_minutes == 5.0
-    assert RP.load_config({}).checkpoint_minutes == 30.0
+    assert RP.load_config({"init": "heuristic"}).checkpoint_minutes == 30.0

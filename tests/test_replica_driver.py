@@ -86,35 +86,98 @@ def test_interior_and_masked_means():
 
 
 def test_config_keys():
-    c = RP.load_config({"replicas": 16, "nchains": 2, "langevin_kappa": {"solar": 900.0}})
+    c = RP.load_config({"init": "heuristic", "replicas": 16, "nchains": 2, "langevin_kappa": {"solar": 900.0}})
     ctl = RP.ReplicaCtl.from_config(c)
     assert ctl.M == 16 and ctl.kappa_s[0] == 900.0 and ctl.kappa_s[1] == 500.0 and ctl.kappa_s[5] == 100.0
     with pytest.raises(ValueError, match="unknown config key"):
-        RP.load_config({"replica_typo": 1})
+        RP.load_config({"init": "heuristic", "replica_typo": 1})
     with pytest.raises(ValueError, match="FFBS"):
-        RP.load_config({"replica_ffbs": True})
-    c = RP.load_config({"storage_long": 0.5, "storage_long_moves": 8, "storage_long_hours": 100.0})
+        RP.load_config({"init": "heuristic", "replica_ffbs": True})
+    c = RP.load_config({"init": "heuristic", "storage_long": 0.5, "storage_long_moves": 8, "storage_long_hours": 100.0})
     ctl = RP.ReplicaCtl.from_config(c)
     assert (ctl.plong, ctl.long_moves, ctl.long_hours) == (0.5, 8, 100.0)
-    d = RP.ReplicaCtl.from_config(RP.load_config({}))
+    d = RP.ReplicaCtl.from_config(RP.load_config({"init": "heuristic"}))
     assert (d.plong, d.long_moves, d.long_hours) == (1.0, 32, 24.0)     # src/run.jl:27-29
     with pytest.raises(ValueError, match="sxpair"):
-        RP.load_config({"storage_xpair": 0.1})
+        RP.load_config({"init": "heuristic", "storage_xpair": 0.1})
     with pytest.raises(ValueError, match="sxpair"):
-        RP.load_config({"storage_xlong": 8})
+        RP.load_config({"init": "heuristic", "storage_xlong": 8})
     with pytest.raises(ValueError, match="storage_long"):
-        RP.load_config({"storage_long_moves": 300})
-    c = RP.load_config({"learned_proposal": True, "learned_block_hours": 12, "learned_components": 4})
+        RP.load_config({"init": "heuristic", "storage_long_moves": 300})
+    c = RP.load_config({"init": "heuristic", "learned_proposal": True, "learned_block_hours": 12, "learned_components": 4})
     assert c.learned_block_hours == 12 and c.learned_components == 4 and c.learned_moves == 1
-    assert RP.load_config({}).learned_proposal is False
+    assert RP.load_config({"init": "heuristic"}).learned_proposal is False
     for bad in ({"learned_block_hours": 25}, {"learned_components": 9}, {"learned_lr": 0.0}, {"learned_batch": 70000}):
         with pytest.raises(ValueError, match="learned_"):
-            RP.load_config({"learned_proposal": True, **bad})
+            RP.load_config({"init": "heuristic", "learned_proposal": True, **bad})
 
 
 def test_config_toml(tmp_path):
     p = tmp_path / "r.toml"
-    p.write_text('sampler = "replicas"\nnchains = 1\nreplicas = 4\nmlda_warmup = 3\nnsamples = 5\n'
+    p.write_text('sampler = "replicas"\ninit = "heuristic"\nnchains = 1\nreplicas = 4\nmlda_warmup = 3\nnsamples = 5\n'
                  'chain_starts = ["high"]\nlangevin_eta = 1e-4\n')
     c = RP.load_config(str(p))
     assert c.nchains == 1 and c.replicas == 4 and c.chain_starts == ["high"] and c.langevin_eta == 1e-4
+
+
+# Key sets of the reference's configuration files (configs/*.toml), written out here: the
+# replica driver accepts every key of the reference's Config, runs the ones the replica path
+# uses, and refuses values it does not build with the reference's message style.
+DEVICE40_REPLICA = dict(out="results/device40_replica.h5", basis="haar", storage="soc", renewables="load",
+                        measure="physical", levels=[[10, 24], [40, 24]], init="heuristic", use_bias=False,
+                        nchains=4, chain_starts=["low", "high"], mlda_warmup=100, nsamples=300,
+                        device_backend="gpu", device_sweeps=10, device_sampler="replicas", replicas=4)
+CHAINS40 = dict(out="results/chains40.h5", basis="haar", measure="physical", levels=[[10, 24], [40, 24]],
+                init="heuristic", use_bias=False, fine_weather=0.05, nchains=4, mlda_warmup=20, nsamples=80)
+
+
+def test_reference_config_key_sets():
+    c = RP.load_config(DEVICE40_REPLICA)     # configs/device40_replica.toml
+    assert (c.nodes, c.hours, c.steps, c.lines) == (40, 24.0, 365, 70)
+    assert c.measure == "physical" and c.K == 4 and c.levels == [(10, 24), (40, 24)]
+    ctl = RP.ReplicaCtl.from_config(c)
+    assert ctl.measure == "physical" and ctl.extra == 4.0
+    assert "use_bias ignored" not in " ".join(c.notes())
+    c = RP.load_config(CHAINS40)             # configs/chains40.toml: fine_weather is not read by the replica path
+    assert c.fine_weather == 0.05 and c.sampler == "chains"
+    # configs/haar_poc.toml: no init key, so the LP start (not built) -> refused, named
+    with pytest.raises(ValueError, match='init = .lp.*not built'):
+        RP.load_config(dict(out="results/haar_poc.h5", basis="haar", levels=[[10, 24], [40, 24], [40, 6]],
+                            use_bias=False, descent_iters=1500, warmup0=30, mlda_warmup=10, nsamples=30))
+    # configs/flow.toml and complete40.toml: the flow basis -> refused, named
+    with pytest.raises(ValueError, match="basis = 'flow'"):
+        RP.load_config(dict(out="results/flow.h5", basis="flow", levels=[[10, 24], [40, 24], [40, 6]], use_bias=False,
+                            descent_iters=0, warmup0=10, mlda_warmup=5, nsamples=20, init="heuristic"))
+    for key, val, msg in (("losses", True, "losses"), ("thermal", "soft", "hard"), ("uc", ["gas"], "unit commitment"),
+                          ("link_losses", True, "links"), ("brownfield", True, "brownfield"),
+                          ("data", "data/processed/model256.h5", "synthetic"), ("storage", "x", "soc"),
+                          ("measure", "x", "physical"), ("co2_cap", 1e8, "cap"), ("K", 6, "regimes")):
+        with pytest.raises(ValueError, match=msg):
+            RP.load_config(dict(DEVICE40_REPLICA, **{key: val}))
+    # removed keys: accepted at their one value only (src/run.jl:133-141)
+    assert RP.load_config(dict(DEVICE40_REPLICA, kernel="local")).replicas == 4
+    with pytest.raises(ValueError, match="no longer supported"):
+        RP.load_config(dict(DEVICE40_REPLICA, kernel="flow"))
+    # every reference key has a field, with the reference's default
+    d = RP.Config()
+    assert (d.storage_long, d.storage_long_moves, d.storage_long_hours, d.local_extra, d.K) == (1.0, 32, 24.0, 4.0, 4)
+    assert d.init == "lp" and d.sampler == "chains" and d.levels == [(10, 24), (40, 24)] and d.use_bias
+
+
+def test_levels_set_the_synthetic_system():
+    c = RP.load_config(dict(DEVICE40_REPLICA, levels=[[8, 24], [40, 3]]))
+    s = RP.system_of(c)
+    assert (s.N, s.L, s.T, s.dt) == (40, 70, 2920, 3.0) and s.ncountry == 8 and s.K == 4
+    np.testing.assert_allclose(s.Ptr.sum(axis=1), 1.0)
+    assert np.all(np.diag(s.Ptr) == 0.98)
+    # explicit synthetic keys win; a CO2 price adds μ·ef to the gas marginal cost
+    c = RP.load_config(dict(DEVICE40_REPLICA, synthetic_nodes=10, synthetic_lines=16, snapshots=48,
+                            resolution_hours=1.0, K=2, co2_price=80.0))
+    s = RP.system_of(c)
+    base = RP.system_of(RP.load_config(dict(DEVICE40_REPLICA, synthetic_nodes=10, synthetic_lines=16, snapshots=48,
+                                            resolution_hours=1.0, K=2)))
+    assert (s.N, s.L, s.T, s.K) == (10, 16, 48, 2)
+    np.testing.assert_allclose(s.mc[3], base.mc[3] + 80.0 * base.ef)
+    assert np.array_equal(s.mc[:3], base.mc[:3])
+    with pytest.raises(ValueError, match="levels"):
+        RP.load_config(dict(DEVICE40_REPLICA, levels=[[40, 24], [10, 24]]))

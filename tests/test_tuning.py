@@ -25,14 +25,14 @@ def test_tuning_table_entries_are_valid_launch_forms():
 
 
 def test_launch_tuning_key():
-    assert replicas.load_config({"sampler": "replicas"}).launch_tuning == "table"
-    assert replicas.load_config({"sampler": "replicas", "launch_tuning": "off"}).launch_tuning == "off"
+    assert replicas.load_config({"init": "heuristic", "sampler": "replicas"}).launch_tuning == "table"
+    assert replicas.load_config({"init": "heuristic", "sampler": "replicas", "launch_tuning": "off"}).launch_tuning == "off"
     with pytest.raises(ValueError, match="launch_tuning"):
-        replicas.load_config({"sampler": "replicas", "launch_tuning": "live"})
+        replicas.load_config({"init": "heuristic", "sampler": "replicas", "launch_tuning": "live"})
 
 
 def test_sweep_batch_key():
-    assert replicas.load_config({"sampler": "replicas"}).sweep_batch == "graph"
-    assert replicas.load_config({"sampler": "replicas", "sweep_batch": "threads"}).sweep_batch == "threads"
+    assert replicas.load_config({"init": "heuristic", "sampler": "replicas"}).sweep_batch == "graph"
+    assert replicas.load_config({"init": "heuristic", "sampler": "replicas", "sweep_batch": "threads"}).sweep_batch == "threads"
     with pytest.raises(ValueError, match="sweep_batch"):
-        replicas.load_config({"sampler": "replicas", "sweep_batch": "persistent"})
+        replicas.load_config({"init": "heuristic", "sampler": "replicas", "sweep_batch": "persistent"})
