@@ -500,6 +500,16 @@ int32_t esm_upload_state(esm_handle *h, const double *v, const double *k, const 
     return ESM_OK;
 }
 
+int32_t esm_heuristic_state(esm_handle *h) {
+    if (!h) { set_error("esm_heuristic_state: null handle"); return ESM_EINVAL; }
+    int rc;
+    if ((rc = launch_heuristic(h))) return rc;
+    h->scores_valid = false;
+    if ((rc = rebuild(h))) return rc;
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return ESM_OK;
+}
+
 int32_t esm_download_state(esm_handle *h, double *v, double *k, double *x) {
     if (!h) { set_error("esm_download_state: null handle"); return ESM_EINVAL; }
     const size_t TR = (size_t)h->T * h->R;

@@ -252,5 +252,7 @@ int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0,
 // Copy replica js of hs into replica jd of hd (operation, raw weather and, with `weather`, the
 // anomalies and regimes) on hd's stream. Both handles have the same shape and precision.
 int launch_copy_replica(const esm_handle *hs, int js, esm_handle *hd, int jd, bool weather);
+// The heuristic start state of every replica from the capacities and the raw weather (k_copy.hip).
+int launch_heuristic(esm_handle *h);
 
 }  // namespace esm

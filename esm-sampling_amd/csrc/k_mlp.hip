@@ -41,10 +41,11 @@ typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 // GEMM: C[i][j] = Σ_k a(i, k) b(k, j) over k < K, i < M, j < N, with
 //   a(i, k) = TA ? A[k·lda + i] : A[i·lda + k],  b(k, j) = TB ? B[j·ldb + k] : B[k·ldb + j]
 // epilogues: EPI_F32 store FP32 (+ bias), EPI_RELU bias + ReLU to BF16, EPI_MASK BF16 of
-// C ⊙ (H > 0) (H: BF16 [M][N], the layer's activation), EPI_ATOMIC FP32 atomic add (split over
-// K by blockIdx.z, K-chunk kc)
+// C ⊙ (H > 0) (H: BF16 [M][N], the layer's activation), EPI_PART the FP32 partial sum of K-split
+// blockIdx.z (K-chunk kc) stored at C[z][i][j] (ldc = N), summed over the splits in split order
+// by k_sum_splits: the gradient does not depend on the order in which workgroups finish
 // ---------------------------------------------------------------------------------------------
-enum { EPI_F32, EPI_RELU, EPI_MASK, EPI_ATOMIC };
+enum { EPI_F32, EPI_RELU, EPI_MASK, EPI_PART };
 constexpr int GB_M = 64, GB_N = 64, GB_K = 32, G_PAD = 8;
 
 template <bool TA, bool TB, int EPI>
@@ -102,8 +103,8 @@ __global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kc, const
         const int i = i0 + wr * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), j = j0 + wc * 32 + (lane & 31);
         if (i >= M || j >= N) continue;
         float x = acc[v];
-        if (EPI == EPI_ATOMIC) {
-            atomicAdd((float *)C + (size_t)i * ldc + j, x);
+        if (EPI == EPI_PART) {
+            ((float *)C)[((size_t)blockIdx.z * M + i) * ldc + j] = x;
             continue;
         }
         if (bias && EPI != EPI_MASK) x += bias[j];
@@ -310,13 +311,36 @@ static int gemm_nt(hipStream_t s, int M, int N, int K, const uint16_t *A, int ld
     return 0;
 }
 
+// C[i][j] += Σ_z part[z][i][j] over the nz K-splits, in split order
+__global__ void k_sum_splits(const float *__restrict__ part, int nz, int M, int N, float *__restrict__ C, int ldc) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (size_t)M * N) return;
+    const int i = (int)(g / N), j = (int)(g % N);
+    float acc = 0.0f;
+    for (int z = 0; z < nz; z++) acc += part[((size_t)z * M + i) * N + j];
+    C[(size_t)i * ldc + j] += acc;
+}
+
+constexpr int KSPLIT_MAX = 32;   // K-splits of the weight-gradient GEMMs (≥ 512 workgroups at 256 × 256)
+
 template <bool TA, bool TB, int EPI>
 static int gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
-                const float *bias, const uint16_t *H, void *C, int ldc, int ksplit = 1) {
+                const float *bias, const uint16_t *H, void *C, int ldc, int ksplit = 1, float *part = nullptr) {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     const int kc = ((K + ksplit - 1) / ksplit + GB_K - 1) / GB_K * GB_K;
-    const dim3 grid((unsigned)((N + GB_N - 1) / GB_N), (unsigned)((M + GB_M - 1) / GB_M), (unsigned)((K + kc - 1) / kc));
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, kc, A, lda, B, ldb, bias, H, C, ldc);
+    const int nz = (K + kc - 1) / kc;
+    const dim3 grid((unsigned)((N + GB_N - 1) / GB_N), (unsigned)((M + GB_M - 1) / GB_M), (unsigned)nz);
+    if (EPI == EPI_PART) {
+        if (!part || nz > KSPLIT_MAX) {
+            set_error("gemm: split partials need a buffer of at most KSPLIT_MAX splits");
+            return ESM_EINVAL;
+        }
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, kc, A, lda, B, ldb, bias, H, part, N);
+        hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)(((size_t)M * N + 255) / 256)), dim3(256), 0, s, part, nz, M, N,
+                           (float *)C, ldc);
+    } else {
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, s, M, N, K, kc, A, lda, B, ldb, bias, H, C, ldc);
+    }
     ESM_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -497,7 +521,7 @@ __global__ void __launch_bounds__(256) k_mlp_head_grad(const float *OUT, const f
         if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+    if (threadIdx.x == 0) loss[blockIdx.x] = red[0];
 }
 
 // the learned move of instances m0 … m0 + cnt − 1: draw y' from the proposal, accept with the
@@ -578,15 +602,23 @@ __global__ void __launch_bounds__(256) k_mlp_propose(DevModel<S> m, SweepState<S
 }
 
 // bias gradient: db[j] += Σ_i DOUT[i][j] (BF16 [cnt][n]); one thread per column and chunk of
-// COLSUM_ROWS rows
+// COLSUM_ROWS rows writes the chunk's sum to part[chunk][j]; k_sum_splits adds the chunks in order
 constexpr int COLSUM_ROWS = 64;
-__global__ void k_colsum(const uint16_t *G, int cnt, int n, float *db) {
+__global__ void k_colsum(const uint16_t *G, int cnt, int n, float *part) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const int r0 = blockIdx.y * COLSUM_ROWS, r1 = min(cnt, r0 + COLSUM_ROWS);
     float s = 0.0f;
     for (int i = r0; i < r1; i++) s += bf2f(G[(size_t)i * n + j]);
-    atomicAdd(db + j, s);
+    part[(size_t)blockIdx.y * n + j] = s;
+}
+
+// loss[0] += Σ_b part[b] over nb block sums, in block order
+__global__ void k_sum_loss(const double *__restrict__ part, int nb, double *__restrict__ loss) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (int b = 0; b < nb; b++) acc += part[b];
+    loss[0] += acc;
 }
 
 // Adam on the FP32 master parameters; the BF16 copy of every layer's Wt [out][in]
@@ -634,6 +666,9 @@ struct MlpState {
     uint16_t *X = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *DG[4] = {nullptr, nullptr, nullptr, nullptr};
     float *OUT = nullptr, *Y = nullptr, *MU0 = nullptr, *YP = nullptr;
     double *LR = nullptr, *dloss = nullptr;   // loss sums of up to MLP_LOSS_CHUNK training steps
+    double *lpart = nullptr;                  // per-block loss sums of one step
+    float *part = nullptr;                    // K-split and row-chunk partial sums of the gradients
+    size_t part_len = 0;
     long long *idx = nullptr;
     unsigned long long *stats = nullptr;   // [ESM_NSTAT][STAT_SLOTS]
     std::vector<void *> allocs;
@@ -680,15 +715,19 @@ static int mlp_backward(esm_handle *h, int cnt) {
     MlpState &s = *h->mlp;
     const MlpDims &D = s.D;
     int rc;
-    const int ksplit = (cnt + 255) / 256;   // 256 rows per workgroup: ≥ 128 workgroups at 4096 rows
+    // 256 rows per K-split (≥ 128 workgroups at 4096 rows), at most KSPLIT_MAX splits
+    const int ksplit = std::min(KSPLIT_MAX, (cnt + 255) / 256);
+    const int nchunk = (cnt + COLSUM_ROWS - 1) / COLSUM_ROWS;
     for (int l = 3; l >= 0; l--) {
         const uint16_t *act = l == 0 ? s.X : s.H[l - 1];   // the layer's input [cnt][lin]
         // dWt [lout][lin] += Σ_m DG[l][m][o] act[m][k]
-        if ((rc = gemm<true, false, EPI_ATOMIC>(h->stream, s.lout[l], s.lin[l], cnt, s.DG[l], s.lout[l], act, s.lin[l],
-                                                 nullptr, nullptr, s.g + s.ow[l], s.lin[l], ksplit)))
+        if ((rc = gemm<true, false, EPI_PART>(h->stream, s.lout[l], s.lin[l], cnt, s.DG[l], s.lout[l], act, s.lin[l],
+                                               nullptr, nullptr, s.g + s.ow[l], s.lin[l], ksplit, s.part)))
             return rc;
-        hipLaunchKernelGGL(k_colsum, dim3((unsigned)((s.lout[l] + 255) / 256), (unsigned)((cnt + COLSUM_ROWS - 1) / COLSUM_ROWS)), dim3(256),
-                           0, h->stream, s.DG[l], cnt, s.lout[l], s.g + s.ob[l]);
+        hipLaunchKernelGGL(k_colsum, dim3((unsigned)((s.lout[l] + 255) / 256), (unsigned)nchunk), dim3(256), 0, h->stream,
+                           s.DG[l], cnt, s.lout[l], s.part);
+        hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((s.lout[l] + 255) / 256)), dim3(256), 0, h->stream, s.part, nchunk,
+                           1, s.lout[l], s.g + s.ob[l], s.lout[l]);
         ESM_HIP_CHECK(hipGetLastError());
         if (l > 0)   // dact [cnt][lin] = DG[l] · Wt, masked by the ReLU of the activation
             if ((rc = gemm<false, false, EPI_MASK>(h->stream, cnt, s.lin[l], s.lout[l], s.DG[l], s.lout[l],
@@ -784,6 +823,16 @@ int32_t esm_mlp_init(esm_handle *h, int32_t block_hours, int32_t ncomp, const ui
         ALLOC(s.YP, c * D.B * 4);
         ALLOC(s.LR, c * 8);
         ALLOC(s.dloss, (size_t)MLP_LOSS_CHUNK * 8);
+        ALLOC(s.lpart, (c / MLP_GPB + 1) * 8);
+        {
+            size_t wmax = 0, omax = 0;
+            for (int l = 0; l < 4; l++) {
+                wmax = std::max(wmax, (size_t)lin[l] * lout[l]);
+                omax = std::max(omax, (size_t)lout[l]);
+            }
+            s.part_len = std::max((size_t)KSPLIT_MAX * wmax, (c + COLSUM_ROWS - 1) / COLSUM_ROWS * omax);
+        }
+        ALLOC(s.part, s.part_len * 4);
         ALLOC(s.idx, c * 8);
         ALLOC(s.stats, (size_t)ESM_NSTAT * STAT_SLOTS * 8);
 #undef ALLOC
@@ -861,8 +910,10 @@ int32_t esm_mlp_gradient(esm_handle *h, const double *x, const double *y, const 
     if ((rc = mlp_forward(h, (int)M))) return rc;
     ESM_HIP_CHECK(hipMemsetAsync(s.g, 0, s.np * 4, h->stream));
     ESM_HIP_CHECK(hipMemsetAsync(s.dloss, 0, 8, h->stream));
-    hipLaunchKernelGGL(k_mlp_head_grad, dim3((unsigned)((M + MLP_GPB - 1) / MLP_GPB)), dim3(256), 0, h->stream, s.OUT, s.MU0, s.Y,
-                       (int)M, D.K, D.B, D.dout, 1.0f / (float)M, s.DG[3], s.dloss);
+    const unsigned nbk = (unsigned)((M + MLP_GPB - 1) / MLP_GPB);
+    hipLaunchKernelGGL(k_mlp_head_grad, dim3(nbk), dim3(256), 0, h->stream, s.OUT, s.MU0, s.Y, (int)M, D.K, D.B, D.dout,
+                       1.0f / (float)M, s.DG[3], s.lpart);
+    hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, h->stream, s.lpart, (int)nbk, s.dloss);
     ESM_HIP_CHECK(hipGetLastError());
     if ((rc = mlp_backward(h, (int)M))) return rc;
     std::vector<float> g(s.np);
@@ -900,8 +951,10 @@ int32_t esm_mlp_train(esm_handle *h, int32_t steps, int32_t batch, double lr, co
             if (rc) return rc;
             if ((rc = mlp_forward(h, batch))) return rc;
             ESM_HIP_CHECK(hipMemsetAsync(s.g, 0, s.np * 4, h->stream));
-            hipLaunchKernelGGL(k_mlp_head_grad, dim3((unsigned)((batch + MLP_GPB - 1) / MLP_GPB)), dim3(256), 0, h->stream, s.OUT, s.MU0,
-                               s.Y, batch, D.K, D.B, D.dout, 1.0f / (float)batch, s.DG[3], s.dloss + (it - c0));
+            const unsigned nbk = (unsigned)((batch + MLP_GPB - 1) / MLP_GPB);
+            hipLaunchKernelGGL(k_mlp_head_grad, dim3(nbk), dim3(256), 0, h->stream, s.OUT, s.MU0, s.Y, batch, D.K, D.B,
+                               D.dout, 1.0f / (float)batch, s.DG[3], s.lpart);
+            hipLaunchKernelGGL(k_sum_loss, dim3(1), dim3(64), 0, h->stream, s.lpart, (int)nbk, s.dloss + (it - c0));
             ESM_HIP_CHECK(hipGetLastError());
             if ((rc = mlp_backward(h, batch))) return rc;
             s.adam_t++;

@@ -7,7 +7,7 @@
 
 namespace esm {
 
-constexpr int PMAX = 48;      // walk points per side (guard: t = 0 beyond)
+constexpr int PMAX = 4096;    // walk points per side (guard: t = 0 beyond)
 
 template <typename S>
 __device__ __forceinline__ S phibal(S x, S cs, S cd) { return x > 0 ? cs * x : -cd * x; }

@@ -82,6 +82,7 @@ SIGNATURES = [
     ("esm_destroy", C.c_int32, [C.c_void_p]),
     ("esm_upload_state", C.c_int32, [C.c_void_p, _dp, _dp, _dp]),
     ("esm_download_state", C.c_int32, [C.c_void_p, _dp, _dp, _dp]),
+    ("esm_heuristic_state", C.c_int32, [C.c_void_p]),
     ("esm_energy", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("esm_rebuild_residual", C.c_int32, [C.c_void_p, _dp]),
     ("esm_download_caches", C.c_int32, [C.c_void_p, _dp, _dp]),

@@ -75,6 +75,13 @@ class Replicas:
             args.append(dptr(a))
         check(self.lib.esm_upload_state(self._h, *args), "esm_upload_state")
 
+    def heuristic(self):
+        """The heuristic start state of every replica, set on the device from the current
+        capacities and raw weather (``esm_heuristic_state``: zero angles, gas covering the net
+        load, storage half full); the caches are rebuilt. The same values as
+        ``synth.heuristic_state`` without staging the state on the host."""
+        check(self.lib.esm_heuristic_state(self._h), "esm_heuristic_state")
+
     def download(self):
         N, T, R = self.N, self.T, self.R
         v = np.empty((NV, N, T, R))
@@ -105,6 +112,14 @@ def LocalCost(R: Replicas):
     check(R.lib.esm_rebuild_residual(R.handle, dptr(drift)), "esm_rebuild_residual")
     r, F = R.caches()
     return r, F, drift
+
+
+def rebuild_drift(R: Replicas):
+    """Rebuild the caches from the node arrays and return the drift of the incremental residual
+    cache per replica, max|r_cache − r| ``[R]`` (without downloading the caches)."""
+    drift = np.empty(R.R)
+    check(R.lib.esm_rebuild_residual(R.handle, dptr(drift)), "esm_rebuild_residual")
+    return drift
 
 
 def capacity_scores(R: Replicas):

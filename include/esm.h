@@ -131,6 +131,16 @@ int32_t esm_upload_state(esm_handle *h, const double *v, const double *k, const 
 int32_t esm_download_state(esm_handle *h, double *v, double *k, double *x);
 
 /*
+ * The heuristic start of every replica on the device (the role of `heuristic_start`,
+ * src/run.jl:469, for this system): zero angles, the gas utilization covering the node's net
+ * load, clamp((load − Σ_c k_c x_c)/k_gas, 0, 1) from the raw capacities and the raw weather of
+ * the replica, both states of charge 0.5; the caches are rebuilt. No host arrays: for replica
+ * sets whose state is too large to stage on the host (e.g. 256 nodes × 192,720 steps × 32
+ * replicas). Usually after esm_weather_init.
+ */
+int32_t esm_heuristic_state(esm_handle *h);
+
+/*
  * `energy_cost` of every replica (src/energy.jl:551-677) and `LocalCost` rebuild
  * (src/local.jl:313-374): rebuilds the residual r [N][T][R] and flow F [L][T][R] caches from
  * the node arrays, and writes per replica the CostParts [R][ESM_NPARTS] (capex, op, gen,

@@ -44,12 +44,21 @@ int orc_weather_step(const esm_model *m, int R, const double *theta, const doubl
                      int *moved);
 /* Explicit-uniform building blocks (the reference's own explicit forms): categorical inversion
  * (0-based result) and the regime window redraw of one regime path m [T] over [a, b] (1-based). */
+/* Centred FFBS of the regimes of the masked replicas (mask NULL: all) given z: m [T][R] redrawn
+ * from p(m | z); loglik [R] (may be NULL): log p(z) of the forward filter (src/weather.jl:160-174). */
+int orc_weather_ffbs(const esm_model *m, int R, const double *z, int *mreg, const int *mask, const uint32_t key[2],
+                     uint32_t ctr, double *loglik);
 int orc_categorical(const double *p, int K, double u);
 void orc_regime_window_path(const esm_model *m, int *path, int a, int b, const double *us);
 
 /* Operation sweeps (oracle_sweep.c), in place on v and the caches r, F (built by orc_energy),
  * with the options of esm_sweep_options (weights NULL: the reference's); stats [ESM_NFAM][5] as
  * esm_sweeps (the evaluation count is the oracle's own); nthreads OpenMP threads over the replicas. */
+/* The reference's own sweep schedule (local_sweep! with the plain families, per replica, shuffled
+ * per step; no storage sweeps): the reference chain of the chain-marginal exactness check. */
+int orc_ref_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
+                   const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights,
+                   int64_t *stats, int nthreads);
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
                const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
                double p_storage_ms, double p_storage_long, int long_moves, double long_hours, int64_t *stats,

@@ -44,6 +44,7 @@ def load():
         lib.orc_weather_init.argtypes = [M, C.c_int, _u32p, C.c_uint32, _ip, _dp, _dp]
         lib.orc_weather_step.argtypes = [M, C.c_int, _dp, _dp, C.c_int, C.c_int, _u32p, C.c_uint32, _ip, _dp, _dp,
                                          _dp, _dp, _ip]
+        lib.orc_weather_ffbs.argtypes = [M, C.c_int, _dp, _ip, _ip, _u32p, C.c_uint32, _dp]
         lib.orc_categorical.argtypes = [_dp, C.c_int, C.c_double]
         lib.orc_categorical.restype = C.c_int
         lib.orc_regime_window_path.argtypes = [M, _ip, C.c_int, C.c_int, _dp]
@@ -52,6 +53,8 @@ def load():
         lib.orc_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double,
                                    C.c_double, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, i64p,
                                    C.c_int]
+        lib.orc_ref_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double,
+                                       C.c_double, _dp, i64p, C.c_int]
         lib.orc_slong_cost.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                        _dp, C.c_int, _dp]
         lib.orc_slong_draw_many.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -171,6 +174,20 @@ class Oracle:
                                          moved.ctypes.data_as(_ip)) == 0
         return m, z, x, v, moved
 
+    def weather_ffbs(self, m, z, mask=None, key=(1, 0), counter=0):
+        """Centred FFBS of the regimes (``ffbs``, src/weather.jl:160-174) of the masked replicas
+        given z, on a copy: returns (m, log p(z) per replica from the forward filter)."""
+        R = m.shape[1]
+        m = np.ascontiguousarray(m, dtype=np.int32).copy()
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        mk = None if mask is None else np.ascontiguousarray(mask, dtype=np.int32)
+        ll = np.full(R, np.nan)
+        kk = (C.c_uint32 * 2)(*key)
+        rc = self.lib.orc_weather_ffbs(C.byref(self.cm), R, _p(z), m.ctypes.data_as(_ip),
+                                       None if mk is None else mk.ctypes.data_as(_ip), kk, int(counter), _p(ll))
+        assert rc == 0, rc
+        return m, ll
+
     def sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1,
                extra=4.0, weights=None, atoms=True, p_storage_ms=0.5, p_storage_long=1.0, long_moves=32,
                long_hours=24.0):
@@ -189,6 +206,27 @@ class Oracle:
                                  float(maxnats), float(extra), _p(w), int(bool(atoms)), float(p_storage_ms),
                                  float(p_storage_long), int(long_moves), float(long_hours),
                                  st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
+        assert rc == 0, rc
+        return v, r, F, st.reshape(13, 5)
+
+    def ref_sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1,
+                   extra=4.0, weights=None):
+        """Sweeps in the reference's own schedule (``local_sweep!`` with the plain families: per
+        replica and step, the atoms and the family draws in one shuffled list; no storage sweeps),
+        in place on copies; returns (v, r, F, stats [13, 5])."""
+        R = v.shape[-1]
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        if r is None or F is None:
+            _, _, r, F = self.energy(v, k, x)
+        r = np.ascontiguousarray(r, dtype=np.float64).copy()
+        F = np.ascontiguousarray(F, dtype=np.float64).copy()
+        st = np.zeros(65, dtype=np.int64)
+        kk = (C.c_uint32 * 2)(*key)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        rc = self.lib.orc_ref_sweeps(C.byref(self.cm), R, _p(v), _p(k), _p(r), _p(F), int(nsweeps), kk, int(counter),
+                                     float(maxnats), float(extra), _p(w), st.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     int(threads))
         assert rc == 0, rc
         return v, r, F, st.reshape(13, 5)
 

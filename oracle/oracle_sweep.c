@@ -23,7 +23,7 @@
 
 #include "esm_oracle.h"
 
-#define PMAX 48
+#define PMAX 4096
 #define MAXE 8      /* entries per step   */
 #define MAXL 16     /* lines per step     */
 #define MAXC 24     /* balance cells      */
@@ -468,19 +468,19 @@ static double trexp_offset(double g, double L, double u) {
  * by inversion of the first uniform, the offset (trexp_offset) with the second (sample_line!,
  * src/local.jl:1530-1560); status 0 drawn, 2 a non-finite draw */
 static double draw_points(const Pts *PLp, const Pts *PRp, double useg, double uoff, int *status) {
-    const Pts PL = *PLp, PR = *PRp;
+    const Pts *PL = PLp, *PR = PRp;
     *status = 0;
-    int np = PL.n + 1 + PR.n;
-    double pts[2 * PMAX + 1], cs[2 * PMAX + 1], lm[2 * PMAX];
-    for (int q = 0; q < PL.n; q++) {
-        pts[q] = PL.p[PL.n - 1 - q];
-        cs[q] = PL.c[PL.n - 1 - q];
+    int np = PL->n + 1 + PR->n;
+    static __thread double pts[2 * PMAX + 1], cs[2 * PMAX + 1], lm[2 * PMAX];
+    for (int q = 0; q < PL->n; q++) {
+        pts[q] = PL->p[PL->n - 1 - q];
+        cs[q] = PL->c[PL->n - 1 - q];
     }
-    pts[PL.n] = 0.0;
-    cs[PL.n] = 0.0;
-    for (int q = 0; q < PR.n; q++) {
-        pts[PL.n + 1 + q] = PR.p[q];
-        cs[PL.n + 1 + q] = PR.c[q];
+    pts[PL->n] = 0.0;
+    cs[PL->n] = 0.0;
+    for (int q = 0; q < PR->n; q++) {
+        pts[PL->n + 1 + q] = PR->p[q];
+        cs[PL->n + 1 + q] = PR->c[q];
     }
     int nseg = np - 1;
     double mx = -INFINITY;
@@ -2582,6 +2582,40 @@ int orc_slong_apply(const esm_model *m, int R, double *v, const double *k, doubl
     return 0;
 }
 
+/* the view of a replica set for a sweep: state pointers, clamped capacities, incident lines */
+static void sys_setup(Sys *S0, const esm_model *m, int R, double *v, const double *k, double *r, double *F) {
+    int N = m->N, L = m->L;
+    size_t NTR = (size_t)N * m->T * R;
+    S0->m = m;
+    S0->R = R;
+    S0->th = v;
+    S0->yg = v + NTR;
+    S0->ye[0] = v + 2 * NTR;
+    S0->ye[1] = v + 3 * NTR;
+    S0->r = r;
+    S0->F = F;
+    for (int u = 0; u < ESM_NK * N; u++) S0->kc[u] = clampd(k[u], m->klo[u], m->khi[u]);
+    S0->adj_ptr = (int *)calloc(N + 1, sizeof(int));
+    S0->adj_line = (int *)malloc(sizeof(int) * 2 * (L > 0 ? L : 1));
+    int *fill = (int *)malloc(sizeof(int) * N);
+    for (int e = 0; e < L; e++) {
+        S0->adj_ptr[m->line_bus0[e] + 1]++;
+        S0->adj_ptr[m->line_bus1[e] + 1]++;
+    }
+    for (int n = 0; n < N; n++) S0->adj_ptr[n + 1] += S0->adj_ptr[n];
+    for (int n = 0; n < N; n++) fill[n] = S0->adj_ptr[n];
+    for (int e = 0; e < L; e++) {
+        S0->adj_line[fill[m->line_bus0[e]]++] = e;
+        S0->adj_line[fill[m->line_bus1[e]]++] = e;
+    }
+    free(fill);
+}
+
+static void sys_free(Sys *S0) {
+    free(S0->adj_ptr);
+    free(S0->adj_line);
+}
+
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
                const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
                double pms, double plong, int long_moves, double long_hours, int64_t *stats, int nthreads) {
@@ -2591,30 +2625,9 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
     if (sms_tables(m, &smst)) return -3;
     static const double wdef[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
     const double *wts = weights ? weights : wdef;
-    size_t NTR = (size_t)N * T * R;
     Sys S0;
-    S0.m = m;
-    S0.R = R;
-    S0.th = v;
-    S0.yg = v + NTR;
-    S0.ye[0] = v + 2 * NTR;
-    S0.ye[1] = v + 3 * NTR;
-    S0.r = r;
-    S0.F = F;
-    for (int u = 0; u < ESM_NK * N; u++) S0.kc[u] = clampd(k[u], m->klo[u], m->khi[u]);
-    S0.adj_ptr = (int *)calloc(N + 1, sizeof(int));
-    S0.adj_line = (int *)malloc(sizeof(int) * 2 * L);
-    int *fill = (int *)malloc(sizeof(int) * N), *colour = (int *)malloc(sizeof(int) * N);
-    for (int e = 0; e < L; e++) {
-        S0.adj_ptr[m->line_bus0[e] + 1]++;
-        S0.adj_ptr[m->line_bus1[e] + 1]++;
-    }
-    for (int n = 0; n < N; n++) S0.adj_ptr[n + 1] += S0.adj_ptr[n];
-    for (int n = 0; n < N; n++) fill[n] = S0.adj_ptr[n];
-    for (int e = 0; e < L; e++) {
-        S0.adj_line[fill[m->line_bus0[e]]++] = e;
-        S0.adj_line[fill[m->line_bus1[e]]++] = e;
-    }
+    sys_setup(&S0, m, R, v, k, r, F);
+    int *colour = (int *)malloc(sizeof(int) * N);
     int nc = colouring(m, S0.adj_ptr, S0.adj_line, colour);
     Tables tb;
     if (build_tables(m, &tb)) return -2;
@@ -2666,8 +2679,182 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
         }
     }
     if (stats) memcpy(stats, st, sizeof(st));
-    free(S0.adj_ptr); free(S0.adj_line); free(fill); free(colour); free(counts);
+    sys_free(&S0);
+    free(colour); free(counts);
     tables_free(&tb);
     return 0;
 }
 
+
+
+/* ============================================================================================
+ * The reference's own schedule of the operation sweep, for the chain-marginal exactness check
+ * (test/test_exact.jl:71-106): `local_sweep!` (src/local.jl:1927-2062) with the plain families.
+ * Per replica and sweep: the time-parity classes in a shuffled order (:1946-1949); per step τ one
+ * list holding every atom (a node unit vector of each field: angle, gas, the two states of
+ * charge; :1963-1966) and round(extra·N) draws over the family weights (:1968-1971), shuffled
+ * (:1972), each move selected, drawn and committed in that order (:1973-2044) with the
+ * selection of prep_random! / prep_xloc! / prep_spair! (:1608-1725): unit, potential, spair
+ * at a uniform node (and storage carrier); xpair at a uniform ordered pair of distinct nodes;
+ * xloc / xlocc at a uniform node and a uniform neighbour; cell at a partition level drawn
+ * ∝ its number of cells and a uniform cell; pair and cellpair are void with one dispatchable
+ * unit per node and one injecting carrier per cell. Every random number of a replica comes from
+ * its own Philox stream (the reference: MersenneTwister per chain, Xoshiro per step), so the
+ * replicas are independent chains. No storage sweeps, no learned moves.
+ * ============================================================================================ */
+
+typedef struct {
+    const uint32_t *key;
+    uint32_t c0, c1, c2, c3;
+    uint32_t w[4];
+    int used;
+} RStream;
+
+static void rs_init(RStream *s, const uint32_t key[2], uint32_t c0, uint32_t c1, uint32_t c3) {
+    s->key = key;
+    s->c0 = c0;
+    s->c1 = c1;
+    s->c2 = 0xB0000000u;
+    s->c3 = c3;
+    s->used = 4;
+}
+static uint32_t rs_u32(RStream *s) {
+    if (s->used == 4) {
+        uint32_t c[4] = {s->c0, s->c1, s->c2++, s->c3};
+        orc_philox(s->key, c, s->w);
+        s->used = 0;
+    }
+    return s->w[s->used++];
+}
+static double rs_u01(RStream *s) {
+    uint32_t hi = rs_u32(s), lo = rs_u32(s);
+    return u01(hi, lo);
+}
+static int rs_pick(RStream *s, int n) { return (int)pick(rs_u32(s), (uint32_t)n); }
+
+int orc_ref_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
+                   const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights,
+                   int64_t *stats, int nthreads) {
+    int N = m->N, T = m->T;
+    if (N > 256) return -1;
+    static const double wdef[ESM_NWEIGHT] = {0.3, 0.25, 0.03, 0.08, 0.04, 0.1, 0.1, 0.1, 0.3};
+    const double *wts = weights ? weights : wdef;
+    double cw[ESM_NWEIGHT], acc = 0.0, tot = 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) tot += wts[q];
+    for (int q = 0; q < ESM_NWEIGHT; q++) { acc += wts[q]; cw[q] = acc / tot; }
+    int nextra = (int)nearbyint(extra * N);
+    Sys S0;
+    sys_setup(&S0, m, R, v, k, r, F);
+    Tables tb;
+    if (build_tables(m, &tb)) return -2;
+    int cfirst[3], ccount[3], ncls;
+    if (T % 2 == 0) {
+        ncls = 2; cfirst[0] = 0; ccount[0] = T / 2; cfirst[1] = 1; ccount[1] = T / 2;
+    } else {
+        ncls = 3; cfirst[0] = 0; ccount[0] = (T - 1) / 2; cfirst[1] = 1; ccount[1] = (T - 1) / 2;
+        cfirst[2] = T - 1; ccount[2] = 1;
+    }
+    int nlist = 4 * N + nextra;
+    int64_t st[ESM_NFAM * ESM_NSTAT];
+    memset(st, 0, sizeof(st));
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(static) num_threads(nthreads) reduction(+ : st[:ESM_NFAM * ESM_NSTAT])
+    for (int j = 0; j < R; j++) {
+        Sys S = S0;
+        int *list = (int *)malloc(sizeof(int) * (nlist > 0 ? nlist : 1));
+        for (int sw = 0; sw < nsweeps; sw++) {
+            uint32_t ctr = ctr0 + (uint32_t)sw;
+            RStream rc;
+            rs_init(&rc, key, (uint32_t)j, 0xFFFFFFFFu, ctr);
+            int corder[3] = {0, 1, 2};
+            for (int i = ncls - 1; i >= 1; i--) {
+                int q = rs_pick(&rc, i + 1), tmp = corder[i];
+                corder[i] = corder[q];
+                corder[q] = tmp;
+            }
+            for (int ci = 0; ci < ncls; ci++) {
+                int c = corder[ci];
+                for (int qq = 0; qq < ccount[c]; qq++) {
+                    int tau = cfirst[c] + 2 * qq;
+                    RStream rs;
+                    rs_init(&rs, key, (uint32_t)j, (uint32_t)tau, ctr);
+                    /* atoms: code −1 − (f·N + n), f 0 θ, 1 gas, 2 battery SoC, 3 H2 SoC; draws: family q */
+                    int nl = 0;
+                    for (int f = 0; f < 4; f++)
+                        for (int n = 0; n < N; n++) {
+                            if (f == 0 && S.adj_ptr[n + 1] == S.adj_ptr[n]) continue;
+                            list[nl++] = -1 - (f * N + n);
+                        }
+                    for (int d = 0; d < nextra; d++) {
+                        double u = rs_u01(&rs);
+                        int fq = ESM_NWEIGHT - 1;
+                        for (int q = ESM_NWEIGHT - 1; q >= 0; q--)
+                            if (cw[q] >= u) fq = q;
+                        list[nl++] = fq;
+                    }
+                    for (int i = nl - 1; i >= 1; i--) {
+                        int q = rs_pick(&rs, i + 1), tmp = list[i];
+                        list[i] = list[q];
+                        list[q] = tmp;
+                    }
+                    for (int i = 0; i < nl; i++) {
+                        int code = list[i];
+                        if (code < 0) {
+                            int f = (-1 - code) / N, n = (-1 - code) % N;
+                            double useg = rs_u01(&rs), uoff = rs_u01(&rs);
+                            if (f == 0) do_move_u(&S, j, 1, 0, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_THETA);
+                            else if (f == 1) do_move_u(&S, j, 0, 0, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_GEN);
+                            else do_move_u(&S, j, 2, f - 2, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_SOC);
+                            continue;
+                        }
+                        if (code == W_PAIR || code == W_CELLPAIR) continue;     /* void in this system */
+                        int n = rs_pick(&rs, N);
+                        if (code == W_UNIT) {
+                            double useg = rs_u01(&rs), uoff = rs_u01(&rs);
+                            do_move_u(&S, j, 0, 0, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_UNIT);
+                        } else if (code == W_POT) {
+                            if (S.adj_ptr[n + 1] == S.adj_ptr[n]) continue;
+                            double useg = rs_u01(&rs), uoff = rs_u01(&rs);
+                            do_move_u(&S, j, 1, 0, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_POT);
+                        } else if (code == W_SPAIR) {
+                            int sc = rs_pick(&rs, ESM_NSTOR);
+                            double useg = rs_u01(&rs), uoff = rs_u01(&rs);
+                            do_move_u(&S, j, 3, sc, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_SPAIR);
+                        } else {
+                            int kind, tid, fam;
+                            if (code == W_CELL) {
+                                double u = rs_u01(&rs);
+                                int lv = tb.nlev - 1;
+                                for (int q = tb.nlev - 1; q >= 0; q--)
+                                    if (tb.qcum.a[q] >= u) lv = q;
+                                int ncell = tb.lev_off.a[lv + 1] - tb.lev_off.a[lv];
+                                tid = tb.cl0 + tb.lev_off.a[lv] + rs_pick(&rs, ncell);
+                                kind = SP_CELL;
+                                fam = ESM_FAM_CELL;
+                            } else if (code == W_XPAIR) {
+                                int mm = rs_pick(&rs, N);
+                                if (mm == n) continue;
+                                tid = tb.xp0 + n * N + mm;
+                                kind = SP_XPAIR;
+                                fam = ESM_FAM_XPAIR;
+                            } else {
+                                int np = tb.xp_off.a[n + 1] - tb.xp_off.a[n];
+                                if (np == 0) continue;
+                                tid = tb.xp_off.a[n] + rs_pick(&rs, np);
+                                kind = code == W_XLOCC ? SP_XLOCC : SP_XLOC;
+                                fam = code == W_XLOCC ? ESM_FAM_XLOCC : ESM_FAM_XLOC;
+                            }
+                            double useg = rs_u01(&rs), uoff = rs_u01(&rs), umh = rs_u01(&rs);
+                            sm_move(&S, &tb, j, tau, tid, kind, useg, uoff, umh, maxnats, st + 5 * fam);
+                        }
+                    }
+                }
+            }
+        }
+        free(list);
+    }
+    if (stats) memcpy(stats, st, sizeof(st));
+    sys_free(&S0);
+    tables_free(&tb);
+    return 0;
+}

@@ -15,6 +15,7 @@
 
 #define TAG_REG 0x52454731u
 #define TAG_WIN 0x57494E44u
+#define ESM_MAXK_ORACLE 8   /* regimes of the FFBS restatement */
 
 static double u01(uint32_t hi, uint32_t lo) {
     uint64_t x = ((uint64_t)hi << 32) | lo;
@@ -208,4 +209,88 @@ int orc_categorical(const double *p, int K, double u) { return categorical(p, K,
 
 void orc_regime_window_path(const esm_model *m, int *path, int a, int b, const double *us) {
     regime_window(m, 1, 0, path, a, b, us);
+}
+
+/* ============================================================================================
+ * Centred FFBS of the regimes given the anomalies (replica_ffbs = true): fine_regime_step!
+ * (src/weather_fine.jl:290-300) at the single complete level — the regime path is redrawn from
+ * p(m | z) by ffbs (src/weather.jl:160-174) with the emission log-likelihoods of
+ * emission_loglik (:117-133) and sample_log (:176-185); z and x are unchanged (no detail
+ * levels). Uniforms: Philox (replica, τ, TAG_FFBS, c) words 0–1 for the draw at step τ.
+ * ============================================================================================ */
+#define TAG_FFBS 0x46464253u
+
+/* emission_loglik of replica j at step τ (0-based) under regime k: step 1 under the stationary
+ * start N(0, σ0² I) (the same for every regime), later steps the VAR(1) residual
+ * e = z_τ − φ z_{τ−1} − μ_k(τ) under N(0, σ² I) */
+static double emission(const esm_model *m, int R, int j, const double *z, int k, int tau) {
+    int T = m->T, nser = ESM_NREN * m->N;
+    double sd = tau == 0 ? m->sigma0 : m->sigma, q = 0.0;
+    for (int i = 0; i < nser; i++) {
+        double e = tau == 0 ? Z(i, 0) : Z(i, tau) - m->phi * Z(i, tau - 1) - var_mean_term(m, k, i, tau);
+        e /= sd;
+        q += e * e;
+    }
+    return -0.5 * (q + 2.0 * nser * log(sd) + nser * log(2 * 3.14159265358979323846));
+}
+
+/* logsumexp (src/weather.jl:136) */
+static double lse(const double *v, int K) {
+    double mx = -INFINITY;
+    for (int k = 0; k < K; k++) mx = fmax(mx, v[k]);
+    if (!isfinite(mx)) return mx;
+    double s = 0.0;
+    for (int k = 0; k < K; k++) s += exp(v[k] - mx);
+    return mx + log(s);
+}
+
+/* sample_log with a given uniform: weights exp(lw − max), inversion of u · Σ w */
+static int sample_log_u(const double *lw, int K, double u) {
+    double mx = -INFINITY, w[ESM_MAXK_ORACLE], tot = 0.0;
+    for (int k = 0; k < K; k++) mx = fmax(mx, lw[k]);
+    for (int k = 0; k < K; k++) {
+        w[k] = exp(lw[k] - mx);
+        tot += w[k];
+    }
+    double t = u * tot, c = 0.0;
+    for (int k = 0; k < K; k++) {
+        c += w[k];
+        if (t <= c) return k;
+    }
+    return K - 1;
+}
+
+int orc_weather_ffbs(const esm_model *m, int R, const double *z, int *mreg, const int *mask, const uint32_t key[2],
+                     uint32_t ctr, double *loglik) {
+    int T = m->T, K = m->K;
+    if (K < 1 || K > ESM_MAXK_ORACLE) return -1;
+    double lP[ESM_MAXK_ORACLE * ESM_MAXK_ORACLE], v[ESM_MAXK_ORACLE];
+    for (int q = 0; q < K * K; q++) lP[q] = log(m->Ptr[q]);
+    double *la = (double *)malloc(sizeof(double) * (size_t)K * T);
+    if (!la) return -3;
+    for (int j = 0; j < R; j++) {
+        if (mask && !mask[j]) continue;
+        /* forward filter: la[k][τ] = lse_i(la[i][τ−1] + log Ptr[i][k]) + ll[k][τ] */
+        for (int k = 0; k < K; k++) la[k] = log(m->pi0[k]) + emission(m, R, j, z, k, 0);
+        for (int t = 1; t < T; t++)
+            for (int k = 0; k < K; k++) {
+                for (int i = 0; i < K; i++) v[i] = la[(size_t)(t - 1) * K + i] + lP[i * K + k];
+                la[(size_t)t * K + k] = lse(v, K) + emission(m, R, j, z, k, t);
+            }
+        if (loglik) loglik[j] = lse(la + (size_t)(T - 1) * K, K);
+        /* backward sampling: m[T] from la[:, T], then m[τ] from la[:, τ] + log Ptr[:, m[τ+1]] */
+        uint32_t c4[4] = {(uint32_t)j, (uint32_t)(T - 1), TAG_FFBS, ctr}, w[4];
+        orc_philox(key, c4, w);
+        int next = sample_log_u(la + (size_t)(T - 1) * K, K, u01(w[0], w[1]));
+        MR(T - 1) = next;
+        for (int t = T - 2; t >= 0; t--) {
+            for (int i = 0; i < K; i++) v[i] = la[(size_t)t * K + i] + lP[i * K + next];
+            uint32_t c5[4] = {(uint32_t)j, (uint32_t)t, TAG_FFBS, ctr};
+            orc_philox(key, c5, w);
+            next = sample_log_u(v, K, u01(w[0], w[1]));
+            MR(t) = next;
+        }
+    }
+    free(la);
+    return 0;
 }

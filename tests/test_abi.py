@@ -51,4 +51,5 @@ def test_batch_and_autotune_reject_null_arguments_without_gpu_work():
     assert lib.esm_sweep_pad(None, 1) == -1
     assert lib.esm_autotune(None, 1, None, 0, 40.0, None, None, None) == -1
     assert lib.esm_score_measure(None, 1) == -1 and b"null" in lib.esm_last_error()
+    assert lib.esm_heuristic_state(None) == -1 and b"null" in lib.esm_last_error()
     assert _lib.BATCH_MAX == 64

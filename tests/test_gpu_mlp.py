@@ -71,10 +71,13 @@ def test_gradient_matches_oracle():
     dev.close()
 
 
-@pytest.mark.parametrize("B", [2, 24])
-def test_learned_move_ratio_and_decisions_fp64(B):
-    print(f"mlp move fp64 B={B}", flush=True)
-    s = synth.make_system(N=6, L=8, T=48, dt=1.0, seed=5)
+@pytest.mark.parametrize("N,L,B", [(6, 8, 2), (6, 8, 24), (40, 70, 24)])
+def test_learned_move_ratio_and_decisions_fp64(N, L, B):
+    """The learned move's FP64 MH log-ratio against −β ΔC of the oracle's gas atoms plus the
+    oracle's proposal densities, and identical decisions from the same Philox uniforms; on a
+    6-node network and on the 40-node / 70-line network of the learned-proposal workload."""
+    print(f"mlp move fp64 N={N} B={B}", flush=True)
+    s = synth.make_system(N=N, L=L, T=48, dt=1.0, seed=5)
     R = 3
     rng = np.random.default_rng(6)
     x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
@@ -192,3 +195,25 @@ def test_training_lowers_loss_and_raises_acceptance():
     assert rate_u < 0.01 and rate_t > 0.08
     dev.close()
     twin.close()
+
+
+def test_training_is_bitwise_reproducible():
+    """Two networks trained from the same start on the same states with the same key end with the
+    same parameters and losses, bit for bit: the gradient reductions (K-split partials, bias
+    column sums, loss) are summed in a fixed order, whatever order the workgroups finish in."""
+    print("mlp training reproducible", flush=True)
+    s = synth.make_system(N=10, L=16, T=96, dt=1.0, seed=9)
+    R = 64
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
+    v = synth.heuristic_state(s, x)
+    out = []
+    for _ in range(2):
+        dev = api.Replicas(s, R, FP32)
+        dev.upload(v=v, k=s.k0, x=x)
+        net = api.LearnedProposal(dev, block_hours=24, ncomp=4, key=(4, 4))
+        loss = net.train(40, batch=8192, lr=1e-3, key=(5, 5))
+        out.append((net.params(), loss))
+        dev.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
