@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -13,6 +14,37 @@ namespace esm {
 
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
+
+// The staging of the allocation-free Langevin step (esm_langevin_noalloc): weights [R], normals
+// [6N], steps [6] and offsets [6], the new capacities [6N] and the clip counter, in this fixed
+// device array instead of a per-call allocation. Every device holds its own copy at its own
+// address, looked up once per device (esm_create, with the device current) and kept in
+// g_scratch; host code uses that address, never the array's name. Calls on one device share the
+// array and keep the last call's contents: each call clears the counter, and calls that overlap
+// on one device race.
+constexpr int SCRATCH_DEVICES = 64;
+static __device__ __align__(256) double scratch[ESM_NOALLOC_DOUBLES];
+static double *g_scratch[SCRATCH_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_init(int device) {
+    if (device < 0 || device >= SCRATCH_DEVICES) {
+        set_error("device index beyond the allocation-free scratch table");
+        return ESM_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return ESM_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error(std::string("hipGetSymbolAddress of the allocation-free scratch: ") + hipGetErrorString(e));
+        return ESM_EHIP;
+    }
+    g_scratch[device] = (double *)p;
+    return ESM_OK;
+}
+
+__global__ void k_zero_i32(int *p) { *p = 0; }
 
 template <typename S>
 static void *upload_conv(const std::vector<double> &src, hipStream_t st, int *rc) {
@@ -360,6 +392,10 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
         free_handle(h);
         return ESM_EHIP;
     }
+    if ((rc = scratch_init(h->device))) {   // the device is current: its copy of the scratch array
+        free_handle(h);
+        return rc;
+    }
     auto imalloc = [&](int **p, const std::vector<int> &src) {
         if (rc) return;
         if (hipMalloc((void **)p, src.size() * sizeof(int)) != hipSuccess) { rc = ESM_ENOMEM; return; }
@@ -644,6 +680,53 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
     ESM_HIP_CHECK(hipMemcpyAsync(knew, d + stage.size(), n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     ESM_HIP_CHECK(hipMemcpyAsync(&nc, d_nc, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     ESM_HIP_CHECK(hipFreeAsync(d, h->stream));
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+    if (nclip) *nclip = nc;
+    return ESM_OK;
+}
+
+int32_t esm_langevin_noalloc(esm_handle *h, const double *weights, const double *xi, const double *eta,
+                             const double *kappa_s, double kmin, double clip, int32_t box, double *knew,
+                             int32_t *nclip) {
+    if (!h || !weights || !xi || !eta || !kappa_s || !knew) {
+        set_error("esm_langevin_noalloc: null argument");
+        return ESM_EINVAL;
+    }
+    const int n = ESM_NK * h->N, R = h->R;
+    // layout in the scratch: weights [R] | xi [n] | eta [6] | kappa_s [6] | knew [n] | counter
+    const size_t need = (size_t)R + 2 * (size_t)n + 2 * ESM_NK + 1;
+    if (need > (size_t)ESM_NOALLOC_DOUBLES) {
+        set_error("esm_langevin_noalloc: R + 12 N + 13 = " + std::to_string(need) + " doubles exceed the fixed scratch of " +
+                  std::to_string(ESM_NOALLOC_DOUBLES) + " (use esm_langevin)");
+        return ESM_EINVAL;
+    }
+    if (!h->scores_valid) {
+        set_error("esm_langevin_noalloc: no scores of the current state (call esm_scores first)");
+        return ESM_ESTATE;
+    }
+    for (int f = 0; f < ESM_NK; f++)
+        if (!(eta[f] >= 0) || !(kappa_s[f] > -kmin)) {
+            set_error("esm_langevin_noalloc: eta must be >= 0 and kappa_s + kmin > 0");
+            return ESM_EINVAL;
+        }
+    double *d = h->device >= 0 && h->device < SCRATCH_DEVICES ? g_scratch[h->device] : nullptr;
+    if (!d) {
+        set_error("esm_langevin_noalloc: the scratch of this device was not set up");
+        return ESM_ESTATE;
+    }
+    double *d_w = d, *d_xi = d + R, *d_eta = d + R + n, *d_ks = d_eta + ESM_NK, *d_knew = d_ks + ESM_NK;
+    int *d_nc = (int *)(d_knew + n);
+    ESM_HIP_CHECK(hipMemcpyAsync(d_w, weights, (size_t)R * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    ESM_HIP_CHECK(hipMemcpyAsync(d_xi, xi, (size_t)n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    ESM_HIP_CHECK(hipMemcpyAsync(d_eta, eta, ESM_NK * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    ESM_HIP_CHECK(hipMemcpyAsync(d_ks, kappa_s, ESM_NK * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, h->stream, d_nc);
+    ESM_HIP_CHECK(hipGetLastError());
+    int rc = launch_langevin(h, d_w, d_xi, d_eta, d_ks, kmin, clip, box, d_knew, d_nc);
+    if (rc) return rc;
+    int nc = 0;
+    ESM_HIP_CHECK(hipMemcpyAsync(knew, d_knew, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    ESM_HIP_CHECK(hipMemcpyAsync(&nc, d_nc, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     if (nclip) *nclip = nc;
     return ESM_OK;

@@ -89,6 +89,8 @@ SIGNATURES = [
     ("esm_scores", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("esm_score_measure", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_langevin", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32, _dp, _ip]),
+    ("esm_langevin_noalloc", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int32, _dp,
+                                         _ip]),
     ("esm_sweeps", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32), C.c_uint32, C.c_double,
                                C.POINTER(C.c_int64)]),
     ("esm_schedule", C.c_int32, [C.c_void_p, _ip, _ip]),

@@ -196,6 +196,22 @@ int32_t esm_langevin(esm_handle *h, const double *weights, const double *xi, con
                      const double *kappa_s, double kmin, double clip, int32_t box,
                      double *knew, int32_t *nclip);
 
+/*
+ * esm_langevin without a device allocation. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). The same step, kernel, arguments and results as esm_langevin; its
+ * staging (weights, normals, steps, offsets, the new capacities, the clip counter) lives in a
+ * fixed device array of ESM_NOALLOC_DOUBLES doubles (64 KiB, held by every device the library
+ * runs on until the process ends, its address looked up by esm_create) instead of a per-call
+ * allocation. Limit: R + 12 N + 13 <= ESM_NOALLOC_DOUBLES, else ESM_EINVAL. Calls on one device
+ * share the array: calls that overlap on one device (several threads or streams) race; every
+ * call clears the clip counter.
+ */
+#define ESM_NOALLOC_DOUBLES 8192
+int32_t esm_langevin_noalloc(esm_handle *h, const double *weights, const double *xi, const double *eta,
+                             const double *kappa_s, double kmin, double clip, int32_t box,
+                             double *knew, int32_t *nclip);
+
 /* move families of the sweep statistics */
 #define ESM_FAM_GEN 0     /* gas utilization atoms                                   */
 #define ESM_FAM_THETA 1   /* voltage-angle atoms                                     */
