@@ -218,7 +218,7 @@ static void free_handle(esm_handle *h) {
                   h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,
                   h->d_out, h->d_sk, h->d_fk, h->d_z, h->d_ytmp, h->d_carry, h->d_zin, h->d_mu, h->d_xbar, h->d_m,
                   h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0, h->d_theta_nodes, h->d_stats,
-                  h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum};
+                  h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum, h->d_mu64, h->d_ffbs, h->d_ffbs_mask};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -489,8 +489,11 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
             harm[2 * t + 1] = sin(2 * M_PI * (t + 1) / T);
         }
         dmalloc(&h->d_harm, harm);
-        dmalloc(&h->d_Ptr, std::vector<double>(mdl->Ptr, mdl->Ptr + K * K));
-        dmalloc(&h->d_pi0, std::vector<double>(mdl->pi0, mdl->pi0 + K));
+        h->Ptr_h.assign(mdl->Ptr, mdl->Ptr + K * K);
+        h->pi0_h.assign(mdl->pi0, mdl->pi0 + K);
+        dmalloc(&h->d_Ptr, h->Ptr_h);
+        dmalloc(&h->d_pi0, h->pi0_h);
+        dmalloc(&h->d_mu64, mu);
         imalloc(&h->d_hod, h->hod_h);
         if (!rc) {
             if (precision == ESM_FP64) {
@@ -902,6 +905,22 @@ int32_t esm_weather_step(esm_handle *h, const double *theta, const double *win_p
     if (moved) ESM_HIP_CHECK(hipMemcpyAsync(moved, h->d_moved, R * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     h->scores_valid = false;
+    return ESM_OK;
+}
+
+int32_t esm_weather_ffbs(esm_handle *h, const int32_t *mask, const uint32_t key[2], uint32_t counter) {
+    if (!h || !key) { set_error("esm_weather_ffbs: null argument"); return ESM_EINVAL; }
+    if (!h->weather_ready) {
+        set_error("esm_weather_ffbs: no weather drawn or uploaded (esm_weather_init / esm_upload_weather)");
+        return ESM_ESTATE;
+    }
+    std::vector<int> mk((size_t)h->R, 1);
+    if (mask)
+        for (int j = 0; j < h->R; j++) mk[j] = mask[j] != 0;
+    int rc = h->prec == ESM_FP64 ? launch_ffbs<double>(h, mk.data(), key[0], key[1], counter)
+                                 : launch_ffbs<float>(h, mk.data(), key[0], key[1], counter);
+    if (rc) return rc;
+    ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     return ESM_OK;
 }
 

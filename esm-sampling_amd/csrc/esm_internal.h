@@ -143,6 +143,10 @@ struct esm_handle {
     double *d_Ptr = nullptr, *d_pi0 = nullptr;
     std::vector<double> xbar_h;  // [3N][24]
     std::vector<int> hod_h;      // [T]
+    std::vector<double> Ptr_h, pi0_h;   // [K][K], [K] (host copies for the FFBS's log tables)
+    double *d_mu64 = nullptr;    // [K][3N][3] regime means in FP64 (FFBS emissions)
+    double *d_ffbs = nullptr;    // [K][T][R] FFBS emissions, then the forward filter (allocated at first use)
+    int *d_ffbs_mask = nullptr;  // [R]
     // sweep schedule
     struct ThetaRound {
         int colour, dmax, off, count;
@@ -248,6 +252,9 @@ int launch_follow(esm_handle *h, const void *d_kc0, const void *d_kc1, const int
 // k_weather.hip
 template <typename S>
 int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0, uint32_t k1, uint32_t ctr);
+// the centred FFBS of the regimes of the replicas whose mask [R] (host) is set
+template <typename S>
+int launch_ffbs(esm_handle *h, const int *mask, uint32_t k0, uint32_t k1, uint32_t ctr);
 
 // Copy replica js of hs into replica jd of hd (operation, raw weather and, with `weather`, the
 // anomalies and regimes) on hd's stream. Both handles have the same shape and precision.

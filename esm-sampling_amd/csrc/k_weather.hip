@@ -24,6 +24,7 @@ namespace esm {
 
 constexpr uint32_t TAG_REG = 0x52454731u;  // prior regime chain
 constexpr uint32_t TAG_WIN = 0x57494E44u;  // window move
+constexpr uint32_t TAG_FFBS = 0x46464253u; // centred FFBS of the regimes
 
 __device__ __forceinline__ double normal_of(uint32_t k0, uint32_t k1, int i, int tau, int j, uint32_t ctr) {
     uint32_t w[4];
@@ -241,7 +242,137 @@ __global__ void __launch_bounds__(256) k_wx_apply(WxArgs<S> a, DevModel<S> md, c
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Centred FFBS of the regimes given the anomalies (replica_ffbs = true): fine_regime_step!
+// (src/weather_fine.jl:290-300) at the single complete level redraws the regime path from
+// p(m | z) by forward filtering, backward sampling (ffbs, src/weather.jl:160-174); z and the
+// weather stay as they are. (1) the emission log-likelihoods (emission_loglik, :117-133) of every
+// (step, replica) under every regime, in FP64 from the stored anomalies: step 1 under the
+// stationary start N(0, σ0² I), later steps the VAR(1) residual under N(0, σ² I); (2) per replica
+// the forward filter over the steps and the backward draws (sample_log, :176-185) with the
+// uniforms Philox (replica, τ, TAG_FFBS, call counter) words 0–1.
+// ---------------------------------------------------------------------------------------------
+struct FfbsConst {
+    double lP[ESM_MAXK * ESM_MAXK];   // log Ptr (row: from), from the host's log
+    double lpi0[ESM_MAXK];            // log π0
+    double c0, c1, cpi;               // 2·n·log σ0, 2·n·log σ, n·log 2π (n = 3N series)
+};
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_ffbs_ll(const S *__restrict__ z, const double *__restrict__ mu64,
+                                                 const double *__restrict__ harm, const int *__restrict__ mask, int N,
+                                                 int T, int R, int K, double phi, double sigma, double sigma0,
+                                                 FfbsConst c, double *__restrict__ ll) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t TR = (size_t)T * R;
+    if (g >= TR) return;
+    const int j = (int)(g % R), t = (int)(g / R);
+    if (!mask[j]) return;
+    const int nser = ESM_NREN * N;
+    const double sd = t == 0 ? sigma0 : sigma, h1 = harm[2 * t], h2 = harm[2 * t + 1];
+    for (int k = 0; k < K; k++) {
+        double q = 0.0;
+        for (int i = 0; i < nser; i++) {
+            const double zt = (double)z[(size_t)i * TR + g];
+            double e;
+            if (t == 0) {
+                e = zt;
+            } else {
+                const double *M = mu64 + ((size_t)k * nser + i) * 3;
+                e = zt - phi * (double)z[(size_t)i * TR + g - R] - (M[0] + h1 * M[1] + h2 * M[2]);
+            }
+            e /= sd;
+            q += e * e;
+        }
+        ll[(size_t)k * TR + g] = -0.5 * (q + (t == 0 ? c.c0 : c.c1) + c.cpi);
+    }
+}
+
+__device__ __forceinline__ double lse_k(const double *v, int K) {
+    double mx = -INFINITY;
+    for (int k = 0; k < K; k++) mx = fmax(mx, v[k]);
+    if (!isfinite(mx)) return mx;
+    double s = 0.0;
+    for (int k = 0; k < K; k++) s += exp(v[k] - mx);
+    return mx + log(s);
+}
+
+__device__ __forceinline__ int sample_log_u(const double *lw, int K, double u) {
+    double mx = -INFINITY, w[ESM_MAXK], tot = 0.0;
+    for (int k = 0; k < K; k++) mx = fmax(mx, lw[k]);
+    for (int k = 0; k < K; k++) {
+        w[k] = exp(lw[k] - mx);
+        tot += w[k];
+    }
+    const double t = u * tot;
+    double acc = 0.0;
+    for (int k = 0; k < K; k++) {
+        acc += w[k];
+        if (t <= acc) return k;
+    }
+    return K - 1;
+}
+
+// one thread per replica: the forward filter in place over the emissions, then the draws
+__global__ void k_ffbs_scan(double *__restrict__ la, int *__restrict__ m, const int *__restrict__ mask, int T, int R,
+                            int K, FfbsConst c, uint32_t k0, uint32_t k1, uint32_t ctr) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= R || !mask[j]) return;
+    const size_t TR = (size_t)T * R;
+    double prev[ESM_MAXK], v[ESM_MAXK];
+    for (int k = 0; k < K; k++) {
+        prev[k] = c.lpi0[k] + la[(size_t)k * TR + j];
+        la[(size_t)k * TR + j] = prev[k];
+    }
+    for (int t = 1; t < T; t++) {
+        double cur[ESM_MAXK];
+        for (int k = 0; k < K; k++) {
+            for (int i = 0; i < K; i++) v[i] = prev[i] + c.lP[i * K + k];
+            cur[k] = lse_k(v, K) + la[(size_t)k * TR + (size_t)t * R + j];
+        }
+        for (int k = 0; k < K; k++) {
+            prev[k] = cur[k];
+            la[(size_t)k * TR + (size_t)t * R + j] = cur[k];
+        }
+    }
+    uint32_t w[4];
+    philox4x32(k0, k1, (uint32_t)j, (uint32_t)(T - 1), TAG_FFBS, ctr, w);
+    int next = sample_log_u(prev, K, u01(w[0], w[1]));
+    m[(size_t)(T - 1) * R + j] = next;
+    for (int t = T - 2; t >= 0; t--) {
+        for (int i = 0; i < K; i++) v[i] = la[(size_t)i * TR + (size_t)t * R + j] + c.lP[i * K + next];
+        philox4x32(k0, k1, (uint32_t)j, (uint32_t)t, TAG_FFBS, ctr, w);
+        next = sample_log_u(v, K, u01(w[0], w[1]));
+        m[(size_t)t * R + j] = next;
+    }
+}
+
 static inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+template <typename S>
+int launch_ffbs(esm_handle *h, const int *mask, uint32_t k0, uint32_t k1, uint32_t ctr) {
+    const int N = h->N, T = h->T, R = h->R, K = h->m.K;
+    const size_t TR = (size_t)T * R;
+    if (!h->d_ffbs) {
+        ESM_HIP_CHECK(hipMalloc((void **)&h->d_ffbs, (size_t)K * TR * sizeof(double)));
+        ESM_HIP_CHECK(hipMalloc((void **)&h->d_ffbs_mask, (size_t)R * sizeof(int)));
+    }
+    FfbsConst c{};
+    for (int q = 0; q < K * K; q++) c.lP[q] = log(h->Ptr_h[q]);
+    for (int k = 0; k < K; k++) c.lpi0[k] = log(h->pi0_h[k]);
+    const int nser = ESM_NREN * N;
+    c.c0 = 2.0 * nser * log(h->m.sigma0);
+    c.c1 = 2.0 * nser * log(h->m.sigma);
+    c.cpi = nser * log(2 * 3.14159265358979323846);
+    ESM_HIP_CHECK(hipMemcpyAsync(h->d_ffbs_mask, mask, (size_t)R * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    ProfScope ps(h, ESM_KERN_WEATHER);
+    hipLaunchKernelGGL((k_ffbs_ll<S>), dim3(nblk(TR, 256)), dim3(256), 0, h->stream, (const S *)h->d_z, h->d_mu64,
+                       h->d_harm, h->d_ffbs_mask, N, T, R, K, h->m.phi, h->m.sigma, h->m.sigma0, c, h->d_ffbs);
+    hipLaunchKernelGGL(k_ffbs_scan, dim3(nblk(R, 64)), dim3(64), 0, h->stream, h->d_ffbs, h->d_m, h->d_ffbs_mask, T, R,
+                       K, c, k0, k1, ctr);
+    ESM_HIP_CHECK(hipGetLastError());
+    return 0;
+}
 
 template <typename S>
 int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0, uint32_t k1, uint32_t ctr) {
@@ -295,6 +426,8 @@ int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0,
     return 0;
 }
 
+template int launch_ffbs<float>(esm_handle *, const int *, uint32_t, uint32_t, uint32_t);
+template int launch_ffbs<double>(esm_handle *, const int *, uint32_t, uint32_t, uint32_t);
 template int launch_weather<float>(esm_handle *, bool, int, int, uint32_t, uint32_t, uint32_t);
 template int launch_weather<double>(esm_handle *, bool, int, int, uint32_t, uint32_t, uint32_t);
 

@@ -98,6 +98,7 @@ SIGNATURES = [
     ("esm_weather_init", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32]),
     ("esm_weather_step", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int32, C.c_int32, C.POINTER(C.c_uint32), C.c_uint32,
                                      _ip]),
+    ("esm_weather_ffbs", C.c_int32, [C.c_void_p, _ip, C.POINTER(C.c_uint32), C.c_uint32]),
     ("esm_download_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
     ("esm_upload_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
     ("esm_copy_replica", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),

@@ -362,6 +362,15 @@ def weather_process_step(R: Replicas, theta, win_prob, window: int = 60, compens
     return moved
 
 
+def weather_ffbs(R: Replicas, mask=None, key=(1, 0), counter: int = 0):
+    """The centred FFBS of the regimes (``replica_ffbs``: ``fine_regime_step!`` by ``ffbs``,
+    src/weather.jl:160-174) of the replicas with ``mask[j]`` set (None: all): every regime path is
+    redrawn from p(m | z); the anomalies, the weather and the dispatch are unchanged."""
+    mk = None if mask is None else np.ascontiguousarray(np.broadcast_to(mask, (R.R,)), dtype=np.int32)
+    check(R.lib.esm_weather_ffbs(R.handle, iptr(mk) if mk is not None else None, _key(key), int(counter)),
+          "esm_weather_ffbs")
+
+
 def weather_state(R: Replicas):
     """(m [T, R] regimes, z [3N, T, R] anomalies) of every replica."""
     m = np.empty((R.T, R.R), dtype=np.int32)

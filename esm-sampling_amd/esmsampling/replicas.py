@@ -287,8 +287,6 @@ class Config:
                              "re-routed on the pair) are the localized pair moves built here")
         if self.storage_xpair_hops != 1 or self.storage_xpair_ball != 1:
             raise ValueError("storage_xpair_hops = 1 and storage_xpair_ball = 1 are the storage pairs built here")
-        if self.replica_ffbs:
-            refuse("replica_ffbs", "the centred FFBS of the regimes is not built (its default is false)")
         if self.tilt_lambda != 0.0:
             refuse("tilt_lambda", "the stress tilt functional is not built (λ = 0 is the untilted drift)")
         if self.storage_xpair != 0.0 or self.storage_xlong != 0:
@@ -410,6 +408,7 @@ class ReplicaCtl:
     tuning: str = "table"
     extra: float = 4.0
     measure: str = "utilization"
+    ffbs: bool = False
     shrink: float = 0.7
     grow: float = 1.2
 
@@ -428,7 +427,7 @@ class ReplicaCtl:
                    adapt=cfg.langevin_adapt, gap_hi=cfg.langevin_gap_hi, gap_lo=cfg.langevin_gap_lo,
                    eta_min=cfg.langevin_eta_min, eta_max=cfg.langevin_eta_max, plong=cfg.storage_long,
                    long_moves=cfg.storage_long_moves, long_hours=cfg.storage_long_hours,
-                   tuning=cfg.launch_tuning, extra=cfg.local_extra, measure=cfg.measure)
+                   tuning=cfg.launch_tuning, extra=cfg.local_extra, measure=cfg.measure, ffbs=cfg.replica_ffbs)
 
     def sweep_options(self, dev):
         """The operation sweep's options of a replica set (``local_extra``, the long-range storage
@@ -602,15 +601,24 @@ def replica_step(R: ReplicaSet, ctl: ReplicaCtl, it: int, keys: Keys, rng, sys, 
     th = np.zeros(M)
     wp = np.zeros(M)
     burst = np.zeros(M, dtype=bool)
+    ffbs = np.zeros(M, dtype=bool)
     for j in range(M):
         th[j], burst[j] = api.weather_schedule(it, j, M, ctl.theta, ctl.burst_every, ctl.phi)
-        regime = (burst[j] or ctl.burst_every == 0) and ctl.window > 0
-        wp[j] = ctl.window_prob if regime else 0.0
+        # replica_weather! (src/replicas.jl:589-591): a regime move with a burst (or every step in
+        # continuous mode) — the window with its probability, the centred FFBS with `ffbs`
+        regime = (burst[j] or ctl.burst_every == 0) and (ctl.window > 0 or ctl.ffbs)
+        wp[j] = ctl.window_prob if regime and ctl.window > 0 else 0.0
+        ffbs[j] = regime and ctl.ffbs
     t0 = time.perf_counter()
     moved = np.zeros(M, dtype=np.int32)
     if (th > 0).any() or (wp > 0).any():
         moved = api.weather_process_step(R.dev, th, wp, window=max(ctl.window, 1), compensate=ctl.comp,
                                          key=keys.weather, counter=it)
+    if ffbs.any():
+        # weather_process_step's FFBS after the pCN and window moves (src/replicas.jl:195); the
+        # anomalies and the weather do not change, so the compensation before it is the same
+        api.weather_ffbs(R.dev, mask=ffbs, key=keys.weather, counter=it)
+        moved = np.maximum(moved, ffbs.astype(np.int32))
     for j in range(M):
         if burst[j]:
             R.since[j] = 0

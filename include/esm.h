@@ -391,6 +391,17 @@ int32_t esm_weather_init(esm_handle *h, const uint32_t key[2], uint32_t counter)
 int32_t esm_weather_step(esm_handle *h, const double *theta, const double *win_prob, int32_t B,
                          int32_t compensate, const uint32_t key[2], uint32_t counter, int32_t *moved);
 
+/*
+ * esm_weather_ffbs: the centred FFBS of the regimes (`replica_ffbs = true`: `fine_regime_step!`,
+ * src/weather_fine.jl:290-300, by `ffbs`, src/weather.jl:160-174) of every replica whose
+ * mask[j] != 0 (mask NULL: all): the regime path is redrawn from p(m | z) given the anomalies —
+ * the emission log-likelihoods of every step under every regime (`emission_loglik`, :117-133),
+ * the forward filter, then the backward draws (`sample_log`, :176-185) with the uniforms Philox
+ * (replica, τ, 0x46464253, counter) words 0–1, in FP64. The anomalies, the weather and the
+ * dispatch are unchanged (at the complete level there are no detail levels to re-derive).
+ */
+int32_t esm_weather_ffbs(esm_handle *h, const int32_t *mask, const uint32_t key[2], uint32_t counter);
+
 /* Regimes m [T][R] (0-based) and the VAR anomalies z [ESM_NREN*N][T][R] (either may be NULL). */
 int32_t esm_download_weather(esm_handle *h, int32_t *m, double *z);
 /* Set the regimes and anomalies; x = sigmoid(xbar + z) follows and the caches are rebuilt. */

@@ -149,3 +149,21 @@ def test_stop_file_snapshot_and_resume(tmp_path):
         assert f["replicas/1/ktot"].shape == (6, 30) and int(f["run/restarts"]) == 1
         assert int(f["run/resumed_at_sweep"]) == 25
     assert (tmp_path / "b" / "summary_replicas.txt").exists()
+
+
+def test_run_replicas_with_ffbs_and_the_reference_replica_configuration():
+    """The reference's replica configuration (configs/device40_replica.toml: physical measure, 40
+    nodes daily, K = 4 regimes, 10 countries, 4 chains of 4 replicas) with a short schedule and the
+    centred FFBS of the regimes at every burst: it runs, regimes move at the bursts, the traces are
+    finite."""
+    import test_replica_driver as TD
+    print("run_replicas reference replica config + ffbs", flush=True)
+    cfg = RP.load_config(dict(TD.DEVICE40_REPLICA, mlda_warmup=6, nsamples=6, replica_init_sweeps=2,
+                              replica_burst_every=4, replica_probe_every=4, replica_probe_len=3, replica_probe_tail=1,
+                              replica_probe_win=1, replica_probe_avg=1, replica_gate_min=1, replica_gate_max=2,
+                              replica_gate_min_window=1, replica_gate_max_window=2, replica_gate_win=2,
+                              replica_ffbs=True, out=""))
+    res = RP.run_replicas(cfg)
+    tr = np.array(res["traces"])
+    assert tr.shape[:2] == (4, 12) and np.all(np.isfinite(tr))
+    assert sum(int(np.sum(r["moved"] > 0)) for r in res["records"][0]) > 0

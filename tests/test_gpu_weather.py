@@ -69,3 +69,31 @@ def test_weather_step_matches_oracle(prec, compensate, K):
     r_ = o.energy(v, k, x)[2]
     np.testing.assert_allclose(r, r_, atol=(1e-9 if prec == FP64 else 1e-5) * np.abs(s.load).max())
     dev.close()
+
+
+@pytest.mark.parametrize("prec,K", [(FP64, 2), (FP32, 2), (FP64, 4)])
+def test_weather_ffbs_matches_oracle(prec, K):
+    """The centred FFBS of the regimes: from the same anomalies and Philox numbers the regime paths
+    are those of the oracle's ffbs (src/weather.jl:160-174), replica for replica; masked-out
+    replicas keep their paths; the anomalies and the weather do not change."""
+    print(f"weather ffbs prec={prec} K={K}", flush=True)
+    s = synth.make_system(N=8, L=12, T=400, seed=5, K=K)
+    R = 6
+    dev = api.Replicas(s, R, prec)
+    api.sample_weather(dev, key=(4, 4), counter=1)
+    m0, z0 = api.weather_state(dev)
+    m_start = m0.copy()
+    _, _, x0 = dev.download()
+    mask = np.array([1, 0, 1, 1, 0, 1], dtype=np.int32)
+    for c in range(3):
+        api.weather_ffbs(dev, mask=mask, key=(4, 5), counter=c)
+        m1, z1 = api.weather_state(dev)
+        m_, _ = oracle.Oracle(s).weather_ffbs(m0, z0, mask=mask, key=(4, 5), counter=c)
+        assert np.array_equal(m1, m_)
+        assert np.array_equal(m1[:, mask == 0], m0[:, mask == 0])
+        assert np.array_equal(z1, z0)
+        m0 = m1
+    assert (m1[:, mask == 1] != m_start[:, mask == 1]).any()
+    _, _, x1 = dev.download()
+    assert np.array_equal(x1, x0)
+    dev.close()

@@ -91,8 +91,7 @@ def test_config_keys():
     assert ctl.M == 16 and ctl.kappa_s[0] == 900.0 and ctl.kappa_s[1] == 500.0 and ctl.kappa_s[5] == 100.0
     with pytest.raises(ValueError, match="unknown config key"):
         RP.load_config({"init": "heuristic", "replica_typo": 1})
-    with pytest.raises(ValueError, match="FFBS"):
-        RP.load_config({"init": "heuristic", "replica_ffbs": True})
+    assert RP.ReplicaCtl.from_config(RP.load_config({"init": "heuristic", "replica_ffbs": True})).ffbs
     c = RP.load_config({"init": "heuristic", "storage_long": 0.5, "storage_long_moves": 8, "storage_long_hours": 100.0})
     ctl = RP.ReplicaCtl.from_config(c)
     assert (ctl.plong, ctl.long_moves, ctl.long_hours) == (0.5, 8, 100.0)
