@@ -8,8 +8,12 @@
 // kernels are exactly those of esm_sweeps, in the same order, with the same arguments. The graph
 // is instantiated once; a later sweep whose kernel nodes have the same functions, grids,
 // workgroups and LDS sizes in the same order updates the executable graph in place
-// (hipGraphExecUpdate: new kernel arguments only), any other sweep (the storage sweeps run with
-// their own probabilities and draw their block lengths' order per sweep) instantiates it again.
+// (hipGraphExecUpdate: new kernel arguments only), any other sweep instantiates it again. The
+// handles are captured in their graph-stable form: the storage sweeps' per-sweep draws (whether
+// they run, the order of the block lengths, offsets and classes) are made by the kernels from the
+// sweep's counter, so the launch sequence is the same every sweep and the graph is instantiated
+// once (a change of the capacities that makes a unit join or leave the long-range sweep, or of
+// the sweep options, is the exception).
 //
 // esm_autotune times the launch forms of the random families on the handle's own state: it saves
 // v and the caches, runs each candidate's sweeps from that saved state with the same key and
@@ -147,7 +151,12 @@ int32_t esm_batch_sweeps(esm_batch *b, int32_t nsweeps, const uint32_t *keys, co
     for (int i = 0; i < n && !rc; i++) {
         esm_handle *h = b->hs[i];
         if (i > 0 && hipStreamWaitEvent(h->stream, b->fork, 0) != hipSuccess) { fail("fork wait"); break; }
-        if ((rc = sweeps_on_stream(h, nsweeps, keys + 2 * i, counters[i], maxnats))) break;
+        // the graph-stable launch sequence: the storage sweeps' draws are made on the device, so
+        // every capture has the same nodes and the executable graph is updated, not rebuilt
+        h->graph_stable = true;
+        rc = sweeps_on_stream(h, nsweeps, keys + 2 * i, counters[i], maxnats);
+        h->graph_stable = false;
+        if (rc) break;
         if (i > 0 && (hipEventRecord(b->join[i], h->stream) != hipSuccess || hipStreamWaitEvent(s0, b->join[i], 0) != hipSuccess))
             fail("join");
     }

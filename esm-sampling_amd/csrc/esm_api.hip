@@ -218,7 +218,7 @@ static void free_handle(esm_handle *h) {
                   h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,
                   h->d_out, h->d_sk, h->d_fk, h->d_z, h->d_ytmp, h->d_carry, h->d_zin, h->d_mu, h->d_xbar, h->d_m,
                   h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0, h->d_theta_nodes, h->d_stats,
-                  h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum, h->d_mu64, h->d_ffbs, h->d_ffbs_mask};
+                  h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum, h->d_mu64, h->d_ffbs, h->d_ffbs_mask, h->d_sched};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -464,6 +464,7 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_stats, (size_t)ESM_NFAM * ESM_NSTAT * STAT_SLOTS * sizeof(unsigned long long));
     zmalloc((void **)&h->d_counts, (size_t)(ESM_NWEIGHT + 1) * T * sizeof(int));
+    zmalloc((void **)&h->d_sched, 32 * sizeof(int));
     if (!rc) rc = precision == ESM_FP64 ? upload_move_tables<double>(h) : upload_move_tables<float>(h);
     // weather process (optional: a model without weather tables has none)
     h->has_weather = mdl->Ptr && mdl->pi0 && mdl->mu && mdl->xbar && mdl->K >= 1 && mdl->K <= ESM_MAXK &&

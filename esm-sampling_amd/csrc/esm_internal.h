@@ -172,6 +172,8 @@ struct esm_handle {
     int sweep_waves = 0;            // waves per step of that launch (0: by network size)
     int rand_block = 256;           // workgroup size of the one-wave random launches (64, 128, 256)
     bool rand_pad = true;           // ... each step on whole waves (R rounded up to a multiple of 64)
+    bool graph_stable = false;      // while a batch captures: a launch sequence independent of the draws
+    int *d_sched = nullptr;         // [32] the storage sweep's schedule in the graph-stable form
     int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step, then T ints
                                                   // (a parity class's steps in launch order)
     esm::MoveTablesHost mt;                       // templates (host, FP64)

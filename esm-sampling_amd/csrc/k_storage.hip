@@ -37,6 +37,11 @@ struct SmsArgs {
     const int *grp_off, *grp_node, *grp_si;
     int ngroups;
     unsigned long long *stats;     // [ESM_NSTAT][STAT_SLOTS]
+    // graph-stable form (sched != null): the launch is slot `slot`, class `ci` (0: the class drawn
+    // first) of the schedule that k_sms_schedule wrote on the device; L, base and nblk follow from
+    // it, and a launch whose segment width is not the slot's does nothing
+    const int *sched;
+    int slot, ci;
 };
 
 template <typename S, int W>
@@ -453,12 +458,58 @@ __device__ __forceinline__ S sample_stor(const F &f, S beta, S maxnats, S useg, 
     return t;
 }
 
+// segment width of block length L: the power of two ≥ L + 1, from 4 to 64
+__host__ __device__ constexpr int sms_width(int L) {
+    return L + 1 <= 4 ? 4 : (L + 1 <= 8 ? 8 : (L + 1 <= 16 ? 16 : (L + 1 <= 32 ? 32 : 64)));
+}
+
+// The storage sweep's schedule on the device (the draws of sms_schedule, one thread): sched[0]
+// whether it runs, sched[1] the number of block lengths, then per slot s (in sweep order) the
+// length, its offset and the class drawn first at sched[2 + 3s …].
+__global__ void k_sms_schedule(int T, double p, uint32_t k0, uint32_t k1, uint32_t ctr, int *sched) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t w[4];
+    philox4x32(k0, k1, 0u, 0u, 9300u, ctr, w);
+    sched[0] = u01(w[0], w[1]) < p ? 1 : 0;
+    int len[8], n = 0;
+    for (int L = 2; L <= 128; L *= 2)
+        if (L < T / 2) len[n++] = L;
+    for (int i = n - 1; i >= 1; i--) {   // Fisher–Yates
+        philox4x32(k0, k1, (uint32_t)i, 0u, 9301u, ctr, w);
+        const int jx = (int)mulhi32(w[0], (uint32_t)(i + 1));
+        const int tmp = len[i];
+        len[i] = len[jx];
+        len[jx] = tmp;
+    }
+    sched[1] = n;
+    for (int q = 0; q < n; q++) {
+        philox4x32(k0, k1, (uint32_t)len[q], 0u, 9302u, ctr, w);
+        sched[2 + 3 * q] = len[q];
+        sched[3 + 3 * q] = (int)mulhi32(w[0], (uint32_t)len[q]);
+        sched[4 + 3 * q] = u01(w[2], w[3]) < 0.5 ? 0 : 1;
+    }
+}
+
 template <typename S, int W>
 __global__ void __launch_bounds__(64, min_waves<S>(4, 2)) k_storage_ms(DevModel<S> m, SweepState<S> st, SmsArgs a, S beta, S maxnats,
                                                   S kmin) {
     extern __shared__ unsigned char sms_lds[];
     constexpr int SEGS = 64 / W;
     constexpr int SP = FStor<S, W>::SP;
+    if (a.sched) {   // graph-stable form: this launch's length, offset and class from the schedule
+        const int *sc = a.sched;
+        if (!sc[0] || a.slot >= sc[1]) return;
+        const int L = sc[2 + 3 * a.slot], o = sc[3 + 3 * a.slot], first = sc[4 + 3 * a.slot];
+        if (sms_width(L) != W || o > m.T - L - 1) return;
+        const int nb = (m.T - L - 1 - o) / L + 1;
+        if (nb < 2) return;
+        const int cls = a.ci == 0 ? first : 1 - first;
+        const int nbc = (nb - cls + 1) / 2;
+        if (nbc <= 0 || (int)blockIdx.x >= nbc * a.nrg) return;
+        a.L = L;
+        a.base = o + cls * L;
+        a.nblk = nbc;
+    }
     const int lane = threadIdx.x;
     const int seg = lane / W, sub = lane % W;
     const int blk = blockIdx.x / a.nrg, rg = blockIdx.x % a.nrg;
@@ -656,10 +707,75 @@ static SmsSchedule sms_schedule(int T, double p, uint32_t k0, uint32_t k1, uint3
     return sc;
 }
 
+// The graph-stable form of one storage sweep (a captured batch): the schedule drawn on the device,
+// then for every slot of the length order and both classes one launch per segment width, each
+// sized for the largest block count and LDS of its width's lengths; the launch whose width is the
+// slot's length's does that length's class, the others return at once. The same moves in the same
+// order as the host-scheduled form, with a launch sequence that does not depend on the draws.
+template <typename S>
+static int launch_storage_ms_stable(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats) {
+    const int T = h->T, R = h->R;
+    hipLaunchKernelGGL(k_sms_schedule, dim3(1), dim3(64), 0, h->stream, T, h->sweep_pms, k0, k1, ctr, h->d_sched);
+    ESM_HIP_CHECK(hipGetLastError());
+    std::vector<int> lengths;
+    for (int L = 2; L <= 128; L *= 2)
+        if (L < T / 2) lengths.push_back(L);
+    if (lengths.empty()) return 0;
+    const DevModel<S> m = dev_model<S>(h);
+    SweepState<S> st{(S *)h->d_v, (S *)h->d_r, (S *)h->d_F, (const S *)h->d_x};
+    const S beta = (S)h->m.beta_v, mxn = (S)maxnats, kmin = S(1);
+    const MoveTablesDev &d = h->dmt;
+    int lds_max = 0, nus = 1;
+    for (size_t g = 0; g + 1 < h->mt.grp_off.size(); g++) nus = std::max(nus, h->mt.grp_off[g + 1] - h->mt.grp_off[g]);
+    ESM_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, h->device));
+    // per segment width: the most blocks of a class (offset 0, the first class) and the largest
+    // LDS over the lengths of that width
+    const int widths[5] = {4, 8, 16, 32, 64};
+    int wblocks[5] = {0, 0, 0, 0, 0};
+    size_t wlds[5] = {0, 0, 0, 0, 0};
+    for (int q = 0; q < 5; q++)
+        for (int L : lengths) {
+            if (sms_width(L) != widths[q]) continue;
+            const int nb = (T - L - 1) / L + 1;
+            wblocks[q] = std::max(wblocks[q], (nb + 1) / 2);
+            wlds[q] = std::max(wlds[q], (sms_unit_words<S>() + (size_t)(64 / widths[q]) * sms_seg_words<S>(L + 1, nus)) *
+                                            sizeof(S));
+        }
+    for (int q = 0; q < 5; q++)
+        if (wlds[q] > (size_t)lds_max) {
+            set_error("storage multiscale sweep: a block length needs more LDS per workgroup than the device allows");
+            return ESM_EINVAL;
+        }
+    ProfScope p(h, ESM_KERN_SMS);
+    for (int slot = 0; slot < (int)lengths.size(); slot++)
+        for (int ci = 0; ci < 2; ci++)
+            for (int q = 0; q < 5; q++) {
+                const int W = widths[q], blocks = wblocks[q];
+                const size_t lds = wlds[q];
+                if (blocks == 0) continue;
+                const int nrg = (R + 64 / W - 1) / (64 / W);
+                SmsArgs a{k0, k1, ctr, 0, 0, 0, nrg, h->sweep_ms_moves, nus, h->sweep_ms_pagg,
+                          d.sms_units, (int)h->mt.sms_units.size(), d.grp_off, d.grp_node, d.grp_si,
+                          (int)h->mt.grp_si.size(), h->d_stats + (size_t)ESM_FAM_SMS * ESM_NSTAT * STAT_SLOTS,
+                          h->d_sched, slot, ci};
+                const dim3 grid((unsigned)blocks * (unsigned)nrg), blk(64);
+                switch (W) {
+                    case 4: hipLaunchKernelGGL((k_storage_ms<S, 4>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                    case 8: hipLaunchKernelGGL((k_storage_ms<S, 8>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                    case 16: hipLaunchKernelGGL((k_storage_ms<S, 16>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                    case 32: hipLaunchKernelGGL((k_storage_ms<S, 32>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                    default: hipLaunchKernelGGL((k_storage_ms<S, 64>), grid, blk, lds, h->stream, m, st, a, beta, mxn, kmin); break;
+                }
+                ESM_HIP_CHECK(hipGetLastError());
+            }
+    return 0;
+}
+
 template <typename S>
 int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats) {
     const int T = h->T, R = h->R;
     if (!(h->sweep_pms > 0) || h->mt.sms_units.empty()) return 0;
+    if (h->graph_stable) return launch_storage_ms_stable<S>(h, k0, k1, ctr, maxnats);
     const SmsSchedule sc = sms_schedule(T, h->sweep_pms, k0, k1, ctr);
     if (!sc.run) return 0;
     const DevModel<S> m = dev_model<S>(h);
@@ -754,6 +870,8 @@ struct SlgArgs {
     int nmoves;                        // plateaus per unit
     uint32_t elig[ESM_NSTOR][8];       // bit n of unit (s, n): takes part (hours ≥ hmin, κs ≥ 1)
     unsigned long long *st_long, *st_level;
+    double pcoin;                      // ≥ 0 (graph-stable form): the sweep runs with this probability,
+                                       // decided on the device from its draw (0, 0, 9410, ctr)
 };
 
 template <typename S, bool YL>
@@ -1021,6 +1139,11 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
                                                     S kmin) {
     extern __shared__ unsigned char slg_lds[];
     __shared__ int msi[SLG_MAXM], ma[SLG_MAXM], mL[SLG_MAXM];
+    if (a.pcoin >= 0) {
+        uint32_t w[4];
+        philox4x32(a.k0, a.k1, 0u, 0u, 9410u, a.ctr, w);
+        if (!(u01(w[0], w[1]) < a.pcoin)) return;
+    }
     const int lane = threadIdx.x;
     const int n = blockIdx.x / m.R, j = blockIdx.x % m.R;
     const int N = m.N, T = m.T, R = m.R;
@@ -1225,7 +1348,8 @@ int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, d
     if (!(h->sweep_plong > 0) || T < 2) return 0;
     uint32_t w[4];
     philox4x32(k0, k1, 0u, 0u, 9410u, ctr, w);
-    if (!(u01(w[0], w[1]) < h->sweep_plong)) return 0;
+    // graph-stable form: always launched, the kernel draws whether the sweep runs
+    if (!h->graph_stable && !(u01(w[0], w[1]) < h->sweep_plong)) return 0;
     if (N > 256 || h->sweep_long_moves > 256) {
         set_error("long-range storage sweep: at most 256 nodes and 256 plateaus per unit");
         return ESM_EINVAL;
@@ -1235,6 +1359,7 @@ int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, d
     a.k1 = k1;
     a.ctr = ctr;
     a.nmoves = h->sweep_long_moves;
+    a.pcoin = h->graph_stable ? h->sweep_plong : -1.0;
     bool any = false;
     for (int si = 0; si < ESM_NSTOR; si++)
         for (int n = 0; n < N; n++) {

@@ -46,9 +46,9 @@ def test_batch_graph_matches_eager_sweeps():
     eager = [p[1] for p in sets]
     b = api.Batch(batched)
     keys = [(3, 4), (5, 6), (7, 8)]
-    # replays 0-3: every storage sweep on, whose block lengths run in a new order each sweep (a
-    # new kernel sequence: instantiated again); 4-6: the multiscale sweep off, the same kernel
-    # sequence with new counters (updated in place after the first)
+    # replays 0-3: every storage sweep on, whose block lengths run in a new order each sweep (the
+    # lengths and their order drawn on the device: one instantiation, then updates in place);
+    # 4-6: the multiscale sweep off, another kernel sequence (instantiated once more, then updated)
     plan = [(1.0, c) for c in range(4)] + [(0.0, c) for c in range(4, 7)]
     for rep, (pms, ctr) in enumerate(plan):
         for d in batched + eager:
@@ -66,7 +66,39 @@ def test_batch_graph_matches_eager_sweeps():
         batched[2].download()
     launches, inst, upd = b.counts()
     assert launches == len(plan) and inst + upd == launches
-    assert inst >= 2 and upd >= 1, (inst, upd)
+    assert (inst, upd) == (2, 5), (inst, upd)
+    b.close()
+    for d in batched + eager:
+        d.close()
+
+
+def test_batch_graph_with_storage_sweeps_is_instantiated_once():
+    """Ten replays of two sweeps each with both storage sweeps on at probability 1/2 (whether each
+    runs, its block lengths, their order, offsets and classes are new draws every sweep): one
+    instantiation, nine in-place updates, and every replay bitwise the eager sweeps' states,
+    caches and statistics."""
+    print("batch graph, storage sweeps at p = 1/2, 10 replays", flush=True)
+    sets = [_make(*sh, seed=41 + i) for i, sh in enumerate(SHAPES)]
+    batched = [p[0] for p in sets]
+    eager = [p[1] for p in sets]
+    for d in batched + eager:
+        api.sweep_options(d, p_storage_ms=0.5)
+        api.storage_long_options(d, p=0.5, nmoves=8, hmin=1.0)
+    b = api.Batch(batched)
+    keys = [(13, 4), (15, 6), (17, 8)]
+    ran = np.zeros(2, dtype=np.int64)
+    for rep in range(10):
+        counters = [100 * rep + 7 * i for i in range(3)]
+        st = b.sweeps(2, keys=keys, counters=counters)
+        for i, d in enumerate(eager):
+            st_e = api.operation_sweep(d, 2, key=keys[i], counter=counters[i])
+            assert np.array_equal(st[i], st_e), f"replay {rep}, set {i}: statistics"
+            assert _same(batched[i], d), f"replay {rep}, set {i}: states or caches"
+        ran += [st[1][api.FAMILY_NAMES.index("storage_ms"), 0] > 0, st[1][api.FAMILY_NAMES.index("slong"), 0] > 0]
+    launches, inst, upd = b.counts()
+    assert (launches, inst, upd) == (10, 1, 9), (launches, inst, upd)
+    # both storage sweeps ran in some replays of set 1
+    assert ran.min() > 0, ran
     b.close()
     for d in batched + eager:
         d.close()
