@@ -61,6 +61,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (0: all cores)")
     ap.add_argument("--p-storage-ms", type=float, default=0.5,
                     help="probability of the storage multiscale sweep per operation sweep (the reference's 1/2)")
+    ap.add_argument("--xpair", type=float, default=0.1,
+                    help="weight of the cross-node storage pair sxpair in the local sweep (storage_xpair, 0.1)")
+    ap.add_argument("--xlong", type=int, default=8,
+                    help="interval storage pairs sxlong per unit in the long-range sweep (storage_xlong, 8)")
     ap.add_argument("--tune", choices=("table", "live", "off"), default="table",
                     help="launch form of the random families: from the committed tuning table when it "
                          "holds this shape (default), tuned on the device now, or the library default")
@@ -196,6 +200,8 @@ def main():
     T, R = args.hours, args.replicas
     wl = Workload(args.nodes, args.lines, T, R, prec)
     api.sweep_options(wl.dev, p_storage_ms=args.p_storage_ms)
+    api.storage_cross_options(wl.dev, p_xpair=args.xpair, nxlong=args.xlong)
+    wl.xpair, wl.xlong = args.xpair, args.xlong
     launch = apply_tuning(api, wl.dev, args.tune)
     lib, h = wl.dev.lib, wl.dev.handle
     for it in range(args.warmup):
@@ -212,7 +218,7 @@ def main():
              _lib.KERN_FOLLOW: "k_follow", _lib.KERN_SWEEP: "sweep", _lib.KERN_UNIT: "k_unit",
              _lib.KERN_POT: "k_pot", _lib.KERN_CELL: "k_cell", _lib.KERN_XLOC: "k_xloc", _lib.KERN_XLOCC: "k_xlocc",
              _lib.KERN_XPAIR: "k_xpair", _lib.KERN_SMS: "k_storage_ms", _lib.KERN_SLONG: "k_storage_long",
-             _lib.KERN_RANDOM: "k_random"}
+             _lib.KERN_RANDOM: "k_random", _lib.KERN_SXPAIR: "k_sxpair", _lib.KERN_SXLONG: "k_sxlong"}
     prof = {}
     for kid, name in names.items():
         tot, cnt = C.c_double(), C.c_int64()
@@ -247,7 +253,8 @@ def main():
         "data": "synthetic (seeded 40-node/70-line system, weather drawn from the VAR prior on the device)",
         "config": {"workload": f"replica sampler step, {wl.s.N} nodes / {wl.s.L} lines, {T} hourly steps, "
                                f"{R} weather replicas (weather + operation sweep + energy + score + Langevin + "
-                               f"following)", "replicas": R, "hours": T, "nodes": wl.s.N, "lines": wl.s.L,
+                               f"following; storage pairs sxpair {args.xpair:g}, sxlong {args.xlong})",
+                   "replicas": R, "hours": T, "nodes": wl.s.N, "lines": wl.s.L,
                    "line_moves_per_step": line_moves, "line_moves_per_s": line_moves * value,
                    "launch": launch},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -543,7 +550,7 @@ def cpu_baseline(wl, threads, it):
     t0 = time.perf_counter()
     m, z, x, v, _ = o.weather_step(m, z, x, v, k, th[:rs], wp[:rs], window=60, compensate=True, key=(1, 11),
                                    counter=it)
-    v, r, F, _ = o.sweeps(v, k, 1, key=(1, 13), counter=it, x=x, threads=nth)
+    v, r, F, _ = o.sweeps(v, k, 1, key=(1, 13), counter=it, x=x, threads=nth, p_xpair=wl.xpair, xlong_moves=wl.xlong)
     o.energy(v, k, x, want_caches=False)
     sk, fk, _ = o.scores(v, k, x)
     knew, _ = o.langevin(k, sk, fk, np.full(rs, 1.0 / rs), np.zeros((6, s.N)), wl.eta, wl.ks)

@@ -171,7 +171,11 @@ static int upload_move_tables(esm_handle *h) {
                                                       &mt.kk_off, &mt.kk_node, &mt.ln_off, &mt.ln_line, &mt.nd_off,
                                                       &mt.nd_node, &mt.nd_ent, &mt.nd_kk, &mt.nd_l, &mt.nl_pos,
                                                       &mt.nl_sgn, &mt.xp_off, &mt.lev_off, &mt.sms_units,
-                                                      &mt.grp_off, &mt.grp_node, &mt.grp_si};
+                                                      &mt.grp_off, &mt.grp_node, &mt.grp_si, &mt.sx_poff, &mt.sx_part,
+                                                      &mt.sx_th_off, &mt.sx_th_node, &mt.sx_kk_off, &mt.sx_kk_node,
+                                                      &mt.sx_ln_off, &mt.sx_ln_line, &mt.sx_nd_off, &mt.sx_nd_node,
+                                                      &mt.sx_nd_kk, &mt.sx_nd_l, &mt.sx_nl_pos, &mt.sx_nl_sgn,
+                                                      &mt.sx_ft_off, &mt.sx_ft_node};
     std::vector<int> ib;
     std::vector<size_t> io;
     for (auto *v : iv) {
@@ -179,7 +183,8 @@ static int upload_move_tables(esm_handle *h) {
         ib.insert(ib.end(), v->begin(), v->end());
         ib.push_back(0);   // keeps every array non-empty
     }
-    const std::vector<const std::vector<double> *> vv = {&mt.th_val, &mt.en_c, &mt.kk_g, &mt.ln_df};
+    const std::vector<const std::vector<double> *> vv = {&mt.th_val,    &mt.en_c,    &mt.kk_g,     &mt.ln_df,
+                                                         &mt.sx_th_val, &mt.sx_kk_g, &mt.sx_ln_ta, &mt.sx_ln_tb};
     std::vector<S> vb;
     std::vector<size_t> vo;
     for (auto *v : vv) {
@@ -196,13 +201,19 @@ static int upload_move_tables(esm_handle *h) {
     int **dst[] = {&d.th_off, &d.th_node, &d.en_off, &d.en_node, &d.en_div, &d.kk_off, &d.kk_node,
                                &d.ln_off, &d.ln_line, &d.nd_off, &d.nd_node, &d.nd_ent, &d.nd_kk, &d.nd_l,
                                &d.nl_pos, &d.nl_sgn, &d.xp_off, &d.lev_off, &d.sms_units, &d.grp_off,
-                               &d.grp_node, &d.grp_si};
+                               &d.grp_node, &d.grp_si, &d.sx_poff, &d.sx_part, &d.sx_th_off, &d.sx_th_node,
+                               &d.sx_kk_off, &d.sx_kk_node, &d.sx_ln_off, &d.sx_ln_line, &d.sx_nd_off, &d.sx_nd_node,
+                               &d.sx_nd_kk, &d.sx_nd_l, &d.sx_nl_pos, &d.sx_nl_sgn, &d.sx_ft_off, &d.sx_ft_node};
     for (size_t q = 0; q < iv.size(); q++) *dst[q] = d.ibuf + io[q];
     const S *vbase = (const S *)d.vbuf;
     d.th_val = vbase + vo[0];
     d.en_c = vbase + vo[1];
     d.kk_g = vbase + vo[2];
     d.ln_df = vbase + vo[3];
+    d.sx_th_val = vbase + vo[4];
+    d.sx_kk_g = vbase + vo[5];
+    d.sx_ln_ta = vbase + vo[6];
+    d.sx_ln_tb = vbase + vo[7];
     d.xp0 = mt.xp0;
     d.cl0 = mt.cl0;
     d.nlev = mt.nlev;
@@ -218,7 +229,8 @@ static void free_handle(esm_handle *h) {
                   h->d_k, h->d_klo, h->d_khi, h->d_capex, h->d_v, h->d_x, h->d_r, h->d_F, h->d_hparts, h->d_part2,
                   h->d_out, h->d_sk, h->d_fk, h->d_z, h->d_ytmp, h->d_carry, h->d_zin, h->d_mu, h->d_xbar, h->d_m,
                   h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0, h->d_theta_nodes, h->d_stats,
-                  h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum, h->d_mu64, h->d_ffbs, h->d_ffbs_mask, h->d_sched};
+                  h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum, h->d_mu64, h->d_ffbs, h->d_ffbs_mask, h->d_sched,
+                  h->d_sxu, h->d_sxm};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -463,7 +475,7 @@ int32_t esm_create(const esm_model *mdl, int32_t nrep, int32_t precision, esm_ha
     zmalloc((void **)&h->d_sk, (size_t)nrep * ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_fk, (size_t)ESM_NK * N * sizeof(double));
     zmalloc((void **)&h->d_stats, (size_t)ESM_NFAM * ESM_NSTAT * STAT_SLOTS * sizeof(unsigned long long));
-    zmalloc((void **)&h->d_counts, (size_t)(ESM_NWEIGHT + 1) * T * sizeof(int));
+    zmalloc((void **)&h->d_counts, (size_t)(ESM_NWEIGHT + 2) * T * sizeof(int));   // sxpair's counts, the step order
     zmalloc((void **)&h->d_sched, 32 * sizeof(int));
     if (!rc) rc = precision == ESM_FP64 ? upload_move_tables<double>(h) : upload_move_tables<float>(h);
     // weather process (optional: a model without weather tables has none)
@@ -771,7 +783,7 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
         if (!(w[q] >= 0) || !std::isfinite(w[q])) { set_error("esm_sweep_options: weights must be finite and >= 0"); return ESM_EINVAL; }
         tot += w[q];
     }
-    if (extra > 0 && !(tot > 0)) { set_error("esm_sweep_options: all weights are zero"); return ESM_EINVAL; }
+    (void)tot;   // all zero: the plain families make no draws (the storage pairs may still)
     if (!(p_storage_ms >= 0 && p_storage_ms <= 1)) { set_error("esm_sweep_options: p_storage_ms must be in [0, 1]"); return ESM_EINVAL; }
     h->sweep_extra = extra;
     for (int q = 0; q < ESM_NWEIGHT; q++) h->sweep_w[q] = w[q];
@@ -795,15 +807,69 @@ int32_t esm_sweep_launch(esm_handle *h, int32_t merged) {
     return ESM_OK;
 }
 
+// The units of the storage pairs (hours ≥ hmin) on the device and the buffer of the sxlong
+// schedule, for the options in force; refuses units that only discharge (the loss share of
+// `sx_loss` is not built: every unit of the pairs must be able to charge).
+static int sx_refresh(esm_handle *h, double pxpair, int nxlong, double hmin, const char *who) {
+    const int N = h->N;
+    std::vector<int> us;
+    for (int n = 0; n < N; n++)
+        for (int si = 0; si < ESM_NSTOR; si++)
+            if (h->hours[si * N + n] >= hmin) us.push_back(si * N + n);
+    if (pxpair > 0 || nxlong > 0) {
+        if (N > 256) {
+            set_error(std::string(who) + ": the storage pairs are built for at most 256 nodes");
+            return ESM_EINVAL;
+        }
+        for (int u : us)
+            if (!(h->eta_in[u] > 0)) {
+                set_error(std::string(who) + ": a storage unit of the pairs cannot charge (eta_in = 0); the loss share "
+                                             "of such units is not built");
+                return ESM_EINVAL;
+            }
+    }
+    if (h->d_sxu) (void)hipFree(h->d_sxu);
+    if (h->d_sxm) (void)hipFree(h->d_sxm);
+    h->d_sxu = nullptr;
+    h->d_sxm = nullptr;
+    h->nsx = 0;
+    h->sxm_cap = 0;
+    if (!us.empty()) {
+        ESM_HIP_CHECK(hipMalloc((void **)&h->d_sxu, us.size() * sizeof(int)));
+        ESM_HIP_CHECK(hipMemcpy(h->d_sxu, us.data(), us.size() * sizeof(int), hipMemcpyHostToDevice));
+        h->nsx = (int)us.size();
+    }
+    if (nxlong > 0 && !us.empty()) {
+        h->sxm_cap = us.size() * (size_t)nxlong;
+        ESM_HIP_CHECK(hipMalloc((void **)&h->d_sxm, esm::sxm_ints(h->sxm_cap) * sizeof(int)));
+        ESM_HIP_CHECK(hipMemset(h->d_sxm, 0, esm::sxm_ints(h->sxm_cap) * sizeof(int)));
+    }
+    h->sweep_pxpair = pxpair;
+    h->sweep_nxlong = nxlong;
+    return ESM_OK;
+}
+
 int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double hmin) {
     if (!h) { set_error("esm_storage_long_options: null handle"); return ESM_EINVAL; }
     if (!(p >= 0 && p <= 1)) { set_error("esm_storage_long_options: p must be in [0, 1]"); return ESM_EINVAL; }
     if (nmoves < 0 || nmoves > 256) { set_error("esm_storage_long_options: nmoves must be in [0, 256]"); return ESM_EINVAL; }
     if (!(hmin >= 0) || !std::isfinite(hmin)) { set_error("esm_storage_long_options: hmin must be finite and >= 0"); return ESM_EINVAL; }
+    const int rc = sx_refresh(h, h->sweep_pxpair, h->sweep_nxlong, hmin, "esm_storage_long_options");
+    if (rc) return rc;
     h->sweep_plong = p;
     h->sweep_long_moves = nmoves;
     h->sweep_long_hours = hmin;
     return ESM_OK;
+}
+
+int32_t esm_storage_cross_options(esm_handle *h, double p_xpair, int32_t nxlong) {
+    if (!h) { set_error("esm_storage_cross_options: null handle"); return ESM_EINVAL; }
+    if (!(p_xpair >= 0) || !std::isfinite(p_xpair)) {
+        set_error("esm_storage_cross_options: p_xpair must be finite and >= 0");
+        return ESM_EINVAL;
+    }
+    if (nxlong < 0 || nxlong > 256) { set_error("esm_storage_cross_options: nxlong must be in [0, 256]"); return ESM_EINVAL; }
+    return sx_refresh(h, p_xpair, nxlong, h->sweep_long_hours, "esm_storage_cross_options");
 }
 
 int32_t esm_follow_capacities(esm_handle *h, const double *knew, double kmin, double *kout) {

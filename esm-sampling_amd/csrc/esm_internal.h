@@ -68,6 +68,13 @@ struct MoveTablesHost {
     // storage multiscale moves: every storage unit (carrier·N + node, carrier-major) and the
     // country–carrier groups (sorted by country, then carrier; nodes increasing)
     std::vector<int> sms_units, grp_off, grp_node, grp_si;
+    // storage pairs (sx_templates): slots sx_poff[n] … sx_poff[n+1]−1 of node n (the node itself,
+    // then its neighbours increasing), per slot the partner, the angle entries (unscaled), the leak
+    // nodes and their requirement change per unit, the lines at the angle nodes with the angle
+    // change at each end, the touched nodes (with their lines and leak) and the footprint
+    std::vector<int> sx_poff, sx_part, sx_th_off, sx_th_node, sx_kk_off, sx_kk_node, sx_ln_off, sx_ln_line, sx_nd_off,
+        sx_nd_node, sx_nd_kk, sx_nd_l, sx_nl_pos, sx_nl_sgn, sx_ft_off, sx_ft_node;
+    std::vector<double> sx_th_val, sx_kk_g, sx_ln_ta, sx_ln_tb;
 };
 
 struct MoveTablesDev {
@@ -81,6 +88,11 @@ struct MoveTablesDev {
     const void *th_val = nullptr, *en_c = nullptr, *kk_g = nullptr, *ln_df = nullptr;
     int xp0 = 0, cl0 = 0, nlev = 0;
     int *sms_units = nullptr, *grp_off = nullptr, *grp_node = nullptr, *grp_si = nullptr;
+    int *sx_poff = nullptr, *sx_part = nullptr, *sx_th_off = nullptr, *sx_th_node = nullptr, *sx_kk_off = nullptr,
+        *sx_kk_node = nullptr, *sx_ln_off = nullptr, *sx_ln_line = nullptr, *sx_nd_off = nullptr, *sx_nd_node = nullptr,
+        *sx_nd_kk = nullptr, *sx_nd_l = nullptr, *sx_nl_pos = nullptr, *sx_nl_sgn = nullptr, *sx_ft_off = nullptr,
+        *sx_ft_node = nullptr;
+    const void *sx_th_val = nullptr, *sx_kk_g = nullptr, *sx_ln_ta = nullptr, *sx_ln_tb = nullptr;
 };
 
 // Build the templates from the model (host, FP64); false with a message on a bad network.
@@ -172,6 +184,14 @@ struct esm_handle {
     int sweep_waves = 0;            // waves per step of that launch (0: by network size)
     int rand_block = 256;           // workgroup size of the one-wave random launches (64, 128, 256)
     bool rand_pad = true;           // ... each step on whole waves (R rounded up to a multiple of 64)
+    // cross-node storage pairs (esm_storage_cross_options): the weight of sxpair in the local
+    // sweep, the sxlong moves per unit in the long-range sweep, the units taking part (hours ≥
+    // sweep_long_hours) and the buffer of the sxlong schedule
+    double sweep_pxpair = 0.0;
+    int sweep_nxlong = 0;
+    int *d_sxu = nullptr, nsx = 0;
+    int *d_sxm = nullptr;
+    size_t sxm_cap = 0;             // moves the schedule buffer holds
     bool graph_stable = false;      // while a batch captures: a launch sequence independent of the draws
     int *d_sched = nullptr;         // [32] the storage sweep's schedule in the graph-stable form
     int *d_counts = nullptr;                      // [ESM_NWEIGHT][T] moves per family and step, then T ints
@@ -246,6 +266,9 @@ int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, dou
 void mlp_free(esm_handle *h);
 template <typename S>
 int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
+template <typename S>
+int launch_sxlong(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats, double pcoin);
+size_t sxm_ints(size_t cap);   // ints of the sxlong schedule buffer for `cap` moves (k_cross.hip)
 
 // k_follow.hip
 template <typename S>

@@ -33,22 +33,24 @@
 // walking order, as `walk!` does; equal roots of several nodes are visited once.
 #include <algorithm>
 
-#include "sweep_core.h"
+#include "cross_move.h"
 
 namespace esm {
 
 // positions of the plain families in the weight list
-enum { W_UNIT = 0, W_PAIR = 1, W_XPAIR = 2, W_XLOCC = 3, W_XLOC = 4, W_POT = 5, W_CELLPAIR = 6, W_CELL = 7, W_SPAIR = 8 };
+enum { W_UNIT = 0, W_PAIR = 1, W_XPAIR = 2, W_XLOCC = 3, W_XLOC = 4, W_POT = 5, W_CELLPAIR = 6, W_CELL = 7, W_SPAIR = 8,
+       W_SXPAIR = 9 };
+constexpr int NW_ALL = ESM_NWEIGHT + 1;   // the plain families, then the storage pair sxpair
 
 struct CumW {
-    double c[ESM_NWEIGHT];
+    double c[NW_ALL];
 };
 
 struct RandArgs {
     uint32_t k0, k1, ctr;
     int tau0, tstride, ntau;
     int w;                       // the family's position in the weight list
-    const int *counts;           // [ESM_NWEIGHT][T]
+    const int *counts;           // [NW_ALL][T]
     unsigned long long *stats;   // [ESM_NSTAT][STAT_SLOTS] of the family
     const int *perm;             // the class's steps in launch order (null: in order)
     int rp;                      // threads per step: R, or R rounded up to a multiple of 64 (padded)
@@ -92,26 +94,26 @@ __device__ __forceinline__ void line_uniforms(const RandArgs &a, int j, int k, i
     uniforms2(a.k0, a.k1, j, (uint32_t)tau, 0x40000000u | ((uint32_t)a.w << 20) | (uint32_t)k, a.ctr, u1, u2);
 }
 
-// per step: round(extra·N) family draws, counted per family
+// per step: round(extra·xmul·N) family draws, counted per family (xmul = 1 without sxpair)
 __global__ void k_movecounts(int T, int nextra, CumW cw, uint32_t k0, uint32_t k1, uint32_t ctr, int *counts) {
     const int tau = blockIdx.x * blockDim.x + threadIdx.x;
     if (tau >= T) return;
-    int cnt[ESM_NWEIGHT];
+    int cnt[NW_ALL];
 #pragma unroll
-    for (int q = 0; q < ESM_NWEIGHT; q++) cnt[q] = 0;
+    for (int q = 0; q < NW_ALL; q++) cnt[q] = 0;
     for (int d = 0; d < nextra; d++) {
         uint32_t w[4];
         philox4x32(k0, k1, 0x80000000u | (uint32_t)d, (uint32_t)tau, 9000u, ctr, w);
         const double u = u01(w[0], w[1]);
-        int f = ESM_NWEIGHT - 1;
+        int f = NW_ALL - 1;
 #pragma unroll
-        for (int q = ESM_NWEIGHT - 1; q >= 0; q--)
+        for (int q = NW_ALL - 1; q >= 0; q--)
             if (cw.c[q] >= u) f = q;
 #pragma unroll
-        for (int q = 0; q < ESM_NWEIGHT; q++) cnt[q] += q == f;
+        for (int q = 0; q < NW_ALL; q++) cnt[q] += q == f;
     }
 #pragma unroll
-    for (int q = 0; q < ESM_NWEIGHT; q++) counts[(size_t)q * T + tau] = cnt[q];
+    for (int q = 0; q < NW_ALL; q++) counts[(size_t)q * T + tau] = cnt[q];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -194,6 +196,61 @@ __global__ void __launch_bounds__(256, min_waves<S>(3)) k_rand_spair(DevModel<S>
     MoveOut out;
     int j;
     if (slot_of(a, g, m.R, j)) run_spair<S, UNI>(m, st, a, step_of<UNI>(a, g, a.rp), j, beta, maxnats, kmin, out);
+    add_counts(a.stats, out);
+}
+
+// ---------------------------------------------------------------------------------------------
+// sxpair: the cross-node storage pair at one instant (`prep_sxpair!`, src/storage_cross.jl:147-167):
+// unit A uniform among the units of the pairs (selection word 0), the partner by `sx_partner`
+// (words 2–3), the instant τ (steps τm and τ); one thread per (step, replica) with the pair move's
+// one-lane form
+// ---------------------------------------------------------------------------------------------
+
+template <typename S, bool UNI>
+__global__ void __launch_bounds__(256) k_rand_sxpair(DevModel<S> m, SweepState<S> st, RandArgs a, SxArgs<S> sx,
+                                                     S beta, S maxnats, S kmin) {
+    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    MoveOut out;
+    int j;
+    if (slot_of(a, g, m.R, j) && sx.nsx > 0) {
+        const int tau = step_of<UNI>(a, g, a.rp), N = m.N, T = m.T;
+        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * T + tau]);
+        for (int k = 0; k < cnt; k++) {
+            uint32_t s[4];
+            selection(a, k, tau, s);
+            uniform4<UNI>(s);
+            const int ua = sx.units[pick(s[0], (uint32_t)sx.nsx)], sa = ua / N, na = ua % N;
+            if (m.kc[(ESM_KSTOR + sa) * N + na] * m.hours[ua] / m.dt < S(1)) continue;
+            int slot, sb, nb;
+            if (!sx_partner(m, sx, sa, na, u01(s[2], s[3]), slot, sb, nb)) continue;
+            FXPair<S, 1> f;
+            if (!fx_prepare(f, m, st, sx.tb, sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j, 0, kmin)) continue;
+            double u1, u2;
+            line_uniforms(a, j, k, tau, u1, u2);
+            int status = 0;
+            S t = sample_stor(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+            if (t != 0) {
+                bool any;
+                const double lj = f.logjac(t, any);
+                if (any) {
+                    uint32_t w[4];
+                    philox4x32(a.k0, a.k1, (uint32_t)j, (uint32_t)tau, 0x60000000u | ((uint32_t)a.w << 20) | (uint32_t)k,
+                               a.ctr, w);
+                    if (!(u01(w[0], w[1]) < exp(fmin(0.0, lj)))) {
+                        t = 0;
+                        status = 2;
+                    }
+                }
+            }
+            out.add(status, t != 0, f.nev);
+            if (t != 0) {
+                const size_t TR = (size_t)T * m.R;
+                f.commit(t, st.r + j, st.F + j, st.v + plane(N, T, m.R) + j, st.v + j,
+                         st.v + plane((2 + sa) * N, T, m.R) + (size_t)na * TR + j,
+                         st.v + plane((2 + sb) * N, T, m.R) + (size_t)nb * TR + j);
+            }
+        }
+    }
     add_counts(a.stats, out);
 }
 
@@ -943,13 +1000,20 @@ static int sparse_waves(const esm_handle *h) { return h->sweep_waves > 0 ? h->sw
 template <typename S>
 int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats) {
     const int N = h->N, T = h->T, R = h->R;
-    const int nextra = (int)nearbyint(h->sweep_extra * N);
-    if (nextra <= 0) return 0;
+    // with sxpair (a positive weight and some unit taking part) its weight follows the plain
+    // families', and the draws are scaled by Σ all / Σ plain so that every plain family keeps its
+    // rate (`local_sweep!`, src/local.jl:1934-1941)
+    const bool usx = h->sweep_pxpair > 0 && h->nsx > 0;
+    double wall[NW_ALL], tot0 = 0.0, tot = 0.0, acc = 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) wall[q] = h->sweep_w[q];
+    wall[ESM_NWEIGHT] = usx ? h->sweep_pxpair : 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) tot0 += wall[q];
+    for (int q = 0; q < NW_ALL; q++) tot += wall[q];
+    const int nextra = (int)nearbyint(h->sweep_extra * (tot0 > 0 ? tot / tot0 : 1.0) * N);
+    if (nextra <= 0 || !(tot > 0)) return 0;
     CumW cw;
-    double acc = 0.0, tot = 0.0;
-    for (int q = 0; q < ESM_NWEIGHT; q++) tot += h->sweep_w[q];
-    for (int q = 0; q < ESM_NWEIGHT; q++) {
-        acc += h->sweep_w[q];
+    for (int q = 0; q < NW_ALL; q++) {
+        acc += wall[q];
         cw.c[q] = acc / tot;
     }
     hipLaunchKernelGGL(k_movecounts, dim3(nblk((size_t)T, 256)), dim3(256), 0, h->stream, T, nextra, cw, k0, k1, ctr,
@@ -992,7 +1056,7 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     // the launches that need more than one round of waves: spair (194 VGPRs) and the merged
     // table-driven families (an xpair evaluation reads every line and node, cells and compensated
     // pairs about half as much)
-    int *perm = h->d_counts + (size_t)ESM_NWEIGHT * T;   // T ints after the counts
+    int *perm = h->d_counts + (size_t)NW_ALL * T;   // T ints after the counts
     auto order_steps = [&](const RandArgs &a, StepWeights sw) -> const int * {
         if (!uni) return nullptr;
         hipLaunchKernelGGL(k_step_order, dim3(1), dim3(1024), 0, h->stream, h->d_counts, T, a.tau0, a.tstride, a.ntau,
@@ -1079,6 +1143,15 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 hipLaunchKernelGGL((k_sparse_all<S, true, true>), dim3((unsigned)((size_t)a.ntau * R / 64)), dim3(64 * nw),
                                    lds, h->stream, m, st, b, sa, tb, beta, mxn, kmin, nw);
             }
+            ESM_HIP_CHECK(hipGetLastError());
+        }
+        if (usx) {   // the storage pairs last
+            RandArgs b = a;
+            b.w = W_SXPAIR;
+            b.stats = h->d_stats + (size_t)ESM_FAM_SXPAIR * ESM_NSTAT * STAT_SLOTS;
+            const SxArgs<S> sx = sx_args<S>(h);
+            ProfScope p(h, ESM_KERN_SXPAIR);
+            RLAUNCH((k_rand_sxpair<S, true>), (k_rand_sxpair<S, false>), m, st, b, sx, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());
         }
     }

@@ -357,6 +357,123 @@ bool build_move_tables(const esm_handle *h, MoveTablesHost &mt, std::string &err
         mt.qcum.push_back(acc);
     }
     for (double &q : mt.qcum) q /= acc;
+    // storage pairs (`sx_templates`, src/storage_cross.jl:29-53): per node its own slot, then per
+    // neighbour m (increasing) the unit dipole n → m re-routed on the ball of radius one around the
+    // pair (`xloc_directions` with rb = 1, src/local.jl:122-191; the exact global re-routing when
+    // the ball holds the whole component), its leak into the first layer outside the ball
+    {
+        mt.sx_poff.assign(1, 0);
+        for (auto *v : {&mt.sx_th_off, &mt.sx_kk_off, &mt.sx_ln_off, &mt.sx_nd_off, &mt.sx_nd_l, &mt.sx_ft_off})
+            v->assign(1, 0);
+        auto slot = [&](int n, int m, const std::vector<std::pair<int, double>> &th,
+                        const std::vector<std::pair<int, double>> &lk) {
+            std::vector<double> dth(N, 0.0);
+            std::vector<char> lmark(L, 0), nmark(N, 0), fmark(N, 0);
+            for (const auto &q : th) {
+                mt.sx_th_node.push_back(q.first);
+                mt.sx_th_val.push_back(q.second);
+                dth[q.first] = q.second;
+                fmark[q.first] = 1;
+                for (int e : net.nodelines[q.first]) {
+                    lmark[e] = 1;
+                    fmark[net.bus0[e]] = fmark[net.bus1[e]] = 1;
+                }
+            }
+            std::vector<int> ll;
+            for (int e = 0; e < L; e++) {
+                if (!lmark[e]) continue;
+                const double ta = dth[net.bus0[e]], tb = dth[net.bus1[e]];
+                if (ta == tb) continue;   // no flow change at any scale
+                mt.sx_ln_line.push_back(e);
+                mt.sx_ln_ta.push_back(ta);
+                mt.sx_ln_tb.push_back(tb);
+                ll.push_back(e);
+                nmark[net.bus0[e]] = nmark[net.bus1[e]] = 1;
+            }
+            for (const auto &q : lk) {
+                mt.sx_kk_node.push_back(q.first);
+                mt.sx_kk_g.push_back(q.second);
+                nmark[q.first] = fmark[q.first] = 1;
+            }
+            nmark[n] = nmark[m] = fmark[n] = fmark[m] = 1;
+            for (int i = 0; i < N; i++) {
+                if (!nmark[i]) continue;
+                int kk = -1;
+                for (size_t q = 0; q < lk.size(); q++)
+                    if (lk[q].first == i) kk = (int)q;
+                mt.sx_nd_node.push_back(i);
+                mt.sx_nd_kk.push_back(kk);
+                for (size_t l = 0; l < ll.size(); l++) {
+                    const int e = ll[l];
+                    if (net.bus0[e] == i || net.bus1[e] == i) {
+                        mt.sx_nl_pos.push_back((int)l);
+                        mt.sx_nl_sgn.push_back(net.bus0[e] == i ? 1 : -1);
+                    }
+                }
+                mt.sx_nd_l.push_back((int)mt.sx_nl_pos.size());
+            }
+            for (int i = 0; i < N; i++)
+                if (fmark[i]) mt.sx_ft_node.push_back(i);
+            mt.sx_part.push_back(m);
+            mt.sx_th_off.push_back((int)mt.sx_th_node.size());
+            mt.sx_kk_off.push_back((int)mt.sx_kk_node.size());
+            mt.sx_ln_off.push_back((int)mt.sx_ln_line.size());
+            mt.sx_nd_off.push_back((int)mt.sx_nd_node.size());
+            mt.sx_ft_off.push_back((int)mt.sx_ft_node.size());
+        };
+        for (int n = 0; n < N; n++) {
+            slot(n, n, {}, {});
+            std::vector<int> part = hops(nbr, n, 1);
+            part.erase(std::remove(part.begin(), part.end(), n), part.end());
+            std::sort(part.begin(), part.end());
+            for (int m : part) {
+                std::vector<int> B = hops(nbr, n, 1), Bm = hops(nbr, m, 1);
+                B.insert(B.end(), Bm.begin(), Bm.end());
+                std::sort(B.begin(), B.end());
+                B.erase(std::unique(B.begin(), B.end()), B.end());
+                std::vector<std::pair<int, double>> th, lk;
+                if (B.size() >= net.cnodes[net.comp[n]].size()) {
+                    reroute(n, m, x);
+                    for (int i : net.cnodes[net.comp[n]])
+                        if (fabs(x[i]) >= 1e-15) th.push_back({i, x[i]});
+                } else {
+                    const int nb = (int)B.size();
+                    std::vector<int> pos(N, -1);
+                    for (int q = 0; q < nb; q++) pos[B[q]] = q;
+                    std::vector<double> A((size_t)nb * nb, 0.0);
+                    for (int q = 0; q < nb; q++)
+                        for (int e : net.nodelines[B[q]]) {
+                            const int o = net.bus0[e] == B[q] ? net.bus1[e] : net.bus0[e];
+                            A[(size_t)q * nb + q] += net.b[e];
+                            if (pos[o] >= 0) A[(size_t)q * nb + pos[o]] -= net.b[e];
+                        }
+                    if (!cholesky(A, nb)) {
+                        err = "move tables: a storage-pair ball Laplacian is not positive definite";
+                        return false;
+                    }
+                    std::vector<double> t(nb, 0.0);
+                    t[pos[n]] += 1.0;
+                    t[pos[m]] -= 1.0;
+                    chol_solve(A, nb, t.data());
+                    for (int q = 0; q < nb; q++)
+                        if (fabs(t[q]) >= 1e-15) th.push_back({B[q], t[q]});
+                    for (int q = 0; q < nb; q++)
+                        for (int e : net.nodelines[B[q]]) {
+                            const int o = net.bus0[e] == B[q] ? net.bus1[e] : net.bus0[e];
+                            if (pos[o] >= 0) continue;
+                            size_t jl = 0;
+                            while (jl < lk.size() && lk[jl].first != o) jl++;
+                            if (jl == lk.size()) lk.push_back({o, 0.0});
+                            lk[jl].second -= net.b[e] * t[q];
+                        }
+                    std::sort(lk.begin(), lk.end(),
+                              [](const std::pair<int, double> &a, const std::pair<int, double> &b) { return a.first < b.first; });
+                }
+                slot(n, m, th, lk);
+            }
+            mt.sx_poff.push_back((int)mt.sx_part.size());
+        }
+    }
     // storage multiscale moves (`storage_ms_sweep!`, src/flowmoves.jl:266-277): the units, and
     // the groups of one carrier in one country (cellof = E.parts[2]; single nodes without countries)
     for (int s = 0; s < ESM_NSTOR; s++)

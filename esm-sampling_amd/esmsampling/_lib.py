@@ -104,6 +104,7 @@ SIGNATURES = [
     ("esm_copy_replica", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     ("esm_sweep_options", C.c_int32, [C.c_void_p, C.c_double, _dp, C.c_int32, C.c_double]),
     ("esm_storage_long_options", C.c_int32, [C.c_void_p, C.c_double, C.c_int32, C.c_double]),
+    ("esm_storage_cross_options", C.c_int32, [C.c_void_p, C.c_double, C.c_int32]),
     ("esm_sweep_launch", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_mlp_init", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _u32p]),
     ("esm_mlp_dims", C.c_int32, [C.c_void_p, _ip]),
@@ -124,8 +125,8 @@ SIGNATURES = [
 ]
 (KERN_ENERGY, KERN_SCORE, KERN_SWEEP, KERN_WEATHER, KERN_GEN, KERN_THETA, KERN_SOC, KERN_SPAIR, KERN_FOLLOW,
  KERN_UNIT, KERN_POT, KERN_CELL, KERN_XLOC, KERN_XLOCC, KERN_XPAIR, KERN_SMS, KERN_SLONG, KERN_MLPF, KERN_MLPP,
- KERN_MLPT, KERN_RANDOM) = range(21)
-NFAM, NSTAT, NWEIGHT = 13, 5, 9
+ KERN_MLPT, KERN_RANDOM, KERN_SXPAIR, KERN_SXLONG) = range(23)
+NFAM, NSTAT, NWEIGHT = 15, 5, 9
 BATCH_MAX = 64
 MEASURES = {"utilization": 0, "physical": 1}   # ESM_MEASURE_*
 # the plain random families and their default weights (plain_families, src/local.jl:1912-1915)

@@ -161,7 +161,7 @@ def _key(key):
 
 
 FAMILY_NAMES = ("gen", "theta", "soc", "spair", "unit", "potential", "cell", "xloc", "xlocc", "xpair",
-                "storage_ms", "slong", "slevel")
+                "storage_ms", "slong", "slevel", "sxpair", "sxlong")
 
 
 def sweep_options(R: Replicas, extra: float = 4.0, weights=None, atoms: bool = True, p_storage_ms: float = 0.5):
@@ -184,6 +184,16 @@ def storage_long_options(R: Replicas, p: float = 1.0, nmoves: int = 32, hmin: fl
     check(R.lib.esm_storage_long_options(R.handle, float(p), int(nmoves), float(hmin)), "esm_storage_long_options")
 
 
+def storage_cross_options(R: Replicas, p_xpair: float = 0.1, nxlong: int = 8):
+    """The cross-node storage pairs (src/storage_cross.jl): ``p_xpair`` the weight of the
+    single-instant pair ``sxpair`` among the plain families of the local sweep (``storage_xpair``),
+    ``nxlong`` the interval pairs ``sxlong`` per storage unit in the long-range sweep
+    (``storage_xlong``); the units taking part have at least the long-range sweep's ``hmin``
+    hours. The defaults here are the reference configuration's (src/run.jl:30-31); a new handle
+    starts with both off, as `operation_sweep!`'s own keywords."""
+    check(R.lib.esm_storage_cross_options(R.handle, float(p_xpair), int(nxlong)), "esm_storage_cross_options")
+
+
 def sweep_launch(R: Replicas, merged=True):
     """The table-driven random families of a parity class in one launch (True or 1, the default)
     or one launch per family (False or 0: per-family kernel times; the same states and
@@ -197,7 +207,7 @@ def operation_sweep(R: Replicas, nsweeps: int = 1, key=(1, 0), counter: int = 0,
     """``operation_sweep!`` / ``local_sweep!`` (src/local.jl:1927-2062, :2401-2425) of every replica,
     ``nsweeps`` times at fixed capacities and weather: the atoms, the random families, then (each
     with its probability) the storage multiscale and long-range storage sweeps. Returns the move
-    statistics [13 families
+    statistics [15 families
     (``FAMILY_NAMES``), 5] (attempted, accepted,
     improper, point-cap / non-finite / Jacobian-rejected, ΔC evaluations)."""
     st = np.zeros(_lib.NFAM * _lib.NSTAT, dtype=np.int64)
@@ -293,7 +303,7 @@ class Batch:
 
     def sweeps(self, nsweeps: int = 1, keys=None, counters=None, maxnats: float = 40.0):
         """``operation_sweep`` of every set (set i under keys[i] and counters[i]); returns the
-        statistics [n sets, 13 families, 5]."""
+        statistics [n sets, 15 families, 5]."""
         n = len(self.sets)
         keys = [(1, 0)] * n if keys is None else list(keys)
         counters = [0] * n if counters is None else list(counters)

@@ -80,8 +80,8 @@ class Config:
     storage_long: float = 1.0
     storage_long_moves: int = 32
     storage_long_hours: float = 24.0
-    storage_xpair: float = 0.0
-    storage_xlong: int = 0
+    storage_xpair: float = 0.1
+    storage_xlong: int = 8
     storage_xpair_hops: int = 1
     storage_xpair_ball: int = 1
     measure: str = "utilization"
@@ -289,9 +289,8 @@ class Config:
             raise ValueError("storage_xpair_hops = 1 and storage_xpair_ball = 1 are the storage pairs built here")
         if self.tilt_lambda != 0.0:
             refuse("tilt_lambda", "the stress tilt functional is not built (λ = 0 is the untilted drift)")
-        if self.storage_xpair != 0.0 or self.storage_xlong != 0:
-            raise ValueError("storage_xpair / storage_xlong > 0: the cross-node storage pairs (sxpair, sxlong) are not "
-                             "built")
+        if not (self.storage_xpair >= 0.0) or not (0 <= self.storage_xlong <= 256):
+            raise ValueError("storage_xpair >= 0 and storage_xlong in [0, 256]")
         # levels: (cells, hours per step), coarse to fine (check_levels, src/run.jl:173-183); the
         # replica sampler runs at the finest level
         try:
@@ -405,6 +404,8 @@ class ReplicaCtl:
     plong: float = 1.0
     long_moves: int = 32
     long_hours: float = 24.0
+    xpair: float = 0.1
+    xlong: int = 8
     tuning: str = "table"
     extra: float = 4.0
     measure: str = "utilization"
@@ -426,15 +427,17 @@ class ReplicaCtl:
                    probe_tail=cfg.replica_probe_tail, probe_win=cfg.replica_probe_win, probe_avg=cfg.replica_probe_avg,
                    adapt=cfg.langevin_adapt, gap_hi=cfg.langevin_gap_hi, gap_lo=cfg.langevin_gap_lo,
                    eta_min=cfg.langevin_eta_min, eta_max=cfg.langevin_eta_max, plong=cfg.storage_long,
-                   long_moves=cfg.storage_long_moves, long_hours=cfg.storage_long_hours,
-                   tuning=cfg.launch_tuning, extra=cfg.local_extra, measure=cfg.measure, ffbs=cfg.replica_ffbs)
+                   long_moves=cfg.storage_long_moves, long_hours=cfg.storage_long_hours, xpair=cfg.storage_xpair,
+                   xlong=cfg.storage_xlong, tuning=cfg.launch_tuning, extra=cfg.local_extra, measure=cfg.measure, ffbs=cfg.replica_ffbs)
 
     def sweep_options(self, dev):
         """The operation sweep's options of a replica set (``local_extra``, the long-range storage
-        sweep's keys), the reference measure of its scores and its launch form (from the tuning
-        table when it holds the set's shape)."""
+        sweep's keys, the cross-node storage pairs ``storage_xpair`` / ``storage_xlong``), the
+        reference measure of its scores and its launch form (from the tuning table when it holds
+        the set's shape)."""
         api.sweep_options(dev, extra=self.extra)
         api.storage_long_options(dev, p=self.plong, nmoves=self.long_moves, hmin=self.long_hours)
+        api.storage_cross_options(dev, p_xpair=self.xpair, nxlong=self.xlong)
         api.score_measure(dev, self.measure)
         if self.tuning == "table":
             from . import tune

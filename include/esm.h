@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 7
+#define ESM_ABI_VERSION 8
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -226,7 +226,9 @@ int32_t esm_langevin_noalloc(esm_handle *h, const double *weights, const double 
 #define ESM_FAM_SMS 10    /* storage multiscale: block plateaus (storage_ms_sweep!)   */
 #define ESM_FAM_SLONG 11  /* long-range storage plateau (slong, storage_long_sweep!)  */
 #define ESM_FAM_SLEVEL 12 /* level move of a storage trajectory (slevel)              */
-#define ESM_NFAM 13
+#define ESM_FAM_SXPAIR 13 /* random: cross-node storage pair at one instant (sxpair)  */
+#define ESM_FAM_SXLONG 14 /* interval storage pair of the long-range sweep (sxlong)   */
+#define ESM_NFAM 15
 #define ESM_NSTAT 5       /* attempted, accepted, improper, guard, ΔC evaluations     */
 /* the plain random families in the reference's order and default weights
  * (plain_families, src/local.jl:1912-1915): unit 0.3, pair 0.25, xpair 0.03, xlocc 0.08,
@@ -293,6 +295,35 @@ int32_t esm_sweep_options(esm_handle *h, double extra, const double *weights, in
  * 0x74000000 + i, c). Statistics: families ESM_FAM_SLONG (plateaus) and ESM_FAM_SLEVEL.
  */
 int32_t esm_storage_long_options(esm_handle *h, double p, int32_t nmoves, double hmin);
+
+/*
+ * Cross-node storage pairs (src/storage_cross.jl; `operation_sweep!` keywords `sxpair` and
+ * `nxmoves`, src/local.jl:2414-2422; configured by `storage_xpair` 0.1 and `storage_xlong` 8,
+ * src/run.jl:30-31; both 0, off, from esm_create on). A pair moves energy between unit A and a
+ * unit B at A's node or a neighbour (partner hops 1, ball radius 1: `storage_xpair_hops`,
+ * `storage_xpair_ball`): A's state of charge rises by t and B's falls by ρ t (ρ = E_A/E_B) at the
+ * instants a+1 … a+L; at the end steps the flow between them is re-routed on the ball of radius
+ * one around the pair (the whole component when the ball covers it) and the leak into the first
+ * layer outside is compensated pro rata by the gas unit of each leak node, with the Jacobian test.
+ * The units taking part have at least the long-range sweep's `hmin` hours; B is drawn with
+ * probability ∝ min(k_A, k_B) among those with κ = k h/Δt ≥ 1 (`sx_partner`, :62-90).
+ *   p_xpair  weight of `sxpair` (L = 1, instant τ: steps τ−1 and τ) among the plain families of
+ *            the local sweep; the draws per step are scaled by Σ weights / Σ plain weights so that
+ *            every plain family keeps its rate (local_sweep!, :1934-1941). Family weight position
+ *            9 in the draw counters: selection (2^31 + 9·2^20 + index, step, 9001, ·) with A from
+ *            word 0 and the partner from words 2–3; line draw and Jacobian test as the other
+ *            families. Statistics ESM_FAM_SXPAIR.
+ *   nxlong   `sxlong` moves per unit of the pairs with κ ≥ 1 in the long-range sweep, after its
+ *            plateaus whenever it runs (0 … 256): partner, a uniform on 0 … T−1 and L uniform on
+ *            1 … T−1 from (u, m, 9420, c) for move m of the unit with index u among the units of
+ *            the pairs; the list shuffled with (i, 0, 9421, c) and run in rounds of disjoint
+ *            footprints (first fit, `sx_rounds` :470-492); the i-th move of the round-ordered list
+ *            draws its line with (replica, i, 0x5C000000, c) and its Jacobian test with (replica,
+ *            i, 0x7C000000, c). Statistics ESM_FAM_SXLONG.
+ * Units that only discharge (eta_in = 0) cannot take part (the loss share of `sx_loss` is not
+ * built): ESM_EINVAL. At most 256 nodes.
+ */
+int32_t esm_storage_cross_options(esm_handle *h, double p_xpair, int32_t nxlong);
 
 /*
  * How the random families of an operation sweep are launched (no counterpart in the reference,
@@ -443,7 +474,9 @@ int32_t esm_copy_replica(esm_handle *src, int32_t js, esm_handle *dst, int32_t j
 #define ESM_KERN_MLPP 18   /* the learned moves of one chunk (draw, MH ratio, commit)         */
 #define ESM_KERN_MLPT 19   /* one training step (features, forward, head, backward, Adam)     */
 #define ESM_KERN_RANDOM 20 /* the table-driven random families of one parity class, merged    */
-#define ESM_NKERN 21
+#define ESM_KERN_SXPAIR 21 /* the storage pairs sxpair of one parity class                     */
+#define ESM_KERN_SXLONG 22 /* the interval storage pairs sxlong of a long-range sweep          */
+#define ESM_NKERN 23
 int32_t esm_profile(esm_handle *h, int32_t enable);
 int32_t esm_profile_read(esm_handle *h, int32_t kernel, double *total_ms, int64_t *launches);
 

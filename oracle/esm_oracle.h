@@ -61,8 +61,19 @@ int orc_ref_sweeps(const esm_model *m, int R, double *v, const double *k, double
                    int64_t *stats, int nthreads);
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
                const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
-               double p_storage_ms, double p_storage_long, int long_moves, double long_hours, int64_t *stats,
-               int nthreads);
+               double p_storage_ms, double p_storage_long, int long_moves, double long_hours, double p_xpair,
+               int xlong_moves, int64_t *stats, int nthreads);
+/* Cross-node storage pairs (oracle_sweep.c): slot k of node n's pair templates (partner, angle
+ * entries, leak nodes and their requirement change per unit, footprint); the partner draw of unit
+ * (sa, na) with the uniform u (1, *sb, *nb; 0 none); one pair move of replica j (units A, B at
+ * the slot of B's node among A's; instants a+1 … a+L; W = 1 the sxpair summation, 64 sxlong's):
+ * op 0 ΔC(t), op 1 draws with given uniforms, op 2 commit t[0]. */
+int orc_sx_slot(const esm_model *m, int n, int k, int *part, int *nth, int *thn, double *thv, int *nlk, int *lkn,
+                double *lkg, int *nft, int *ft);
+int orc_sx_partner(const esm_model *m, const double *k, double hmin, int sa, int na, double u, int *sb, int *nb);
+int orc_sx_move(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int sa, int na,
+                int sb, int nb, int a, int L, int W, int op, const double *t, int nt, double maxnats, const double *useg,
+                const double *uoff, double *out, int *status);
 /* One storage multiscale plateau (oracle_sweep.c) on the block b0 … b0 + L of replica j for the
  * units of carrier si at nodes[nu]: ΔC(t), draws with given uniforms (no commit), the commit. */
 int orc_sms_cost(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int L, int b0,

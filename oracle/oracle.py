@@ -24,6 +24,9 @@ _u32p = C.POINTER(C.c_uint32)
 _lib = None
 
 
+NFAM = 15   # move families of the statistics (include/esm.h ESM_NFAM)
+
+
 def load():
     global _lib
     if _lib is None:
@@ -51,8 +54,12 @@ def load():
         lib.orc_regime_window_path.restype = None
         i64p = C.POINTER(C.c_int64)
         lib.orc_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double,
-                                   C.c_double, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, i64p,
-                                   C.c_int]
+                                   C.c_double, _dp, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                   C.c_int, i64p, C.c_int]
+        lib.orc_sx_slot.argtypes = [M, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]
+        lib.orc_sx_partner.argtypes = [M, _dp, C.c_double, C.c_int, C.c_int, C.c_double, _ip, _ip]
+        lib.orc_sx_move.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
         lib.orc_ref_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double,
                                        C.c_double, _dp, i64p, C.c_int]
         lib.orc_slong_cost.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -190,8 +197,10 @@ class Oracle:
 
     def sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1,
                extra=4.0, weights=None, atoms=True, p_storage_ms=0.5, p_storage_long=1.0, long_moves=32,
-               long_hours=24.0):
-        """Operation sweeps in place on copies; returns (v, r, F, stats [13, 5])."""
+               long_hours=24.0, p_xpair=0.0, xlong_moves=0):
+        """Operation sweeps in place on copies; returns (v, r, F, stats [NFAM, 5]). ``p_xpair``: the
+        weight of the cross-node storage pair family sxpair in the local sweep, ``xlong_moves``:
+        the interval pairs sxlong per storage unit in the long-range sweep (0, 0: off)."""
         R = v.shape[-1]
         v = np.ascontiguousarray(v, dtype=np.float64).copy()
         k = np.ascontiguousarray(k, dtype=np.float64)
@@ -199,21 +208,21 @@ class Oracle:
             _, _, r, F = self.energy(v, k, x)
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         F = np.ascontiguousarray(F, dtype=np.float64).copy()
-        st = np.zeros(65, dtype=np.int64)
+        st = np.zeros(NFAM * 5, dtype=np.int64)
         kk = (C.c_uint32 * 2)(*key)
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         rc = self.lib.orc_sweeps(C.byref(self.cm), R, _p(v), _p(k), _p(r), _p(F), int(nsweeps), kk, int(counter),
                                  float(maxnats), float(extra), _p(w), int(bool(atoms)), float(p_storage_ms),
-                                 float(p_storage_long), int(long_moves), float(long_hours),
-                                 st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
+                                 float(p_storage_long), int(long_moves), float(long_hours), float(p_xpair),
+                                 int(xlong_moves), st.ctypes.data_as(C.POINTER(C.c_int64)), int(threads))
         assert rc == 0, rc
-        return v, r, F, st.reshape(13, 5)
+        return v, r, F, st.reshape(NFAM, 5)
 
     def ref_sweeps(self, v, k, nsweeps, key=(1, 0), counter=0, maxnats=40.0, x=None, r=None, F=None, threads=1,
                    extra=4.0, weights=None):
         """Sweeps in the reference's own schedule (``local_sweep!`` with the plain families: per
         replica and step, the atoms and the family draws in one shuffled list; no storage sweeps),
-        in place on copies; returns (v, r, F, stats [13, 5])."""
+        in place on copies; returns (v, r, F, stats [NFAM, 5])."""
         R = v.shape[-1]
         v = np.ascontiguousarray(v, dtype=np.float64).copy()
         k = np.ascontiguousarray(k, dtype=np.float64)
@@ -221,14 +230,64 @@ class Oracle:
             _, _, r, F = self.energy(v, k, x)
         r = np.ascontiguousarray(r, dtype=np.float64).copy()
         F = np.ascontiguousarray(F, dtype=np.float64).copy()
-        st = np.zeros(65, dtype=np.int64)
+        st = np.zeros(NFAM * 5, dtype=np.int64)
         kk = (C.c_uint32 * 2)(*key)
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
         rc = self.lib.orc_ref_sweeps(C.byref(self.cm), R, _p(v), _p(k), _p(r), _p(F), int(nsweeps), kk, int(counter),
                                      float(maxnats), float(extra), _p(w), st.ctypes.data_as(C.POINTER(C.c_int64)),
                                      int(threads))
         assert rc == 0, rc
-        return v, r, F, st.reshape(13, 5)
+        return v, r, F, st.reshape(NFAM, 5)
+
+    # -- cross-node storage pairs (sxpair: L = 1, W = 1; sxlong: W = 64) -------------------------
+    def sx_slot(self, n, k):
+        """Slot k of node n's storage-pair templates: dict(partner, th (nodes, values), leak (nodes,
+        requirement change per unit), foot) or None past the last slot."""
+        N = self.sys.N
+        part, nth, nlk, nft = (C.c_int(), C.c_int(), C.c_int(), C.c_int())
+        thn, lkn, ft = (np.zeros(N, dtype=np.int32) for _ in range(3))
+        thv, lkg = np.zeros(N), np.zeros(N)
+        rc = self.lib.orc_sx_slot(C.byref(self.cm), int(n), int(k), C.byref(part), C.byref(nth),
+                                  thn.ctypes.data_as(_ip), _p(thv), C.byref(nlk), lkn.ctypes.data_as(_ip), _p(lkg),
+                                  C.byref(nft), ft.ctypes.data_as(_ip))
+        if rc == -2:
+            return None
+        assert rc == 0, rc
+        return dict(partner=part.value, th=(thn[:nth.value].copy(), thv[:nth.value].copy()),
+                    leak=(lkn[:nlk.value].copy(), lkg[:nlk.value].copy()), foot=ft[:nft.value].copy())
+
+    def sx_partner(self, k, sa, na, u, hmin=24.0):
+        """The partner (sb, nb) of unit (sa, na) drawn with the uniform u, or None."""
+        sb, nb = C.c_int(), C.c_int()
+        k = np.ascontiguousarray(k, dtype=np.float64)
+        rc = self.lib.orc_sx_partner(C.byref(self.cm), _p(k), float(hmin), int(sa), int(na), float(u), C.byref(sb),
+                                     C.byref(nb))
+        assert rc >= 0, rc
+        return (sb.value, nb.value) if rc == 1 else None
+
+    def _sx(self, v, k, r, F, j, A, B, a, L, W, op, ts, maxnats=40.0, useg=None, uoff=None):
+        R = v.shape[-1]
+        ts = np.ascontiguousarray(ts, dtype=np.float64)
+        n = len(ts) if useg is None else len(useg)
+        out = np.zeros(max(n, 1))
+        st = np.zeros(max(n, 1), dtype=np.int32)
+        us = None if useg is None else np.ascontiguousarray(useg, dtype=np.float64)
+        uo = None if uoff is None else np.ascontiguousarray(uoff, dtype=np.float64)
+        rc = self.lib.orc_sx_move(C.byref(self.cm), R, *self._mv_args(v, k, r, F), int(j), int(A[0]), int(A[1]),
+                                  int(B[0]), int(B[1]), int(a), int(L), int(W), int(op), _p(ts), n, float(maxnats),
+                                  _p(us), _p(uo), _p(out), st.ctypes.data_as(_ip))
+        assert rc == 0, rc
+        return out[:n], st[:n]
+
+    def sx_cost(self, v, k, r, F, j, A, B, a, L, ts, W=1):
+        """ΔC(t) of the pair move of units A = (carrier, node) and B at instants a+1 … a+L."""
+        return self._sx(v, k, r, F, j, A, B, a, L, W, 0, ts)[0]
+
+    def sx_draw_many(self, v, k, r, F, j, A, B, a, L, useg, uoff, W=1, maxnats=40.0):
+        return self._sx(v, k, r, F, j, A, B, a, L, W, 1, np.zeros(1), maxnats, useg, uoff)
+
+    def sx_apply(self, v, k, r, F, j, A, B, a, L, t, W=1):
+        self._sx(v, k, r, F, j, A, B, a, L, W, 2, np.array([float(t)]))
 
     # -- storage multiscale plateaus -----------------------------------------------------------
     def sms_cost(self, v, k, r, F, j, L, b0, si, nodes, ts):

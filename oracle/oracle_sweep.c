@@ -667,13 +667,27 @@ typedef struct {
         nd_l, nl_pos, nl_sgn, xp_off, lev_off;
     DVec th_val, en_c, kk_g, ln_df, qcum;
     int xp0, cl0, ntpl, nlev;
+    /* storage pairs (sx_templates, storage_cross.jl:29-53): the slots sx_poff[n] … sx_poff[n+1]−1
+     * of node n, the first the node itself, then its partners within one hop in increasing order;
+     * per slot the partner, the angle entries of a unit dipole re-routed on the ball of radius one
+     * around the pair (or on the whole component when the ball covers it), the leak nodes with
+     * their requirement change per unit, the lines at the angle nodes with the angle change at
+     * each end (unscaled: the move scales and filters them as `finalize!` does), the touched nodes
+     * (line ends, leak nodes, the pair) with their lines and leak, and the footprint (sxfoot) */
+    IVec sx_poff, sx_part, sx_th_off, sx_th_node, sx_kk_off, sx_kk_node, sx_ln_off, sx_ln_line, sx_nd_off,
+        sx_nd_node, sx_nd_kk, sx_nd_l, sx_nl_pos, sx_nl_sgn, sx_ft_off, sx_ft_node;
+    DVec sx_th_val, sx_kk_g, sx_ln_ta, sx_ln_tb;
 } Tables;
 
 static void tables_free(Tables *tb) {
     IVec *iv[] = {&tb->th_off, &tb->th_node, &tb->en_off, &tb->en_node, &tb->en_div, &tb->kk_off, &tb->kk_node,
                   &tb->ln_off, &tb->ln_line, &tb->nd_off, &tb->nd_node, &tb->nd_ent, &tb->nd_kk, &tb->nd_l,
-                  &tb->nl_pos, &tb->nl_sgn, &tb->xp_off, &tb->lev_off};
-    DVec *dv[] = {&tb->th_val, &tb->en_c, &tb->kk_g, &tb->ln_df, &tb->qcum};
+                  &tb->nl_pos, &tb->nl_sgn, &tb->xp_off, &tb->lev_off, &tb->sx_poff, &tb->sx_part, &tb->sx_th_off,
+                  &tb->sx_th_node, &tb->sx_kk_off, &tb->sx_kk_node, &tb->sx_ln_off, &tb->sx_ln_line, &tb->sx_nd_off,
+                  &tb->sx_nd_node, &tb->sx_nd_kk, &tb->sx_nd_l, &tb->sx_nl_pos, &tb->sx_nl_sgn, &tb->sx_ft_off,
+                  &tb->sx_ft_node};
+    DVec *dv[] = {&tb->th_val, &tb->en_c, &tb->kk_g, &tb->ln_df, &tb->qcum, &tb->sx_th_val, &tb->sx_kk_g,
+                  &tb->sx_ln_ta, &tb->sx_ln_tb};
     for (size_t q = 0; q < sizeof(iv) / sizeof(iv[0]); q++) free(iv[q]->a);
     for (size_t q = 0; q < sizeof(dv) / sizeof(dv[0]); q++) free(dv[q]->a);
 }
@@ -809,10 +823,10 @@ static void reroute(const Net *net, int n, int mm, double *x) {
     free(rhs);
 }
 
-static int build_tables(const esm_model *m, Tables *tb) {
+/* the network of the templates: incident lines and neighbours per node, components (nodes
+ * increasing) and the grounded Laplacian factor of each; -2 if one is not positive definite */
+static int net_build(const esm_model *m, Net *netp) {
     int N = m->N, L = m->L;
-    memset(tb, 0, sizeof(*tb));
-    if (N > 256) return -1;
     Net net;
     memset(&net, 0, sizeof(net));
     net.m = m;
@@ -888,6 +902,178 @@ static int build_tables(const esm_model *m, Tables *tb) {
         if (!chol(A, k)) return -2;
         net.G[c] = A;
     }
+    free(stack);
+    *netp = net;
+    return 0;
+}
+
+static void net_free(Net *net) {
+    for (int c = 0; c < net->ncomp; c++) free(net->G[c]);
+    free(net->G); free(net->lptr); free(net->lines); free(net->nptr); free(net->nbr); free(net->comp);
+    free(net->cnodes); free(net->cptr); free(net->loc);
+}
+
+/* one storage-pair slot (pair n, mm): angle entries thn/thv, leak nodes lkn/lkg (sorted by node) */
+static void sx_emit(Tables *tb, const Net *net, int n, int mm, const int *thn, const double *thv, int nth,
+                    const int *lkn, const double *lkg, int nlk) {
+    const esm_model *m = net->m;
+    int N = net->N, L = net->L;
+    double *dth = (double *)calloc(N, sizeof(double));
+    int *lmark = (int *)calloc(L > 0 ? L : 1, sizeof(int)), *nmark = (int *)calloc(N, sizeof(int)),
+        *fmark = (int *)calloc(N, sizeof(int));
+    for (int q = 0; q < nth; q++) {
+        int i = thn[q];
+        ipush(&tb->sx_th_node, i);
+        dpush(&tb->sx_th_val, thv[q]);
+        dth[i] = thv[q];
+        fmark[i] = 1;
+        for (int p2 = net->lptr[i]; p2 < net->lptr[i + 1]; p2++) {
+            int e = net->lines[p2];
+            lmark[e] = 1;
+            fmark[m->line_bus0[e]] = fmark[m->line_bus1[e]] = 1;
+        }
+    }
+    int *ll = (int *)malloc(sizeof(int) * (L + 1)), nl = 0;
+    for (int e = 0; e < L; e++) {
+        if (!lmark[e]) continue;
+        double ta = dth[m->line_bus0[e]], tb2 = dth[m->line_bus1[e]];
+        if (ta == tb2) continue;                     /* no flow change at any scale */
+        ipush(&tb->sx_ln_line, e);
+        dpush(&tb->sx_ln_ta, ta);
+        dpush(&tb->sx_ln_tb, tb2);
+        ll[nl++] = e;
+        nmark[m->line_bus0[e]] = nmark[m->line_bus1[e]] = 1;
+    }
+    for (int q = 0; q < nlk; q++) {
+        ipush(&tb->sx_kk_node, lkn[q]);
+        dpush(&tb->sx_kk_g, lkg[q]);
+        nmark[lkn[q]] = fmark[lkn[q]] = 1;
+    }
+    nmark[n] = nmark[mm] = fmark[n] = fmark[mm] = 1;
+    for (int i = 0; i < N; i++) {
+        if (!nmark[i]) continue;
+        int kk = -1;
+        for (int q = 0; q < nlk; q++)
+            if (lkn[q] == i) kk = q;
+        ipush(&tb->sx_nd_node, i);
+        ipush(&tb->sx_nd_kk, kk);
+        for (int l = 0; l < nl; l++) {
+            int e = ll[l];
+            if (m->line_bus0[e] == i || m->line_bus1[e] == i) {
+                ipush(&tb->sx_nl_pos, l);
+                ipush(&tb->sx_nl_sgn, m->line_bus0[e] == i ? 1 : -1);
+            }
+        }
+        ipush(&tb->sx_nd_l, tb->sx_nl_pos.n);
+    }
+    for (int i = 0; i < N; i++)
+        if (fmark[i]) ipush(&tb->sx_ft_node, i);
+    ipush(&tb->sx_part, mm);
+    ipush(&tb->sx_th_off, tb->sx_th_node.n);
+    ipush(&tb->sx_kk_off, tb->sx_kk_node.n);
+    ipush(&tb->sx_ln_off, tb->sx_ln_line.n);
+    ipush(&tb->sx_nd_off, tb->sx_nd_node.n);
+    ipush(&tb->sx_ft_off, tb->sx_ft_node.n);
+    free(dth); free(lmark); free(nmark); free(fmark); free(ll);
+}
+
+/* sx_templates (storage_cross.jl:29-53) with xloc_directions (src/local.jl:122-191) at partner
+ * hops 1 and ball radius 1: per node its own slot, then per partner m (neighbours, increasing)
+ * the ball B = ball(n) ∪ ball(m), L_BB Δθ = e_n − e_m with Δθ = 0 outside B (entries ≥ 1e-15
+ * kept), the leak −Σ b Δθ_i into the first layer outside B (accumulated in B and line order,
+ * sorted by node); the exact re-routing when B holds the whole component (no leak) */
+static int build_sx(const Net *net, Tables *tb) {
+    const esm_model *m = net->m;
+    int N = net->N;
+    int *inb = (int *)calloc(N, sizeof(int)), *B = (int *)malloc(sizeof(int) * N), *pos = (int *)malloc(sizeof(int) * N);
+    int *thn = (int *)malloc(sizeof(int) * N), *lkn = (int *)malloc(sizeof(int) * N);
+    double *thv = (double *)malloc(sizeof(double) * N), *lkg = (double *)malloc(sizeof(double) * N);
+    double *x = (double *)malloc(sizeof(double) * N);
+    ipush(&tb->sx_poff, 0);
+    ipush(&tb->sx_th_off, 0);
+    ipush(&tb->sx_kk_off, 0);
+    ipush(&tb->sx_ln_off, 0);
+    ipush(&tb->sx_nd_off, 0);
+    ipush(&tb->sx_nd_l, 0);
+    ipush(&tb->sx_ft_off, 0);
+    int rc = 0;
+    for (int n = 0; n < N && !rc; n++) {
+        sx_emit(tb, net, n, n, NULL, NULL, 0, NULL, NULL, 0);
+        int np = 0, part[256];
+        for (int q = net->nptr[n]; q < net->nptr[n + 1]; q++) {
+            int mm = net->nbr[q], seen = mm == n;
+            for (int r = 0; r < np; r++) seen |= part[r] == mm;
+            if (!seen) part[np++] = mm;
+        }
+        qsort(part, np, sizeof(int), cmp_int);
+        for (int pi = 0; pi < np && !rc; pi++) {
+            int mm = part[pi], nb = 0, nth = 0, nlk = 0;
+            for (int i = 0; i < N; i++) inb[i] = 0;
+            int ends[2] = {n, mm};
+            for (int e2 = 0; e2 < 2; e2++) {
+                int c = ends[e2];
+                inb[c] = 1;
+                for (int q = net->nptr[c]; q < net->nptr[c + 1]; q++) inb[net->nbr[q]] = 1;
+            }
+            for (int i = 0; i < N; i++) {
+                pos[i] = -1;
+                if (inb[i]) { pos[i] = nb; B[nb++] = i; }
+            }
+            int comp = net->comp[n], csize = net->cptr[comp + 1] - net->cptr[comp];
+            if (nb >= csize) {
+                reroute(net, n, mm, x);
+                for (int q = net->cptr[comp]; q < net->cptr[comp + 1]; q++) {
+                    int i = net->cnodes[q];
+                    if (fabs(x[i]) >= 1e-15) { thn[nth] = i; thv[nth++] = x[i]; }
+                }
+            } else {
+                double *A = (double *)calloc((size_t)nb * nb, sizeof(double)), *th = (double *)calloc(nb, sizeof(double));
+                for (int q = 0; q < nb; q++)
+                    for (int p2 = net->lptr[B[q]]; p2 < net->lptr[B[q] + 1]; p2++) {
+                        int e = net->lines[p2], a = m->line_bus0[e], b = m->line_bus1[e];
+                        int o = a == B[q] ? b : a;
+                        A[(size_t)q * nb + q] += m->line_b[e];
+                        if (pos[o] >= 0) A[(size_t)q * nb + pos[o]] -= m->line_b[e];
+                    }
+                if (!chol(A, nb)) rc = -2;
+                if (!rc) {
+                    th[pos[n]] += 1.0;
+                    th[pos[mm]] -= 1.0;
+                    chol_solve(A, nb, th);
+                    for (int q = 0; q < nb; q++)
+                        if (fabs(th[q]) >= 1e-15) { thn[nth] = B[q]; thv[nth++] = th[q]; }
+                    for (int q = 0; q < nb; q++)
+                        for (int p2 = net->lptr[B[q]]; p2 < net->lptr[B[q] + 1]; p2++) {
+                            int e = net->lines[p2], a = m->line_bus0[e], b = m->line_bus1[e];
+                            int o = a == B[q] ? b : a;
+                            if (pos[o] >= 0) continue;
+                            int jl = 0;
+                            while (jl < nlk && lkn[jl] != o) jl++;
+                            if (jl == nlk) { lkn[nlk] = o; lkg[nlk] = 0.0; nlk++; }
+                            lkg[jl] -= m->line_b[e] * th[q];
+                        }
+                    for (int a = 1; a < nlk; a++)      /* insertion sort by node */
+                        for (int b = a; b > 0 && lkn[b - 1] > lkn[b]; b--) {
+                            int tn = lkn[b]; lkn[b] = lkn[b - 1]; lkn[b - 1] = tn;
+                            double tg = lkg[b]; lkg[b] = lkg[b - 1]; lkg[b - 1] = tg;
+                        }
+                }
+                free(A); free(th);
+            }
+            if (!rc) sx_emit(tb, net, n, mm, thn, thv, nth, lkn, lkg, nlk);
+        }
+        ipush(&tb->sx_poff, tb->sx_part.n);
+    }
+    free(inb); free(B); free(pos); free(thn); free(lkn); free(thv); free(lkg); free(x);
+    return rc;
+}
+
+static int build_tables(const esm_model *m, Tables *tb) {
+    int N = m->N;
+    memset(tb, 0, sizeof(*tb));
+    if (N > 256) return -1;
+    Net net;
+    if (net_build(m, &net)) return -2;
     ipush(&tb->th_off, 0);
     ipush(&tb->en_off, 0);
     ipush(&tb->kk_off, 0);
@@ -1004,10 +1190,9 @@ static int build_tables(const esm_model *m, Tables *tb) {
     for (int q = 0; q < tb->qcum.n; q++) tb->qcum.a[q] /= acc;
     tb->nlev = nlev;
     free(x);
-    for (int c = 0; c < net.ncomp; c++) free(net.G[c]);
-    free(net.G); free(net.lptr); free(net.lines); free(net.nptr); free(net.nbr); free(net.comp); free(stack);
-    free(net.cnodes); free(net.cptr); free(net.loc);
-    return 0;
+    int rc = build_sx(&net, tb);
+    net_free(&net);
+    return rc;
 }
 
 /* ---- a table-driven move of replica j at τ, evaluated on the fly from the caches ---- */
@@ -1306,39 +1491,53 @@ static double sm_move(Sys *S, const Tables *tb, int j, int tau, int tid, int kin
 
 static uint32_t pick(uint32_t w, uint32_t n) { return (uint32_t)(((uint64_t)w * n) >> 32); }
 
-/* family counts of every step: round(extra·N) draws over the cumulative weights */
-static void move_counts(const esm_model *m, double extra, const double *wts, const uint32_t key[2], uint32_t ctr,
-                        int *counts) {
-    int T = m->T, nextra = (int)nearbyint(extra * m->N);
-    double cw[ESM_NWEIGHT], acc = 0.0, tot = 0.0;
-    for (int q = 0; q < ESM_NWEIGHT; q++) tot += wts[q];
-    for (int q = 0; q < ESM_NWEIGHT; q++) { acc += wts[q]; cw[q] = acc / tot; }
+/* the families of the draws: the plain ones, then the storage pair sxpair */
+#define NW_ALL (ESM_NWEIGHT + 1)
+
+/* family counts of every step: round(extra·xmul·N) draws over the cumulative weights; with sxpair
+ * (weight px > 0, some storage unit taking part) its weight follows the plain ones and xmul =
+ * Σ all / Σ plain keeps every plain family's rate (local_sweep! :1934-1941) */
+static void move_counts(const esm_model *m, double extra, const double *wts, double px, const uint32_t key[2],
+                        uint32_t ctr, int *counts) {
+    int T = m->T;
+    double wall[NW_ALL], cw[NW_ALL], acc = 0.0, tot = 0.0, tot0 = 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) wall[q] = wts[q];
+    wall[ESM_NWEIGHT] = px > 0 ? px : 0.0;
+    for (int q = 0; q < ESM_NWEIGHT; q++) tot0 += wall[q];
+    for (int q = 0; q < NW_ALL; q++) tot += wall[q];
+    int nextra = tot > 0 ? (int)nearbyint(extra * (tot0 > 0 ? tot / tot0 : 1.0) * m->N) : 0;
+    for (int q = 0; q < NW_ALL; q++) { acc += wall[q]; cw[q] = acc / tot; }
     for (int tau = 0; tau < T; tau++) {
-        int cnt[ESM_NWEIGHT] = {0};
+        int cnt[NW_ALL] = {0};
         for (int d = 0; d < nextra; d++) {
             uint32_t c4[4] = {0x80000000u | (uint32_t)d, (uint32_t)tau, 9000u, ctr}, w[4];
             orc_philox(key, c4, w);
             double u = u01(w[0], w[1]);
-            int f = ESM_NWEIGHT - 1;
-            for (int q = ESM_NWEIGHT - 1; q >= 0; q--)
+            int f = NW_ALL - 1;
+            for (int q = NW_ALL - 1; q >= 0; q--)
                 if (cw[q] >= u) f = q;
             cnt[f]++;
         }
-        for (int q = 0; q < ESM_NWEIGHT; q++) counts[(size_t)q * T + tau] = nextra > 0 ? cnt[q] : 0;
+        for (int q = 0; q < NW_ALL; q++) counts[(size_t)q * T + tau] = nextra > 0 ? cnt[q] : 0;
     }
 }
 
-enum { W_UNIT = 0, W_PAIR = 1, W_XPAIR = 2, W_XLOCC = 3, W_XLOC = 4, W_POT = 5, W_CELLPAIR = 6, W_CELL = 7, W_SPAIR = 8 };
+enum { W_UNIT = 0, W_PAIR = 1, W_XPAIR = 2, W_XLOCC = 3, W_XLOC = 4, W_POT = 5, W_CELLPAIR = 6, W_CELL = 7, W_SPAIR = 8,
+       W_SXPAIR = 9 };
 
-/* the random moves of step τ for replica j, family by family (pair and cellpair are void) */
-static void random_step(Sys *S, const Tables *tb, int j, int tau, const int *counts, const double *wts,
-                        const uint32_t key[2], uint32_t ctr, double maxnats, int64_t *st) {
+static void sxpair_move(Sys *S, const Tables *tb, int j, int tau, const uint32_t *s, double hmin, double useg,
+                        double uoff, double umh, double maxnats, int64_t *stats);
+
+/* the random moves of step τ for replica j, family by family (pair and cellpair are void); the
+ * storage pairs last, with weight px and units of at least hmin hours */
+static void random_step(Sys *S, const Tables *tb, int j, int tau, const int *counts, const double *wts, double px,
+                        double hmin, const uint32_t key[2], uint32_t ctr, double maxnats, int64_t *st) {
     const esm_model *m = S->m;
     int N = m->N, T = m->T;
-    static const int order[7] = {W_UNIT, W_POT, W_SPAIR, W_XPAIR, W_XLOCC, W_XLOC, W_CELL};
-    for (int oi = 0; oi < 7; oi++) {
+    static const int order[8] = {W_UNIT, W_POT, W_SPAIR, W_XPAIR, W_XLOCC, W_XLOC, W_CELL, W_SXPAIR};
+    for (int oi = 0; oi < 8; oi++) {
         int w = order[oi];
-        if (!(wts[w] > 0)) continue;
+        if (!((w == W_SXPAIR ? px : wts[w]) > 0)) continue;
         int cnt = counts[(size_t)w * T + tau];
         int nbucket = w == W_POT ? 3 : 1;
         for (int bk = 0; bk < nbucket; bk++)
@@ -1349,7 +1548,11 @@ static void random_step(Sys *S, const Tables *tb, int j, int tau, const int *cou
                 orc_philox(key, cl, wl);
                 double useg = u01(wl[0], wl[1]), uoff = u01(wl[2], wl[3]);
                 int n = (int)pick(s[0], (uint32_t)N);
-                if (w == W_UNIT) {
+                if (w == W_SXPAIR) {
+                    uint32_t cm[4] = {(uint32_t)j, (uint32_t)tau, 0x60000000u | ((uint32_t)w << 20) | (uint32_t)k, ctr}, wm[4];
+                    orc_philox(key, cm, wm);
+                    sxpair_move(S, tb, j, tau, s, hmin, useg, uoff, u01(wm[0], wm[1]), maxnats, st + 5 * ESM_FAM_SXPAIR);
+                } else if (w == W_UNIT) {
                     do_move_u(S, j, 0, 0, n, tau, useg, uoff, maxnats, st + 5 * ESM_FAM_UNIT);
                 } else if (w == W_POT) {
                     int deg = S->adj_ptr[n + 1] - S->adj_ptr[n];
@@ -2516,12 +2719,12 @@ static int slg_moves(const esm_model *m, const double *kc, int n, int nmoves, do
     return cnt;
 }
 
-static void slg_sweep(Sys *S, int j, double plong, int nmoves, double hmin, const uint32_t key[2], uint32_t ctr,
-                      double maxnats, int64_t *st_long, int64_t *st_level) {
+static int slg_sweep(Sys *S, int j, double plong, int nmoves, double hmin, const uint32_t key[2], uint32_t ctr,
+                     double maxnats, int64_t *st_long, int64_t *st_level) {
     const esm_model *m = S->m;
     uint32_t c0[4] = {0u, 0u, 9410u, ctr}, w[4];
     orc_philox(key, c0, w);
-    if (!(u01(w[0], w[1]) < plong)) return;
+    if (!(u01(w[0], w[1]) < plong)) return 0;
     int cap = ESM_NSTOR * (nmoves + 1);
     int *msi = (int *)malloc(sizeof(int) * cap), *ma = (int *)malloc(sizeof(int) * cap), *mL = (int *)malloc(sizeof(int) * cap);
     for (int n = 0; n < m->N; n++) {
@@ -2536,6 +2739,7 @@ static void slg_sweep(Sys *S, int j, double plong, int nmoves, double hmin, cons
         }
     }
     free(msi); free(ma); free(mL);
+    return 1;
 }
 
 /* ---- explicit long-range moves, for the pins ---- */
@@ -2582,6 +2786,683 @@ int orc_slong_apply(const esm_model *m, int R, double *v, const double *k, doubl
     return 0;
 }
 
+/* ============================================================================================
+ * Cross-node storage pairs (storage_cross.jl): unit A's state of charge rises by t and unit B's
+ * (at A's node or a partner node) falls by ρ t, ρ = E_A/E_B (equal nominal energy), at the cyclic
+ * instants a+1 … a+L. At the end steps a (nominal injection −κ_A t at A's node, +κ_A t at B's) and
+ * b = a+L (the opposite) the flow between them is re-routed on the pair's ball template and the
+ * leak is compensated pro rata by the gas unit of each leak node (prorata!, Jacobian test); the
+ * interior steps a+1 … a+L−1 hold both units' storage terms and the balances of their nodes.
+ * L = 1 is `sxpair`, a family of the local sweep (prep_sxpair! :147-167, sxpair_move! :170-190);
+ * L drawn on 1 … T−1 is `sxlong` (prep_sxlong! :252-350, mv_eval! :370-427, commit_sxlong!
+ * :430-461, storage_cross_phase! :501-544). Every unit of this system can charge, so the loss
+ * share λ of sx_loss (:108) is 0: the losses go into the residuals of the units' nodes.
+ * The arithmetic is the device's: step q (0 … L: step a+q mod T) belongs to lane q mod W (W = 1
+ * for sxpair, 64 for sxlong); a lane sums its steps' terms in increasing q and the lanes' sums
+ * are combined by the W-lane butterfly.
+ * ============================================================================================ */
+
+typedef struct {
+    Sys *S;
+    const Tables *tb;
+    int j, a0, L, W, slot;
+    int si[2], nd[2];      /* units A and B: carrier, node */
+    double cu[2];          /* their instants move by cu t: 1 and −ρ */
+    double kst[2], kap;    /* clamped capacities; κ_A = k_A h_A/Δt */
+} XMove;
+
+typedef struct {
+    double Q, g, ak, P, e, y0;
+} XKink;
+
+static int xm_tau(const XMove *f, int q) { return (f->a0 + q) % f->S->m->T; }
+static double xm_y(const XMove *f, int u, int tau) {
+    Sys *S = f->S;
+    int j = f->j;
+    return AT(S->ye[f->si[u]], f->nd[u], tau);
+}
+/* scale of the end step e (0: step a, 1: step b) */
+static double xm_scale(const XMove *f, int e) { return e == 0 ? -f->kap : f->kap; }
+/* an angle entry c θ as finalize! keeps it (below 1e-14 it is zero) */
+static double xm_th(double c, double th) {
+    double a = c * th;
+    return fabs(a) < 1e-14 ? 0.0 : a;
+}
+/* flow change per unit t of template line l at scale c (0: finalize! drops the line) */
+static double xm_df(const XMove *f, double c, int l) {
+    const Tables *tb = f->tb;
+    int e = tb->sx_ln_line.a[l];
+    double df = f->S->m->line_b[e] * (xm_th(c, tb->sx_ln_ta.a[l]) - xm_th(c, tb->sx_ln_tb.a[l]));
+    return fabs(df) <= 1e-14 ? 0.0 : df;
+}
+/* the box interval of an instant y + c t (sx_boxint :231) */
+static void xm_box(double y, double c, double *lo, double *hi) {
+    if (c > 0) { *lo = -y / c; *hi = (1.0 - y) / c; }
+    else { *lo = (1.0 - y) / c; *hi = -y / c; }
+}
+/* unit u at interior step q keeps both its instants in [0, 1] at t (its terms do not change) */
+static int xm_inside(const XMove *f, int u, int q, double t) {
+    int T = f->S->m->T, tau = xm_tau(f, q), tau1 = tau + 1 == T ? 0 : tau + 1;
+    double y = xm_y(f, u, tau), y1 = xm_y(f, u, tau1), l0, h0, l1, h1;
+    if (!(0.0 <= y && y <= 1.0 && 0.0 <= y1 && y1 <= 1.0)) return 0;
+    xm_box(y, f->cu[u], &l0, &h0);
+    xm_box(y1, f->cu[u], &l1, &h1);
+    return fmax(l0, l1) <= t && t <= fmin(h0, h1);
+}
+/* prorata! (qadj = 0) of the gas unit of leak kk at end step e: 0 when no kink is made */
+static int xm_kink(const XMove *f, int e, int kk, XKink *k) {
+    Sys *S = f->S;
+    const esm_model *m = S->m;
+    int j = f->j, n = f->tb->sx_kk_node.a[kk], tau = xm_tau(f, e == 0 ? 0 : f->L);
+    double g = xm_scale(f, e) * f->tb->sx_kk_g.a[kk];
+    if (g == 0.0) return 0;
+    double kap = S->kc[ESM_KGAS * m->N + n];
+    if (!(fabs(kap) >= 1.0)) return 0;
+    double w = AT(S->yg, n, tau);
+    double P = 0.0 + kap * w, Qs = 0.0 + kap * clampd(w, 0.0, 1.0);
+    if (fabs(P) < 1e-6) return 0;
+    k->Q = AT(S->r, n, tau) + Qs;
+    k->g = g;
+    k->e = P - fmax(k->Q, 0.0);
+    k->P = P;
+    double sig = kap * w / P;
+    k->ak = sig == 0.0 ? 0.0 : sig / kap;
+    k->y0 = w;
+    return 1;
+}
+static void xm_gen(const XMove *f, int n, double y, double *c, double *p) {
+    const esm_model *m = f->S->m;
+    double k = f->S->kc[ESM_KGAS * m->N + n];
+    gen_cost(m->mc[ESM_KGAS * m->N + n], 1.0, k, k + m->kappa0, m->dt, m->c_viol, y, c, p);
+}
+/* storage term of unit u at step q at t: cost change; d at t (*d1), injection change (*dj) */
+static double xm_stor(const XMove *f, int u, int q, double t, double *d1, double *dj) {
+    const esm_model *m = f->S->m;
+    int T = m->T, tau = xm_tau(f, q), tau1 = tau + 1 == T ? 0 : tau + 1;
+    double y = xm_y(f, u, tau), y1 = xm_y(f, u, tau1), c = f->cu[u];
+    double dye = q >= 1 ? c : 0.0, dye1 = q <= f->L - 1 ? c : 0.0;
+    double w = f->kst[u] + m->kappa0, d0, j0, j1;
+    double c0 = soc_cost(m, f->si[u], f->nd[u], f->kst[u], w, m->dt, m->c_viol, y, y1, &d0, &j0);
+    double c1 = soc_cost(m, f->si[u], f->nd[u], f->kst[u], w, m->dt, m->c_viol, y + t * dye, y1 + t * dye1, d1, &j1);
+    *dj = j1 - j0;
+    return c1 - c0;
+}
+/* end step e at t (eval_cached! order: kinked gas entries, lines, storage A and B, balances of the
+ * template's nodes); d[2] the units' d, bal (may be NULL) the balances */
+static double xm_end(const XMove *f, int e, double t, double *d, double *bal) {
+    Sys *S = f->S;
+    const esm_model *m = S->m;
+    const Tables *tb = f->tb;
+    int j = f->j, sl = f->slot, tau = xm_tau(f, e == 0 ? 0 : f->L);
+    double c = xm_scale(f, e), D = 0.0, dj[2];
+    int k0 = tb->sx_kk_off.a[sl], k1 = tb->sx_kk_off.a[sl + 1], l0 = tb->sx_ln_off.a[sl], l1 = tb->sx_ln_off.a[sl + 1];
+    int n0 = tb->sx_nd_off.a[sl], n1 = tb->sx_nd_off.a[sl + 1];
+    for (int kq = k0; kq < k1; kq++) {
+        XKink k;
+        if (!xm_kink(f, e, kq, &k) || k.ak == 0.0) continue;
+        double c0, p0, c1, p1;
+        int n = tb->sx_kk_node.a[kq];
+        xm_gen(f, n, k.y0, &c0, &p0);
+        xm_gen(f, n, k.y0 + k.ak * sm_phi(k.Q, k.g, t), &c1, &p1);
+        D += c1 - c0;
+    }
+    for (int l = l0; l < l1; l++) {
+        double df = xm_df(f, c, l);
+        if (df == 0.0) continue;
+        int ln = tb->sx_ln_line.a[l];
+        double f0 = AT(S->F, ln, tau), se = m->line_s[ln], f1 = f0 + t * df;
+        D += m->c_viol * m->dt * (fmax(fabs(f1) - se, 0.0) - fmax(fabs(f0) - se, 0.0));
+    }
+    for (int u = 0; u < 2; u++) D += xm_stor(f, u, e == 0 ? 0 : f->L, t, &d[u], &dj[u]);
+    for (int p = n0; p < n1; p++) {
+        int n = tb->sx_nd_node.a[p], ik = tb->sx_nd_kk.a[p];
+        double S0 = AT(S->r, n, tau), dr = 0.0;
+        if (ik >= 0) {
+            XKink k;
+            if (xm_kink(f, e, k0 + ik, &k) && k.ak != 0.0) {
+                double c0, p0, c1, p1;
+                xm_gen(f, n, k.y0, &c0, &p0);
+                xm_gen(f, n, k.y0 + k.ak * sm_phi(k.Q, k.g, t), &c1, &p1);
+                dr -= p1 - p0;
+            }
+        }
+        for (int q = tb->sx_nd_l.a[p]; q < tb->sx_nd_l.a[p + 1]; q++) {
+            int pos = l0 + tb->sx_nl_pos.a[q];
+            double df = xm_df(f, c, pos);
+            if (df == 0.0) continue;
+            double f0 = AT(S->F, tb->sx_ln_line.a[pos], tau), dd = (f0 + t * df) - f0;
+            dr += tb->sx_nl_sgn.a[q] > 0 ? dd : -dd;
+        }
+        if (n == f->nd[0]) dr += dj[0];
+        if (n == f->nd[1]) dr += dj[1];
+        double S1 = S0 + dr;
+        D += m->dt * (phibal(m, S1) - phibal(m, S0));
+        if (bal) bal[p - n0] = S1;
+    }
+    return D;
+}
+/* interior step q at t: ΔC; d[2] (the value at t = 0 for a unit inside its box), the balances of
+ * the units' nodes bal[2] (one when they share a node); *moved 0 when both units are inside */
+static double xm_int(const XMove *f, int q, double t, double *d, double *bal, int *moved) {
+    Sys *S = f->S;
+    const esm_model *m = S->m;
+    int j = f->j, T = m->T, tau = xm_tau(f, q), tau1 = tau + 1 == T ? 0 : tau + 1;
+    double D = 0.0, dj[2] = {0.0, 0.0};
+    int mv = 0;
+    for (int u = 0; u < 2; u++) {
+        if (xm_inside(f, u, q, t)) {
+            double y = xm_y(f, u, tau), y1 = xm_y(f, u, tau1), j0;
+            soc_cost(m, f->si[u], f->nd[u], f->kst[u], f->kst[u] + m->kappa0, m->dt, m->c_viol, y, y1, &d[u], &j0);
+            continue;
+        }
+        D += xm_stor(f, u, q, t, &d[u], &dj[u]);
+        mv = 1;
+    }
+    if (f->nd[0] == f->nd[1]) {
+        double S0 = AT(S->r, f->nd[0], tau), S1 = S0 + (dj[0] + dj[1]);
+        D += m->dt * (phibal(m, S1) - phibal(m, S0));
+        bal[0] = bal[1] = S1;
+    } else {
+        for (int u = 0; u < 2; u++) {
+            double S0 = AT(S->r, f->nd[u], tau), S1 = S0 + dj[u];
+            D += m->dt * (phibal(m, S1) - phibal(m, S0));
+            bal[u] = S1;
+        }
+    }
+    *moved = mv;
+    return D;
+}
+static void xm_butterfly(double *part, int W) {
+    for (int off = W / 2; off > 0; off >>= 1) {
+        double nx[64];
+        for (int l = 0; l < W; l++) nx[l] = part[l] + part[l ^ off];
+        memcpy(part, nx, sizeof(double) * W);
+    }
+}
+static double xm_eval(const XMove *f, double t) {
+    double part[64] = {0}, d[2], bal[2];
+    g_nev++;
+    for (int q = 0; q <= f->L; q++) {
+        double x;
+        if (q == 0 || q == f->L) {
+            x = xm_end(f, q == 0 ? 0 : 1, t, d, NULL);
+        } else {
+            int mv;
+            x = xm_int(f, q, t, d, bal, &mv);
+            if (!mv) continue;
+        }
+        part[q % f->W] += x;
+    }
+    xm_butterfly(part, f->W);
+    return part[0];
+}
+static int xm_next_bp(const XMove *f, double tp, double sgn, double *t) {
+    Sys *S = f->S;
+    const esm_model *m = S->m;
+    const Tables *tb = f->tb;
+    int j = f->j, sl = f->slot, have = 0;
+#define XCAND(bv)                                                                    \
+    do {                                                                             \
+        double b_ = (bv);                                                            \
+        if (isfinite(b_) && sgn * (b_ - tp) > 0 && (!have || sgn * (*t - b_) > 0)) { \
+            *t = b_;                                                                 \
+            have = 1;                                                                \
+        }                                                                            \
+    } while (0)
+    for (int e = 0; e < 2; e++) {
+        int tau = xm_tau(f, e == 0 ? 0 : f->L);
+        double c = xm_scale(f, e);
+        for (int kq = tb->sx_kk_off.a[sl]; kq < tb->sx_kk_off.a[sl + 1]; kq++) {
+            XKink k;
+            if (!xm_kink(f, e, kq, &k)) continue;
+            XCAND(-k.Q / k.g);
+            if (k.ak != 0.0) {
+                double s1 = k.ak * k.g, off = k.ak * k.Q;
+                XCAND((0.0 - k.y0 - 0.0) / s1);
+                XCAND((1.0 - k.y0 - 0.0) / s1);
+                XCAND((0.0 - k.y0 - off) / s1);
+                XCAND((1.0 - k.y0 - off) / s1);
+            }
+        }
+        for (int l = tb->sx_ln_off.a[sl]; l < tb->sx_ln_off.a[sl + 1]; l++) {
+            double df = xm_df(f, c, l);
+            if (df == 0.0) continue;
+            int ln = tb->sx_ln_line.a[l];
+            double f0 = AT(S->F, ln, tau), se = m->line_s[ln];
+            XCAND((se - f0) / df);
+            XCAND((-se - f0) / df);
+        }
+    }
+    for (int q = 1; q <= f->L; q++)
+        for (int u = 0; u < 2; u++) {
+            double y = xm_y(f, u, xm_tau(f, q));
+            XCAND((0.0 - y) / f->cu[u]);
+            XCAND((1.0 - y) / f->cu[u]);
+        }
+#undef XCAND
+    return have;
+}
+/* the storage secondaries of unit u (storage_mask!: 3 per storage term) */
+static double xm_sec(const XMove *f, int u, int k, double d) {
+    const esm_model *m = f->S->m;
+    int us = f->si[u] * m->N + f->nd[u];
+    if (k == 0) return d;
+    if (k == 1) return m->stor_eta_in[us] > 0 ? d - m->stor_eta_in[us] : 1.0;
+    return -d * m->stor_eta_out[us] - 1.0;
+}
+/* the nearest sign change beyond x, strictly inside (a, b), of the storage secondaries (stor) or
+ * of the balances, from their values at a and b (linear in between) */
+static int xm_next_root(const XMove *f, int stor, double a, double b, double x, double sgn, double *xr) {
+    int have = 0, sl = f->slot, nn = f->tb->sx_nd_off.a[sl + 1] - f->tb->sx_nd_off.a[sl];
+    double da[2], db[2], *ba = (double *)malloc(sizeof(double) * (nn + 2)), *bb = (double *)malloc(sizeof(double) * (nn + 2));
+#define XTEST(A_, B_)                                                                                      \
+    do {                                                                                                   \
+        double A = (A_), B = (B_);                                                                         \
+        if ((A < 0) != (B < 0) && A != B) {                                                                \
+            double r_ = a + (b - a) * A / (A - B);                                                         \
+            if (sgn * (r_ - a) > 0 && sgn * (b - r_) > 0 && sgn * (r_ - x) > 0 && (!have || sgn * (*xr - r_) > 0)) { \
+                *xr = r_;                                                                                  \
+                have = 1;                                                                                  \
+            }                                                                                              \
+        }                                                                                                  \
+    } while (0)
+    for (int e = 0; e < 2; e++) {
+        xm_end(f, e, a, da, ba);
+        xm_end(f, e, b, db, bb);
+        if (stor) {
+            for (int u = 0; u < 2; u++)
+                for (int k = 0; k < 3; k++) XTEST(xm_sec(f, u, k, da[u]), xm_sec(f, u, k, db[u]));
+        } else {
+            for (int p = 0; p < nn; p++) XTEST(ba[p], bb[p]);
+        }
+    }
+    for (int q = 1; q < f->L; q++) {
+        if (xm_inside(f, 0, q, a) && xm_inside(f, 0, q, b) && xm_inside(f, 1, q, a) && xm_inside(f, 1, q, b)) continue;
+        int mv;
+        xm_int(f, q, a, da, ba, &mv);
+        xm_int(f, q, b, db, bb, &mv);
+        if (stor) {
+            for (int u = 0; u < 2; u++)
+                for (int k = 0; k < 3; k++) XTEST(xm_sec(f, u, k, da[u]), xm_sec(f, u, k, db[u]));
+        } else {
+            for (int u = 0; u < (f->nd[0] == f->nd[1] ? 1 : 2); u++) XTEST(ba[u], bb[u]);
+        }
+    }
+#undef XTEST
+    free(ba);
+    free(bb);
+    return have;
+}
+static int xm_walk(const XMove *f, double beta, double sgn, double maxnats, BigPts *P) {
+    double cmin = 0.0, tp = 0.0, delta = 0.0, cprev = 0.0;
+    int nx = 0;
+    while (1) {
+        double t = 0.0;
+        if (!xm_next_bp(f, tp, sgn, &t)) {
+            double hp = 1e-6 * fmax(1.0, fabs(tp));
+            double sl = (beta * xm_eval(f, tp + sgn * hp) - cprev) / hp;
+            if (sl > 0) delta = fmin(2 * maxnats / sl, 1e3 * fmax(fabs(tp), 1e-3));
+            else delta = delta == 0.0 ? fmax(fabs(tp), 1e-6) : 2 * delta;
+            t = tp + sgn * delta;
+            if (++nx > 80) return 0;
+            if (!isfinite(t)) return 0;
+        }
+        double c = beta * xm_eval(f, t);
+        double a0 = tp, xs = tp, x = 0.0;
+        while (xm_next_root(f, 1, tp, t, xs, sgn, &x)) {
+            double cx = beta * xm_eval(f, x);
+            double xb = a0, y = 0.0;
+            while (xm_next_root(f, 0, a0, x, xb, sgn, &y)) {
+                double cy = beta * xm_eval(f, y);
+                bpush(P, y, cy);
+                cmin = fmin(cmin, cy);
+                xb = y;
+            }
+            bpush(P, x, cx);
+            cmin = fmin(cmin, cx);
+            a0 = x;
+            xs = x;
+        }
+        double xb = a0, y = 0.0;
+        while (xm_next_root(f, 0, a0, t, xb, sgn, &y)) {
+            double cy = beta * xm_eval(f, y);
+            bpush(P, y, cy);
+            cmin = fmin(cmin, cy);
+            xb = y;
+        }
+        bpush(P, t, c);
+        if (P->full) return -1;
+        cmin = fmin(cmin, c);
+        cprev = c;
+        if (c - cmin > maxnats) return 1;
+        tp = t;
+    }
+}
+static double xm_sample(const XMove *f, double beta, double maxnats, double useg, double uoff, int *status) {
+    static __thread double bufs[4][SMS_PMAX];
+    BigPts PR = {bufs[0], bufs[1], 0, 0}, PL = {bufs[2], bufs[3], 0, 0};
+    int w = xm_walk(f, beta, 1.0, maxnats, &PR);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    w = xm_walk(f, beta, -1.0, maxnats, &PL);
+    if (w != 1) { *status = w == 0 ? 1 : 2; return 0.0; }
+    return draw_points_big(&PL, &PR, useg, uoff, status);
+}
+/* units of the node with |κ| ≥ 1 among its dispatchable units (this system: the gas unit) */
+static int gas_units(const Sys *S, int n) { return fabs(S->kc[ESM_KGAS * S->m->N + n]) >= 1.0 ? 1 : 0; }
+/* logjac (src/local.jl:1874): Σ (|S| − 1) log|P(t)/P| over the kinks made, |S| from the model */
+static double xm_logjac(const XMove *f, double t, int *any) {
+    double part[64] = {0};
+    int cnt = 0, sl = f->slot;
+    for (int e = 0; e < 2; e++)
+        for (int kq = f->tb->sx_kk_off.a[sl]; kq < f->tb->sx_kk_off.a[sl + 1]; kq++) {
+            XKink k;
+            if (!xm_kink(f, e, kq, &k)) continue;
+            cnt++;
+            double nm = (double)(gas_units(f->S, f->tb->sx_kk_node.a[kq]) - 1);
+            part[(e == 0 ? 0 : f->L) % f->W] += nm * log(fabs((fmax(k.Q + k.g * t, 0.0) + k.e) / k.P));
+        }
+    *any = cnt > 0;
+    xm_butterfly(part, f->W);
+    return part[0];
+}
+/* commit_sxlong! / commit_block!: every new value from the state before the move, then the writes */
+static void xm_commit(const XMove *f, double t) {
+    if (t == 0.0) return;
+    Sys *S = f->S;
+    const esm_model *m = S->m;
+    const Tables *tb = f->tb;
+    int j = f->j, sl = f->slot, n0 = tb->sx_nd_off.a[sl], nn = tb->sx_nd_off.a[sl + 1] - n0;
+    int k0 = tb->sx_kk_off.a[sl], nk = tb->sx_kk_off.a[sl + 1] - k0, L = f->L;
+    double *bal = (double *)malloc(sizeof(double) * 2 * (nn + 1)), *gy = (double *)malloc(sizeof(double) * 2 * (nk + 1));
+    int *gw = (int *)calloc(2 * (nk + 1), sizeof(int));
+    double *ir = (double *)malloc(sizeof(double) * 2 * (L + 1));
+    int *imv = (int *)calloc(L + 1, sizeof(int));
+    double d[2];
+    for (int e = 0; e < 2; e++) {
+        xm_end(f, e, t, d, bal + e * nn);
+        for (int q = 0; q < nk; q++) {
+            XKink k;
+            if (xm_kink(f, e, k0 + q, &k) && k.ak != 0.0) {
+                gy[e * nk + q] = k.y0 + k.ak * sm_phi(k.Q, k.g, t);
+                gw[e * nk + q] = 1;
+            }
+        }
+    }
+    for (int q = 1; q < L; q++) xm_int(f, q, t, d, ir + 2 * q, &imv[q]);
+    for (int e = 0; e < 2; e++) {
+        int tau = xm_tau(f, e == 0 ? 0 : L);
+        double c = xm_scale(f, e);
+        for (int p = 0; p < nn; p++) AT(S->r, tb->sx_nd_node.a[n0 + p], tau) = bal[e * nn + p];
+        for (int q = 0; q < nk; q++)
+            if (gw[e * nk + q]) AT(S->yg, tb->sx_kk_node.a[k0 + q], tau) = gy[e * nk + q];
+        for (int l = tb->sx_ln_off.a[sl]; l < tb->sx_ln_off.a[sl + 1]; l++) {
+            double df = xm_df(f, c, l);
+            if (df != 0.0) AT(S->F, tb->sx_ln_line.a[l], tau) += t * df;
+        }
+        for (int q = tb->sx_th_off.a[sl]; q < tb->sx_th_off.a[sl + 1]; q++) {
+            double a = xm_th(c, tb->sx_th_val.a[q]);
+            if (a != 0.0) AT(S->th, tb->sx_th_node.a[q], tau) += t * a;
+        }
+    }
+    for (int q = 1; q < L; q++) {
+        if (!imv[q]) continue;
+        int tau = xm_tau(f, q);
+        AT(S->r, f->nd[0], tau) = ir[2 * q];
+        if (f->nd[1] != f->nd[0]) AT(S->r, f->nd[1], tau) = ir[2 * q + 1];
+    }
+    for (int q = 1; q <= L; q++) {
+        int tau = xm_tau(f, q);
+        for (int u = 0; u < 2; u++) AT(S->ye[f->si[u]], f->nd[u], tau) += f->cu[u] * t;
+    }
+    free(bal); free(gy); free(gw); free(ir); free(imv);
+}
+/* a pair move of replica j: units A = (sa, na) and B = (sb, nb) at slot `slot` of A's node,
+ * instants a+1 … a+L (a 0-based); 0 when it is not made (κ_A or κ_B below 1) */
+static int xm_prepare(XMove *f, Sys *S, const Tables *tb, int j, int sa, int na, int sb, int nb, int slot, int a, int L,
+                      int W) {
+    const esm_model *m = S->m;
+    int N = m->N;
+    memset(f, 0, sizeof(*f));
+    f->S = S; f->tb = tb; f->j = j; f->a0 = a; f->L = L; f->W = W; f->slot = slot;
+    f->si[0] = sa; f->nd[0] = na; f->si[1] = sb; f->nd[1] = nb;
+    f->kst[0] = S->kc[(ESM_KSTOR + sa) * N + na];
+    f->kst[1] = S->kc[(ESM_KSTOR + sb) * N + nb];
+    double EA = f->kst[0] * m->stor_hours[sa * N + na], EB = f->kst[1] * m->stor_hours[sb * N + nb];
+    f->kap = EA / m->dt;
+    if (f->kap < 1.0 || EB / m->dt < 1.0) return 0;
+    f->cu[0] = 1.0;
+    f->cu[1] = -(EA / EB);
+    return 1;
+}
+static double xm_move(XMove *f, double useg, double uoff, double umh, double maxnats, int64_t *stats) {
+    int status;
+    g_nev = 0;
+    double t = xm_sample(f, f->S->m->beta_v, maxnats, useg, uoff, &status);
+    if (t != 0.0) {
+        int any;
+        double lj = xm_logjac(f, t, &any);
+        if (any && !(umh < exp(fmin(0.0, lj)))) { t = 0.0; status = 2; }
+    }
+    long nev = g_nev;
+    xm_commit(f, t);
+    if (stats) stat_add(stats, status, t, nev);
+    return t;
+}
+/* sx_partner (:62-90): among the units with hours ≥ hmin and κ = k h/Δt ≥ 1 at the slots of A's
+ * node (A itself excluded), one with probability ∝ min(k_A, k_B), from the uniform u; 0: none */
+static int xm_partner(const Sys *S, const Tables *tb, double hmin, int sa, int na, double u, int *slot, int *sb,
+                      int *nb) {
+    const esm_model *m = S->m;
+    int N = m->N, found = 0;
+    double kA = S->kc[(ESM_KSTOR + sa) * N + na], W = 0.0;
+    for (int pass = 0; pass < 2; pass++) {
+        double x = u * W, acc = 0.0;
+        for (int s = tb->sx_poff.a[na]; s < tb->sx_poff.a[na + 1]; s++) {
+            int mm = tb->sx_part.a[s];
+            for (int si = 0; si < ESM_NSTOR; si++) {
+                if (si == sa && mm == na) continue;
+                if (!(m->stor_hours[si * N + mm] >= hmin)) continue;
+                double kB = S->kc[(ESM_KSTOR + si) * N + mm];
+                if (!(kB * m->stor_hours[si * N + mm] / m->dt >= 1.0)) continue;
+                if (pass == 0) {
+                    W += fmin(kA, kB);
+                    continue;
+                }
+                acc += fmin(kA, kB);
+                *slot = s; *sb = si; *nb = mm;
+                found = 1;
+                if (x <= acc) return 1;
+            }
+        }
+        if (pass == 0 && !(W > 0.0)) return 0;
+    }
+    return found;
+}
+/* the storage units of the pairs (LM.sxunits): hours ≥ hmin, node-major in carrier order */
+static int sx_units(const esm_model *m, double hmin, int *us) {
+    int n = 0;
+    for (int i = 0; i < m->N; i++)
+        for (int si = 0; si < ESM_NSTOR; si++)
+            if (m->stor_hours[si * m->N + i] >= hmin) us[n++] = si * m->N + i;
+    return n;
+}
+/* sxpair at step τ of replica j, selection words s (unit A pick(s[0]), partner u01(s[2], s[3])) */
+static void sxpair_move(Sys *S, const Tables *tb, int j, int tau, const uint32_t *s, double hmin, double useg,
+                        double uoff, double umh, double maxnats, int64_t *stats) {
+    const esm_model *m = S->m;
+    int N = m->N, T = m->T, us[2 * 256];
+    int nsx = sx_units(m, hmin, us);
+    if (nsx == 0) return;
+    int ua = us[pick(s[0], (uint32_t)nsx)], sa = ua / N, na = ua % N, slot, sb, nb;
+    if (S->kc[(ESM_KSTOR + sa) * N + na] * m->stor_hours[ua] / m->dt < 1.0) return;
+    if (!xm_partner(S, tb, hmin, sa, na, u01(s[2], s[3]), &slot, &sb, &nb)) return;
+    XMove f;
+    if (!xm_prepare(&f, S, tb, j, sa, na, sb, nb, slot, (tau - 1 + T) % T, 1, 1)) return;
+    xm_move(&f, useg, uoff, umh, maxnats, stats);
+}
+/* the sxlong moves of a sweep (the same for every replica): per unit of the pairs with κ ≥ 1 and
+ * move m (unit index ui among them all) the partner, a and L from (ui, m, 9420, c); shuffled by
+ * Fisher–Yates with (i, 0, 9421, c); rounds of disjoint footprints by first fit (sx_rounds
+ * :470-492), listed round by round. mv[5·i …]: sa, na, sb, nb, slot, then ma[i], mL[i]; returns
+ * the count, round offsets in roff (nround + 1 entries) */
+static int sxlong_moves(const Sys *S, const Tables *tb, int nmoves, double hmin, const uint32_t key[2], uint32_t ctr,
+                        int *mv, int *ma, int *mL, int *roff, int *nround) {
+    const esm_model *m = S->m;
+    int N = m->N, T = m->T, us[2 * 256], cnt = 0;
+    int nsx = sx_units(m, hmin, us);
+    for (int ui = 0; ui < nsx; ui++) {
+        int sa = us[ui] / N, na = us[ui] % N;
+        if (S->kc[(ESM_KSTOR + sa) * N + na] * m->stor_hours[us[ui]] / m->dt < 1.0) continue;
+        for (int mm = 0; mm < nmoves; mm++) {
+            uint32_t c4[4] = {(uint32_t)ui, (uint32_t)mm, 9420u, ctr}, w[4];
+            orc_philox(key, c4, w);
+            int slot, sb, nb;
+            if (!xm_partner(S, tb, hmin, sa, na, u01(w[0], w[1]), &slot, &sb, &nb)) continue;
+            int *o = mv + 5 * cnt;
+            o[0] = sa; o[1] = na; o[2] = sb; o[3] = nb; o[4] = slot;
+            ma[cnt] = (int)pick(w[2], (uint32_t)T);
+            mL[cnt] = 1 + (int)pick(w[3], (uint32_t)(T - 1));
+            cnt++;
+        }
+    }
+    for (int i = cnt - 1; i >= 1; i--) {
+        uint32_t c4[4] = {(uint32_t)i, 0u, 9421u, ctr}, w[4];
+        orc_philox(key, c4, w);
+        int jx = (int)pick(w[0], (uint32_t)(i + 1)), tmp[7];
+        memcpy(tmp, mv + 5 * i, sizeof(int) * 5); tmp[5] = ma[i]; tmp[6] = mL[i];
+        memcpy(mv + 5 * i, mv + 5 * jx, sizeof(int) * 5); ma[i] = ma[jx]; mL[i] = mL[jx];
+        memcpy(mv + 5 * jx, tmp, sizeof(int) * 5); ma[jx] = tmp[5]; mL[jx] = tmp[6];
+    }
+    /* first fit into rounds of disjoint footprints, then the list round by round */
+    int *rd = (int *)malloc(sizeof(int) * (cnt + 1)), nr = 0;
+    unsigned char *mark = (unsigned char *)calloc((size_t)(cnt + 1) * N, 1);
+    for (int i = 0; i < cnt; i++) {
+        int sl = mv[5 * i + 4], f0 = tb->sx_ft_off.a[sl], f1 = tb->sx_ft_off.a[sl + 1], r = 0;
+        for (; r < nr; r++) {
+            int clash = 0;
+            for (int q = f0; q < f1 && !clash; q++) clash = mark[(size_t)r * N + tb->sx_ft_node.a[q]];
+            if (!clash) break;
+        }
+        if (r == nr) nr++;
+        for (int q = f0; q < f1; q++) mark[(size_t)r * N + tb->sx_ft_node.a[q]] = 1;
+        rd[i] = r;
+    }
+    int *mv2 = (int *)malloc(sizeof(int) * 5 * (cnt + 1)), *ma2 = (int *)malloc(sizeof(int) * (cnt + 1)),
+        *mL2 = (int *)malloc(sizeof(int) * (cnt + 1)), k = 0;
+    roff[0] = 0;
+    for (int r = 0; r < nr; r++) {
+        for (int i = 0; i < cnt; i++)
+            if (rd[i] == r) {
+                memcpy(mv2 + 5 * k, mv + 5 * i, sizeof(int) * 5);
+                ma2[k] = ma[i]; mL2[k] = mL[i];
+                k++;
+            }
+        roff[r + 1] = k;
+    }
+    memcpy(mv, mv2, sizeof(int) * 5 * cnt);
+    memcpy(ma, ma2, sizeof(int) * cnt);
+    memcpy(mL, mL2, sizeof(int) * cnt);
+    *nround = nr;
+    free(rd); free(mark); free(mv2); free(ma2); free(mL2);
+    return cnt;
+}
+/* the sxlong phase of the long-range storage sweep for replica j: move i of the list (round by
+ * round) with its line draw (replica, i, 0x5C000000, c) and Jacobian test (replica, i,
+ * 0x7C000000, c) */
+static void sxlong_sweep(Sys *S, const Tables *tb, int j, int nmoves, double hmin, const uint32_t key[2], uint32_t ctr,
+                         double maxnats, int64_t *st) {
+    const esm_model *m = S->m;
+    int cap = 2 * m->N * (nmoves > 0 ? nmoves : 1) + 1;
+    int *mv = (int *)malloc(sizeof(int) * 5 * cap), *ma = (int *)malloc(sizeof(int) * cap),
+        *mL = (int *)malloc(sizeof(int) * cap), *roff = (int *)malloc(sizeof(int) * (cap + 1)), nr;
+    int cnt = m->T >= 2 && nmoves > 0 ? sxlong_moves(S, tb, nmoves, hmin, key, ctr, mv, ma, mL, roff, &nr) : 0;
+    for (int i = 0; i < cnt; i++) {
+        const int *o = mv + 5 * i;
+        XMove f;
+        if (!xm_prepare(&f, S, tb, j, o[0], o[1], o[2], o[3], o[4], ma[i], mL[i], 64)) continue;
+        uint32_t cl[4] = {(uint32_t)j, (uint32_t)i, 0x5C000000u, ctr}, wl[4];
+        orc_philox(key, cl, wl);
+        uint32_t cm[4] = {(uint32_t)j, (uint32_t)i, 0x7C000000u, ctr}, wm[4];
+        orc_philox(key, cm, wm);
+        xm_move(&f, u01(wl[0], wl[1]), u01(wl[2], wl[3]), u01(wm[0], wm[1]), maxnats, st);
+    }
+    free(mv); free(ma); free(mL); free(roff);
+}
+
+/* ---- explicit pair moves, for the pins ---- */
+static int xm_args_ok(const esm_model *m, int sa, int na, int sb, int nb, int a, int L) {
+    return m->N <= 256 && m->L <= 2048 && sa >= 0 && sa < ESM_NSTOR && sb >= 0 && sb < ESM_NSTOR && na >= 0 &&
+           na < m->N && nb >= 0 && nb < m->N && a >= 0 && a < m->T && L >= 1 && L <= m->T - 1;
+}
+/* the slot of partner node nb among node na's slots (-1: none) */
+static int xm_slot(const Tables *tb, int na, int nb) {
+    for (int s = tb->sx_poff.a[na]; s < tb->sx_poff.a[na + 1]; s++)
+        if (tb->sx_part.a[s] == nb) return s;
+    return -1;
+}
+
+int orc_sx_slot(const esm_model *m, int n, int k, int *part, int *nth, int *thn, double *thv, int *nlk, int *lkn,
+                double *lkg, int *nft, int *ft) {
+    Tables tb;
+    if (m->N > 256 || n < 0 || n >= m->N || build_tables(m, &tb)) return -1;
+    int s = tb.sx_poff.a[n] + k, rc = 0;
+    if (k < 0 || s >= tb.sx_poff.a[n + 1]) rc = -2;
+    if (!rc) {
+        *part = tb.sx_part.a[s];
+        *nth = tb.sx_th_off.a[s + 1] - tb.sx_th_off.a[s];
+        for (int q = 0; q < *nth; q++) {
+            thn[q] = tb.sx_th_node.a[tb.sx_th_off.a[s] + q];
+            thv[q] = tb.sx_th_val.a[tb.sx_th_off.a[s] + q];
+        }
+        *nlk = tb.sx_kk_off.a[s + 1] - tb.sx_kk_off.a[s];
+        for (int q = 0; q < *nlk; q++) {
+            lkn[q] = tb.sx_kk_node.a[tb.sx_kk_off.a[s] + q];
+            lkg[q] = tb.sx_kk_g.a[tb.sx_kk_off.a[s] + q];
+        }
+        *nft = tb.sx_ft_off.a[s + 1] - tb.sx_ft_off.a[s];
+        for (int q = 0; q < *nft; q++) ft[q] = tb.sx_ft_node.a[tb.sx_ft_off.a[s] + q];
+    }
+    tables_free(&tb);
+    return rc;
+}
+
+int orc_sx_partner(const esm_model *m, const double *k, double hmin, int sa, int na, double u, int *sb, int *nb) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    Tables tb;
+    if (m->N > 256 || m->L > 2048 || build_tables(m, &tb)) return -1;
+    (void)ptr; (void)lines;
+    S.m = m;
+    S.R = 1;
+    for (int u = 0; u < ESM_NK * m->N; u++) S.kc[u] = clampd(k[u], m->klo[u], m->khi[u]);
+    int slot;
+    int rc = xm_partner(&S, &tb, hmin, sa, na, u, &slot, sb, nb);
+    tables_free(&tb);
+    return rc;
+}
+
+/* op: 0 ΔC at t[0 … nt−1] (into out), 1 draws with useg/uoff (t into out, status), 2 commit t[0] */
+int orc_sx_move(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int j, int sa, int na,
+                int sb, int nb, int a, int L, int W, int op, const double *t, int nt, double maxnats, const double *useg,
+                const double *uoff, double *out, int *status) {
+    static __thread Sys S;
+    int ptr[257], lines[2 * 2048];
+    Tables tb;
+    if (!xm_args_ok(m, sa, na, sb, nb, a, L) || (W != 1 && W != 64) || build_tables(m, &tb)) return -1;
+    int slot = xm_slot(&tb, na, nb), rc = 0;
+    sys_view(&S, m, R, v, k, r, F, ptr, lines);
+    XMove f;
+    if (slot < 0 || (sa == sb && na == nb)) rc = -2;
+    else if (!xm_prepare(&f, &S, &tb, j, sa, na, sb, nb, slot, a, L, W)) rc = -3;
+    if (!rc) {
+        if (op == 0) for (int q = 0; q < nt; q++) out[q] = xm_eval(&f, t[q]);
+        else if (op == 1) for (int q = 0; q < nt; q++) out[q] = xm_sample(&f, m->beta_v, maxnats, useg[q], uoff[q], &status[q]);
+        else xm_commit(&f, t[0]);
+    }
+    tables_free(&tb);
+    return rc;
+}
+
 /* the view of a replica set for a sweep: state pointers, clamped capacities, incident lines */
 static void sys_setup(Sys *S0, const esm_model *m, int R, double *v, const double *k, double *r, double *F) {
     int N = m->N, L = m->L;
@@ -2618,7 +3499,8 @@ static void sys_free(Sys *S0) {
 
 int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r, double *F, int nsweeps,
                const uint32_t key[2], uint32_t ctr0, double maxnats, double extra, const double *weights, int atoms,
-               double pms, double plong, int long_moves, double long_hours, int64_t *stats, int nthreads) {
+               double pms, double plong, int long_moves, double long_hours, double pxpair, int xlong_moves,
+               int64_t *stats, int nthreads) {
     int N = m->N, L = m->L, T = m->T;
     if (N > 256) return -1;
     SmsTables smst;
@@ -2639,8 +3521,11 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
         ncls = 3; cfirst[0] = 0; ccount[0] = (T - 1) / 2; cfirst[1] = 1; ccount[1] = (T - 1) / 2;
         cfirst[2] = T - 1; ccount[2] = 1;
     }
-    int *counts = (int *)malloc(sizeof(int) * ESM_NWEIGHT * T * (nsweeps > 0 ? nsweeps : 1));
-    for (int sw = 0; sw < nsweeps; sw++) move_counts(m, extra, wts, key, ctr0 + (uint32_t)sw, counts + (size_t)sw * ESM_NWEIGHT * T);
+    int usx[2 * 256];
+    double px = pxpair > 0 && sx_units(m, long_hours, usx) > 0 ? pxpair : 0.0;
+    int *counts = (int *)malloc(sizeof(int) * NW_ALL * T * (nsweeps > 0 ? nsweeps : 1));
+    for (int sw = 0; sw < nsweeps; sw++)
+        move_counts(m, extra, wts, px, key, ctr0 + (uint32_t)sw, counts + (size_t)sw * NW_ALL * T);
     int64_t st[ESM_NFAM * ESM_NSTAT];
     memset(st, 0, sizeof(st));
     if (nthreads < 1) nthreads = 1;
@@ -2668,14 +3553,16 @@ int orc_sweeps(const esm_model *m, int R, double *v, const double *k, double *r,
                                         st + 5 * ESM_FAM_SOC);
                             }
             }
-            const int *cnt = counts + (size_t)sw * ESM_NWEIGHT * T;
+            const int *cnt = counts + (size_t)sw * NW_ALL * T;
             for (int c = 0; c < ncls; c++)
                 for (int q = 0; q < ccount[c]; q++)
-                    random_step(&S, &tb, j, cfirst[c] + 2 * q, cnt, wts, key, ctr, maxnats, st);
+                    random_step(&S, &tb, j, cfirst[c] + 2 * q, cnt, wts, px, long_hours, key, ctr, maxnats, st);
             if (pms > 0) sms_sweep(&S, &smst, j, pms, key, ctr, maxnats, st + 5 * ESM_FAM_SMS);
-            if (plong > 0)
+            if (plong > 0 &&
                 slg_sweep(&S, j, plong, long_moves, long_hours, key, ctr, maxnats, st + 5 * ESM_FAM_SLONG,
-                          st + 5 * ESM_FAM_SLEVEL);
+                          st + 5 * ESM_FAM_SLEVEL) &&
+                xlong_moves > 0)
+                sxlong_sweep(&S, &tb, j, xlong_moves, long_hours, key, ctr, maxnats, st + 5 * ESM_FAM_SXLONG);
         }
     }
     if (stats) memcpy(stats, st, sizeof(st));

@@ -19,7 +19,7 @@ class FakeBatch:
 
     def sweeps(self, nsweeps, keys, counters, maxnats):
         FakeBatch.calls.append((list(keys), list(counters), maxnats))
-        return np.stack([np.full((13, 5), 100 * c + counters[c], dtype=np.int64) for c in range(len(self.devs))])
+        return np.stack([np.full((15, 5), 100 * c + counters[c], dtype=np.int64) for c in range(len(self.devs))])
 
     def close(self):
         pass

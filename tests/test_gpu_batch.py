@@ -74,9 +74,10 @@ def test_batch_graph_matches_eager_sweeps():
 
 def test_batch_graph_with_storage_sweeps_is_instantiated_once():
     """Ten replays of two sweeps each with both storage sweeps on at probability 1/2 (whether each
-    runs, its block lengths, their order, offsets and classes are new draws every sweep): one
-    instantiation, nine in-place updates, and every replay bitwise the eager sweeps' states,
-    caches and statistics."""
+    runs, its block lengths, their order, offsets and classes are new draws every sweep) and the
+    storage pairs on (the sxlong list and its rounds are drawn on the device): one instantiation,
+    nine in-place updates, and every replay bitwise the eager sweeps' states, caches and
+    statistics."""
     print("batch graph, storage sweeps at p = 1/2, 10 replays", flush=True)
     sets = [_make(*sh, seed=41 + i) for i, sh in enumerate(SHAPES)]
     batched = [p[0] for p in sets]
@@ -84,9 +85,10 @@ def test_batch_graph_with_storage_sweeps_is_instantiated_once():
     for d in batched + eager:
         api.sweep_options(d, p_storage_ms=0.5)
         api.storage_long_options(d, p=0.5, nmoves=8, hmin=1.0)
+        api.storage_cross_options(d, p_xpair=0.1, nxlong=4)
     b = api.Batch(batched)
     keys = [(13, 4), (15, 6), (17, 8)]
-    ran = np.zeros(2, dtype=np.int64)
+    ran = np.zeros(4, dtype=np.int64)
     for rep in range(10):
         counters = [100 * rep + 7 * i for i in range(3)]
         st = b.sweeps(2, keys=keys, counters=counters)
@@ -94,10 +96,10 @@ def test_batch_graph_with_storage_sweeps_is_instantiated_once():
             st_e = api.operation_sweep(d, 2, key=keys[i], counter=counters[i])
             assert np.array_equal(st[i], st_e), f"replay {rep}, set {i}: statistics"
             assert _same(batched[i], d), f"replay {rep}, set {i}: states or caches"
-        ran += [st[1][api.FAMILY_NAMES.index("storage_ms"), 0] > 0, st[1][api.FAMILY_NAMES.index("slong"), 0] > 0]
+        ran += [st[1][api.FAMILY_NAMES.index(f), 0] > 0 for f in ("storage_ms", "slong", "sxpair", "sxlong")]
     launches, inst, upd = b.counts()
     assert (launches, inst, upd) == (10, 1, 9), (launches, inst, upd)
-    # both storage sweeps ran in some replays of set 1
+    # both storage sweeps and both pair families ran in some replays of set 1
     assert ran.min() > 0, ran
     b.close()
     for d in batched + eager:

@@ -2,8 +2,9 @@
 
 On the seeded 3-node network at fixed capacities and weather, the GPU's operation sweep as the
 replica sampler runs it — atoms in conflict-free rounds, the random families in a fixed order
-with the move selection shared by the replicas, the storage multiscale sweep with probability
-1/2, the long-range storage sweep, and (FP32 run) one learned move of every (node, step,
+with the move selection shared by the replicas and the cross-node storage pairs sxpair (weight
+0.1), the storage multiscale sweep with probability 1/2, the long-range storage sweep with 8
+interval storage pairs sxlong per H2 unit, and (FP32 run) one learned move of every (node, step,
 replica) after every sweep, its network trained on the chain's own states — must
 reproduce the equilibrium of the reference's own schedule (the oracle's ``ref_sweeps``: atoms
 plus the plain families, per replica and step one shuffled list, FP64): the means of op cost,
@@ -31,6 +32,7 @@ def test_gpu_kernel_keeps_the_reference_target(prec, learned):
     print("  GPU chain", flush=True)
     dev = api.Replicas(s, R, prec)
     dev.upload(v=v, k=s.k0, x=x)
+    api.storage_cross_options(dev, p_xpair=0.1, nxlong=8)   # the reference configuration's storage pairs
     key = (23, 1 + int(learned))
     api.operation_sweep(dev, 200, key=key, counter=0)
     net = None

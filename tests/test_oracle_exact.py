@@ -1,8 +1,10 @@
 """The build's sweep schedule keeps the reference's target (CPU, oracle only).
 
 The oracle's operation sweep restates the schedule of the GPU kernels: atoms in conflict-free
-rounds, the random families in a fixed order with the selection shared by all replicas, the
-storage multiscale sweep with probability 1/2 and the long-range storage sweep. Its chain is
+rounds, the random families in a fixed order with the selection shared by all replicas (with the
+cross-node storage pairs sxpair at the reference configuration's weight 0.1), the storage
+multiscale sweep with probability 1/2 and the long-range storage sweep (with 8 interval storage
+pairs sxlong per H2 unit). Its chain is
 compared with the reference's own schedule (``Oracle.ref_sweeps``: local_sweep! with the plain
 families, per replica and step one shuffled list) on the 3-node network: the means of op, shed,
 mean state of charge and a line flow agree, |z| < 4 (test/test_exact.jl:71-106). A kernel that
@@ -16,10 +18,11 @@ import oracle
 
 def _build_chain(o, s, v, x, burn, nsweeps, key=(22, 1), threads=8):
     k = s.k0
-    v, r, F, _ = o.sweeps(v, k, burn, key=key, counter=0, x=x, threads=threads)
+    kw = dict(p_xpair=0.1, xlong_moves=8)   # the storage pairs of the reference configuration
+    v, r, F, _ = o.sweeps(v, k, burn, key=key, counter=0, x=x, threads=threads, **kw)
     tr = []
     for q in range(nsweeps):
-        v, r, F, _ = o.sweeps(v, k, 1, key=key, counter=burn + q, x=x, r=r, F=F, threads=threads)
+        v, r, F, _ = o.sweeps(v, k, 1, key=key, counter=burn + q, x=x, r=r, F=F, threads=threads, **kw)
         tr.append(X.observables(o.energy(v, k, x, want_caches=False)[0], v, F))
     return np.array(tr)
 

@@ -229,7 +229,7 @@ def test_family_mix_of_sweeps():
     assert abs(st[9, 0] / R - mean) < 5 * math.sqrt(mean)
     # spair: valid where κs ≥ 1 and κ ≥ 1 (all nodes here)
     assert abs(st[3, 0] / R - ndraw * p[8]) < 5 * math.sqrt(ndraw * p[8])
-    assert np.all(st[3:, 1] > 0.3 * st[3:, 0])
+    assert np.all(st[3:13, 1] > 0.3 * st[3:13, 0])
 
 
 def test_sweeps_with_random_families_keep_caches():
@@ -239,7 +239,7 @@ def test_sweeps_with_random_families_keep_caches():
     x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
     v = synth.heuristic_state(s, x)
     o = oracle.Oracle(s)
-    v2, r2, F2, st = o.sweeps(v, s.k0, 2, key=(1, 9), counter=3, x=x, threads=2)
+    v2, r2, F2, st = o.sweeps(v, s.k0, 2, key=(1, 9), counter=3, x=x, threads=2, p_xpair=0.1, xlong_moves=8)
     _, _, r_, F_ = o.energy(v2, s.k0, x)
     assert np.max(np.abs(r2 - r_)) < 1e-6 * np.abs(s.load).max()
     assert np.max(np.abs(F2 - F_)) < 1e-6 * max(np.abs(F_).max(), 1)

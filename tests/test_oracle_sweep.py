@@ -105,7 +105,7 @@ def test_sweeps_keep_caches_and_stats():
     R = 3
     v, k, x = random_state(s, R, seed=12, spread=False)
     o = oracle.Oracle(s)
-    v2, r2, F2, stats = o.sweeps(v, k, 2, key=(3, 4), counter=10, x=x, threads=2)
+    v2, r2, F2, stats = o.sweeps(v, k, 2, key=(3, 4), counter=10, x=x, threads=2, p_xpair=0.1, xlong_moves=8)
     _, _, r_, F_ = o.energy(v2, k, x)
     assert np.max(np.abs(r2 - r_)) < 1e-6 * np.abs(s.load).max()
     assert np.max(np.abs(F2 - F_)) < 1e-6 * np.abs(F_).max()

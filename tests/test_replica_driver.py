@@ -97,10 +97,13 @@ def test_config_keys():
     assert (ctl.plong, ctl.long_moves, ctl.long_hours) == (0.5, 8, 100.0)
     d = RP.ReplicaCtl.from_config(RP.load_config({"init": "heuristic"}))
     assert (d.plong, d.long_moves, d.long_hours) == (1.0, 32, 24.0)     # src/run.jl:27-29
-    with pytest.raises(ValueError, match="sxpair"):
-        RP.load_config({"init": "heuristic", "storage_xpair": 0.1})
-    with pytest.raises(ValueError, match="sxpair"):
-        RP.load_config({"init": "heuristic", "storage_xlong": 8})
+    assert (d.xpair, d.xlong) == (0.1, 8)                               # src/run.jl:30-31
+    c = RP.load_config({"init": "heuristic", "storage_xpair": 0.0, "storage_xlong": 3})
+    assert (RP.ReplicaCtl.from_config(c).xpair, RP.ReplicaCtl.from_config(c).xlong) == (0.0, 3)
+    with pytest.raises(ValueError, match="storage_xlong"):
+        RP.load_config({"init": "heuristic", "storage_xlong": 300})
+    with pytest.raises(ValueError, match="storage_xpair_ball"):
+        RP.load_config({"init": "heuristic", "storage_xpair_ball": 2})
     with pytest.raises(ValueError, match="storage_long"):
         RP.load_config({"init": "heuristic", "storage_long_moves": 300})
     c = RP.load_config({"init": "heuristic", "learned_proposal": True, "learned_block_hours": 12, "learned_components": 4})
