@@ -434,8 +434,7 @@ struct FXPair {
                 XKinkD<S> k;
                 if (!kink(e, kq, k)) continue;
                 cnt++;
-                const int nu = fabs(m->kc[ESM_KGAS * m->N + tb.kk_node[kq]]) >= kmin ? 1 : 0;
-                const double nm = (double)(nu - 1);
+                const double nm = kink_nm(*m, tb.kk_node[kq], kmin);
                 lj += nm * log(fabs((fmax((double)k.Q + (double)k.g * (double)t, 0.0) + (double)k.e) / (double)k.P));
             }
         }

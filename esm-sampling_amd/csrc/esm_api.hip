@@ -230,7 +230,7 @@ static void free_handle(esm_handle *h) {
                   h->d_out, h->d_sk, h->d_fk, h->d_z, h->d_ytmp, h->d_carry, h->d_zin, h->d_mu, h->d_xbar, h->d_m,
                   h->d_mnew, h->d_hod, h->d_moved, h->d_harm, h->d_wpar, h->d_Ptr, h->d_pi0, h->d_theta_nodes, h->d_stats,
                   h->d_counts, h->dmt.ibuf, h->dmt.vbuf, h->dmt.qcum, h->d_mu64, h->d_ffbs, h->d_ffbs_mask, h->d_sched,
-                  h->d_sxu, h->d_sxm};
+                  h->d_sxu, h->d_sxm, h->d_feat};
     for (void *p : ps)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -989,6 +989,17 @@ int32_t esm_weather_ffbs(esm_handle *h, const int32_t *mask, const uint32_t key[
     if (rc) return rc;
     ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
     return ESM_OK;
+}
+
+int32_t esm_replica_features(esm_handle *h, const double *kref, double *out) {
+    if (!h || !kref || !out) { set_error("esm_replica_features: null argument"); return ESM_EINVAL; }
+    if (!h->weather_ready) {
+        set_error("esm_replica_features: no weather drawn or uploaded (esm_weather_init / esm_upload_weather)");
+        return ESM_ESTATE;
+    }
+    for (int q = 0; q < ESM_NREN * h->N; q++)
+        if (!std::isfinite(kref[q])) { set_error("esm_replica_features: non-finite reference capacity"); return ESM_EINVAL; }
+    return h->prec == ESM_FP64 ? launch_features<double>(h, kref, out) : launch_features<float>(h, kref, out);
 }
 
 int32_t esm_download_weather(esm_handle *h, int32_t *m, double *z) {

@@ -39,6 +39,15 @@ struct DevModel {
 template <typename S>
 __device__ __forceinline__ S clampv(S x, S lo, S hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
+// |S| − 1 of a pro-rata kink at node n (`prorata!`, src/local.jl:1784-1802), S the node's
+// dispatchable units with |κ| ≥ κmin. The model's one dispatchable unit per node is its gas unit
+// (renewables enter as load), so this is 0 at every node that carries a kink.
+template <typename S>
+__device__ __forceinline__ double kink_nm(const DevModel<S> &m, int n, S kmin) {
+    const int nu = fabs(m.kc[ESM_KGAS * m.N + n]) >= kmin ? 1 : 0;
+    return (double)(nu - 1);
+}
+
 // Neumaier compensated sum.
 template <typename S>
 struct CSum {

@@ -158,6 +158,8 @@ struct esm_handle {
     std::vector<double> Ptr_h, pi0_h;   // [K][K], [K] (host copies for the FFBS's log tables)
     double *d_mu64 = nullptr;    // [K][3N][3] regime means in FP64 (FFBS emissions)
     double *d_ffbs = nullptr;    // [K][T][R] FFBS emissions, then the forward filter (allocated at first use)
+    double *d_feat = nullptr;    // replica features: kref [3][N], D [T], rows [R][nf] (allocated at first use)
+    size_t feat_len = 0;
     int *d_ffbs_mask = nullptr;  // [R]
     // sweep schedule
     struct ThetaRound {
@@ -280,6 +282,10 @@ int launch_weather(esm_handle *h, bool init, int B, int compensate, uint32_t k0,
 // the centred FFBS of the regimes of the replicas whose mask [R] (host) is set
 template <typename S>
 int launch_ffbs(esm_handle *h, const int *mask, uint32_t k0, uint32_t k1, uint32_t ctr);
+
+// k_features.hip: the weather features of every replica into out [R][3 + 5 + K] (host)
+template <typename S>
+int launch_features(esm_handle *h, const double *kref, double *out);
 
 // Copy replica js of hs into replica jd of hd (operation, raw weather and, with `weather`, the
 // anomalies and regimes) on hd's stream. Both handles have the same shape and precision.

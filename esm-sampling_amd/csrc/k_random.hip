@@ -585,7 +585,7 @@ struct FSparse {
         combine(P, sgn, true);
     }
 
-    // log Jacobian Σ (|S| − 1) log|P(t)/P| of the compensated kinks (FP64); one unit per node
+    // log Jacobian Σ (|S| − 1) log|P(t)/P| of the compensated kinks (FP64)
     __device__ double logjac(S t, bool &any) const {
         double lj = 0.0;
         any = false;
@@ -593,10 +593,10 @@ struct FSparse {
             S Q, g, ak, P, e, y0;
             if (!kink(q, Q, g, ak, P, e, y0)) continue;
             any = true;
-            const double nm = 0.0;   // units of the node with |κ| ≥ κmin, minus one
+            const double nm = kink_nm(*m, tb->kk_node[q], kmin);
             lj += nm * log(fabs((fmax((double)Q + (double)g * (double)t, 0.0) + (double)e) / (double)P));
         }
-        if (nw > 1) {   // (lj is exactly 0 here: one unit per node, |S| − 1 = 0)
+        if (nw > 1) {
             any = sum_waves(any ? S(1) : S(0)) > S(0);
             lj = (double)sum_waves((S)lj);
         }

@@ -73,6 +73,7 @@ struct FStor {
     S *pl;                             // SMS_NPLANE planes [q][u] of stride (L + 1)·nus
     const SmsKink<S> *kk_l;            // [2][u]: the kinks at steps 0 and L
     const SmsUnit<S> *un_l;            // [u]
+    S kmin;
     mutable int nev = 0;
     enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
     static constexpr bool CACHED = true;               // root searches read the recorded secondaries
@@ -235,7 +236,7 @@ struct FStor {
         return seg_best<S, W>(have, best, sgn, xr);
     }
 
-    // log Jacobian of the compensated kinks (one unit per node: |S| − 1 = 0), segment sum
+    // log Jacobian Σ (|S| − 1) log|P(t)/P| of the compensated kinks (FP64), segment sum
     __device__ double logjac(S t, bool &any) const {
         double lj = 0.0;
         int cnt = 0;
@@ -248,7 +249,7 @@ struct FStor {
                 const SmsKink<S> kk = kk_l[kidx * nus + u];
                 if (kk.valid == S(0)) continue;
                 cnt++;
-                const double nm = 0.0;
+                const double nm = kink_nm(*m, un_l[u].node, kmin);
                 lj += nm * log(fabs((fmax((double)kk.Q + (double)kk.g * (double)t, 0.0) + (double)kk.e) / (double)kk.P));
             }
         }
@@ -380,6 +381,7 @@ __global__ void __launch_bounds__(64, min_waves<S>(4, 2)) k_storage_ms(DevModel<
         f.pl = base;
         f.kk_l = kk_l;
         f.un_l = un_l;
+        f.kmin = kmin;
         const S *ye_g = st.v + plane((2 + si) * m.N, m.T, m.R);
         const S *yg_g = st.v + plane(m.N, m.T, m.R);
         // stage the block's values of the segment's replica (each lane its own steps)
@@ -696,7 +698,7 @@ struct FLong {
     const S *gv;          // the node's gas plane at (n, ·, j), stride R
     int T, R, lane, L, level, a0, n;
     StoPar<S> par;
-    S kst, w;
+    S kst, w, kmin;
     const SmsKink<S> *kk; // [2] in LDS: the kinks at steps a and b
     S llo, lhi;           // t in [llo, lhi]: every interior step of the lane stays in the box
     // the same per chunk of `ch` consecutive interior steps of the lane (LDS [chunk][lane], bounds
@@ -932,7 +934,7 @@ struct FLong {
             const SmsKink<S> k = kk[e];
             if (k.valid == S(0)) continue;
             cnt++;
-            const double nm = 0.0;
+            const double nm = kink_nm(*m, n, kmin);
             lj += nm * log(fabs((fmax((double)k.Q + (double)k.g * (double)t, 0.0) + (double)k.e) / (double)k.P));
         }
         any = seg_sum<double, 64>((double)cnt) > 0;
@@ -1028,6 +1030,7 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
         f.par = {m.hours[us], m.eta_in[us], m.eta_out[us], m.qt[us], m.mc[(ESM_KSTOR + si) * N + n]};
         f.kst = m.kc[(ESM_KSTOR + si) * N + n];
         f.w = f.kst + m.kappa0;
+        f.kmin = kmin;
         f.kk = kk_l;
         f.cwl = cwl;
         f.cwh = cwh;

@@ -100,6 +100,7 @@ SIGNATURES = [
                                      _ip]),
     ("esm_weather_ffbs", C.c_int32, [C.c_void_p, _ip, C.POINTER(C.c_uint32), C.c_uint32]),
     ("esm_download_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
+    ("esm_replica_features", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("esm_upload_weather", C.c_int32, [C.c_void_p, _ip, _dp]),
     ("esm_copy_replica", C.c_int32, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     ("esm_sweep_options", C.c_int32, [C.c_void_p, C.c_double, _dp, C.c_int32, C.c_double]),

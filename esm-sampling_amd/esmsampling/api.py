@@ -21,7 +21,7 @@ import ctypes as C
 
 import numpy as np
 
-from . import _lib
+from . import _lib, synth
 from ._lib import FP32, FP64, NK, NPARTS, NREN, NV, check, dptr, f64, iptr
 
 PART_NAMES = ("capex", "op", "gen", "storage", "shed", "dump", "penalty", "co2")
@@ -387,6 +387,29 @@ def weather_state(R: Replicas):
     z = np.empty((NREN * R.N, R.T, R.R))
     check(R.lib.esm_download_weather(R.handle, iptr(m), dptr(z)), "esm_download_weather")
     return m, z
+
+
+def replica_feature_names(sys) -> list:
+    """Names of the replica weather features (``replica_feature_names``, src/replicas.jl:274-276):
+    the annual mean load, the capacity factors of the weather carriers, the residual load's mean,
+    peak and worst 7-step mean (GW), the regime frequencies and the number of regime switches."""
+    K = int(getattr(sys, "K", 1) or 1)
+    return (["load_GW"] + [f"cf_{c}" for c in synth.CARRIERS[:NREN]] +
+            ["resload_mean_GW", "resload_peak_GW", "resload_7step_GW"] + [f"regime_{k}" for k in range(1, K + 1)] +
+            ["regime_switches"])
+
+
+def replica_features(R: Replicas, kref) -> np.ndarray:
+    """The weather features [R, nf] of every replica (``replica_features``, src/replicas.jl:270-273)
+    at the reference capacities ``kref`` [6, N] (the renewable fields are read), in the order of
+    ``replica_feature_names``."""
+    kref = f64(kref)
+    if kref.shape != (len(synth.CARRIERS), R.N):
+        raise ValueError(f"kref must be [{len(synth.CARRIERS)}, N]")
+    nf = len(replica_feature_names(R.sys))
+    out = np.empty((R.R, nf))
+    check(R.lib.esm_replica_features(R.handle, dptr(kref), dptr(out)), "esm_replica_features")
+    return out
 
 
 def set_weather(R: Replicas, m, z):

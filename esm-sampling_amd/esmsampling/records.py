@@ -58,7 +58,7 @@ def record_matrix(v):
 
 # the replica records of one chain per step (keys of replicas/<c>/..., docs/system/data.md §3)
 REPLICA_KEYS = ("ktot", "capex", "op", "shed", "penalty", "pfcost", "weights", "active", "since", "phi", "moved",
-                "rs", "fk", "eta_w", "nclip", "time", "learned")
+                "rs", "fk", "eta_w", "nclip", "time", "learned", "features")
 PROBE_KEYS = ("it", "gap", "rs_eq", "rs_in", "fk", "cost", "eta_before", "eta_after")
 
 
@@ -91,9 +91,12 @@ def _columns(recs, keys):
 
 def write_replica_outputs(path, cfg, sys, ctl, rrecs, precs, restarts=0, resumed_at=-1):
     """``write_replica_outputs`` (src/replicas.jl:961-986): the replica and probe records of every
-    chain, the carriers, M, κ_s, the tilt, the weather transition matrix, β and the restarts.
-    (No replica weather features: ``replicas/feature_names`` is empty.)"""
-    out = {"replicas/carriers": np.array(synth.CARRIERS), "replicas/feature_names": np.array([], dtype=str),
+    chain (``replicas/<c>/features``: the replicas' weather features stacked replica by replica),
+    the carriers, the feature names, M, κ_s, the tilt, the weather transition matrix, β and the
+    restarts."""
+    from . import api
+    out = {"replicas/carriers": np.array(synth.CARRIERS),
+           "replicas/feature_names": np.array(api.replica_feature_names(sys), dtype=str),
            "replicas/M": np.array(ctl.M), "replicas/kappa_s": np.asarray(ctl.kappa_s),
            "replicas/tilt": np.array([0.0, np.nan, np.nan]), "run/restarts": np.array(restarts),
            "run/resumed_at_sweep": np.array(resumed_at), "calibration/beta": np.array([cfg.beta_k, cfg.beta_v])}
@@ -113,11 +116,12 @@ def summary_path(out: str) -> str:
     return os.path.join(os.path.dirname(os.path.abspath(out)), "summary_replicas.txt")
 
 
-def write_replica_summary(path, ctl, rrecs, precs, kinds, nwarm: int, passed: bool, wall: float):
+def write_replica_summary(path, ctl, rrecs, precs, kinds, nwarm: int, passed: bool, wall: float, feature_names=()):
     """``write_replica_summary`` (src/replicas.jl:991-1047) over the production steps (all steps
     if there are none) per chain: carrier totals (mean, sd, last), the last Langevin steps, the
     drift and the probes' equilibrated scores and gaps in units of β_k ∂K, capex, penalty-free
-    cost, opex, shed, the mean number of active replicas."""
+    cost, opex, shed, the mean number of active replicas, and the replica weather features
+    (mean and sd over all replicas and steps)."""
     fmt = lambda x: f"{x:.6g}"
     cars = synth.CARRIERS
 
@@ -162,6 +166,13 @@ def write_replica_summary(path, ctl, rrecs, precs, kinds, nwarm: int, passed: bo
             A = cols[key]
             lines.append(lab.ljust(26) + "mean " + fmt(A.mean()) + "  sd " + fmt(A.std(ddof=1) if A.size > 1 else 0.0))
         lines.append("active replicas".ljust(26) + "mean " + fmt(cols["active"].sum(axis=0).mean()) + f" of {ctl.M}")
+        nf = len(feature_names)
+        if nf and "features" in cols and cols["features"].size:
+            X = cols["features"].T.reshape(-1, nf)
+            lines.append("replica weather features (all replicas and steps): mean, sd")
+            for i, name in enumerate(feature_names):
+                sd = X[:, i].std(ddof=1) if X.shape[0] > 1 else 0.0
+                lines.append("  " + name.ljust(24) + fmt(X[:, i].mean()).rjust(13) + fmt(sd).rjust(13))
     d = os.path.dirname(os.path.abspath(path)) or "."
     fd, tmp = tempfile.mkstemp(prefix=".tmp_", suffix=".txt", dir=d)
     with os.fdopen(fd, "w") as f:

@@ -878,6 +878,8 @@ def run_replicas(cfg: Config, log=None):
         traces[c].append(replica_monitor(R, st["parts"]))
         rec = rec_io.replica_step_record(R.k, st["parts"], st["wts"], R.active, R.since, st["moved"], st["rs"], st["fk"],
                                          R.eta_w, st["nclip"], time.perf_counter() - t_step, st["tw"], st["learned"])
+        # the replicas' weather features at the set-up start's capacities (src/replicas.jl:818, :863)
+        rec["features"] = api.replica_features(R.dev, sys.k0).ravel()
         rec.update(it=it, k=R.k.copy(), nclip=int(st["nclip"]))
         records[c].append(rec)
         if it % 25 == 0 or it == nwarm + nprod:
@@ -956,7 +958,7 @@ def run_replicas(cfg: Config, log=None):
     if cfg.out and status == 0:
         rec_io.write_replica_outputs(cfg.out, cfg, sys, ctl, records, probes, restarts, it0 if snap is not None else -1)
         rec_io.write_replica_summary(rec_io.summary_path(cfg.out), ctl, records, probes, kinds, nwarm, passed,
-                                     t_end - t_setup)
+                                     t_end - t_setup, feature_names=api.replica_feature_names(sys))
     lsum = None
     if cfg.learned_proposal:
         lsum = dict(attempted=sum(q.att for q in learned), accepted=sum(q.acc for q in learned),

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define ESM_ABI_VERSION 8
+#define ESM_ABI_VERSION 9
 
 /* carriers */
 #define ESM_NREN 3   /* renewables as load: 0 solar, 1 onwind, 2 offwind              */
@@ -435,6 +435,18 @@ int32_t esm_weather_ffbs(esm_handle *h, const int32_t *mask, const uint32_t key[
 
 /* Regimes m [T][R] (0-based) and the VAR anomalies z [ESM_NREN*N][T][R] (either may be NULL). */
 int32_t esm_download_weather(esm_handle *h, int32_t *m, double *z);
+
+/*
+ * esm_replica_features: the weather features of every replica (`replica_features`,
+ * src/replicas.jl:270-273, over `weather_features`, :250-267), the rows of the driver's
+ * `replicas/<c>/features` record. kref [ESM_NK][N]: the fixed reference capacities (the
+ * renewable fields 0 … ESM_NREN−1 are read). out [R][ESM_NREN + 5 + K] (FP64), per replica in
+ * the order of `replica_feature_names`: load_GW (mean over t of Σ_n max(load, 0), /1e3), cf of
+ * solar, onwind, offwind (mean of clamp(x, 0, 1) over nodes and steps), residual load
+ * res_t = (D_t − Σ kref·A_t)/1e3: its mean, peak and worst cyclic 7-step mean (GW), the
+ * fraction of steps in each regime 1 … K, the number of regime switches.
+ */
+int32_t esm_replica_features(esm_handle *h, const double *kref, double *out);
 /* Set the regimes and anomalies; x = sigmoid(xbar + z) follows and the caches are rebuilt. */
 int32_t esm_upload_weather(esm_handle *h, const int32_t *m, const double *z);
 

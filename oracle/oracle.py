@@ -461,3 +461,37 @@ def philox(key, ctr):
     o = (C.c_uint32 * 4)()
     lib.orc_philox(k, c, o)
     return list(o)
+
+
+def weather_features(load, x, kref, m, K):
+    """The replica weather features (``replica_features``, src/replicas.jl:270-273, over
+    ``weather_features``, :250-267), numpy FP64, replica by replica.
+
+    load [N, T] (the system's load; here not a weather series), x [3, N, T, R] the complete-level
+    weather after the data map, kref [6, N] the reference capacities (fields 0-2 read), m [T, R]
+    regimes (0-based), K regimes. Per replica: D = Σ_n max(load, 0) (:253), for each weather
+    carrier A = clamp(x, 0, 1), cf = mean(A) and R += kref' A (:256-261), res = (D − R)/1e3
+    (:262), w7 = max_t mean(res[t … t+6 cyclic]) (:264), then
+    [mean(D)/1e3, cf…, mean(res), max(res), w7, count(m == k)/T…, count(diff(m) != 0)] (:271-273).
+    """
+    load = np.asarray(load, dtype=np.float64)
+    x = np.asarray(x, dtype=np.float64)
+    kref = np.asarray(kref, dtype=np.float64)
+    m = np.asarray(m)
+    T, R = m.shape
+    D = np.maximum(load, 0.0).sum(axis=0)
+    rows = []
+    for j in range(R):
+        Rt = np.zeros(T)
+        cf = []
+        for c in range(NREN):
+            A = np.clip(x[c, :, :, j], 0.0, 1.0)
+            cf.append(A.mean())
+            Rt += kref[c] @ A
+        res = (D - Rt) / 1e3
+        idx = (np.arange(T)[:, None] + np.arange(7)[None, :]) % T
+        w7 = res[idx].mean(axis=1).max()
+        mj = m[:, j]
+        rows.append([D.mean() / 1e3, *cf, res.mean(), res.max(), w7,
+                     *[np.count_nonzero(mj == k) / T for k in range(K)], float(np.count_nonzero(np.diff(mj) != 0))])
+    return np.array(rows)

@@ -1195,6 +1195,10 @@ static int build_tables(const esm_model *m, Tables *tb) {
     return rc;
 }
 
+/* |S| of a pro-rata kink at node n (prorata!, src/local.jl:1784-1802): the node's dispatchable
+ * units with |κ| ≥ κmin (κmin = 1), in this model its gas unit */
+static int gas_units(const Sys *S, int n) { return fabs(S->kc[ESM_KGAS * S->m->N + n]) >= 1.0 ? 1 : 0; }
+
 /* ---- a table-driven move of replica j at τ, evaluated on the fly from the caches ---- */
 enum { SP_CELL = 0, SP_XLOC = 1, SP_XLOCC = 2, SP_XPAIR = 3 };
 
@@ -1414,7 +1418,7 @@ static double sm_sample(const SMove *f, double beta, double maxnats, double useg
     return draw_points(&PL, &PR, useg, uoff, status);
 }
 
-/* logjac (:1874): Σ (|S| − 1) log|P(t)/P| over the kinks made; one unit per node here */
+/* logjac (:1874): Σ (|S| − 1) log|P(t)/P| over the kinks made, |S| from the model */
 static double sm_logjac(const SMove *f, double t, int *any) {
     double lj = 0.0;
     *any = 0;
@@ -1422,7 +1426,7 @@ static double sm_logjac(const SMove *f, double t, int *any) {
         double Q, g, ak, P, e, y0;
         if (!sm_kink(f, q, &Q, &g, &ak, &P, &e, &y0)) continue;
         *any = 1;
-        double nm = 0.0;
+        double nm = (double)(gas_units(f->S, f->tb->kk_node.a[q]) - 1);
         lj += nm * log(fabs((fmax(Q + g * t, 0.0) + e) / P));
     }
     return lj;
@@ -2196,7 +2200,7 @@ static double sms_logjac(const SmsMove *f, double t, int *any) {
             const SmsKink *kk = &f->kk[kidx][u];
             if (!kk->valid) continue;
             cnt++;
-            double nm = 0.0;
+            double nm = (double)(gas_units(f->S, f->node[u]) - 1);
             part[lane] += nm * log(fabs((fmax(kk->Q + kk->g * t, 0.0) + kk->e) / kk->P));
         }
     }
@@ -2656,7 +2660,7 @@ static double slg_logjac(const SlgMove *f, double t, int *any) {
             const SmsKink *kk = &f->kk[kidx];
             if (!kk->valid) continue;
             cnt++;
-            double nm = 0.0;
+            double nm = (double)(gas_units(f->S, f->n) - 1);
             part[(kidx == 0 ? 0 : f->L) % 64] += nm * log(fabs((fmax(kk->Q + kk->g * t, 0.0) + kk->e) / kk->P));
         }
     *any = cnt > 0;
@@ -3148,7 +3152,6 @@ static double xm_sample(const XMove *f, double beta, double maxnats, double useg
     return draw_points_big(&PL, &PR, useg, uoff, status);
 }
 /* units of the node with |κ| ≥ 1 among its dispatchable units (this system: the gas unit) */
-static int gas_units(const Sys *S, int n) { return fabs(S->kc[ESM_KGAS * S->m->N + n]) >= 1.0 ? 1 : 0; }
 /* logjac (src/local.jl:1874): Σ (|S| − 1) log|P(t)/P| over the kinks made, |S| from the model */
 static double xm_logjac(const XMove *f, double t, int *any) {
     double part[64] = {0};

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_error_without_gpu_work():
     from esmsampling import _lib
     lib = _lib.load()
-    assert lib.esm_version() == 8
+    assert lib.esm_version() == 9
     # an invalid argument is rejected on the host, before any device call
     h = C.c_void_p()
     assert lib.esm_create(None, 1, 64, C.byref(h)) == -1

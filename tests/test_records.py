@@ -5,12 +5,13 @@ ragged driver buffers of a snapshot, and the snapshot checks of ``read_replica_s
 import numpy as np
 import pytest
 
+from esmsampling import api
 from esmsampling import records as rio
 from esmsampling import replicas as RP
 from esmsampling import synth
 
 
-def _recs(rng, steps, M, N):
+def _recs(rng, steps, M, N, nf=0):
     out = []
     for it in range(1, steps + 1):
         parts = rng.uniform(1, 2, (M, 8))
@@ -19,6 +20,8 @@ def _recs(rng, steps, M, N):
                                     np.arange(M), np.zeros(M), rng.normal(size=6), rng.uniform(1, 2, 6), np.full(6, 5e-4),
                                     3, 0.5, 0.4, [10, 4])
         r.update(it=it, k=np.zeros((6, N)), nclip=3)
+        if nf:
+            r["features"] = rng.normal(size=M * nf)
         out.append(r)
     return out
 
@@ -28,7 +31,11 @@ def test_outputs_layout_and_summary(tmp_path):
     cfg = RP.load_config({"init": "heuristic", "replicas": 3, "nchains": 2, "mlda_warmup": 2, "nsamples": 3})
     ctl = RP.ReplicaCtl.from_config(cfg)
     s = synth.make_system(N=5, L=6, T=24, dt=1.0, seed=2)
-    recs = [_recs(rng, 5, 3, s.N) for _ in range(2)]
+    names = api.replica_feature_names(s)
+    nf = len(names)
+    assert names[:4] == ["load_GW", "cf_solar", "cf_onwind", "cf_offwind"] and names[-1] == "regime_switches"
+    assert nf == 3 + 5 + s.K
+    recs = [_recs(rng, 5, 3, s.N, nf) for _ in range(2)]
     pr = {"gap": np.ones(6), "rs_eq": np.ones(6), "rs_in": np.ones(6), "fk": np.ones(6), "op_in": 1.0, "op_eq": 2.0,
           "shed_in": 0.0, "shed_eq": 0.0}
     probes = [[rio.probe_record(4, 1, True, pr, np.ones(6), 2 * np.ones(6))], []]
@@ -45,10 +52,19 @@ def test_outputs_layout_and_summary(tmp_path):
     assert "replicas/2/probe/it" not in d
     assert int(d["run/restarts"]) == 1 and int(d["run/resumed_at_sweep"]) == 3
     assert d["weather/transition"].shape == s.Ptr.shape
+    # features stacked replica by replica, one column per step (src/replicas.jl:818)
+    assert list(d["replicas/feature_names"]) == names
+    assert d["replicas/2/features"].shape == (3 * nf, 5)
+    assert np.array_equal(d["replicas/2/features"][:, 4], recs[1][4]["features"])
     sp = rio.summary_path(str(path))
-    rio.write_replica_summary(sp, ctl, recs, probes, ["low", "high"], 2, False, 12.0)
+    rio.write_replica_summary(sp, ctl, recs, probes, ["low", "high"], 2, False, 12.0, feature_names=names)
     txt = open(sp).read()
     assert "## chain 1 (low), steps 3–5" in txt and "## chain 2 (high)" in txt and "probes" in txt
+    assert "replica weather features" in txt
+    # the summary's feature means over the production steps 3–5 and all replicas
+    X = np.concatenate([r["features"].reshape(3, nf) for r in recs[0][2:]])
+    line = [q for q in txt.splitlines() if q.strip().startswith("resload_peak_GW")][0]
+    assert abs(float(line.split()[1]) - X[:, names.index("resload_peak_GW")].mean()) < 1e-5 * (1 + abs(X[:, 6].mean()))
 
 
 def test_ragged_buffers_round_trip():
