@@ -64,10 +64,10 @@ def test_run_replicas_short():
     for p in res["probes"][0]:
         assert np.all(np.isfinite(p["rs_eq"][np.isfinite(p["fk"])]))
     # the replicas' weather features, stacked replica by replica: finite, regime fractions sum to 1
-    nf = 3 + 5 + 2
+    K = cfg.K
     for rec in res["records"][1]:
-        F = rec["features"].reshape(4, nf)
-        assert np.all(np.isfinite(F)) and np.allclose(F[:, 7:9].sum(axis=1), 1.0) and np.all(F[:, 1:4] <= 1.0)
+        F = rec["features"].reshape(4, 3 + 5 + K)
+        assert np.all(np.isfinite(F)) and np.allclose(F[:, 7:7 + K].sum(axis=1), 1.0) and np.all(F[:, 1:4] <= 1.0)
     # bursts reset the gates: some steps have inactive replicas
     assert any(not r["active"].all() for r in res["records"][0])
     for rec in res["records"][0]:
