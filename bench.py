@@ -301,7 +301,8 @@ def ess_leg(args):
     batched sampler). R-hat is the rank-normalized split R-hat of the same traces."""
     from esmsampling import replicas as RP
     nw = max(args.ess_steps // 6, 2)
-    base = dict(nchains=args.ess_chains, replicas=args.replicas, mlda_warmup=nw, nsamples=args.ess_steps - nw,
+    base = dict(init="heuristic", nchains=args.ess_chains, replicas=args.replicas, mlda_warmup=nw,
+                nsamples=args.ess_steps - nw,
                 replica_init_sweeps=10, synthetic_nodes=args.nodes, synthetic_lines=args.lines,
                 snapshots=args.hours, resolution_hours=8760.0 / args.hours, precision="fp32", seed=7)
     # a short run first (discarded): the first run of a process pays one-time costs (code objects

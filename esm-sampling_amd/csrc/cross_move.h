@@ -12,11 +12,9 @@
 // instants of a unit stay in [0, 1] (`mv_eval!`, :370-427). Every unit of this system can
 // charge, so the losses go into the residuals of the units' nodes (sx_loss = 0, :108).
 //
-// Lanes: a segment of W lanes (W = 1 for sxpair, one thread per replica; W = 64 for sxlong, a
-// wave per replica) owns the move; lane q mod W owns step q (0 … L: step a+q), lane 0 end a and
-// lane L mod W end b, and each lane sums its steps' terms in increasing q; ΔC(t) is the segment's
-// butterfly sum. The walk is `walk_stor` (storage_walk.h) without recording: the secondaries at
-// the ends of an interval are evaluated again by the root searches.
+// Lanes: one wave per (move, replica) (FXLane below): the lanes hold the items of the end
+// steps' template and the interior steps q mod 64; the walk is `walk_stor` (storage_walk.h) with
+// recording.
 #pragma once
 #include "storage_walk.h"
 
@@ -96,149 +94,208 @@ struct XKinkD {
     S Q, g, ak, P, e, y0;
 };
 
-template <typename S, int W>
-struct FXPair {
+
+// ---------------------------------------------------------------------------------------------
+// FXLane: the pair move of one replica on one wave. The end steps' terms are items of four kinds
+// (`eval_cached!`, src/local.jl): the kinked gas entries of the leak nodes, the template's lines,
+// the storage terms of A and B, the balances of the template's nodes. Items of one kind over both
+// ends (end a first) are spread over the lanes, item k of a kind on lane k mod 64, so that a pass
+// runs each kind's code once on all lanes at the same time. Each lane reads its items' values at
+// t = 0 once (prepare) and keeps them in registers; a pass at t computes the kinks', lines' and
+// storage terms, which publish their change of the node balances in LDS, then the balances; the
+// storage d and the balances are recorded at the walk's interval and piece ends (the storage
+// walk's CACHED protocol), so the root searches evaluate nothing. The interior steps a+1 … a+L−1
+// (sxlong) belong to lane q mod 64. ΔC(t): a lane sums its kinks, lines, storage terms, balances
+// (each kind in increasing item index), then its interior steps in increasing q; the lanes' sums
+// are combined by the 64-lane butterfly.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int XL_KS = 1;                 // kink slots per lane: at most 64 kinks of both ends
+constexpr int XL_LS = 3;                 // line slots: at most 192 lines of both ends
+constexpr int XL_NS = 2;                 // node slots: at most 128 balances of both ends
+constexpr int XL_NLMAX = 512;            // node–line entries of a move (both ends)
+constexpr int XL_NEG = 1 << 20;          // sign flag of a node–line entry (the node is the line's bus1)
+constexpr int XL_XK = 0, XL_XL = 64 * XL_KS, XL_XS = XL_XL + 64 * XL_LS, XL_NX = XL_XS + 4;
+
+// the LDS of one wave's pair move
+template <typename S>
+struct XlLds {
+    S xv[XL_NX];        // at the pass's t: each kink's, line's and storage term's change of the balances
+    int nl[XL_NLMAX];   // per end, the template's node–line entries: line's xv index | XL_NEG, −1 (none)
+};
+
+// whether the template of slot `slot` fits the lane form (checked on the host for every slot)
+template <typename S>
+__host__ __device__ __forceinline__ bool xl_fits_counts(int K, int Lw, int Nn, int NLp) {
+    return 2 * K <= 64 * XL_KS && 2 * Lw <= 64 * XL_LS && 2 * Nn <= 64 * XL_NS && 2 * NLp <= XL_NLMAX;
+}
+template <typename S>
+__device__ __forceinline__ bool xl_fits(const SxTab<S> &tb, int slot) {
+    return xl_fits_counts<S>(tb.kk_off[slot + 1] - tb.kk_off[slot], tb.ln_off[slot + 1] - tb.ln_off[slot],
+                             tb.nd_off[slot + 1] - tb.nd_off[slot],
+                             tb.nd_l[tb.nd_off[slot + 1]] - tb.nd_l[tb.nd_off[slot]]);
+}
+
+// IN: the move has interior steps (sxlong); sxpair (L = 1) compiles without them
+template <typename S, bool IN>
+struct FXLane {
     enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
-    static constexpr bool CACHED = false;   // the root searches evaluate the interval ends again
+    static constexpr bool CACHED = true;
+    static constexpr int W = 64;
     const DevModel<S> *m;
     SxTab<S> tb;
+    XlLds<S> *sh;
     const S *ye0, *ye1;    // the units' state-of-charge planes at (node, ·, replica), stride R
     const S *rp, *Fp, *gp; // residual, flow and gas planes at (·, ·, replica)
     int T, R, lane, L, a0, slot;
     int nd0, nd1;
+    int K, Lw, Nn, NLp;
     S cu0, cu1, kst0, kst1, w0, w1, kap, kmin;
     StoPar<S> par0, par1;
     S llo, lhi;            // t in [llo, lhi]: the lane's interior steps keep every instant in [0, 1]
+    // kinks (item e·K + k): node, made (bit 0), ak ≠ 0 (bit 1); Q, g, ak, y0, c0, p0, P, e
+    int kn[XL_KS], kok[XL_KS];
+    S kQ[XL_KS], kg[XL_KS], ka[XL_KS], ky[XL_KS], kc0[XL_KS], kp0[XL_KS], kP[XL_KS], ke[XL_KS];
+    // lines (item e·Lw + l): line; f0, df (0: dropped), limit, max(|f0| − limit, 0)
+    int ll[XL_LS];
+    S lf[XL_LS], ld[XL_LS], ls_[XL_LS], lp0[XL_LS];
+    // storage terms (item 2e + u on lane 2e + u): y0, y1, c0, j0, dye, dye1; d at the interval's ends
+    int su, sq;
+    S sy0, sy1, sc0, sj0, sdy0, sdy1;
+    mutable S sdA, sdB;
+    // balances (item e·Nn + p): node, first node–line entry, entries, xv index of its kink (−1:
+    // none made); S0, φ(S0); the balance at the piece's ends
+    int nn_[XL_NS], no[XL_NS], nc[XL_NS], nk[XL_NS];
+    S nS0[XL_NS], nph[XL_NS];
+    mutable S nA[XL_NS], nB[XL_NS];
     mutable int nev = 0;
     // the lane's nearest clamp point of its rising instants beyond the walk's last point (the
     // points are fixed during a move and a walk only moves forward)
     mutable S bp_tp = 0, bp_sgn = 0, bp_next = 0;
     mutable bool bp_have = false;
 
-    __device__ int tau(int q) const {
+    // unit u's parameters, selected field by field as values (a selected member, an lvalue, would
+    // keep the walker in scratch memory)
+    __device__ __forceinline__ StoPar<S> par(int u) const {
+        StoPar<S> p;
+        p.h = u == 0 ? +par0.h : +par1.h;
+        p.ein = u == 0 ? +par0.ein : +par1.ein;
+        p.eout = u == 0 ? +par0.eout : +par1.eout;
+        p.qt = u == 0 ? +par0.qt : +par1.qt;
+        p.mcs = u == 0 ? +par0.mcs : +par1.mcs;
+        return p;
+    }
+    __device__ __forceinline__ int tau(int q) const {
         const int x = a0 + q;
         return x >= T ? x - T : x;
     }
-    __device__ int next(int t) const { return t + 1 == T ? 0 : t + 1; }
-    __device__ S y(int u, int t) const { return (u == 0 ? ye0 : ye1)[(size_t)t * R]; }
-    __device__ S rv(int n, int t) const { return rp[((size_t)n * T + t) * R]; }
-    __device__ S Fv(int e, int t) const { return Fp[((size_t)e * T + t) * R]; }
-    __device__ S gv(int n, int t) const { return gp[((size_t)n * T + t) * R]; }
-    __device__ int first(int q0) const { return lane >= q0 ? lane : lane + W * ((q0 - lane + W - 1) / W); }
-    __device__ bool owns_a() const { return lane == 0; }
-    __device__ bool owns_b() const { return lane == L % W; }
-    __device__ S sc(int e) const { return e == 0 ? -kap : kap; }
-    __device__ static S thv(S c, S v) {
+    __device__ __forceinline__ int next(int t) const { return t + 1 == T ? 0 : t + 1; }
+    __device__ __forceinline__ S y(int u, int t) const { return (u == 0 ? +ye0 : +ye1)[(size_t)t * R]; }
+    __device__ __forceinline__ S rv(int n, int t) const { return rp[((size_t)n * T + t) * R]; }
+    __device__ __forceinline__ S Fv(int e, int t) const { return Fp[((size_t)e * T + t) * R]; }
+    __device__ __forceinline__ S gv(int n, int t) const { return gp[((size_t)n * T + t) * R]; }
+    __device__ __forceinline__ int first(int q0) const {
+        return lane >= q0 ? lane : lane + W * ((q0 - lane + W - 1) / W);
+    }
+    __device__ __forceinline__ S sc(int e) const { return e == 0 ? -kap : kap; }
+    __device__ __forceinline__ static S thv(S c, S v) {
         const S a = c * v;
         return fabs(a) < S(1e-14) ? S(0) : a;
     }
     // flow change per unit t of template line l at scale c (0: dropped)
-    __device__ S dfl(S c, int l) const {
+    __device__ __forceinline__ S dfl(S c, int l) const {
         const S df = m->lb[tb.ln_line[l]] * (thv(c, tb.ln_ta[l]) - thv(c, tb.ln_tb[l]));
         return fabs(df) <= S(1e-14) ? S(0) : df;
     }
-    // `prorata!` (qadj = 0) of the gas unit of leak kq at end step e
-    __device__ bool kink(int e, int kq, XKinkD<S> &k) const {
-        const int n = tb.kk_node[kq], t = tau(e == 0 ? 0 : L);
-        const S g = sc(e) * tb.kk_g[kq];
-        if (g == S(0)) return false;
-        const S kp = m->kc[ESM_KGAS * m->N + n];
-        if (!(fabs(kp) >= kmin)) return false;
-        const S w = gv(n, t);
-        const S P = S(0) + kp * w, Qs = S(0) + kp * clampv(w, S(0), S(1));
-        if (fabs(P) < S(1e-6)) return false;
-        k.Q = rv(n, t) + Qs;
-        k.g = g;
-        k.e = P - fmax(k.Q, S(0));
-        k.P = P;
-        const S sig = kp * w / P;
-        k.ak = sig == S(0) ? S(0) : sig / kp;
-        k.y0 = w;
-        return true;
-    }
-    __device__ static S kphi(S Q, S g, S t) { return fmax(Q + g * t, S(0)) - fmax(Q, S(0)); }
-    __device__ void gen(int n, S yy, S &c, S &p) const {
+    __device__ __forceinline__ static S kphi(S Q, S g, S t) { return fmax(Q + g * t, S(0)) - fmax(Q, S(0)); }
+    __device__ __forceinline__ void gen(int n, S yy, S &c, S &p) const {
         const S k = m->kc[ESM_KGAS * m->N + n];
         gen_cost(m->mc[ESM_KGAS * m->N + n], k, k + m->kappa0, m->dt, m->cv, yy, c, p);
     }
-    // storage term of unit u at step q at t: cost change; d at t, injection change
-    __device__ S sterm(int u, int q, S t, S &d1, S &dj) const {
+    __device__ __forceinline__ void sync() const {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+
+    // the lane's end-step terms at t summed in its item order (kinks, lines, storage, balances);
+    // sd: the storage term's d at t, sb[s]: the balances at t
+    __device__ __forceinline__ S items(S t, S &sd, S *sb) const {
+        S D = 0;
+#pragma unroll
+        for (int s = 0; s < XL_KS; s++) {
+            S x = 0;
+            if (kok[s] & 2) {
+                S c1, p1;
+                gen(kn[s], ky[s] + ka[s] * kphi(kQ[s], kg[s], t), c1, p1);
+                D += c1 - kc0[s];
+                x = p1 - kp0[s];
+            }
+            sh->xv[XL_XK + lane + 64 * s] = x;
+        }
+#pragma unroll
+        for (int s = 0; s < XL_LS; s++) {
+            S x = 0;
+            if (ld[s] != S(0)) {
+                const S f1 = lf[s] + t * ld[s];
+                D += m->cv * m->dt * (fmax(fabs(f1) - ls_[s], S(0)) - lp0[s]);
+                x = (lf[s] + t * ld[s]) - lf[s];
+            }
+            sh->xv[XL_XL + lane + 64 * s] = x;
+        }
+        sd = 0;
+        if (su >= 0) {
+            S j1;
+            const S c1 = soc_cost_p(par(su), m->dt, m->cv, su == 0 ? +kst0 : +kst1, su == 0 ? +w0 : +w1,
+                                    sy0 + t * sdy0, sy1 + t * sdy1, sd, j1);
+            D += c1 - sc0;
+            sh->xv[XL_XS + lane] = j1 - sj0;
+        }
+        sync();
+#pragma unroll
+        for (int s = 0; s < XL_NS; s++) {
+            sb[s] = 0;
+            if (nn_[s] < 0) continue;
+            const int e = lane + 64 * s >= Nn ? 1 : 0;
+            S dr = 0;
+            if (nk[s] >= 0) dr -= sh->xv[nk[s]];
+            for (int q = no[s]; q < no[s] + nc[s]; q++) {
+                const int en = sh->nl[q];
+                if (en < 0) continue;
+                const S dd = sh->xv[en & (XL_NEG - 1)];
+                dr += (en & XL_NEG) ? -dd : dd;
+            }
+            if (nn_[s] == nd0) dr += sh->xv[XL_XS + 2 * e];
+            if (nn_[s] == nd1) dr += sh->xv[XL_XS + 2 * e + 1];
+            const S S1 = nS0[s] + dr;
+            D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - nph[s]);
+            sb[s] = S1;
+        }
+        sync();   // the next pass writes xv again
+        return D;
+    }
+
+    // storage term of unit u at interior step q at t: cost change; d at t, injection change
+    __device__ __forceinline__ S sterm(int u, int q, S t, S &d1, S &dj) const {
         const int ta = tau(q);
-        const S y0 = y(u, ta), y1 = y(u, next(ta)), c = u == 0 ? cu0 : cu1;
+        const S y0 = y(u, ta), y1 = y(u, next(ta)), c = u == 0 ? +cu0 : +cu1;
         const S dye = q >= 1 ? c : S(0), dye1 = q <= L - 1 ? c : S(0);
-        const StoPar<S> &p = u == 0 ? par0 : par1;
-        const S k = u == 0 ? kst0 : kst1, w = u == 0 ? w0 : w1;
+        const StoPar<S> p = par(u);
+        const S k = u == 0 ? +kst0 : +kst1, w = u == 0 ? +w0 : +w1;
         S d0, j0, j1;
         const S c0 = soc_cost_p(p, m->dt, m->cv, k, w, y0, y1, d0, j0);
         const S c1 = soc_cost_p(p, m->dt, m->cv, k, w, y0 + t * dye, y1 + t * dye1, d1, j1);
         dj = j1 - j0;
         return c1 - c0;
     }
-    __device__ S sec(int u, int k, S d) const {
-        const StoPar<S> &p = u == 0 ? par0 : par1;
+    __device__ __forceinline__ S sec(int u, int k, S d) const {
+        const StoPar<S> p = par(u);
         if (k == 0) return d;
         if (k == 1) return p.ein > 0 ? d - p.ein : S(1);
         return -d * p.eout - S(1);
     }
-    // the balance of template node p at end step e at t, given the units' injection changes
-    __device__ S node_S(int e, int p, S t, S djA, S djB) const {
-        const int n = tb.nd_node[p], ta = tau(e == 0 ? 0 : L);
-        const S c = sc(e);
-        S dr = 0;
-        const int ik = tb.nd_kk[p];
-        if (ik >= 0) {
-            XKinkD<S> k;
-            if (kink(e, tb.kk_off[slot] + ik, k) && k.ak != S(0)) {
-                S c0, p0, c1, p1;
-                gen(n, k.y0, c0, p0);
-                gen(n, k.y0 + k.ak * kphi(k.Q, k.g, t), c1, p1);
-                dr -= p1 - p0;
-            }
-        }
-        const int l0 = tb.ln_off[slot];
-        for (int q = tb.nd_l[p]; q < tb.nd_l[p + 1]; q++) {
-            const int pos = l0 + tb.nl_pos[q];
-            const S df = dfl(c, pos);
-            if (df == S(0)) continue;
-            const S f0 = Fv(tb.ln_line[pos], ta), dd = (f0 + t * df) - f0;
-            dr += tb.nl_sgn[q] > 0 ? dd : -dd;
-        }
-        if (n == nd0) dr += djA;
-        if (n == nd1) dr += djB;
-        return rv(n, ta) + dr;
-    }
-    // end step e at t (eval_cached! order: kinked gas, lines, storage A and B, balances)
-    __device__ S end_term(int e, S t) const {
-        const int ta = tau(e == 0 ? 0 : L), q = e == 0 ? 0 : L;
-        const S c = sc(e);
-        S D = 0;
-        for (int kq = tb.kk_off[slot]; kq < tb.kk_off[slot + 1]; kq++) {
-            XKinkD<S> k;
-            if (!kink(e, kq, k) || k.ak == S(0)) continue;
-            S c0, p0, c1, p1;
-            const int n = tb.kk_node[kq];
-            gen(n, k.y0, c0, p0);
-            gen(n, k.y0 + k.ak * kphi(k.Q, k.g, t), c1, p1);
-            D += c1 - c0;
-        }
-        for (int l = tb.ln_off[slot]; l < tb.ln_off[slot + 1]; l++) {
-            const S df = dfl(c, l);
-            if (df == S(0)) continue;
-            const int ln = tb.ln_line[l];
-            const S f0 = Fv(ln, ta), se = m->ls[ln], f1 = f0 + t * df;
-            D += m->cv * m->dt * (fmax(fabs(f1) - se, S(0)) - fmax(fabs(f0) - se, S(0)));
-        }
-        S d, djA, djB;
-        D += sterm(0, q, t, d, djA);
-        D += sterm(1, q, t, d, djB);
-        for (int p = tb.nd_off[slot]; p < tb.nd_off[slot + 1]; p++) {
-            const S S0 = rv(tb.nd_node[p], ta), S1 = node_S(e, p, t, djA, djB);
-            D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - phibal(S0, m->c_shed, m->c_dump));
-        }
-        return D;
-    }
     // the box interval of an instant y + c t
-    __device__ static void box(S yy, S c, S &lo, S &hi) {
+    __device__ __forceinline__ static void box(S yy, S c, S &lo, S &hi) {
         if (c > 0) {
             lo = -yy / c;
             hi = (S(1) - yy) / c;
@@ -247,9 +304,9 @@ struct FXPair {
             hi = -yy / c;
         }
     }
-    __device__ bool inside(int u, int q, S t) const {
+    __device__ __forceinline__ bool inside(int u, int q, S t) const {
         const int ta = tau(q);
-        const S y0 = y(u, ta), y1 = y(u, next(ta)), c = u == 0 ? cu0 : cu1;
+        const S y0 = y(u, ta), y1 = y(u, next(ta)), c = u == 0 ? +cu0 : +cu1;
         if (!(S(0) <= y0 && y0 <= S(1) && S(0) <= y1 && y1 <= S(1))) return false;
         S l0, h0, l1, h1;
         box(y0, c, l0, h0);
@@ -258,21 +315,27 @@ struct FXPair {
     }
     // interior step q at t: false when both units stay inside (nothing changes); else its term x,
     // the units' d (the value at t = 0 for a unit inside) and the balances bA, bB of their nodes
-    __device__ bool int_term(int q, S t, S &x, S &dA, S &dB, S &bA, S &bB) const {
+    __device__ __forceinline__ bool int_term(int q, S t, S &x, S &dA, S &dB, S &bA, S &bB) const {
         const int ta = tau(q);
         S D = 0, djA = 0, djB = 0;
         bool mv = false;
         for (int u = 0; u < 2; u++) {
-            S &d = u == 0 ? dA : dB;
-            S &dj = u == 0 ? djA : djB;
+            S d = 0, dj = 0;
             if (inside(u, q, t)) {
                 S j0;
-                soc_cost_p(u == 0 ? par0 : par1, m->dt, m->cv, u == 0 ? kst0 : kst1, u == 0 ? w0 : w1, y(u, ta),
-                           y(u, next(ta)), d, j0);
-                continue;
+                soc_cost_p(par(u), m->dt, m->cv, u == 0 ? +kst0 : +kst1, u == 0 ? +w0 : +w1, y(u, ta), y(u, next(ta)),
+                           d, j0);
+            } else {
+                D += sterm(u, q, t, d, dj);
+                mv = true;
             }
-            D += sterm(u, q, t, d, dj);
-            mv = true;
+            if (u == 0) {
+                dA = d;
+                djA = dj;
+            } else {
+                dB = d;
+                djB = dj;
+            }
         }
         if (nd0 == nd1) {
             const S S0 = rv(nd0, ta), S1 = S0 + (djA + djB);
@@ -290,26 +353,38 @@ struct FXPair {
         return mv;
     }
 
-    // ΔC(t): the lane's steps in increasing q, summed over the segment
-    __device__ S eval(S t, int) const {
-        nev++;
-        const bool oa = owns_a(), ob = owns_b();
-        S xa = 0, xb = 0;
-        if (oa) xa = end_term(0, t);
-        if (ob) xb = end_term(1, t);
-        S D = 0;
-        if (oa) D += xa;
-        if (!(llo <= t && t <= lhi))
+    // ΔC(t); recording (storage walk protocol): REC_B d and balances at t (an interval's end),
+    // REC_X balances (a storage root), REC_A d and balances at the walk's start, REC_T balances at
+    // an interval's end after a split. Counted as evaluations: REC_NONE, REC_B, REC_X. (Every
+    // recorded value is written as a selected value: stores under branches would be merged into
+    // stores through selected addresses, which keep the walker in scratch memory.)
+    __device__ __forceinline__ S eval(S t, int rec) const {
+        if (rec <= REC_X) nev++;
+        S sd, sb[XL_NS];
+        S D = items(t, sd, sb);
+        sdA = rec == REC_A ? sd : sdA;
+        sdB = rec == REC_B ? sd : sdB;
+        const bool wa = rec == REC_A, wb = rec != REC_NONE && rec != REC_A;
+#pragma unroll
+        for (int s = 0; s < XL_NS; s++) {
+            nA[s] = wa ? sb[s] : nA[s];
+            nB[s] = wb ? sb[s] : nB[s];
+        }
+        if (IN && !(llo <= t && t <= lhi))
             for (int q = first(1); q <= L - 1; q += W) {
                 S x, d0, d1, b0, b1;
                 if (int_term(q, t, x, d0, d1, b0, b1)) D += x;
             }
-        if (ob) D += xb;
         return seg_sum<S, W>(D);
     }
-    __device__ void shift(bool) const {}
+    // the piece's end becomes the next piece's start (balances; with d: the interval's end)
+    __device__ __forceinline__ void shift(bool d) const {
+        sdA = d ? sdB : sdA;
+#pragma unroll
+        for (int s = 0; s < XL_NS; s++) nA[s] = nB[s];
+    }
 
-    __device__ bool next_bp(S tp, S sgn, S &t) const {
+    __device__ __forceinline__ bool next_bp(S tp, S sgn, S &t) const {
         bool have = false;
         S best = 0;
         auto cand = [&](S b) {
@@ -318,30 +393,23 @@ struct FXPair {
                 have = true;
             }
         };
-        for (int e = 0; e < 2; e++) {
-            if (!(e == 0 ? owns_a() : owns_b())) continue;
-            const int ta = tau(e == 0 ? 0 : L);
-            const S c = sc(e);
-            for (int kq = tb.kk_off[slot]; kq < tb.kk_off[slot + 1]; kq++) {
-                XKinkD<S> k;
-                if (!kink(e, kq, k)) continue;
-                cand(-k.Q / k.g);
-                if (k.ak != S(0)) {
-                    const S s1 = k.ak * k.g, off = k.ak * k.Q;
-                    cand((S(0) - k.y0 - S(0)) / s1);
-                    cand((S(1) - k.y0 - S(0)) / s1);
-                    cand((S(0) - k.y0 - off) / s1);
-                    cand((S(1) - k.y0 - off) / s1);
-                }
+#pragma unroll
+        for (int s = 0; s < XL_KS; s++) {
+            if (!(kok[s] & 1)) continue;
+            cand(-kQ[s] / kg[s]);
+            if (kok[s] & 2) {
+                const S s1 = ka[s] * kg[s], off = ka[s] * kQ[s];
+                cand((S(0) - ky[s] - S(0)) / s1);
+                cand((S(1) - ky[s] - S(0)) / s1);
+                cand((S(0) - ky[s] - off) / s1);
+                cand((S(1) - ky[s] - off) / s1);
             }
-            for (int l = tb.ln_off[slot]; l < tb.ln_off[slot + 1]; l++) {
-                const S df = dfl(c, l);
-                if (df == S(0)) continue;
-                const int ln = tb.ln_line[l];
-                const S f0 = Fv(ln, ta), se = m->ls[ln];
-                cand((se - f0) / df);
-                cand((-se - f0) / df);
-            }
+        }
+#pragma unroll
+        for (int s = 0; s < XL_LS; s++) {
+            if (ld[s] == S(0)) continue;
+            cand((ls_[s] - lf[s]) / ld[s]);
+            cand((-ls_[s] - lf[s]) / ld[s]);
         }
         // the clamp points of the rising instants a+1 … a+L (cached while the walk moves forward)
         if (!(bp_sgn == sgn && sgn * (tp - bp_tp) >= 0 && (!bp_have || sgn * (bp_next - tp) > 0))) {
@@ -370,9 +438,10 @@ struct FXPair {
         return seg_best<S, W>(have, best, sgn, t);
     }
 
-    // the nearest sign change beyond x, strictly inside (a, b), of the storage secondaries (stor)
-    // or of the balances, from their values at a and b (linear in between)
-    __device__ bool next_root(bool stor, S a, S b, S x, S sgn, S &xr) const {
+    // the nearest sign change beyond x, strictly inside (a, b), of the storage secondaries (stor,
+    // from d recorded at a and b) or of the balances (recorded at the piece's ends a and b); the
+    // interior steps evaluate theirs at a and b
+    __device__ __forceinline__ bool next_root(bool stor, S a, S b, S x, S sgn, S &xr) const {
         bool have = false;
         S best = 0;
         auto test = [&](S A, S B) {
@@ -384,26 +453,17 @@ struct FXPair {
                 }
             }
         };
-        for (int e = 0; e < 2; e++) {
-            if (!(e == 0 ? owns_a() : owns_b())) continue;
-            const int q = e == 0 ? 0 : L;
-            S dAa, dBa, dAb, dBb, jAa, jBa, jAb, jBb;
-            sterm(0, q, a, dAa, jAa);
-            sterm(1, q, a, dBa, jBa);
-            sterm(0, q, b, dAb, jAb);
-            sterm(1, q, b, dBb, jBb);
-            if (stor) {
+        if (stor) {
+            if (su >= 0) {
 #pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    test(sec(0, k, dAa), sec(0, k, dAb));
-                    test(sec(1, k, dBa), sec(1, k, dBb));
-                }
-            } else {
-                for (int p = tb.nd_off[slot]; p < tb.nd_off[slot + 1]; p++)
-                    test(node_S(e, p, a, jAa, jBa), node_S(e, p, b, jAb, jBb));
+                for (int k = 0; k < 3; k++) test(sec(su, k, sdA), sec(su, k, sdB));
             }
+        } else {
+#pragma unroll
+            for (int s = 0; s < XL_NS; s++)
+                if (nn_[s] >= 0) test(nA[s], nB[s]);
         }
-        if (!(llo <= a && a <= lhi && llo <= b && b <= lhi))
+        if (IN && !(llo <= a && a <= lhi && llo <= b && b <= lhi))
             for (int q = first(1); q <= L - 1; q += W) {
                 if (inside(0, q, a) && inside(0, q, b) && inside(1, q, a) && inside(1, q, b)) continue;
                 S xx, dAa, dBa, bAa, bBa, dAb, dBb, bAb, bBb;
@@ -425,92 +485,73 @@ struct FXPair {
 
     // log Jacobian Σ (|S| − 1) log|P(t)/P| of the kinks made (FP64), |S| the node's dispatchable
     // units with |κ| ≥ κmin: the gas unit, so every term is 0 in this system
-    __device__ double logjac(S t, bool &any) const {
+    __device__ __forceinline__ double logjac(S t, bool &any) const {
         double lj = 0.0;
         int cnt = 0;
-        for (int e = 0; e < 2; e++) {
-            if (!(e == 0 ? owns_a() : owns_b())) continue;
-            for (int kq = tb.kk_off[slot]; kq < tb.kk_off[slot + 1]; kq++) {
-                XKinkD<S> k;
-                if (!kink(e, kq, k)) continue;
-                cnt++;
-                const double nm = kink_nm(*m, tb.kk_node[kq], kmin);
-                lj += nm * log(fabs((fmax((double)k.Q + (double)k.g * (double)t, 0.0) + (double)k.e) / (double)k.P));
-            }
+#pragma unroll
+        for (int s = 0; s < XL_KS; s++) {
+            if (!(kok[s] & 1)) continue;
+            cnt++;
+            const double nm = kink_nm(*m, kn[s], kmin);
+            lj += nm * log(fabs((fmax((double)kQ[s] + (double)kg[s] * (double)t, 0.0) + (double)ke[s]) / (double)kP[s]));
         }
         any = seg_sum<double, W>((double)cnt) > 0;
         return seg_sum<double, W>(lj);
     }
 
-    // commit at t ≠ 0: the end steps (node by node: the balance and the leak's gas read only that
-    // node's values and the flows, which are written after all nodes; then flows and angles), the
-    // interior balances, then — after every lane has read the instants — the rising instants
-    __device__ void commit(S t, S *r_w, S *F_w, S *g_w, S *th_w, S *yA_w, S *yB_w) const {
-        for (int e = 0; e < 2; e++) {
-            if (!(e == 0 ? owns_a() : owns_b())) continue;
-            const int q = e == 0 ? 0 : L, ta = tau(q);
-            const S c = sc(e);
-            S d, djA, djB;
-            sterm(0, q, t, d, djA);
-            sterm(1, q, t, d, djB);
-            for (int p = tb.nd_off[slot]; p < tb.nd_off[slot + 1]; p++) {
-                const int n = tb.nd_node[p], ik = tb.nd_kk[p];
-                const S S1 = node_S(e, p, t, djA, djB);
-                bool wy = false;
-                S ynew = 0;
-                if (ik >= 0) {
-                    XKinkD<S> k;
-                    if (kink(e, tb.kk_off[slot] + ik, k) && k.ak != S(0)) {
-                        ynew = k.y0 + k.ak * kphi(k.Q, k.g, t);
-                        wy = true;
-                    }
-                }
-                r_w[((size_t)n * T + ta) * R] = S1;
-                if (wy) g_w[((size_t)n * T + ta) * R] = ynew;
-            }
-            for (int l = tb.ln_off[slot]; l < tb.ln_off[slot + 1]; l++) {
-                const S df = dfl(c, l);
-                if (df == S(0)) continue;
-                S &F1 = F_w[((size_t)tb.ln_line[l] * T + ta) * R];
-                F1 = F1 + t * df;
-            }
-            for (int k = tb.th_off[slot]; k < tb.th_off[slot + 1]; k++) {
-                const S a = thv(c, tb.th_val[k]);
-                if (a == S(0)) continue;
-                S &th = th_w[((size_t)tb.th_node[k] * T + ta) * R];
-                th = th + t * a;
-            }
+    // commit at t ≠ 0: the end steps' balances, leak gas, flows and angles, the interior balances,
+    // then — after every lane has read the instants — the rising instants
+    __device__ __forceinline__ void commit(S t, S *r_w, S *F_w, S *g_w, S *th_w, S *yA_w, S *yB_w) const {
+        S sd, sb[XL_NS];
+        items(t, sd, sb);
+        const int tA = tau(0), tB = tau(L);
+#pragma unroll
+        for (int s = 0; s < XL_NS; s++)
+            if (nn_[s] >= 0) r_w[((size_t)nn_[s] * T + (lane + 64 * s >= Nn ? tB : tA)) * R] = sb[s];
+#pragma unroll
+        for (int s = 0; s < XL_KS; s++)
+            if (kok[s] & 2)
+                g_w[((size_t)kn[s] * T + (lane + 64 * s >= K ? tB : tA)) * R] = ky[s] + ka[s] * kphi(kQ[s], kg[s], t);
+#pragma unroll
+        for (int s = 0; s < XL_LS; s++)
+            if (ld[s] != S(0)) F_w[((size_t)ll[s] * T + (lane + 64 * s >= Lw ? tB : tA)) * R] = lf[s] + t * ld[s];
+        const int nth = tb.th_off[slot + 1] - tb.th_off[slot];
+        for (int i = lane; i < 2 * nth; i += W) {
+            const int e = i >= nth ? 1 : 0, k = tb.th_off[slot] + (i - e * nth);
+            const S a = thv(sc(e), tb.th_val[k]);
+            if (a == S(0)) continue;
+            S &th = th_w[((size_t)tb.th_node[k] * T + (e == 0 ? tA : tB)) * R];
+            th = th + t * a;
         }
-        for (int q = first(1); q <= L - 1; q += W) {
+        for (int q = first(1); IN && q <= L - 1; q += W) {
             S x, dA, dB, bA, bB;
             if (!int_term(q, t, x, dA, dB, bA, bB)) continue;
             const int ta = tau(q);
             r_w[((size_t)nd0 * T + ta) * R] = bA;
             if (nd1 != nd0) r_w[((size_t)nd1 * T + ta) * R] = bB;
         }
-        if (W > 1) {
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        }
+        sync();
         for (int q = first(1); q <= L; q += W) {
             const int ta = tau(q);
             yA_w[(size_t)ta * R] = y(0, ta) + cu0 * t;
             yB_w[(size_t)ta * R] = y(1, ta) + cu1 * t;
         }
+        sync();
     }
 };
 
 // Set up the pair move of units A = (sa, na) and B = (sb, nb) at template slot `slot`, instants
-// a+1 … a+L, for replica j seen by `lane` of its segment; false when it is not made (κ_A or κ_B
-// below 1: the same in every lane and replica).
-template <typename S, int W>
-__device__ bool fx_prepare(FXPair<S, W> &f, const DevModel<S> &m, const SweepState<S> &st, const SxTab<S> &tb, int sa,
-                           int na, int sb, int nb, int slot, int a, int L, int j, int lane, S kmin) {
+// a+1 … a+L, for replica j on the calling wave (lane `lane`); false when it is not made (κ_A or κ_B
+// below 1: the same in every lane and replica). Every lane reads its items' values at t = 0.
+template <typename S, bool IN>
+__device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &m, const SweepState<S> &st,
+                                           const SxTab<S> &tb, XlLds<S> *sh, int sa, int na, int sb, int nb,
+                                           int slot, int a, int L, int j, int lane, S kmin) {
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
     f.m = &m;
     f.tb = tb;
+    f.sh = sh;
     f.T = T;
     f.R = R;
     f.lane = lane;
@@ -539,14 +580,14 @@ __device__ bool fx_prepare(FXPair<S, W> &f, const DevModel<S> &m, const SweepSta
     f.gp = st.v + plane(N, T, R) + j;
     // the lane's window over its interior steps
     S lo = -INFINITY, hi = INFINITY;
-    for (int q = f.first(1); q <= L - 1; q += W)
+    for (int q = f.first(1); IN && q <= L - 1; q += FXLane<S, IN>::W)
         for (int u = 0; u < 2; u++) {
             const int ta = f.tau(q);
-            const S y0 = f.y(u, ta), y1 = f.y(u, f.next(ta)), c = u == 0 ? f.cu0 : f.cu1;
+            const S y0 = f.y(u, ta), y1 = f.y(u, f.next(ta)), c = u == 0 ? +f.cu0 : +f.cu1;
             if (S(0) <= y0 && y0 <= S(1) && S(0) <= y1 && y1 <= S(1)) {
                 S l0, h0, l1, h1;
-                FXPair<S, W>::box(y0, c, l0, h0);
-                FXPair<S, W>::box(y1, c, l1, h1);
+                FXLane<S, IN>::box(y0, c, l0, h0);
+                FXLane<S, IN>::box(y1, c, l1, h1);
                 lo = fmax(lo, fmax(l0, l1));
                 hi = fmin(hi, fmin(h0, h1));
             } else {
@@ -556,6 +597,135 @@ __device__ bool fx_prepare(FXPair<S, W> &f, const DevModel<S> &m, const SweepSta
         }
     f.llo = lo;
     f.lhi = hi;
+    const int k0 = tb.kk_off[slot], l0 = tb.ln_off[slot], n0 = tb.nd_off[slot];
+    f.K = tb.kk_off[slot + 1] - k0;
+    f.Lw = tb.ln_off[slot + 1] - l0;
+    f.Nn = tb.nd_off[slot + 1] - n0;
+    const int t0 = tb.nd_l[n0];
+    f.NLp = tb.nd_l[n0 + f.Nn] - t0;
+    const int tA = f.tau(0), tB = f.tau(L);
+    // kinks: `prorata!` (qadj = 0) of the gas unit of a leak node. (Each slot's values are set as
+    // locals and stored once: stores to different fields under branches would be merged into stores
+    // through selected addresses.)
+#pragma unroll
+    for (int s = 0; s < XL_KS; s++) {
+        const int i = lane + 64 * s;
+        int n = 0, okv = 0;
+        S Q = 0, g = 0, ak = 0, w = 0, c0 = 0, p0 = 0, P = 0, e0 = 0;
+        if (i < 2 * f.K) {
+            const int e = i >= f.K ? 1 : 0, kq = k0 + (i - e * f.K), ta = e == 0 ? tA : tB;
+            n = tb.kk_node[kq];
+            g = f.sc(e) * tb.kk_g[kq];
+            const S kp = m.kc[ESM_KGAS * N + n];
+            if (g != S(0) && fabs(kp) >= kmin) {
+                w = f.gv(n, ta);
+                P = S(0) + kp * w;
+                const S Qs = S(0) + kp * clampv(w, S(0), S(1));
+                if (!(fabs(P) < S(1e-6))) {
+                    Q = f.rv(n, ta) + Qs;
+                    e0 = P - fmax(Q, S(0));
+                    const S sig = kp * w / P;
+                    ak = sig == S(0) ? S(0) : sig / kp;
+                    okv = ak != S(0) ? 3 : 1;
+                    if (okv & 2) f.gen(n, w, c0, p0);
+                }
+            }
+        }
+        f.kn[s] = n;
+        f.kok[s] = okv;
+        f.kQ[s] = Q;
+        f.kg[s] = g;
+        f.ka[s] = ak;
+        f.ky[s] = w;
+        f.kc0[s] = c0;
+        f.kp0[s] = p0;
+        f.kP[s] = P;
+        f.ke[s] = e0;
+    }
+#pragma unroll
+    for (int s = 0; s < XL_LS; s++) {
+        const int i = lane + 64 * s;
+        int ln = 0;
+        S f0 = 0, df = 0, se = 0, p0 = 0;
+        if (i < 2 * f.Lw) {
+            const int e = i >= f.Lw ? 1 : 0, l = l0 + (i - e * f.Lw);
+            ln = tb.ln_line[l];
+            df = f.dfl(f.sc(e), l);
+            if (df != S(0)) {
+                f0 = f.Fv(ln, e == 0 ? tA : tB);
+                se = m.ls[ln];
+                p0 = fmax(fabs(f0) - se, S(0));
+            }
+        }
+        f.ll[s] = ln;
+        f.lf[s] = f0;
+        f.ld[s] = df;
+        f.ls_[s] = se;
+        f.lp0[s] = p0;
+    }
+    {   // storage: lane 2e + u holds unit u's term at end e
+        int u = -1, q = 0;
+        S y0 = 0, y1 = 0, c0 = 0, j0 = 0, dy0 = 0, dy1 = 0;
+        if (lane < 4) {
+            const int e = lane >> 1;
+            u = lane & 1;
+            q = e == 0 ? 0 : L;
+            const int ta = e == 0 ? tA : tB;
+            const S cu = u == 0 ? +f.cu0 : +f.cu1;
+            y0 = f.y(u, ta);
+            y1 = f.y(u, f.next(ta));
+            dy0 = q >= 1 ? cu : S(0);
+            dy1 = q <= L - 1 ? cu : S(0);
+            S d0;
+            c0 = soc_cost_p(f.par(u), m.dt, m.cv, u == 0 ? +f.kst0 : +f.kst1, u == 0 ? +f.w0 : +f.w1, y0, y1, d0, j0);
+        }
+        f.su = u;
+        f.sq = q;
+        f.sy0 = y0;
+        f.sy1 = y1;
+        f.sc0 = c0;
+        f.sj0 = j0;
+        f.sdy0 = dy0;
+        f.sdy1 = dy1;
+        f.sdA = 0;
+        f.sdB = 0;
+    }
+#pragma unroll
+    for (int s = 0; s < XL_NS; s++) {
+        const int i = lane + 64 * s;
+        int n = -1, off = 0, cnt = 0, kx = -1;
+        S S0 = 0, ph = 0;
+        if (i < 2 * f.Nn) {
+            const int e = i >= f.Nn ? 1 : 0, p = n0 + (i - e * f.Nn);
+            n = tb.nd_node[p];
+            off = e * f.NLp + (tb.nd_l[p] - t0);
+            cnt = tb.nd_l[p + 1] - tb.nd_l[p];
+            kx = tb.nd_kk[p] >= 0 ? XL_XK + e * f.K + tb.nd_kk[p] : -1;
+            S0 = f.rv(n, e == 0 ? tA : tB);
+            ph = phibal(S0, m.c_shed, m.c_dump);
+            const S c = f.sc(e);
+            for (int q2 = tb.nd_l[p]; q2 < tb.nd_l[p + 1]; q2++) {
+                const int pos = tb.nl_pos[q2];
+                sh->nl[off + (q2 - tb.nd_l[p])] =
+                    f.dfl(c, l0 + pos) == S(0) ? -1 : ((XL_XL + e * f.Lw + pos) | (tb.nl_sgn[q2] > 0 ? 0 : XL_NEG));
+            }
+        }
+        f.nn_[s] = n;
+        f.no[s] = off;
+        f.nc[s] = cnt;
+        f.nk[s] = kx;
+        f.nS0[s] = S0;
+        f.nph[s] = ph;
+        f.nA[s] = 0;
+        f.nB[s] = 0;
+    }
+    // which kinks are made with a nonzero share (the balances take their gas change)
+#pragma unroll
+    for (int s = 0; s < XL_KS; s++) sh->xv[XL_XK + lane + 64 * s] = (f.kok[s] & 2) ? S(1) : S(0);
+    f.sync();
+#pragma unroll
+    for (int s = 0; s < XL_NS; s++) f.nk[s] = (f.nk[s] >= 0 && sh->xv[f.nk[s]] != S(0)) ? f.nk[s] : -1;
+    f.sync();
     return true;
 }
 

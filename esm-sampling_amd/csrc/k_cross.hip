@@ -12,8 +12,9 @@
 // draws, so a captured batch graph does not change with them): lane 0 draws and shuffles, the 64
 // lanes test the rounds of a move's first fit in parallel. k_sxlong runs one workgroup of
 // SXL_WAVES waves per replica: round by round, wave w takes moves w, w + SXL_WAVES, … of the
-// round, one move per wave with the pair move's 64-lane form (cross_move.h); a workgroup barrier
-// separates the rounds. Draws (Philox under the sweep's key and counter c): move m of the unit
+// round, one move per wave (cross_move.h); a workgroup barrier
+// separates the rounds. The move's lanes hold the items of its end steps' template and its
+// interior steps (FXLane). Draws (Philox under the sweep's key and counter c): move m of the unit
 // with index u among the units of the pairs (u, m, 9420, c) — partner u01(w0, w1), a = ⌊w2 T⌋,
 // L = 1 + ⌊w3 (T − 1)⌋; the shuffle (i, 0, 9421, c); the i-th move of the round-ordered list:
 // line draw (replica, i, 0x5C000000, c), Jacobian test (replica, i, 0x7C000000, c).
@@ -140,6 +141,7 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
                                                            const int *buf, int cap, uint32_t k0, uint32_t k1,
                                                            uint32_t ctr, unsigned long long *stats, S beta, S maxnats,
                                                            S kmin) {
+    __shared__ XlLds<S> shm[SXL_WAVES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = blockIdx.x;
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
@@ -150,8 +152,12 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
         for (int i = roff[r] + wave; i < roff[r + 1]; i += SXL_WAVES) {
             const int *o = mv + 8 * i;
             const int sa = o[0], na = o[1], sb = o[2], nb = o[3], slot = o[4], a = o[5], L = o[6];
-            FXPair<S, 64> f;
-            if (!fx_prepare(f, m, st, sx.tb, sa, na, sb, nb, slot, a, L, j, lane, kmin)) continue;
+            if (!xl_fits(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
+                if (lane == 0) out.add(2, false, 0);
+                continue;
+            }
+            FXLane<S, true> f;
+            if (!xl_prepare(f, m, st, sx.tb, &shm[wave], sa, na, sb, nb, slot, a, L, j, lane, kmin)) continue;
             double u1, u2;
             uniforms2(k0, k1, j, (uint32_t)i, 0x5C000000u, ctr, u1, u2);
             int status = 0;

@@ -202,29 +202,41 @@ __global__ void __launch_bounds__(256, min_waves<S>(3)) k_rand_spair(DevModel<S>
 // ---------------------------------------------------------------------------------------------
 // sxpair: the cross-node storage pair at one instant (`prep_sxpair!`, src/storage_cross.jl:147-167):
 // unit A uniform among the units of the pairs (selection word 0), the partner by `sx_partner`
-// (words 2–3), the instant τ (steps τm and τ); one thread per (step, replica) with the pair move's
-// one-lane form
+// (words 2–3), the instant τ (steps τm and τ). One wave per (step, replica), grid order replica
+// fastest (the waves of one step read the same cache lines); its lanes hold the items of the
+// pair's template (FXLane), and it runs the step's sxpair moves in draw order.
 // ---------------------------------------------------------------------------------------------
 
-template <typename S, bool UNI>
-__global__ void __launch_bounds__(256) k_rand_sxpair(DevModel<S> m, SweepState<S> st, RandArgs a, SxArgs<S> sx,
-                                                     S beta, S maxnats, S kmin) {
-    const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+constexpr int SXP_WAVES = 4;   // waves per workgroup of the sxpair launch
+
+template <typename S>
+__global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxpair(DevModel<S> m, SweepState<S> st, RandArgs a,
+                                                                SxArgs<S> sx, S beta, S maxnats, S kmin) {
+    __shared__ XlLds<S> shm[SXP_WAVES];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const size_t g = (size_t)blockIdx.x * SXP_WAVES + wv;
     MoveOut out;
-    int j;
-    if (slot_of(a, g, m.R, j) && sx.nsx > 0) {
-        const int tau = step_of<UNI>(a, g, a.rp), N = m.N, T = m.T;
-        const int cnt = uniform<UNI>(a.counts[(size_t)a.w * T + tau]);
+    if (g < (size_t)a.ntau * m.R && sx.nsx > 0) {
+        const int N = m.N, T = m.T;
+        const int i = (int)(g / m.R), j = (int)(g % m.R);
+        const int tau = __builtin_amdgcn_readfirstlane(a.perm ? a.perm[i] : a.tau0 + i * a.tstride);
+        const int cnt = __builtin_amdgcn_readfirstlane(a.counts[(size_t)a.w * T + tau]);
         for (int k = 0; k < cnt; k++) {
             uint32_t s[4];
             selection(a, k, tau, s);
-            uniform4<UNI>(s);
+            uniform4<true>(s);
             const int ua = sx.units[pick(s[0], (uint32_t)sx.nsx)], sa = ua / N, na = ua % N;
             if (m.kc[(ESM_KSTOR + sa) * N + na] * m.hours[ua] / m.dt < S(1)) continue;
             int slot, sb, nb;
             if (!sx_partner(m, sx, sa, na, u01(s[2], s[3]), slot, sb, nb)) continue;
-            FXPair<S, 1> f;
-            if (!fx_prepare(f, m, st, sx.tb, sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j, 0, kmin)) continue;
+            if (!xl_fits(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
+                if (lane == 0) out.add(2, false, 0);
+                continue;
+            }
+            FXLane<S, false> f;
+            if (!xl_prepare(f, m, st, sx.tb, &shm[wv], sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j, lane,
+                            kmin))
+                continue;
             double u1, u2;
             line_uniforms(a, j, k, tau, u1, u2);
             int status = 0;
@@ -242,7 +254,7 @@ __global__ void __launch_bounds__(256) k_rand_sxpair(DevModel<S> m, SweepState<S
                     }
                 }
             }
-            out.add(status, t != 0, f.nev);
+            if (lane == 0) out.add(status, t != 0, f.nev);
             if (t != 0) {
                 const size_t TR = (size_t)T * m.R;
                 f.commit(t, st.r + j, st.F + j, st.v + plane(N, T, m.R) + j, st.v + j,
@@ -1151,7 +1163,9 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
             b.stats = h->d_stats + (size_t)ESM_FAM_SXPAIR * ESM_NSTAT * STAT_SLOTS;
             const SxArgs<S> sx = sx_args<S>(h);
             ProfScope p(h, ESM_KERN_SXPAIR);
-            RLAUNCH((k_rand_sxpair<S, true>), (k_rand_sxpair<S, false>), m, st, b, sx, beta, mxn, kmin);
+            const size_t nwaves = (size_t)a.ntau * R;
+            hipLaunchKernelGGL((k_rand_sxpair<S>), dim3((unsigned)((nwaves + SXP_WAVES - 1) / SXP_WAVES)),
+                               dim3(64 * SXP_WAVES), 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());
         }
     }

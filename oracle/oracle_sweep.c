@@ -2801,9 +2801,8 @@ int orc_slong_apply(const esm_model *m, int R, double *v, const double *k, doubl
  * L drawn on 1 … T−1 is `sxlong` (prep_sxlong! :252-350, mv_eval! :370-427, commit_sxlong!
  * :430-461, storage_cross_phase! :501-544). Every unit of this system can charge, so the loss
  * share λ of sx_loss (:108) is 0: the losses go into the residuals of the units' nodes.
- * The arithmetic is the device's: step q (0 … L: step a+q mod T) belongs to lane q mod W (W = 1
- * for sxpair, 64 for sxlong); a lane sums its steps' terms in increasing q and the lanes' sums
- * are combined by the W-lane butterfly.
+ * The arithmetic is the device's, one wave per move and replica: the end steps' items on lanes by
+ * kind (xm_ends), the interior steps on lanes q mod 64, the 64-lane butterfly (xm_eval).
  * ============================================================================================ */
 
 typedef struct {
@@ -2891,33 +2890,47 @@ static double xm_stor(const XMove *f, int u, int q, double t, double *d1, double
     *dj = j1 - j0;
     return c1 - c0;
 }
-/* end step e at t (eval_cached! order: kinked gas entries, lines, storage A and B, balances of the
- * template's nodes); d[2] the units' d, bal (may be NULL) the balances */
-static double xm_end(const XMove *f, int e, double t, double *d, double *bal) {
+/* the items of a move's end steps: per end the kinked gas entries of the leak nodes, the lines, storage
+ * A and B, the balances of the template's nodes (eval_cached! order); end a then end b */
+static int xm_nper(const XMove *f) {
+    const Tables *tb = f->tb;
+    int sl = f->slot;
+    return (tb->sx_kk_off.a[sl + 1] - tb->sx_kk_off.a[sl]) + (tb->sx_ln_off.a[sl + 1] - tb->sx_ln_off.a[sl]) + 2 +
+           (tb->sx_nd_off.a[sl + 1] - tb->sx_nd_off.a[sl]);
+}
+/* end step e at t: the terms of its kinks tk, lines tl, storage terms ts and balances tn, each with a
+ * flag (hk, hl, hn: whether it counts; the storage terms always do); d[2] the units' d, bal (may be
+ * NULL) the balances */
+static void xm_end(const XMove *f, int e, double t, double *tk, int *hk, double *tl, int *hl, double *ts, double *tn,
+                   double *d, double *bal) {
     Sys *S = f->S;
     const esm_model *m = S->m;
     const Tables *tb = f->tb;
     int j = f->j, sl = f->slot, tau = xm_tau(f, e == 0 ? 0 : f->L);
-    double c = xm_scale(f, e), D = 0.0, dj[2];
+    double c = xm_scale(f, e), dj[2];
     int k0 = tb->sx_kk_off.a[sl], k1 = tb->sx_kk_off.a[sl + 1], l0 = tb->sx_ln_off.a[sl], l1 = tb->sx_ln_off.a[sl + 1];
     int n0 = tb->sx_nd_off.a[sl], n1 = tb->sx_nd_off.a[sl + 1];
     for (int kq = k0; kq < k1; kq++) {
         XKink k;
+        hk[kq - k0] = 0;
         if (!xm_kink(f, e, kq, &k) || k.ak == 0.0) continue;
         double c0, p0, c1, p1;
         int n = tb->sx_kk_node.a[kq];
         xm_gen(f, n, k.y0, &c0, &p0);
         xm_gen(f, n, k.y0 + k.ak * sm_phi(k.Q, k.g, t), &c1, &p1);
-        D += c1 - c0;
+        tk[kq - k0] = c1 - c0;
+        hk[kq - k0] = 1;
     }
     for (int l = l0; l < l1; l++) {
         double df = xm_df(f, c, l);
+        hl[l - l0] = 0;
         if (df == 0.0) continue;
         int ln = tb->sx_ln_line.a[l];
         double f0 = AT(S->F, ln, tau), se = m->line_s[ln], f1 = f0 + t * df;
-        D += m->c_viol * m->dt * (fmax(fabs(f1) - se, 0.0) - fmax(fabs(f0) - se, 0.0));
+        tl[l - l0] = m->c_viol * m->dt * (fmax(fabs(f1) - se, 0.0) - fmax(fabs(f0) - se, 0.0));
+        hl[l - l0] = 1;
     }
-    for (int u = 0; u < 2; u++) D += xm_stor(f, u, e == 0 ? 0 : f->L, t, &d[u], &dj[u]);
+    for (int u = 0; u < 2; u++) ts[u] = xm_stor(f, u, e == 0 ? 0 : f->L, t, &d[u], &dj[u]);
     for (int p = n0; p < n1; p++) {
         int n = tb->sx_nd_node.a[p], ik = tb->sx_nd_kk.a[p];
         double S0 = AT(S->r, n, tau), dr = 0.0;
@@ -2940,10 +2953,43 @@ static double xm_end(const XMove *f, int e, double t, double *d, double *bal) {
         if (n == f->nd[0]) dr += dj[0];
         if (n == f->nd[1]) dr += dj[1];
         double S1 = S0 + dr;
-        D += m->dt * (phibal(m, S1) - phibal(m, S0));
+        tn[p - n0] = m->dt * (phibal(m, S1) - phibal(m, S0));
         if (bal) bal[p - n0] = S1;
     }
-    return D;
+}
+/* the end steps' terms on the lanes of the device's wave: kinks (item e·K + k), lines (e·Lw + l),
+ * storage terms (2e + u), balances (e·Nn + p), each kind over both ends on lanes item mod 64, a lane
+ * summing its kinks, lines, storage terms and balances in increasing item index */
+static void xm_ends(const XMove *f, double t, double *part) {
+    const Tables *tb = f->tb;
+    int sl = f->slot;
+    int K = tb->sx_kk_off.a[sl + 1] - tb->sx_kk_off.a[sl], Lw = tb->sx_ln_off.a[sl + 1] - tb->sx_ln_off.a[sl];
+    int Nn = tb->sx_nd_off.a[sl + 1] - tb->sx_nd_off.a[sl];
+    double *tk = (double *)malloc(sizeof(double) * 2 * (K + Lw + Nn + 3)), *tl = tk + 2 * (K + 1),
+           *tn = tl + 2 * (Lw + 1), ts[4], d[2];
+    int *hk = (int *)malloc(sizeof(int) * 2 * (K + Lw + 2)), *hl = hk + 2 * (K + 1);
+    for (int e = 0; e < 2; e++)
+        xm_end(f, e, t, tk + e * K, hk + e * K, tl + e * Lw, hl + e * Lw, ts + 2 * e, tn + e * Nn, d, NULL);
+    for (int i = 0; i < 2 * K; i++)
+        if (hk[i]) part[i % 64] += tk[i];
+    for (int i = 0; i < 2 * Lw; i++)
+        if (hl[i]) part[i % 64] += tl[i];
+    for (int i = 0; i < 4; i++) part[i] += ts[i];
+    for (int i = 0; i < 2 * Nn; i++) part[i % 64] += tn[i];
+    free(tk);
+    free(hk);
+}
+/* the units' d and the template's balances at end step e at t */
+static void xm_end_bal(const XMove *f, int e, double t, double *d, double *bal) {
+    const Tables *tb = f->tb;
+    int sl = f->slot;
+    int K = tb->sx_kk_off.a[sl + 1] - tb->sx_kk_off.a[sl], Lw = tb->sx_ln_off.a[sl + 1] - tb->sx_ln_off.a[sl];
+    int Nn = tb->sx_nd_off.a[sl + 1] - tb->sx_nd_off.a[sl];
+    double *tk = (double *)malloc(sizeof(double) * (K + Lw + Nn + 3)), *tl = tk + K + 1, *tn = tl + Lw + 1, ts[2];
+    int *hk = (int *)malloc(sizeof(int) * (K + Lw + 2)), *hl = hk + K + 1;
+    xm_end(f, e, t, tk, hk, tl, hl, ts, tn, d, bal);
+    free(tk);
+    free(hk);
 }
 /* interior step q at t: ΔC; d[2] (the value at t = 0 for a unit inside its box), the balances of
  * the units' nodes bal[2] (one when they share a node); *moved 0 when both units are inside */
@@ -2983,21 +3029,20 @@ static void xm_butterfly(double *part, int W) {
         memcpy(part, nx, sizeof(double) * W);
     }
 }
+/* ΔC(t) in the device's arithmetic: a wave per (move, replica); the end steps' items on lanes
+ * i mod 64, then the interior steps q = 1 … L−1 on lanes q mod 64, each lane in increasing order;
+ * the lanes' sums combined by the 64-lane butterfly */
 static double xm_eval(const XMove *f, double t) {
     double part[64] = {0}, d[2], bal[2];
     g_nev++;
-    for (int q = 0; q <= f->L; q++) {
-        double x;
-        if (q == 0 || q == f->L) {
-            x = xm_end(f, q == 0 ? 0 : 1, t, d, NULL);
-        } else {
-            int mv;
-            x = xm_int(f, q, t, d, bal, &mv);
-            if (!mv) continue;
-        }
-        part[q % f->W] += x;
+    xm_ends(f, t, part);
+    for (int q = 1; q <= f->L - 1; q++) {
+        int mv;
+        double x = xm_int(f, q, t, d, bal, &mv);
+        if (!mv) continue;
+        part[q % 64] += x;
     }
-    xm_butterfly(part, f->W);
+    xm_butterfly(part, 64);
     return part[0];
 }
 static int xm_next_bp(const XMove *f, double tp, double sgn, double *t) {
@@ -3071,8 +3116,8 @@ static int xm_next_root(const XMove *f, int stor, double a, double b, double x, 
         }                                                                                                  \
     } while (0)
     for (int e = 0; e < 2; e++) {
-        xm_end(f, e, a, da, ba);
-        xm_end(f, e, b, db, bb);
+        xm_end_bal(f, e, a, da, ba);
+        xm_end_bal(f, e, b, db, bb);
         if (stor) {
             for (int u = 0; u < 2; u++)
                 for (int k = 0; k < 3; k++) XTEST(xm_sec(f, u, k, da[u]), xm_sec(f, u, k, db[u]));
@@ -3162,10 +3207,11 @@ static double xm_logjac(const XMove *f, double t, int *any) {
             if (!xm_kink(f, e, kq, &k)) continue;
             cnt++;
             double nm = (double)(gas_units(f->S, f->tb->sx_kk_node.a[kq]) - 1);
-            part[(e == 0 ? 0 : f->L) % f->W] += nm * log(fabs((fmax(k.Q + k.g * t, 0.0) + k.e) / k.P));
+            part[(e * (f->tb->sx_kk_off.a[sl + 1] - f->tb->sx_kk_off.a[sl]) + kq - f->tb->sx_kk_off.a[sl]) % 64] +=
+                nm * log(fabs((fmax(k.Q + k.g * t, 0.0) + k.e) / k.P));
         }
     *any = cnt > 0;
-    xm_butterfly(part, f->W);
+    xm_butterfly(part, 64);
     return part[0];
 }
 /* commit_sxlong! / commit_block!: every new value from the state before the move, then the writes */
@@ -3182,7 +3228,7 @@ static void xm_commit(const XMove *f, double t) {
     int *imv = (int *)calloc(L + 1, sizeof(int));
     double d[2];
     for (int e = 0; e < 2; e++) {
-        xm_end(f, e, t, d, bal + e * nn);
+        xm_end_bal(f, e, t, d, bal + e * nn);
         for (int q = 0; q < nk; q++) {
             XKink k;
             if (xm_kink(f, e, k0 + q, &k) && k.ak != 0.0) {
