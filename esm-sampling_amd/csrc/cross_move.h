@@ -110,44 +110,47 @@ struct XKinkD {
 // are combined by the 64-lane butterfly.
 // ---------------------------------------------------------------------------------------------
 
-constexpr int XL_KS = 1;                 // kink slots per lane: at most 64 kinks of both ends
-constexpr int XL_LS = 3;                 // line slots: at most 192 lines of both ends
-constexpr int XL_NS = 2;                 // node slots: at most 128 balances of both ends
-constexpr int XL_NLMAX = 512;            // node–line entries of a move (both ends)
+// A move is owned by a segment of W lanes (W = 32 or 64: the segments of a wave are replicas).
+constexpr int XL_KS = 1;                 // kink slots per lane: at most W kinks of both ends
+constexpr int XL_LS = 3;                 // line slots: at most 3 W lines of both ends
+constexpr int XL_NS = 2;                 // node slots: at most 2 W balances of both ends
 constexpr int XL_NEG = 1 << 20;          // sign flag of a node–line entry (the node is the line's bus1)
-constexpr int XL_XK = 0, XL_XL = 64 * XL_KS, XL_XS = XL_XL + 64 * XL_LS, XL_NX = XL_XS + 4;
+__host__ __device__ constexpr int xl_nlmax(int W) { return 8 * W; }   // node–line entries (both ends)
+__host__ __device__ constexpr int xl_nx(int W) { return (XL_KS + XL_LS) * W + 4; }
 
-// the LDS of one wave's pair move
-template <typename S>
+// the LDS of one segment's pair move
+template <typename S, int W>
 struct XlLds {
-    S xv[XL_NX];        // at the pass's t: each kink's, line's and storage term's change of the balances
-    int nl[XL_NLMAX];   // per end, the template's node–line entries: line's xv index | XL_NEG, −1 (none)
+    S xv[xl_nx(W)];          // at the pass's t: each kink's, line's and storage term's change of the balances
+    int nl[xl_nlmax(W)];     // per end, the template's node–line entries: line's xv index | XL_NEG, −1 (none)
 };
 
-// whether the template of slot `slot` fits the lane form (checked on the host for every slot)
-template <typename S>
-__host__ __device__ __forceinline__ bool xl_fits_counts(int K, int Lw, int Nn, int NLp) {
-    return 2 * K <= 64 * XL_KS && 2 * Lw <= 64 * XL_LS && 2 * Nn <= 64 * XL_NS && 2 * NLp <= XL_NLMAX;
+// whether a template of K kinks, Lw lines, Nn balances and NLp node–line entries per end fits a
+// segment of W lanes (checked on the host for every slot)
+__host__ __device__ __forceinline__ bool xl_fits_counts(int W, int K, int Lw, int Nn, int NLp) {
+    return 2 * K <= W * XL_KS && 2 * Lw <= W * XL_LS && 2 * Nn <= W * XL_NS && 2 * NLp <= xl_nlmax(W);
 }
-template <typename S>
+template <typename S, int W>
 __device__ __forceinline__ bool xl_fits(const SxTab<S> &tb, int slot) {
-    return xl_fits_counts<S>(tb.kk_off[slot + 1] - tb.kk_off[slot], tb.ln_off[slot + 1] - tb.ln_off[slot],
-                             tb.nd_off[slot + 1] - tb.nd_off[slot],
-                             tb.nd_l[tb.nd_off[slot + 1]] - tb.nd_l[tb.nd_off[slot]]);
+    return xl_fits_counts(W, tb.kk_off[slot + 1] - tb.kk_off[slot], tb.ln_off[slot + 1] - tb.ln_off[slot],
+                          tb.nd_off[slot + 1] - tb.nd_off[slot],
+                          tb.nd_l[tb.nd_off[slot + 1]] - tb.nd_l[tb.nd_off[slot]]);
 }
 
-// IN: the move has interior steps (sxlong); sxpair (L = 1) compiles without them
-template <typename S, bool IN>
+// IN: the move has interior steps (sxlong); sxpair (L = 1) compiles without them. WW: the
+// segment's lanes.
+template <typename S, bool IN, int WW>
 struct FXLane {
     enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
     static constexpr bool CACHED = true;
-    static constexpr int W = 64;
+    static constexpr int W = WW;
+    static constexpr int XK = 0, XL = XL_KS * W, XS = XL + XL_LS * W;
     const DevModel<S> *m;
     SxTab<S> tb;
-    XlLds<S> *sh;
+    XlLds<S, W> *sh;
     const S *ye0, *ye1;    // the units' state-of-charge planes at (node, ·, replica), stride R
     const S *rp, *Fp, *gp; // residual, flow and gas planes at (·, ·, replica)
-    int T, R, lane, L, a0, slot;
+    int T, R, lane, L, a0, slot;   // lane: the lane within the segment
     int nd0, nd1;
     int K, Lw, Nn, NLp;
     S cu0, cu1, kst0, kst1, w0, w1, kap, kmin;
@@ -231,7 +234,7 @@ struct FXLane {
                 D += c1 - kc0[s];
                 x = p1 - kp0[s];
             }
-            sh->xv[XL_XK + lane + 64 * s] = x;
+            sh->xv[XK + lane + W * s] = x;
         }
 #pragma unroll
         for (int s = 0; s < XL_LS; s++) {
@@ -241,7 +244,7 @@ struct FXLane {
                 D += m->cv * m->dt * (fmax(fabs(f1) - ls_[s], S(0)) - lp0[s]);
                 x = (lf[s] + t * ld[s]) - lf[s];
             }
-            sh->xv[XL_XL + lane + 64 * s] = x;
+            sh->xv[XL + lane + W * s] = x;
         }
         sd = 0;
         if (su >= 0) {
@@ -249,14 +252,14 @@ struct FXLane {
             const S c1 = soc_cost_p(par(su), m->dt, m->cv, su == 0 ? +kst0 : +kst1, su == 0 ? +w0 : +w1,
                                     sy0 + t * sdy0, sy1 + t * sdy1, sd, j1);
             D += c1 - sc0;
-            sh->xv[XL_XS + lane] = j1 - sj0;
+            sh->xv[XS + lane] = j1 - sj0;
         }
         sync();
 #pragma unroll
         for (int s = 0; s < XL_NS; s++) {
             sb[s] = 0;
             if (nn_[s] < 0) continue;
-            const int e = lane + 64 * s >= Nn ? 1 : 0;
+            const int e = lane + W * s >= Nn ? 1 : 0;
             S dr = 0;
             if (nk[s] >= 0) dr -= sh->xv[nk[s]];
             for (int q = no[s]; q < no[s] + nc[s]; q++) {
@@ -265,8 +268,8 @@ struct FXLane {
                 const S dd = sh->xv[en & (XL_NEG - 1)];
                 dr += (en & XL_NEG) ? -dd : dd;
             }
-            if (nn_[s] == nd0) dr += sh->xv[XL_XS + 2 * e];
-            if (nn_[s] == nd1) dr += sh->xv[XL_XS + 2 * e + 1];
+            if (nn_[s] == nd0) dr += sh->xv[XS + 2 * e];
+            if (nn_[s] == nd1) dr += sh->xv[XS + 2 * e + 1];
             const S S1 = nS0[s] + dr;
             D += m->dt * (phibal(S1, m->c_shed, m->c_dump) - nph[s]);
             sb[s] = S1;
@@ -507,14 +510,14 @@ struct FXLane {
         const int tA = tau(0), tB = tau(L);
 #pragma unroll
         for (int s = 0; s < XL_NS; s++)
-            if (nn_[s] >= 0) r_w[((size_t)nn_[s] * T + (lane + 64 * s >= Nn ? tB : tA)) * R] = sb[s];
+            if (nn_[s] >= 0) r_w[((size_t)nn_[s] * T + (lane + W * s >= Nn ? tB : tA)) * R] = sb[s];
 #pragma unroll
         for (int s = 0; s < XL_KS; s++)
             if (kok[s] & 2)
-                g_w[((size_t)kn[s] * T + (lane + 64 * s >= K ? tB : tA)) * R] = ky[s] + ka[s] * kphi(kQ[s], kg[s], t);
+                g_w[((size_t)kn[s] * T + (lane + W * s >= K ? tB : tA)) * R] = ky[s] + ka[s] * kphi(kQ[s], kg[s], t);
 #pragma unroll
         for (int s = 0; s < XL_LS; s++)
-            if (ld[s] != S(0)) F_w[((size_t)ll[s] * T + (lane + 64 * s >= Lw ? tB : tA)) * R] = lf[s] + t * ld[s];
+            if (ld[s] != S(0)) F_w[((size_t)ll[s] * T + (lane + W * s >= Lw ? tB : tA)) * R] = lf[s] + t * ld[s];
         const int nth = tb.th_off[slot + 1] - tb.th_off[slot];
         for (int i = lane; i < 2 * nth; i += W) {
             const int e = i >= nth ? 1 : 0, k = tb.th_off[slot] + (i - e * nth);
@@ -543,9 +546,9 @@ struct FXLane {
 // Set up the pair move of units A = (sa, na) and B = (sb, nb) at template slot `slot`, instants
 // a+1 … a+L, for replica j on the calling wave (lane `lane`); false when it is not made (κ_A or κ_B
 // below 1: the same in every lane and replica). Every lane reads its items' values at t = 0.
-template <typename S, bool IN>
-__device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &m, const SweepState<S> &st,
-                                           const SxTab<S> &tb, XlLds<S> *sh, int sa, int na, int sb, int nb,
+template <typename S, bool IN, int W>
+__device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S> &m, const SweepState<S> &st,
+                                           const SxTab<S> &tb, XlLds<S, W> *sh, int sa, int na, int sb, int nb,
                                            int slot, int a, int L, int j, int lane, S kmin) {
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
@@ -580,14 +583,14 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &
     f.gp = st.v + plane(N, T, R) + j;
     // the lane's window over its interior steps
     S lo = -INFINITY, hi = INFINITY;
-    for (int q = f.first(1); IN && q <= L - 1; q += FXLane<S, IN>::W)
+    for (int q = f.first(1); IN && q <= L - 1; q += W)
         for (int u = 0; u < 2; u++) {
             const int ta = f.tau(q);
             const S y0 = f.y(u, ta), y1 = f.y(u, f.next(ta)), c = u == 0 ? +f.cu0 : +f.cu1;
             if (S(0) <= y0 && y0 <= S(1) && S(0) <= y1 && y1 <= S(1)) {
                 S l0, h0, l1, h1;
-                FXLane<S, IN>::box(y0, c, l0, h0);
-                FXLane<S, IN>::box(y1, c, l1, h1);
+                FXLane<S, IN, W>::box(y0, c, l0, h0);
+                FXLane<S, IN, W>::box(y1, c, l1, h1);
                 lo = fmax(lo, fmax(l0, l1));
                 hi = fmin(hi, fmin(h0, h1));
             } else {
@@ -609,7 +612,7 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &
     // through selected addresses.)
 #pragma unroll
     for (int s = 0; s < XL_KS; s++) {
-        const int i = lane + 64 * s;
+        const int i = lane + W * s;
         int n = 0, okv = 0;
         S Q = 0, g = 0, ak = 0, w = 0, c0 = 0, p0 = 0, P = 0, e0 = 0;
         if (i < 2 * f.K) {
@@ -644,7 +647,7 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &
     }
 #pragma unroll
     for (int s = 0; s < XL_LS; s++) {
-        const int i = lane + 64 * s;
+        const int i = lane + W * s;
         int ln = 0;
         S f0 = 0, df = 0, se = 0, p0 = 0;
         if (i < 2 * f.Lw) {
@@ -692,7 +695,7 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &
     }
 #pragma unroll
     for (int s = 0; s < XL_NS; s++) {
-        const int i = lane + 64 * s;
+        const int i = lane + W * s;
         int n = -1, off = 0, cnt = 0, kx = -1;
         S S0 = 0, ph = 0;
         if (i < 2 * f.Nn) {
@@ -700,14 +703,14 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &
             n = tb.nd_node[p];
             off = e * f.NLp + (tb.nd_l[p] - t0);
             cnt = tb.nd_l[p + 1] - tb.nd_l[p];
-            kx = tb.nd_kk[p] >= 0 ? XL_XK + e * f.K + tb.nd_kk[p] : -1;
+            kx = tb.nd_kk[p] >= 0 ? FXLane<S, IN, W>::XK + e * f.K + tb.nd_kk[p] : -1;
             S0 = f.rv(n, e == 0 ? tA : tB);
             ph = phibal(S0, m.c_shed, m.c_dump);
             const S c = f.sc(e);
             for (int q2 = tb.nd_l[p]; q2 < tb.nd_l[p + 1]; q2++) {
                 const int pos = tb.nl_pos[q2];
                 sh->nl[off + (q2 - tb.nd_l[p])] =
-                    f.dfl(c, l0 + pos) == S(0) ? -1 : ((XL_XL + e * f.Lw + pos) | (tb.nl_sgn[q2] > 0 ? 0 : XL_NEG));
+                    f.dfl(c, l0 + pos) == S(0) ? -1 : ((FXLane<S, IN, W>::XL + e * f.Lw + pos) | (tb.nl_sgn[q2] > 0 ? 0 : XL_NEG));
             }
         }
         f.nn_[s] = n;
@@ -721,7 +724,7 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN> &f, const DevModel<S> &
     }
     // which kinks are made with a nonzero share (the balances take their gas change)
 #pragma unroll
-    for (int s = 0; s < XL_KS; s++) sh->xv[XL_XK + lane + 64 * s] = (f.kok[s] & 2) ? S(1) : S(0);
+    for (int s = 0; s < XL_KS; s++) sh->xv[FXLane<S, IN, W>::XK + lane + W * s] = (f.kok[s] & 2) ? S(1) : S(0);
     f.sync();
 #pragma unroll
     for (int s = 0; s < XL_NS; s++) f.nk[s] = (f.nk[s] >= 0 && sh->xv[f.nk[s]] != S(0)) ? f.nk[s] : -1;

@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
                                                            const int *buf, int cap, uint32_t k0, uint32_t k1,
                                                            uint32_t ctr, unsigned long long *stats, S beta, S maxnats,
                                                            S kmin) {
-    __shared__ XlLds<S> shm[SXL_WAVES];
+    __shared__ XlLds<S, 64> shm[SXL_WAVES];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = blockIdx.x;
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
@@ -152,11 +152,11 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
         for (int i = roff[r] + wave; i < roff[r + 1]; i += SXL_WAVES) {
             const int *o = mv + 8 * i;
             const int sa = o[0], na = o[1], sb = o[2], nb = o[3], slot = o[4], a = o[5], L = o[6];
-            if (!xl_fits(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
+            if (!xl_fits<S, 64>(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
                 if (lane == 0) out.add(2, false, 0);
                 continue;
             }
-            FXLane<S, true> f;
+            FXLane<S, true, 64> f;
             if (!xl_prepare(f, m, st, sx.tb, &shm[wave], sa, na, sb, nb, slot, a, L, j, lane, kmin)) continue;
             double u1, u2;
             uniforms2(k0, k1, j, (uint32_t)i, 0x5C000000u, ctr, u1, u2);

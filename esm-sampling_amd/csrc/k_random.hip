@@ -209,19 +209,23 @@ __global__ void __launch_bounds__(256, min_waves<S>(3)) k_rand_spair(DevModel<S>
 
 constexpr int SXP_WAVES = 4;   // waves per workgroup of the sxpair launch
 
-template <typename S>
-__global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxpair(DevModel<S> m, SweepState<S> st, RandArgs a,
-                                                                SxArgs<S> sx, S beta, S maxnats, S kmin) {
-    __shared__ XlLds<S> shm[SXP_WAVES];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const size_t g = (size_t)blockIdx.x * SXP_WAVES + wv;
+// W: the lanes of a move (32: two replicas per wave, when every template fits; else 64)
+template <typename S, int W>
+__global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxpair(DevModel<S> m, SweepState<S> st,
+                                                                                     RandArgs a, SxArgs<S> sx, S beta,
+                                                                                     S maxnats, S kmin) {
+    constexpr int SEGS = 64 / W;
+    __shared__ XlLds<S, W> shm[SXP_WAVES][SEGS];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, seg = lane / W, sub = lane % W;
+    const size_t g = (size_t)blockIdx.x * SXP_WAVES + wv;        // the wave: (step, group of SEGS replicas)
+    const int ngrp = (m.R + SEGS - 1) / SEGS;
     MoveOut out;
-    if (g < (size_t)a.ntau * m.R && sx.nsx > 0) {
+    if (g < (size_t)a.ntau * ngrp && sx.nsx > 0) {
         const int N = m.N, T = m.T;
-        const int i = (int)(g / m.R), j = (int)(g % m.R);
+        const int i = (int)(g / ngrp), j = (int)(g % ngrp) * SEGS + seg;
         const int tau = __builtin_amdgcn_readfirstlane(a.perm ? a.perm[i] : a.tau0 + i * a.tstride);
         const int cnt = __builtin_amdgcn_readfirstlane(a.counts[(size_t)a.w * T + tau]);
-        for (int k = 0; k < cnt; k++) {
+        for (int k = 0; k < cnt && j < m.R; k++) {
             uint32_t s[4];
             selection(a, k, tau, s);
             uniform4<true>(s);
@@ -229,13 +233,13 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxp
             if (m.kc[(ESM_KSTOR + sa) * N + na] * m.hours[ua] / m.dt < S(1)) continue;
             int slot, sb, nb;
             if (!sx_partner(m, sx, sa, na, u01(s[2], s[3]), slot, sb, nb)) continue;
-            if (!xl_fits(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
-                if (lane == 0) out.add(2, false, 0);
+            if (!xl_fits<S, W>(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
+                if (sub == 0) out.add(2, false, 0);
                 continue;
             }
-            FXLane<S, false> f;
-            if (!xl_prepare(f, m, st, sx.tb, &shm[wv], sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j, lane,
-                            kmin))
+            FXLane<S, false, W> f;
+            if (!xl_prepare(f, m, st, sx.tb, &shm[wv][seg], sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j,
+                            sub, kmin))
                 continue;
             double u1, u2;
             line_uniforms(a, j, k, tau, u1, u2);
@@ -254,7 +258,7 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxp
                     }
                 }
             }
-            if (lane == 0) out.add(status, t != 0, f.nev);
+            if (sub == 0) out.add(status, t != 0, f.nev);
             if (t != 0) {
                 const size_t TR = (size_t)T * m.R;
                 f.commit(t, st.r + j, st.F + j, st.v + plane(N, T, m.R) + j, st.v + j,
@@ -264,6 +268,22 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxp
         }
     }
     add_counts(a.stats, out);
+}
+
+// the lanes of an sxpair move: 32 when every pair template of the system fits 32 lanes, else 64;
+// 0 when some template is too large for the lane form
+static int sxpair_width(const esm_handle *h) {
+    const MoveTablesHost &t = h->mt;
+    int w = 32;
+    for (size_t sl = 0; sl + 1 < t.sx_kk_off.size(); sl++) {
+        const int K = t.sx_kk_off[sl + 1] - t.sx_kk_off[sl], Lw = t.sx_ln_off[sl + 1] - t.sx_ln_off[sl];
+        const int Nn = t.sx_nd_off[sl + 1] - t.sx_nd_off[sl];
+        const int NLp = t.sx_nd_l[t.sx_nd_off[sl + 1]] - t.sx_nd_l[t.sx_nd_off[sl]];
+        if (xl_fits_counts(w, K, Lw, Nn, NLp)) continue;
+        if (w == 32 && xl_fits_counts(64, K, Lw, Nn, NLp)) w = 64;
+        else return 0;
+    }
+    return w;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1163,9 +1183,15 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
             b.stats = h->d_stats + (size_t)ESM_FAM_SXPAIR * ESM_NSTAT * STAT_SLOTS;
             const SxArgs<S> sx = sx_args<S>(h);
             ProfScope p(h, ESM_KERN_SXPAIR);
-            const size_t nwaves = (size_t)a.ntau * R;
-            hipLaunchKernelGGL((k_rand_sxpair<S>), dim3((unsigned)((nwaves + SXP_WAVES - 1) / SXP_WAVES)),
-                               dim3(64 * SXP_WAVES), 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            const int wx = sxpair_width(h);
+            if (wx == 0) {
+                set_error("storage pairs: a pair template is larger than the lane form holds");
+                return ESM_EINVAL;
+            }
+            const size_t nwaves = (size_t)a.ntau * ((R + 64 / wx - 1) / (64 / wx));
+            const dim3 grid((unsigned)((nwaves + SXP_WAVES - 1) / SXP_WAVES)), blk(64 * SXP_WAVES);
+            if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair<S, 32>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            else hipLaunchKernelGGL((k_rand_sxpair<S, 64>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());
         }
     }

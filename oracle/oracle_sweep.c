@@ -2957,9 +2957,9 @@ static void xm_end(const XMove *f, int e, double t, double *tk, int *hk, double 
         if (bal) bal[p - n0] = S1;
     }
 }
-/* the end steps' terms on the lanes of the device's wave: kinks (item e·K + k), lines (e·Lw + l),
- * storage terms (2e + u), balances (e·Nn + p), each kind over both ends on lanes item mod 64, a lane
- * summing its kinks, lines, storage terms and balances in increasing item index */
+/* the end steps' terms on the W lanes of the device's segment: kinks (item e·K + k), lines
+ * (e·Lw + l), storage terms (2e + u), balances (e·Nn + p), each kind over both ends on lanes item
+ * mod W, a lane summing its kinks, lines, storage terms and balances in increasing item index */
 static void xm_ends(const XMove *f, double t, double *part) {
     const Tables *tb = f->tb;
     int sl = f->slot;
@@ -2970,12 +2970,13 @@ static void xm_ends(const XMove *f, double t, double *part) {
     int *hk = (int *)malloc(sizeof(int) * 2 * (K + Lw + 2)), *hl = hk + 2 * (K + 1);
     for (int e = 0; e < 2; e++)
         xm_end(f, e, t, tk + e * K, hk + e * K, tl + e * Lw, hl + e * Lw, ts + 2 * e, tn + e * Nn, d, NULL);
+    int W = f->W;
     for (int i = 0; i < 2 * K; i++)
-        if (hk[i]) part[i % 64] += tk[i];
+        if (hk[i]) part[i % W] += tk[i];
     for (int i = 0; i < 2 * Lw; i++)
-        if (hl[i]) part[i % 64] += tl[i];
-    for (int i = 0; i < 4; i++) part[i] += ts[i];
-    for (int i = 0; i < 2 * Nn; i++) part[i % 64] += tn[i];
+        if (hl[i]) part[i % W] += tl[i];
+    for (int i = 0; i < 4; i++) part[i % W] += ts[i];
+    for (int i = 0; i < 2 * Nn; i++) part[i % W] += tn[i];
     free(tk);
     free(hk);
 }
@@ -3029,9 +3030,9 @@ static void xm_butterfly(double *part, int W) {
         memcpy(part, nx, sizeof(double) * W);
     }
 }
-/* ΔC(t) in the device's arithmetic: a wave per (move, replica); the end steps' items on lanes
- * i mod 64, then the interior steps q = 1 … L−1 on lanes q mod 64, each lane in increasing order;
- * the lanes' sums combined by the 64-lane butterfly */
+/* ΔC(t) in the device's arithmetic: a segment of W lanes per (move, replica); the end steps' items
+ * on lanes by kind (xm_ends), then the interior steps q = 1 … L−1 on lanes q mod W, each lane in
+ * increasing order; the lanes' sums combined by the W-lane butterfly */
 static double xm_eval(const XMove *f, double t) {
     double part[64] = {0}, d[2], bal[2];
     g_nev++;
@@ -3040,10 +3041,27 @@ static double xm_eval(const XMove *f, double t) {
         int mv;
         double x = xm_int(f, q, t, d, bal, &mv);
         if (!mv) continue;
-        part[q % 64] += x;
+        part[q % f->W] += x;
     }
-    xm_butterfly(part, 64);
+    xm_butterfly(part, f->W);
     return part[0];
+}
+/* the lanes of the device's sxpair segment: 32 when every pair template fits 32 lanes (kinks ≤ W/2,
+ * lines ≤ 3W/2, balances ≤ W, node–line entries ≤ 4W per end), else 64 (the device refuses a system
+ * whose templates do not fit 64 lanes) */
+static int xm_pair_width(const Tables *tb) {
+    int w = 32;
+    for (int sl = 0; sl + 1 < tb->sx_kk_off.n; sl++) {
+        int K = tb->sx_kk_off.a[sl + 1] - tb->sx_kk_off.a[sl], Lw = tb->sx_ln_off.a[sl + 1] - tb->sx_ln_off.a[sl];
+        int Nn = tb->sx_nd_off.a[sl + 1] - tb->sx_nd_off.a[sl];
+        int NLp = tb->sx_nd_l.a[tb->sx_nd_off.a[sl + 1]] - tb->sx_nd_l.a[tb->sx_nd_off.a[sl]];
+        for (;;) {
+            if (2 * K <= w && 2 * Lw <= 3 * w && 2 * Nn <= 2 * w && 2 * NLp <= 8 * w) break;
+            if (w == 64) return 64;
+            w = 64;
+        }
+    }
+    return w;
 }
 static int xm_next_bp(const XMove *f, double tp, double sgn, double *t) {
     Sys *S = f->S;
@@ -3207,11 +3225,11 @@ static double xm_logjac(const XMove *f, double t, int *any) {
             if (!xm_kink(f, e, kq, &k)) continue;
             cnt++;
             double nm = (double)(gas_units(f->S, f->tb->sx_kk_node.a[kq]) - 1);
-            part[(e * (f->tb->sx_kk_off.a[sl + 1] - f->tb->sx_kk_off.a[sl]) + kq - f->tb->sx_kk_off.a[sl]) % 64] +=
+            part[(e * (f->tb->sx_kk_off.a[sl + 1] - f->tb->sx_kk_off.a[sl]) + kq - f->tb->sx_kk_off.a[sl]) % f->W] +=
                 nm * log(fabs((fmax(k.Q + k.g * t, 0.0) + k.e) / k.P));
         }
     *any = cnt > 0;
-    xm_butterfly(part, 64);
+    xm_butterfly(part, f->W);
     return part[0];
 }
 /* commit_sxlong! / commit_block!: every new value from the state before the move, then the writes */
@@ -3346,7 +3364,7 @@ static void sxpair_move(Sys *S, const Tables *tb, int j, int tau, const uint32_t
     if (S->kc[(ESM_KSTOR + sa) * N + na] * m->stor_hours[ua] / m->dt < 1.0) return;
     if (!xm_partner(S, tb, hmin, sa, na, u01(s[2], s[3]), &slot, &sb, &nb)) return;
     XMove f;
-    if (!xm_prepare(&f, S, tb, j, sa, na, sb, nb, slot, (tau - 1 + T) % T, 1, 1)) return;
+    if (!xm_prepare(&f, S, tb, j, sa, na, sb, nb, slot, (tau - 1 + T) % T, 1, xm_pair_width(tb))) return;
     xm_move(&f, useg, uoff, umh, maxnats, stats);
 }
 /* the sxlong moves of a sweep (the same for every replica): per unit of the pairs with κ ≥ 1 and
