@@ -114,21 +114,19 @@ struct XKinkD {
 constexpr int XL_KS = 1;                 // kink slots per lane: at most W kinks of both ends
 constexpr int XL_LS = 3;                 // line slots: at most 3 W lines of both ends
 constexpr int XL_NS = 2;                 // node slots: at most 2 W balances of both ends
-constexpr int XL_NEG = 1 << 20;          // sign flag of a node–line entry (the node is the line's bus1)
-__host__ __device__ constexpr int xl_nlmax(int W) { return 8 * W; }   // node–line entries (both ends)
-__host__ __device__ constexpr int xl_nx(int W) { return (XL_KS + XL_LS) * W + 4; }
+__host__ __device__ constexpr int xl_nx(int W) { return XL_KS * W + 4; }
 
 // the LDS of one segment's pair move
 template <typename S, int W>
 struct XlLds {
-    S xv[xl_nx(W)];          // at the pass's t: each kink's, line's and storage term's change of the balances
-    int nl[xl_nlmax(W)];     // per end, the template's node–line entries: line's xv index | XL_NEG, −1 (none)
+    S xv[xl_nx(W)];          // at the pass's t: each kink's and storage term's change of the balances
 };
 
-// whether a template of K kinks, Lw lines, Nn balances and NLp node–line entries per end fits a
-// segment of W lanes (checked on the host for every slot)
+// whether a template of K kinks, Lw lines and Nn balances per end fits a segment of W lanes
+// (checked on the host for every slot)
 __host__ __device__ __forceinline__ bool xl_fits_counts(int W, int K, int Lw, int Nn, int NLp) {
-    return 2 * K <= W * XL_KS && 2 * Lw <= W * XL_LS && 2 * Nn <= W * XL_NS && 2 * NLp <= xl_nlmax(W);
+    (void)NLp;
+    return 2 * K <= W * XL_KS && 2 * Lw <= W * XL_LS && 2 * Nn <= W * XL_NS;
 }
 template <typename S, int W>
 __device__ __forceinline__ bool xl_fits(const SxTab<S> &tb, int slot) {
@@ -144,7 +142,7 @@ struct FXLane {
     enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
     static constexpr bool CACHED = true;
     static constexpr int W = WW;
-    static constexpr int XK = 0, XL = XL_KS * W, XS = XL + XL_LS * W;
+    static constexpr int XK = 0, XS = XL_KS * W;
     const DevModel<S> *m;
     SxTab<S> tb;
     XlLds<S, W> *sh;
@@ -159,17 +157,20 @@ struct FXLane {
     // kinks (item e·K + k): node, made (bit 0), ak ≠ 0 (bit 1); Q, g, ak, y0, c0, p0, P, e
     int kn[XL_KS], kok[XL_KS];
     S kQ[XL_KS], kg[XL_KS], ka[XL_KS], ky[XL_KS], kc0[XL_KS], kp0[XL_KS], kP[XL_KS], ke[XL_KS];
-    // lines (item e·Lw + l): line; f0, df (0: dropped), limit, max(|f0| − limit, 0)
+    S kb[XL_KS][5];       // its primary breakpoints (the kink, the clamp points of the gas; NaN: none)
+    // lines (item e·Lw + l): line; f0, df (0: dropped), limit, max(|f0| − limit, 0); the breakpoints
+    // where |f| reaches the limit
     int ll[XL_LS];
-    S lf[XL_LS], ld[XL_LS], ls_[XL_LS], lp0[XL_LS];
+    S lf[XL_LS], ld[XL_LS], ls_[XL_LS], lp0[XL_LS], lb0[XL_LS], lb1[XL_LS];
     // storage terms (item 2e + u on lane 2e + u): y0, y1, c0, j0, dye, dye1; d at the interval's ends
     int su, sq;
     S sy0, sy1, sc0, sj0, sdy0, sdy1;
     mutable S sdA, sdB;
-    // balances (item e·Nn + p): node, first node–line entry, entries, xv index of its kink (−1:
-    // none made); S0, φ(S0); the balance at the piece's ends
-    int nn_[XL_NS], no[XL_NS], nc[XL_NS], nk[XL_NS];
-    S nS0[XL_NS], nph[XL_NS];
+    // balances (item e·Nn + p): node, xv index of its kink (−1: none made); S0, φ(S0), the change of
+    // the node's inflow per unit t over its template lines (Σ ± df in line order); the balance at
+    // the piece's ends
+    int nn_[XL_NS], nk[XL_NS];
+    S nS0[XL_NS], nph[XL_NS], ncf[XL_NS];
     mutable S nA[XL_NS], nB[XL_NS];
     mutable int nev = 0;
     // the lane's nearest clamp point of its rising instants beyond the walk's last point (the
@@ -238,13 +239,10 @@ struct FXLane {
         }
 #pragma unroll
         for (int s = 0; s < XL_LS; s++) {
-            S x = 0;
             if (ld[s] != S(0)) {
                 const S f1 = lf[s] + t * ld[s];
                 D += m->cv * m->dt * (fmax(fabs(f1) - ls_[s], S(0)) - lp0[s]);
-                x = (lf[s] + t * ld[s]) - lf[s];
             }
-            sh->xv[XL + lane + W * s] = x;
         }
         sd = 0;
         if (su >= 0) {
@@ -262,12 +260,7 @@ struct FXLane {
             const int e = lane + W * s >= Nn ? 1 : 0;
             S dr = 0;
             if (nk[s] >= 0) dr -= sh->xv[nk[s]];
-            for (int q = no[s]; q < no[s] + nc[s]; q++) {
-                const int en = sh->nl[q];
-                if (en < 0) continue;
-                const S dd = sh->xv[en & (XL_NEG - 1)];
-                dr += (en & XL_NEG) ? -dd : dd;
-            }
+            dr += t * ncf[s];
             if (nn_[s] == nd0) dr += sh->xv[XS + 2 * e];
             if (nn_[s] == nd1) dr += sh->xv[XS + 2 * e + 1];
             const S S1 = nS0[s] + dr;
@@ -397,22 +390,13 @@ struct FXLane {
             }
         };
 #pragma unroll
-        for (int s = 0; s < XL_KS; s++) {
-            if (!(kok[s] & 1)) continue;
-            cand(-kQ[s] / kg[s]);
-            if (kok[s] & 2) {
-                const S s1 = ka[s] * kg[s], off = ka[s] * kQ[s];
-                cand((S(0) - ky[s] - S(0)) / s1);
-                cand((S(1) - ky[s] - S(0)) / s1);
-                cand((S(0) - ky[s] - off) / s1);
-                cand((S(1) - ky[s] - off) / s1);
-            }
-        }
+        for (int s = 0; s < XL_KS; s++)
+#pragma unroll
+            for (int c = 0; c < 5; c++) cand(kb[s][c]);   // NaN (none) fails isfinite
 #pragma unroll
         for (int s = 0; s < XL_LS; s++) {
-            if (ld[s] == S(0)) continue;
-            cand((ls_[s] - lf[s]) / ld[s]);
-            cand((-ls_[s] - lf[s]) / ld[s]);
+            cand(lb0[s]);
+            cand(lb1[s]);
         }
         // the clamp points of the rising instants a+1 … a+L (cached while the walk moves forward)
         if (!(bp_sgn == sgn && sgn * (tp - bp_tp) >= 0 && (!bp_have || sgn * (bp_next - tp) > 0))) {
@@ -604,8 +588,7 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
     f.K = tb.kk_off[slot + 1] - k0;
     f.Lw = tb.ln_off[slot + 1] - l0;
     f.Nn = tb.nd_off[slot + 1] - n0;
-    const int t0 = tb.nd_l[n0];
-    f.NLp = tb.nd_l[n0 + f.Nn] - t0;
+    f.NLp = tb.nd_l[n0 + f.Nn] - tb.nd_l[n0];
     const int tA = f.tau(0), tB = f.tau(L);
     // kinks: `prorata!` (qadj = 0) of the gas unit of a leak node. (Each slot's values are set as
     // locals and stored once: stores to different fields under branches would be merged into stores
@@ -634,6 +617,22 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
                 }
             }
         }
+        S b0 = S(NAN), b1 = S(NAN), b2 = S(NAN), b3 = S(NAN), b4 = S(NAN);
+        if (okv & 1) {
+            b0 = -Q / g;
+            if (okv & 2) {
+                const S s1 = ak * g, off = ak * Q;
+                b1 = (S(0) - w - S(0)) / s1;
+                b2 = (S(1) - w - S(0)) / s1;
+                b3 = (S(0) - w - off) / s1;
+                b4 = (S(1) - w - off) / s1;
+            }
+        }
+        f.kb[s][0] = b0;
+        f.kb[s][1] = b1;
+        f.kb[s][2] = b2;
+        f.kb[s][3] = b3;
+        f.kb[s][4] = b4;
         f.kn[s] = n;
         f.kok[s] = okv;
         f.kQ[s] = Q;
@@ -665,6 +664,8 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
         f.ld[s] = df;
         f.ls_[s] = se;
         f.lp0[s] = p0;
+        f.lb0[s] = df != S(0) ? (se - f0) / df : S(NAN);
+        f.lb1[s] = df != S(0) ? (-se - f0) / df : S(NAN);
     }
     {   // storage: lane 2e + u holds unit u's term at end e
         int u = -1, q = 0;
@@ -696,29 +697,26 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
 #pragma unroll
     for (int s = 0; s < XL_NS; s++) {
         const int i = lane + W * s;
-        int n = -1, off = 0, cnt = 0, kx = -1;
-        S S0 = 0, ph = 0;
+        int n = -1, kx = -1;
+        S S0 = 0, ph = 0, cf = 0;
         if (i < 2 * f.Nn) {
             const int e = i >= f.Nn ? 1 : 0, p = n0 + (i - e * f.Nn);
             n = tb.nd_node[p];
-            off = e * f.NLp + (tb.nd_l[p] - t0);
-            cnt = tb.nd_l[p + 1] - tb.nd_l[p];
             kx = tb.nd_kk[p] >= 0 ? FXLane<S, IN, W>::XK + e * f.K + tb.nd_kk[p] : -1;
             S0 = f.rv(n, e == 0 ? tA : tB);
             ph = phibal(S0, m.c_shed, m.c_dump);
             const S c = f.sc(e);
             for (int q2 = tb.nd_l[p]; q2 < tb.nd_l[p + 1]; q2++) {
-                const int pos = tb.nl_pos[q2];
-                sh->nl[off + (q2 - tb.nd_l[p])] =
-                    f.dfl(c, l0 + pos) == S(0) ? -1 : ((FXLane<S, IN, W>::XL + e * f.Lw + pos) | (tb.nl_sgn[q2] > 0 ? 0 : XL_NEG));
+                const S df = f.dfl(c, l0 + tb.nl_pos[q2]);
+                if (df == S(0)) continue;
+                cf += tb.nl_sgn[q2] > 0 ? df : -df;
             }
         }
         f.nn_[s] = n;
-        f.no[s] = off;
-        f.nc[s] = cnt;
         f.nk[s] = kx;
         f.nS0[s] = S0;
         f.nph[s] = ph;
+        f.ncf[s] = cf;
         f.nA[s] = 0;
         f.nB[s] = 0;
     }

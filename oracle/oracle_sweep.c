@@ -2898,6 +2898,19 @@ static int xm_nper(const XMove *f) {
     return (tb->sx_kk_off.a[sl + 1] - tb->sx_kk_off.a[sl]) + (tb->sx_ln_off.a[sl + 1] - tb->sx_ln_off.a[sl]) + 2 +
            (tb->sx_nd_off.a[sl + 1] - tb->sx_nd_off.a[sl]);
 }
+/* the change of node item p's inflow per unit t at scale c: Σ ± df over its template lines (line
+ * order; + where the node is the line's bus0) */
+static double xm_inflow(const XMove *f, double c, int p) {
+    const Tables *tb = f->tb;
+    int l0 = tb->sx_ln_off.a[f->slot];
+    double cf = 0.0;
+    for (int q = tb->sx_nd_l.a[p]; q < tb->sx_nd_l.a[p + 1]; q++) {
+        double df = xm_df(f, c, l0 + tb->sx_nl_pos.a[q]);
+        if (df == 0.0) continue;
+        cf += tb->sx_nl_sgn.a[q] > 0 ? df : -df;
+    }
+    return cf;
+}
 /* end step e at t: the terms of its kinks tk, lines tl, storage terms ts and balances tn, each with a
  * flag (hk, hl, hn: whether it counts; the storage terms always do); d[2] the units' d, bal (may be
  * NULL) the balances */
@@ -2943,13 +2956,7 @@ static void xm_end(const XMove *f, int e, double t, double *tk, int *hk, double 
                 dr -= p1 - p0;
             }
         }
-        for (int q = tb->sx_nd_l.a[p]; q < tb->sx_nd_l.a[p + 1]; q++) {
-            int pos = l0 + tb->sx_nl_pos.a[q];
-            double df = xm_df(f, c, pos);
-            if (df == 0.0) continue;
-            double f0 = AT(S->F, tb->sx_ln_line.a[pos], tau), dd = (f0 + t * df) - f0;
-            dr += tb->sx_nl_sgn.a[q] > 0 ? dd : -dd;
-        }
+        dr += t * xm_inflow(f, c, p);
         if (n == f->nd[0]) dr += dj[0];
         if (n == f->nd[1]) dr += dj[1];
         double S1 = S0 + dr;
