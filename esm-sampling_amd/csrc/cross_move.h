@@ -117,9 +117,12 @@ constexpr int XL_NS = 2;                 // node slots: at most 2 W balances of 
 __host__ __device__ constexpr int xl_nx(int W) { return XL_KS * W + 4; }
 
 // the LDS of one segment's pair move
+constexpr int XL_NP = 32;   // walk points kept per side for the selection
+
 template <typename S, int W>
 struct XlLds {
     S xv[xl_nx(W)];          // at the pass's t: each kink's and storage term's change of the balances
+    S pts[2 * XL_NP * 2];    // the mass walks' points (sample_stor_pts)
 };
 
 // whether a template of K kinks, Lw lines and Nn balances per end fits a segment of W lanes

@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
             double u1, u2;
             uniforms2(k0, k1, j, (uint32_t)i, 0x5C000000u, ctr, u1, u2);
             int status = 0;
-            S t = sample_stor(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+            S t = sample_stor_pts<XL_NP>(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status, shm[wave].pts, lane);
             if (t != 0) {
                 bool any;
                 const double lj = f.logjac(t, any);

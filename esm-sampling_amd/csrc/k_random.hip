@@ -244,7 +244,8 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxp
             double u1, u2;
             line_uniforms(a, j, k, tau, u1, u2);
             int status = 0;
-            S t = sample_stor(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status);
+            S t = sample_stor_pts<XL_NP>(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status, shm[wv][seg].pts,
+                                         sub);
             if (t != 0) {
                 bool any;
                 const double lj = f.logjac(t, any);

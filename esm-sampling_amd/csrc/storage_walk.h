@@ -200,4 +200,92 @@ __device__ __forceinline__ S sample_stor(const F &f, S beta, S maxnats, S useg, 
     return t;
 }
 
+// The same with the points of the two mass walks kept (t, c in walk order, up to NP per side, in
+// the segment's LDS `pts` [2][NP][2]): the selection then reads the chosen side's points instead of
+// walking it again, and walks again only when that side had more than NP points. Same draws as
+// sample_stor: the selection sees the same points in the same order.
+template <typename S>
+struct VisSP {
+    VisS<S> v;
+    S *pts = nullptr;   // this side's [NP][2]
+    int np = 0, cap = 0, sub = 0;
+    bool rec = false;
+    __device__ __forceinline__ bool operator()(S t, S c) {
+        if (rec) {
+            if (np < cap && sub == 0) {
+                pts[2 * np] = t;
+                pts[2 * np + 1] = c;
+            }
+            np++;
+        }
+        return v(t, c);
+    }
+};
+
+template <int NP, class F, typename S>
+__device__ __forceinline__ S sample_stor_pts(const F &f, S beta, S maxnats, S useg, S uoff, int &status, S *pts,
+                                             int sub) {
+    VisSP<S> vis;
+    vis.cap = NP;
+    vis.sub = sub;
+    S Mx0 = S(-INFINITY), Mx1 = S(-INFINITY), Ms0 = 0, Ms1 = 0;
+    int n0 = 0, n1 = 0;
+#pragma nounroll
+    for (int k = 0; k < 2; k++) {
+        vis.v.mode = 0;
+        vis.v.ma = MassAcc<S>();
+        vis.rec = true;
+        vis.np = 0;
+        vis.pts = pts + 2 * NP * k;
+        const int w = walk_stor(f, beta, maxnats, k == 0 ? S(1) : S(-1), vis);
+        if (w != 1) {
+            status = w == 0 ? 1 : 2;
+            return 0;
+        }
+        if (k == 0) {
+            Mx0 = vis.v.ma.mx;
+            Ms0 = vis.v.ma.s;
+            n0 = vis.np;
+        } else {
+            Mx1 = vis.v.ma.mx;
+            Ms1 = vis.v.ma.s;
+            n1 = vis.np;
+        }
+    }
+    const S MX = fmax(Mx0, Mx1);
+    const S Mr = Ms0 * exp(Mx0 - MX), Ml = Ms1 * exp(Mx1 - MX);
+    const S u = useg * (Ml + Mr);
+    SegSel<S> ss;
+    ss.MX = MX;
+    ss.left = u <= Ml;
+    ss.target = ss.left ? Ml - u : u - Ml;
+    const int cnt = ss.left ? n1 : n0;
+    if (cnt <= NP) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const S *ps = pts + 2 * NP * (ss.left ? 1 : 0);
+        for (int i = 0; i < cnt; i++)
+            if (ss(ps[2 * i], ps[2 * i + 1])) break;
+        __builtin_amdgcn_wave_barrier();   // the next move's walks write the points again
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        vis.rec = false;
+        vis.v.mode = 1;
+        vis.v.ss = ss;
+        walk_stor(f, beta, maxnats, ss.left ? S(-1) : S(1), vis);
+        ss = vis.v.ss;
+    }
+    status = 0;
+    const S L = ss.tb - ss.ta;
+    const S g = (ss.cb - ss.ca) / L;
+    const S t = (g >= 0 || fabs(g * L) < S(1e-10)) ? ss.ta + trexp_offset(g, L, uoff) : ss.tb - trexp_offset(-g, L, uoff);
+    if (!isfinite(t)) {
+        status = 2;
+        return 0;
+    }
+    return t;
+}
+
 }  // namespace esm
