@@ -118,6 +118,21 @@ __host__ __device__ constexpr int xl_nx(int W) { return XL_KS * W + 4; }
 
 // the LDS of one segment's pair move
 constexpr int XL_NP = 32;   // walk points kept per side for the selection
+constexpr int XL_MAXCH = 16;   // window chunks per lane of the interior steps (sxlong)
+
+// interior steps of a lane per window chunk (at least 8) over a horizon of T, and the chunks
+__host__ __device__ constexpr int xl_ch(int T) {
+    return ((T + 63) / 64 + XL_MAXCH - 1) / XL_MAXCH > 8 ? ((T + 63) / 64 + XL_MAXCH - 1) / XL_MAXCH : 8;
+}
+// float bounds of a window rounded inward (a conservative filter of the exact test)
+__device__ __forceinline__ float xl_inward_lo(double x) {
+    float f = (float)x;
+    return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+__device__ __forceinline__ float xl_inward_hi(double x) {
+    float f = (float)x;
+    return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
 
 template <typename S, int W>
 struct XlLds {
@@ -157,6 +172,9 @@ struct FXLane {
     S cu0, cu1, kst0, kst1, w0, w1, kap, kmin;
     StoPar<S> par0, par1;
     S llo, lhi;            // t in [llo, lhi]: the lane's interior steps keep every instant in [0, 1]
+    // the same per chunk of ch consecutive interior steps of the lane (LDS [chunk][64], IN only)
+    const float *cwl = nullptr, *cwh = nullptr;
+    int ch = 8;
     // kinks (item e·K + k): node, made (bit 0), ak ≠ 0 (bit 1); Q, g, ak, y0, c0, p0, P, e
     int kn[XL_KS], kok[XL_KS];
     S kQ[XL_KS], kg[XL_KS], ka[XL_KS], ky[XL_KS], kc0[XL_KS], kp0[XL_KS], kP[XL_KS], ke[XL_KS];
@@ -370,9 +388,12 @@ struct FXLane {
             nB[s] = wb ? sb[s] : nB[s];
         }
         if (IN && !(llo <= t && t <= lhi))
-            for (int q = first(1); q <= L - 1; q += W) {
-                S x, d0, d1, b0, b1;
-                if (int_term(q, t, x, d0, d1, b0, b1)) D += x;
+            for (int q0 = first(1), c = 0; q0 <= L - 1; q0 += W * ch, c++) {
+                if (cwl[c * 64 + lane] <= t && t <= cwh[c * 64 + lane]) continue;
+                for (int q = q0; q <= L - 1 && q < q0 + W * ch; q += W) {
+                    S x, d0, d1, b0, b1;
+                    if (int_term(q, t, x, d0, d1, b0, b1)) D += x;
+                }
             }
         return seg_sum<S, W>(D);
     }
@@ -454,7 +475,10 @@ struct FXLane {
                 if (nn_[s] >= 0) test(nA[s], nB[s]);
         }
         if (IN && !(llo <= a && a <= lhi && llo <= b && b <= lhi))
-            for (int q = first(1); q <= L - 1; q += W) {
+            for (int q = first(1), c = 0; q <= L - 1; q += W) {
+                if ((q - first(1)) / W == (c + 1) * ch) c++;
+                const float wl = cwl[c * 64 + lane], wh = cwh[c * 64 + lane];
+                if (wl <= a && a <= wh && wl <= b && b <= wh) continue;
                 if (inside(0, q, a) && inside(0, q, b) && inside(1, q, a) && inside(1, q, b)) continue;
                 S xx, dAa, dBa, bAa, bBa, dAb, dBb, bAb, bBb;
                 int_term(q, a, xx, dAa, dBa, bAa, bBa);
@@ -513,7 +537,9 @@ struct FXLane {
             S &th = th_w[((size_t)tb.th_node[k] * T + (e == 0 ? tA : tB)) * R];
             th = th + t * a;
         }
-        for (int q = first(1); IN && q <= L - 1; q += W) {
+        for (int q = first(1), c = 0; IN && q <= L - 1; q += W) {
+            if ((q - first(1)) / W == (c + 1) * ch) c++;
+            if (cwl[c * 64 + lane] <= t && t <= cwh[c * 64 + lane]) continue;
             S x, dA, dB, bA, bB;
             if (!int_term(q, t, x, dA, dB, bA, bB)) continue;
             const int ta = tau(q);
@@ -533,10 +559,11 @@ struct FXLane {
 // Set up the pair move of units A = (sa, na) and B = (sb, nb) at template slot `slot`, instants
 // a+1 … a+L, for replica j on the calling wave (lane `lane`); false when it is not made (κ_A or κ_B
 // below 1: the same in every lane and replica). Every lane reads its items' values at t = 0.
+// cw (IN): the wave's LDS of 2 XL_MAXCH·64 floats for the chunk windows.
 template <typename S, bool IN, int W>
 __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S> &m, const SweepState<S> &st,
                                            const SxTab<S> &tb, XlLds<S, W> *sh, int sa, int na, int sb, int nb,
-                                           int slot, int a, int L, int j, int lane, S kmin) {
+                                           int slot, int a, int L, int j, int lane, S kmin, float *cw = nullptr) {
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
     f.m = &m;
@@ -568,23 +595,35 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
     f.rp = st.r + j;
     f.Fp = st.F + j;
     f.gp = st.v + plane(N, T, R) + j;
-    // the lane's window over its interior steps
+    // the lane's window over its interior steps, and per chunk of ch of them
     S lo = -INFINITY, hi = INFINITY;
-    for (int q = f.first(1); IN && q <= L - 1; q += W)
-        for (int u = 0; u < 2; u++) {
-            const int ta = f.tau(q);
-            const S y0 = f.y(u, ta), y1 = f.y(u, f.next(ta)), c = u == 0 ? +f.cu0 : +f.cu1;
-            if (S(0) <= y0 && y0 <= S(1) && S(0) <= y1 && y1 <= S(1)) {
-                S l0, h0, l1, h1;
-                FXLane<S, IN, W>::box(y0, c, l0, h0);
-                FXLane<S, IN, W>::box(y1, c, l1, h1);
-                lo = fmax(lo, fmax(l0, l1));
-                hi = fmin(hi, fmin(h0, h1));
-            } else {
-                lo = INFINITY;
-                hi = -INFINITY;
-            }
+    if (IN) {
+        f.ch = xl_ch(T);
+        f.cwl = cw;
+        f.cwh = cw + XL_MAXCH * 64;
+        for (int q0 = f.first(1), c = 0; q0 <= L - 1; q0 += W * f.ch, c++) {
+            S clo = -INFINITY, chi = INFINITY;
+            for (int q = q0; q <= L - 1 && q < q0 + W * f.ch; q += W)
+                for (int u = 0; u < 2; u++) {
+                    const int ta = f.tau(q);
+                    const S y0 = f.y(u, ta), y1 = f.y(u, f.next(ta)), cu = u == 0 ? +f.cu0 : +f.cu1;
+                    if (S(0) <= y0 && y0 <= S(1) && S(0) <= y1 && y1 <= S(1)) {
+                        S l0, h0, l1, h1;
+                        FXLane<S, IN, W>::box(y0, cu, l0, h0);
+                        FXLane<S, IN, W>::box(y1, cu, l1, h1);
+                        clo = fmax(clo, fmax(l0, l1));
+                        chi = fmin(chi, fmin(h0, h1));
+                    } else {
+                        clo = INFINITY;
+                        chi = -INFINITY;
+                    }
+                }
+            cw[c * 64 + lane] = xl_inward_lo((double)clo);
+            cw[XL_MAXCH * 64 + c * 64 + lane] = xl_inward_hi((double)chi);
+            lo = fmax(lo, clo);
+            hi = fmin(hi, chi);
         }
+    }
     f.llo = lo;
     f.lhi = hi;
     const int k0 = tb.kk_off[slot], l0 = tb.ln_off[slot], n0 = tb.nd_off[slot];

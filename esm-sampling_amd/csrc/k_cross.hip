@@ -142,6 +142,7 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
                                                            uint32_t ctr, unsigned long long *stats, S beta, S maxnats,
                                                            S kmin) {
     __shared__ XlLds<S, 64> shm[SXL_WAVES];
+    __shared__ float cwin[SXL_WAVES][2 * XL_MAXCH * 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = blockIdx.x;
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
@@ -157,7 +158,7 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
                 continue;
             }
             FXLane<S, true, 64> f;
-            if (!xl_prepare(f, m, st, sx.tb, &shm[wave], sa, na, sb, nb, slot, a, L, j, lane, kmin)) continue;
+            if (!xl_prepare(f, m, st, sx.tb, &shm[wave], sa, na, sb, nb, slot, a, L, j, lane, kmin, cwin[wave])) continue;
             double u1, u2;
             uniforms2(k0, k1, j, (uint32_t)i, 0x5C000000u, ctr, u1, u2);
             int status = 0;

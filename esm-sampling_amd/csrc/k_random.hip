@@ -211,7 +211,7 @@ constexpr int SXP_WAVES = 4;   // waves per workgroup of the sxpair launch
 
 // W: the lanes of a move (32: two replicas per wave, when every template fits; else 64)
 template <typename S, int W>
-__global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(2, 1)) k_rand_sxpair(DevModel<S> m, SweepState<S> st,
+__global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxpair(DevModel<S> m, SweepState<S> st,
                                                                                      RandArgs a, SxArgs<S> sx, S beta,
                                                                                      S maxnats, S kmin) {
     constexpr int SEGS = 64 / W;
