@@ -174,6 +174,15 @@ def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
     nm, nl = float(stats[11, 0]), float(stats[12, 0])
     if nm + nl > 0:
         out["k_storage_long"] = es * (nm * (2 * (T / 2) + 9) + nl * 2 * T)
+    # storage pairs: sxpair (one launch per parity class) on the merged launch's per-(replica, hour)
+    # basis; sxlong (one launch per long-range sweep): a move reads both units' instants and the
+    # residuals of their nodes over its interval (T/2 steps on average) and writes the instants:
+    # 6 values per interior step
+    if float(stats[13, 0]) > 0:
+        out["k_sxpair"] = sweep_unit_bytes(s, es) * (T // 2) * R
+    nx = float(stats[14, 0])
+    if nx > 0:
+        out["k_sxlong"] = es * nx * 6 * (T / 2)
     return out
 
 

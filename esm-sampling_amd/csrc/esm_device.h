@@ -36,6 +36,17 @@ struct DevModel {
     const S *hours, *eta_in, *eta_out, *qt;   // [ESM_NSTOR][N]
 };
 
+// XCD-aware block order. The workgroups of a 1-D launch go to the GPU's 8 XCDs (each with its own
+// L2) in turn: block b runs on XCD b mod 8. For a launch whose consecutive groups of `grp` blocks
+// read the same cache lines (the replicas of one step, neighbouring replicas), the logical block of
+// physical block b puts each group on one XCD and deals the groups to the XCDs in order:
+// XCD x runs groups x, x + 8, x + 16, … (a bijection when nb is a multiple of 8·grp; else b).
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb, unsigned grp) {
+    if (grp == 0 || nb % (8u * grp) != 0) return b;
+    const unsigned x = b % 8u, k = b / 8u;
+    return ((k / grp) * 8u + x) * grp + k % grp;
+}
+
 template <typename S>
 __device__ __forceinline__ S clampv(S x, S lo, S hi) { return x < lo ? lo : (x > hi ? hi : x); }
 

@@ -143,7 +143,9 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
                                                            S kmin) {
     __shared__ XlLds<S, 64> shm[SXL_WAVES];
     __shared__ float cwin[SXL_WAVES][2 * XL_MAXCH * 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, j = blockIdx.x;
+    // replica j: a contiguous eighth of the replicas per XCD (neighbouring replicas share cache lines)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j = (int)xcd_block(blockIdx.x, gridDim.x, gridDim.x % 8 == 0 ? gridDim.x / 8 : 0);
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
     const int cnt = buf[0], nr = buf[1];

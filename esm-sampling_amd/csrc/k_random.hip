@@ -217,8 +217,11 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxp
     constexpr int SEGS = 64 / W;
     __shared__ XlLds<S, W> shm[SXP_WAVES][SEGS];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, seg = lane / W, sub = lane % W;
-    const size_t g = (size_t)blockIdx.x * SXP_WAVES + wv;        // the wave: (step, group of SEGS replicas)
     const int ngrp = (m.R + SEGS - 1) / SEGS;
+    // the wave: (step, group of SEGS replicas); the blocks of one step on one XCD (their replicas
+    // share cache lines)
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x, ngrp % SXP_WAVES == 0 ? ngrp / SXP_WAVES : 0);
+    const size_t g = (size_t)blk * SXP_WAVES + wv;
     MoveOut out;
     if (g < (size_t)a.ntau * ngrp && sx.nsx > 0) {
         const int N = m.N, T = m.T;

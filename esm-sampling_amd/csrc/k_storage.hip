@@ -1006,11 +1006,25 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
     const S *rv = st.r + (size_t)n * TR + j;
     S *gv = st.v + plane(N, T, R) + (size_t)n * TR + j;
     MoveOut out_long, out_level;
+    // With the trajectory in LDS (YL) the moves of a unit read and write it there, and it goes back
+    // to memory once, when the wave turns to the node's other unit or has made its last move: every
+    // read of the unit's instants in between is this wave's (the node's residuals and gas stay in
+    // memory, written at once), and a write-back per unit instead of per move takes the stores,
+    // one cache line per instant in the replica-fastest layout, from Σ L over the moves to T.
     int cur = -1;
+    auto write_back = [&]() {
+        if (!YL || cur < 0) return;
+        S *yc = st.v + plane((2 + cur) * N, T, R) + (size_t)n * TR + j;
+        for (int tau = lane; tau < T; tau += 64) yc[(size_t)tau * R] = ys[tau];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    };
     for (int i = 0; i < cnt; i++) {
         const int si = msi[i], L = mL[i], level = L >= T;
         S *yg = st.v + plane((2 + si) * N, T, R) + (size_t)n * TR + j;
         if (YL && si != cur) {          // stage the unit's trajectory
+            write_back();
             for (int tau = lane; tau < T; tau += 64) ys[tau] = yg[(size_t)tau * R];
             cur = si;
         }
@@ -1145,14 +1159,15 @@ __global__ void __launch_bounds__(64) k_storage_long(DevModel<S> m, SweepState<S
             for (int q = f.first(1); q <= f.L; q += 64) {
                 const int tau = f.tau(q);
                 const S nv = f.y(tau) + t;
-                yg[(size_t)tau * R] = nv;
                 if (YL) ys[tau] = nv;
+                else yg[(size_t)tau * R] = nv;
             }
         }
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
+    write_back();
     add_counts(a.st_long, out_long);
     add_counts(a.st_level, out_level);
 }
