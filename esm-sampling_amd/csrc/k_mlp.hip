@@ -873,6 +873,32 @@ int32_t esm_mlp_params(esm_handle *h, double *get, const double *set) {
     return ESM_OK;
 }
 
+// Adam's state: [m (np)][v (np)][t], FP32 moments as doubles (exact both ways)
+int32_t esm_mlp_optimizer(esm_handle *h, double *get, const double *set) {
+    if (!h || !h->mlp) { set_error("esm_mlp_optimizer: no network (esm_mlp_init)"); return ESM_EINVAL; }
+    MlpState &s = *h->mlp;
+    std::vector<float> q(2 * s.np);
+    if (set) {
+        if (!(set[2 * s.np] >= 0) || set[2 * s.np] != nearbyint(set[2 * s.np])) {
+            set_error("esm_mlp_optimizer: the step count must be a whole number >= 0");
+            return ESM_EINVAL;
+        }
+        for (size_t i = 0; i < 2 * s.np; i++) q[i] = (float)set[i];
+        ESM_HIP_CHECK(hipMemcpyAsync(s.am, q.data(), s.np * 4, hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipMemcpyAsync(s.av, q.data() + s.np, s.np * 4, hipMemcpyHostToDevice, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+        s.adam_t = (long long)set[2 * s.np];
+    }
+    if (get) {
+        ESM_HIP_CHECK(hipMemcpyAsync(q.data(), s.am, s.np * 4, hipMemcpyDeviceToHost, h->stream));
+        ESM_HIP_CHECK(hipMemcpyAsync(q.data() + s.np, s.av, s.np * 4, hipMemcpyDeviceToHost, h->stream));
+        ESM_HIP_CHECK(hipStreamSynchronize(h->stream));
+        for (size_t i = 0; i < 2 * s.np; i++) get[i] = q[i];
+        get[2 * s.np] = (double)s.adam_t;
+    }
+    return ESM_OK;
+}
+
 // forward of M explicit input rows x [M][din] (rounded to BF16): out [M][dout]
 int32_t esm_mlp_forward(esm_handle *h, const double *x, int64_t M, double *out) {
     if (!h || !h->mlp || !x || !out) { set_error("esm_mlp_forward: null argument or no network"); return ESM_EINVAL; }

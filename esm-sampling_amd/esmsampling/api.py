@@ -484,6 +484,16 @@ class LearnedProposal:
         check(self.R.lib.esm_mlp_params(self.R.handle, dptr(out), dptr(src) if src is not None else None), "esm_mlp_params")
         return out
 
+    def optimizer(self, set_to=None):
+        """Adam's state [m (nparam)][v (nparam)][t] (``esm_mlp_optimizer``); with ``set_to``, set it first."""
+        out = np.zeros(2 * self.nparam + 1)
+        src = None if set_to is None else f64(set_to)
+        if src is not None and src.shape != (2 * self.nparam + 1,):
+            raise ValueError(f"{2 * self.nparam + 1} values expected")
+        check(self.R.lib.esm_mlp_optimizer(self.R.handle, dptr(out), dptr(src) if src is not None else None),
+              "esm_mlp_optimizer")
+        return out
+
     def layers(self, p=None):
         """[(Wt [out][in], b [out])] of the four layers, from ``p`` (or the current parameters)."""
         p = self.params() if p is None else p

@@ -14,8 +14,10 @@ Snapshot layout (version 1, ``driver = "replicas"``): ``phase``, ``it`` (replica
 ``window``, ``active``, ``trace/<key>``, ``probe/<key>``. The replica states are stored per chain
 in the device layout ``[field][node][hour][replica]`` (``state/v``, ``state/x``, ``state/k``, the
 weather ``state/m`` and ``state/z``) instead of one group per replica; the driver's own state
-for an exact continuation is stored beside them (``driver/gbuf``, ``driver/hist``, the Langevin
-normals' generator state ``driver/rng``, the probe counter, the learned proposal's parameters).
+for an exact continuation is stored beside them (``driver/gbuf``, ``driver/hist``, the step
+adaptation's pending gaps ``driver/gaps``, the Langevin normals' generator state ``driver/rng``,
+the probe counter, the learned proposal's parameters and Adam state ``learned/adam``), so that a
+run resumed anywhere, the warmup included, continues as the uninterrupted run.
 """
 from __future__ import annotations
 
@@ -63,18 +65,19 @@ PROBE_KEYS = ("it", "gap", "rs_eq", "rs_in", "fk", "cost", "eta_before", "eta_af
 
 
 def replica_step_record(k, parts, wts, active, since, moved, rs, fk, eta_w, nclip, wall, loop, learned):
-    """One step's replica record: carrier totals of the shared capacities, capex, per replica
+    """One step's replica record (every array a copy, as the driver updates its own in place):
+    carrier totals of the shared capacities, capex, per replica
     opex, shed, penalty and penalty-free cost (capex + generation + storage), the drift weights,
     active flags, steps since the last burst, Φ (NaN: no declared tilt functional), whether the
     weather moved, the interior means of r s̄ and β_k ∂K, the Langevin steps, the clipped drifts,
     (wall time, time of the replica loop) and the learned proposal's (attempted, accepted)."""
     return {"ktot": np.asarray(k).sum(axis=1), "capex": [float(parts[0, 0])], "op": parts[:, 1].copy(),
             "shed": parts[:, 4].copy(), "penalty": parts[:, 6].copy(),
-            "pfcost": parts[:, 0] + parts[:, 2] + parts[:, 3], "weights": np.asarray(wts, dtype=np.float64),
-            "active": np.asarray(active, dtype=bool).copy(), "since": np.minimum(np.asarray(since), 2 ** 31).astype(float),
-            "phi": np.full(len(wts), np.nan), "moved": np.asarray(moved, dtype=np.float64), "rs": np.asarray(rs),
-            "fk": np.asarray(fk), "eta_w": np.asarray(eta_w, dtype=np.float64), "nclip": [float(nclip)],
-            "time": [float(wall), float(loop)], "learned": np.asarray(learned, dtype=np.float64)}
+            "pfcost": parts[:, 0] + parts[:, 2] + parts[:, 3], "weights": np.array(wts, dtype=np.float64),
+            "active": np.array(active, dtype=bool), "since": np.minimum(np.asarray(since), 2 ** 31).astype(float),
+            "phi": np.full(len(wts), np.nan), "moved": np.array(moved, dtype=np.float64), "rs": np.array(rs),
+            "fk": np.array(fk), "eta_w": np.array(eta_w, dtype=np.float64), "nclip": [float(nclip)],
+            "time": [float(wall), float(loop)], "learned": np.array(learned, dtype=np.float64)}
 
 
 def probe_record(it, j, adapted, pr, eta_before, eta_after):
@@ -82,7 +85,8 @@ def probe_record(it, j, adapted, pr, eta_before, eta_after):
     return {"it": [float(it), float(j + 1), float(bool(adapted))], "gap": pr["gap"], "rs_eq": pr["rs_eq"],
             "rs_in": pr["rs_in"], "fk": pr["fk"],
             "cost": [pr["op_in"], pr["op_eq"], pr["shed_in"], pr["shed_eq"]],
-            "eta_before": np.asarray(eta_before), "eta_after": np.asarray(eta_after)}
+            "eta_before": np.array(eta_before), "eta_after": np.array(eta_after)}   # copies: the driver
+                                                                                  # adapts its η in place
 
 
 def _columns(recs, keys):
@@ -220,6 +224,9 @@ def write_snapshot(path, cfg, it, restarts, kinds, sets, traces, rrecs, precs, r
                     f"{g}/state/m": m, f"{g}/state/z": z,
                     f"{g}/driver/rng": np.array(rng_state_json(rngs[c].bit_generator.state)),
                     f"{g}/driver/nprobe": np.array(nprobe[c])})
+        # the step adaptation's gaps collected since its last update (probe_avg > 1)
+        out[f"{g}/driver/gaps"] = (np.array(R.gaps, dtype=np.float64).reshape(len(R.gaps), -1) if R.gaps
+                                   else np.zeros((0, 0)))
         gv, gl, gs = _ragged(R.gbuf)
         hv, hl, hs = _ragged(R.hist)
         out.update({f"{g}/driver/gbuf": gv, f"{g}/driver/gbuf_len": gl, f"{g}/driver/gbuf_shape": gs,
@@ -232,7 +239,8 @@ def write_snapshot(path, cfg, it, restarts, kinds, sets, traces, rrecs, precs, r
             out[f"{g}/probe/{key}"] = A
         if learned is not None and learned[c] is not None:
             L = learned[c]
-            out.update({f"{g}/learned/params": L.net.params(), f"{g}/learned/ntrain": np.array(L.ntrain),
+            out.update({f"{g}/learned/params": L.net.params(), f"{g}/learned/adam": L.net.optimizer(),
+                        f"{g}/learned/ntrain": np.array(L.ntrain),
                         f"{g}/learned/counts": np.array([L.att, L.acc]), f"{g}/learned/loss": np.asarray(L.loss),
                         f"{g}/learned/wall": np.array(L.wall)})
     _atomic_savez(path, out)

@@ -845,12 +845,15 @@ def run_replicas(cfg: Config, log=None):
             R.window = snap[f"{g}/window"].astype(bool)
             R.active = snap[f"{g}/active"].astype(bool)
             R.gbuf, R.hist = rec_io.restore_driver_lists(snap, c)
+            R.gaps = [np.array(q) for q in snap.get(f"{g}/driver/gaps", np.zeros((0, 0)))]
             rngs[c].bit_generator.state = rec_io.rng_state_from_json(str(snap[f"{g}/driver/rng"]))
             nprobe[c] = int(snap[f"{g}/driver/nprobe"])
             traces[c], records[c], probes[c] = rec_io.chain_records(snap, c)
             if learned[c] is not None and f"{g}/learned/params" in snap:
                 L = learned[c]
                 L.net.params(set_to=snap[f"{g}/learned/params"])
+                if f"{g}/learned/adam" in snap:
+                    L.net.optimizer(set_to=snap[f"{g}/learned/adam"])
                 L.ntrain = int(snap[f"{g}/learned/ntrain"])
                 L.att, L.acc = (int(q) for q in snap[f"{g}/learned/counts"])
                 L.loss = list(snap[f"{g}/learned/loss"])
@@ -933,16 +936,19 @@ def run_replicas(cfg: Config, log=None):
     nq = len(MONITOR_NAMES)
     qtr = [[np.array([tr[t][q] for t in range(len(tr))]) for tr in traces] for q in range(nq)]
     good = [all(np.all(np.isfinite(c)) for c in qtr[q]) for q in range(nq)]
+    # the steps in the traces (fewer than planned when a stop file ended the run; inside the warmup
+    # there is no production window to report on)
+    ndone = min(len(tr) for tr in traces)
     w0 = nwarm
-    if nwarm + nprod >= 20:
+    if ndone >= 20:
         wdet = warmup_end([qtr[q] for q in range(nq) if good[q]], threshold=cfg.rhat_screen,
-                          every=max(5, (nwarm + nprod) // 40))
+                          every=max(5, ndone // 40))
         w0 = max(nwarm, wdet)
-        if w0 >= nwarm + nprod - 4:
+        if w0 >= ndone - 4:
             w0 = nwarm
     summ = {}
     for q in range(nq):
-        if good[q] and nwarm + nprod - w0 >= 4:
+        if good[q] and ndone - w0 >= 4:
             summ[MONITOR_NAMES[q]] = convergence_summary([c[w0:] for c in qtr[q]], rhat_max=cfg.rhat_max,
                                                          ess_min=cfg.ess_min, mcse_max=cfg.mcse_max)
     prod_s = prod_before + (t_end - t_prod0)

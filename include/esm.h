@@ -373,6 +373,11 @@ int32_t esm_sweep_launch(esm_handle *h, int32_t merged);
 int32_t esm_mlp_init(esm_handle *h, int32_t block_hours, int32_t ncomp, const uint32_t key[2]);
 int32_t esm_mlp_dims(esm_handle *h, int32_t *dims);
 int32_t esm_mlp_params(esm_handle *h, double *get, const double *set);
+/* esm_mlp_optimizer: get and/or set Adam's state [m (np)][v (np)][t] (np from esm_mlp_dims; t the
+ * steps taken), so that a snapshot resumes training exactly; either pointer may be NULL. The
+ * learned proposal has no counterpart in the reference; this serves its driver's snapshot
+ * (`write_replica_snapshot`, src/replicas.jl:1082-1124, extended by the network's state). */
+int32_t esm_mlp_optimizer(esm_handle *h, double *get, const double *set);
 int32_t esm_mlp_forward(esm_handle *h, const double *x, int64_t M, double *out);
 int32_t esm_mlp_gradient(esm_handle *h, const double *x, const double *y, const double *mu0, int64_t M, double *grad,
                          double *loss);

@@ -156,6 +156,48 @@ def test_stop_file_snapshot_and_resume(tmp_path):
     assert (tmp_path / "b" / "summary_replicas.txt").exists()
 
 
+def test_resume_inside_the_warmup_with_training_and_averaged_probes(tmp_path):
+    """A run stopped inside the warmup, while the learned proposal still trains (Adam's moments and
+    step count in the snapshot) and the step adaptation holds gaps of an unfinished average
+    (probe_avg = 2), resumed from its snapshot: the same traces, capacities, Langevin steps and
+    learned counts as the run without a stop (FP64 sampler; the network's training is
+    reproducible bit for bit)."""
+    print("stop file inside the warmup, learned training, averaged probes", flush=True)
+    base = dict(init="heuristic", resolution_hours=1.0, nchains=1, replicas=4, mlda_warmup=30, nsamples=6,
+                replica_init_sweeps=2, replica_burst_every=4, replica_probe_every=5, replica_probe_len=3,
+                replica_probe_tail=1, replica_probe_win=1, replica_probe_avg=2, replica_gate_min=1,
+                replica_gate_max=2, replica_gate_min_window=1, replica_gate_max_window=3, replica_gate_win=2,
+                synthetic_nodes=10, synthetic_lines=16, snapshots=48, precision="fp64", seed=17,
+                learned_proposal=True, learned_train_steps=40, learned_train_every=4, learned_train_more=20,
+                learned_components=2)
+    a = RP.run_replicas(RP.load_config(dict(base, out=str(tmp_path / "a" / "out.npz"))))
+    ck, stop = tmp_path / "ck.npz", tmp_path / "STOP"
+
+    def log(msg):
+        if "step 25/" in msg:   # inside the warmup (30 steps), after the fifth probe
+            stop.write_text("")
+
+    cfg = RP.load_config(dict(base, checkpoint=str(ck), stop_file=str(stop), out=str(tmp_path / "b" / "out.npz")))
+    b1 = RP.run_replicas(cfg, log=log)
+    assert b1["status"] == 75 and b1["steps_done"] == 25 and ck.exists()
+    with np.load(ck, allow_pickle=False) as f:
+        assert f["chains/1/learned/adam"][-1] > 0 and f["chains/1/driver/gaps"].shape[0] == 1
+    b2 = RP.run_replicas(cfg)
+    assert b2["status"] == 0 and b2["resumed_at"] == 25 and b2["steps_done"] == 36
+    np.testing.assert_allclose(np.array(b2["traces"][0]), np.array(a["traces"][0]), rtol=1e-12)
+    assert len(b2["probes"][0]) == len(a["probes"][0])
+    for pa, pb in zip(a["probes"][0], b2["probes"][0]):
+        np.testing.assert_allclose(pb["it"], pa["it"], err_msg="probe step, replica, adapted")
+        np.testing.assert_allclose(pb["gap"], pa["gap"], rtol=1e-9, err_msg=f"probe gap at step {pa['it'][0]}")
+        np.testing.assert_allclose(pb["eta_after"], pa["eta_after"], rtol=1e-12, err_msg=f"probe eta at {pa['it'][0]}")
+    for ra, rb in zip(a["records"][0], b2["records"][0]):
+        assert ra["it"] == rb["it"]
+        np.testing.assert_allclose(rb["k"], ra["k"], rtol=1e-12, err_msg=f"k at step {ra['it']}")
+        np.testing.assert_allclose(rb["eta_w"], ra["eta_w"], rtol=1e-12, err_msg=f"eta_w at step {ra['it']}")
+        assert np.array_equal(ra["learned"], rb["learned"])
+    assert b2["learned"]["train_steps"] == a["learned"]["train_steps"]
+
+
 def test_run_replicas_with_ffbs_and_the_reference_replica_configuration():
     """The reference's replica configuration (configs/device40_replica.toml: physical measure, 40
     nodes daily, K = 4 regimes, 10 countries, 4 chains of 4 replicas) with a short schedule and the

@@ -109,3 +109,24 @@ def test_snapshot_config_keys():
     c = RP.load_config({"init": "heuristic", "stop_file": "/tmp/x", "checkpoint_minutes": 5.0})
     assert c.checkpoint == "" and c.stop_file == "/tmp/x" and c.checkpoint_minutes == 5.0
     assert RP.load_config({"init": "heuristic"}).checkpoint_minutes == 30.0
+
+
+def test_records_copy_the_drivers_arrays():
+    """The driver adapts its η and other buffers in place: a step's or a probe's record keeps the
+    values of its own step (an alias would show the last step's values in every column)."""
+    M = 3
+    eta, wts, act = np.full(6, 5e-4), np.full(M, 1.0 / M), np.ones(M, dtype=bool)
+    rec = rio.replica_step_record(np.ones((6, 2)), np.ones((M, 8)), wts, act, np.arange(M), np.zeros(M),
+                                  np.zeros(6), np.ones(6), eta, 0, 0.1, 0.1, [1, 1])
+    pr = dict(gap=np.zeros(6), rs_eq=np.zeros(6), rs_in=np.zeros(6), fk=np.ones(6), op_in=1.0, op_eq=1.0,
+              shed_in=0.0, shed_eq=0.0)
+    before = eta.copy()
+    eta *= 1.2
+    prec = rio.probe_record(5, 0, True, pr, before, eta)
+    eta *= 1.2
+    wts[:] = 0.0
+    act[:] = False
+    np.testing.assert_array_equal(rec["eta_w"], np.full(6, 5e-4))
+    np.testing.assert_array_equal(rec["weights"], np.full(M, 1.0 / M))
+    assert rec["active"].all()
+    np.testing.assert_array_equal(prec["eta_after"], before * 1.2)
