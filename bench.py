@@ -536,8 +536,12 @@ def measured_traffic(kernel, s, T, R, prec, form=1):
              "k_xloc": "k_sparse<float, 1, true>", "k_xlocc": "k_sparse<float, 2, true>",
              "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms",
              "k_storage_long": "k_storage_long<float, true>",
-             "k_random": f"k_sparse_all<float, true, {'true' if form > 1 else 'false'}>"}
-    return json.loads(p.read_text())["bytes_per_dispatch"].get(names.get(kernel, ""))
+             "k_random": f"k_sparse_all<float, true, {'true' if form > 1 else 'false'}>",
+             "k_sxlong": "k_sxlong<float>"}
+    got = json.loads(p.read_text())["bytes_per_dispatch"]
+    if kernel == "k_sxpair":   # the pair kernel's segment width (32 or 64 lanes) is in its name
+        return got.get("k_rand_sxpair<float, 32>", got.get("k_rand_sxpair<float, 64>"))
+    return got.get(names.get(kernel, ""))
 
 
 def cpu_baseline(wl, threads, it):
