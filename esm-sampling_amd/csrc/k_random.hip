@@ -1192,8 +1192,15 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 set_error("storage pairs: a pair template is larger than the lane form holds");
                 return ESM_EINVAL;
             }
-            const size_t nwaves = (size_t)a.ntau * ((R + 64 / wx - 1) / (64 / wx));
-            const dim3 grid((unsigned)((nwaves + SXP_WAVES - 1) / SXP_WAVES)), blk(64 * SXP_WAVES);
+            // the blocks of one step on one XCD: the grid padded to whole rounds of 8 XCDs × the
+            // step's blocks (the extra blocks find no wave of theirs and exit)
+            const size_t ngrp = (size_t)(R + 64 / wx - 1) / (64 / wx), nwaves = (size_t)a.ntau * ngrp;
+            size_t nblocks = (nwaves + SXP_WAVES - 1) / SXP_WAVES;
+            if (ngrp % SXP_WAVES == 0) {
+                const size_t round = 8 * (ngrp / SXP_WAVES);
+                nblocks = (nblocks + round - 1) / round * round;
+            }
+            const dim3 grid((unsigned)nblocks), blk(64 * SXP_WAVES);
             if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair<S, 32>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             else hipLaunchKernelGGL((k_rand_sxpair<S, 64>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());
