@@ -531,7 +531,7 @@ def measured_traffic(kernel, s, T, R, prec, form=1):
     if p is None:
         return None
     names = {"k_spair": "k_rand_spair<float, true>", "k_soc": "k_soc<float>", "k_gen": "k_gen<float>",
-             "k_energy": "k_energy<float, true, false>", "k_follow": "k_follow<float>",
+             "k_energy": "k_energy<float, true, false, true>", "k_follow": "k_follow<float>",
              "k_unit": "k_rand_unit<float, true>", "k_cell": "k_sparse<float, 0, true>",
              "k_xloc": "k_sparse<float, 1, true>", "k_xlocc": "k_sparse<float, 2, true>",
              "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms",

@@ -12,9 +12,13 @@ namespace esm {
 // per-step cost parts. Restates energy_cost (src/energy.jl:551-677) per step and rebuild_step!
 // (src/local.jl:332-374). Per-step parts: 0 gen, 1 storage running cost, 2 shed MWh, 3 dump MWh,
 // 4 penalty, 5 co2. drift: part 0 receives max_n |r_cache − r| instead (rebuild_flat).
-template <typename S, bool COST, bool DRIFT>
+// STAGE: the thread's N angles are read from memory once into its own LDS column ([n][thread]:
+// consecutive threads, consecutive banks), and the lines' ends and the nodes' neighbours read
+// them there (each angle is used by 1 + 2·degree terms); no thread reads another's column.
+template <typename S, bool COST, bool DRIFT, bool STAGE>
 __global__ void __launch_bounds__(256) k_energy(DevModel<S> m, const S *__restrict__ v, const S *__restrict__ x,
                                                 S *__restrict__ r, S *__restrict__ F, double *__restrict__ hp) {
+    extern __shared__ unsigned char en_lds[];
     const size_t TR = (size_t)m.T * m.R;
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= TR) return;
@@ -27,18 +31,21 @@ __global__ void __launch_bounds__(256) k_energy(DevModel<S> m, const S *__restri
     const S *ye1 = v + plane(3 * m.N, m.T, m.R);
     double gen = 0, stor = 0, shed = 0, dump = 0, pen = 0, co2 = 0, drift = 0;
     const S cvdt = m.cv * m.dt;
+    S *col = (S *)en_lds + threadIdx.x;   // the thread's column, stride blockDim.x
+    if (STAGE)
+        for (int n = 0; n < m.N; n++) col[(size_t)n * blockDim.x] = th[plane(n, m.T, m.R) + g];
+    auto theta = [&](int n) -> S { return STAGE ? col[(size_t)n * blockDim.x] : th[plane(n, m.T, m.R) + g]; };
     for (int e = 0; e < m.L; e++) {
         const int a = m.bus0[e], b = m.bus1[e];
-        const S f = m.lb[e] * (th[plane(a, m.T, m.R) + g] - th[plane(b, m.T, m.R) + g]);
+        const S f = m.lb[e] * (theta(a) - theta(b));
         F[plane(e, m.T, m.R) + g] = f;
         if (COST) pen += (double)(cvdt * fmax(fabs(f) - m.ls[e], S(0)));
     }
     for (int n = 0; n < m.N; n++) {
         const size_t pn = plane(n, m.T, m.R) + g;
         CSum<S> rn(fmax(m.load[(size_t)n * m.T + t], S(0)));
-        const S thn = th[pn];
-        for (int q = m.adj_ptr[n]; q < m.adj_ptr[n + 1]; q++)
-            rn.add(m.adj_b[q] * (thn - th[plane(m.adj_other[q], m.T, m.R) + g]));
+        const S thn = theta(n);
+        for (int q = m.adj_ptr[n]; q < m.adj_ptr[n + 1]; q++) rn.add(m.adj_b[q] * (thn - theta(m.adj_other[q])));
 #pragma unroll
         for (int c = 0; c < ESM_NREN; c++) {
             const S A = clampv(x[plane(c * m.N + n, m.T, m.R) + g], S(0), S(1));
@@ -268,21 +275,31 @@ template <typename S>
 int launch_energy(esm_handle *h, bool cost, bool drift) {
     const DevModel<S> m = dev_model<S>(h);
     const size_t TR = (size_t)h->T * h->R;
-    const unsigned B = 256;
     const S *v = (const S *)h->d_v;
     const S *x = (const S *)h->d_x;
     S *r = (S *)h->d_r;
     S *F = (S *)h->d_F;
+    // the angles staged in LDS when the columns of a block of at least 128 threads fit 40 KB (four
+    // blocks per CU): N = 40 with 256 threads in FP32, 128 in FP64; not at N = 256
+    unsigned B = 256;
+    while (B > 128 && (size_t)h->N * B * sizeof(S) > 40960) B /= 2;
+    const bool stage = (size_t)h->N * B * sizeof(S) <= 40960;
+    if (!stage) B = 256;
+    const size_t lds = stage ? (size_t)h->N * B * sizeof(S) : 0;
     ProfScope ps(h, ESM_KERN_ENERGY);
-    if (cost)
-        hipLaunchKernelGGL((k_energy<S, true, false>), dim3(nblk(TR, B)), dim3(B), 0, h->stream, m, v, x, r, F,
-                           h->d_hparts);
-    else if (drift)
-        hipLaunchKernelGGL((k_energy<S, false, true>), dim3(nblk(TR, B)), dim3(B), 0, h->stream, m, v, x, r, F,
-                           h->d_hparts);
-    else
-        hipLaunchKernelGGL((k_energy<S, false, false>), dim3(nblk(TR, B)), dim3(B), 0, h->stream, m, v, x, r, F,
-                           h->d_hparts);
+#define ENERGY_LAUNCH(C, D)                                                                                        \
+    do {                                                                                                           \
+        if (stage)                                                                                                 \
+            hipLaunchKernelGGL((k_energy<S, C, D, true>), dim3(nblk(TR, B)), dim3(B), lds, h->stream, m, v, x, r, F, \
+                               h->d_hparts);                                                                       \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_energy<S, C, D, false>), dim3(nblk(TR, B)), dim3(B), 0, h->stream, m, v, x, r, F, \
+                               h->d_hparts);                                                                       \
+    } while (0)
+    if (cost) ENERGY_LAUNCH(true, false);
+    else if (drift) ENERGY_LAUNCH(false, true);
+    else ENERGY_LAUNCH(false, false);
+#undef ENERGY_LAUNCH
     ESM_HIP_CHECK(hipGetLastError());
     return 0;
 }
