@@ -118,18 +118,18 @@ __global__ void __launch_bounds__(64) k_sx_schedule(DevModel<S> m, SxArgs<S> sx,
         }
         __syncthreads();
     }
-    // the moves round by round (a counting pass per round over the drawn order)
+    // the moves round by round, each round in the drawn order: a counting sort on the round (the
+    // marks are done, their words hold the rounds' cursors)
     if (lane == 0) {
         const int nr = nr_s;
-        int k = 0;
-        roff[0] = 0;
-        for (int r = 0; r < nr; r++) {
-            for (int i = 0; i < cnt; i++)
-                if (rid[i] == r) {
-                    for (int q = 0; q < 8; q++) mv[8 * k + q] = tmp[8 * i + q];
-                    k++;
-                }
-            roff[r + 1] = k;
+        int *cur = (int *)mark;
+        for (int r = 0; r <= nr; r++) cur[r] = 0;
+        for (int i = 0; i < cnt; i++) cur[rid[i] + 1]++;
+        for (int r = 0; r < nr; r++) cur[r + 1] += cur[r];
+        for (int r = 0; r <= nr; r++) roff[r] = cur[r];
+        for (int i = 0; i < cnt; i++) {
+            const int k = cur[rid[i]]++;
+            for (int q = 0; q < 8; q++) mv[8 * k + q] = tmp[8 * i + q];
         }
         buf[0] = cnt;
         buf[1] = nr;
