@@ -448,7 +448,7 @@ def learned_leg(args, prec, R=128, K=8, B=24, train_steps=300, sweeps=3):
             "acceptance": acc / max(att, 1), "train_steps": train_steps, "train_step_ms": train_ms,
             "train_loss_first_last": [float(loss[0]), float(loss[-1])],
             "forward_ms_per_sweep": fwd_ms / (sweeps + 1), "propose_kernel_ms_per_sweep": prof["propose"][0] / (sweeps + 1),
-            "mfma": {"kernel": "k_gemm (forward pass, 4 layers)", "bound": "mfma", "achieved": tflops,
+            "mfma": {"kernel": "k_mlp_fwd (fused forward pass, 4 layers)", "bound": "mfma", "achieved": tflops,
                      "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
                      "flops_per_row": flops_row, "rows": rows, "launches": launches,
                      "kernel_ms_mean": fwd_ms / max(launches, 1)}}

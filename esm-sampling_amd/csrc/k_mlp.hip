@@ -348,6 +348,135 @@ static int gemm(hipStream_t s, int M, int N, int K, const uint16_t *A, int lda, 
 // ---------------------------------------------------------------------------------------------
 // the instances, their features and the mixture head
 // ---------------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------------
+// Fused forward pass: the four layers of a 128-row tile in one workgroup. The tile's activations
+// stay in LDS (128 × 256 BF16, written in place by each hidden layer's epilogue after every wave
+// has read them); the weights stream through a double-buffered LDS ring in K steps of 64, 64·NB
+// columns per pass (NB = 4 for the hidden layers, 3 for the output layer's 192-column passes);
+// eight waves of 32 rows × 32·NB columns (NB MFMA tiles), two per SIMD (141 KB of LDS per
+// workgroup). Measured per 65,536-row chunk: 114 µs (four waves with 32-wide K steps: 175 µs,
+// the time of the four layered launches). Each output element accumulates
+// over K in the same order (steps of 32, two v_mfma_f32_32x32x16_bf16 per step) and passes the
+// same bias, ReLU and BF16 rounding as in the layered kernels, so the outputs are bit for bit
+// theirs; the activations never reach HBM (the layered pass writes and re-reads 3 × 128 KB per
+// 256 rows and launches four kernels).
+// ---------------------------------------------------------------------------------------------
+constexpr int FF_M = 128, FF_LD = 256 + 8;
+
+struct FwdArgs {
+    const uint16_t *w[4];   // BF16 Wt [out][in] of each layer
+    const float *b[4];      // FP32 bias [out]
+    int din, dh, dout;
+};
+
+// one pass of columns j0 … j0 + 64·NB of W [nout][K] against the tile's activations [128][K]:
+// FF_WAVES waves in a (FF_WAVES/2) × 2 grid, each 128/(FF_WAVES/2) rows (MT MFMA row tiles) × 32·NB
+// columns
+constexpr int FF_WAVES = 8, FF_MT = 128 / (FF_WAVES / 2) / 32, FF_K = 64, FF_BLD = FF_K + 8;
+template <int NB>
+__device__ __forceinline__ void ff_pass(uint16_t (*Act)[FF_LD], uint16_t (*Bs)[256][FF_BLD], const uint16_t *W, int K,
+                                        int nout, int j0, f32x16 (&acc)[FF_MT][NB]) {
+    constexpr int NT = 64 * FF_WAVES, CPR = FF_K / 8, CH = 64 * NB * CPR, E = (CH + NT - 1) / NT;   // 8-element chunks of a ring slot
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wc = wave & 1;
+    uint4 rb[E];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int c = tid + NT * e, r = c / CPR, kc = (c % CPR) * 8, j = j0 + r;
+            rb[e] = c < CH && j < nout && k0 + kc < K ? *(const uint4 *)(W + (size_t)j * K + k0 + kc)
+                                                     : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int c = tid + NT * e;
+            if (c < CH) *(uint4 *)&Bs[buf][c / CPR][(c % CPR) * 8] = rb[e];
+        }
+    };
+#pragma unroll
+    for (int a = 0; a < FF_MT; a++)
+#pragma unroll
+        for (int b = 0; b < NB; b++) acc[a][b] = f32x16{0};
+    load(0);
+    __syncthreads();   // the previous pass's readers of the ring and writers of the activations are done
+    store(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < K; k0 += FF_K) {
+        const bool more = k0 + FF_K < K;
+        if (more) load(k0 + FF_K);
+#pragma unroll
+        for (int kk = 0; kk < FF_K; kk += 16) {
+            if (k0 + kk >= K) break;   // K a multiple of 32: the last step may hold half
+            const int ko = kk + 8 * (lane >> 5);
+            bf16x8 fa[FF_MT], fb[NB];
+#pragma unroll
+            for (int q = 0; q < FF_MT; q++)
+                fa[q] = *(const bf16x8 *)&Act[wr * 32 * FF_MT + q * 32 + (lane & 31)][k0 + ko];
+#pragma unroll
+            for (int q = 0; q < NB; q++) fb[q] = *(const bf16x8 *)&Bs[cur][wc * 32 * NB + q * 32 + (lane & 31)][ko];
+#pragma unroll
+            for (int a = 0; a < FF_MT; a++)
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+        if (more) store(cur ^ 1);   // the other buffer: its last readers passed the previous barrier
+        __syncthreads();            // after the last step: every wave has read the activations
+        cur ^= 1;
+    }
+}
+
+// rows [M][din] BF16 → outputs [M][dout] FP32; din a multiple of 32 and ≤ 256, dh = 256
+__global__ void __launch_bounds__(64 * FF_WAVES, 1) k_mlp_fwd(int M, FwdArgs f, const uint16_t *__restrict__ X,
+                                                              float *__restrict__ OUT) {
+    __shared__ __attribute__((aligned(16))) uint16_t Act[FF_M][FF_LD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][256][FF_BLD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wr = wave >> 1, wc = wave & 1;
+    const int i0 = blockIdx.x * FF_M;
+    const int nch = f.din / 8;   // 8-element chunks of an input row
+    for (int c = tid; c < FF_M * nch; c += 64 * FF_WAVES) {
+        const int r = c / nch, kc = (c % nch) * 8, i = i0 + r;
+        *(uint4 *)&Act[r][kc] = i < M ? *(const uint4 *)(X + (size_t)i * f.din + kc) : make_uint4(0, 0, 0, 0);
+    }
+    // hidden layers: the tile's 256 outputs in one pass, ReLU'd BF16 back into the activations
+    for (int l = 0; l < 3; l++) {
+        f32x16 acc[FF_MT][4];
+        ff_pass<4>(Act, Bs, f.w[l], l == 0 ? f.din : f.dh, f.dh, 0, acc);
+#pragma unroll
+        for (int a = 0; a < FF_MT; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int j = wc * 128 + b * 32 + (lane & 31);
+                const float bj = f.b[l][j];
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int r = wr * 32 * FF_MT + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    Act[r][j] = f2bf(fmaxf(acc[a][b][v] + bj, 0.0f));
+                }
+            }
+    }
+    // output layer: 192 columns per pass
+    for (int j0 = 0; j0 < f.dout; j0 += 192) {
+        f32x16 acc[FF_MT][3];
+        ff_pass<3>(Act, Bs, f.w[3], f.dh, f.dout, j0, acc);
+#pragma unroll
+        for (int a = 0; a < FF_MT; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const int j = j0 + wc * 96 + b * 32 + (lane & 31);
+                if (j >= f.dout) continue;
+                const float bj = f.b[3][j];
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int i = i0 + wr * 32 * FF_MT + a * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                    if (i < M) OUT[(size_t)i * f.dout + j] = acc[a][b][v] + bj;
+                }
+            }
+    }
+}
+
 struct MlpDims {
     int B, K, din, dh, dout;
     int nb;                  // blocks per node: T / B
@@ -662,6 +791,7 @@ struct MlpState {
     float *p = nullptr, *g = nullptr, *am = nullptr, *av = nullptr;   // master params, gradient, Adam moments
     uint16_t *wt16 = nullptr;                                         // BF16 Wt of every layer (offsets ow)
     long long adam_t = 0;
+    bool fused = true;   // the proposals' and explicit rows' forward in one kernel (k_mlp_fwd)
     int cap = MLP_CAP;
     uint16_t *X = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *DG[4] = {nullptr, nullptr, nullptr, nullptr};
     float *OUT = nullptr, *Y = nullptr, *MU0 = nullptr, *YP = nullptr;
@@ -693,10 +823,20 @@ static int mlp_refresh_bf16(esm_handle *h) {
 }
 
 // forward of cnt rows of s.X into s.OUT (with prof: every GEMM launch timed on its own)
-static int mlp_forward(esm_handle *h, int cnt, bool prof = false) {
+// keep: the layered kernels, which leave the activations in H for the backward pass
+static int mlp_forward(esm_handle *h, int cnt, bool prof = false, bool keep = false) {
     MlpState &s = *h->mlp;
     const MlpDims &D = s.D;
     int rc;
+    if (!keep && s.fused && D.dh == 256 && D.din % GN_K == 0 && D.din <= 256 && cnt > 0) {
+        ProfScope ps(h, prof ? ESM_KERN_MLPF : -1);
+        FwdArgs f{{s.wt16 + s.ow[0], s.wt16 + s.ow[1], s.wt16 + s.ow[2], s.wt16 + s.ow[3]},
+                  {s.p + s.ob[0], s.p + s.ob[1], s.p + s.ob[2], s.p + s.ob[3]}, D.din, D.dh, D.dout};
+        hipLaunchKernelGGL(k_mlp_fwd, dim3((unsigned)((cnt + FF_M - 1) / FF_M)), dim3(64 * FF_WAVES), 0, h->stream, cnt,
+                           f, s.X, s.OUT);
+        ESM_HIP_CHECK(hipGetLastError());
+        return 0;
+    }
     const uint16_t *in = s.X;
     for (int l = 0; l < 3; l++) {
         ProfScope ps(h, prof ? ESM_KERN_MLPF : -1);
@@ -873,6 +1013,14 @@ int32_t esm_mlp_params(esm_handle *h, double *get, const double *set) {
     return ESM_OK;
 }
 
+// the forward pass of the proposals and of esm_mlp_forward: fused (1, default) or layered (0)
+int32_t esm_mlp_options(esm_handle *h, int32_t fused) {
+    if (!h || !h->mlp) { set_error("esm_mlp_options: no network (esm_mlp_init)"); return ESM_EINVAL; }
+    if (fused != 0 && fused != 1) { set_error("esm_mlp_options: fused is 0 or 1"); return ESM_EINVAL; }
+    h->mlp->fused = fused == 1;
+    return ESM_OK;
+}
+
 // Adam's state: [m (np)][v (np)][t], FP32 moments as doubles (exact both ways)
 int32_t esm_mlp_optimizer(esm_handle *h, double *get, const double *set) {
     if (!h || !h->mlp) { set_error("esm_mlp_optimizer: no network (esm_mlp_init)"); return ESM_EINVAL; }
@@ -933,7 +1081,7 @@ int32_t esm_mlp_gradient(esm_handle *h, const double *x, const double *y, const 
     ESM_HIP_CHECK(hipMemcpyAsync(s.Y, yf.data(), yf.size() * 4, hipMemcpyHostToDevice, h->stream));
     ESM_HIP_CHECK(hipMemcpyAsync(s.MU0, mf.data(), mf.size() * 4, hipMemcpyHostToDevice, h->stream));
     int rc;
-    if ((rc = mlp_forward(h, (int)M))) return rc;
+    if ((rc = mlp_forward(h, (int)M, false, true))) return rc;
     ESM_HIP_CHECK(hipMemsetAsync(s.g, 0, s.np * 4, h->stream));
     ESM_HIP_CHECK(hipMemsetAsync(s.dloss, 0, 8, h->stream));
     const unsigned nbk = (unsigned)((M + MLP_GPB - 1) / MLP_GPB);
@@ -975,7 +1123,7 @@ int32_t esm_mlp_train(esm_handle *h, int32_t steps, int32_t batch, double lr, co
             ESM_HIP_CHECK(hipGetLastError());
             rc = h->prec == ESM_FP64 ? mlp_features<double>(h, 0, s.idx, batch) : mlp_features<float>(h, 0, s.idx, batch);
             if (rc) return rc;
-            if ((rc = mlp_forward(h, batch))) return rc;
+            if ((rc = mlp_forward(h, batch, false, true))) return rc;
             ESM_HIP_CHECK(hipMemsetAsync(s.g, 0, s.np * 4, h->stream));
             const unsigned nbk = (unsigned)((batch + MLP_GPB - 1) / MLP_GPB);
             hipLaunchKernelGGL(k_mlp_head_grad, dim3(nbk), dim3(256), 0, h->stream, s.OUT, s.MU0, s.Y, batch, D.K, D.B,

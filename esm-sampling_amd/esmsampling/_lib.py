@@ -111,6 +111,7 @@ SIGNATURES = [
     ("esm_mlp_dims", C.c_int32, [C.c_void_p, _ip]),
     ("esm_mlp_params", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("esm_mlp_optimizer", C.c_int32, [C.c_void_p, _dp, _dp]),
+    ("esm_mlp_options", C.c_int32, [C.c_void_p, C.c_int32]),
     ("esm_mlp_forward", C.c_int32, [C.c_void_p, _dp, C.c_int64, _dp]),
     ("esm_mlp_gradient", C.c_int32, [C.c_void_p, _dp, _dp, _dp, C.c_int64, _dp, _dp]),
     ("esm_mlp_train", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, _u32p, C.c_uint32, _dp]),

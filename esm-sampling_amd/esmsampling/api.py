@@ -484,6 +484,10 @@ class LearnedProposal:
         check(self.R.lib.esm_mlp_params(self.R.handle, dptr(out), dptr(src) if src is not None else None), "esm_mlp_params")
         return out
 
+    def options(self, fused: bool = True):
+        """The forward pass fused into one kernel per 128-row tile (default) or one kernel per layer."""
+        check(self.R.lib.esm_mlp_options(self.R.handle, 1 if fused else 0), "esm_mlp_options")
+
     def optimizer(self, set_to=None):
         """Adam's state [m (nparam)][v (nparam)][t] (``esm_mlp_optimizer``); with ``set_to``, set it first."""
         out = np.zeros(2 * self.nparam + 1)

@@ -378,6 +378,10 @@ int32_t esm_mlp_params(esm_handle *h, double *get, const double *set);
  * learned proposal has no counterpart in the reference; this serves its driver's snapshot
  * (`write_replica_snapshot`, src/replicas.jl:1082-1124, extended by the network's state). */
 int32_t esm_mlp_optimizer(esm_handle *h, double *get, const double *set);
+/* esm_mlp_options: the forward pass of esm_mlp_propose and esm_mlp_forward as one fused kernel
+ * per 128-row tile (fused = 1, the default) or as one kernel per layer (0); the outputs are the
+ * same bit for bit. No counterpart in the reference (the learned proposal is new). */
+int32_t esm_mlp_options(esm_handle *h, int32_t fused);
 int32_t esm_mlp_forward(esm_handle *h, const double *x, int64_t M, double *out);
 int32_t esm_mlp_gradient(esm_handle *h, const double *x, const double *y, const double *mu0, int64_t M, double *grad,
                          double *loss);

@@ -49,6 +49,29 @@ def test_forward_matches_oracle(ncomp):
     dev.close()
 
 
+@pytest.mark.parametrize("ncomp", [2, 4, 8])   # 144, 320 and 576 outputs: 192-column passes, masked or full
+def test_fused_forward_is_the_layered_forward_bit_for_bit(ncomp):
+    """The fused forward (one kernel per 128-row tile, activations in LDS) against the one-kernel-
+    per-layer form: identical outputs for row counts that leave a partial tile."""
+    print(f"mlp fused forward ncomp={ncomp}", flush=True)
+    s = synth.make_system(N=6, L=8, T=48, dt=1.0, seed=3)
+    dev = api.Replicas(s, 2, FP64)
+    net = api.LearnedProposal(dev, block_hours=24, ncomp=ncomp, key=(3, 5))
+    _perturbed(net, 4, scale=2.0)
+    for M in (1, 700, 4096 + 77):
+        X = np.random.default_rng(M).normal(0, 1, (M, net.din))
+        net.options(fused=True)
+        a = net.forward(X)
+        net.options(fused=False)
+        b = net.forward(X)
+        assert np.array_equal(a, b), (M, np.max(np.abs(a - b)))
+        assert np.all(np.isfinite(a)) and np.abs(a).max() > 0
+    net.options(fused=True)
+    with pytest.raises(Exception, match="fused is 0 or 1"):
+        api.check(dev.lib.esm_mlp_options(dev.handle, 2), "esm_mlp_options")
+    dev.close()
+
+
 def test_gradient_matches_oracle():
     print("mlp gradient", flush=True)
     s = synth.make_system(N=6, L=8, T=48, dt=1.0, seed=3)
