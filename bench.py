@@ -82,7 +82,8 @@ class Workload:
         import numpy as np
         from esmsampling import api, synth
         self.np, self.api = np, api
-        self.s = synth.make_system(N=N, L=L, T=T, dt=8760.0 / T, seed=seed)
+        # one weather year over T steps; longer horizons are several years at hourly resolution
+        self.s = synth.make_system(N=N, L=L, T=T, dt=8760.0 / T if T <= 8760 else 1.0, seed=seed)
         self.R, self.seed = R, seed
         self.dev = api.Replicas(self.s, R, prec)
         api.sample_weather(self.dev, key=(seed, 7), counter=0)
@@ -207,19 +208,28 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     T, R = args.hours, args.replicas
+    tw = time.perf_counter()
     wl = Workload(args.nodes, args.lines, T, R, prec)
+    print(f"workload ready ({args.nodes} n x {T} h x {R} r): {time.perf_counter() - tw:.1f} s", file=sys.stderr,
+          flush=True)
     api.sweep_options(wl.dev, p_storage_ms=args.p_storage_ms)
     api.storage_cross_options(wl.dev, p_xpair=args.xpair, nxlong=args.xlong)
     wl.xpair, wl.xlong = args.xpair, args.xlong
     launch = apply_tuning(api, wl.dev, args.tune)
     lib, h = wl.dev.lib, wl.dev.handle
+    tw = time.perf_counter()
     for it in range(args.warmup):
         wl.step(it)
+        print(f"warmup step {it + 1}/{args.warmup}: {time.perf_counter() - tw:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     lib.esm_profile(h, 1)
     t0 = time.perf_counter()
+    tp = t0
     for it in range(args.warmup, args.warmup + args.steps):
         parts = wl.step(it)
+        if time.perf_counter() - tp > 20.0:   # a progress line on long steps (the full-size configuration)
+            tp = time.perf_counter()
+            print(f"timed step {it + 1 - args.warmup}/{args.steps}: {tp - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     names = {_lib.KERN_ENERGY: "k_energy", _lib.KERN_SCORE: "k_score", _lib.KERN_WEATHER: "weather",

@@ -111,10 +111,12 @@ struct XKinkD {
 // ---------------------------------------------------------------------------------------------
 
 // A move is owned by a segment of W lanes (W = 32 or 64: the segments of a wave are replicas).
-constexpr int XL_KS = 1;                 // kink slots per lane: at most W kinks of both ends
+// kink slots per lane KS (a template parameter): at most KS·W kinks of both ends; 1, or 2 for the
+// largest templates (hub nodes of 256-node networks: up to 46 leak nodes per end), which also take
+// a third balance slot (up to 66 balances per end)
 constexpr int XL_LS = 3;                 // line slots: at most 3 W lines of both ends
-constexpr int XL_NS = 2;                 // node slots: at most 2 W balances of both ends
-__host__ __device__ constexpr int xl_nx(int W) { return XL_KS * W + 4; }
+__host__ __device__ constexpr int xl_ns(int KS) { return KS + 1; }   // node slots: (KS + 1)·W balances
+__host__ __device__ constexpr int xl_nx(int W, int KS) { return KS * W + 4; }
 
 // the LDS of one segment's pair move
 constexpr int XL_NP = 32;   // walk points kept per side for the selection
@@ -134,36 +136,37 @@ __device__ __forceinline__ float xl_inward_hi(double x) {
     return (double)f > x ? nextafterf(f, -INFINITY) : f;
 }
 
-template <typename S, int W>
+template <typename S, int W, int KS = 1>
 struct XlLds {
-    S xv[xl_nx(W)];          // at the pass's t: each kink's and storage term's change of the balances
+    S xv[xl_nx(W, KS)];          // at the pass's t: each kink's and storage term's change of the balances
     S pts[2 * XL_NP * 2];    // the mass walks' points (sample_stor_pts)
 };
 
 // whether a template of K kinks, Lw lines and Nn balances per end fits a segment of W lanes
 // (checked on the host for every slot)
-__host__ __device__ __forceinline__ bool xl_fits_counts(int W, int K, int Lw, int Nn, int NLp) {
+__host__ __device__ __forceinline__ bool xl_fits_counts(int W, int KS, int K, int Lw, int Nn, int NLp) {
     (void)NLp;
-    return 2 * K <= W * XL_KS && 2 * Lw <= W * XL_LS && 2 * Nn <= W * XL_NS;
+    return 2 * K <= W * KS && 2 * Lw <= W * XL_LS && 2 * Nn <= W * xl_ns(KS);
 }
-template <typename S, int W>
+template <typename S, int W, int KS>
 __device__ __forceinline__ bool xl_fits(const SxTab<S> &tb, int slot) {
-    return xl_fits_counts(W, tb.kk_off[slot + 1] - tb.kk_off[slot], tb.ln_off[slot + 1] - tb.ln_off[slot],
+    return xl_fits_counts(W, KS, tb.kk_off[slot + 1] - tb.kk_off[slot], tb.ln_off[slot + 1] - tb.ln_off[slot],
                           tb.nd_off[slot + 1] - tb.nd_off[slot],
                           tb.nd_l[tb.nd_off[slot + 1]] - tb.nd_l[tb.nd_off[slot]]);
 }
 
 // IN: the move has interior steps (sxlong); sxpair (L = 1) compiles without them. WW: the
 // segment's lanes.
-template <typename S, bool IN, int WW>
+template <typename S, bool IN, int WW, int KSS = 1>
 struct FXLane {
     enum { REC_NONE, REC_B, REC_X, REC_A, REC_T };
     static constexpr bool CACHED = true;
     static constexpr int W = WW;
+    static constexpr int XL_KS = KSS, XL_NS = xl_ns(KSS);
     static constexpr int XK = 0, XS = XL_KS * W;
     const DevModel<S> *m;
     SxTab<S> tb;
-    XlLds<S, W> *sh;
+    XlLds<S, W, XL_KS> *sh;
     const S *ye0, *ye1;    // the units' state-of-charge planes at (node, ·, replica), stride R
     const S *rp, *Fp, *gp; // residual, flow and gas planes at (·, ·, replica)
     int T, R, lane, L, a0, slot;   // lane: the lane within the segment
@@ -560,9 +563,9 @@ struct FXLane {
 // a+1 … a+L, for replica j on the calling wave (lane `lane`); false when it is not made (κ_A or κ_B
 // below 1: the same in every lane and replica). Every lane reads its items' values at t = 0.
 // cw (IN): the wave's LDS of 2 XL_MAXCH·64 floats for the chunk windows.
-template <typename S, bool IN, int W>
-__device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S> &m, const SweepState<S> &st,
-                                           const SxTab<S> &tb, XlLds<S, W> *sh, int sa, int na, int sb, int nb,
+template <typename S, bool IN, int W, int KS>
+__device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W, KS> &f, const DevModel<S> &m, const SweepState<S> &st,
+                                           const SxTab<S> &tb, XlLds<S, W, KS> *sh, int sa, int na, int sb, int nb,
                                            int slot, int a, int L, int j, int lane, S kmin, float *cw = nullptr) {
     const int N = m.N, T = m.T, R = m.R;
     const size_t TR = (size_t)T * R;
@@ -609,8 +612,8 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
                     const S y0 = f.y(u, ta), y1 = f.y(u, f.next(ta)), cu = u == 0 ? +f.cu0 : +f.cu1;
                     if (S(0) <= y0 && y0 <= S(1) && S(0) <= y1 && y1 <= S(1)) {
                         S l0, h0, l1, h1;
-                        FXLane<S, IN, W>::box(y0, cu, l0, h0);
-                        FXLane<S, IN, W>::box(y1, cu, l1, h1);
+                        FXLane<S, IN, W, KS>::box(y0, cu, l0, h0);
+                        FXLane<S, IN, W, KS>::box(y1, cu, l1, h1);
                         clo = fmax(clo, fmax(l0, l1));
                         chi = fmin(chi, fmin(h0, h1));
                     } else {
@@ -636,7 +639,7 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
     // locals and stored once: stores to different fields under branches would be merged into stores
     // through selected addresses.)
 #pragma unroll
-    for (int s = 0; s < XL_KS; s++) {
+    for (int s = 0; s < KS; s++) {
         const int i = lane + W * s;
         int n = 0, okv = 0;
         S Q = 0, g = 0, ak = 0, w = 0, c0 = 0, p0 = 0, P = 0, e0 = 0;
@@ -737,14 +740,14 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
         f.sdB = 0;
     }
 #pragma unroll
-    for (int s = 0; s < XL_NS; s++) {
+    for (int s = 0; s < xl_ns(KS); s++) {
         const int i = lane + W * s;
         int n = -1, kx = -1;
         S S0 = 0, ph = 0, cf = 0;
         if (i < 2 * f.Nn) {
             const int e = i >= f.Nn ? 1 : 0, p = n0 + (i - e * f.Nn);
             n = tb.nd_node[p];
-            kx = tb.nd_kk[p] >= 0 ? FXLane<S, IN, W>::XK + e * f.K + tb.nd_kk[p] : -1;
+            kx = tb.nd_kk[p] >= 0 ? FXLane<S, IN, W, KS>::XK + e * f.K + tb.nd_kk[p] : -1;
             S0 = f.rv(n, e == 0 ? tA : tB);
             ph = phibal(S0, m.c_shed, m.c_dump);
             const S c = f.sc(e);
@@ -764,10 +767,10 @@ __device__ __forceinline__ bool xl_prepare(FXLane<S, IN, W> &f, const DevModel<S
     }
     // which kinks are made with a nonzero share (the balances take their gas change)
 #pragma unroll
-    for (int s = 0; s < XL_KS; s++) sh->xv[FXLane<S, IN, W>::XK + lane + W * s] = (f.kok[s] & 2) ? S(1) : S(0);
+    for (int s = 0; s < KS; s++) sh->xv[FXLane<S, IN, W, KS>::XK + lane + W * s] = (f.kok[s] & 2) ? S(1) : S(0);
     f.sync();
 #pragma unroll
-    for (int s = 0; s < XL_NS; s++) f.nk[s] = (f.nk[s] >= 0 && sh->xv[f.nk[s]] != S(0)) ? f.nk[s] : -1;
+    for (int s = 0; s < xl_ns(KS); s++) f.nk[s] = (f.nk[s] >= 0 && sh->xv[f.nk[s]] != S(0)) ? f.nk[s] : -1;
     f.sync();
     return true;
 }

@@ -270,6 +270,8 @@ template <typename S>
 int launch_storage_long(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats);
 template <typename S>
 int launch_sxlong(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double maxnats, double pcoin);
+// k_random.hip: the storage pairs' lane form (lanes 32 or 64, kink slots ks) that every template fits
+int sx_lane_form(const esm_handle *h, int wmin, int &ks);
 size_t sxm_ints(size_t cap);   // ints of the sxlong schedule buffer for `cap` moves (k_cross.hip)
 
 // k_follow.hip

@@ -136,12 +136,12 @@ __global__ void __launch_bounds__(64) k_sx_schedule(DevModel<S> m, SxArgs<S> sx,
     }
 }
 
-template <typename S>
+template <typename S, int KS>
 __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepState<S> st, SxArgs<S> sx,
                                                            const int *buf, int cap, uint32_t k0, uint32_t k1,
                                                            uint32_t ctr, unsigned long long *stats, S beta, S maxnats,
                                                            S kmin) {
-    __shared__ XlLds<S, 64> shm[SXL_WAVES];
+    __shared__ XlLds<S, 64, KS> shm[SXL_WAVES];
     __shared__ float cwin[SXL_WAVES][2 * XL_MAXCH * 64];
     // replica j: a contiguous eighth of the replicas per XCD (neighbouring replicas share cache lines)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -155,11 +155,11 @@ __global__ void __launch_bounds__(64 * SXL_WAVES) k_sxlong(DevModel<S> m, SweepS
         for (int i = roff[r] + wave; i < roff[r + 1]; i += SXL_WAVES) {
             const int *o = mv + 8 * i;
             const int sa = o[0], na = o[1], sb = o[2], nb = o[3], slot = o[4], a = o[5], L = o[6];
-            if (!xl_fits<S, 64>(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
+            if (!xl_fits<S, 64, KS>(sx.tb, slot)) {   // larger than the lane form: a guard (t = 0)
                 if (lane == 0) out.add(2, false, 0);
                 continue;
             }
-            FXLane<S, true, 64> f;
+            FXLane<S, true, 64, KS> f;
             if (!xl_prepare(f, m, st, sx.tb, &shm[wave], sa, na, sb, nb, slot, a, L, j, lane, kmin, cwin[wave])) continue;
             double u1, u2;
             uniforms2(k0, k1, j, (uint32_t)i, 0x5C000000u, ctr, u1, u2);
@@ -202,9 +202,17 @@ int launch_sxlong(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
     hipLaunchKernelGGL((k_sx_schedule<S>), dim3(1), dim3(64), 0, h->stream, m, sx, h->sweep_nxlong, cap, k0, k1, ctr,
                        pcoin, h->d_sxm);
     ESM_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL((k_sxlong<S>), dim3((unsigned)h->R), dim3(64 * SXL_WAVES), 0, h->stream, m, st, sx,
-                       (const int *)h->d_sxm, cap, k0, k1, ctr,
-                       h->d_stats + (size_t)ESM_FAM_SXLONG * ESM_NSTAT * STAT_SLOTS, (S)h->m.beta_v, (S)maxnats, S(1));
+    // one kink slot per lane unless some template has more than 32 leak nodes per end; a template
+    // beyond two slots (none in the networks built here) makes its moves guards
+    int ks = 1;
+    if (sx_lane_form(h, 64, ks) == 0) ks = 2;
+    unsigned long long *stats = h->d_stats + (size_t)ESM_FAM_SXLONG * ESM_NSTAT * STAT_SLOTS;
+    if (ks == 1)
+        hipLaunchKernelGGL((k_sxlong<S, 1>), dim3((unsigned)h->R), dim3(64 * SXL_WAVES), 0, h->stream, m, st, sx,
+                           (const int *)h->d_sxm, cap, k0, k1, ctr, stats, (S)h->m.beta_v, (S)maxnats, S(1));
+    else
+        hipLaunchKernelGGL((k_sxlong<S, 2>), dim3((unsigned)h->R), dim3(64 * SXL_WAVES), 0, h->stream, m, st, sx,
+                           (const int *)h->d_sxm, cap, k0, k1, ctr, stats, (S)h->m.beta_v, (S)maxnats, S(1));
     ESM_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -209,13 +209,14 @@ __global__ void __launch_bounds__(256, min_waves<S>(3)) k_rand_spair(DevModel<S>
 
 constexpr int SXP_WAVES = 4;   // waves per workgroup of the sxpair launch
 
-// W: the lanes of a move (32: two replicas per wave, when every template fits; else 64)
-template <typename S, int W>
+// W: the lanes of a move (32: two replicas per wave, when every template fits; else 64); KS:
+// kink slots per lane (2 when some template has more than W/2 leak nodes per end)
+template <typename S, int W, int KS>
 __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxpair(DevModel<S> m, SweepState<S> st,
                                                                                      RandArgs a, SxArgs<S> sx, S beta,
                                                                                      S maxnats, S kmin) {
     constexpr int SEGS = 64 / W;
-    __shared__ XlLds<S, W> shm[SXP_WAVES][SEGS];
+    __shared__ XlLds<S, W, KS> shm[SXP_WAVES][SEGS];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, seg = lane / W, sub = lane % W;
     const int ngrp = (m.R + SEGS - 1) / SEGS;
     // the wave: (step, group of SEGS replicas); the blocks of one step on one XCD (their replicas
@@ -236,11 +237,11 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxp
             if (m.kc[(ESM_KSTOR + sa) * N + na] * m.hours[ua] / m.dt < S(1)) continue;
             int slot, sb, nb;
             if (!sx_partner(m, sx, sa, na, u01(s[2], s[3]), slot, sb, nb)) continue;
-            if (!xl_fits<S, W>(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
+            if (!xl_fits<S, W, KS>(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
                 if (sub == 0) out.add(2, false, 0);
                 continue;
             }
-            FXLane<S, false, W> f;
+            FXLane<S, false, W, KS> f;
             if (!xl_prepare(f, m, st, sx.tb, &shm[wv][seg], sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j,
                             sub, kmin))
                 continue;
@@ -274,20 +275,28 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxp
     add_counts(a.stats, out);
 }
 
-// the lanes of an sxpair move: 32 when every pair template of the system fits 32 lanes, else 64;
-// 0 when some template is too large for the lane form
-static int sxpair_width(const esm_handle *h) {
+// the lane form of the storage pairs: the narrowest of (32 lanes, 1 kink slot), (64, 1), (64, 2)
+// with at least wmin lanes that every pair template of the system fits; 0 when none does (ks: the
+// kink slots)
+int sx_lane_form(const esm_handle *h, int wmin, int &ks) {
     const MoveTablesHost &t = h->mt;
-    int w = 32;
-    for (size_t sl = 0; sl + 1 < t.sx_kk_off.size(); sl++) {
-        const int K = t.sx_kk_off[sl + 1] - t.sx_kk_off[sl], Lw = t.sx_ln_off[sl + 1] - t.sx_ln_off[sl];
-        const int Nn = t.sx_nd_off[sl + 1] - t.sx_nd_off[sl];
-        const int NLp = t.sx_nd_l[t.sx_nd_off[sl + 1]] - t.sx_nd_l[t.sx_nd_off[sl]];
-        if (xl_fits_counts(w, K, Lw, Nn, NLp)) continue;
-        if (w == 32 && xl_fits_counts(64, K, Lw, Nn, NLp)) w = 64;
-        else return 0;
+    static const int forms[3][2] = {{32, 1}, {64, 1}, {64, 2}};
+    for (const auto &f : forms) {
+        if (f[0] < wmin) continue;
+        bool ok = true;
+        for (size_t sl = 0; ok && sl + 1 < t.sx_kk_off.size(); sl++) {
+            const int K = t.sx_kk_off[sl + 1] - t.sx_kk_off[sl], Lw = t.sx_ln_off[sl + 1] - t.sx_ln_off[sl];
+            const int Nn = t.sx_nd_off[sl + 1] - t.sx_nd_off[sl];
+            const int NLp = t.sx_nd_l[t.sx_nd_off[sl + 1]] - t.sx_nd_l[t.sx_nd_off[sl]];
+            ok = xl_fits_counts(f[0], f[1], K, Lw, Nn, NLp);
+        }
+        if (ok) {
+            ks = f[1];
+            return f[0];
+        }
     }
-    return w;
+    ks = 0;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1187,7 +1196,8 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
             b.stats = h->d_stats + (size_t)ESM_FAM_SXPAIR * ESM_NSTAT * STAT_SLOTS;
             const SxArgs<S> sx = sx_args<S>(h);
             ProfScope p(h, ESM_KERN_SXPAIR);
-            const int wx = sxpair_width(h);
+            int ks = 1;
+            const int wx = sx_lane_form(h, 32, ks);
             if (wx == 0) {
                 set_error("storage pairs: a pair template is larger than the lane form holds");
                 return ESM_EINVAL;
@@ -1201,8 +1211,9 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 nblocks = (nblocks + round - 1) / round * round;
             }
             const dim3 grid((unsigned)nblocks), blk(64 * SXP_WAVES);
-            if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair<S, 32>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
-            else hipLaunchKernelGGL((k_rand_sxpair<S, 64>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair<S, 32, 1>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            else if (ks == 1) hipLaunchKernelGGL((k_rand_sxpair<S, 64, 1>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            else hipLaunchKernelGGL((k_rand_sxpair<S, 64, 2>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());
         }
     }

@@ -58,6 +58,7 @@ def load():
                                    C.c_int, i64p, C.c_int]
         lib.orc_sx_slot.argtypes = [M, C.c_int, C.c_int, _ip, _ip, _ip, _dp, _ip, _ip, _dp, _ip, _ip]
         lib.orc_sx_partner.argtypes = [M, _dp, C.c_double, C.c_int, C.c_int, C.c_double, _ip, _ip]
+        lib.orc_sx_sizes.argtypes = [M, C.c_int, _ip, _ip]
         lib.orc_sx_move.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_int, C.c_double, _dp, _dp, _dp, _ip]
         lib.orc_ref_sweeps.argtypes = [M, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _u32p, C.c_uint32, C.c_double,
@@ -255,6 +256,14 @@ class Oracle:
         assert rc == 0, rc
         return dict(partner=part.value, th=(thn[:nth.value].copy(), thv[:nth.value].copy()),
                     leak=(lkn[:nlk.value].copy(), lkg[:nlk.value].copy()), foot=ft[:nft.value].copy())
+
+    def sx_sizes(self):
+        """[slots, 4] per storage-pair slot: kinks, lines, balances, node–line entries per end."""
+        cap = 8 * 256 * 16
+        out = np.zeros(4 * cap, dtype=np.int32)
+        n = C.c_int()
+        assert self.lib.orc_sx_sizes(C.byref(self.cm), cap, C.byref(n), out.ctypes.data_as(_ip)) == 0
+        return out[:4 * min(n.value, cap)].reshape(-1, 4).copy()
 
     def sx_partner(self, k, sa, na, u, hmin=24.0):
         """The partner (sb, nb) of unit (sa, na) drawn with the uniform u, or None."""

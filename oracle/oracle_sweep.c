@@ -3500,6 +3500,22 @@ int orc_sx_slot(const esm_model *m, int n, int k, int *part, int *nth, int *thn,
     return rc;
 }
 
+/* per storage-pair slot s < *nslot (at most cap): out[4 s …] = kinks K, lines Lw, balances Nn and
+ * node–line entries per end (the sizes that decide the device's lane form) */
+int orc_sx_sizes(const esm_model *m, int cap, int *nslot, int *out) {
+    Tables tb;
+    if (m->N > 256 || build_tables(m, &tb)) return -1;
+    int n = tb.sx_kk_off.n - 1;
+    *nslot = n;
+    for (int sl = 0; sl < n && sl < cap; sl++) {
+        out[4 * sl + 0] = tb.sx_kk_off.a[sl + 1] - tb.sx_kk_off.a[sl];
+        out[4 * sl + 1] = tb.sx_ln_off.a[sl + 1] - tb.sx_ln_off.a[sl];
+        out[4 * sl + 2] = tb.sx_nd_off.a[sl + 1] - tb.sx_nd_off.a[sl];
+        out[4 * sl + 3] = tb.sx_nd_l.a[tb.sx_nd_off.a[sl + 1]] - tb.sx_nd_l.a[tb.sx_nd_off.a[sl]];
+    }
+    tables_free(&tb);
+    return 0;
+}
 int orc_sx_partner(const esm_model *m, const double *k, double hmin, int sa, int na, double u, int *sb, int *nb) {
     static __thread Sys S;
     int ptr[257], lines[2 * 2048];

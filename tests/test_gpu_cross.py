@@ -98,6 +98,34 @@ def test_full_sweep_with_storage_pairs_fp64_matches_oracle_40_nodes():
     dev.close()
 
 
+def test_storage_pairs_fp64_256_nodes_with_the_largest_templates():
+    """At 256 nodes / 450 lines the balls around hub nodes leave up to 46 leak nodes and 66
+    balances per end (the oracle's template sizes), beyond one kink slot and two balance slots of
+    the 64-lane form: both pair kernels take two kink slots and three balance slots per lane, and
+    sxpair and sxlong give the oracle's statistics and states (FP64)."""
+    print("storage pairs fp64 256 nodes", flush=True)
+    s = synth.make_system(N=256, L=450, T=12, dt=1.0, seed=1)
+    o = oracle.Oracle(s)
+    sizes = o.sx_sizes()
+    assert sizes[:, 0].max() > 32 and sizes[:, 2].max() > 64
+    R = 2
+    v, x = _state(s, R, seed=5)
+    dev = api.Replicas(s, R, FP64)
+    dev.upload(v=v, k=s.k0, x=x)
+    api.sweep_options(dev, extra=2.0, weights=np.zeros(9), atoms=False, p_storage_ms=0.0)
+    api.storage_long_options(dev, p=1.0, nmoves=0, hmin=24.0)
+    api.storage_cross_options(dev, p_xpair=0.1, nxlong=2)
+    st = api.operation_sweep(dev, 1, key=(5, 5), counter=2)
+    vd, _, _ = dev.download()
+    rd, Fd = dev.caches()
+    v_, r_, F_, st_ = o.sweeps(v, s.k0, 1, key=(5, 5), counter=2, x=x, threads=8, extra=2.0, weights=np.zeros(9),
+                               atoms=False, p_storage_ms=0.0, p_storage_long=1.0, long_moves=0, long_hours=24.0,
+                               p_xpair=0.1, xlong_moves=2)
+    assert st_[13, 0] > 0 and st_[14, 0] > 0 and st[13, 3] == 0 and st[14, 3] == 0
+    _check(s, vd, rd, Fd, st, v_, r_, F_, st_)
+    dev.close()
+
+
 def test_storage_pairs_fp32_consistent():
     """FP32 at 64 replicas: the pair families make the oracle's moves (counts exact, acceptances
     within round-off-driven differences) and the caches stay within FP32 round-off of a rebuild."""
