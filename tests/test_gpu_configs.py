@@ -172,7 +172,8 @@ def test_full_size_256_nodes_192720_hours_32_replicas():
     """The full-size configuration (256 nodes / 450 lines, 22 hourly weather years = 192,720
     steps, 32 replicas, FP32: 1.6·10⁹ elements per node plane, 2.8·10⁹ per line field) on the
     device alone: prior weather, heuristic start, energy, one operation sweep with both storage
-    sweeps, energy again. Checks: per-step costs sum to the totals; the incremental caches equal
+    sweeps and the reference configuration's storage pairs (sxpair 0.1, sxlong 8: hub templates
+    in the 64-lane form with two kink and three balance slots), energy again. Checks: per-step costs sum to the totals; the incremental caches equal
     a rebuild (FP32 drift); all parts finite (a non-finite value anywhere in the state would
     reach them); no improper lines and no guard rejections; as many atoms and random moves as
     the schedule makes, every family accepting moves, at least one ΔC evaluation per move."""
@@ -189,6 +190,7 @@ def test_full_size_256_nodes_192720_hours_32_replicas():
     np.testing.assert_allclose(ph0.sum(axis=0), parts0[:, 1], rtol=1e-9)
     print("  one operation sweep", flush=True)
     api.sweep_options(dev, p_storage_ms=1.0)
+    api.storage_cross_options(dev, p_xpair=0.1, nxlong=8)
     st = api.operation_sweep(dev, 1, key=(5, 2), counter=0)
     print(f"  sweep statistics: {st[:, 0].tolist()}", flush=True)
     drift = api.rebuild_drift(dev)
@@ -198,11 +200,11 @@ def test_full_size_256_nodes_192720_hours_32_replicas():
     assert np.all(drift < 1e-2 * np.abs(s.load).max()), drift.max()
     # atoms: gas and angles at every (node, step), SoC for both carriers at every (node, step)
     assert st[0, 0] == N * T * R and st[1, 0] == N * T * R and st[2, 0] == 2 * N * T * R
-    # random moves: round(4N) draws per step over the plain families (pair, cellpair void)
-    assert 0 < st[3:10, 0].sum() <= 4 * N * T * R
+    # random moves: round(4N) draws per step over the plain families (pair, cellpair void) and sxpair
+    assert 0 < st[3:10, 0].sum() + st[13, 0] <= 4 * N * T * R * 1.2
     assert st[10, 0] > 0 and st[11, 0] > 0 and st[12, 0] > 0
     assert np.all(st[:, 2] == 0) and np.all(st[:, 3] == 0), st[:, 2:4]
-    # the cross-node storage pairs (the last two families) are off on a new handle
-    assert np.all(st[13:, :] == 0), st[13:]
-    assert np.all(st[:13, 1] > 0) and np.all(st[:, 4] >= st[:, 0])
+    # the cross-node storage pairs (the last two families) made and accepted moves
+    assert st[13, 0] > 0 and st[14, 0] > 0
+    assert np.all(st[:, 1] > 0) and np.all(st[:, 4] >= st[:, 0])
     dev.close()
