@@ -209,6 +209,56 @@ __global__ void __launch_bounds__(256, min_waves<S>(3)) k_rand_spair(DevModel<S>
 
 constexpr int SXP_WAVES = 4;   // waves per workgroup of the sxpair launch
 
+// one sxpair move of replica j on its segment of W lanes (lane `sub` of the segment)
+template <typename S, int W, int KS>
+__device__ __forceinline__ void sx_pair_move(const DevModel<S> &m, SweepState<S> &st, const RandArgs &a,
+                                             const SxArgs<S> &sx, XlLds<S, W, KS> *sh, int sa, int na, int sb,
+                                             int nb, int slot, int tau, int k, int j, int sub, S beta, S maxnats,
+                                             S kmin, MoveOut &out) {
+    const int N = m.N, T = m.T;
+    FXLane<S, false, W, KS> f;
+    if (!xl_prepare(f, m, st, sx.tb, sh, sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j, sub, kmin)) return;
+    double u1, u2;
+    line_uniforms(a, j, k, tau, u1, u2);
+    int status = 0;
+    S t = sample_stor_pts<XL_NP>(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status, sh->pts, sub);
+    if (t != 0) {
+        bool any;
+        const double lj = f.logjac(t, any);
+        if (any) {
+            uint32_t w[4];
+            philox4x32(a.k0, a.k1, (uint32_t)j, (uint32_t)tau, 0x60000000u | ((uint32_t)a.w << 20) | (uint32_t)k, a.ctr,
+                       w);
+            if (!(u01(w[0], w[1]) < exp(fmin(0.0, lj)))) {
+                t = 0;
+                status = 2;
+            }
+        }
+    }
+    if (sub == 0) out.add(status, t != 0, f.nev);
+    if (t != 0) {
+        const size_t TR = (size_t)T * m.R;
+        f.commit(t, st.r + j, st.F + j, st.v + plane(N, T, m.R) + j, st.v + j,
+                 st.v + plane((2 + sa) * N, T, m.R) + (size_t)na * TR + j,
+                 st.v + plane((2 + sb) * N, T, m.R) + (size_t)nb * TR + j);
+    }
+}
+
+// the move's selection (unit A, partner, slot), the same in every lane; false: no move
+template <typename S>
+__device__ __forceinline__ bool sx_select(const DevModel<S> &m, const SxArgs<S> &sx, const RandArgs &a, int k, int tau,
+                                          int &sa, int &na, int &sb, int &nb, int &slot) {
+    uint32_t s[4];
+    selection(a, k, tau, s);
+    uniform4<true>(s);
+    const int N = m.N;
+    const int ua = sx.units[pick(s[0], (uint32_t)sx.nsx)];
+    sa = ua / N;
+    na = ua % N;
+    if (m.kc[(ESM_KSTOR + sa) * N + na] * m.hours[ua] / m.dt < S(1)) return false;
+    return sx_partner(m, sx, sa, na, u01(s[2], s[3]), slot, sb, nb);
+}
+
 // W: the lanes of a move (32: two replicas per wave, when every template fits; else 64); KS:
 // kink slots per lane (2 when some template has more than W/2 leak nodes per end)
 template <typename S, int W, int KS>
@@ -225,50 +275,59 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxp
     const size_t g = (size_t)blk * SXP_WAVES + wv;
     MoveOut out;
     if (g < (size_t)a.ntau * ngrp && sx.nsx > 0) {
-        const int N = m.N, T = m.T;
         const int i = (int)(g / ngrp), j = (int)(g % ngrp) * SEGS + seg;
         const int tau = __builtin_amdgcn_readfirstlane(a.perm ? a.perm[i] : a.tau0 + i * a.tstride);
-        const int cnt = __builtin_amdgcn_readfirstlane(a.counts[(size_t)a.w * T + tau]);
+        const int cnt = __builtin_amdgcn_readfirstlane(a.counts[(size_t)a.w * m.T + tau]);
         for (int k = 0; k < cnt && j < m.R; k++) {
-            uint32_t s[4];
-            selection(a, k, tau, s);
-            uniform4<true>(s);
-            const int ua = sx.units[pick(s[0], (uint32_t)sx.nsx)], sa = ua / N, na = ua % N;
-            if (m.kc[(ESM_KSTOR + sa) * N + na] * m.hours[ua] / m.dt < S(1)) continue;
-            int slot, sb, nb;
-            if (!sx_partner(m, sx, sa, na, u01(s[2], s[3]), slot, sb, nb)) continue;
+            int sa, na, sb, nb, slot;
+            if (!sx_select(m, sx, a, k, tau, sa, na, sb, nb, slot)) continue;
             if (!xl_fits<S, W, KS>(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
                 if (sub == 0) out.add(2, false, 0);
                 continue;
             }
-            FXLane<S, false, W, KS> f;
-            if (!xl_prepare(f, m, st, sx.tb, &shm[wv][seg], sa, na, sb, nb, slot, tau == 0 ? T - 1 : tau - 1, 1, j,
-                            sub, kmin))
-                continue;
-            double u1, u2;
-            line_uniforms(a, j, k, tau, u1, u2);
-            int status = 0;
-            S t = sample_stor_pts<XL_NP>(f, beta, maxnats, unit_open<S>(u1), unit_open<S>(u2), status, shm[wv][seg].pts,
-                                         sub);
-            if (t != 0) {
-                bool any;
-                const double lj = f.logjac(t, any);
-                if (any) {
-                    uint32_t w[4];
-                    philox4x32(a.k0, a.k1, (uint32_t)j, (uint32_t)tau, 0x60000000u | ((uint32_t)a.w << 20) | (uint32_t)k,
-                               a.ctr, w);
-                    if (!(u01(w[0], w[1]) < exp(fmin(0.0, lj)))) {
-                        t = 0;
-                        status = 2;
-                    }
+            sx_pair_move<S, W, KS>(m, st, a, sx, &shm[wv][seg], sa, na, sb, nb, slot, tau, k, j, sub, beta, maxnats,
+                                   kmin, out);
+        }
+    }
+    add_counts(a.stats, out);
+}
+
+// The 32-lane form with four replicas per wave: a move whose template fits 16 lanes runs on four
+// 16-lane segments at once, any other on two 32-lane segments, twice (replicas 0–1, then 2–3 of the
+// wave). The moves of a replica keep their draw order.
+template <typename S>
+__global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxpair_mix(DevModel<S> m,
+                                                                                         SweepState<S> st, RandArgs a,
+                                                                                         SxArgs<S> sx, S beta,
+                                                                                         S maxnats, S kmin) {
+    __shared__ XlLds<S, 16, 1> sh16[SXP_WAVES][4];
+    __shared__ XlLds<S, 32, 1> sh32[SXP_WAVES][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ngrp = (m.R + 3) / 4;
+    const unsigned blk = xcd_block(blockIdx.x, gridDim.x, ngrp % SXP_WAVES == 0 ? ngrp / SXP_WAVES : 0);
+    const size_t g = (size_t)blk * SXP_WAVES + wv;
+    MoveOut out;
+    if (g < (size_t)a.ntau * ngrp && sx.nsx > 0) {
+        const int i = (int)(g / ngrp), j0 = (int)(g % ngrp) * 4;
+        const int tau = __builtin_amdgcn_readfirstlane(a.perm ? a.perm[i] : a.tau0 + i * a.tstride);
+        const int cnt = __builtin_amdgcn_readfirstlane(a.counts[(size_t)a.w * m.T + tau]);
+        for (int k = 0; k < cnt; k++) {
+            int sa, na, sb, nb, slot;
+            if (!sx_select(m, sx, a, k, tau, sa, na, sb, nb, slot)) continue;
+            if (xl_fits<S, 16, 1>(sx.tb, slot)) {
+                const int j = j0 + lane / 16;
+                if (j < m.R)
+                    sx_pair_move<S, 16, 1>(m, st, a, sx, &sh16[wv][lane / 16], sa, na, sb, nb, slot, tau, k, j,
+                                           lane % 16, beta, maxnats, kmin, out);
+            } else if (xl_fits<S, 32, 1>(sx.tb, slot)) {
+                for (int h = 0; h < 2; h++) {
+                    const int j = j0 + 2 * h + lane / 32;
+                    if (j < m.R)
+                        sx_pair_move<S, 32, 1>(m, st, a, sx, &sh32[wv][lane / 32], sa, na, sb, nb, slot, tau, k, j,
+                                               lane % 32, beta, maxnats, kmin, out);
                 }
-            }
-            if (sub == 0) out.add(status, t != 0, f.nev);
-            if (t != 0) {
-                const size_t TR = (size_t)T * m.R;
-                f.commit(t, st.r + j, st.F + j, st.v + plane(N, T, m.R) + j, st.v + j,
-                         st.v + plane((2 + sa) * N, T, m.R) + (size_t)na * TR + j,
-                         st.v + plane((2 + sb) * N, T, m.R) + (size_t)nb * TR + j);
+            } else if ((lane & 15) == 0 && j0 + lane / 16 < m.R) {   // refused on the host; a guard if it ever came
+                out.add(2, false, 0);
             }
         }
     }
@@ -1203,15 +1262,19 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 return ESM_EINVAL;
             }
             // the blocks of one step on one XCD: the grid padded to whole rounds of 8 XCDs × the
-            // step's blocks (the extra blocks find no wave of theirs and exit)
-            const size_t ngrp = (size_t)(R + 64 / wx - 1) / (64 / wx), nwaves = (size_t)a.ntau * ngrp;
-            size_t nblocks = (nwaves + SXP_WAVES - 1) / SXP_WAVES;
-            if (ngrp % SXP_WAVES == 0) {
-                const size_t round = 8 * (ngrp / SXP_WAVES);
-                nblocks = (nblocks + round - 1) / round * round;
-            }
-            const dim3 grid((unsigned)nblocks), blk(64 * SXP_WAVES);
-            if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair<S, 32, 1>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            // step's blocks (the extra blocks find no wave of theirs and exit); a wave holds 64/W
+            // replicas (the 32-lane form: four, in 16- or 32-lane segments per move)
+            auto blocks_for = [&](size_t per_wave) {
+                const size_t ngrp = ((size_t)R + per_wave - 1) / per_wave, nwaves = (size_t)a.ntau * ngrp;
+                size_t nblocks = (nwaves + SXP_WAVES - 1) / SXP_WAVES;
+                if (ngrp % SXP_WAVES == 0) {
+                    const size_t round = 8 * (ngrp / SXP_WAVES);
+                    nblocks = (nblocks + round - 1) / round * round;
+                }
+                return dim3((unsigned)nblocks);
+            };
+            const dim3 grid = blocks_for((size_t)(64 / wx)), gridm = blocks_for(4), blk(64 * SXP_WAVES);
+            if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair_mix<S>), gridm, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             else if (ks == 1) hipLaunchKernelGGL((k_rand_sxpair<S, 64, 1>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             else hipLaunchKernelGGL((k_rand_sxpair<S, 64, 2>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());

@@ -3371,7 +3371,13 @@ static void sxpair_move(Sys *S, const Tables *tb, int j, int tau, const uint32_t
     if (S->kc[(ESM_KSTOR + sa) * N + na] * m->stor_hours[ua] / m->dt < 1.0) return;
     if (!xm_partner(S, tb, hmin, sa, na, u01(s[2], s[3]), &slot, &sb, &nb)) return;
     XMove f;
-    if (!xm_prepare(&f, S, tb, j, sa, na, sb, nb, slot, (tau - 1 + T) % T, 1, xm_pair_width(tb))) return;
+    /* the device's 32-lane form runs a move whose template fits 16 lanes (kinks ≤ 8, lines ≤ 24,
+     * balances ≤ 16 per end) on a 16-lane segment */
+    int W = xm_pair_width(tb);
+    int K = tb->sx_kk_off.a[slot + 1] - tb->sx_kk_off.a[slot], Lw = tb->sx_ln_off.a[slot + 1] - tb->sx_ln_off.a[slot];
+    int Nn = tb->sx_nd_off.a[slot + 1] - tb->sx_nd_off.a[slot];
+    if (W == 32 && 2 * K <= 16 && 2 * Lw <= 48 && 2 * Nn <= 32) W = 16;
+    if (!xm_prepare(&f, S, tb, j, sa, na, sb, nb, slot, (tau - 1 + T) % T, 1, W)) return;
     xm_move(&f, useg, uoff, umh, maxnats, stats);
 }
 /* the sxlong moves of a sweep (the same for every replica): per unit of the pairs with κ ≥ 1 and
