@@ -547,10 +547,13 @@ def measured_traffic(kernel, s, T, R, prec, form=1):
              "k_xpair": "k_sparse<float, 3, true>", "k_storage_ms": "k_storage_ms",
              "k_storage_long": "k_storage_long<float, true>",
              "k_random": f"k_sparse_all<float, true, {'true' if form > 1 else 'false'}>",
-             "k_sxlong": "k_sxlong<float>"}
+             "k_sxlong": "k_sxlong<float, 1>"}
     got = json.loads(p.read_text())["bytes_per_dispatch"]
-    if kernel == "k_sxpair":   # the pair kernel's segment width (32 or 64 lanes) is in its name
-        return got.get("k_rand_sxpair<float, 32>", got.get("k_rand_sxpair<float, 64>"))
+    if kernel == "k_sxpair":   # the pair kernel's form (mixed 16/32 lanes, or 64 lanes) is in its name
+        for name in ("k_rand_sxpair_mix<float>", "k_rand_sxpair<float, 64, 1>", "k_rand_sxpair<float, 64, 2>"):
+            if name in got:
+                return got[name]
+        return None
     return got.get(names.get(kernel, ""))
 
 
