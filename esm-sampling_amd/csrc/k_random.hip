@@ -259,49 +259,18 @@ __device__ __forceinline__ bool sx_select(const DevModel<S> &m, const SxArgs<S> 
     return sx_partner(m, sx, sa, na, u01(s[2], s[3]), slot, sb, nb);
 }
 
-// W: the lanes of a move (32: two replicas per wave, when every template fits; else 64); KS:
-// kink slots per lane (2 when some template has more than W/2 leak nodes per end)
-template <typename S, int W, int KS>
-__global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxpair(DevModel<S> m, SweepState<S> st,
-                                                                                     RandArgs a, SxArgs<S> sx, S beta,
-                                                                                     S maxnats, S kmin) {
-    constexpr int SEGS = 64 / W;
-    __shared__ XlLds<S, W, KS> shm[SXP_WAVES][SEGS];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, seg = lane / W, sub = lane % W;
-    const int ngrp = (m.R + SEGS - 1) / SEGS;
-    // the wave: (step, group of SEGS replicas); the blocks of one step on one XCD (their replicas
-    // share cache lines)
-    const unsigned blk = xcd_block(blockIdx.x, gridDim.x, ngrp % SXP_WAVES == 0 ? ngrp / SXP_WAVES : 0);
-    const size_t g = (size_t)blk * SXP_WAVES + wv;
-    MoveOut out;
-    if (g < (size_t)a.ntau * ngrp && sx.nsx > 0) {
-        const int i = (int)(g / ngrp), j = (int)(g % ngrp) * SEGS + seg;
-        const int tau = __builtin_amdgcn_readfirstlane(a.perm ? a.perm[i] : a.tau0 + i * a.tstride);
-        const int cnt = __builtin_amdgcn_readfirstlane(a.counts[(size_t)a.w * m.T + tau]);
-        for (int k = 0; k < cnt && j < m.R; k++) {
-            int sa, na, sb, nb, slot;
-            if (!sx_select(m, sx, a, k, tau, sa, na, sb, nb, slot)) continue;
-            if (!xl_fits<S, W, KS>(sx.tb, slot)) {   // refused on the host; counted as a guard if it ever came
-                if (sub == 0) out.add(2, false, 0);
-                continue;
-            }
-            sx_pair_move<S, W, KS>(m, st, a, sx, &shm[wv][seg], sa, na, sb, nb, slot, tau, k, j, sub, beta, maxnats,
-                                   kmin, out);
-        }
-    }
-    add_counts(a.stats, out);
-}
-
-// The 32-lane form with four replicas per wave: a move whose template fits 16 lanes runs on four
-// 16-lane segments at once, any other on two 32-lane segments, twice (replicas 0–1, then 2–3 of the
-// wave). The moves of a replica keep their draw order.
-template <typename S>
+// Four replicas per wave: each move runs on the narrowest segments its template fits — four
+// 16-lane segments at once, two 32-lane segments twice, or (K64 > 0: a system with templates
+// beyond 32 lanes) one 64-lane segment per replica, four times, with K64 kink slots per lane. The
+// moves of a replica keep their draw order; the oracle sums each move over the same lanes.
+template <typename S, int K64>
 __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxpair_mix(DevModel<S> m,
                                                                                          SweepState<S> st, RandArgs a,
                                                                                          SxArgs<S> sx, S beta,
                                                                                          S maxnats, S kmin) {
     __shared__ XlLds<S, 16, 1> sh16[SXP_WAVES][4];
     __shared__ XlLds<S, 32, 1> sh32[SXP_WAVES][2];
+    __shared__ XlLds<S, 64, (K64 > 0 ? K64 : 1)> sh64[SXP_WAVES];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int ngrp = (m.R + 3) / 4;
     const unsigned blk = xcd_block(blockIdx.x, gridDim.x, ngrp % SXP_WAVES == 0 ? ngrp / SXP_WAVES : 0);
@@ -325,6 +294,13 @@ __global__ void __launch_bounds__(64 * SXP_WAVES, min_waves<S>(3, 1)) k_rand_sxp
                     if (j < m.R)
                         sx_pair_move<S, 32, 1>(m, st, a, sx, &sh32[wv][lane / 32], sa, na, sb, nb, slot, tau, k, j,
                                                lane % 32, beta, maxnats, kmin, out);
+                }
+            } else if (K64 > 0 && xl_fits<S, 64, (K64 > 0 ? K64 : 1)>(sx.tb, slot)) {
+                for (int h = 0; h < 4; h++) {
+                    const int j = j0 + h;
+                    if (j < m.R)
+                        sx_pair_move<S, 64, (K64 > 0 ? K64 : 1)>(m, st, a, sx, &sh64[wv], sa, na, sb, nb, slot, tau, k,
+                                                                  j, lane, beta, maxnats, kmin, out);
                 }
             } else if ((lane & 15) == 0 && j0 + lane / 16 < m.R) {   // refused on the host; a guard if it ever came
                 out.add(2, false, 0);
@@ -1261,22 +1237,18 @@ int launch_random(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, double 
                 set_error("storage pairs: a pair template is larger than the lane form holds");
                 return ESM_EINVAL;
             }
-            // the blocks of one step on one XCD: the grid padded to whole rounds of 8 XCDs × the
-            // step's blocks (the extra blocks find no wave of theirs and exit); a wave holds 64/W
-            // replicas (the 32-lane form: four, in 16- or 32-lane segments per move)
-            auto blocks_for = [&](size_t per_wave) {
-                const size_t ngrp = ((size_t)R + per_wave - 1) / per_wave, nwaves = (size_t)a.ntau * ngrp;
-                size_t nblocks = (nwaves + SXP_WAVES - 1) / SXP_WAVES;
-                if (ngrp % SXP_WAVES == 0) {
-                    const size_t round = 8 * (ngrp / SXP_WAVES);
-                    nblocks = (nblocks + round - 1) / round * round;
-                }
-                return dim3((unsigned)nblocks);
-            };
-            const dim3 grid = blocks_for((size_t)(64 / wx)), gridm = blocks_for(4), blk(64 * SXP_WAVES);
-            if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair_mix<S>), gridm, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
-            else if (ks == 1) hipLaunchKernelGGL((k_rand_sxpair<S, 64, 1>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
-            else hipLaunchKernelGGL((k_rand_sxpair<S, 64, 2>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            // four replicas per wave; the blocks of one step on one XCD: the grid padded to whole
+            // rounds of 8 XCDs × the step's blocks (the extra blocks find no wave of theirs and exit)
+            const size_t ngrp = ((size_t)R + 3) / 4, nwaves = (size_t)a.ntau * ngrp;
+            size_t nblocks = (nwaves + SXP_WAVES - 1) / SXP_WAVES;
+            if (ngrp % SXP_WAVES == 0) {
+                const size_t round = 8 * (ngrp / SXP_WAVES);
+                nblocks = (nblocks + round - 1) / round * round;
+            }
+            const dim3 grid((unsigned)nblocks), blk(64 * SXP_WAVES);
+            if (wx == 32) hipLaunchKernelGGL((k_rand_sxpair_mix<S, 0>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            else if (ks == 1) hipLaunchKernelGGL((k_rand_sxpair_mix<S, 1>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
+            else hipLaunchKernelGGL((k_rand_sxpair_mix<S, 2>), grid, blk, 0, h->stream, m, st, b, sx, beta, mxn, kmin);
             ESM_HIP_CHECK(hipGetLastError());
         }
     }

@@ -3053,23 +3053,6 @@ static double xm_eval(const XMove *f, double t) {
     xm_butterfly(part, f->W);
     return part[0];
 }
-/* the lanes of the device's sxpair segment: 32 when every pair template fits 32 lanes (kinks ≤ W/2,
- * lines ≤ 3W/2, balances ≤ W, node–line entries ≤ 4W per end), else 64 (the device refuses a system
- * whose templates do not fit 64 lanes) */
-static int xm_pair_width(const Tables *tb) {
-    int w = 32;
-    for (int sl = 0; sl + 1 < tb->sx_kk_off.n; sl++) {
-        int K = tb->sx_kk_off.a[sl + 1] - tb->sx_kk_off.a[sl], Lw = tb->sx_ln_off.a[sl + 1] - tb->sx_ln_off.a[sl];
-        int Nn = tb->sx_nd_off.a[sl + 1] - tb->sx_nd_off.a[sl];
-        int NLp = tb->sx_nd_l.a[tb->sx_nd_off.a[sl + 1]] - tb->sx_nd_l.a[tb->sx_nd_off.a[sl]];
-        for (;;) {
-            if (2 * K <= w && 2 * Lw <= 3 * w && 2 * Nn <= 2 * w && 2 * NLp <= 8 * w) break;
-            if (w == 64) return 64;
-            w = 64;
-        }
-    }
-    return w;
-}
 static int xm_next_bp(const XMove *f, double tp, double sgn, double *t) {
     Sys *S = f->S;
     const esm_model *m = S->m;
@@ -3371,12 +3354,13 @@ static void sxpair_move(Sys *S, const Tables *tb, int j, int tau, const uint32_t
     if (S->kc[(ESM_KSTOR + sa) * N + na] * m->stor_hours[ua] / m->dt < 1.0) return;
     if (!xm_partner(S, tb, hmin, sa, na, u01(s[2], s[3]), &slot, &sb, &nb)) return;
     XMove f;
-    /* the device's 32-lane form runs a move whose template fits 16 lanes (kinks ≤ 8, lines ≤ 24,
-     * balances ≤ 16 per end) on a 16-lane segment */
-    int W = xm_pair_width(tb);
+    /* the device runs each move on the narrowest segment its template fits: 16 lanes (kinks ≤ 8,
+     * lines ≤ 24, balances ≤ 16 per end), 32 (twice that), else 64 */
     int K = tb->sx_kk_off.a[slot + 1] - tb->sx_kk_off.a[slot], Lw = tb->sx_ln_off.a[slot + 1] - tb->sx_ln_off.a[slot];
     int Nn = tb->sx_nd_off.a[slot + 1] - tb->sx_nd_off.a[slot];
-    if (W == 32 && 2 * K <= 16 && 2 * Lw <= 48 && 2 * Nn <= 32) W = 16;
+    int W = 64;
+    if (2 * K <= 32 && 2 * Lw <= 96 && 2 * Nn <= 64) W = 32;
+    if (2 * K <= 16 && 2 * Lw <= 48 && 2 * Nn <= 32) W = 16;
     if (!xm_prepare(&f, S, tb, j, sa, na, sb, nb, slot, (tau - 1 + T) % T, 1, W)) return;
     xm_move(&f, useg, uoff, umh, maxnats, stats);
 }
