@@ -190,9 +190,8 @@ def kernel_bytes(s, R, es, ntheta_launch, stats, prof, steps):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: every box of this build has a single MI355X (asked for --gpus "
-              f"{args.gpus}); the multi-GPU design (one RCCL allreduce of the weighted score per step) is "
-              f"described in DESIGN.md, not run", file=sys.stderr)
+        print(f"bench.py measures one MI355X (asked for --gpus {args.gpus}); the multi-GPU form (one RCCL "
+              f"allreduce of the weighted score per step) is designed, not run", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

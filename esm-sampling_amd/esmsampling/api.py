@@ -464,7 +464,7 @@ class LearnedProposal:
     gas utilization of one node over a block of ``block_hours`` hours from the block's context
     (the requirement the unit meets, the node's capacity factors and capacities); every proposal
     is accepted or rejected by the exact Metropolis–Hastings ratio (FP64), so the target is
-    unchanged and the plain line moves stay the judge. Trained on the sampler's own states by
+    unchanged and the plain line moves remain both fallback and reference. Trained on the sampler's own states by
     maximum likelihood (Adam). No counterpart in the reference."""
 
     def __init__(self, R: Replicas, block_hours: int = 24, ncomp: int = 4, key=(7, 7)):

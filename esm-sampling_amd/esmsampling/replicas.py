@@ -10,7 +10,7 @@ chain at once.
 
 Configuration: TOML with every key of the reference's ``Config`` (src/run.jl:7-127) and its
 defaults; an unknown key raises, as ``load_config`` does (src/run.jl:135-156). Keys that the
-replica path uses are implemented, or accepted at the values this build supports (others are
+replica path uses are implemented, or accepted at the values this package supports (others are
 refused); keys that the replica path never reads are accepted and have no effect. The system is
 the seeded synthetic one (``synth.make_system``): its nodes and hours per step come from the
 finest level of ``levels`` (or the keys ``synthetic_nodes``, ``resolution_hours``), its regimes
@@ -42,7 +42,7 @@ NK = synth.NK
 LANGEVIN_KAPPA = {"onwind": 500.0, "solar": 1000.0, "offwind": 200.0, "gas": 300.0, "battery": 100.0, "H2": 100.0}
 
 
-# Default lines of a synthetic network of N nodes (the BASELINE networks: 40 nodes / 70 lines and
+# Default lines of a synthetic network of N nodes (the benchmark networks: 40 nodes / 70 lines and
 # 256 nodes / 450 lines; otherwise 1.75 lines per node, at least a spanning tree).
 def default_lines(N: int) -> int:
     return {40: 70, 256: 450}.get(N, max(N - 1, int(round(1.75 * N))))
@@ -53,7 +53,7 @@ class Config:
     """Every key of the reference's ``Config`` (src/run.jl:7-127) with its default, plus this
     driver's own keys (the synthetic system, the learned proposal, the launch form and the sweep
     batching). ``check`` sorts the reference keys into three kinds: those the replica path uses
-    (implemented here), those it uses at values this build supports (others are refused with the
+    (implemented here), those it uses at values this package supports (others are refused with the
     reference's message style) and those the replica path never reads (accepted, no effect:
     level-0 warmup, bias learning, the chains' fine step, the device driver's capacity moves)."""
     # -- data and outputs (src/run.jl:8-14) --
@@ -259,7 +259,7 @@ class Config:
             refuse("tilt_functional", '"none" or "resload_mean"')
         if self.tilt_lambda != 0 and self.tilt_functional == "none":
             raise ValueError("tilt_lambda ≠ 0 needs a tilt_functional")
-        # values the replica path uses that this build does not implement
+        # values the replica path uses that this package does not implement
         if self.basis != "haar":
             refuse("basis", 'the flow basis and its transport families are not built here; use "haar"')
         if self.init != "heuristic":

@@ -142,8 +142,8 @@ def make_system(N: int = 40, L: int = 70, T: int = 8760, dt: float = 1.0, seed: 
     """The synthetic system with N nodes, L lines and T steps of dt hours (T·dt = 8760 for one year),
     K weather regimes (switch probability 0.02 per step, spread evenly over the other regimes;
     regime intercepts evenly spaced in [−a, a]) and ``ncountry`` countries (max(2, N // 5) by
-    default). K and ncountry use no random numbers: at K = 2 the weather is the BASELINE
-    2-regime process."""
+    default). K and ncountry use no random numbers: at K = 2 the weather is the benchmark
+    configurations' 2-regime process."""
     if not 2 <= K <= 4:
         raise ValueError("K: 2 to 4 regimes")
     if ncountry is not None and not 2 <= ncountry <= N:

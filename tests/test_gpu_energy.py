@@ -1,6 +1,6 @@
 """GPU parity of the energy, flow and residual caches, scores and Langevin step against the oracle.
 
-Tolerances (BASELINE north star): FP64 state 1e-12 relative, FP32 state 1e-5 relative; the
+Tolerances: FP64 state 1e-12 relative, FP32 state 1e-5 relative; the
 oracle evaluates the same (FP32-rounded, in FP32 mode) state in FP64.
 """
 import sys
