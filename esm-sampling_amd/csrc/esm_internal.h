@@ -14,6 +14,8 @@ namespace esm {
 
 constexpr int ESM_MAXK = 4;       // largest regime count of the weather model
 constexpr int ESM_MAXWIN = 256;   // longest regime window
+constexpr int SMS_NU = 64;        // units of one storage multiscale move (the largest country–carrier
+                                  // group): one lane of the wave stages each unit
 constexpr int STAT_SLOTS = 256;   // sweep counters are spread over this many addresses per statistic
 
 // Minimum waves per SIMD asked of a latency-bound sweep kernel (second __launch_bounds__

@@ -23,7 +23,6 @@
 
 namespace esm {
 
-constexpr int SMS_NU = 8;          // units per move (largest country–carrier group)
 constexpr int SMS_SP = 3;          // block steps per lane (L + 1 ≤ 3·64)
 
 struct SmsArgs {
@@ -263,10 +262,11 @@ template <typename S>
 __host__ __device__ constexpr size_t sms_seg_words(int nstep, int nus) {
     return SMS_NPLANE * (size_t)nstep * nus + sizeof(SmsKink<S>) / sizeof(S) * 2 * nus;
 }
-// words (of S) of the wave's moved units, shared by its segments (the move is the wave's)
+// words (of S) of the wave's moved units (nus: the largest group), shared by its segments (the
+// move is the wave's)
 template <typename S>
-__host__ __device__ constexpr size_t sms_unit_words() {
-    return (sizeof(SmsUnit<S>) + sizeof(S) - 1) / sizeof(S) * SMS_NU;
+__host__ __device__ constexpr size_t sms_unit_words(int nus) {
+    return (sizeof(SmsUnit<S>) + sizeof(S) - 1) / sizeof(S) * (size_t)nus;
 }
 
 
@@ -331,10 +331,10 @@ __global__ void __launch_bounds__(64, min_waves<S>(4, 2)) k_storage_ms(DevModel<
     const bool valid = j < m.R;
     const int nstep = L + 1, nus = a.nus;
     const size_t TR = (size_t)m.T * m.R;
-    // LDS: the wave's units [SMS_NU], then per segment the planes [SMS_NPLANE][nstep][nus] and
+    // LDS: the wave's units [nus], then per segment the planes [SMS_NPLANE][nstep][nus] and
     // the kinks [2][nus]
     SmsUnit<S> *un_l = (SmsUnit<S> *)sms_lds;
-    S *base = (S *)sms_lds + sms_unit_words<S>() + (size_t)seg * sms_seg_words<S>(nstep, nus);
+    S *base = (S *)sms_lds + sms_unit_words<S>(nus) + (size_t)seg * sms_seg_words<S>(nstep, nus);
     S *ye_l = base + P_YE * nstep * nus, *ye1_l = base + P_YE1 * nstep * nus;
     S *S0_l = base + P_S0 * nstep * nus, *c0_l = base + P_C0 * nstep * nus;
     S *j0_l = base + P_J0 * nstep * nus, *ph0_l = base + P_PH0 * nstep * nus;
@@ -552,7 +552,7 @@ static int launch_storage_ms_stable(esm_handle *h, uint32_t k0, uint32_t k1, uin
             if (sms_width(L) != widths[q]) continue;
             const int nb = (T - L - 1) / L + 1;
             wblocks[q] = std::max(wblocks[q], (nb + 1) / 2);
-            wlds[q] = std::max(wlds[q], (sms_unit_words<S>() + (size_t)(64 / widths[q]) * sms_seg_words<S>(L + 1, nus)) *
+            wlds[q] = std::max(wlds[q], (sms_unit_words<S>(nus) + (size_t)(64 / widths[q]) * sms_seg_words<S>(L + 1, nus)) *
                                             sizeof(S));
         }
     for (int q = 0; q < 5; q++)
@@ -615,7 +615,7 @@ int launch_storage_ms(esm_handle *h, uint32_t k0, uint32_t k1, uint32_t ctr, dou
             SmsArgs a{k0, k1, ctr, L, o + cls * L, nbc, nrg, h->sweep_ms_moves, nus, h->sweep_ms_pagg,
                       d.sms_units, (int)h->mt.sms_units.size(), d.grp_off, d.grp_node, d.grp_si,
                       (int)h->mt.grp_si.size(), h->d_stats + (size_t)ESM_FAM_SMS * ESM_NSTAT * STAT_SLOTS};
-            const size_t lds = (sms_unit_words<S>() + (size_t)segs * sms_seg_words<S>(L + 1, nus)) * sizeof(S);
+            const size_t lds = (sms_unit_words<S>(nus) + (size_t)segs * sms_seg_words<S>(L + 1, nus)) * sizeof(S);
             if (lds > (size_t)lds_max) {
                 set_error("storage multiscale sweep: block length " + std::to_string(L) + " needs " +
                           std::to_string(lds) + " B of LDS per workgroup, the device allows " + std::to_string(lds_max));

@@ -492,8 +492,9 @@ bool build_move_tables(const esm_handle *h, MoveTablesHost &mt, std::string &err
                     mt.grp_node.push_back(n);
                     cnt++;
                 }
-            if (cnt > 8) {
-                err = "move tables: a country holds more than 8 nodes (the storage group limit)";
+            if (cnt > SMS_NU) {
+                err = "move tables: a country holds more than " + std::to_string(SMS_NU) +
+                      " nodes (the storage group limit: one lane stages each unit)";
                 return false;
             }
             mt.grp_si.push_back(s);

@@ -1865,7 +1865,7 @@ int orc_sparse_draw_many(const esm_model *m, int R, double *v, const double *k, 
  * the balances' roots on the pieces (walk!, src/local.jl:1294-1443); equal roots are visited once.
  * ============================================================================================ */
 
-#define SMS_NU 8
+#define SMS_NU 64   /* units per move: the largest country–carrier group */
 #define SMS_PMAX 4096
 
 typedef struct {

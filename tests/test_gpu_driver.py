@@ -198,6 +198,24 @@ def test_resume_inside_the_warmup_with_training_and_averaged_probes(tmp_path):
     assert b2["learned"]["train_steps"] == a["learned"]["train_steps"]
 
 
+def test_run_replicas_256_nodes_with_the_reference_storage_pairs():
+    """The driver on a 256-node / 450-line network with the reference configuration's defaults
+    (storage pairs sxpair 0.1 and sxlong 8 on: the hub templates need the 64-lane segments with
+    two kink and three balance slots): weather, sweeps, scores, Langevin steps and following run,
+    the traces stay finite and the capacities move."""
+    print("run_replicas 256 nodes with storage pairs", flush=True)
+    cfg = RP.load_config(dict(init="heuristic", resolution_hours=1.0, nchains=1, replicas=4, mlda_warmup=2, nsamples=3,
+                              replica_init_sweeps=1, replica_probe_every=0, synthetic_nodes=256,
+                              synthetic_lines=450, snapshots=48, precision="fp32", seed=3))
+    assert cfg.storage_xpair == 0.1 and cfg.storage_xlong == 8
+    res = RP.run_replicas(cfg)
+    assert res["status"] == 0 and res["steps_done"] == 5
+    tr = np.array(res["traces"][0])
+    assert np.all(np.isfinite(tr))
+    ks = [r["k"] for r in res["records"][0]]
+    assert not np.array_equal(ks[0], ks[-1])
+
+
 def test_run_replicas_with_ffbs_and_the_reference_replica_configuration():
     """The reference's replica configuration (configs/device40_replica.toml: physical measure, 40
     nodes daily, K = 4 regimes, 10 countries, 4 chains of 4 replicas) with a short schedule and the

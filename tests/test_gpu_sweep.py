@@ -220,11 +220,12 @@ def test_sweep_fp64_waves_per_step_matches_oracle(waves):
     dev.close()
 
 
-@pytest.mark.parametrize("R,T", [(4, 47), (64, 20), (3, 300)])
-def test_storage_ms_fp64_matches_oracle(R, T):
-    """The storage multiscale sweep alone (every block length below T/2, segments of 4 … 64 lanes)."""
-    print(f"storage_ms fp64 R={R} T={T}", flush=True)
-    s = synth.make_system(N=10, L=16, T=T, dt=1.0, seed=12)
+@pytest.mark.parametrize("R,T,N,C", [(4, 47, 10, None), (64, 20, 10, None), (3, 300, 10, None), (8, 200, 40, 2)])
+def test_storage_ms_fp64_matches_oracle(R, T, N, C):
+    """The storage multiscale sweep alone (every block length below T/2, segments of 4 … 64
+    lanes); with two countries of 20 nodes, the groups move 20 units at once."""
+    print(f"storage_ms fp64 R={R} T={T} N={N} countries={C}", flush=True)
+    s = synth.make_system(N=N, L={10: 16, 40: 70}[N], T=T, dt=1.0, seed=12, ncountry=C)
     rng = np.random.default_rng(R + T)
     x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
     v = synth.heuristic_state(s, x)
@@ -244,9 +245,10 @@ def test_storage_ms_fp64_matches_oracle(R, T):
     dev.close()
 
 
-def test_storage_ms_fp32_consistent():
-    print("storage_ms fp32", flush=True)
-    s = synth.make_system(N=10, L=16, T=300, dt=1.0, seed=13)
+@pytest.mark.parametrize("N,C", [(10, None), (40, 2)])
+def test_storage_ms_fp32_consistent(N, C):
+    print(f"storage_ms fp32 N={N} countries={C}", flush=True)
+    s = synth.make_system(N=N, L={10: 16, 40: 70}[N], T=300, dt=1.0, seed=13, ncountry=C)
     R = 64
     rng = np.random.default_rng(3)
     x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R)).astype(np.float32).astype(np.float64)

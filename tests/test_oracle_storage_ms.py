@@ -17,8 +17,8 @@ import oracle
 from esmsampling import synth
 
 
-def _state(seed=3, N=10, L=16, T=40):
-    s = synth.make_system(N=N, L=L, T=T, dt=1.0, seed=seed)
+def _state(seed=3, N=10, L=16, T=40, C=None):
+    s = synth.make_system(N=N, L=L, T=T, dt=1.0, seed=seed, ncountry=C)
     R = 2
     rng = np.random.default_rng(seed)
     x = rng.uniform(0.05, 0.95, (3, s.N, s.T, R))
@@ -70,9 +70,11 @@ def test_plateau_directions():
         assert np.all(dr[:, :, 0] == 0)
 
 
-@pytest.mark.parametrize("seed", [4, 5])
-def test_incremental_cost_equals_global(seed):
-    s, v, k, x = _state(seed=seed)
+@pytest.mark.parametrize("seed,N,L,C", [(4, 10, 16, None), (5, 10, 16, None), (6, 40, 70, 2)])
+def test_incremental_cost_equals_global(seed, N, L, C):
+    """ΔC(t) of a move equals the change of the global energy; the last case moves groups of 20
+    units (two countries of 20 nodes)."""
+    s, v, k, x = _state(seed=seed, N=N, L=L, C=C)
     v[2] = np.random.default_rng(seed).uniform(-0.05, 1.05, v[2].shape)   # also states outside the box
     o = oracle.Oracle(s)
     parts, _, r, F = o.energy(v, k, x)
