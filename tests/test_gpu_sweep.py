@@ -245,6 +245,26 @@ def test_storage_ms_fp64_matches_oracle(R, T, N, C):
     dev.close()
 
 
+def test_storage_ms_refuses_groups_beyond_the_lds():
+    """FP64 groups of 20 units at block length 128 need 10·129·20·8 B of LDS planes, more than a
+    workgroup's 160 KB: the host refuses the launch with an error naming the need (no kernel of
+    that length runs); FP32 takes the same groups."""
+    print("storage_ms fp64 LDS refusal", flush=True)
+    # T = 400: every offset of L = 128 leaves at least two blocks, so that length always runs
+    s = synth.make_system(N=40, L=70, T=400, dt=1.0, seed=12, ncountry=2)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.05, 0.95, (3, s.N, s.T, 2))
+    v = synth.heuristic_state(s, x)
+    dev = api.Replicas(s, 2, FP64)
+    try:
+        dev.upload(v=v, k=s.k0, x=x)
+        api.sweep_options(dev, extra=0.0, atoms=False, p_storage_ms=1.0)
+        with pytest.raises(Exception, match="LDS"):
+            api.operation_sweep(dev, 1, key=(4, 9), counter=2)
+    finally:
+        dev.close()
+
+
 @pytest.mark.parametrize("N,C", [(10, None), (40, 2)])
 def test_storage_ms_fp32_consistent(N, C):
     print(f"storage_ms fp32 N={N} countries={C}", flush=True)
